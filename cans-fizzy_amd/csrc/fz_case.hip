@@ -1,0 +1,448 @@
+// A whole case on the device: the time loop of src/main.f90:414-562 and the
+// AB2 integrators rk / rk_2fl of src/rk.f90, with the history arrays that the
+// reference keeps in `save` variables held per case (or per fz_rk_state).
+//
+// The reference's full-array copies psio = psi, pp = p and po = pn; pn = p
+// (src/main.f90:430-432, :488-490, :505-508) become pointer rotations:
+//  - the phase-field update writes psi(n+1) into the buffer that held
+//    psi(n-1), so the previous buffer is psio with its halos intact;
+//  - fillps/solver/correc work on a separate correction buffer dp, and
+//    updatep writes p + dp into the buffer that held the pressure of the
+//    previous step, which then becomes po; pn is p itself (pn == p whenever
+//    mom_xyz_oth reads it).
+#include <cmath>
+#include <cstring>
+
+#include "fz_internal.h"
+
+int fz_launch_rk_psi(fz_ctx *c, double f1, double f2, const double *dpsidt, const double *dpsidto, double *psi);
+int fz_launch_rk_mom(fz_ctx *c, double f1, double f2, const double rho12[2], const double *psio, const double *psi,
+                     const double *du, const double *dv, const double *dw, const double *duo, const double *dvo,
+                     const double *dwo, double *u, double *v, double *w);
+int fz_launch_rk_oth(fz_ctx *c, double f12, const double rho12[2], const double *psi, const double *du,
+                     const double *dv, const double *dw, double *u, double *v, double *w);
+int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
+int fz_chkdiv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
+int fz_chkdt_adv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
+double fz_chkdt_finish(const fz_ctx *c, double dti, int is_solve_ns, int is_track_interface, int vof,
+                       const double mu12[2], const double rho12[2], double sigma, const double gacc[3], double gam,
+                       double seps);
+int fz_bulk_mean_dev(fz_ctx *c, int idir, const double *psi, const double p12[2], const double *gvr, double *d_out);
+
+// out-of-place AB2 phase-field update (psi_out may alias psi_in)
+__global__ void k_rk_psi_oop(fz_dims d, double f1, double f2, const double *__restrict__ dpsidt,
+                             const double *__restrict__ dpsidto, const double *psi_in, double *psi_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > d.n1 || j > d.n2) return;
+  const long q = i + d.sy * j + d.sz * k;
+  const long r = (i - 1) + (long)d.n1 * ((j - 1) + (long)d.n2 * (k - 1));
+  psi_out[q] = psi_in[q] + f1 * dpsidt[r] + f2 * dpsidto[r];
+}
+
+// out-of-place updatep: p_out = p + dp over the whole array, src/updatep.f90:21-23
+__global__ void k_updatep_oop(long nh, const double *__restrict__ dp, const double *__restrict__ p,
+                              double *__restrict__ p_out) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
+    p_out[q] = dp[q] + p[q];
+}
+
+enum { F_U, F_V, F_W, F_P, F_PN, F_PO, F_PSI, F_PHI, F_KAPPA, F_NX, F_NY, F_NZ, F_FX, F_FY, F_FZ, F_DPSIO, F_DUO,
+       F_DVO, F_DWO, F_N };
+enum { S_TIME, S_DT, S_DTO, S_GAM, S_SEPS, S_RHO0, S_DIVTOT, S_DIVMAX, S_DTCFL, S_ISTEP, S_FIRST, S_N };
+
+struct fz_case {
+  fz_params prm;
+  fz_ctx *ctx = nullptr;
+  double *f[F_N] = {nullptr};
+  double *psio = nullptr, *dpsidt = nullptr, *du = nullptr, *dv = nullptr, *dw = nullptr, *dp = nullptr;
+  double *dsc = nullptr;  // device scalars: 0 gam, 1-2 div, 3 adv dt, 4-6 rho_av
+  double *gvr_c = nullptr, *gvr_f = nullptr;
+  double sc[12] = {0};
+  char cbccur[6];
+  double bccur[6];
+  bool gam_on_device = false;
+};
+
+static int case_alloc(fz_case *cs) {
+  fz_ctx *c = cs->ctx;
+  for (int q = 0; q < F_N; q++) {
+    if (q == F_PN) continue;  // pn is p (see above)
+    const long n = q < F_DPSIO ? c->nh : c->ni;
+    FZ_HIP(hipMalloc((void **)&cs->f[q], sizeof(double) * n));
+    FZ_HIP(hipMemsetAsync(cs->f[q], 0, sizeof(double) * n, c->st));
+  }
+  double **hal[2] = {&cs->psio, &cs->dp};
+  for (int q = 0; q < 2; q++) {
+    FZ_HIP(hipMalloc((void **)hal[q], sizeof(double) * c->nh));
+    FZ_HIP(hipMemsetAsync(*hal[q], 0, sizeof(double) * c->nh, c->st));
+  }
+  double **itr[4] = {&cs->dpsidt, &cs->du, &cs->dv, &cs->dw};
+  for (int q = 0; q < 4; q++) {
+    FZ_HIP(hipMalloc((void **)itr[q], sizeof(double) * c->ni));
+    FZ_HIP(hipMemsetAsync(*itr[q], 0, sizeof(double) * c->ni, c->st));
+  }
+  FZ_HIP(hipMalloc((void **)&cs->dsc, sizeof(double) * 16));
+  FZ_HIP(hipMemsetAsync(cs->dsc, 0, sizeof(double) * 16, c->st));
+  const int nz = c->d.n3 + 2;
+  std::vector<double> gc(nz), gf(nz);
+  const double vol = c->l[0] * c->l[1] * c->l[2];
+  for (int k = 0; k < nz; k++) {  // src/main.f90:268-269
+    gc[k] = c->dl[0] * c->dl[1] * c->h_dzc[k] / vol;
+    gf[k] = c->dl[0] * c->dl[1] * c->h_dzf[k] / vol;
+  }
+  FZ_HIP(hipMalloc((void **)&cs->gvr_c, sizeof(double) * nz));
+  FZ_HIP(hipMalloc((void **)&cs->gvr_f, sizeof(double) * nz));
+  FZ_HIP(hipMemcpy(cs->gvr_c, gc.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+  FZ_HIP(hipMemcpy(cs->gvr_f, gf.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+  return FZ_OK;
+}
+
+extern "C" int fz_case_destroy(fz_case *cs) {
+  if (!cs) return FZ_OK;
+  if (cs->ctx) (void)hipStreamSynchronize(cs->ctx->st);
+  for (int q = 0; q < F_N; q++)
+    if (cs->f[q]) (void)hipFree(cs->f[q]);
+  double *o[9] = {cs->psio, cs->dpsidt, cs->du, cs->dv, cs->dw, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f};
+  for (int q = 0; q < 9; q++)
+    if (o[q]) (void)hipFree(o[q]);
+  if (cs->ctx) fz_ctx_destroy(cs->ctx);
+  delete cs;
+  return FZ_OK;
+}
+
+extern "C" int fz_case_create(fz_case **out, const fz_params *prm, void *hip_stream) {
+  if (!out || !prm) return fz_set_error(FZ_ERR_ARG, "fz_case_create: null argument");
+  const char *codes[4] = {prm->cbcvel, prm->cbcpre, prm->cbcpsi, prm->cbcnor};
+  const int lens[4] = {18, 6, 6, 18};
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < lens[a]; b++)
+      if (codes[a][b] != 'P' && codes[a][b] != 'D' && codes[a][b] != 'N')
+        return fz_set_error(FZ_ERR_ARG, "fz_case_create: bad boundary code '%c'", codes[a][b]);
+  fz_case *cs = new fz_case();
+  cs->prm = *prm;
+  const int n3 = prm->ng[2];
+  std::vector<double> dzc(n3 + 2), dzf(n3 + 2), zc(n3 + 2), zf(n3 + 2);
+  int rc = fz_initgrid(prm->gtype, n3, prm->gr, prm->l[2], dzc.data(), dzf.data(), zc.data(), zf.data());
+  if (rc == FZ_OK) rc = fz_ctx_create(&cs->ctx, prm->ng, prm->l, dzc.data(), dzf.data(), zc.data(), zf.data(),
+                                      prm->cbcpre, hip_stream);
+  if (rc == FZ_OK) rc = fz_initsolver(cs->ctx, prm->bcpre, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, nullptr);
+  if (rc == FZ_OK) rc = case_alloc(cs);
+  if (rc != FZ_OK) {
+    fz_case_destroy(cs);
+    return rc;
+  }
+  memcpy(cs->cbccur, prm->cbcpsi, 6);  // cbccur = cbcpsi, bccur = bcpre, src/param.f90:172-173
+  memcpy(cs->bccur, prm->bcpre, sizeof(double) * 6);
+  cs->sc[S_RHO0] = fmin(prm->rho12[0], prm->rho12[1]);  // src/param.f90:174-177
+  cs->sc[S_FIRST] = 1.0;
+  *out = cs;
+  return FZ_OK;
+}
+
+extern "C" int fz_case_ctx(fz_case *cs, fz_ctx **ctx) {
+  if (!cs || !ctx) return fz_set_error(FZ_ERR_ARG, "fz_case_ctx: null argument");
+  *ctx = cs->ctx;
+  return FZ_OK;
+}
+
+extern "C" int fz_case_field(fz_case *cs, int id, double **dptr, size_t *count) {
+  if (!cs || !dptr || id < 0 || id >= F_N) return fz_set_error(FZ_ERR_ARG, "fz_case_field: bad argument");
+  *dptr = (id == F_PN) ? cs->f[F_P] : cs->f[id];
+  if (count) *count = (size_t)(id < F_DPSIO ? cs->ctx->nh : cs->ctx->ni);
+  return FZ_OK;
+}
+
+static int sync_gam(fz_case *cs) {
+  if (!cs->gam_on_device) return FZ_OK;
+  fz_ctx *c = cs->ctx;
+  FZ_HIP(hipMemcpyAsync(c->h_small, cs->dsc, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  FZ_HIP(hipStreamSynchronize(c->st));
+  cs->sc[S_GAM] = c->h_small[0];
+  return FZ_OK;
+}
+
+extern "C" int fz_case_get_scalars(fz_case *cs, double out[12]) {
+  if (!cs || !out) return fz_set_error(FZ_ERR_ARG, "fz_case_get_scalars: null argument");
+  FZ_TRY(sync_gam(cs));
+  memcpy(out, cs->sc, sizeof(cs->sc));
+  return FZ_OK;
+}
+
+extern "C" int fz_case_set_scalars(fz_case *cs, const double in[12]) {
+  if (!cs || !in) return fz_set_error(FZ_ERR_ARG, "fz_case_set_scalars: null argument");
+  memcpy(cs->sc, in, sizeof(cs->sc));
+  fz_ctx *c = cs->ctx;
+  c->h_small[0] = cs->sc[S_GAM];
+  FZ_HIP(hipMemcpyAsync(cs->dsc, c->h_small, sizeof(double), hipMemcpyHostToDevice, c->st));
+  FZ_HIP(hipStreamSynchronize(c->st));
+  cs->gam_on_device = true;
+  return FZ_OK;
+}
+
+static int phase_geometry(fz_case *cs) {
+  fz_ctx *c = cs->ctx;
+  const fz_params &P = cs->prm;
+  double **f = cs->f;
+  if (!P.vof) FZ_TRY(fz_acdi_cmpt_phi(c, cs->sc[S_SEPS], f[F_PSI], f[F_PHI]));
+  else FZ_TRY(fz_vof_thinc_cmpt_phi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_PSI], f[F_PHI]));
+  FZ_TRY(fz_cmpt_norm_curv_youngs(c, f[F_PHI], f[F_NX], f[F_NY], f[F_NZ], f[F_KAPPA]));
+  FZ_TRY(fz_boundp(c, cs->cbccur, cs->bccur, f[F_KAPPA]));
+  FZ_TRY(fz_boundp(c, P.cbcnor + 0, P.bcnor + 0, f[F_NX]));
+  FZ_TRY(fz_boundp(c, P.cbcnor + 6, P.bcnor + 6, f[F_NY]));
+  FZ_TRY(fz_boundp(c, P.cbcnor + 12, P.bcnor + 12, f[F_NZ]));
+  return FZ_OK;
+}
+
+// chkdt + the dt selection of src/main.f90:396-404 and :532-550
+static int new_dt(fz_case *cs, double *dt, int *kill) {
+  fz_ctx *c = cs->ctx;
+  const fz_params &P = cs->prm;
+  FZ_TRY(fz_chkdt_adv_dev(c, cs->f[F_U], cs->f[F_V], cs->f[F_W], cs->dsc + 3));
+  FZ_HIP(hipMemcpyAsync(c->h_small, cs->dsc, sizeof(double) * 4, hipMemcpyDeviceToHost, c->st));
+  FZ_HIP(hipStreamSynchronize(c->st));
+  if (cs->gam_on_device) cs->sc[S_GAM] = c->h_small[0];
+  const double dti = c->h_small[3];
+  const double dt_cfl = fz_chkdt_finish(c, dti, P.is_solve_ns, P.is_track_interface, P.vof, P.mu12, P.rho12,
+                                        P.sigma, P.gacc, cs->sc[S_GAM], cs->sc[S_SEPS]);
+  cs->sc[S_DTCFL] = dt_cfl;
+  double d = fmin(P.cfl * dt_cfl, P.dtmax);
+  if (P.dt_f > 0.0) {
+    if (P.dt_f > d && kill) *kill = 1;
+    d = P.dt_f;
+  }
+  *dt = d;
+  return FZ_OK;
+}
+
+extern "C" int fz_case_init(fz_case *cs) {
+  if (!cs) return fz_set_error(FZ_ERR_ARG, "fz_case_init: null argument");
+  fz_ctx *c = cs->ctx;
+  const fz_params &P = cs->prm;
+  double **f = cs->f;
+  double seps;
+  FZ_TRY(fz_acdi_set_epsilon(c, P.psi_thickness_factor, &seps));  // src/main.f90:301
+  cs->sc[S_SEPS] = seps;
+  for (int q = F_U; q <= F_P; q++) FZ_TRY(fz_launch_fill(c, c->nh, 0.0, f[q]));  // initflow 'zer'
+  FZ_TRY(fz_init2fl(c, &P, seps, f[F_PSI]));
+  FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
+  FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));
+  FZ_HIP(hipMemcpyAsync(f[F_PO], f[F_P], sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
+  FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
+  FZ_TRY(phase_geometry(cs));
+  if (!P.vof) {
+    FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
+    cs->gam_on_device = true;
+  }
+  double dt;
+  FZ_TRY(new_dt(cs, &dt, nullptr));
+  cs->sc[S_TIME] = 0.0;
+  cs->sc[S_DT] = dt;
+  cs->sc[S_DTO] = dt;
+  cs->sc[S_ISTEP] = 0.0;
+  cs->sc[S_FIRST] = 1.0;
+  FZ_HIP(hipStreamSynchronize(c->st));
+  return FZ_OK;
+}
+
+template <typename T>
+static void swap_ptr(T *&a, T *&b) {
+  T *t = a;
+  a = b;
+  b = t;
+}
+
+extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
+  if (!cs || nsteps < 0) return fz_set_error(FZ_ERR_ARG, "fz_case_step: bad argument");
+  fz_ctx *c = cs->ctx;
+  const fz_params &P = cs->prm;
+  double **f = cs->f;
+  const double rho0 = cs->sc[S_RHO0];
+  const long bi = sizeof(double) * c->ni;
+  int kill = 0;
+  for (int s = 0; s < nsteps && !kill; s++) {
+    const double dt = cs->sc[S_DT], dto = cs->sc[S_DTO];
+    const bool is_first = cs->sc[S_FIRST] != 0.0;
+    cs->sc[S_ISTEP] += 1.0;
+    cs->sc[S_TIME] += dt;
+    const double dt_r = dt / dto;  // src/main.f90:422-424, src/rk.f90:53-55
+    const double tm1 = 1.0 + 0.5 * dt_r, tm2 = -0.5 * dt_r;
+    const double factor1 = tm1 * dt, factor2 = tm2 * dt, factor12 = factor1 + factor2;
+    // psio <- psi by rotation: the update writes into the older buffer
+    swap_ptr(cs->psio, f[F_PSI]);
+    if (P.is_track_interface) {
+      if (!P.vof) {
+        FZ_TRY(fz_acdi_transport_dev(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS],
+                                     f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ], f[F_PHI], cs->psio,
+                                     cs->dpsidt, f[F_FX], f[F_FY], f[F_FZ]));
+      } else {
+        FZ_TRY(fz_vof_thinc_transport_psi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_U], f[F_V], f[F_W], f[F_NX],
+                                          f[F_NY], f[F_NZ], f[F_PHI], cs->psio, cs->dpsidt, f[F_FX], f[F_FY],
+                                          f[F_FZ]));
+      }
+      if (is_first) FZ_HIP(hipMemcpyAsync(f[F_DPSIO], cs->dpsidt, bi, hipMemcpyDeviceToDevice, c->st));
+      hipLaunchKernelGGL(k_rk_psi_oop, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, factor1, factor2,
+                         cs->dpsidt, f[F_DPSIO], cs->psio, f[F_PSI]);
+      FZ_LAUNCHED();
+      const double mm[2] = {0.0, 1.0};
+      FZ_TRY(fz_clip_field(c, mm, f[F_PSI]));  // src/rk.f90:248
+      swap_ptr(cs->dpsidt, f[F_DPSIO]);         // src/rk.f90:252
+      FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_FX], f[F_FY], f[F_FZ]));
+      FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
+      FZ_TRY(phase_geometry(cs));
+    } else {
+      FZ_HIP(hipMemcpyAsync(f[F_PSI], cs->psio, sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
+      for (int q = F_FX; q <= F_FZ; q++) FZ_TRY(fz_launch_fill(c, c->nh, 0.0, f[q]));
+    }
+    if (P.is_solve_ns) {
+      double rho_av[3] = {0.0, 0.0, 0.0};
+      for (int d = 0; d < 3; d++) {
+        if (fabs(P.gacc[d]) > 0.0 && c->per[d]) {  // src/main.f90:470-477
+          FZ_TRY(fz_bulk_mean_dev(c, d + 1, f[F_PSI], P.rho12, d < 2 ? cs->gvr_c : cs->gvr_f, cs->dsc + 4 + d));
+          FZ_HIP(hipMemcpyAsync(c->h_small + 8, cs->dsc + 4 + d, sizeof(double), hipMemcpyDeviceToHost, c->st));
+          FZ_HIP(hipStreamSynchronize(c->st));
+          rho_av[d] = c->h_small[8];
+        }
+      }
+      // rk, src/rk.f90:72-130
+      FZ_TRY(fz_mom_xyz_ad(c, P.rho12, P.mu12, f[F_FX], f[F_FY], f[F_FZ], f[F_U], f[F_V], f[F_W], f[F_PSI], cs->du,
+                           cs->dv, cs->dw));
+      if (is_first) {
+        FZ_HIP(hipMemcpyAsync(f[F_DUO], cs->du, bi, hipMemcpyDeviceToDevice, c->st));
+        FZ_HIP(hipMemcpyAsync(f[F_DVO], cs->dv, bi, hipMemcpyDeviceToDevice, c->st));
+        FZ_HIP(hipMemcpyAsync(f[F_DWO], cs->dw, bi, hipMemcpyDeviceToDevice, c->st));
+      }
+      FZ_TRY(fz_launch_rk_mom(c, factor1, factor2, P.rho12, cs->psio, f[F_PSI], cs->du, cs->dv, cs->dw, f[F_DUO],
+                              f[F_DVO], f[F_DWO], f[F_U], f[F_V], f[F_W]));
+      swap_ptr(cs->du, f[F_DUO]);
+      swap_ptr(cs->dv, f[F_DVO]);
+      swap_ptr(cs->dw, f[F_DWO]);
+      const double zero2[2] = {0.0, 0.0};
+      FZ_TRY(fz_mom_xyz_oth(c, dt_r, P.rho12, zero2, P.bforce, P.gacc, P.sigma, rho0, rho_av, f[F_P], f[F_PSI],
+                            f[F_KAPPA], f[F_P], f[F_PO], cs->du, cs->dv, cs->dw));
+      FZ_TRY(fz_launch_rk_oth(c, factor12, P.rho12, f[F_PSI], cs->du, cs->dv, cs->dw, f[F_U], f[F_V], f[F_W]));
+      if (P.is_forced_hit) FZ_TRY(fz_lscale_forcing(c, 2, 0.5, dt, f[F_U], f[F_V], f[F_W]));
+      FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
+      // projection on the correction buffer dp, src/main.f90:488-502
+      FZ_TRY(fz_fillps(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp));
+      FZ_TRY(fz_updt_rhs_b(c, cs->dp));
+      FZ_TRY(fz_solver_dev(c, cs->dp));
+      FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, cs->dp));
+      FZ_TRY(fz_correc(c, rho0, P.rho12, dt, cs->dp, f[F_PSI], f[F_U], f[F_V], f[F_W]));
+      FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 1, f[F_U], f[F_V], f[F_W]));
+      // updatep into the buffer of the previous pressure, then rotate
+      hipLaunchKernelGGL(k_updatep_oop, dim3((int)std::min<long>((c->nh + 255) / 256, 8192)), dim3(256), 0, c->st,
+                         c->nh, cs->dp, f[F_P], f[F_PO]);
+      FZ_LAUNCHED();
+      swap_ptr(f[F_P], f[F_PO]);  // p <- p + dp, po <- p of the step before (= pn)
+      FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));
+    } else {
+      FZ_HIP(hipMemcpyAsync(f[F_PO], f[F_P], sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
+    }
+    cs->sc[S_FIRST] = 0.0;
+    cs->sc[S_DTO] = dt;  // dto = dt, src/main.f90:523
+    if (!P.vof) {
+      FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
+      cs->gam_on_device = true;
+    }
+    const int istep = (int)cs->sc[S_ISTEP];
+    if (P.icheck > 0 && istep % P.icheck == 0) {  // src/main.f90:530-562
+      double dtn;
+      FZ_TRY(new_dt(cs, &dtn, &kill));
+      if (cs->sc[S_DTCFL] < FZ_EPS * 1e7) kill = 1;  // small, src/param.f90:21
+      cs->sc[S_DT] = dtn;
+      FZ_TRY(fz_chkdiv_dev(c, f[F_U], f[F_V], f[F_W], cs->dsc + 1));
+      FZ_HIP(hipMemcpyAsync(c->h_small, cs->dsc + 1, sizeof(double) * 2, hipMemcpyDeviceToHost, c->st));
+      FZ_HIP(hipStreamSynchronize(c->st));
+      cs->sc[S_DIVTOT] = c->h_small[0];
+      cs->sc[S_DIVMAX] = c->h_small[1];
+      if (cs->sc[S_DIVMAX] > FZ_EPS * 1e7 || std::isnan(cs->sc[S_DIVTOT])) kill = 1;
+    }
+  }
+  if (killed) *killed = kill;
+  return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rk / rk_2fl with an explicit history object
+// ---------------------------------------------------------------------------
+struct fz_rk_state {
+  fz_ctx *ctx;
+  double *dpsi = nullptr, *dpsio = nullptr, *du = nullptr, *dv = nullptr, *dw = nullptr, *duo = nullptr,
+         *dvo = nullptr, *dwo = nullptr;
+  bool first_mom = true, first_psi = true;
+};
+
+extern "C" int fz_rk_state_create(fz_ctx *c, fz_rk_state **st) {
+  if (!c || !st) return fz_set_error(FZ_ERR_ARG, "fz_rk_state_create: null argument");
+  fz_rk_state *s = new fz_rk_state();
+  s->ctx = c;
+  double **b[8] = {&s->dpsi, &s->dpsio, &s->du, &s->dv, &s->dw, &s->duo, &s->dvo, &s->dwo};
+  for (int q = 0; q < 8; q++) {
+    hipError_t e = hipMalloc((void **)b[q], sizeof(double) * c->ni);
+    if (e != hipSuccess) {
+      fz_rk_state_destroy(s);
+      return fz_set_error(FZ_ERR_HIP, "fz_rk_state_create: %s", hipGetErrorString(e));
+    }
+  }
+  *st = s;
+  return FZ_OK;
+}
+
+extern "C" int fz_rk_state_destroy(fz_rk_state *s) {
+  if (!s) return FZ_OK;
+  if (s->ctx) (void)hipStreamSynchronize(s->ctx->st);
+  double *b[8] = {s->dpsi, s->dpsio, s->du, s->dv, s->dw, s->duo, s->dvo, s->dwo};
+  for (int q = 0; q < 8; q++)
+    if (b[q]) (void)hipFree(b[q]);
+  delete s;
+  return FZ_OK;
+}
+
+extern "C" int fz_rk(fz_ctx *c, fz_rk_state *s, const double rkpar[2], double dt, double dt_r, const double bforce[3],
+                     const double gacc[3], double sigma, const double rho_av[3], const double rho12[2],
+                     const double mu12[2], const double beta12[2], double rho0, const double *psi,
+                     const double *kappa, const double *p, const double *pn, const double *po, const double *psio,
+                     const double *psiflx_x, const double *psiflx_y, const double *psiflx_z, double *u, double *v,
+                     double *w) {
+  if (!c || !s || !rkpar || s->ctx != c) return fz_set_error(FZ_ERR_ARG, "fz_rk: bad argument");
+  const double factor1 = rkpar[0] * dt, factor2 = rkpar[1] * dt, factor12 = factor1 + factor2;
+  const long bi = sizeof(double) * c->ni;
+  FZ_TRY(fz_mom_xyz_ad(c, rho12, mu12, psiflx_x, psiflx_y, psiflx_z, u, v, w, psi, s->du, s->dv, s->dw));
+  if (s->first_mom) {
+    FZ_HIP(hipMemcpyAsync(s->duo, s->du, bi, hipMemcpyDeviceToDevice, c->st));
+    FZ_HIP(hipMemcpyAsync(s->dvo, s->dv, bi, hipMemcpyDeviceToDevice, c->st));
+    FZ_HIP(hipMemcpyAsync(s->dwo, s->dw, bi, hipMemcpyDeviceToDevice, c->st));
+    s->first_mom = false;
+  }
+  FZ_TRY(fz_launch_rk_mom(c, factor1, factor2, rho12, psio, psi, s->du, s->dv, s->dw, s->duo, s->dvo, s->dwo, u, v, w));
+  swap_ptr(s->du, s->duo);
+  swap_ptr(s->dv, s->dvo);
+  swap_ptr(s->dw, s->dwo);
+  FZ_TRY(fz_mom_xyz_oth(c, dt_r, rho12, beta12, bforce, gacc, sigma, rho0, rho_av, p, psi, kappa, pn, po, s->du, s->dv,
+                        s->dw));
+  return fz_launch_rk_oth(c, factor12, rho12, psi, s->du, s->dv, s->dw, u, v, w);
+}
+
+extern "C" int fz_rk_2fl(fz_ctx *c, fz_rk_state *s, const double rkpar[2], double dt, double gam, double seps,
+                         double beta, int vof, const double *u, const double *v, const double *w, const double *normx,
+                         const double *normy, const double *normz, const double *phi, double *psi, double *psiflx_x,
+                         double *psiflx_y, double *psiflx_z) {
+  if (!c || !s || !rkpar || s->ctx != c) return fz_set_error(FZ_ERR_ARG, "fz_rk_2fl: bad argument");
+  const double factor1 = rkpar[0] * dt, factor2 = rkpar[1] * dt;
+  if (!vof) FZ_TRY(fz_acdi_transport_pf(c, gam, seps, u, v, w, normx, normy, normz, phi, psi, s->dpsi, psiflx_x,
+                                        psiflx_y, psiflx_z));
+  else FZ_TRY(fz_vof_thinc_transport_psi(c, beta, u, v, w, normx, normy, normz, phi, psi, s->dpsi, psiflx_x, psiflx_y,
+                                         psiflx_z));
+  if (s->first_psi) {
+    FZ_HIP(hipMemcpyAsync(s->dpsio, s->dpsi, sizeof(double) * c->ni, hipMemcpyDeviceToDevice, c->st));
+    s->first_psi = false;
+  }
+  FZ_TRY(fz_launch_rk_psi(c, factor1, factor2, s->dpsi, s->dpsio, psi));
+  const double mm[2] = {0.0, 1.0};
+  FZ_TRY(fz_clip_field(c, mm, psi));
+  swap_ptr(s->dpsi, s->dpsio);
+  return FZ_OK;
+}

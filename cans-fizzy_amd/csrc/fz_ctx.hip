@@ -1,0 +1,203 @@
+// Context, memory and grid set-up of libfizzy.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "fz_internal.h"
+
+static thread_local char g_err[512] = "";
+
+int fz_set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+extern "C" const char *fz_last_error(void) { return g_err; }
+
+// initgrid, src/initgrid.f90:14-78. The normalised face coordinate
+// (k-0.)/(1.*n) is a default-real (single precision) quotient in the
+// reference, so it is formed in float here as well.
+static double gridpoint(int gtype, double alpha, double z0) {
+  if (alpha == 0.0) return z0;
+  switch (gtype) {
+    case 2: return 1.0 * (1.0 + tanh((z0 - 1.0) * alpha) / tanh(alpha / 1.0));
+    case 3: return 1.0 - 1.0 * (1.0 + tanh((1.0 - z0 - 1.0) * alpha) / tanh(alpha / 1.0));
+    case 4:
+      if (z0 <= 0.5) return 0.5 * (1.0 - 1.0 + tanh(2.0 * alpha * (z0 - 0.0)) / tanh(alpha));
+      return 0.5 * (1.0 + 1.0 + tanh(2.0 * alpha * (z0 - 1.0)) / tanh(alpha));
+    default: return 0.5 * (1.0 + tanh((z0 - 0.5) * alpha) / tanh(alpha / 2.0));
+  }
+}
+
+extern "C" int fz_initgrid(int gtype, int n, double gr, double lz, double *dzc, double *dzf, double *zc,
+                           double *zf) {
+  if (n < 1 || !dzc || !dzf || !zc || !zf) return fz_set_error(FZ_ERR_ARG, "fz_initgrid: bad arguments");
+  zf[0] = 0.0;
+  for (int k = 1; k <= n; k++) {
+    const double z0 = (double)((float)k / (float)n);
+    zf[k] = gridpoint(gtype, gr, z0) * lz;
+  }
+  for (int k = 1; k <= n; k++) dzf[k] = zf[k] - zf[k - 1];
+  dzf[0] = dzf[1];
+  dzf[n + 1] = dzf[n];
+  for (int k = 0; k <= n; k++) dzc[k] = 0.5 * (dzf[k] + dzf[k + 1]);
+  dzc[n + 1] = dzc[n];
+  zc[0] = -dzc[0] / 2.0;
+  zf[0] = 0.0;
+  for (int k = 1; k <= n + 1; k++) {
+    zc[k] = zc[k - 1] + dzc[k - 1];
+    zf[k] = zf[k - 1] + dzf[k];
+  }
+  return FZ_OK;
+}
+
+static bool valid_bc(char c) { return c == 'P' || c == 'D' || c == 'N'; }
+
+extern "C" int fz_ctx_create(fz_ctx **out, const int n[3], const double l[3], const double *dzc, const double *dzf,
+                             const double *zc, const double *zf, const char cbcpre[6], void *hip_stream) {
+  if (!out || !n || !l || !dzc || !dzf || !zc || !zf || !cbcpre)
+    return fz_set_error(FZ_ERR_ARG, "fz_ctx_create: null argument");
+  for (int d = 0; d < 3; d++)
+    if (n[d] < 1) return fz_set_error(FZ_ERR_ARG, "fz_ctx_create: n(%d)=%d must be >= 1", d + 1, n[d]);
+  for (int q = 0; q < 6; q++)
+    if (!valid_bc(cbcpre[q])) return fz_set_error(FZ_ERR_ARG, "fz_ctx_create: bad cbcpre code '%c'", cbcpre[q]);
+  fz_ctx *c = new fz_ctx();
+  c->d.n1 = n[0];
+  c->d.n2 = n[1];
+  c->d.n3 = n[2];
+  c->d.sy = n[0] + 2;
+  c->d.sz = (long)(n[0] + 2) * (n[1] + 2);
+  c->nh = c->d.sz * (n[2] + 2);
+  c->ni = (long)n[0] * n[1] * n[2];
+  for (int d = 0; d < 3; d++) {
+    c->l[d] = l[d];
+    // dl(:) = l(:)/(1.*ng(:)), src/param.f90:170-171
+    c->dl[d] = l[d] / (double)(float)n[d];
+    c->dli[d] = 1.0 / c->dl[d];
+  }
+  const int nz = n[2] + 2;
+  c->h_dzc.assign(dzc, dzc + nz);
+  c->h_dzf.assign(dzf, dzf + nz);
+  c->h_zc.assign(zc, zc + nz);
+  c->h_zf.assign(zf, zf + nz);
+  c->h_dzci.resize(nz);
+  c->h_dzfi.resize(nz);
+  for (int k = 0; k < nz; k++) {  // dzci = dzc**(-1), src/main.f90:259-260
+    c->h_dzci[k] = 1.0 / dzc[k];
+    c->h_dzfi[k] = 1.0 / dzf[k];
+  }
+  memcpy(c->cbcpre, cbcpre, 6);
+  for (int d = 0; d < 3; d++) c->per[d] = (cbcpre[2 * d] == 'P' && cbcpre[2 * d + 1] == 'P');
+  hipError_t e;
+  if (hip_stream) {
+    c->st = (hipStream_t)hip_stream;
+  } else {
+    e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return fz_set_error(FZ_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    c->own_stream = true;
+  }
+  double **dev[6] = {&c->dzc, &c->dzf, &c->zc, &c->zf, &c->dzci, &c->dzfi};
+  const std::vector<double> *host[6] = {&c->h_dzc, &c->h_dzf, &c->h_zc, &c->h_zf, &c->h_dzci, &c->h_dzfi};
+  for (int q = 0; q < 6; q++) {
+    e = hipMalloc((void **)dev[q], sizeof(double) * nz);
+    if (e == hipSuccess) e = hipMemcpy(*dev[q], host[q]->data(), sizeof(double) * nz, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      fz_ctx_destroy(c);
+      return fz_set_error(FZ_ERR_HIP, "fz_ctx_create grid upload: %s", hipGetErrorString(e));
+    }
+  }
+  e = hipHostMalloc((void **)&c->h_small, 64 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    fz_ctx_destroy(c);
+    return fz_set_error(FZ_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return FZ_OK;
+}
+
+extern "C" int fz_ctx_destroy(fz_ctx *c) {
+  if (!c) return FZ_OK;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  double *dev[6] = {c->dzc, c->dzf, c->zc, c->zf, c->dzci, c->dzfi};
+  for (int q = 0; q < 6; q++)
+    if (dev[q]) (void)hipFree(dev[q]);
+  if (c->pois) fz_poisson_destroy(c->pois);
+  if (c->red) (void)hipFree(c->red);
+  if (c->h_small) (void)hipHostFree(c->h_small);
+  if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return FZ_OK;
+}
+
+extern "C" int fz_ctx_sync(fz_ctx *c) {
+  if (!c) return fz_set_error(FZ_ERR_ARG, "fz_ctx_sync: null context");
+  FZ_HIP(hipStreamSynchronize(c->st));
+  return FZ_OK;
+}
+
+extern "C" int fz_malloc(fz_ctx *c, size_t bytes, void **dptr) {
+  if (!c || !dptr) return fz_set_error(FZ_ERR_ARG, "fz_malloc: null argument");
+  FZ_HIP(hipMalloc(dptr, bytes ? bytes : 1));
+  return FZ_OK;
+}
+
+extern "C" int fz_free(fz_ctx *c, void *dptr) {
+  if (!c) return fz_set_error(FZ_ERR_ARG, "fz_free: null context");
+  if (dptr) {
+    FZ_HIP(hipStreamSynchronize(c->st));
+    FZ_HIP(hipFree(dptr));
+  }
+  return FZ_OK;
+}
+
+extern "C" int fz_upload(fz_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return fz_set_error(FZ_ERR_ARG, "fz_upload: null argument");
+  FZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->st));
+  FZ_HIP(hipStreamSynchronize(c->st));
+  return FZ_OK;
+}
+
+extern "C" int fz_download(fz_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return fz_set_error(FZ_ERR_ARG, "fz_download: null argument");
+  FZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->st));
+  FZ_HIP(hipStreamSynchronize(c->st));
+  return FZ_OK;
+}
+
+extern "C" int fz_copy(fz_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return fz_set_error(FZ_ERR_ARG, "fz_copy: null argument");
+  FZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
+  return FZ_OK;
+}
+
+int fz_reduce_ensure(fz_ctx *c, size_t nblocks) {
+  if (nblocks <= c->red_cap) return FZ_OK;
+  if (c->red) {
+    FZ_HIP(hipStreamSynchronize(c->st));
+    FZ_HIP(hipFree(c->red));
+    c->red = nullptr;
+  }
+  FZ_HIP(hipMalloc((void **)&c->red, sizeof(double) * 2 * nblocks));
+  c->red_cap = nblocks;
+  return FZ_OK;
+}
+
+// acdi_set_epsilon, src/acdi.f90:16-30: dlmax over dl(1:2) and 1/dzfi(0:n3+1)
+extern "C" int fz_acdi_set_epsilon(fz_ctx *c, double seps_factor, double *seps) {
+  if (!c || !seps) return fz_set_error(FZ_ERR_ARG, "fz_acdi_set_epsilon: null argument");
+  double dlmax = c->dl[0] > c->dl[1] ? c->dl[0] : c->dl[1];
+  double m = 1.0 / c->h_dzfi[0];
+  for (size_t k = 1; k < c->h_dzfi.size(); k++) {
+    const double v = 1.0 / c->h_dzfi[k];
+    if (v > m) m = v;
+  }
+  if (m > dlmax) dlmax = m;
+  *seps = dlmax * seps_factor;
+  return FZ_OK;
+}

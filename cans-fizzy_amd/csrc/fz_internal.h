@@ -1,0 +1,94 @@
+// Internal declarations of libfizzy (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/fizzy.h"
+
+#define FZ_EPS 2.220446049250313080847e-16  // epsilon(1._rp)
+#define FZ_PI 3.141592653589793115998        // acos(-1._rp)
+
+int fz_set_error(int code, const char *fmt, ...);
+
+#define FZ_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) return fz_set_error(FZ_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+#define FZ_LAUNCHED() FZ_HIP(hipGetLastError())
+#define FZ_TRY(call)          \
+  do {                        \
+    int r_ = (call);          \
+    if (r_ != FZ_OK) return r_; \
+  } while (0)
+
+// Geometry of one field in HBM: the reference layout, column-major with a
+// one-cell halo; (i,j,k) lives at i + sy*j + sz*k.
+struct fz_dims {
+  int n1, n2, n3;
+  long sy, sz;
+};
+
+// Batched line transform plan along one axis (r2r kinds as FFTW's).
+struct fz_fftplan {
+  int n = 0;         // real line length
+  int m = 0;         // complex FFT length (n, or 2n for the quarter-wave kinds)
+  int kind_f = 0, kind_b = 0;
+  int nrad = 0;
+  int rad[24] = {0};
+  double2 *tw = nullptr;  // exp(-2 pi i j / m), j < m
+  double2 *q = nullptr;   // (cos, sin)(pi k / 2n), k <= n
+  double2 *t2 = nullptr;  // (cos, sin)(pi (2k+1) / 4n), k < n
+};
+
+struct fz_poisson {
+  fz_fftplan px, py;
+  double normfft = 1.0;
+  double *lambdaxy = nullptr;  // device n1*n2
+  double *a = nullptr, *b = nullptr, *c = nullptr;  // device n3
+  double *rhsbx = nullptr, *rhsby = nullptr, *rhsbz = nullptr;  // device
+  double *work = nullptr;   // spectral buffer, n1*n2*n3
+  double *piv = nullptr;    // Thomas pivots, n1*n2*n3
+  double *aux = nullptr;    // periodic-z second solve, n1*n2*n3
+  int periodic_z = 0;
+  char cbc[6];
+};
+
+struct fz_ctx {
+  fz_dims d;
+  long nh = 0, ni = 0;  // element counts of a halo'd and of an interior array
+  double l[3], dl[3], dli[3];
+  std::vector<double> h_dzc, h_dzf, h_zc, h_zf, h_dzci, h_dzfi;
+  double *dzc = nullptr, *dzf = nullptr, *zc = nullptr, *zf = nullptr, *dzci = nullptr, *dzfi = nullptr;
+  char cbcpre[6];
+  int per[3];
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  fz_poisson *pois = nullptr;
+  double *red = nullptr;      // per-block partial results of reductions
+  size_t red_cap = 0;
+  double *h_small = nullptr;  // pinned host landing zone for reduction results
+};
+
+// ---- launch helpers ----
+static inline dim3 fz_block3() { return dim3(64, 4, 1); }
+static inline dim3 fz_grid_interior(const fz_dims &d) {
+  return dim3((d.n1 + 63) / 64, (d.n2 + 3) / 4, d.n3);
+}
+static inline dim3 fz_grid_halo(const fz_dims &d) {
+  return dim3((d.n1 + 2 + 63) / 64, (d.n2 + 2 + 3) / 4, d.n3 + 2);
+}
+
+// internal entry points shared between translation units
+int fz_reduce_ensure(fz_ctx *ctx, size_t nblocks);
+int fz_poisson_destroy(fz_poisson *p);
+int fz_solver_dev(fz_ctx *ctx, double *p);
+int fz_acdi_gamma_dev(fz_ctx *ctx, double gam_factor, double gam_min, const double *u, const double *v,
+                      const double *w, double *d_gam);
+int fz_acdi_transport_dev(fz_ctx *ctx, const double *d_gam, double gam, double seps, const double *u,
+                          const double *v, const double *w, const double *normx, const double *normy,
+                          const double *normz, const double *phi, const double *psi, double *dpsidt,
+                          double *flux_x, double *flux_y, double *flux_z);

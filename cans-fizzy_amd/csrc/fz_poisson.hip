@@ -1,0 +1,540 @@
+// FFT-based pressure Poisson solver on one MI355X.
+//
+// Replaces initsolver (src/initsolver.f90:15-228), solver/solver_gpu
+// (src/solver.f90:19-177, src/solver_gpu.f90:31-370) and the r2r transforms
+// of src/fft.f90. The x and y transforms are batched line transforms done in
+// LDS (Stockham autosort, mixed radix) with the FFTW r2r definitions the
+// reference plans (find_fft, src/fft.f90:189-242); the z direction is the
+// reference's Thomas recurrence, one column per thread. The pencil
+// transposes of the reference are not needed: the transforms address the
+// lines of one HBM-resident field directly.
+#include <cmath>
+#include <cstring>
+
+#include "fz_internal.h"
+
+enum { K_R2HC = 0, K_HC2R = 1, K_REDFT01 = 4, K_REDFT10 = 5, K_REDFT11 = 6, K_RODFT01 = 8, K_RODFT10 = 9, K_RODFT11 = 10 };
+
+struct RadList {
+  int r[24];
+};
+struct LineMap {  // element e of line (l1,l2) sits at base + l1*s1 + l2*s2 + e*se
+  long base, s1, s2, se;
+};
+
+static __host__ __device__ inline bool kind_is_quarter(int kind) { return kind == K_REDFT11 || kind == K_RODFT11; }
+static __host__ __device__ inline bool kind_is_forward(int kind) {
+  return kind == K_R2HC || kind == K_REDFT10 || kind == K_RODFT10 || kind == K_REDFT11 || kind == K_RODFT11;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// One Stockham pass of radix R over nfft complex lines of length m (LDS to LDS).
+__device__ inline void stockham_pass(const double2 *__restrict__ X, double2 *__restrict__ Y, int nfft, int m, int R,
+                                     int Ns, double sign, const double2 *__restrict__ tw) {
+  const int nb = m / R;
+  const int total = nfft * nb;
+  const int tstride = m / (Ns * R);
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int f = idx / nb, b = idx - f * nb;
+    const double2 *x = X + (long)f * m;
+    double2 *y = Y + (long)f * m;
+    const int k = b % Ns;
+    const int base = (b / Ns) * Ns * R + k;
+    double2 v[16];
+    for (int t = 0; t < R; t++) {
+      double2 xv = x[b + t * nb];
+      if (t > 0 && k > 0) {
+        double2 w = tw[t * k * tstride];  // exp(-2 pi i j/m); the inverse uses its conjugate
+        if (sign > 0) w.y = -w.y;
+        xv = cmul(xv, w);
+      }
+      v[t] = xv;
+    }
+    if (R == 2) {
+      y[base] = cadd(v[0], v[1]);
+      y[base + Ns] = csub(v[0], v[1]);
+    } else if (R == 4) {
+      const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+      const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+      // i*sign*d13
+      const double2 r = make_double2(-sign * d13.y, sign * d13.x);
+      y[base] = cadd(s02, s13);
+      y[base + Ns] = cadd(d02, r);
+      y[base + 2 * Ns] = csub(s02, s13);
+      y[base + 3 * Ns] = csub(d02, r);
+    } else if (R == 3) {
+      const double c3 = 0.86602540378443864676;  // sqrt(3)/2
+      const double2 t1 = cadd(v[1], v[2]);
+      const double2 t2 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+      const double2 d = csub(v[1], v[2]);
+      const double2 t3 = make_double2(-sign * c3 * d.y, sign * c3 * d.x);
+      y[base] = cadd(v[0], t1);
+      y[base + Ns] = cadd(t2, t3);
+      y[base + 2 * Ns] = csub(t2, t3);
+    } else {
+      const int st = m / R;
+      for (int s = 0; s < R; s++) {
+        double2 acc = v[0];
+        for (int t = 1; t < R; t++) {
+          const double2 w0 = tw[((t * s) % R) * st];
+          const double2 w = make_double2(w0.x, sign > 0 ? -w0.y : w0.y);
+          acc = cadd(acc, cmul(v[t], w));
+        }
+        y[base + s * Ns] = acc;
+      }
+    }
+  }
+}
+
+// Batched r2r line transform: G lines per block, all in LDS.
+__global__ void __launch_bounds__(256) k_r2r(const double *__restrict__ src, LineMap in, double *__restrict__ dst,
+                                             LineMap out, int L1, int n, int m, int kind, int G, double scale,
+                                             const double2 *__restrict__ tw, const double2 *__restrict__ q,
+                                             const double2 *__restrict__ t2, int nrad, RadList rl) {
+  extern __shared__ double2 smem[];
+  const bool quarter = kind_is_quarter(kind);
+  const int nfft = quarter ? G : G / 2;
+  double2 *A = smem;
+  double2 *B = smem + (long)nfft * m;
+  double *Rw = (double *)B;  // raw staging, G*n doubles
+  const int l1_0 = blockIdx.x * G;
+  const long l2 = blockIdx.y;
+  const int gcount = min(G, L1 - l1_0);
+  const int tot = G * n;
+  // 1. load the G lines
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    int g, e;
+    if (in.se == 1) { g = idx / n; e = idx - g * n; }
+    else { e = idx / G; g = idx - e * G; }
+    Rw[g * n + e] = (g < gcount) ? src[in.base + (long)(l1_0 + g) * in.s1 + l2 * in.s2 + (long)e * in.se] : 0.0;
+  }
+  __syncthreads();
+  // 2. pre-processing into complex lines
+  const int totc = nfft * m;
+  for (int idx = threadIdx.x; idx < totc; idx += blockDim.x) {
+    const int f = idx / m, k = idx - f * m;
+    double2 z;
+    switch (kind) {
+      case K_R2HC: z = make_double2(Rw[(2 * f) * n + k], Rw[(2 * f + 1) * n + k]); break;
+      case K_REDFT10:
+      case K_RODFT10: {
+        const int e = (k <= (n - 1) / 2) ? 2 * k : 2 * (n - 1 - k) + 1;
+        const double sg = (kind == K_RODFT10 && (e & 1)) ? -1.0 : 1.0;
+        z = make_double2(sg * Rw[(2 * f) * n + e], sg * Rw[(2 * f + 1) * n + e]);
+        break;
+      }
+      case K_REDFT11:
+      case K_RODFT11: {
+        if (k < n) {
+          const double sg = (kind == K_RODFT11 && (k & 1)) ? -1.0 : 1.0;
+          const double x = sg * Rw[f * n + k];
+          z = make_double2(x * q[k].x, -x * q[k].y);
+        } else {
+          z = make_double2(0.0, 0.0);
+        }
+        break;
+      }
+      case K_HC2R: {
+        const double *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
+        double ra, ia, rb, ib;
+        if (k == 0) { ra = Ha[0]; ia = 0.0; rb = Hb[0]; ib = 0.0; }
+        else if (2 * k < n) { ra = Ha[k]; ia = Ha[n - k]; rb = Hb[k]; ib = Hb[n - k]; }
+        else if (2 * k == n) { ra = Ha[k]; ia = 0.0; rb = Hb[k]; ib = 0.0; }
+        else { ra = Ha[n - k]; ia = -Ha[k]; rb = Hb[n - k]; ib = -Hb[k]; }
+        z = make_double2(ra - ib, ia + rb);
+        break;
+      }
+      case K_REDFT01:
+      case K_RODFT01: {
+        const double *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
+        double xa, xna, xb, xnb;
+        if (kind == K_REDFT01) {
+          xa = Ha[k]; xb = Hb[k];
+          xna = (k == 0) ? 0.0 : Ha[n - k];
+          xnb = (k == 0) ? 0.0 : Hb[n - k];
+        } else {  // input reversed
+          xa = Ha[n - 1 - k]; xb = Hb[n - 1 - k];
+          xna = (k == 0) ? 0.0 : Ha[k - 1];
+          xnb = (k == 0) ? 0.0 : Hb[k - 1];
+        }
+        const double cs = q[k].x, sn = q[k].y;
+        const double rca = xa * cs + xna * sn, ica = xa * sn - xna * cs;
+        const double rcb = xb * cs + xnb * sn, icb = xb * sn - xnb * cs;
+        z = make_double2(rca - icb, ica + rcb);
+        break;
+      }
+      default: z = make_double2(0.0, 0.0);
+    }
+    A[idx] = z;
+  }
+  __syncthreads();
+  // 3. complex FFT passes
+  const double sign = kind_is_forward(kind) ? -1.0 : 1.0;
+  double2 *cur = A, *oth = B;
+  int Ns = 1;
+  for (int s = 0; s < nrad; s++) {
+    const int R = rl.r[s];
+    stockham_pass(cur, oth, nfft, m, R, Ns, sign, tw);
+    __syncthreads();
+    double2 *t = cur; cur = oth; oth = t;
+    Ns *= R;
+  }
+  // 4. post-processing into real lines
+  double *O = (double *)oth;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int g = idx / n, k = idx - g * n;
+    double val;
+    switch (kind) {
+      case K_R2HC: {
+        const double2 *Z = cur + (long)(g >> 1) * m;
+        const int kk = (2 * k <= n) ? k : n - k;
+        const double2 zk = Z[kk], znk = Z[(m - kk) % m];
+        double re, im;
+        if ((g & 1) == 0) { re = 0.5 * (zk.x + znk.x); im = 0.5 * (zk.y - znk.y); }
+        else { re = 0.5 * (zk.y + znk.y); im = 0.5 * (znk.x - zk.x); }
+        val = (2 * k <= n) ? re : im;
+        break;
+      }
+      case K_REDFT10:
+      case K_RODFT10: {
+        const double2 *Z = cur + (long)(g >> 1) * m;
+        const int kk = (kind == K_REDFT10) ? k : n - 1 - k;
+        const double2 zk = Z[kk], znk = Z[(m - kk) % m];
+        double re, im;
+        if ((g & 1) == 0) { re = 0.5 * (zk.x + znk.x); im = 0.5 * (zk.y - znk.y); }
+        else { re = 0.5 * (zk.y + znk.y); im = 0.5 * (znk.x - zk.x); }
+        val = 2.0 * (q[kk].x * re + q[kk].y * im);
+        break;
+      }
+      case K_REDFT11:
+      case K_RODFT11: {
+        const int kk = (kind == K_REDFT11) ? k : n - 1 - k;
+        const double2 zk = cur[(long)g * m + kk];
+        val = 2.0 * (t2[kk].x * zk.x + t2[kk].y * zk.y);
+        break;
+      }
+      case K_HC2R: {
+        const double2 zk = cur[(long)(g >> 1) * m + k];
+        val = (g & 1) ? zk.y : zk.x;
+        break;
+      }
+      case K_REDFT01:
+      case K_RODFT01: {
+        const int srcm = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
+        const double2 zk = cur[(long)(g >> 1) * m + srcm];
+        val = (g & 1) ? zk.y : zk.x;
+        if (kind == K_RODFT01 && (k & 1)) val = -val;
+        break;
+      }
+      default: val = 0.0;
+    }
+    O[idx] = val;
+  }
+  __syncthreads();
+  // 5. store
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    int g, e;
+    if (out.se == 1) { g = idx / n; e = idx - g * n; }
+    else { e = idx / G; g = idx - e * G; }
+    if (g < gcount) dst[out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se] = O[g * n + e] * scale;
+  }
+}
+
+// Thomas recurrence of dgtsv_homebrewed (src/solver.f90:151-177) on one
+// column of the spectral buffer; pivots go to piv (same layout).
+__device__ inline void thomas_col(double *__restrict__ col, double *__restrict__ dcol, long ps, int n,
+                                  const double *__restrict__ a, const double *__restrict__ b,
+                                  const double *__restrict__ c, double lam) {
+  double bb = b[0] + lam;
+  double z = 1.0 / (bb + FZ_EPS);
+  double d = c[0] * z;
+  double pprev = col[0] * z;
+  dcol[0] = d;
+  col[0] = pprev;
+  for (int l = 1; l < n; l++) {
+    bb = b[l] + lam;
+    z = 1.0 / (bb - a[l] * d + FZ_EPS);
+    d = c[l] * z;
+    pprev = (col[l * ps] - a[l] * pprev) * z;
+    dcol[l * ps] = d;
+    col[l * ps] = pprev;
+  }
+  double pnext = pprev;
+  for (int l = n - 2; l >= 0; l--) {
+    pnext = col[l * ps] - dcol[l * ps] * pnext;
+    col[l * ps] = pnext;
+  }
+}
+
+__global__ void k_thomas(double *__restrict__ W, double *__restrict__ D, const double *__restrict__ lambdaxy,
+                         const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ c,
+                         int n1, int n2, int n3) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n1 * n2) return;
+  const long ps = (long)n1 * n2;
+  thomas_col(W + idx, D + idx, ps, n3, a, b, c, lambdaxy[idx]);
+}
+
+// gaussel_periodic, src/solver.f90:107-149
+__global__ void k_thomas_periodic(double *__restrict__ W, double *__restrict__ D, double *__restrict__ P2,
+                                  const double *__restrict__ lambdaxy, const double *__restrict__ a,
+                                  const double *__restrict__ b, const double *__restrict__ c, int n1, int n2,
+                                  int n3) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n1 * n2) return;
+  const long ps = (long)n1 * n2;
+  const double lam = lambdaxy[idx];
+  const int m = n3 - 1;
+  double *col = W + idx, *p2 = P2 + idx;
+  thomas_col(col, D + idx, ps, m, a, b, c, lam);
+  for (int l = 0; l < m; l++) p2[l * ps] = 0.0;
+  p2[0] = -a[0];
+  p2[(m - 1) * ps] = -c[m - 1];
+  thomas_col(p2, D + idx, ps, m, a, b, c, lam);
+  const double bbm = b[m] + lam;
+  const double pn = (col[m * ps] - c[m] * col[0] - a[m] * col[(m - 1) * ps]) /
+                    (bbm + c[m] * p2[0] + a[m] * p2[(m - 1) * ps] + FZ_EPS);
+  col[m * ps] = pn;
+  for (int l = 0; l < m; l++) col[l * ps] = col[l * ps] + p2[l * ps] * pn;
+}
+
+// ---------------------------------------------------------------------------
+// host side: operator set-up
+// ---------------------------------------------------------------------------
+static void eigenvalues_c(int n, char b0, char b1, double *lambda) {
+  for (int l = 1; l <= n; l++) {  // src/initsolver.f90:89-138, cell-centred
+    double arg;
+    if (b0 == 'P' && b1 == 'P') arg = ((double)(2 * (l - 1)) * FZ_PI) / (double)n;
+    else if (b0 == 'N' && b1 == 'N') arg = ((double)(l - 1) * FZ_PI) / (double)n;
+    else if (b0 == 'D' && b1 == 'D') arg = ((double)l * FZ_PI) / (double)n;
+    else arg = ((double)(2 * l - 1) * FZ_PI) / (2.0 * n);
+    lambda[l - 1] = -2.0 * (1.0 - cos(arg));
+  }
+}
+
+static void find_fft_c(char b0, char b1, int *kf, int *kb, double *norm) {  // src/fft.f90:195-217
+  if (b0 == 'P' && b1 == 'P') { *kf = K_R2HC; *kb = K_HC2R; *norm = 1.0; }
+  else if (b0 == 'N' && b1 == 'N') { *kf = K_REDFT10; *kb = K_REDFT01; *norm = 2.0; }
+  else if (b0 == 'D' && b1 == 'D') { *kf = K_RODFT10; *kb = K_RODFT01; *norm = 2.0; }
+  else if (b0 == 'N' && b1 == 'D') { *kf = K_REDFT11; *kb = K_REDFT11; *norm = 2.0; }
+  else { *kf = K_RODFT11; *kb = K_RODFT11; *norm = 2.0; }
+}
+
+static double bc_factor_c(char cbc, int ib, double bc, double dlc) {  // src/initsolver.f90:196-209
+  if (cbc == 'P') return 0.0;
+  if (cbc == 'D') return -2.0 * bc;
+  const double sgn = (ib == 0) ? 1.0 : -1.0;
+  return sgn * dlc * bc;
+}
+
+static int factorize(int m, int *rad, int *nrad) {
+  int k = 0;
+  while (m % 4 == 0) { rad[k++] = 4; m /= 4; }
+  while (m % 2 == 0) { rad[k++] = 2; m /= 2; }
+  for (int p = 3; m > 1; p += 2) {
+    while (m % p == 0) {
+      if (p > 13 || k >= 24) return -1;
+      rad[k++] = p;
+      m /= p;
+    }
+  }
+  *nrad = k;
+  return 0;
+}
+
+static int fftplan_init(fz_fftplan *P, char b0, char b1, int n) {
+  double nrm;
+  find_fft_c(b0, b1, &P->kind_f, &P->kind_b, &nrm);
+  P->n = n;
+  P->m = kind_is_quarter(P->kind_f) ? 2 * n : n;
+  if (P->m > 2048)
+    return fz_set_error(FZ_ERR_UNSUPPORTED, "line transforms longer than 2048 points are not supported (n=%d)", n);
+  if (factorize(P->m, P->rad, &P->nrad) != 0)
+    return fz_set_error(FZ_ERR_UNSUPPORTED, "transform length %d has a prime factor above 13", P->m);
+  std::vector<double2> tw(P->m), q(n + 1), t2(n + 1);
+  for (int j = 0; j < P->m; j++) tw[j] = make_double2(cos(2.0 * FZ_PI * (double)j / (double)P->m),
+                                                      -sin(2.0 * FZ_PI * (double)j / (double)P->m));
+  for (int k = 0; k <= n; k++) {
+    q[k] = make_double2(cos(FZ_PI * (double)k / (2.0 * n)), sin(FZ_PI * (double)k / (2.0 * n)));
+    t2[k] = make_double2(cos(FZ_PI * (double)(2 * k + 1) / (4.0 * n)), sin(FZ_PI * (double)(2 * k + 1) / (4.0 * n)));
+  }
+  FZ_HIP(hipMalloc((void **)&P->tw, sizeof(double2) * P->m));
+  FZ_HIP(hipMalloc((void **)&P->q, sizeof(double2) * (n + 1)));
+  FZ_HIP(hipMalloc((void **)&P->t2, sizeof(double2) * (n + 1)));
+  FZ_HIP(hipMemcpy(P->tw, tw.data(), sizeof(double2) * P->m, hipMemcpyHostToDevice));
+  FZ_HIP(hipMemcpy(P->q, q.data(), sizeof(double2) * (n + 1), hipMemcpyHostToDevice));
+  FZ_HIP(hipMemcpy(P->t2, t2.data(), sizeof(double2) * (n + 1), hipMemcpyHostToDevice));
+  return FZ_OK;
+}
+
+static void fftplan_free(fz_fftplan *P) {
+  if (P->tw) (void)hipFree(P->tw);
+  if (P->q) (void)hipFree(P->q);
+  if (P->t2) (void)hipFree(P->t2);
+  P->tw = P->q = P->t2 = nullptr;
+}
+
+int fz_poisson_destroy(fz_poisson *p) {
+  if (!p) return FZ_OK;
+  fftplan_free(&p->px);
+  fftplan_free(&p->py);
+  double *bufs[9] = {p->lambdaxy, p->a, p->b, p->c, p->rhsbx, p->rhsby, p->rhsbz, p->work, p->piv};
+  for (int q = 0; q < 9; q++)
+    if (bufs[q]) (void)hipFree(bufs[q]);
+  if (p->aux) (void)hipFree(p->aux);
+  delete p;
+  return FZ_OK;
+}
+
+extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdaxy_h, double *a_h, double *b_h,
+                             double *c_h, double *normfft_h, double *rhsbx_h, double *rhsby_h, double *rhsbz_h) {
+  if (!ctx) return fz_set_error(FZ_ERR_ARG, "fz_initsolver: null context");
+  const int n1 = ctx->d.n1, n2 = ctx->d.n2, n3 = ctx->d.n3;
+  const char *cbc = ctx->cbcpre;
+  double bc[6] = {0, 0, 0, 0, 0, 0};
+  if (bcpre) memcpy(bc, bcpre, sizeof(bc));
+  std::vector<double> lx(n1), ly(n2), lxy((size_t)n1 * n2), a(n3), b(n3), c(n3);
+  eigenvalues_c(n1, cbc[0], cbc[1], lx.data());
+  for (int i = 0; i < n1; i++) lx[i] = lx[i] * (ctx->dli[0] * ctx->dli[0]);
+  eigenvalues_c(n2, cbc[2], cbc[3], ly.data());
+  for (int j = 0; j < n2; j++) ly[j] = ly[j] * (ctx->dli[1] * ctx->dli[1]);
+  for (int j = 0; j < n2; j++)
+    for (int i = 0; i < n1; i++) lxy[i + (size_t)n1 * j] = lx[i] + ly[j];
+  const double *dzci = ctx->h_dzci.data(), *dzfi = ctx->h_dzfi.data();
+  for (int k = 1; k <= n3; k++) {  // tridmatrix, src/initsolver.f90:153-182
+    a[k - 1] = dzfi[k] * dzci[k - 1];
+    c[k - 1] = dzfi[k] * dzci[k];
+  }
+  for (int k = 0; k < n3; k++) b[k] = -(a[k] + c[k]);
+  double f0 = 0.0, f1 = 0.0;
+  if (cbc[4] == 'D') f0 = -1.0; else if (cbc[4] == 'N') f0 = 1.0;
+  if (cbc[5] == 'D') f1 = -1.0; else if (cbc[5] == 'N') f1 = 1.0;
+  b[0] = b[0] + f0 * a[0];
+  b[n3 - 1] = b[n3 - 1] + f1 * c[n3 - 1];
+  // boundary right-hand sides, src/initsolver.f90:65-74
+  const double dl0 = 1.0 / ctx->dli[0], dl1 = 1.0 / ctx->dli[1];
+  const double dzc0 = 1.0 / dzci[0], dzcn = 1.0 / dzci[n3], dzf1 = 1.0 / dzfi[1], dzfn = 1.0 / dzfi[n3];
+  std::vector<double> rbx((size_t)n2 * n3 * 2), rby((size_t)n1 * n3 * 2), rbz((size_t)n1 * n2 * 2);
+  for (int ib = 0; ib < 2; ib++) {
+    const double fx = bc_factor_c(cbc[0 + ib], ib, bc[0 + ib], dl0) / dl0 / dl0;
+    const double fy = bc_factor_c(cbc[2 + ib], ib, bc[2 + ib], dl1) / dl1 / dl1;
+    const double dlc = ib == 0 ? dzc0 : dzcn, dlf = ib == 0 ? dzf1 : dzfn;
+    const double fz = bc_factor_c(cbc[4 + ib], ib, bc[4 + ib], dlc) / dlc / dlf;
+    for (size_t q = 0; q < (size_t)n2 * n3; q++) rbx[q + (size_t)n2 * n3 * ib] = fx;
+    for (size_t q = 0; q < (size_t)n1 * n3; q++) rby[q + (size_t)n1 * n3 * ib] = fy;
+    for (size_t q = 0; q < (size_t)n1 * n2; q++) rbz[q + (size_t)n1 * n2 * ib] = fz;
+  }
+  double nx, ny;
+  int kf, kb;
+  find_fft_c(cbc[0], cbc[1], &kf, &kb, &nx);
+  find_fft_c(cbc[2], cbc[3], &kf, &kb, &ny);
+  double nf = 1.0;  // normfft, src/fft.f90:59-139
+  nf = nf * nx * (double)n1;
+  nf = nf * ny * (double)n2;
+  const double normfft = 1.0 / nf;
+  if (lambdaxy_h) memcpy(lambdaxy_h, lxy.data(), sizeof(double) * lxy.size());
+  if (a_h) memcpy(a_h, a.data(), sizeof(double) * n3);
+  if (b_h) memcpy(b_h, b.data(), sizeof(double) * n3);
+  if (c_h) memcpy(c_h, c.data(), sizeof(double) * n3);
+  if (normfft_h) *normfft_h = normfft;
+  if (rhsbx_h) memcpy(rhsbx_h, rbx.data(), sizeof(double) * rbx.size());
+  if (rhsby_h) memcpy(rhsby_h, rby.data(), sizeof(double) * rby.size());
+  if (rhsbz_h) memcpy(rhsbz_h, rbz.data(), sizeof(double) * rbz.size());
+  // device plan
+  if (ctx->pois) {
+    FZ_HIP(hipStreamSynchronize(ctx->st));
+    fz_poisson_destroy(ctx->pois);
+    ctx->pois = nullptr;
+  }
+  fz_poisson *P = new fz_poisson();
+  memcpy(P->cbc, cbc, 6);
+  P->normfft = normfft;
+  P->periodic_z = (cbc[4] == 'P' && cbc[5] == 'P');
+  int rc = fftplan_init(&P->px, cbc[0], cbc[1], n1);
+  if (rc == FZ_OK) rc = fftplan_init(&P->py, cbc[2], cbc[3], n2);
+  if (rc != FZ_OK) {
+    fz_poisson_destroy(P);
+    return rc;
+  }
+  const size_t N = (size_t)n1 * n2 * n3;
+  struct {
+    double **d;
+    const std::vector<double> *h;
+  } up[7] = {{&P->lambdaxy, &lxy}, {&P->a, &a}, {&P->b, &b}, {&P->c, &c}, {&P->rhsbx, &rbx}, {&P->rhsby, &rby},
+             {&P->rhsbz, &rbz}};
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < 7 && e == hipSuccess; q++) {
+    e = hipMalloc((void **)up[q].d, sizeof(double) * up[q].h->size());
+    if (e == hipSuccess) e = hipMemcpy(*up[q].d, up[q].h->data(), sizeof(double) * up[q].h->size(), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipMalloc((void **)&P->work, sizeof(double) * N);
+  if (e == hipSuccess) e = hipMalloc((void **)&P->piv, sizeof(double) * N);
+  if (e == hipSuccess && P->periodic_z) e = hipMalloc((void **)&P->aux, sizeof(double) * N);
+  if (e != hipSuccess) {
+    fz_poisson_destroy(P);
+    return fz_set_error(FZ_ERR_HIP, "fz_initsolver: %s", hipGetErrorString(e));
+  }
+  ctx->pois = P;
+  return FZ_OK;
+}
+
+static int launch_r2r(fz_ctx *ctx, const fz_fftplan &P, int kind, const double *src, LineMap in, double *dst,
+                      LineMap out, int L1, int L2, double scale) {
+  const bool quarter = kind_is_quarter(kind);
+  int G;
+  size_t lds;
+  if (quarter) {
+    G = 2048 / P.m;
+    if (G < 1) G = 1;
+    if (G > 64) G = 64;
+    lds = (size_t)G * P.m * sizeof(double2) * 2;
+  } else {
+    G = 4096 / P.m;
+    if (G < 2) G = 2;
+    if (G > 64) G = 64;
+    G &= ~1;
+    lds = (size_t)(G / 2) * P.m * sizeof(double2) * 2;
+  }
+  RadList rl;
+  for (int s = 0; s < 24; s++) rl.r[s] = P.rad[s];
+  dim3 grid((L1 + G - 1) / G, L2, 1);
+  hipLaunchKernelGGL(k_r2r, grid, dim3(256), lds, ctx->st, src, in, dst, out, L1, P.n, P.m, kind, G, scale, P.tw,
+                     P.q, P.t2, P.nrad, rl);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+int fz_solver_dev(fz_ctx *ctx, double *p) {
+  fz_poisson *P = ctx->pois;
+  if (!P) return fz_set_error(FZ_ERR_STATE, "fz_solver: fz_initsolver has not been called");
+  const int n1 = ctx->d.n1, n2 = ctx->d.n2, n3 = ctx->d.n3;
+  const long sy = ctx->d.sy, sz = ctx->d.sz;
+  const long ps = (long)n1 * n2;
+  double *W = P->work;
+  const LineMap p_x = {1 + sy + sz, sy, sz, 1};  // x-lines of the halo'd field, (j,k) -> line
+  const LineMap w_x = {0, n1, ps, 1};
+  const LineMap w_y = {0, 1, ps, n1};
+  FZ_TRY(launch_r2r(ctx, P->px, P->px.kind_f, p, p_x, W, w_x, n2, n3, 1.0));
+  FZ_TRY(launch_r2r(ctx, P->py, P->py.kind_f, W, w_y, W, w_y, n1, n3, 1.0));
+  const int tpb = 256;
+  const int nb = (int)((ps + tpb - 1) / tpb);
+  if (!P->periodic_z) {
+    hipLaunchKernelGGL(k_thomas, dim3(nb), dim3(tpb), 0, ctx->st, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
+                       n3);
+  } else {
+    hipLaunchKernelGGL(k_thomas_periodic, dim3(nb), dim3(tpb), 0, ctx->st, W, P->piv, P->aux, P->lambdaxy, P->a,
+                       P->b, P->c, n1, n2, n3);
+  }
+  FZ_LAUNCHED();
+  FZ_TRY(launch_r2r(ctx, P->py, P->py.kind_b, W, w_y, W, w_y, n1, n3, 1.0));
+  FZ_TRY(launch_r2r(ctx, P->px, P->px.kind_b, W, w_x, p, p_x, n2, n3, P->normfft));
+  return FZ_OK;
+}
+
+extern "C" int fz_solver(fz_ctx *ctx, double *p) {
+  if (!ctx || !p) return fz_set_error(FZ_ERR_ARG, "fz_solver: null argument");
+  return fz_solver_dev(ctx, p);
+}

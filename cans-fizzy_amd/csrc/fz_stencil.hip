@@ -1,0 +1,760 @@
+// Cell stencils of the per-timestep hot path: pressure RHS and projection,
+// ACDI and THINC/QQ phase-field transport, Youngs normals and curvature,
+// momentum advection/diffusion and the pressure/surface-tension/gravity
+// split, AB2 updates, forcing and the initial phase field.
+//
+// One thread per cell, x fastest across a wavefront so every field load is a
+// coalesced 512-byte row segment; neighbours in y and z come from L1/L2.
+// Expressions keep the reference's operation order (and the library is built
+// without FMA contraction) so that, transcendental functions aside, results
+// round exactly as the reference's own CPU build does.
+#include <cmath>
+
+#include "fz_internal.h"
+
+#define CELL_IJK(d)                                   \
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1; \
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1; \
+  const int k = blockIdx.z + 1;                       \
+  if (i > (d).n1 || j > (d).n2) return;               \
+  const long q = i + (d).sy * j + (d).sz * k;         \
+  const long r = (i - 1) + (long)(d).n1 * ((j - 1) + (long)(d).n2 * (k - 1));
+
+#define HALO_IJK(d)                                   \
+  const int i = blockIdx.x * blockDim.x + threadIdx.x; \
+  const int j = blockIdx.y * blockDim.y + threadIdx.y; \
+  const int k = blockIdx.z;                           \
+  if (i > (d).n1 + 1 || j > (d).n2 + 1) return;       \
+  const long q = i + (d).sy * j + (d).sz * k;
+
+// ---------------------------------------------------------------------------
+// fillps, src/fillps.f90:33-50
+__global__ void k_fillps(fz_dims d, const double *__restrict__ dzfi, double dti, double dtidxi, double dtidyi,
+                         double rho0, const double *__restrict__ u, const double *__restrict__ v,
+                         const double *__restrict__ w, double *__restrict__ p) {
+  CELL_IJK(d);
+  (void)r;
+  p[q] = rho0 * ((w[q] - w[q - d.sz]) * dti * dzfi[k] + (v[q] - v[q - d.sy]) * dtidyi + (u[q] - u[q - 1]) * dtidxi);
+}
+
+// correc, src/correc.f90:34-66 (constant coefficients: rhop = rho0); the
+// three loops cover different halo-inclusive ranges and are fused here.
+__global__ void k_correc(fz_dims d, const double *__restrict__ dzci, double factori, double factorj, double dt,
+                         double rhop, const double *__restrict__ p, double *__restrict__ u, double *__restrict__ v,
+                         double *__restrict__ w) {
+  HALO_IJK(d);
+  if (i <= d.n1) u[q] = u[q] - factori * (p[q + 1] - p[q]) / rhop;
+  if (j <= d.n2) v[q] = v[q] - factorj * (p[q + d.sy] - p[q]) / rhop;
+  if (k <= d.n3) w[q] = w[q] - dt * dzci[k] * (p[q + d.sz] - p[q]) / rhop;
+}
+
+// updatep, src/updatep.f90:21-23, whole array
+__global__ void k_updatep(long nh, const double *__restrict__ pp, double *__restrict__ p) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
+    p[q] = p[q] + pp[q];
+}
+
+// updt_rhs_b, src/bound.f90:357-395 (one launch per face)
+__global__ void k_add_face(fz_dims d, int dir, int plane, const double *__restrict__ rhs, double *__restrict__ p) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;  // first in-plane index (0-based interior)
+  const int b = blockIdx.y;
+  int i, j, k, na;
+  if (dir == 0) { na = d.n2; i = plane; j = a + 1; k = b + 1; }
+  else if (dir == 1) { na = d.n1; i = a + 1; j = plane; k = b + 1; }
+  else { na = d.n1; i = a + 1; j = b + 1; k = plane; }
+  if (a >= na) return;
+  p[i + d.sy * j + d.sz * k] += rhs[a + (long)na * b];
+}
+
+// ---------------------------------------------------------------------------
+// acdi_transport_pf, src/acdi.f90:102-224 (SDF-normal sharpening)
+__global__ void __launch_bounds__(256) k_acdi_transport(fz_dims d, const double *__restrict__ dzci,
+                                                        const double *__restrict__ dzfi, double dxi, double dyi,
+                                                        const double *__restrict__ d_gam, double gam_v, double seps,
+                                                        const double *__restrict__ u, const double *__restrict__ v,
+                                                        const double *__restrict__ w, const double *__restrict__ normx,
+                                                        const double *__restrict__ normy,
+                                                        const double *__restrict__ normz,
+                                                        const double *__restrict__ phi, const double *__restrict__ psi,
+                                                        double *__restrict__ dpsidt, double *__restrict__ flux_x,
+                                                        double *__restrict__ flux_y, double *__restrict__ flux_z) {
+  CELL_IJK(d);
+  const long sy = d.sy, sz = d.sz;
+  const double gam = d_gam ? *d_gam : gam_v;
+  const double dzfi_c = dzfi[k], dzci_c = dzci[k], dzci_m = dzci[k - 1];
+  const double psi_ccm = psi[q - sz], psi_cmc = psi[q - sy], psi_mcc = psi[q - 1], psi_ccc = psi[q],
+               psi_pcc = psi[q + 1], psi_cpc = psi[q + sy], psi_ccp = psi[q + sz];
+  const double phi_ccm = phi[q - sz], phi_cmc = phi[q - sy], phi_mcc = phi[q - 1], phi_ccc = phi[q],
+               phi_pcc = phi[q + 1], phi_cpc = phi[q + sy], phi_ccp = phi[q + sz];
+  const double u_mcc = u[q - 1], u_ccc = u[q], v_cmc = v[q - sy], v_ccc = v[q], w_ccm = w[q - sz], w_ccc = w[q];
+  const double nxc = normx[q], nyc = normy[q], nzc = normz[q];
+  const double nx_xm = 0.5 * (normx[q - 1] + nxc), nx_xp = 0.5 * (normx[q + 1] + nxc);
+  const double nx_ym = 0.5 * (normx[q - sy] + nxc), nx_yp = 0.5 * (normx[q + sy] + nxc);
+  const double nx_zm = 0.5 * (normx[q - sz] + nxc), nx_zp = 0.5 * (normx[q + sz] + nxc);
+  const double ny_xm = 0.5 * (normy[q - 1] + nyc), ny_xp = 0.5 * (normy[q + 1] + nyc);
+  const double ny_ym = 0.5 * (normy[q - sy] + nyc), ny_yp = 0.5 * (normy[q + sy] + nyc);
+  const double ny_zm = 0.5 * (normy[q - sz] + nyc), ny_zp = 0.5 * (normy[q + sz] + nyc);
+  const double nz_xm = 0.5 * (normz[q - 1] + nzc), nz_xp = 0.5 * (normz[q + 1] + nzc);
+  const double nz_ym = 0.5 * (normz[q - sy] + nzc), nz_yp = 0.5 * (normz[q + sy] + nzc);
+  const double nz_zm = 0.5 * (normz[q - sz] + nzc), nz_zp = 0.5 * (normz[q + sz] + nzc);
+  const double upsiim = 0.5 * (psi_mcc + psi_ccc) * u_mcc, upsiip = 0.5 * (psi_pcc + psi_ccc) * u_ccc;
+  const double vpsijm = 0.5 * (psi_cmc + psi_ccc) * v_cmc, vpsijp = 0.5 * (psi_cpc + psi_ccc) * v_ccc;
+  const double wpsikm = 0.5 * (psi_ccm + psi_ccc) * w_ccm, wpsikp = 0.5 * (psi_ccp + psi_ccc) * w_ccc;
+  const double adv = dxi * (upsiip - upsiim) + dyi * (vpsijp - vpsijm) + dzfi_c * (wpsikp - wpsikm);
+  const double gs = gam * seps;
+  const double dpsidxm = gs * (psi_ccc - psi_mcc) * dxi, dpsidxp = gs * (psi_pcc - psi_ccc) * dxi;
+  const double dpsidym = gs * (psi_ccc - psi_cmc) * dyi, dpsidyp = gs * (psi_cpc - psi_ccc) * dyi;
+  const double dpsidzm = gs * (psi_ccc - psi_ccm) * dzci_m, dpsidzp = gs * (psi_ccp - psi_ccc) * dzci_c;
+  const double rn_01 = nx_xm / (sqrt(nx_xm * nx_xm + ny_xm * ny_xm + nz_xm * nz_xm) + FZ_EPS);
+  const double rn_11 = nx_xp / (sqrt(nx_xp * nx_xp + ny_xp * ny_xp + nz_xp * nz_xp) + FZ_EPS);
+  const double rn_02 = ny_ym / (sqrt(nx_ym * nx_ym + ny_ym * ny_ym + nz_ym * nz_ym) + FZ_EPS);
+  const double rn_12 = ny_yp / (sqrt(nx_yp * nx_yp + ny_yp * ny_yp + nz_yp * nz_yp) + FZ_EPS);
+  const double rn_03 = nz_zm / (sqrt(nx_zm * nx_zm + ny_zm * ny_zm + nz_zm * nz_zm) + FZ_EPS);
+  const double rn_13 = nz_zp / (sqrt(nx_zp * nx_zp + ny_zp * ny_zp + nz_zp * nz_zp) + FZ_EPS);
+  const double g4 = 0.25 * gam;
+  double t;
+  t = tanh(0.25 * (phi_ccc + phi_mcc) / seps); const double sharpxm = g4 * ((1.0 - t * t) * rn_01);
+  t = tanh(0.25 * (phi_pcc + phi_ccc) / seps); const double sharpxp = g4 * ((1.0 - t * t) * rn_11);
+  t = tanh(0.25 * (phi_ccc + phi_cmc) / seps); const double sharpym = g4 * ((1.0 - t * t) * rn_02);
+  t = tanh(0.25 * (phi_cpc + phi_ccc) / seps); const double sharpyp = g4 * ((1.0 - t * t) * rn_12);
+  t = tanh(0.25 * (phi_ccc + phi_ccm) / seps); const double sharpzm = g4 * ((1.0 - t * t) * rn_03);
+  t = tanh(0.25 * (phi_ccp + phi_ccc) / seps); const double sharpzp = g4 * ((1.0 - t * t) * rn_13);
+  const double rglrxm = dpsidxm - sharpxm, rglrxp = dpsidxp - sharpxp;
+  const double rglrym = dpsidym - sharpym, rglryp = dpsidyp - sharpyp;
+  const double rglrzm = dpsidzm - sharpzm, rglrzp = dpsidzp - sharpzp;
+  const double rglr = (rglrxp - rglrxm) * dxi + (rglryp - rglrym) * dyi + (rglrzp - rglrzm) * dzfi_c;
+  dpsidt[r] = -adv + rglr;
+  flux_x[q] = upsiip - rglrxp;
+  flux_y[q] = vpsijp - rglryp;
+  flux_z[q] = wpsikp - rglrzp;
+}
+
+// acdi_cmpt_phi, src/acdi.f90:239-247 (halo-inclusive)
+__global__ void k_acdi_phi(long nh, double seps, const double *__restrict__ psi, double *__restrict__ phi) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
+    phi[q] = seps * log((psi[q] + FZ_EPS) / (1.0 - psi[q] + FZ_EPS));
+}
+
+// vof_thinc_cmpt_phi, src/vof_thinc_qq.f90:148-156
+__global__ void k_vof_phi(long nh, double beta, const double *__restrict__ psi, double *__restrict__ phi) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x) {
+    const double pa = fmin(fmax(psi[q], FZ_EPS), 1.0 - FZ_EPS);
+    phi[q] = 0.5 / beta * log(pa / (1.0 - pa));
+  }
+}
+
+// vof_thinc_transport_psi fluxes, src/vof_thinc_qq.f90:66-121 (faces 0..n)
+__global__ void k_vof_flux(fz_dims d, double beta, const double *__restrict__ u, const double *__restrict__ v,
+                           const double *__restrict__ w, const double *__restrict__ normx,
+                           const double *__restrict__ normy, const double *__restrict__ normz,
+                           const double *__restrict__ phi, double *__restrict__ flux_x, double *__restrict__ flux_y,
+                           double *__restrict__ flux_z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > d.n1 || j > d.n2) return;
+  const long q = i + d.sy * j + d.sz * k;
+  const double vel_x = u[q], vel_y = v[q], vel_z = w[q];
+  long qx, qy, qz;
+  double xx, yy, zz;
+  if (vel_x > 0.0) { qx = q; xx = 0.5; } else { qx = q + 1; xx = -0.5; }
+  if (vel_y > 0.0) { qy = q; yy = 0.5; } else { qy = q + d.sy; yy = -0.5; }
+  if (vel_z > 0.0) { qz = q; zz = 0.5; } else { qz = q + d.sz; zz = -0.5; }
+  const double lfx = 1.0 / (1.0 + exp(-2.0 * beta * (normx[qx] * xx + phi[qx])));
+  const double lfy = 1.0 / (1.0 + exp(-2.0 * beta * (normy[qy] * yy + phi[qy])));
+  const double lfz = 1.0 / (1.0 + exp(-2.0 * beta * (normz[qz] * zz + phi[qz])));
+  flux_x[q] = vel_x * lfx;
+  flux_y[q] = vel_y * lfy;
+  flux_z[q] = vel_z * lfz;
+}
+
+// flux divergence, src/vof_thinc_qq.f90:123-132
+__global__ void k_vof_div(fz_dims d, const double *__restrict__ dzfi, double dxi, double dyi,
+                          const double *__restrict__ flux_x, const double *__restrict__ flux_y,
+                          const double *__restrict__ flux_z, double *__restrict__ dpsidt) {
+  CELL_IJK(d);
+  dpsidt[r] = -(dxi * (flux_x[q] - flux_x[q - 1]) + dyi * (flux_y[q] - flux_y[q - d.sy]) +
+                dzfi[k] * (flux_z[q] - flux_z[q - d.sz]));
+}
+
+// clip_field, src/two_fluid.f90:54-57
+__global__ void k_clip(long nh, double rmin, double rmax, double *__restrict__ p) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
+    p[q] = fmin(fmax(rmin, p[q]), rmax);
+}
+
+// cmpt_norm_curv_youngs, src/two_fluid.f90:81-174
+__global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restrict__ dzci,
+                                                const double *__restrict__ dzfi, double dxi, double dyi,
+                                                const double *__restrict__ psi, double *__restrict__ normx,
+                                                double *__restrict__ normy, double *__restrict__ normz,
+                                                double *__restrict__ kappa) {
+  CELL_IJK(d);
+  (void)r;
+  const long sy = d.sy, sz = d.sz;
+#define P_(a, b, c) psi[q + (a) + (b)*sy + (c)*sz]
+  const double mmm = P_(-1, -1, -1), mcm = P_(-1, 0, -1), mpm = P_(-1, 1, -1), cmm = P_(0, -1, -1), ccm = P_(0, 0, -1),
+               cpm = P_(0, 1, -1), pmm = P_(1, -1, -1), pcm = P_(1, 0, -1), ppm = P_(1, 1, -1);
+  const double mmc = P_(-1, -1, 0), mcc = P_(-1, 0, 0), mpc = P_(-1, 1, 0), cmc = P_(0, -1, 0), ccc = P_(0, 0, 0),
+               cpc = P_(0, 1, 0), pmc = P_(1, -1, 0), pcc = P_(1, 0, 0), ppc = P_(1, 1, 0);
+  const double mmp = P_(-1, -1, 1), mcp = P_(-1, 0, 1), mpp = P_(-1, 1, 1), cmp = P_(0, -1, 1), ccp = P_(0, 0, 1),
+               cpp = P_(0, 1, 1), pmp = P_(1, -1, 1), pcp = P_(1, 0, 1), ppp = P_(1, 1, 1);
+#undef P_
+  const double zc0 = dzci[k], zcm = dzci[k - 1];
+  double mx[8], my[8], mz[8];
+  mx[0] = 0.25 * ((pcc + ppc + pcp + ppp) - (ccc + cpc + ccp + cpp)) * dxi;
+  mx[1] = 0.25 * ((pcc + pmc + pcp + pmp) - (ccc + cmc + ccp + cmp)) * dxi;
+  mx[2] = 0.25 * ((pcc + ppc + pcm + ppm) - (ccc + cpc + ccm + cpm)) * dxi;
+  mx[3] = 0.25 * ((pcc + pmc + pcm + pmm) - (ccc + cmc + ccm + cmm)) * dxi;
+  mx[4] = 0.25 * ((ccc + cpc + ccp + cpp) - (mcc + mpc + mcp + mpp)) * dxi;
+  mx[5] = 0.25 * ((ccc + cmc + ccp + cmp) - (mcc + mmc + mcp + mmp)) * dxi;
+  mx[6] = 0.25 * ((ccc + cpc + ccm + cpm) - (mcc + mpc + mcm + mpm)) * dxi;
+  mx[7] = 0.25 * ((ccc + cmc + ccm + cmm) - (mcc + mmc + mcm + mmm)) * dxi;
+  my[0] = 0.25 * ((cpc + ppc + cpp + ppp) - (ccc + pcc + ccp + pcp)) * dyi;
+  my[1] = 0.25 * ((ccc + pcc + ccp + pcp) - (cmc + pmc + cmp + pmp)) * dyi;
+  my[2] = 0.25 * ((cpc + ppc + cpm + ppm) - (ccc + pcc + ccm + pcm)) * dyi;
+  my[3] = 0.25 * ((ccc + pcc + ccm + pcm) - (cmc + pmc + cmm + pmm)) * dyi;
+  my[4] = 0.25 * ((cpc + mpc + cpp + mpp) - (ccc + mcc + ccp + mcp)) * dyi;
+  my[5] = 0.25 * ((ccc + mcc + ccp + mcp) - (cmc + mmc + cmp + mmp)) * dyi;
+  my[6] = 0.25 * ((cpc + mpc + cpm + mpm) - (ccc + mcc + ccm + mcm)) * dyi;
+  my[7] = 0.25 * ((ccc + mcc + ccm + mcm) - (cmc + mmc + cmm + mmm)) * dyi;
+  mz[0] = 0.25 * ((ccp + pcp + cpp + ppp) - (ccc + pcc + cpc + ppc)) * zc0;
+  mz[1] = 0.25 * ((ccp + pcp + cmp + pmp) - (ccc + pcc + cmc + pmc)) * zc0;
+  mz[2] = 0.25 * ((ccc + pcc + cpc + ppc) - (ccm + pcm + cpm + ppm)) * zcm;
+  mz[3] = 0.25 * ((ccc + pcc + cmc + pmc) - (ccm + pcm + cmm + pmm)) * zcm;
+  mz[4] = 0.25 * ((ccp + mcp + cpp + mpp) - (ccc + mcc + cpc + mpc)) * zc0;
+  mz[5] = 0.25 * ((ccp + mcp + cmp + mmp) - (ccc + mcc + cmc + mmc)) * zc0;
+  mz[6] = 0.25 * ((ccc + mcc + cpc + mpc) - (ccm + mcm + cpm + mpm)) * zcm;
+  mz[7] = 0.25 * ((ccc + mcc + cmc + mmc) - (ccm + mcm + cmm + mmm)) * zcm;
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    const double nrm = sqrt(mx[m] * mx[m] + my[m] * my[m] + mz[m] * mz[m]) + FZ_EPS;
+    mx[m] = mx[m] / nrm;
+    my[m] = my[m] / nrm;
+    mz[m] = mz[m] / nrm;
+  }
+  const double nx = .125 * (mx[0] + mx[1] + mx[2] + mx[3] + mx[4] + mx[5] + mx[6] + mx[7]);
+  const double ny = .125 * (my[0] + my[1] + my[2] + my[3] + my[4] + my[5] + my[6] + my[7]);
+  const double nz = .125 * (mz[0] + mz[1] + mz[2] + mz[3] + mz[4] + mz[5] + mz[6] + mz[7]);
+  const double nrm = sqrt(nx * nx + ny * ny + nz * nz) + FZ_EPS;
+  normx[q] = nx / nrm;
+  normy[q] = ny / nrm;
+  normz[q] = nz / nrm;
+  kappa[q] = -(0.25 * ((mx[0] + mx[1] + mx[2] + mx[3]) - (mx[4] + mx[5] + mx[6] + mx[7])) * dxi +
+               0.25 * ((my[0] + my[2] + my[4] + my[6]) - (my[1] + my[3] + my[5] + my[7])) * dyi +
+               0.25 * ((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7])) * dzfi[k]);
+}
+
+// ---------------------------------------------------------------------------
+// mom_xyz_ad, src/mom.f90:545-850 (non-split viscous diffusion)
+__global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restrict__ dzci,
+                                                const double *__restrict__ dzfi, double dxi, double dyi, double rho,
+                                                double drho, double mu, double dmu,
+                                                const double *__restrict__ psiflx_x,
+                                                const double *__restrict__ psiflx_y,
+                                                const double *__restrict__ psiflx_z, const double *__restrict__ u,
+                                                const double *__restrict__ v, const double *__restrict__ w,
+                                                const double *__restrict__ psi, double *__restrict__ dudt,
+                                                double *__restrict__ dvdt, double *__restrict__ dwdt) {
+  CELL_IJK(d);
+  const long sy = d.sy, sz = d.sz;
+#define S_(f, a, b, c) f[q + (a) + (b)*sy + (c)*sz]
+  const double u_ccm = S_(u, 0, 0, -1), u_cmc = S_(u, 0, -1, 0), u_mcc = S_(u, -1, 0, 0), u_ccc = S_(u, 0, 0, 0),
+               u_pcc = S_(u, 1, 0, 0), u_mpc = S_(u, -1, 1, 0), u_cpc = S_(u, 0, 1, 0), u_mcp = S_(u, -1, 0, 1),
+               u_ccp = S_(u, 0, 0, 1);
+  const double v_ccm = S_(v, 0, 0, -1), v_cmc = S_(v, 0, -1, 0), v_pmc = S_(v, 1, -1, 0), v_mcc = S_(v, -1, 0, 0),
+               v_ccc = S_(v, 0, 0, 0), v_pcc = S_(v, 1, 0, 0), v_cpc = S_(v, 0, 1, 0), v_cmp = S_(v, 0, -1, 1),
+               v_ccp = S_(v, 0, 0, 1);
+  const double w_ccm = S_(w, 0, 0, -1), w_pcm = S_(w, 1, 0, -1), w_cpm = S_(w, 0, 1, -1), w_cmc = S_(w, 0, -1, 0),
+               w_mcc = S_(w, -1, 0, 0), w_ccc = S_(w, 0, 0, 0), w_pcc = S_(w, 1, 0, 0), w_cpc = S_(w, 0, 1, 0),
+               w_ccp = S_(w, 0, 0, 1);
+  const double c_ccm = S_(psi, 0, 0, -1), c_pcm = S_(psi, 1, 0, -1), c_cpm = S_(psi, 0, 1, -1),
+               c_cmc = S_(psi, 0, -1, 0), c_pmc = S_(psi, 1, -1, 0), c_mcc = S_(psi, -1, 0, 0),
+               c_ccc = S_(psi, 0, 0, 0), c_pcc = S_(psi, 1, 0, 0), c_mpc = S_(psi, -1, 1, 0),
+               c_cpc = S_(psi, 0, 1, 0), c_cmp = S_(psi, 0, -1, 1), c_mcp = S_(psi, -1, 0, 1),
+               c_ccp = S_(psi, 0, 0, 1), c_cpp = S_(psi, 0, 1, 1), c_ppc = S_(psi, 1, 1, 0),
+               c_pcp = S_(psi, 1, 0, 1);
+  const double flx_x_mcc = rho * u_mcc + drho * S_(psiflx_x, -1, 0, 0);
+  const double flx_x_ccc = rho * u_ccc + drho * S_(psiflx_x, 0, 0, 0);
+  const double flx_x_pcc = rho * u_pcc + drho * S_(psiflx_x, 1, 0, 0);
+  const double flx_x_mpc = rho * u_mpc + drho * S_(psiflx_x, -1, 1, 0);
+  const double flx_x_cpc = rho * u_cpc + drho * S_(psiflx_x, 0, 1, 0);
+  const double flx_x_mcp = rho * u_mcp + drho * S_(psiflx_x, -1, 0, 1);
+  const double flx_x_ccp = rho * u_ccp + drho * S_(psiflx_x, 0, 0, 1);
+  const double flx_y_cmc = rho * v_cmc + drho * S_(psiflx_y, 0, -1, 0);
+  const double flx_y_ccc = rho * v_ccc + drho * S_(psiflx_y, 0, 0, 0);
+  const double flx_y_cpc = rho * v_cpc + drho * S_(psiflx_y, 0, 1, 0);
+  const double flx_y_pmc = rho * v_pmc + drho * S_(psiflx_y, 1, -1, 0);
+  const double flx_y_pcc = rho * v_pcc + drho * S_(psiflx_y, 1, 0, 0);
+  const double flx_y_cmp = rho * v_cmp + drho * S_(psiflx_y, 0, -1, 1);
+  const double flx_y_ccp = rho * v_ccp + drho * S_(psiflx_y, 0, 0, 1);
+  const double flx_z_ccm = rho * w_ccm + drho * S_(psiflx_z, 0, 0, -1);
+  const double flx_z_ccc = rho * w_ccc + drho * S_(psiflx_z, 0, 0, 0);
+  const double flx_z_ccp = rho * w_ccp + drho * S_(psiflx_z, 0, 0, 1);
+  const double flx_z_pcm = rho * w_pcm + drho * S_(psiflx_z, 1, 0, -1);
+  const double flx_z_pcc = rho * w_pcc + drho * S_(psiflx_z, 1, 0, 0);
+  const double flx_z_cpm = rho * w_cpm + drho * S_(psiflx_z, 0, 1, -1);
+  const double flx_z_cpc = rho * w_cpc + drho * S_(psiflx_z, 0, 1, 0);
+#undef S_
+  const double dzci_c = dzci[k], dzci_m = dzci[k - 1], dzfi_c = dzfi[k], dzfi_p = dzfi[k + 1];
+  const double uuip = 0.25 * (flx_x_pcc + flx_x_ccc) * (u_ccc + u_pcc);
+  const double uuim = 0.25 * (flx_x_mcc + flx_x_ccc) * (u_ccc + u_mcc);
+  const double vujp = 0.25 * (flx_y_pcc + flx_y_ccc) * (u_ccc + u_cpc);
+  const double vujm = 0.25 * (flx_y_pmc + flx_y_cmc) * (u_ccc + u_cmc);
+  const double wukp = 0.25 * (flx_z_pcc + flx_z_ccc) * (u_ccc + u_ccp);
+  const double wukm = 0.25 * (flx_z_pcm + flx_z_ccm) * (u_ccc + u_ccm);
+  double dudt_aux = dxi * (-uuip + uuim) + dyi * (-vujp + vujm) + dzfi_c * (-wukp + wukm);
+  const double uvip = 0.25 * (flx_x_ccc + flx_x_cpc) * (v_ccc + v_pcc);
+  const double uvim = 0.25 * (flx_x_mcc + flx_x_mpc) * (v_ccc + v_mcc);
+  const double vvjp = 0.25 * (flx_y_ccc + flx_y_cpc) * (v_ccc + v_cpc);
+  const double vvjm = 0.25 * (flx_y_ccc + flx_y_cmc) * (v_ccc + v_cmc);
+  const double wvkp = 0.25 * (flx_z_ccc + flx_z_cpc) * (v_ccc + v_ccp);
+  const double wvkm = 0.25 * (flx_z_ccm + flx_z_cpm) * (v_ccc + v_ccm);
+  double dvdt_aux = dxi * (-uvip + uvim) + dyi * (-vvjp + vvjm) + dzfi_c * (-wvkp + wvkm);
+  const double uwip = 0.25 * (flx_x_ccc + flx_x_ccp) * (w_ccc + w_pcc);
+  const double uwim = 0.25 * (flx_x_mcc + flx_x_mcp) * (w_ccc + w_mcc);
+  const double vwjp = 0.25 * (flx_y_ccc + flx_y_ccp) * (w_ccc + w_cpc);
+  const double vwjm = 0.25 * (flx_y_cmc + flx_y_cmp) * (w_ccc + w_cmc);
+  const double wwkp = 0.25 * (flx_z_ccc + flx_z_ccp) * (w_ccc + w_ccp);
+  const double wwkm = 0.25 * (flx_z_ccc + flx_z_ccm) * (w_ccc + w_ccm);
+  double dwdt_aux = dxi * (-uwip + uwim) + dyi * (-vwjp + vwjm) + dzci_c * (-wwkp + wwkm);
+  {
+    const double dudxp = (u_pcc - u_ccc) * dxi, dudxm = (u_ccc - u_mcc) * dxi;
+    const double dvdxp = (v_pcc - v_ccc) * dxi, dvdxm = (v_pmc - v_cmc) * dxi;
+    const double dudyp = (u_cpc - u_ccc) * dyi, dudym = (u_ccc - u_cmc) * dyi;
+    const double dudzp = (u_ccp - u_ccc) * dzci_c, dudzm = (u_ccc - u_ccm) * dzci_m;
+    const double dwdxp = (w_pcc - w_ccc) * dxi, dwdxm = (w_pcm - w_ccm) * dxi;
+    const double muxp = mu + dmu * c_pcc, muxm = mu + dmu * c_ccc;
+    const double muyp = mu + dmu * 0.25 * (c_ccc + c_cpc + c_ppc + c_pcc);
+    const double muym = mu + dmu * 0.25 * (c_ccc + c_cmc + c_pmc + c_pcc);
+    const double muzp = mu + dmu * 0.25 * (c_ccc + c_pcc + c_ccp + c_pcp);
+    const double muzm = mu + dmu * 0.25 * (c_ccc + c_pcc + c_ccm + c_pcm);
+    dudt_aux = dudt_aux + dxi * ((dudxp + dudxp) * muxp - (dudxm + dudxm) * muxm) +
+               dyi * ((dudyp + dvdxp) * muyp - (dudym + dvdxm) * muym) +
+               dzfi_c * ((dudzp + dwdxp) * muzp - (dudzm + dwdxm) * muzm);
+  }
+  {
+    const double dvdxp = (v_pcc - v_ccc) * dxi, dvdxm = (v_ccc - v_mcc) * dxi;
+    const double dudyp = (u_cpc - u_ccc) * dyi, dudym = (u_mpc - u_mcc) * dyi;
+    const double dvdyp = (v_cpc - v_ccc) * dyi, dvdym = (v_ccc - v_cmc) * dyi;
+    const double dvdzp = (v_ccp - v_ccc) * dzci_c, dvdzm = (v_ccc - v_ccm) * dzci_m;
+    const double dwdyp = (w_cpc - w_ccc) * dyi, dwdym = (w_cpm - w_ccm) * dyi;
+    const double muxp = mu + dmu * 0.25 * (c_ccc + c_pcc + c_ppc + c_cpc);
+    const double muxm = mu + dmu * 0.25 * (c_ccc + c_mcc + c_mpc + c_cpc);
+    const double muyp = mu + dmu * c_cpc, muym = mu + dmu * c_ccc;
+    const double muzp = mu + dmu * 0.25 * (c_ccc + c_cpc + c_cpp + c_ccp);
+    const double muzm = mu + dmu * 0.25 * (c_ccc + c_cpc + c_cpm + c_ccm);
+    dvdt_aux = dvdt_aux + dxi * ((dvdxp + dudyp) * muxp - (dvdxm + dudym) * muxm) +
+               dyi * ((dvdyp + dvdyp) * muyp - (dvdym + dvdym) * muym) +
+               dzfi_c * ((dvdzp + dwdyp) * muzp - (dvdzm + dwdym) * muzm);
+  }
+  {
+    const double dwdxp = (w_pcc - w_ccc) * dxi, dwdxm = (w_ccc - w_mcc) * dxi;
+    const double dudzp = (u_ccp - u_ccc) * dzci_c, dudzm = (u_mcp - u_mcc) * dzci_c;
+    const double dwdyp = (w_cpc - w_ccc) * dyi, dwdym = (w_ccc - w_cmc) * dyi;
+    const double dvdzp = (v_ccp - v_ccc) * dzci_c, dvdzm = (v_cmp - v_cmc) * dzci_c;
+    const double dwdzp = (w_ccp - w_ccc) * dzfi_p, dwdzm = (w_ccc - w_ccm) * dzfi_c;
+    const double muxp = mu + dmu * 0.25 * (c_ccc + c_pcc + c_ccp + c_pcp);
+    const double muxm = mu + dmu * 0.25 * (c_ccc + c_mcc + c_ccp + c_mcp);
+    const double muyp = mu + dmu * 0.25 * (c_ccc + c_cpc + c_ccp + c_cpp);
+    const double muym = mu + dmu * 0.25 * (c_ccc + c_cmc + c_ccp + c_cmp);
+    const double muzp = mu + dmu * c_ccp, muzm = mu + dmu * c_ccc;
+    dwdt_aux = dwdt_aux + dxi * ((dwdxp + dudzp) * muxp - (dwdxm + dudzm) * muxm) +
+               dyi * ((dwdyp + dvdzp) * muyp - (dwdym + dvdzm) * muym) +
+               dzci_c * ((dwdzp + dwdzp) * muzp - (dwdzm + dwdzm) * muzm);
+  }
+  dudt[r] = dudt_aux;
+  dvdt[r] = dvdt_aux;
+  dwdt[r] = dwdt_aux;
+}
+
+// mom_xyz_oth, src/mom.f90:912-1038 (constant-coefficient split, Brackbill)
+__global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__restrict__ dzci, double dxi, double dyi,
+                                                 double dt_r, double rho, double drho, double surf_factor,
+                                                 double rho0, double bfx, double bfy, double bfz, double gx, double gy,
+                                                 double gz, double rax, double ray, double raz,
+                                                 const double *__restrict__ p, const double *__restrict__ psi,
+                                                 const double *__restrict__ kappa, const double *__restrict__ pn,
+                                                 const double *__restrict__ po, double *__restrict__ dudt,
+                                                 double *__restrict__ dvdt, double *__restrict__ dwdt) {
+  CELL_IJK(d);
+  const long sy = d.sy, sz = d.sz;
+  const double dzci_c = dzci[k];
+  const double p_ccc = p[q], p_pcc = p[q + 1], p_cpc = p[q + sy], p_ccp = p[q + sz];
+  const double q_ccc = (1.0 + dt_r) * pn[q] - dt_r * po[q];
+  const double q_pcc = (1.0 + dt_r) * pn[q + 1] - dt_r * po[q + 1];
+  const double q_cpc = (1.0 + dt_r) * pn[q + sy] - dt_r * po[q + sy];
+  const double q_ccp = (1.0 + dt_r) * pn[q + sz] - dt_r * po[q + sz];
+  const double k_ccc = kappa[q], k_pcc = kappa[q + 1], k_cpc = kappa[q + sy], k_ccp = kappa[q + sz];
+  const double c_ccc = psi[q], c_pcc = psi[q + 1], c_cpc = psi[q + sy], c_ccp = psi[q + sz];
+  const double psixp = 0.5 * (c_pcc + c_ccc), psiyp = 0.5 * (c_cpc + c_ccc), psizp = 0.5 * (c_ccp + c_ccc);
+  const double rhoxp = rho + drho * psixp, rhoyp = rho + drho * psiyp, rhozp = rho + drho * psizp;
+  const double dpdx = (p_pcc - p_ccc) * dxi, dpdy = (p_cpc - p_ccc) * dyi, dpdz = (p_ccp - p_ccc) * dzci_c;
+  double dudt_aux = bfx + (rhoxp - rax) * gx;
+  double dvdt_aux = bfy + (rhoyp - ray) * gy;
+  double dwdt_aux = bfz + (rhozp - raz) * gz;
+  double skx = 0.5 * (k_pcc + k_ccc) * surf_factor, sky = 0.5 * (k_cpc + k_ccc) * surf_factor,
+         skz = 0.5 * (k_ccp + k_ccc) * surf_factor;
+  skx = skx * rhoxp;
+  sky = sky * rhoyp;
+  skz = skz * rhozp;
+  const double surfx = skx * (c_pcc - c_ccc) * dxi, surfy = sky * (c_cpc - c_ccc) * dyi,
+               surfz = skz * (c_ccp - c_ccc) * dzci_c;
+  dudt_aux = dudt_aux + surfx;
+  dvdt_aux = dvdt_aux + surfy;
+  dwdt_aux = dwdt_aux + surfz;
+  const double dpdx_e = (q_pcc - q_ccc) * dxi, dpdy_e = (q_cpc - q_ccc) * dyi, dpdz_e = (q_ccp - q_ccc) * dzci_c;
+  dudt_aux = dudt_aux - (dpdx_e + rhoxp / rho0 * (dpdx - dpdx_e));
+  dvdt_aux = dvdt_aux - (dpdy_e + rhoyp / rho0 * (dpdy - dpdy_e));
+  dwdt_aux = dwdt_aux - (dpdz_e + rhozp / rho0 * (dpdz - dpdz_e));
+  dudt[r] = dudt_aux;
+  dvdt[r] = dvdt_aux;
+  dwdt[r] = dwdt_aux;
+}
+
+// AB2 momentum update with the density rescale, src/rk.f90:86-101
+__global__ void k_rk_mom_ab2(fz_dims d, double factor1, double factor2, double rho, double drho,
+                             const double *__restrict__ psio, const double *__restrict__ psi,
+                             const double *__restrict__ dudt, const double *__restrict__ dvdt,
+                             const double *__restrict__ dwdt, const double *__restrict__ dudto,
+                             const double *__restrict__ dvdto, const double *__restrict__ dwdto,
+                             double *__restrict__ u, double *__restrict__ v, double *__restrict__ w) {
+  CELL_IJK(d);
+  const long sy = d.sy, sz = d.sz;
+  const double rhox_n = rho + drho * 0.5 * (psio[q] + psio[q + 1]);
+  const double rhoy_n = rho + drho * 0.5 * (psio[q] + psio[q + sy]);
+  const double rhoz_n = rho + drho * 0.5 * (psio[q] + psio[q + sz]);
+  const double rhox_p = rho + drho * 0.5 * (psi[q] + psi[q + 1]);
+  const double rhoy_p = rho + drho * 0.5 * (psi[q] + psi[q + sy]);
+  const double rhoz_p = rho + drho * 0.5 * (psi[q] + psi[q + sz]);
+  u[q] = u[q] * rhox_n / rhox_p + (factor1 * dudt[r] + factor2 * dudto[r]) / rhox_p;
+  v[q] = v[q] * rhoy_n / rhoy_p + (factor1 * dvdt[r] + factor2 * dvdto[r]) / rhoy_p;
+  w[q] = w[q] * rhoz_n / rhoz_p + (factor1 * dwdt[r] + factor2 * dwdto[r]) / rhoz_p;
+}
+
+// src/rk.f90:118-130
+__global__ void k_rk_mom_oth(fz_dims d, double factor12, double rho, double drho, const double *__restrict__ psi,
+                             const double *__restrict__ dudt, const double *__restrict__ dvdt,
+                             const double *__restrict__ dwdt, double *__restrict__ u, double *__restrict__ v,
+                             double *__restrict__ w) {
+  CELL_IJK(d);
+  const long sy = d.sy, sz = d.sz;
+  const double rhox_p = rho + drho * 0.5 * (psi[q] + psi[q + 1]);
+  const double rhoy_p = rho + drho * 0.5 * (psi[q] + psi[q + sy]);
+  const double rhoz_p = rho + drho * 0.5 * (psi[q] + psi[q + sz]);
+  u[q] = u[q] + factor12 * dudt[r] / rhox_p;
+  v[q] = v[q] + factor12 * dvdt[r] / rhoy_p;
+  w[q] = w[q] + factor12 * dwdt[r] / rhoz_p;
+}
+
+// src/rk.f90:240-247
+__global__ void k_rk_psi_ab2(fz_dims d, double factor1, double factor2, const double *__restrict__ dpsidt,
+                             const double *__restrict__ dpsidto, double *__restrict__ psi) {
+  CELL_IJK(d);
+  psi[q] = psi[q] + factor1 * dpsidt[r] + factor2 * dpsidto[r];
+}
+
+// lscale_forcing, src/forcing.f90:27-63
+__global__ void k_forcing(fz_dims d, int ftype, double f0, double factor, double l1, double l2, double l3, double dl1,
+                          double dl2, const double *__restrict__ zc, double *__restrict__ u, double *__restrict__ v,
+                          double *__restrict__ w) {
+  CELL_IJK(d);
+  (void)r;
+  const double k0 = 2.0;
+  const double xxc = (double)((float)i - 0.5f) * dl1 / l1;
+  const double yyc = (double)((float)j - 0.5f) * dl2 / l2;
+  const double zzc = zc[k] / l3;
+  if (ftype == 1) {  // ABC, a=b=c=1
+    u[q] = u[q] + factor * (f0 * (1.0 * sin(k0 * zzc) + 1.0 * cos(k0 * yyc)));
+    v[q] = v[q] + factor * (f0 * (1.0 * sin(k0 * xxc) + 1.0 * cos(k0 * zzc)));
+    w[q] = w[q] + factor * (f0 * (1.0 * sin(k0 * yyc) + 1.0 * cos(k0 * xxc)));
+  } else {
+    const double xxf = xxc + 0.5 * dl1 / l1;
+    const double yyf = yyc + 0.5 * dl2 / l2;
+    u[q] = u[q] + factor * (f0 * sin(k0 * xxf) * cos(k0 * yyc) * cos(k0 * zzc));
+    v[q] = v[q] - factor * (f0 * cos(k0 * xxc) * sin(k0 * yyf) * cos(k0 * zzc));
+  }
+}
+
+// init2fl sphere/cylinder/film branch, src/two_fluid.f90:392-436
+__global__ void k_init_spheres(fz_dims d, const double *__restrict__ zc, double dl1, double dl2, double l1, double l2,
+                               double l3, int ip1, int ip2, int ip3, int id1, int id2, int id3, double sdist0,
+                               double xxc, double yyc, double zzc, double rad, double seps, double *__restrict__ psi) {
+  CELL_IJK(d);
+  (void)r;
+  const double z = zc[k];
+  const double y = (double)((float)j - 0.5f) * dl2;
+  const double x = (double)((float)i - 0.5f) * dl1;
+  double sdistmin = sdist0;
+  for (int kk = -1; kk <= 1; kk++)
+    for (int jj = -1; jj <= 1; jj++)
+      for (int ii = -1; ii <= 1; ii++) {
+        const double ax = x + (double)(ii * ip1) * l1 - xxc;
+        const double ay = y + (double)(jj * ip2) * l2 - yyc;
+        const double az = z + (double)(kk * ip3) * l3 - zzc;
+        const double sd = sqrt(ax * ax * id1 + ay * ay * id2 + az * az * id3) - rad;
+        if (fabs(sd) <= sdistmin) sdistmin = sd;
+      }
+  const double ps = .5 * (1.0 + tanh(sdistmin / (2.0 * seps)));
+  psi[q] = fmin(psi[q], ps);
+}
+
+__global__ void k_init_film(fz_dims d, const double *__restrict__ zc, double zmid, double seps,
+                            double *__restrict__ psi) {
+  CELL_IJK(d);
+  (void)r;
+  psi[q] = .5 * (1.0 + tanh((zc[k] - zmid) / (2.0 * seps)));
+}
+
+__global__ void k_fill(long nh, double val, double *__restrict__ p) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x) p[q] = val;
+}
+
+__global__ void k_one_minus(long nh, double *__restrict__ p) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
+    p[q] = 1.0 - p[q];
+}
+
+// ---------------------------------------------------------------------------
+// host wrappers
+// ---------------------------------------------------------------------------
+static inline int flat_grid(long n) {
+  long b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  return (int)(b < 1 ? 1 : b);
+}
+
+extern "C" int fz_fillps(fz_ctx *c, double dti, double rho0, const double *u, const double *v, const double *w,
+                         double *p) {
+  if (!c || !u || !v || !w || !p) return fz_set_error(FZ_ERR_ARG, "fz_fillps: null argument");
+  hipLaunchKernelGGL(k_fillps, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzfi, dti, dti * c->dli[0],
+                     dti * c->dli[1], rho0, u, v, w, p);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_correc(fz_ctx *c, double rho0, const double rho12[2], double dt, const double *p,
+                         const double *psi, double *u, double *v, double *w) {
+  (void)rho12;
+  (void)psi;
+  if (!c || !p || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_correc: null argument");
+  hipLaunchKernelGGL(k_correc, fz_grid_halo(c->d), fz_block3(), 0, c->st, c->d, c->dzci, dt * c->dli[0],
+                     dt * c->dli[1], dt, rho0, p, u, v, w);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_updatep(fz_ctx *c, const double *pp, double *p) {
+  if (!c || !pp || !p) return fz_set_error(FZ_ERR_ARG, "fz_updatep: null argument");
+  hipLaunchKernelGGL(k_updatep, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, pp, p);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_updt_rhs_b(fz_ctx *c, double *p) {
+  if (!c || !p) return fz_set_error(FZ_ERR_ARG, "fz_updt_rhs_b: null argument");
+  if (!c->pois) return fz_set_error(FZ_ERR_STATE, "fz_updt_rhs_b: fz_initsolver has not been called");
+  const fz_dims &d = c->d;
+  // x is a boundary direction of the x-pencil; y and z when not periodic
+  const double *rb[3] = {c->pois->rhsbx, c->pois->rhsby, c->pois->rhsbz};
+  const int na[3] = {d.n2, d.n1, d.n1}, nb[3] = {d.n3, d.n3, d.n2}, nn[3] = {d.n1, d.n2, d.n3};
+  for (int dir = 0; dir < 3; dir++) {
+    if (dir > 0 && c->per[dir]) continue;
+    for (int ib = 0; ib < 2; ib++) {
+      const double *rhs = rb[dir] + (long)na[dir] * nb[dir] * ib;
+      hipLaunchKernelGGL(k_add_face, dim3((na[dir] + 63) / 64, nb[dir]), dim3(64), 0, c->st, d, dir,
+                         ib == 0 ? 1 : nn[dir], rhs, p);
+      FZ_LAUNCHED();
+    }
+  }
+  return FZ_OK;
+}
+
+int fz_acdi_transport_dev(fz_ctx *c, const double *d_gam, double gam, double seps, const double *u, const double *v,
+                          const double *w, const double *normx, const double *normy, const double *normz,
+                          const double *phi, const double *psi, double *dpsidt, double *flux_x, double *flux_y,
+                          double *flux_z) {
+  hipLaunchKernelGGL(k_acdi_transport, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzci, c->dzfi,
+                     c->dli[0], c->dli[1], d_gam, gam, seps, u, v, w, normx, normy, normz, phi, psi, dpsidt, flux_x,
+                     flux_y, flux_z);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_acdi_transport_pf(fz_ctx *c, double gam, double seps, const double *u, const double *v,
+                                    const double *w, const double *normx, const double *normy, const double *normz,
+                                    const double *phi, const double *psi, double *dpsidt, double *flux_x,
+                                    double *flux_y, double *flux_z) {
+  if (!c || !u || !v || !w || !normx || !normy || !normz || !phi || !psi || !dpsidt || !flux_x || !flux_y || !flux_z)
+    return fz_set_error(FZ_ERR_ARG, "fz_acdi_transport_pf: null argument");
+  return fz_acdi_transport_dev(c, nullptr, gam, seps, u, v, w, normx, normy, normz, phi, psi, dpsidt, flux_x, flux_y,
+                               flux_z);
+}
+
+extern "C" int fz_acdi_cmpt_phi(fz_ctx *c, double seps, const double *psi, double *phi) {
+  if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_acdi_cmpt_phi: null argument");
+  hipLaunchKernelGGL(k_acdi_phi, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, seps, psi, phi);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_vof_thinc_cmpt_phi(fz_ctx *c, double beta, const double *psi, double *phi) {
+  if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_vof_thinc_cmpt_phi: null argument");
+  hipLaunchKernelGGL(k_vof_phi, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, beta, psi, phi);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_vof_thinc_transport_psi(fz_ctx *c, double beta, const double *u, const double *v, const double *w,
+                                          const double *normx, const double *normy, const double *normz,
+                                          const double *phi, const double *psi, double *dpsidt, double *flux_x,
+                                          double *flux_y, double *flux_z) {
+  (void)psi;
+  if (!c || !u || !v || !w || !normx || !normy || !normz || !phi || !dpsidt || !flux_x || !flux_y || !flux_z)
+    return fz_set_error(FZ_ERR_ARG, "fz_vof_thinc_transport_psi: null argument");
+  const fz_dims &d = c->d;
+  dim3 g((d.n1 + 1 + 63) / 64, (d.n2 + 1 + 3) / 4, d.n3 + 1);
+  hipLaunchKernelGGL(k_vof_flux, g, fz_block3(), 0, c->st, d, beta, u, v, w, normx, normy, normz, phi, flux_x, flux_y,
+                     flux_z);
+  FZ_LAUNCHED();
+  hipLaunchKernelGGL(k_vof_div, fz_grid_interior(d), fz_block3(), 0, c->st, d, c->dzfi, c->dli[0], c->dli[1], flux_x,
+                     flux_y, flux_z, dpsidt);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_clip_field(fz_ctx *c, const double minmax[2], double *p) {
+  if (!c || !minmax || !p) return fz_set_error(FZ_ERR_ARG, "fz_clip_field: null argument");
+  hipLaunchKernelGGL(k_clip, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, minmax[0], minmax[1], p);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_cmpt_norm_curv_youngs(fz_ctx *c, const double *psi, double *normx, double *normy, double *normz,
+                                        double *kappa) {
+  if (!c || !psi || !normx || !normy || !normz || !kappa)
+    return fz_set_error(FZ_ERR_ARG, "fz_cmpt_norm_curv_youngs: null argument");
+  hipLaunchKernelGGL(k_youngs, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzci, c->dzfi, c->dli[0],
+                     c->dli[1], psi, normx, normy, normz, kappa);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_mom_xyz_ad(fz_ctx *c, const double rho12[2], const double mu12[2], const double *psiflx_x,
+                             const double *psiflx_y, const double *psiflx_z, const double *u, const double *v,
+                             const double *w, const double *psi, double *dudt, double *dvdt, double *dwdt) {
+  if (!c || !rho12 || !mu12 || !psiflx_x || !psiflx_y || !psiflx_z || !u || !v || !w || !psi || !dudt || !dvdt ||
+      !dwdt)
+    return fz_set_error(FZ_ERR_ARG, "fz_mom_xyz_ad: null argument");
+  hipLaunchKernelGGL(k_mom_ad, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzci, c->dzfi, c->dli[0],
+                     c->dli[1], rho12[1], rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], psiflx_x, psiflx_y,
+                     psiflx_z, u, v, w, psi, dudt, dvdt, dwdt);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_mom_xyz_oth(fz_ctx *c, double dt_r, const double rho12[2], const double beta12[2],
+                              const double bforce[3], const double gacc[3], double sigma, double rho0,
+                              const double rho_av[3], const double *p, const double *psi, const double *kappa,
+                              const double *pn, const double *po, double *dudt, double *dvdt, double *dwdt) {
+  (void)beta12;
+  if (!c || !rho12 || !bforce || !gacc || !rho_av || !p || !psi || !kappa || !pn || !po || !dudt || !dvdt || !dwdt)
+    return fz_set_error(FZ_ERR_ARG, "fz_mom_xyz_oth: null argument");
+  const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
+  hipLaunchKernelGGL(k_mom_oth, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzci, c->dli[0], c->dli[1],
+                     dt_r, rho12[1], rho12[0] - rho12[1], surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0],
+                     gacc[1], gacc[2], rho_av[0], rho_av[1], rho_av[2], p, psi, kappa, pn, po, dudt, dvdt, dwdt);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_lscale_forcing(fz_ctx *c, int ftype, double alpha, double dt, double *u, double *v, double *w) {
+  if (!c || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_lscale_forcing: null argument");
+  const double f0 = alpha * ((2.0 * FZ_PI / c->l[2]) * (2.0 * FZ_PI / c->l[2]));
+  hipLaunchKernelGGL(k_forcing, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, ftype, f0, dt, c->l[0], c->l[1],
+                     c->l[2], c->dl[0], c->dl[1], c->zc, u, v, w);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+// rk_2fl / rk AB2 kernels for the case driver and fz_rk / fz_rk_2fl
+int fz_launch_rk_psi(fz_ctx *c, double f1, double f2, const double *dpsidt, const double *dpsidto, double *psi) {
+  hipLaunchKernelGGL(k_rk_psi_ab2, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, f1, f2, dpsidt, dpsidto, psi);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+int fz_launch_rk_mom(fz_ctx *c, double f1, double f2, const double rho12[2], const double *psio, const double *psi,
+                     const double *du, const double *dv, const double *dw, const double *duo, const double *dvo,
+                     const double *dwo, double *u, double *v, double *w) {
+  hipLaunchKernelGGL(k_rk_mom_ab2, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, f1, f2, rho12[1],
+                     rho12[0] - rho12[1], psio, psi, du, dv, dw, duo, dvo, dwo, u, v, w);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+int fz_launch_rk_oth(fz_ctx *c, double f12, const double rho12[2], const double *psi, const double *du,
+                     const double *dv, const double *dw, double *u, double *v, double *w) {
+  hipLaunchKernelGGL(k_rk_mom_oth, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, f12, rho12[1],
+                     rho12[0] - rho12[1], psi, du, dv, dw, u, v, w);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+int fz_launch_fill(fz_ctx *c, long n, double val, double *p) {
+  hipLaunchKernelGGL(k_fill, dim3(flat_grid(n)), dim3(256), 0, c->st, n, val, p);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+extern "C" int fz_init2fl(fz_ctx *c, const fz_params *prm, double seps, double *psi) {
+  if (!c || !prm || !psi) return fz_set_error(FZ_ERR_ARG, "fz_init2fl: null argument");
+  const fz_dims &d = c->d;
+  FZ_TRY(fz_launch_fill(c, c->nh, 1.0, psi));
+  int is_sphere = 0, is_swap = 0, is_dim[3] = {1, 1, 1};
+  switch (prm->inipsi) {
+    case 0: break;
+    case 1: FZ_TRY(fz_launch_fill(c, c->nh, 0.0, psi)); break;
+    case 2: is_sphere = 1; break;
+    case 3: is_sphere = 1; is_dim[1] = 0; break;
+    case 4: is_sphere = 1; is_dim[0] = 0; is_dim[1] = 0; break;
+    case 5: is_sphere = 1; is_swap = 1; break;
+    case 6: is_sphere = 1; is_swap = 1; is_dim[1] = 0; break;
+    case 7: is_sphere = 1; is_swap = 1; is_dim[0] = 0; is_dim[1] = 0; break;
+    case 8:
+      hipLaunchKernelGGL(k_init_film, fz_grid_interior(d), fz_block3(), 0, c->st, d, c->zc, c->l[2] / 2.0, seps, psi);
+      FZ_LAUNCHED();
+      break;
+    default: return fz_set_error(FZ_ERR_ARG, "fz_init2fl: unknown inipsi %d", prm->inipsi);
+  }
+  if (is_sphere) {
+    int nsph = prm->nspheres;
+    double def[4];
+    const double *sph = prm->spheres;
+    if (nsph < 0 || nsph > 64) return fz_set_error(FZ_ERR_ARG, "fz_init2fl: nspheres must be in 0..64");
+    if (nsph == 0) {  // no spheres.in: one in the centre, src/two_fluid.f90:393-401
+      double lmin = 1e300;
+      for (int q = 0; q < 3; q++)
+        if (is_dim[q] && c->l[q] < lmin) lmin = c->l[q];
+      def[0] = c->l[0] / 2;
+      def[1] = c->l[1] / 2;
+      def[2] = c->l[2] / 2;
+      def[3] = 0.25 * lmin;
+      sph = def;
+      nsph = 1;
+    }
+    int ip[3];
+    for (int q = 0; q < 3; q++) ip[q] = (prm->cbcpsi[2 * q] == 'P' && prm->cbcpsi[2 * q + 1] == 'P');
+    double lmax = 0.0;
+    for (int q = 0; q < 3; q++)
+      if (c->l[q] * is_dim[q] > lmax) lmax = c->l[q] * is_dim[q];
+    for (int s = 0; s < nsph; s++) {
+      hipLaunchKernelGGL(k_init_spheres, fz_grid_interior(d), fz_block3(), 0, c->st, d, c->zc, c->dl[0], c->dl[1],
+                         c->l[0], c->l[1], c->l[2], ip[0], ip[1], ip[2], is_dim[0], is_dim[1], is_dim[2], lmax * 2.0,
+                         sph[4 * s], sph[4 * s + 1], sph[4 * s + 2], sph[4 * s + 3], seps, psi);
+      FZ_LAUNCHED();
+    }
+  }
+  if (is_swap) {
+    hipLaunchKernelGGL(k_one_minus, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, psi);
+    FZ_LAUNCHED();
+  }
+  return FZ_OK;
+}

@@ -1,0 +1,206 @@
+/*
+ * fizzy.h -- C ABI of the MI355X-native CaNS-Fizzy hot path (libfizzy.so).
+ *
+ * Every entry point replaces one routine of the reference's Fortran module
+ * interface; the reference file:line it stands in for is given next to each
+ * declaration. INTEGRATION.md shows the iso_c_binding interfaces a Fortran
+ * caller binds (the modules under cans-fizzy_amd/fortran/ are those
+ * bindings, with the reference's module and routine names).
+ *
+ * Conventions
+ *  - Fields are device pointers (hipMalloc'd, e.g. through fz_malloc) to
+ *    Fortran-ordered halo'd arrays: element (i,j,k), 0 <= i <= n1+1, lives at
+ *    offset i + (n1+2)*(j + (n2+2)*k) (src/main.f90:162-204). AB2 right-hand
+ *    sides are interior-only arrays, (i,j,k) at (i-1) + n1*((j-1) + n2*(k-1))
+ *    (src/rk.f90:60-61).
+ *  - Boundary-condition codes are 'P','D','N' laid out like the Fortran
+ *    arrays cbc(0:1,3) (6 chars: x0 x1 y0 y1 z0 z1) and cbcvel(0:1,3,3)
+ *    (18 chars: the 6 of u, then v, then w).
+ *  - Scalars are passed by value; results the reference returns as scalars
+ *    are written to host pointers (the call then synchronises its stream).
+ *  - All work is enqueued on the context's HIP stream in call order, as the
+ *    reference enqueues on OpenACC queue 1.
+ *  - Return value: FZ_OK (0) or a negative FZ_ERR_* code; fz_last_error()
+ *    gives a message. No exception crosses the ABI.
+ *  - A context (and a case) is driven by one host thread; distinct contexts
+ *    are independent.
+ */
+#ifndef FIZZY_H
+#define FIZZY_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FZ_OK 0
+#define FZ_ERR_ARG (-1)
+#define FZ_ERR_HIP (-2)
+#define FZ_ERR_UNSUPPORTED (-3)
+#define FZ_ERR_STATE (-4)
+
+typedef struct fz_ctx fz_ctx;
+typedef struct fz_case fz_case;
+typedef struct fz_rk_state fz_rk_state;
+
+/* Case parameters: the &dns / &two_fluid namelist entries the hot path reads
+ * (src/param.f90:91-115) and the build switches it depends on
+ * (configs/flags.mk: ACDI or THINC/QQ, SDF normals, Brackbill normalisation,
+ * constant-coefficient pressure splitting are the supported build). */
+typedef struct {
+  int ng[3];
+  double l[3];
+  int gtype;
+  double gr;
+  double cfl, dtmax, dt_f;
+  int is_solve_ns, is_track_interface, is_forced_hit;
+  int icheck;
+  int vof;            /* 0: ACDI, 1: THINC/QQ (_INTERFACE_CAPTURING_VOF) */
+  char cbcvel[18];
+  char cbcpre[6];
+  char cbcpsi[6];
+  char cbcnor[18];
+  double bcvel[18];
+  double bcpre[6];
+  double bcpsi[6];
+  double bcnor[18];
+  double bforce[3], gacc[3];
+  double sigma;
+  double rho12[2], mu12[2], beta12[2];
+  double psi_thickness_factor, acdi_gam_factor, acdi_gam_min;
+  int nspheres;       /* spheres.in records (x,y,z,r), src/two_fluid.f90:439-471 */
+  int inipsi;         /* 0 uni,1 zer,2 bub3,3 bub2,4 bub1,5 drp3,6 drp2,7 drp1,8 flm */
+  double spheres[4 * 64];
+} fz_params;
+
+const char *fz_last_error(void);
+
+/* ---- context: grid, boundary conditions of the pressure, stream ---- */
+/* replaces the set-up of src/main.f90:156-291 (initmpi, initgrid, inverse
+ * spacings, workspaces); dzc..zf are host arrays of length n3+2 */
+int fz_ctx_create(fz_ctx **ctx, const int n[3], const double l[3], const double *dzc, const double *dzf,
+                  const double *zc, const double *zf, const char cbcpre[6], void *hip_stream);
+int fz_ctx_destroy(fz_ctx *ctx);
+int fz_ctx_sync(fz_ctx *ctx);
+int fz_malloc(fz_ctx *ctx, size_t bytes, void **dptr);
+int fz_free(fz_ctx *ctx, void *dptr);
+int fz_upload(fz_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int fz_download(fz_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+int fz_copy(fz_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
+
+/* initgrid, src/initgrid.f90:14-78 (host routine, arrays of length n+2) */
+int fz_initgrid(int gtype, int n, double gr, double lz, double *dzc, double *dzf, double *zc, double *zf);
+
+/* ---- Poisson solver ---- */
+/* initsolver, src/initsolver.f90:15-79; host outputs as in the reference
+ * (lambdaxy n1*n2, a,b,c n3, rhsbx n2*n3*2, rhsby n1*n3*2, rhsbz n1*n2*2;
+ * any of them may be NULL) and builds the device plan held by ctx */
+int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdaxy, double *a, double *b, double *c,
+                  double *normfft, double *rhsbx, double *rhsby, double *rhsbz);
+/* solver / solver_gpu, src/solver.f90:19-78, src/solver_gpu.f90:31-163 */
+int fz_solver(fz_ctx *ctx, double *p);
+/* fillps, src/fillps.f90:13-51 */
+int fz_fillps(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v, const double *w, double *p);
+/* updt_rhs_b, src/bound.f90:341-396 (boundary terms built by fz_initsolver) */
+int fz_updt_rhs_b(fz_ctx *ctx, double *p);
+/* correc, src/correc.f90:13-67 (constant-coefficient build; psi, rho12 unused) */
+int fz_correc(fz_ctx *ctx, double rho0, const double rho12[2], double dt, const double *p, const double *psi,
+              double *u, double *v, double *w);
+/* updatep, src/updatep.f90:13-24 */
+int fz_updatep(fz_ctx *ctx, const double *pp, double *p);
+/* chk_poisson, src/debug.f90:81-114: max |L(fpp) - fp| */
+int fz_chk_poisson(fz_ctx *ctx, const double *fp, const double *fpp, double *diffmax);
+
+/* ---- boundary conditions ---- */
+/* boundp, src/bound.f90:81-124 */
+int fz_boundp(fz_ctx *ctx, const char cbc[6], const double bc[6], double *p);
+/* bounduvw, src/bound.f90:15-79 */
+int fz_bounduvw(fz_ctx *ctx, const char cbc[18], const double bc[18], int is_correc, double *u, double *v,
+                double *w);
+
+/* ---- phase field ---- */
+/* acdi_transport_pf, src/acdi.f90:64-225 */
+int fz_acdi_transport_pf(fz_ctx *ctx, double gam, double seps, const double *u, const double *v, const double *w,
+                         const double *normx, const double *normy, const double *normz, const double *phi,
+                         const double *psi, double *dpsidt, double *flux_x, double *flux_y, double *flux_z);
+/* acdi_cmpt_phi, src/acdi.f90:227-248 */
+int fz_acdi_cmpt_phi(fz_ctx *ctx, double seps, const double *psi, double *phi);
+/* acdi_set_gamma, src/acdi.f90:32-62 (synchronises) */
+int fz_acdi_set_gamma(fz_ctx *ctx, double gam_factor, const double *u, const double *v, const double *w,
+                      double *gam);
+/* acdi_set_epsilon, src/acdi.f90:16-30 (host routine) */
+int fz_acdi_set_epsilon(fz_ctx *ctx, double seps_factor, double *seps);
+/* vof_thinc_transport_psi, src/vof_thinc_qq.f90:39-133 */
+int fz_vof_thinc_transport_psi(fz_ctx *ctx, double beta, const double *u, const double *v, const double *w,
+                               const double *normx, const double *normy, const double *normz, const double *phi,
+                               const double *psi, double *dpsidt, double *flux_x, double *flux_y, double *flux_z);
+/* vof_thinc_cmpt_phi, src/vof_thinc_qq.f90:135-157 */
+int fz_vof_thinc_cmpt_phi(fz_ctx *ctx, double beta, const double *psi, double *phi);
+/* clip_field, src/two_fluid.f90:35-58 (nh = 1) */
+int fz_clip_field(fz_ctx *ctx, const double minmax[2], double *p);
+/* cmpt_norm_curv_youngs, src/two_fluid.f90:60-175 */
+int fz_cmpt_norm_curv_youngs(fz_ctx *ctx, const double *psi, double *normx, double *normy, double *normz,
+                             double *kappa);
+/* init2fl, src/two_fluid.f90:247-437 (spheres taken from prm) */
+int fz_init2fl(fz_ctx *ctx, const fz_params *prm, double seps, double *psi);
+
+/* ---- momentum ---- */
+/* mom_xyz_ad, src/mom.f90:505-851 */
+int fz_mom_xyz_ad(fz_ctx *ctx, const double rho12[2], const double mu12[2], const double *psiflx_x,
+                  const double *psiflx_y, const double *psiflx_z, const double *u, const double *v,
+                  const double *w, const double *psi, double *dudt, double *dvdt, double *dwdt);
+/* mom_xyz_oth, src/mom.f90:853-1039 (beta12 unused without the scalar) */
+int fz_mom_xyz_oth(fz_ctx *ctx, double dt_r, const double rho12[2], const double beta12[2], const double bforce[3],
+                   const double gacc[3], double sigma, double rho0, const double rho_av[3], const double *p,
+                   const double *psi, const double *kappa, const double *pn, const double *po, double *dudt,
+                   double *dvdt, double *dwdt);
+/* lscale_forcing, src/forcing.f90:13-65 (ftype 1 ABC, 2 Taylor-Green) */
+int fz_lscale_forcing(fz_ctx *ctx, int ftype, double alpha, double dt, double *u, double *v, double *w);
+/* bulk_mean_12_stag, src/utils.f90:68-136 (synchronises) */
+int fz_bulk_mean_12_stag(fz_ctx *ctx, int idir, const double *psi, const double p12[2], double *mean);
+
+/* ---- time integration: rk and rk_2fl with their AB2 history held in a
+ * state object instead of `save` variables (src/rk.f90:44-70, :214-228) ---- */
+int fz_rk_state_create(fz_ctx *ctx, fz_rk_state **st);
+int fz_rk_state_destroy(fz_rk_state *st);
+/* rk, src/rk.f90:16-131 */
+int fz_rk(fz_ctx *ctx, fz_rk_state *st, const double rkpar[2], double dt, double dt_r, const double bforce[3],
+          const double gacc[3], double sigma, const double rho_av[3], const double rho12[2], const double mu12[2],
+          const double beta12[2], double rho0, const double *psi, const double *kappa, const double *p,
+          const double *pn, const double *po, const double *psio, const double *psiflx_x, const double *psiflx_y,
+          const double *psiflx_z, double *u, double *v, double *w);
+/* rk_2fl, src/rk.f90:193-254 (vof selects THINC/QQ, else ACDI) */
+int fz_rk_2fl(fz_ctx *ctx, fz_rk_state *st, const double rkpar[2], double dt, double gam, double seps, double beta,
+              int vof, const double *u, const double *v, const double *w, const double *normx, const double *normy,
+              const double *normz, const double *phi, double *psi, double *psiflx_x, double *psiflx_y,
+              double *psiflx_z);
+
+/* ---- checks ---- */
+/* chkdiv, src/chkdiv.f90:15-49 (synchronises) */
+int fz_chkdiv(fz_ctx *ctx, const double *u, const double *v, const double *w, double *divtot, double *divmax);
+/* chkdt, src/chkdt.f90:15-94 (synchronises) */
+int fz_chkdt(fz_ctx *ctx, int is_solve_ns, int is_track_interface, int vof, const double mu12[2],
+             const double rho12[2], double sigma, const double gacc[3], const double *u, const double *v,
+             const double *w, double gam, double seps, double *dtmax);
+
+/* ---- a whole case: the time loop of src/main.f90:414-562 on the device ---- */
+/* field ids: 0 u,1 v,2 w,3 p,4 pn,5 po,6 psi,7 phi,8 kappa,9 normx,10 normy,11 normz,
+ * 12 psiflx_x,13 psiflx_y,14 psiflx_z (halo'd); 15 dpsidt_old,16 dudt_old,17 dvdt_old,
+ * 18 dwdt_old (interior) */
+/* scalars: 0 time,1 dt,2 dto,3 gam,4 seps,5 rho0,6 divtot,7 divmax,8 dt_cfl,9 istep,10 is_first */
+int fz_case_create(fz_case **cs, const fz_params *prm, void *hip_stream);
+int fz_case_destroy(fz_case *cs);
+int fz_case_ctx(fz_case *cs, fz_ctx **ctx);
+int fz_case_field(fz_case *cs, int id, double **dptr, size_t *count);
+int fz_case_get_scalars(fz_case *cs, double out[12]);
+int fz_case_set_scalars(fz_case *cs, const double in[12]);
+/* initflow 'zer' + init2fl + boundary conditions + phi/normals + gamma + dt
+ * (src/main.f90:300-407) */
+int fz_case_init(fz_case *cs);
+/* nsteps time steps; *killed = 1 when a divergence or time-step check
+ * stopped the run as src/main.f90:530-562 does */
+int fz_case_step(fz_case *cs, int nsteps, int *killed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

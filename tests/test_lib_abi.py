@@ -1,0 +1,40 @@
+"""The C-ABI library loads and exports every entry point include/fizzy.h
+declares (no GPU needed: nothing is called)."""
+import ctypes as C
+import os
+
+import fizzy_amd as F
+from fizzy_amd import _lib
+
+
+def test_library_exports_every_declared_symbol():
+    L = F.lib()
+    declared = F.declared_symbols()
+    assert len(declared) >= 40
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert sorted(_lib.SIGNATURES) == F.declared_symbols()
+
+
+def test_params_layout_matches_oracle():
+    import oracle as O
+    a, b = F.FzParams, O.OrParams
+    assert C.sizeof(a) == C.sizeof(b)
+    for (na, ta), (nb, tb) in zip(a._fields_, b._fields_):
+        assert na == nb and getattr(a, na).offset == getattr(b, nb).offset and C.sizeof(ta) == C.sizeof(tb)
+
+
+def test_library_is_gfx950_code_object():
+    # the offload bundle in .hip_fatbin names its target
+    blob = open(F.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_host_only_entry_points_run_without_gpu():
+    import numpy as np
+    dzc, dzf, zc, zf = F.initgrid(1, 8, 0.0, 2.0)
+    assert np.allclose(dzf[1:-1], 0.25) and np.isclose(zc[1], 0.125)
+    assert os.path.exists(F.LIB_PATH)
