@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Throughput of the full FP64 time step on one MI355X.
+
+Workload: the 256x256x512 ACDI rising bubble (density ratio 1000, viscosity
+ratio 100, Re=35, Eo=10, periodic x/y, no-slip z; one sphere r=0.25 at
+(0.5,0.5,0.5) in a 1x1x2 box), synthetic initial state from the reference's
+own initial-condition rules. One "step" is one reference AB2 time step
+(src/main.f90:414-562): phase-field transport, phi/normals/curvature,
+momentum, projection with the FFT Poisson solve, gamma, and the dt/divergence
+checks every 10 steps. Fields stay resident in HBM; the timed region holds
+only device work and the checks' 8-byte read-backs.
+
+Prints one JSON line: metric Mcell-steps/s (cells x steps / seconds / 1e6),
+the dominant kernel's roofline from per-launch HIP events recorded over the
+timed region, and the CPU oracle timed on this host.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("cans-fizzy_amd", "oracle", ""):
+    p = os.path.join(ROOT, sub)
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# algorithmic bytes per cell of one launch: distinct fields read + written once, 8 B each
+# (the minimal dataflow of each routine; halo'd-array kernels count halo cells)
+ALG_BYTES = {
+    "fft_x_fwd": 16, "fft_y_fwd": 16, "fft_y_bwd": 16, "fft_x_bwd": 16, "thomas": 16, "thomas_periodic": 16,
+    "mom_ad": 80, "acdi_transport": 96, "youngs": 40, "mom_oth": 56, "rk_mom_ab2": 112, "rk_mom_oth": 80,
+    "rk_psi_oop": 32, "acdi_phi": 16, "clip": 16, "fillps": 32, "correc": 56, "updatep_oop": 24,
+    "red_gamma": 24, "red_div": 24, "red_dt": 24, "vof_flux": 88, "vof_div": 32, "vof_phi": 16, "forcing": 32,
+}
+HALO_KERNELS = {"acdi_phi", "clip", "updatep_oop", "vof_phi", "correc"}
+# minimal per-step dataflow of the whole step (stage model S1-S9): 61 passes of 8 B
+STEP_ALG_BYTES_PER_CELL = 488
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def prof_report(F, ctx):
+    buf = C.create_string_buffer(1 << 16)
+    F.call("fz_prof_report", ctx, buf, len(buf))
+    rows = {}
+    for line in buf.value.decode().splitlines():
+        name, calls, ms = line.split()
+        rows[name] = (int(calls), float(ms))
+    return rows
+
+
+def cpu_baseline(spec, budget_s=15.0, max_steps=20):
+    """The oracle (plain C, OpenMP over k-planes) on this host's cores: AB2
+    steps of the same case after the first (is_first) step, until about
+    budget_s seconds of stepping or max_steps steps."""
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    t0 = time.perf_counter()
+    oc = O.Case(spec)
+    oc.init_fields()
+    oc.step(1)  # the is_first step (history copies), not timed
+    t1 = time.perf_counter()
+    steps = 0
+    while steps < max_steps and time.perf_counter() - t1 < budget_s:
+        oc.step(1)
+        steps += 1
+    dt = time.perf_counter() - t1
+    n = spec["ng"]
+    cells = n[0] * n[1] * n[2]
+    log(f"cpu oracle: set-up + first step {t1 - t0:.1f} s, {steps} timed steps {dt:.2f} s on {threads} threads")
+    return {"value": round(cells * steps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"AB2 steps 2-{steps + 1} of the same {n[0]}x{n[1]}x{n[2]} case by the C oracle "
+                      f"(OpenMP, {threads} threads), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["c2", "c0"], default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-prof", action="store_true", help="time without per-launch events")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one MI355X: this build runs the hot path on a single GPU "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world}); nothing measured.", file=sys.stderr)
+        sys.exit(2)
+
+    import fizzy_amd as F
+    from fizzy_amd import cases
+
+    spec = cases.config2() if args.workload == "c2" else cases.config0()
+    n = spec["ng"]
+    cells = n[0] * n[1] * n[2]
+    log(f"workload {args.workload}: {n[0]}x{n[1]}x{n[2]} = {cells} cells, warmup {args.warmup}, steps {args.steps}")
+    dc = F.DeviceCase(spec)
+    dc.init()
+    if args.warmup:
+        dc.step(args.warmup)
+    dc.sync()
+    F.call("fz_prof_enable", dc.ctx, 0 if args.no_prof else 1)
+    prof_report(F, dc.ctx)  # clear
+    t0 = time.perf_counter()
+    killed = dc.step(args.steps)
+    dc.sync()
+    t1 = time.perf_counter()
+    F.call("fz_prof_enable", dc.ctx, 0)
+    rows = {} if args.no_prof else prof_report(F, dc.ctx)
+    sc = dc.scalars()
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = cells * args.steps / elapsed / 1e6
+    log(f"{ms_per_step:.3f} ms/step, {value:.1f} Mcell-steps/s, divmax {sc[7]:.2e}, killed {killed}")
+
+    roof = None
+    if rows:
+        total_ms = sum(ms for _, ms in rows.values())
+        ranked = sorted(rows.items(), key=lambda kv: -kv[1][1])
+        for name, (calls, ms) in ranked[:12]:
+            log(f"  {name:18s} {calls:5d} launches {ms / calls * 1e3:9.1f} us/launch {100 * ms / total_ms:5.1f}%")
+        dom = next(name for name, _ in ranked if name in ALG_BYTES)
+        calls, ms = rows[dom]
+        units = (n[0] + 2) * (n[1] + 2) * (n[2] + 2) if dom in HALO_KERNELS else cells
+        bytes_per_launch = ALG_BYTES[dom] * units
+        avg_s = ms / calls / 1e3
+        achieved = bytes_per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bytes_per_launch,
+                "kernel_time_share": round(ms / total_ms, 4)}
+    step_bw = STEP_ALG_BYTES_PER_CELL * cells / (elapsed / args.steps) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(spec)
+    out = {
+        "metric": "Mcell-steps/s per RK3 step, FP64 vs mixed; ensemble p50/p99 step latency",
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: the reference's rising-bubble initial condition (sphere, fluid at rest)",
+        "config": {"workload": ("acdi_rising_bubble_256x256x512" if args.workload == "c2"
+                                else "acdi_rising_bubble_64x64x128"),
+                   "grid": n, "density_ratio": spec["rho12"][0] / spec["rho12"][1],
+                   "viscosity_ratio": spec["mu12"][0] / spec["mu12"][1], "precision": "fp64",
+                   "step": "AB2 (one reference time step)"},
+        "roofline": roof,
+        "step_roofline": {"alg_bytes_per_cell": STEP_ALG_BYTES_PER_CELL, "achieved_GBs": round(step_bw, 1),
+                          "frac": round(step_bw / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

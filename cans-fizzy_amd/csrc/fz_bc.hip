@@ -105,7 +105,7 @@ static int launch_dir(fz_ctx *c, int dir, double *f0, double *f1, double *f2, co
   if (dir == 0) { na = d.n2 + 2; nb = d.n3 + 2; }
   else if (dir == 1) { na = d.n1 + 2; nb = d.n3 + 2; }
   else { na = d.n1 + 2; nb = d.n2 + 2; }
-  hipLaunchKernelGGL(k_plane_ops, dim3((na + 63) / 64, nb), dim3(64), 0, c->st, d, dir, f0, f1, f2, ops);
+  FZ_LAUNCH(c, "plane_ops", k_plane_ops, dim3((na + 63) / 64, nb), dim3(64), 0, d, dir, f0, f1, f2, ops);
   FZ_LAUNCHED();
   return FZ_OK;
 }

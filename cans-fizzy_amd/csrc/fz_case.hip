@@ -283,7 +283,7 @@ extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
                                           f[F_FZ]));
       }
       if (is_first) FZ_HIP(hipMemcpyAsync(f[F_DPSIO], cs->dpsidt, bi, hipMemcpyDeviceToDevice, c->st));
-      hipLaunchKernelGGL(k_rk_psi_oop, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, factor1, factor2,
+      FZ_LAUNCH(c, "rk_psi_oop", k_rk_psi_oop, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1, factor2,
                          cs->dpsidt, f[F_DPSIO], cs->psio, f[F_PSI]);
       FZ_LAUNCHED();
       const double mm[2] = {0.0, 1.0};
@@ -333,8 +333,7 @@ extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
       FZ_TRY(fz_correc(c, rho0, P.rho12, dt, cs->dp, f[F_PSI], f[F_U], f[F_V], f[F_W]));
       FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 1, f[F_U], f[F_V], f[F_W]));
       // updatep into the buffer of the previous pressure, then rotate
-      hipLaunchKernelGGL(k_updatep_oop, dim3((int)std::min<long>((c->nh + 255) / 256, 8192)), dim3(256), 0, c->st,
-                         c->nh, cs->dp, f[F_P], f[F_PO]);
+      FZ_LAUNCH(c, "updatep_oop", k_updatep_oop, dim3((int)std::min<long>((c->nh + 255) / 256, 8192)), dim3(256), 0, c->nh, cs->dp, f[F_P], f[F_PO]);
       FZ_LAUNCHED();
       swap_ptr(f[F_P], f[F_PO]);  // p <- p + dp, po <- p of the step before (= pn)
       FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));

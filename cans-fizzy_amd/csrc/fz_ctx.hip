@@ -128,6 +128,11 @@ extern "C" int fz_ctx_destroy(fz_ctx *c) {
   for (int q = 0; q < 6; q++)
     if (dev[q]) (void)hipFree(dev[q]);
   if (c->pois) fz_poisson_destroy(c->pois);
+  for (const fz_prof_rec &r : c->prof_recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
   if (c->red) (void)hipFree(c->red);
   if (c->h_small) (void)hipHostFree(c->h_small);
   if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
@@ -173,6 +178,71 @@ extern "C" int fz_download(fz_ctx *c, void *dst, const void *src, size_t bytes) 
 extern "C" int fz_copy(fz_ctx *c, void *dst, const void *src, size_t bytes) {
   if (!c || (!dst && bytes) || (!src && bytes)) return fz_set_error(FZ_ERR_ARG, "fz_copy: null argument");
   FZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
+  return FZ_OK;
+}
+
+// ---- per-launch timing ----
+static hipEvent_t prof_event(fz_ctx *c) {
+  if (!c->prof_pool.empty()) {
+    hipEvent_t e = c->prof_pool.back();
+    c->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+fz_prof_scope::fz_prof_scope(fz_ctx *c_, const char *n) : c(c_), name(n) {
+  if (!c || !c->prof_on) return;
+  a = prof_event(c);
+  if (a) (void)hipEventRecord(a, c->st);
+}
+
+fz_prof_scope::~fz_prof_scope() {
+  if (!a) return;
+  hipEvent_t b = prof_event(c);
+  if (!b) return;
+  (void)hipEventRecord(b, c->st);
+  c->prof_recs.push_back(fz_prof_rec{name, a, b});
+}
+
+extern "C" int fz_prof_enable(fz_ctx *c, int on) {
+  if (!c) return fz_set_error(FZ_ERR_ARG, "fz_prof_enable: null context");
+  c->prof_on = on != 0;
+  return FZ_OK;
+}
+
+// "name calls total_ms" lines, one per launch site, in first-seen order; clears the records
+extern "C" int fz_prof_report(fz_ctx *c, char *buf, size_t cap) {
+  if (!c || !buf || cap == 0) return fz_set_error(FZ_ERR_ARG, "fz_prof_report: bad argument");
+  FZ_HIP(hipStreamSynchronize(c->st));
+  std::vector<const char *> names;
+  std::vector<double> tot;
+  std::vector<long> cnt;
+  for (const fz_prof_rec &r : c->prof_recs) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    size_t q = 0;
+    while (q < names.size() && strcmp(names[q], r.name) != 0) q++;
+    if (q == names.size()) {
+      names.push_back(r.name);
+      tot.push_back(0.0);
+      cnt.push_back(0);
+    }
+    tot[q] += ms;
+    cnt[q] += 1;
+    c->prof_pool.push_back(r.a);
+    c->prof_pool.push_back(r.b);
+  }
+  c->prof_recs.clear();
+  size_t used = 0;
+  buf[0] = '\0';
+  for (size_t q = 0; q < names.size() && used + 1 < cap; q++) {
+    const int w = snprintf(buf + used, cap - used, "%s %ld %.6f\n", names[q], cnt[q], tot[q]);
+    if (w < 0) break;
+    used += (size_t)w;
+  }
   return FZ_OK;
 }
 

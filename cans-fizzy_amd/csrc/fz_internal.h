@@ -57,7 +57,16 @@ struct fz_poisson {
   char cbc[6];
 };
 
+// per-launch timing with HIP events on the context's stream (off by default)
+struct fz_prof_rec {
+  const char *name;
+  hipEvent_t a, b;
+};
+
 struct fz_ctx {
+  bool prof_on = false;
+  std::vector<fz_prof_rec> prof_recs;
+  std::vector<hipEvent_t> prof_pool;
   fz_dims d;
   long nh = 0, ni = 0;  // element counts of a halo'd and of an interior array
   double l[3], dl[3], dli[3];
@@ -72,6 +81,20 @@ struct fz_ctx {
   size_t red_cap = 0;
   double *h_small = nullptr;  // pinned host landing zone for reduction results
 };
+
+// brackets one launch with events when profiling is on
+struct fz_prof_scope {
+  fz_ctx *c;
+  const char *name;
+  hipEvent_t a = nullptr;
+  fz_prof_scope(fz_ctx *c_, const char *n);
+  ~fz_prof_scope();
+};
+#define FZ_LAUNCH(c, name, kernel, grid, block, shmem, ...)                              \
+  do {                                                                                   \
+    fz_prof_scope fz_ps_((c), name);                                                     \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, (c)->st, __VA_ARGS__);                \
+  } while (0)
 
 // ---- launch helpers ----
 static inline dim3 fz_block3() { return dim3(64, 4, 1); }

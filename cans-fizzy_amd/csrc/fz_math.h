@@ -1,0 +1,208 @@
+// FP64 exp, expm1, log, tanh, sin and cos for the hot-path kernels.
+//
+// Why not the device math library: the reference's phase field goes through
+// phi = eps*log((psi+eps)/(1-psi+eps)) (src/acdi.f90:244), which maps a
+// one-ulp difference of psi in the saturated bulk (psi within 1e-14 of 0 or 1)
+// to an O(eps) change of phi, hence of the Youngs normals and curvature and
+// of the surface-tension force one step later. Two builds whose tanh/log
+// differ by one ulp therefore drift apart at O(1e-1) within a few steps. These
+// routines are written out with +,-,*,/ only (Cody-Waite reduction and
+// Taylor/atanh series, within about two ulps), so every build that uses
+// them rounds identically; the CPU oracle carries its own copy of the same
+// method, and tests pin both against the C library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define FZ_HD __host__ __device__ inline
+
+namespace fzm {
+
+FZ_HD double bits_to_double(uint64_t b) { return __builtin_bit_cast(double, b); }
+FZ_HD uint64_t double_to_bits(double x) { return __builtin_bit_cast(uint64_t, x); }
+
+// x * 2^k for integer k in [-2044, 2046] (two exact power-of-two factors)
+FZ_HD double scale2(double x, int k) {
+  if (k > 1023) {
+    x *= bits_to_double((uint64_t)(1023 + 1023) << 52);
+    k -= 1023;
+  } else if (k < -1022) {
+    x *= bits_to_double((uint64_t)(1023 - 1022) << 52);
+    k += 1022;
+  }
+  return x * bits_to_double((uint64_t)(k + 1023) << 52);
+}
+
+constexpr double LN2_HI = 0.6931467056274414;      // ln 2 with 32 trailing zero bits
+constexpr double LN2_LO = 4.7493250390316726e-07;  // ln 2 - LN2_HI
+constexpr double INV_LN2 = 1.4426950408889634;
+
+// exp(r) - 1 for |r| <= ln2/2: Taylor series to r^14
+FZ_HD double expm1_kernel(double r) {
+  double p = 1.1470745597729725e-11;
+  p = 1.6059043836821613e-10 + r * p;
+  p = 2.08767569878681e-09 + r * p;
+  p = 2.505210838544172e-08 + r * p;
+  p = 2.755731922398589e-07 + r * p;
+  p = 2.7557319223985893e-06 + r * p;
+  p = 2.48015873015873e-05 + r * p;
+  p = 0.0001984126984126984 + r * p;
+  p = 0.001388888888888889 + r * p;
+  p = 0.008333333333333333 + r * p;
+  p = 0.041666666666666664 + r * p;
+  p = 0.16666666666666666 + r * p;
+  p = 0.5 + r * p;
+  return r + (r * r) * p;
+}
+
+/* exp(r) - 1 for |r| <= 1.05: Taylor series to r^20 */
+FZ_HD double expm1_kernel20(double r) {
+  double p = 4.110317623312165e-19;
+  p = 8.22063524662433e-18 + r * p;
+  p = 1.5619206968586225e-16 + r * p;
+  p = 2.8114572543455206e-15 + r * p;
+  p = 4.779477332387385e-14 + r * p;
+  p = 7.647163731819816e-13 + r * p;
+  p = 1.1470745597729725e-11 + r * p;
+  p = 1.6059043836821613e-10 + r * p;
+  p = 2.08767569878681e-09 + r * p;
+  p = 2.505210838544172e-08 + r * p;
+  p = 2.755731922398589e-07 + r * p;
+  p = 2.7557319223985893e-06 + r * p;
+  p = 2.48015873015873e-05 + r * p;
+  p = 0.0001984126984126984 + r * p;
+  p = 0.001388888888888889 + r * p;
+  p = 0.008333333333333333 + r * p;
+  p = 0.041666666666666664 + r * p;
+  p = 0.16666666666666666 + r * p;
+  p = 0.5 + r * p;
+  return r + (r * r) * p;
+}
+
+FZ_HD double exp(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return __builtin_huge_val();
+  if (x < -745.1332191019412) return 0.0;
+  const double kf = __builtin_floor(x * INV_LN2 + 0.5);
+  const int k = (int)kf;
+  const double r = (x - kf * LN2_HI) - kf * LN2_LO;
+  return scale2(1.0 + expm1_kernel(r), k);
+}
+
+FZ_HD double expm1(double x) {
+  if (x != x) return x;
+  const double a = x < 0.0 ? -x : x;
+  if (a <= 1.04) return expm1_kernel20(x);
+  if (x > 709.782712893384) return __builtin_huge_val();
+  if (x < -40.0) return -1.0;
+  return exp(x) - 1.0;
+}
+
+// log(x): x = 2^k m, m in [sqrt(2)/2, sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1)
+FZ_HD double log(double x) {
+  if (x != x) return x;
+  if (x < 0.0) return __builtin_nan("");
+  if (x == 0.0) return -__builtin_huge_val();
+  if (x == __builtin_huge_val()) return x;
+  int k = 0;
+  if (x < 2.2250738585072014e-308) {  // subnormal
+    x *= 18014398509481984.0;          // 2^54
+    k = -54;
+  }
+  const uint64_t b = double_to_bits(x);
+  k += (int)((b >> 52) & 0x7ff) - 1023;
+  double m = bits_to_double((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);  // [1,2)
+  if (m > 1.4142135623730951) {
+    m = 0.5 * m;
+    k += 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double p = 0.08695652173913043;
+  p = 0.09523809523809523 + z * p;
+  p = 0.10526315789473684 + z * p;
+  p = 0.11764705882352941 + z * p;
+  p = 0.13333333333333333 + z * p;
+  p = 0.15384615384615385 + z * p;
+  p = 0.18181818181818182 + z * p;
+  p = 0.2222222222222222 + z * p;
+  p = 0.2857142857142857 + z * p;
+  p = 0.4 + z * p;
+  p = 0.6666666666666666 + z * p;
+  const double lm = 2.0 * s + (s * z) * p;
+  const double kd = (double)k;
+  return kd * LN2_HI + (lm + kd * LN2_LO);
+}
+
+FZ_HD double tanh(double x) {
+  if (x != x) return x;
+  const double a = x < 0.0 ? -x : x;
+  double r;
+  if (a > 22.0) r = 1.0;
+  else if (a < 3.725290298461914e-09) r = a;
+  else if (a < 0.52) {
+    const double t = expm1_kernel20(2.0 * a);
+    r = t / (t + 2.0);
+  } else {
+    r = 1.0 - 2.0 / (exp(2.0 * a) + 1.0);
+  }
+  return x < 0.0 ? -r : r;
+}
+
+constexpr double PIO2_1 = 1.5707963109016418;     // pi/2 in three 26-bit pieces
+constexpr double PIO2_2 = 1.5893254712295857e-08;
+constexpr double PIO2_3 = 6.123233995736766e-17;
+constexpr double TWO_OVER_PI = 0.6366197723675814;
+
+FZ_HD double sin_kernel(double r) {  // |r| <= pi/4
+  const double z = r * r;
+  double p = -7.647163731819816e-13;
+  p = 1.6059043836821613e-10 + z * p;
+  p = -2.505210838544172e-08 + z * p;
+  p = 2.7557319223985893e-06 + z * p;
+  p = -0.0001984126984126984 + z * p;
+  p = 0.008333333333333333 + z * p;
+  p = -0.16666666666666666 + z * p;
+  return r + (r * z) * p;
+}
+FZ_HD double cos_kernel(double r) {
+  const double z = r * r;
+  double p = 4.779477332387385e-14;
+  p = -1.1470745597729725e-11 + z * p;
+  p = 2.08767569878681e-09 + z * p;
+  p = -2.755731922398589e-07 + z * p;
+  p = 2.48015873015873e-05 + z * p;
+  p = -0.001388888888888889 + z * p;
+  p = 0.041666666666666664 + z * p;
+  return (1.0 - 0.5 * z) + (z * z) * p;
+}
+// quadrant reduction for |x| < 2^20
+FZ_HD double reduce_pio2(double x, int *q) {
+  const double kf = __builtin_floor(x * TWO_OVER_PI + 0.5);
+  *q = ((int)kf) & 3;
+  return ((x - kf * PIO2_1) - kf * PIO2_2) - kf * PIO2_3;
+}
+FZ_HD double sin(double x) {
+  int q;
+  const double r = reduce_pio2(x, &q);
+  switch (q) {
+    case 0: return sin_kernel(r);
+    case 1: return cos_kernel(r);
+    case 2: return -sin_kernel(r);
+    default: return -cos_kernel(r);
+  }
+}
+FZ_HD double cos(double x) {
+  int q;
+  const double r = reduce_pio2(x, &q);
+  switch (q) {
+    case 0: return cos_kernel(r);
+    case 1: return -sin_kernel(r);
+    case 2: return -cos_kernel(r);
+    default: return sin_kernel(r);
+  }
+}
+
+}  // namespace fzm

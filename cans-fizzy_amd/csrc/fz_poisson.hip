@@ -12,6 +12,7 @@
 #include <cstring>
 
 #include "fz_internal.h"
+#include "fz_math.h"
 
 enum { K_R2HC = 0, K_HC2R = 1, K_REDFT01 = 4, K_REDFT10 = 5, K_REDFT11 = 6, K_RODFT01 = 8, K_RODFT10 = 9, K_RODFT11 = 10 };
 
@@ -345,9 +346,9 @@ static int factorize(int m, int *rad, int *nrad) {
   return 0;
 }
 
-static int fftplan_init(fz_fftplan *P, char b0, char b1, int n) {
-  double nrm;
-  find_fft_c(b0, b1, &P->kind_f, &P->kind_b, &nrm);
+static int fftplan_init_kinds(fz_fftplan *P, int kind_f, int kind_b, int n) {
+  P->kind_f = kind_f;
+  P->kind_b = kind_b;
   P->n = n;
   P->m = kind_is_quarter(P->kind_f) ? 2 * n : n;
   if (P->m > 2048)
@@ -355,11 +356,11 @@ static int fftplan_init(fz_fftplan *P, char b0, char b1, int n) {
   if (factorize(P->m, P->rad, &P->nrad) != 0)
     return fz_set_error(FZ_ERR_UNSUPPORTED, "transform length %d has a prime factor above 13", P->m);
   std::vector<double2> tw(P->m), q(n + 1), t2(n + 1);
-  for (int j = 0; j < P->m; j++) tw[j] = make_double2(cos(2.0 * FZ_PI * (double)j / (double)P->m),
-                                                      -sin(2.0 * FZ_PI * (double)j / (double)P->m));
+  for (int j = 0; j < P->m; j++) tw[j] = make_double2(fzm::cos(2.0 * FZ_PI * (double)j / (double)P->m),
+                                                      -fzm::sin(2.0 * FZ_PI * (double)j / (double)P->m));
   for (int k = 0; k <= n; k++) {
-    q[k] = make_double2(cos(FZ_PI * (double)k / (2.0 * n)), sin(FZ_PI * (double)k / (2.0 * n)));
-    t2[k] = make_double2(cos(FZ_PI * (double)(2 * k + 1) / (4.0 * n)), sin(FZ_PI * (double)(2 * k + 1) / (4.0 * n)));
+    q[k] = make_double2(fzm::cos(FZ_PI * (double)k / (2.0 * n)), fzm::sin(FZ_PI * (double)k / (2.0 * n)));
+    t2[k] = make_double2(fzm::cos(FZ_PI * (double)(2 * k + 1) / (4.0 * n)), fzm::sin(FZ_PI * (double)(2 * k + 1) / (4.0 * n)));
   }
   FZ_HIP(hipMalloc((void **)&P->tw, sizeof(double2) * P->m));
   FZ_HIP(hipMalloc((void **)&P->q, sizeof(double2) * (n + 1)));
@@ -368,6 +369,13 @@ static int fftplan_init(fz_fftplan *P, char b0, char b1, int n) {
   FZ_HIP(hipMemcpy(P->q, q.data(), sizeof(double2) * (n + 1), hipMemcpyHostToDevice));
   FZ_HIP(hipMemcpy(P->t2, t2.data(), sizeof(double2) * (n + 1), hipMemcpyHostToDevice));
   return FZ_OK;
+}
+
+static int fftplan_init(fz_fftplan *P, char b0, char b1, int n) {
+  double nrm;
+  int kf, kb;
+  find_fft_c(b0, b1, &kf, &kb, &nrm);
+  return fftplan_init_kinds(P, kf, kb, n);
 }
 
 static void fftplan_free(fz_fftplan *P) {
@@ -449,6 +457,8 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
     fz_poisson_destroy(ctx->pois);
     ctx->pois = nullptr;
   }
+  if (cbc[4] == 'P' && cbc[5] == 'P' && n3 < 2)
+    return fz_set_error(FZ_ERR_UNSUPPORTED, "fz_initsolver: periodic z needs n3 >= 2 (gaussel_periodic)");
   fz_poisson *P = new fz_poisson();
   memcpy(P->cbc, cbc, 6);
   P->normfft = normfft;
@@ -481,8 +491,8 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
   return FZ_OK;
 }
 
-static int launch_r2r(fz_ctx *ctx, const fz_fftplan &P, int kind, const double *src, LineMap in, double *dst,
-                      LineMap out, int L1, int L2, double scale) {
+static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
+                      double *dst, LineMap out, int L1, int L2, double scale) {
   const bool quarter = kind_is_quarter(kind);
   int G;
   size_t lds;
@@ -501,7 +511,7 @@ static int launch_r2r(fz_ctx *ctx, const fz_fftplan &P, int kind, const double *
   RadList rl;
   for (int s = 0; s < 24; s++) rl.r[s] = P.rad[s];
   dim3 grid((L1 + G - 1) / G, L2, 1);
-  hipLaunchKernelGGL(k_r2r, grid, dim3(256), lds, ctx->st, src, in, dst, out, L1, P.n, P.m, kind, G, scale, P.tw,
+  FZ_LAUNCH(ctx, name, k_r2r, grid, dim3(256), lds, src, in, dst, out, L1, P.n, P.m, kind, G, scale, P.tw,
                      P.q, P.t2, P.nrad, rl);
   FZ_LAUNCHED();
   return FZ_OK;
@@ -517,24 +527,42 @@ int fz_solver_dev(fz_ctx *ctx, double *p) {
   const LineMap p_x = {1 + sy + sz, sy, sz, 1};  // x-lines of the halo'd field, (j,k) -> line
   const LineMap w_x = {0, n1, ps, 1};
   const LineMap w_y = {0, 1, ps, n1};
-  FZ_TRY(launch_r2r(ctx, P->px, P->px.kind_f, p, p_x, W, w_x, n2, n3, 1.0));
-  FZ_TRY(launch_r2r(ctx, P->py, P->py.kind_f, W, w_y, W, w_y, n1, n3, 1.0));
+  FZ_TRY(launch_r2r(ctx, "fft_x_fwd", P->px, P->px.kind_f, p, p_x, W, w_x, n2, n3, 1.0));
+  FZ_TRY(launch_r2r(ctx, "fft_y_fwd", P->py, P->py.kind_f, W, w_y, W, w_y, n1, n3, 1.0));
   const int tpb = 256;
   const int nb = (int)((ps + tpb - 1) / tpb);
   if (!P->periodic_z) {
-    hipLaunchKernelGGL(k_thomas, dim3(nb), dim3(tpb), 0, ctx->st, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
+    FZ_LAUNCH(ctx, "thomas", k_thomas, dim3(nb), dim3(tpb), 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
                        n3);
   } else {
-    hipLaunchKernelGGL(k_thomas_periodic, dim3(nb), dim3(tpb), 0, ctx->st, W, P->piv, P->aux, P->lambdaxy, P->a,
+    FZ_LAUNCH(ctx, "thomas_periodic", k_thomas_periodic, dim3(nb), dim3(tpb), 0, W, P->piv, P->aux, P->lambdaxy, P->a,
                        P->b, P->c, n1, n2, n3);
   }
   FZ_LAUNCHED();
-  FZ_TRY(launch_r2r(ctx, P->py, P->py.kind_b, W, w_y, W, w_y, n1, n3, 1.0));
-  FZ_TRY(launch_r2r(ctx, P->px, P->px.kind_b, W, w_x, p, p_x, n2, n3, P->normfft));
+  FZ_TRY(launch_r2r(ctx, "fft_y_bwd", P->py, P->py.kind_b, W, w_y, W, w_y, n1, n3, 1.0));
+  FZ_TRY(launch_r2r(ctx, "fft_x_bwd", P->px, P->px.kind_b, W, w_x, p, p_x, n2, n3, P->normfft));
   return FZ_OK;
 }
 
 extern "C" int fz_solver(fz_ctx *ctx, double *p) {
   if (!ctx || !p) return fz_set_error(FZ_ERR_ARG, "fz_solver: null argument");
   return fz_solver_dev(ctx, p);
+}
+
+// fft, src/fft.f90:176-187: one r2r kind over `howmany` contiguous lines of
+// length n (in place, unnormalised); builds a transient plan
+extern "C" int fz_r2r(fz_ctx *ctx, int kind, int n, int howmany, double *x) {
+  if (!ctx || !x || n < 1 || howmany < 0) return fz_set_error(FZ_ERR_ARG, "fz_r2r: bad argument");
+  const int kinds[8] = {K_R2HC, K_HC2R, K_REDFT01, K_REDFT10, K_REDFT11, K_RODFT01, K_RODFT10, K_RODFT11};
+  bool ok = false;
+  for (int q = 0; q < 8; q++) ok = ok || (kinds[q] == kind);
+  if (!ok) return fz_set_error(FZ_ERR_UNSUPPORTED, "fz_r2r: unsupported kind %d", kind);
+  if (howmany == 0) return FZ_OK;
+  fz_fftplan P;
+  FZ_TRY(fftplan_init_kinds(&P, kind, kind, n));
+  const LineMap lines = {0, n, 0, 1};
+  int rc = launch_r2r(ctx, "fft_lines", P, kind, x, lines, x, lines, howmany, 1, 1.0);
+  if (rc == FZ_OK && hipStreamSynchronize(ctx->st) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "fz_r2r: sync");
+  fftplan_free(&P);
+  return rc;
 }

@@ -188,9 +188,9 @@ int fz_acdi_gamma_dev(fz_ctx *c, double gam_factor, double gam_min, const double
                       const double *w, double *d_gam) {
   const int nb = red_blocks(c);
   FZ_TRY(fz_reduce_ensure(c, nb));
-  hipLaunchKernelGGL(k_red_gamma, dim3(nb), dim3(RED_THREADS), 0, c->st, c->d, u, v, w, c->red);
+  FZ_LAUNCH(c, "red_gamma", k_red_gamma, dim3(nb), dim3(RED_THREADS), 0, c->d, u, v, w, c->red);
   FZ_LAUNCHED();
-  hipLaunchKernelGGL(k_fin_gamma, dim3(1), dim3(RED_THREADS), 0, c->st, nb, c->red, gam_factor, gam_min, d_gam);
+  FZ_LAUNCH(c, "fin_gamma", k_fin_gamma, dim3(1), dim3(RED_THREADS), 0, nb, c->red, gam_factor, gam_min, d_gam);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -215,10 +215,10 @@ extern "C" int fz_acdi_set_gamma(fz_ctx *c, double gam_factor, const double *u, 
 int fz_chkdiv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out) {
   const int nb = red_blocks(c);
   FZ_TRY(fz_reduce_ensure(c, nb + 8));
-  hipLaunchKernelGGL(k_red_div, dim3(nb), dim3(RED_THREADS), 0, c->st, c->d, c->dzfi, c->dli[0], c->dli[1], u, v, w,
+  FZ_LAUNCH(c, "red_div", k_red_div, dim3(nb), dim3(RED_THREADS), 0, c->d, c->dzfi, c->dli[0], c->dli[1], u, v, w,
                      c->red);
   FZ_LAUNCHED();
-  hipLaunchKernelGGL(k_fin_div, dim3(1), dim3(RED_THREADS), 0, c->st, nb, c->red, d_out);
+  FZ_LAUNCH(c, "fin_div", k_fin_div, dim3(1), dim3(RED_THREADS), 0, nb, c->red, d_out);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -240,10 +240,10 @@ extern "C" int fz_chkdiv(fz_ctx *c, const double *u, const double *v, const doub
 int fz_chkdt_adv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out) {
   const int nb = red_blocks(c);
   FZ_TRY(fz_reduce_ensure(c, nb + 8));
-  hipLaunchKernelGGL(k_red_dt, dim3(nb), dim3(RED_THREADS), 0, c->st, c->d, c->dzci, c->dzfi, 1.0 / c->dl[0],
+  FZ_LAUNCH(c, "red_dt", k_red_dt, dim3(nb), dim3(RED_THREADS), 0, c->d, c->dzci, c->dzfi, 1.0 / c->dl[0],
                      1.0 / c->dl[1], u, v, w, c->red);
   FZ_LAUNCHED();
-  hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(RED_THREADS), 0, c->st, nb, c->red, d_out);
+  FZ_LAUNCH(c, "fin_max", k_fin_max, dim3(1), dim3(RED_THREADS), 0, nb, c->red, d_out);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -297,9 +297,9 @@ int fz_bulk_mean_dev(fz_ctx *c, int idir, const double *psi, const double p12[2]
   const int nb = red_blocks(c);
   FZ_TRY(fz_reduce_ensure(c, nb + 8));
   const long off = idir == 1 ? 1 : (idir == 2 ? c->d.sy : c->d.sz);
-  hipLaunchKernelGGL(k_red_bulk, dim3(nb), dim3(RED_THREADS), 0, c->st, c->d, off, gvr, p12[0], p12[1], psi, c->red);
+  FZ_LAUNCH(c, "red_bulk", k_red_bulk, dim3(nb), dim3(RED_THREADS), 0, c->d, off, gvr, p12[0], p12[1], psi, c->red);
   FZ_LAUNCHED();
-  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(RED_THREADS), 0, c->st, nb, c->red, d_out);
+  FZ_LAUNCH(c, "fin_sum", k_fin_sum, dim3(1), dim3(RED_THREADS), 0, nb, c->red, d_out);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -328,11 +328,11 @@ extern "C" int fz_chk_poisson(fz_ctx *c, const double *fp, const double *fpp, do
   if (!c || !fp || !fpp || !diffmax) return fz_set_error(FZ_ERR_ARG, "fz_chk_poisson: null argument");
   const int nb = red_blocks(c);
   FZ_TRY(fz_reduce_ensure(c, nb + 8));
-  hipLaunchKernelGGL(k_red_poisson, dim3(nb), dim3(RED_THREADS), 0, c->st, c->d, c->dzci, c->dzfi, c->dli[0],
+  FZ_LAUNCH(c, "red_poisson", k_red_poisson, dim3(nb), dim3(RED_THREADS), 0, c->d, c->dzci, c->dzfi, c->dli[0],
                      c->dli[1], fp, fpp, c->red);
   FZ_LAUNCHED();
   double *res = c->red + 2 * (c->red_cap - 4);
-  hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(RED_THREADS), 0, c->st, nb, c->red, res);
+  FZ_LAUNCH(c, "fin_max", k_fin_max, dim3(1), dim3(RED_THREADS), 0, nb, c->red, res);
   FZ_LAUNCHED();
   return fetch(c, res, 1, diffmax);
 }

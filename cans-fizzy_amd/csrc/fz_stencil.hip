@@ -11,6 +11,7 @@
 #include <cmath>
 
 #include "fz_internal.h"
+#include "fz_math.h"
 
 #define CELL_IJK(d)                                   \
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1; \
@@ -113,12 +114,12 @@ __global__ void __launch_bounds__(256) k_acdi_transport(fz_dims d, const double 
   const double rn_13 = nz_zp / (sqrt(nx_zp * nx_zp + ny_zp * ny_zp + nz_zp * nz_zp) + FZ_EPS);
   const double g4 = 0.25 * gam;
   double t;
-  t = tanh(0.25 * (phi_ccc + phi_mcc) / seps); const double sharpxm = g4 * ((1.0 - t * t) * rn_01);
-  t = tanh(0.25 * (phi_pcc + phi_ccc) / seps); const double sharpxp = g4 * ((1.0 - t * t) * rn_11);
-  t = tanh(0.25 * (phi_ccc + phi_cmc) / seps); const double sharpym = g4 * ((1.0 - t * t) * rn_02);
-  t = tanh(0.25 * (phi_cpc + phi_ccc) / seps); const double sharpyp = g4 * ((1.0 - t * t) * rn_12);
-  t = tanh(0.25 * (phi_ccc + phi_ccm) / seps); const double sharpzm = g4 * ((1.0 - t * t) * rn_03);
-  t = tanh(0.25 * (phi_ccp + phi_ccc) / seps); const double sharpzp = g4 * ((1.0 - t * t) * rn_13);
+  t = fzm::tanh(0.25 * (phi_ccc + phi_mcc) / seps); const double sharpxm = g4 * ((1.0 - t * t) * rn_01);
+  t = fzm::tanh(0.25 * (phi_pcc + phi_ccc) / seps); const double sharpxp = g4 * ((1.0 - t * t) * rn_11);
+  t = fzm::tanh(0.25 * (phi_ccc + phi_cmc) / seps); const double sharpym = g4 * ((1.0 - t * t) * rn_02);
+  t = fzm::tanh(0.25 * (phi_cpc + phi_ccc) / seps); const double sharpyp = g4 * ((1.0 - t * t) * rn_12);
+  t = fzm::tanh(0.25 * (phi_ccc + phi_ccm) / seps); const double sharpzm = g4 * ((1.0 - t * t) * rn_03);
+  t = fzm::tanh(0.25 * (phi_ccp + phi_ccc) / seps); const double sharpzp = g4 * ((1.0 - t * t) * rn_13);
   const double rglrxm = dpsidxm - sharpxm, rglrxp = dpsidxp - sharpxp;
   const double rglrym = dpsidym - sharpym, rglryp = dpsidyp - sharpyp;
   const double rglrzm = dpsidzm - sharpzm, rglrzp = dpsidzp - sharpzp;
@@ -132,14 +133,14 @@ __global__ void __launch_bounds__(256) k_acdi_transport(fz_dims d, const double 
 // acdi_cmpt_phi, src/acdi.f90:239-247 (halo-inclusive)
 __global__ void k_acdi_phi(long nh, double seps, const double *__restrict__ psi, double *__restrict__ phi) {
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
-    phi[q] = seps * log((psi[q] + FZ_EPS) / (1.0 - psi[q] + FZ_EPS));
+    phi[q] = seps * fzm::log((psi[q] + FZ_EPS) / (1.0 - psi[q] + FZ_EPS));
 }
 
 // vof_thinc_cmpt_phi, src/vof_thinc_qq.f90:148-156
 __global__ void k_vof_phi(long nh, double beta, const double *__restrict__ psi, double *__restrict__ phi) {
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x) {
     const double pa = fmin(fmax(psi[q], FZ_EPS), 1.0 - FZ_EPS);
-    phi[q] = 0.5 / beta * log(pa / (1.0 - pa));
+    phi[q] = 0.5 / beta * fzm::log(pa / (1.0 - pa));
   }
 }
 
@@ -160,9 +161,9 @@ __global__ void k_vof_flux(fz_dims d, double beta, const double *__restrict__ u,
   if (vel_x > 0.0) { qx = q; xx = 0.5; } else { qx = q + 1; xx = -0.5; }
   if (vel_y > 0.0) { qy = q; yy = 0.5; } else { qy = q + d.sy; yy = -0.5; }
   if (vel_z > 0.0) { qz = q; zz = 0.5; } else { qz = q + d.sz; zz = -0.5; }
-  const double lfx = 1.0 / (1.0 + exp(-2.0 * beta * (normx[qx] * xx + phi[qx])));
-  const double lfy = 1.0 / (1.0 + exp(-2.0 * beta * (normy[qy] * yy + phi[qy])));
-  const double lfz = 1.0 / (1.0 + exp(-2.0 * beta * (normz[qz] * zz + phi[qz])));
+  const double lfx = 1.0 / (1.0 + fzm::exp(-2.0 * beta * (normx[qx] * xx + phi[qx])));
+  const double lfy = 1.0 / (1.0 + fzm::exp(-2.0 * beta * (normy[qy] * yy + phi[qy])));
+  const double lfz = 1.0 / (1.0 + fzm::exp(-2.0 * beta * (normz[qz] * zz + phi[qz])));
   flux_x[q] = vel_x * lfx;
   flux_y[q] = vel_y * lfy;
   flux_z[q] = vel_z * lfz;
@@ -465,14 +466,14 @@ __global__ void k_forcing(fz_dims d, int ftype, double f0, double factor, double
   const double yyc = (double)((float)j - 0.5f) * dl2 / l2;
   const double zzc = zc[k] / l3;
   if (ftype == 1) {  // ABC, a=b=c=1
-    u[q] = u[q] + factor * (f0 * (1.0 * sin(k0 * zzc) + 1.0 * cos(k0 * yyc)));
-    v[q] = v[q] + factor * (f0 * (1.0 * sin(k0 * xxc) + 1.0 * cos(k0 * zzc)));
-    w[q] = w[q] + factor * (f0 * (1.0 * sin(k0 * yyc) + 1.0 * cos(k0 * xxc)));
+    u[q] = u[q] + factor * (f0 * (1.0 * fzm::sin(k0 * zzc) + 1.0 * fzm::cos(k0 * yyc)));
+    v[q] = v[q] + factor * (f0 * (1.0 * fzm::sin(k0 * xxc) + 1.0 * fzm::cos(k0 * zzc)));
+    w[q] = w[q] + factor * (f0 * (1.0 * fzm::sin(k0 * yyc) + 1.0 * fzm::cos(k0 * xxc)));
   } else {
     const double xxf = xxc + 0.5 * dl1 / l1;
     const double yyf = yyc + 0.5 * dl2 / l2;
-    u[q] = u[q] + factor * (f0 * sin(k0 * xxf) * cos(k0 * yyc) * cos(k0 * zzc));
-    v[q] = v[q] - factor * (f0 * cos(k0 * xxc) * sin(k0 * yyf) * cos(k0 * zzc));
+    u[q] = u[q] + factor * (f0 * fzm::sin(k0 * xxf) * fzm::cos(k0 * yyc) * fzm::cos(k0 * zzc));
+    v[q] = v[q] - factor * (f0 * fzm::cos(k0 * xxc) * fzm::sin(k0 * yyf) * fzm::cos(k0 * zzc));
   }
 }
 
@@ -495,7 +496,7 @@ __global__ void k_init_spheres(fz_dims d, const double *__restrict__ zc, double 
         const double sd = sqrt(ax * ax * id1 + ay * ay * id2 + az * az * id3) - rad;
         if (fabs(sd) <= sdistmin) sdistmin = sd;
       }
-  const double ps = .5 * (1.0 + tanh(sdistmin / (2.0 * seps)));
+  const double ps = .5 * (1.0 + fzm::tanh(sdistmin / (2.0 * seps)));
   psi[q] = fmin(psi[q], ps);
 }
 
@@ -503,7 +504,7 @@ __global__ void k_init_film(fz_dims d, const double *__restrict__ zc, double zmi
                             double *__restrict__ psi) {
   CELL_IJK(d);
   (void)r;
-  psi[q] = .5 * (1.0 + tanh((zc[k] - zmid) / (2.0 * seps)));
+  psi[q] = .5 * (1.0 + fzm::tanh((zc[k] - zmid) / (2.0 * seps)));
 }
 
 __global__ void k_fill(long nh, double val, double *__restrict__ p) {
@@ -527,7 +528,7 @@ static inline int flat_grid(long n) {
 extern "C" int fz_fillps(fz_ctx *c, double dti, double rho0, const double *u, const double *v, const double *w,
                          double *p) {
   if (!c || !u || !v || !w || !p) return fz_set_error(FZ_ERR_ARG, "fz_fillps: null argument");
-  hipLaunchKernelGGL(k_fillps, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzfi, dti, dti * c->dli[0],
+  FZ_LAUNCH(c, "fillps", k_fillps, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzfi, dti, dti * c->dli[0],
                      dti * c->dli[1], rho0, u, v, w, p);
   FZ_LAUNCHED();
   return FZ_OK;
@@ -538,7 +539,7 @@ extern "C" int fz_correc(fz_ctx *c, double rho0, const double rho12[2], double d
   (void)rho12;
   (void)psi;
   if (!c || !p || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_correc: null argument");
-  hipLaunchKernelGGL(k_correc, fz_grid_halo(c->d), fz_block3(), 0, c->st, c->d, c->dzci, dt * c->dli[0],
+  FZ_LAUNCH(c, "correc", k_correc, fz_grid_halo(c->d), fz_block3(), 0, c->d, c->dzci, dt * c->dli[0],
                      dt * c->dli[1], dt, rho0, p, u, v, w);
   FZ_LAUNCHED();
   return FZ_OK;
@@ -546,7 +547,7 @@ extern "C" int fz_correc(fz_ctx *c, double rho0, const double rho12[2], double d
 
 extern "C" int fz_updatep(fz_ctx *c, const double *pp, double *p) {
   if (!c || !pp || !p) return fz_set_error(FZ_ERR_ARG, "fz_updatep: null argument");
-  hipLaunchKernelGGL(k_updatep, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, pp, p);
+  FZ_LAUNCH(c, "updatep", k_updatep, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, pp, p);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -562,7 +563,7 @@ extern "C" int fz_updt_rhs_b(fz_ctx *c, double *p) {
     if (dir > 0 && c->per[dir]) continue;
     for (int ib = 0; ib < 2; ib++) {
       const double *rhs = rb[dir] + (long)na[dir] * nb[dir] * ib;
-      hipLaunchKernelGGL(k_add_face, dim3((na[dir] + 63) / 64, nb[dir]), dim3(64), 0, c->st, d, dir,
+      FZ_LAUNCH(c, "add_face", k_add_face, dim3((na[dir] + 63) / 64, nb[dir]), dim3(64), 0, d, dir,
                          ib == 0 ? 1 : nn[dir], rhs, p);
       FZ_LAUNCHED();
     }
@@ -574,7 +575,7 @@ int fz_acdi_transport_dev(fz_ctx *c, const double *d_gam, double gam, double sep
                           const double *w, const double *normx, const double *normy, const double *normz,
                           const double *phi, const double *psi, double *dpsidt, double *flux_x, double *flux_y,
                           double *flux_z) {
-  hipLaunchKernelGGL(k_acdi_transport, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzci, c->dzfi,
+  FZ_LAUNCH(c, "acdi_transport", k_acdi_transport, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi,
                      c->dli[0], c->dli[1], d_gam, gam, seps, u, v, w, normx, normy, normz, phi, psi, dpsidt, flux_x,
                      flux_y, flux_z);
   FZ_LAUNCHED();
@@ -593,14 +594,14 @@ extern "C" int fz_acdi_transport_pf(fz_ctx *c, double gam, double seps, const do
 
 extern "C" int fz_acdi_cmpt_phi(fz_ctx *c, double seps, const double *psi, double *phi) {
   if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_acdi_cmpt_phi: null argument");
-  hipLaunchKernelGGL(k_acdi_phi, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, seps, psi, phi);
+  FZ_LAUNCH(c, "acdi_phi", k_acdi_phi, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 
 extern "C" int fz_vof_thinc_cmpt_phi(fz_ctx *c, double beta, const double *psi, double *phi) {
   if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_vof_thinc_cmpt_phi: null argument");
-  hipLaunchKernelGGL(k_vof_phi, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, beta, psi, phi);
+  FZ_LAUNCH(c, "vof_phi", k_vof_phi, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, beta, psi, phi);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -614,10 +615,10 @@ extern "C" int fz_vof_thinc_transport_psi(fz_ctx *c, double beta, const double *
     return fz_set_error(FZ_ERR_ARG, "fz_vof_thinc_transport_psi: null argument");
   const fz_dims &d = c->d;
   dim3 g((d.n1 + 1 + 63) / 64, (d.n2 + 1 + 3) / 4, d.n3 + 1);
-  hipLaunchKernelGGL(k_vof_flux, g, fz_block3(), 0, c->st, d, beta, u, v, w, normx, normy, normz, phi, flux_x, flux_y,
+  FZ_LAUNCH(c, "vof_flux", k_vof_flux, g, fz_block3(), 0, d, beta, u, v, w, normx, normy, normz, phi, flux_x, flux_y,
                      flux_z);
   FZ_LAUNCHED();
-  hipLaunchKernelGGL(k_vof_div, fz_grid_interior(d), fz_block3(), 0, c->st, d, c->dzfi, c->dli[0], c->dli[1], flux_x,
+  FZ_LAUNCH(c, "vof_div", k_vof_div, fz_grid_interior(d), fz_block3(), 0, d, c->dzfi, c->dli[0], c->dli[1], flux_x,
                      flux_y, flux_z, dpsidt);
   FZ_LAUNCHED();
   return FZ_OK;
@@ -625,7 +626,7 @@ extern "C" int fz_vof_thinc_transport_psi(fz_ctx *c, double beta, const double *
 
 extern "C" int fz_clip_field(fz_ctx *c, const double minmax[2], double *p) {
   if (!c || !minmax || !p) return fz_set_error(FZ_ERR_ARG, "fz_clip_field: null argument");
-  hipLaunchKernelGGL(k_clip, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, minmax[0], minmax[1], p);
+  FZ_LAUNCH(c, "clip", k_clip, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, minmax[0], minmax[1], p);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -634,7 +635,7 @@ extern "C" int fz_cmpt_norm_curv_youngs(fz_ctx *c, const double *psi, double *no
                                         double *kappa) {
   if (!c || !psi || !normx || !normy || !normz || !kappa)
     return fz_set_error(FZ_ERR_ARG, "fz_cmpt_norm_curv_youngs: null argument");
-  hipLaunchKernelGGL(k_youngs, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzci, c->dzfi, c->dli[0],
+  FZ_LAUNCH(c, "youngs", k_youngs, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
                      c->dli[1], psi, normx, normy, normz, kappa);
   FZ_LAUNCHED();
   return FZ_OK;
@@ -646,7 +647,7 @@ extern "C" int fz_mom_xyz_ad(fz_ctx *c, const double rho12[2], const double mu12
   if (!c || !rho12 || !mu12 || !psiflx_x || !psiflx_y || !psiflx_z || !u || !v || !w || !psi || !dudt || !dvdt ||
       !dwdt)
     return fz_set_error(FZ_ERR_ARG, "fz_mom_xyz_ad: null argument");
-  hipLaunchKernelGGL(k_mom_ad, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzci, c->dzfi, c->dli[0],
+  FZ_LAUNCH(c, "mom_ad", k_mom_ad, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
                      c->dli[1], rho12[1], rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], psiflx_x, psiflx_y,
                      psiflx_z, u, v, w, psi, dudt, dvdt, dwdt);
   FZ_LAUNCHED();
@@ -661,7 +662,7 @@ extern "C" int fz_mom_xyz_oth(fz_ctx *c, double dt_r, const double rho12[2], con
   if (!c || !rho12 || !bforce || !gacc || !rho_av || !p || !psi || !kappa || !pn || !po || !dudt || !dvdt || !dwdt)
     return fz_set_error(FZ_ERR_ARG, "fz_mom_xyz_oth: null argument");
   const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
-  hipLaunchKernelGGL(k_mom_oth, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, c->dzci, c->dli[0], c->dli[1],
+  FZ_LAUNCH(c, "mom_oth", k_mom_oth, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dli[0], c->dli[1],
                      dt_r, rho12[1], rho12[0] - rho12[1], surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0],
                      gacc[1], gacc[2], rho_av[0], rho_av[1], rho_av[2], p, psi, kappa, pn, po, dudt, dvdt, dwdt);
   FZ_LAUNCHED();
@@ -671,7 +672,7 @@ extern "C" int fz_mom_xyz_oth(fz_ctx *c, double dt_r, const double rho12[2], con
 extern "C" int fz_lscale_forcing(fz_ctx *c, int ftype, double alpha, double dt, double *u, double *v, double *w) {
   if (!c || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_lscale_forcing: null argument");
   const double f0 = alpha * ((2.0 * FZ_PI / c->l[2]) * (2.0 * FZ_PI / c->l[2]));
-  hipLaunchKernelGGL(k_forcing, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, ftype, f0, dt, c->l[0], c->l[1],
+  FZ_LAUNCH(c, "forcing", k_forcing, fz_grid_interior(c->d), fz_block3(), 0, c->d, ftype, f0, dt, c->l[0], c->l[1],
                      c->l[2], c->dl[0], c->dl[1], c->zc, u, v, w);
   FZ_LAUNCHED();
   return FZ_OK;
@@ -679,27 +680,27 @@ extern "C" int fz_lscale_forcing(fz_ctx *c, int ftype, double alpha, double dt, 
 
 // rk_2fl / rk AB2 kernels for the case driver and fz_rk / fz_rk_2fl
 int fz_launch_rk_psi(fz_ctx *c, double f1, double f2, const double *dpsidt, const double *dpsidto, double *psi) {
-  hipLaunchKernelGGL(k_rk_psi_ab2, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, f1, f2, dpsidt, dpsidto, psi);
+  FZ_LAUNCH(c, "rk_psi_ab2", k_rk_psi_ab2, fz_grid_interior(c->d), fz_block3(), 0, c->d, f1, f2, dpsidt, dpsidto, psi);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 int fz_launch_rk_mom(fz_ctx *c, double f1, double f2, const double rho12[2], const double *psio, const double *psi,
                      const double *du, const double *dv, const double *dw, const double *duo, const double *dvo,
                      const double *dwo, double *u, double *v, double *w) {
-  hipLaunchKernelGGL(k_rk_mom_ab2, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, f1, f2, rho12[1],
+  FZ_LAUNCH(c, "rk_mom_ab2", k_rk_mom_ab2, fz_grid_interior(c->d), fz_block3(), 0, c->d, f1, f2, rho12[1],
                      rho12[0] - rho12[1], psio, psi, du, dv, dw, duo, dvo, dwo, u, v, w);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 int fz_launch_rk_oth(fz_ctx *c, double f12, const double rho12[2], const double *psi, const double *du,
                      const double *dv, const double *dw, double *u, double *v, double *w) {
-  hipLaunchKernelGGL(k_rk_mom_oth, fz_grid_interior(c->d), fz_block3(), 0, c->st, c->d, f12, rho12[1],
+  FZ_LAUNCH(c, "rk_mom_oth", k_rk_mom_oth, fz_grid_interior(c->d), fz_block3(), 0, c->d, f12, rho12[1],
                      rho12[0] - rho12[1], psi, du, dv, dw, u, v, w);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 int fz_launch_fill(fz_ctx *c, long n, double val, double *p) {
-  hipLaunchKernelGGL(k_fill, dim3(flat_grid(n)), dim3(256), 0, c->st, n, val, p);
+  FZ_LAUNCH(c, "fill", k_fill, dim3(flat_grid(n)), dim3(256), 0, n, val, p);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -719,7 +720,7 @@ extern "C" int fz_init2fl(fz_ctx *c, const fz_params *prm, double seps, double *
     case 6: is_sphere = 1; is_swap = 1; is_dim[1] = 0; break;
     case 7: is_sphere = 1; is_swap = 1; is_dim[0] = 0; is_dim[1] = 0; break;
     case 8:
-      hipLaunchKernelGGL(k_init_film, fz_grid_interior(d), fz_block3(), 0, c->st, d, c->zc, c->l[2] / 2.0, seps, psi);
+      FZ_LAUNCH(c, "init_film", k_init_film, fz_grid_interior(d), fz_block3(), 0, d, c->zc, c->l[2] / 2.0, seps, psi);
       FZ_LAUNCHED();
       break;
     default: return fz_set_error(FZ_ERR_ARG, "fz_init2fl: unknown inipsi %d", prm->inipsi);
@@ -746,14 +747,14 @@ extern "C" int fz_init2fl(fz_ctx *c, const fz_params *prm, double seps, double *
     for (int q = 0; q < 3; q++)
       if (c->l[q] * is_dim[q] > lmax) lmax = c->l[q] * is_dim[q];
     for (int s = 0; s < nsph; s++) {
-      hipLaunchKernelGGL(k_init_spheres, fz_grid_interior(d), fz_block3(), 0, c->st, d, c->zc, c->dl[0], c->dl[1],
+      FZ_LAUNCH(c, "init_spheres", k_init_spheres, fz_grid_interior(d), fz_block3(), 0, d, c->zc, c->dl[0], c->dl[1],
                          c->l[0], c->l[1], c->l[2], ip[0], ip[1], ip[2], is_dim[0], is_dim[1], is_dim[2], lmax * 2.0,
                          sph[4 * s], sph[4 * s + 1], sph[4 * s + 2], sph[4 * s + 3], seps, psi);
       FZ_LAUNCHED();
     }
   }
   if (is_swap) {
-    hipLaunchKernelGGL(k_one_minus, dim3(flat_grid(c->nh)), dim3(256), 0, c->st, c->nh, psi);
+    FZ_LAUNCH(c, "one_minus", k_one_minus, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, psi);
     FZ_LAUNCHED();
   }
   return FZ_OK;

@@ -45,9 +45,12 @@ SIGNATURES = {
     "fz_upload": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "fz_download": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "fz_copy": (C.c_int, [vp, vp, vp, C.c_size_t]),
+    "fz_prof_enable": (C.c_int, [vp, C.c_int]),
+    "fz_prof_report": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
     "fz_initgrid": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp]),
     "fz_initsolver": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp]),
     "fz_solver": (C.c_int, [vp, vp]),
+    "fz_r2r": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
     "fz_fillps": (C.c_int, [vp, C.c_double, C.c_double, vp, vp, vp, vp]),
     "fz_updt_rhs_b": (C.c_int, [vp, vp]),
     "fz_correc": (C.c_int, [vp, C.c_double, dp, C.c_double, vp, vp, vp, vp, vp]),
@@ -55,11 +58,11 @@ SIGNATURES = {
     "fz_chk_poisson": (C.c_int, [vp, vp, vp, dp]),
     "fz_boundp": (C.c_int, [vp, C.c_char_p, dp, vp]),
     "fz_bounduvw": (C.c_int, [vp, C.c_char_p, dp, C.c_int, vp, vp, vp]),
-    "fz_acdi_transport_pf": (C.c_int, [vp, C.c_double, C.c_double] + [vp] * 14),
+    "fz_acdi_transport_pf": (C.c_int, [vp, C.c_double, C.c_double] + [vp] * 12),
     "fz_acdi_cmpt_phi": (C.c_int, [vp, C.c_double, vp, vp]),
     "fz_acdi_set_gamma": (C.c_int, [vp, C.c_double, vp, vp, vp, dp]),
     "fz_acdi_set_epsilon": (C.c_int, [vp, C.c_double, dp]),
-    "fz_vof_thinc_transport_psi": (C.c_int, [vp, C.c_double] + [vp] * 14),
+    "fz_vof_thinc_transport_psi": (C.c_int, [vp, C.c_double] + [vp] * 12),
     "fz_vof_thinc_cmpt_phi": (C.c_int, [vp, C.c_double, vp, vp]),
     "fz_clip_field": (C.c_int, [vp, dp, vp]),
     "fz_cmpt_norm_curv_youngs": (C.c_int, [vp, vp, vp, vp, vp, vp]),

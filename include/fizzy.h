@@ -86,6 +86,11 @@ int fz_free(fz_ctx *ctx, void *dptr);
 int fz_upload(fz_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 int fz_download(fz_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
 int fz_copy(fz_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
+/* per-launch timing with HIP events on the context stream (no reference
+ * counterpart; the reference times whole steps under _TIMING, src/main.f90:415-418) */
+int fz_prof_enable(fz_ctx *ctx, int on);
+/* one line per launch site, "name calls total_ms"; clears the records */
+int fz_prof_report(fz_ctx *ctx, char *buf, size_t cap);
 
 /* initgrid, src/initgrid.f90:14-78 (host routine, arrays of length n+2) */
 int fz_initgrid(int gtype, int n, double gr, double lz, double *dzc, double *dzf, double *zc, double *zf);
@@ -98,6 +103,10 @@ int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdaxy, double *
                   double *normfft, double *rhsbx, double *rhsby, double *rhsbz);
 /* solver / solver_gpu, src/solver.f90:19-78, src/solver_gpu.f90:31-163 */
 int fz_solver(fz_ctx *ctx, double *p);
+/* fft, src/fft.f90:176-187: an r2r transform (FFTW kind codes 0 R2HC, 1 HC2R,
+ * 4 REDFT01, 5 REDFT10, 6 REDFT11, 8 RODFT01, 9 RODFT10, 10 RODFT11) over
+ * `howmany` contiguous lines of length n, in place and unnormalised */
+int fz_r2r(fz_ctx *ctx, int kind, int n, int howmany, double *x);
 /* fillps, src/fillps.f90:13-51 */
 int fz_fillps(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v, const double *w, double *p);
 /* updt_rhs_b, src/bound.f90:341-396 (boundary terms built by fz_initsolver) */

@@ -21,6 +21,7 @@
  *    own test (tests/rising_bubble/c1g2l3s.txt), run on the GPU path.
  */
 #include "fz_oracle.h"
+#include "fz_oracle_math.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -87,14 +88,14 @@ static rplan *rplan_new(int kind, int n) {
   p->q = (cplx *)malloc(sizeof(cplx) * (n + 1));
   p->t2 = (cplx *)malloc(sizeof(cplx) * (n + 1));
   for (int j = 0; j < p->m; j++) {
-    p->tw[j].x = cos(2.0 * PI * (double)j / (double)p->m);
-    p->tw[j].y = -sin(2.0 * PI * (double)j / (double)p->m);
+    p->tw[j].x = om_cos(2.0 * PI * (double)j / (double)p->m);
+    p->tw[j].y = -om_sin(2.0 * PI * (double)j / (double)p->m);
   }
   for (int j = 0; j <= n; j++) {
-    p->q[j].x = cos(PI * (double)j / (2.0 * n));
-    p->q[j].y = sin(PI * (double)j / (2.0 * n));
-    p->t2[j].x = cos(PI * (double)(2 * j + 1) / (4.0 * n));
-    p->t2[j].y = sin(PI * (double)(2 * j + 1) / (4.0 * n));
+    p->q[j].x = om_cos(PI * (double)j / (2.0 * n));
+    p->q[j].y = om_sin(PI * (double)j / (2.0 * n));
+    p->t2[j].x = om_cos(PI * (double)(2 * j + 1) / (4.0 * n));
+    p->t2[j].y = om_sin(PI * (double)(2 * j + 1) / (4.0 * n));
   }
   return p;
 }
@@ -770,12 +771,12 @@ void or_acdi_transport_pf(const int n[3], const double dli[3], const double *dzc
         const double rn_13 = nz_zp / (sqrt(nx_zp * nx_zp + ny_zp * ny_zp + nz_zp * nz_zp) + EPS);
         const double g4 = 0.25 * gam;
         double t;
-        t = tanh(0.25 * (phi_ccc + phi_mcc) / seps); const double sharpxm = g4 * ((1.0 - t * t) * rn_01);
-        t = tanh(0.25 * (phi_pcc + phi_ccc) / seps); const double sharpxp = g4 * ((1.0 - t * t) * rn_11);
-        t = tanh(0.25 * (phi_ccc + phi_cmc) / seps); const double sharpym = g4 * ((1.0 - t * t) * rn_02);
-        t = tanh(0.25 * (phi_cpc + phi_ccc) / seps); const double sharpyp = g4 * ((1.0 - t * t) * rn_12);
-        t = tanh(0.25 * (phi_ccc + phi_ccm) / seps); const double sharpzm = g4 * ((1.0 - t * t) * rn_03);
-        t = tanh(0.25 * (phi_ccp + phi_ccc) / seps); const double sharpzp = g4 * ((1.0 - t * t) * rn_13);
+        t = om_tanh(0.25 * (phi_ccc + phi_mcc) / seps); const double sharpxm = g4 * ((1.0 - t * t) * rn_01);
+        t = om_tanh(0.25 * (phi_pcc + phi_ccc) / seps); const double sharpxp = g4 * ((1.0 - t * t) * rn_11);
+        t = om_tanh(0.25 * (phi_ccc + phi_cmc) / seps); const double sharpym = g4 * ((1.0 - t * t) * rn_02);
+        t = om_tanh(0.25 * (phi_cpc + phi_ccc) / seps); const double sharpyp = g4 * ((1.0 - t * t) * rn_12);
+        t = om_tanh(0.25 * (phi_ccc + phi_ccm) / seps); const double sharpzm = g4 * ((1.0 - t * t) * rn_03);
+        t = om_tanh(0.25 * (phi_ccp + phi_ccc) / seps); const double sharpzp = g4 * ((1.0 - t * t) * rn_13);
         const double rglrxm = dpsidxm - sharpxm, rglrxp = dpsidxp - sharpxp;
         const double rglrym = dpsidym - sharpym, rglryp = dpsidyp - sharpyp;
         const double rglrzm = dpsidzm - sharpzm, rglrzp = dpsidzp - sharpzp;
@@ -791,7 +792,7 @@ void or_acdi_cmpt_phi(const int n[3], double seps, const double *psi, double *ph
   /* src/acdi.f90:239-247, halo-inclusive */
   const long N = (long)(n[0] + 2) * (n[1] + 2) * (n[2] + 2);
 #pragma omp parallel for schedule(static)
-  for (long q = 0; q < N; q++) phi[q] = seps * log((psi[q] + EPS) / (1.0 - psi[q] + EPS));
+  for (long q = 0; q < N; q++) phi[q] = seps * om_log((psi[q] + EPS) / (1.0 - psi[q] + EPS));
 }
 
 double or_acdi_set_gamma(const int n[3], double gam_factor, const double *u, const double *v, const double *w) {
@@ -836,9 +837,9 @@ void or_vof_thinc_transport_psi(const int n[3], const double dli[3], const doubl
         if (vel_z > 0.0) { qz = q; zz = 0.5; } else { qz = q + sz; zz = -0.5; }
         const double lnx = normx[qx], lny = normy[qy], lnz = normz[qz];
         const double d_x = phi[qx], d_y = phi[qy], d_z = phi[qz];
-        const double lfx = 1.0 / (1.0 + exp(-2.0 * beta * (lnx * xx + d_x)));
-        const double lfy = 1.0 / (1.0 + exp(-2.0 * beta * (lny * yy + d_y)));
-        const double lfz = 1.0 / (1.0 + exp(-2.0 * beta * (lnz * zz + d_z)));
+        const double lfx = 1.0 / (1.0 + om_exp(-2.0 * beta * (lnx * xx + d_x)));
+        const double lfy = 1.0 / (1.0 + om_exp(-2.0 * beta * (lny * yy + d_y)));
+        const double lfz = 1.0 / (1.0 + om_exp(-2.0 * beta * (lnz * zz + d_z)));
         flux_x[q] = vel_x * lfx;
         flux_y[q] = vel_y * lfy;
         flux_z[q] = vel_z * lfz;
@@ -859,7 +860,7 @@ void or_vof_thinc_cmpt_phi(const int n[3], double beta, const double *psi, doubl
 #pragma omp parallel for schedule(static)
   for (long q = 0; q < N; q++) {
     const double pa = dmin(dmax(psi[q], EPS), 1.0 - EPS);
-    phi[q] = 0.5 / beta * log(pa / (1.0 - pa));
+    phi[q] = 0.5 / beta * om_log(pa / (1.0 - pa));
   }
 }
 
@@ -1189,8 +1190,8 @@ void or_lscale_forcing_tg(const int n[3], double alpha, double dt, const double 
         const double yyf = yyc + 0.5 * dl[1] / l[1];
         const double zzc = zc[k] / l[2];
         (void)zf;
-        u[q] = u[q] + factor * (f0 * sin(k0 * xxf) * cos(k0 * yyc) * cos(k0 * zzc));
-        v[q] = v[q] - factor * (f0 * cos(k0 * xxc) * sin(k0 * yyf) * cos(k0 * zzc));
+        u[q] = u[q] + factor * (f0 * om_sin(k0 * xxf) * om_cos(k0 * yyc) * om_cos(k0 * zzc));
+        v[q] = v[q] - factor * (f0 * om_cos(k0 * xxc) * om_sin(k0 * yyf) * om_cos(k0 * zzc));
       }
 }
 
@@ -1300,7 +1301,7 @@ double or_chkdt(const int n[3], const double dl[3], const double *dzci, const do
 /* ======================================================================
  * Initial phase field, src/two_fluid.f90:247-437 and :503-513
  * ====================================================================== */
-static double smooth_step_tanh(double r, double eps) { return .5 * (1.0 + tanh(r / (2.0 * eps))); }
+static double smooth_step_tanh(double r, double eps) { return .5 * (1.0 + om_tanh(r / (2.0 * eps))); }
 
 void or_init2fl(const or_params *prm, double seps, const double *zc, double *psi) {
   const int *n = prm->ng;
@@ -1593,4 +1594,18 @@ int or_case_step(or_case *c, int nsteps) {
     }
   }
   return kill;
+}
+
+/* evaluation hook for tests/test_oracle_math.py: fn 0 exp,1 expm1,2 log,3 tanh,4 sin,5 cos */
+void or_math_eval(int fn, long n, const double *x, double *y) {
+  for (long q = 0; q < n; q++) {
+    switch (fn) {
+    case 0: y[q] = om_exp(x[q]); break;
+    case 1: y[q] = om_expm1(x[q]); break;
+    case 2: y[q] = om_log(x[q]); break;
+    case 3: y[q] = om_tanh(x[q]); break;
+    case 4: y[q] = om_sin(x[q]); break;
+    default: y[q] = om_cos(x[q]); break;
+    }
+  }
 }

@@ -134,6 +134,8 @@ double or_chkdt(const int n[3], const double dl[3], const double *dzci, const do
 /* --- initial condition --- */
 void or_init2fl(const or_params *prm, double seps, const double *zc, double *psi);
 
+void or_math_eval(int fn, long n, const double *x, double *y);
+
 /* --- a whole case: the step loop of src/main.f90:414-562 --- */
 typedef struct or_case or_case;
 or_case *or_case_create(const or_params *prm);

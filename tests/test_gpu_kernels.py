@@ -114,8 +114,8 @@ def test_youngs_and_clip(pair, rng):
     ref = [O.halo(n) for _ in range(4)]
     L.or_cmpt_norm_curv_youngs(I(n), D(pair.dli), P(pair.dzci), P(pair.dzfi), P(phi), *[P(r) for r in ref])
     c = (slice(1, -1),) * 3
-    for o, r in zip(outs, ref):
-        assert rel_err(o.download()[c], r[c]) <= 1e-15
+    for o, r in zip(outs, ref):  # +,-,*,/ and sqrt only: correctly rounded on both sides
+        assert np.array_equal(o.download()[c], r[c]), rel_err(o.download()[c], r[c])
     x = rand(rng, n, -0.5, 1.5)
     dx = pair.h().upload(x)
     F.call("fz_clip_field", pair.ctx.h, dvec([0.0, 1.0]), dx.ptr)
@@ -227,6 +227,30 @@ def test_boundary_conditions(cbc, rng):
                 assert np.array_equal(d.download(), a)
     finally:
         pr.close()
+
+
+KINDS = {"R2HC": 0, "HC2R": 1, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6, "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_r2r_lines_bitwise(kind, rng):
+    """fz_r2r (the fft routine, src/fft.f90:176-187) against the oracle's r2r, line by line."""
+    ctx = F.Context([4, 4, 4], [1, 1, 1], "PPPPPP")
+    try:
+        for n in [1, 2, 3, 5, 6, 7, 8, 10, 12, 13, 16, 18, 36, 64, 96, 128, 256, 512, 1024]:
+            if KINDS[kind] in (6, 10) and 2 * n > 2048:
+                continue
+            howmany = 7
+            x = rng.standard_normal((howmany, n))
+            d = F.DArray(ctx, (n * howmany,))
+            d.upload(np.ascontiguousarray(x).reshape(-1))
+            F.call("fz_r2r", ctx.h, KINDS[kind], n, howmany, d.ptr)
+            got = d.download().reshape(howmany, n)
+            ref = O.r2r(KINDS[kind], x)
+            d.free()
+            assert np.array_equal(got, ref), (kind, n, rel_err(got, ref))
+    finally:
+        ctx.close()
 
 
 POISSON = [([16, 12, 10], "PPPPNN"), ([64, 64, 32], "PPPPNN"), ([30, 18, 7], "NNPPNN"), ([12, 10, 9], "DDNNDD"),

@@ -91,19 +91,24 @@ def test_per_step_parity(name):
         dc.close()
 
 
-def test_free_run_stays_close():
-    """20 steps run independently from the same initial state."""
-    spec = cases.rising_bubble_3d(ng=(32, 32, 64))
+@pytest.mark.parametrize("name", ["bubble3d_32x32x64", "hysing2d_64x2x128", "vof_droplets_32cubed"])
+def test_free_run_stays_on_the_oracle(name):
+    """25 steps run independently from the same initial state (two dt updates).
+    The device and the oracle share the operation order and the transcendental
+    method, so the trajectories must not separate; the reference's log(psi)
+    amplifies any one-ulp difference in the saturated bulk, so this is the
+    test that would expose a single differently-rounded operation."""
+    spec = CASES[name]()
     dc = F.DeviceCase(spec)
     oc = O.Case(spec)
     try:
         dc.init()
         oc.init_fields()
         load_oracle_from_device(oc, dc)
-        dc.step(20)
-        oc.step(20)
-        errs = compare(oc, dc, ["u", "v", "w", "psi"])
-        print({k: f"{v:.1e}" for k, v in errs.items()})
-        assert max(errs[k] for k in ("u", "v", "w", "psi")) <= 1e-10, errs
+        dc.step(25)
+        oc.step(25)
+        errs = compare(oc, dc)
+        print(name, {k: f"{v:.1e}" for k, v in errs.items()})
+        assert max(errs.values()) <= TOL, errs
     finally:
         dc.close()
