@@ -32,6 +32,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # algorithmic bytes per cell of one launch: distinct fields read + written once, 8 B each
 # (the minimal dataflow of each routine; halo'd-array kernels count halo cells)
 ALG_BYTES = {
+    # fused momentum: reads u,v,w,psi,psio,flux x3,history x3,p,po,kappa (14 fields), writes u,v,w,history x3
+    "mom_fused": 160,
+    # fused phase-field step: reads u,v,w,normals x3,phi,psio,history (9), writes psi,history,flux x3 (5)
+    "acdi_step": 112,
     "fft_x_fwd": 16, "fft_y_fwd": 16, "fft_y_bwd": 16, "fft_x_bwd": 16, "thomas": 16, "thomas_periodic": 16,
     "mom_ad": 80, "acdi_transport": 96, "youngs": 40, "mom_oth": 56, "rk_mom_ab2": 112, "rk_mom_oth": 80,
     "rk_psi_oop": 32, "acdi_phi": 16, "clip": 16, "fillps": 32, "correc": 56, "updatep_oop": 24,
@@ -54,6 +58,31 @@ def prof_report(F, ctx):
         name, calls, ms = line.split()
         rows[name] = (int(calls), float(ms))
     return rows
+
+
+def dc_count(dc):
+    return dc.field_ptr("u")[1]
+
+
+def copy_bandwidth(F, ctx, nbytes, reps=10):
+    """HBM rate of the library's plain copy kernel (k_copy) over nbytes: the
+    achievable streaming reference, and the calibration kernel of the
+    FETCH_SIZE/WRITE_SIZE counter runs."""
+    src, dst = C.c_void_p(), C.c_void_p()
+    F.call("fz_malloc", ctx, nbytes, C.byref(src))
+    F.call("fz_malloc", ctx, nbytes, C.byref(dst))
+    try:
+        F.call("fz_copy", ctx, dst, src, nbytes)
+        F.call("fz_ctx_sync", ctx)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            F.call("fz_copy", ctx, dst, src, nbytes)
+        F.call("fz_ctx_sync", ctx)
+        t1 = time.perf_counter()
+    finally:
+        F.call("fz_free", ctx, src)
+        F.call("fz_free", ctx, dst)
+    return 2 * nbytes * reps / (t1 - t0) / 1e9
 
 
 def cpu_baseline(spec, budget_s=15.0, max_steps=20):
@@ -139,6 +168,8 @@ def main():
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bytes_per_launch,
                 "kernel_time_share": round(ms / total_ms, 4)}
     step_bw = STEP_ALG_BYTES_PER_CELL * cells / (elapsed / args.steps) / 1e9
+    copy_gbs = copy_bandwidth(F, dc.ctx, 8 * dc_count(dc))
+    log(f"plain copy kernel over one field ({8 * dc_count(dc) / 1e6:.0f} MB): {copy_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -165,6 +196,7 @@ def main():
         "step_roofline": {"alg_bytes_per_cell": STEP_ALG_BYTES_PER_CELL, "achieved_GBs": round(step_bw, 1),
                           "frac": round(step_bw / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "hbm_copy_GBs": round(copy_gbs, 1),
     }
     print(json.dumps(out), flush=True)
 

@@ -21,7 +21,17 @@ int fz_launch_rk_mom(fz_ctx *c, double f1, double f2, const double rho12[2], con
                      const double *dwo, double *u, double *v, double *w);
 int fz_launch_rk_oth(fz_ctx *c, double f12, const double rho12[2], const double *psi, const double *du,
                      const double *dv, const double *dw, double *u, double *v, double *w);
+int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], double factor1, double factor2,
+                        int first, double factor12, double dt_r, double sigma, double rho0, const double bforce[3],
+                        const double gacc[3], const double rho_av[3], const double *fx, const double *fy,
+                        const double *fz, const double *u, const double *v, const double *w, const double *psi,
+                        const double *psio, const double *p, const double *po, const double *kappa, double *duo,
+                        double *dvo, double *dwo, double *un, double *vn, double *wn);
 int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
+int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
+                        int first, const double *u, const double *v, const double *w, const double *normx,
+                        const double *normy, const double *normz, const double *phi, const double *psio,
+                        double *dpsio, double *psi_out, double *flux_x, double *flux_y, double *flux_z);
 int fz_chkdiv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
 int fz_chkdt_adv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
 double fz_chkdt_finish(const fz_ctx *c, double dti, int is_solve_ns, int is_track_interface, int vof,
@@ -56,7 +66,8 @@ struct fz_case {
   fz_params prm;
   fz_ctx *ctx = nullptr;
   double *f[F_N] = {nullptr};
-  double *psio = nullptr, *dpsidt = nullptr, *du = nullptr, *dv = nullptr, *dw = nullptr, *dp = nullptr;
+  double *psio = nullptr, *dpsidt = nullptr, *dp = nullptr;
+  double *un = nullptr, *vn = nullptr, *wn = nullptr;  // the next velocity (rotated with u, v, w)
   double *dsc = nullptr;  // device scalars: 0 gam, 1-2 div, 3 adv dt, 4-6 rho_av
   double *gvr_c = nullptr, *gvr_f = nullptr;
   double sc[12] = {0};
@@ -73,13 +84,13 @@ static int case_alloc(fz_case *cs) {
     FZ_HIP(hipMalloc((void **)&cs->f[q], sizeof(double) * n));
     FZ_HIP(hipMemsetAsync(cs->f[q], 0, sizeof(double) * n, c->st));
   }
-  double **hal[2] = {&cs->psio, &cs->dp};
-  for (int q = 0; q < 2; q++) {
+  double **hal[5] = {&cs->psio, &cs->dp, &cs->un, &cs->vn, &cs->wn};
+  for (int q = 0; q < 5; q++) {
     FZ_HIP(hipMalloc((void **)hal[q], sizeof(double) * c->nh));
     FZ_HIP(hipMemsetAsync(*hal[q], 0, sizeof(double) * c->nh, c->st));
   }
-  double **itr[4] = {&cs->dpsidt, &cs->du, &cs->dv, &cs->dw};
-  for (int q = 0; q < 4; q++) {
+  double **itr[1] = {&cs->dpsidt};
+  for (int q = 0; q < 1; q++) {
     FZ_HIP(hipMalloc((void **)itr[q], sizeof(double) * c->ni));
     FZ_HIP(hipMemsetAsync(*itr[q], 0, sizeof(double) * c->ni, c->st));
   }
@@ -104,7 +115,7 @@ extern "C" int fz_case_destroy(fz_case *cs) {
   if (cs->ctx) (void)hipStreamSynchronize(cs->ctx->st);
   for (int q = 0; q < F_N; q++)
     if (cs->f[q]) (void)hipFree(cs->f[q]);
-  double *o[9] = {cs->psio, cs->dpsidt, cs->du, cs->dv, cs->dw, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f};
+  double *o[9] = {cs->psio, cs->dpsidt, cs->un, cs->vn, cs->wn, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f};
   for (int q = 0; q < 9; q++)
     if (o[q]) (void)hipFree(o[q]);
   if (cs->ctx) fz_ctx_destroy(cs->ctx);
@@ -274,21 +285,22 @@ extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
     swap_ptr(cs->psio, f[F_PSI]);
     if (P.is_track_interface) {
       if (!P.vof) {
-        FZ_TRY(fz_acdi_transport_dev(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS],
-                                     f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ], f[F_PHI], cs->psio,
-                                     cs->dpsidt, f[F_FX], f[F_FY], f[F_FZ]));
+        // transport, AB2 update and clip in one pass; the history is updated in place
+        FZ_TRY(fz_launch_acdi_step(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS], factor1,
+                                   factor2, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ],
+                                   f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ]));
       } else {
         FZ_TRY(fz_vof_thinc_transport_psi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_U], f[F_V], f[F_W], f[F_NX],
                                           f[F_NY], f[F_NZ], f[F_PHI], cs->psio, cs->dpsidt, f[F_FX], f[F_FY],
                                           f[F_FZ]));
+        if (is_first) FZ_HIP(hipMemcpyAsync(f[F_DPSIO], cs->dpsidt, bi, hipMemcpyDeviceToDevice, c->st));
+        FZ_LAUNCH(c, "rk_psi_oop", k_rk_psi_oop, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1, factor2,
+                  cs->dpsidt, f[F_DPSIO], cs->psio, f[F_PSI]);
+        FZ_LAUNCHED();
+        const double mm[2] = {0.0, 1.0};
+        FZ_TRY(fz_clip_field(c, mm, f[F_PSI]));  // src/rk.f90:248
+        swap_ptr(cs->dpsidt, f[F_DPSIO]);         // src/rk.f90:252
       }
-      if (is_first) FZ_HIP(hipMemcpyAsync(f[F_DPSIO], cs->dpsidt, bi, hipMemcpyDeviceToDevice, c->st));
-      FZ_LAUNCH(c, "rk_psi_oop", k_rk_psi_oop, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1, factor2,
-                         cs->dpsidt, f[F_DPSIO], cs->psio, f[F_PSI]);
-      FZ_LAUNCHED();
-      const double mm[2] = {0.0, 1.0};
-      FZ_TRY(fz_clip_field(c, mm, f[F_PSI]));  // src/rk.f90:248
-      swap_ptr(cs->dpsidt, f[F_DPSIO]);         // src/rk.f90:252
       FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_FX], f[F_FY], f[F_FZ]));
       FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
       FZ_TRY(phase_geometry(cs));
@@ -306,23 +318,14 @@ extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
           rho_av[d] = c->h_small[8];
         }
       }
-      // rk, src/rk.f90:72-130
-      FZ_TRY(fz_mom_xyz_ad(c, P.rho12, P.mu12, f[F_FX], f[F_FY], f[F_FZ], f[F_U], f[F_V], f[F_W], f[F_PSI], cs->du,
-                           cs->dv, cs->dw));
-      if (is_first) {
-        FZ_HIP(hipMemcpyAsync(f[F_DUO], cs->du, bi, hipMemcpyDeviceToDevice, c->st));
-        FZ_HIP(hipMemcpyAsync(f[F_DVO], cs->dv, bi, hipMemcpyDeviceToDevice, c->st));
-        FZ_HIP(hipMemcpyAsync(f[F_DWO], cs->dw, bi, hipMemcpyDeviceToDevice, c->st));
-      }
-      FZ_TRY(fz_launch_rk_mom(c, factor1, factor2, P.rho12, cs->psio, f[F_PSI], cs->du, cs->dv, cs->dw, f[F_DUO],
-                              f[F_DVO], f[F_DWO], f[F_U], f[F_V], f[F_W]));
-      swap_ptr(cs->du, f[F_DUO]);
-      swap_ptr(cs->dv, f[F_DVO]);
-      swap_ptr(cs->dw, f[F_DWO]);
-      const double zero2[2] = {0.0, 0.0};
-      FZ_TRY(fz_mom_xyz_oth(c, dt_r, P.rho12, zero2, P.bforce, P.gacc, P.sigma, rho0, rho_av, f[F_P], f[F_PSI],
-                            f[F_KAPPA], f[F_P], f[F_PO], cs->du, cs->dv, cs->dw));
-      FZ_TRY(fz_launch_rk_oth(c, factor12, P.rho12, f[F_PSI], cs->du, cs->dv, cs->dw, f[F_U], f[F_V], f[F_W]));
+      // rk, src/rk.f90:72-130, in one pass into the spare velocity buffers
+      FZ_TRY(fz_launch_mom_fused(c, P.rho12, P.mu12, factor1, factor2, is_first ? 1 : 0, factor12, dt_r, P.sigma,
+                                 rho0, P.bforce, P.gacc, rho_av, f[F_FX], f[F_FY], f[F_FZ], f[F_U], f[F_V], f[F_W],
+                                 f[F_PSI], cs->psio, f[F_P], f[F_PO], f[F_KAPPA], f[F_DUO], f[F_DVO], f[F_DWO],
+                                 cs->un, cs->vn, cs->wn));
+      swap_ptr(f[F_U], cs->un);
+      swap_ptr(f[F_V], cs->vn);
+      swap_ptr(f[F_W], cs->wn);
       if (P.is_forced_hit) FZ_TRY(fz_lscale_forcing(c, 2, 0.5, dt, f[F_U], f[F_V], f[F_W]));
       FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
       // projection on the correction buffer dp, src/main.f90:488-502

@@ -96,6 +96,14 @@ struct fz_prof_scope {
     hipLaunchKernelGGL(kernel, grid, block, shmem, (c)->st, __VA_ARGS__);                \
   } while (0)
 
+// Workgroups are dealt round-robin to the 8 XCDs in dispatch order, each XCD
+// with its own L2. Remapping the linear block id so that XCD x runs the
+// contiguous range [x*nb/8, (x+1)*nb/8) keeps the neighbouring rows and
+// planes of a stencil in one L2. Identity when nb is not a multiple of 8.
+__device__ inline unsigned fz_xcd_block(unsigned P, unsigned nb) {
+  return (nb & 7u) ? P : (P & 7u) * (nb >> 3) + (P >> 3);
+}
+
 // ---- launch helpers ----
 static inline dim3 fz_block3() { return dim3(64, 4, 1); }
 static inline dim3 fz_grid_interior(const fz_dims &d) {

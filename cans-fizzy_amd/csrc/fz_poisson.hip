@@ -32,6 +32,17 @@ __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x *
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
+// radix-4 butterfly in place on twiddled inputs; sign -1 forward, +1 inverse
+__device__ inline void bfly4(double2 *v, double sign) {
+  const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+  const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+  const double2 r = make_double2(-sign * d13.y, sign * d13.x);  // i*sign*d13
+  v[0] = cadd(s02, s13);
+  v[1] = cadd(d02, r);
+  v[2] = csub(s02, s13);
+  v[3] = csub(d02, r);
+}
+
 // One Stockham pass of radix R over nfft complex lines of length m (LDS to LDS).
 __device__ inline void stockham_pass(const double2 *__restrict__ X, double2 *__restrict__ Y, int nfft, int m, int R,
                                      int Ns, double sign, const double2 *__restrict__ tw) {
@@ -58,14 +69,8 @@ __device__ inline void stockham_pass(const double2 *__restrict__ X, double2 *__r
       y[base] = cadd(v[0], v[1]);
       y[base + Ns] = csub(v[0], v[1]);
     } else if (R == 4) {
-      const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-      const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-      // i*sign*d13
-      const double2 r = make_double2(-sign * d13.y, sign * d13.x);
-      y[base] = cadd(s02, s13);
-      y[base + Ns] = cadd(d02, r);
-      y[base + 2 * Ns] = csub(s02, s13);
-      y[base + 3 * Ns] = csub(d02, r);
+      bfly4(v, sign);
+      for (int t = 0; t < 4; t++) y[base + t * Ns] = v[t];
     } else if (R == 3) {
       const double c3 = 0.86602540378443864676;  // sqrt(3)/2
       const double2 t1 = cadd(v[1], v[2]);
@@ -88,6 +93,115 @@ __device__ inline void stockham_pass(const double2 *__restrict__ X, double2 *__r
       }
     }
   }
+}
+
+// complex input k of packed line f, from the real lines staged in Rw (n values each)
+__device__ inline double2 pre_value(int kind, const double *__restrict__ Rw, int f, int k, int n,
+                                  const double2 *__restrict__ q) {
+  double2 z;
+  switch (kind) {
+    case K_R2HC: z = make_double2(Rw[(2 * f) * n + k], Rw[(2 * f + 1) * n + k]); break;
+    case K_REDFT10:
+    case K_RODFT10: {
+      const int e = (k <= (n - 1) / 2) ? 2 * k : 2 * (n - 1 - k) + 1;
+      const double sg = (kind == K_RODFT10 && (e & 1)) ? -1.0 : 1.0;
+      z = make_double2(sg * Rw[(2 * f) * n + e], sg * Rw[(2 * f + 1) * n + e]);
+      break;
+    }
+    case K_REDFT11:
+    case K_RODFT11: {
+      if (k < n) {
+        const double sg = (kind == K_RODFT11 && (k & 1)) ? -1.0 : 1.0;
+        const double x = sg * Rw[f * n + k];
+        z = make_double2(x * q[k].x, -x * q[k].y);
+      } else {
+        z = make_double2(0.0, 0.0);
+      }
+      break;
+    }
+    case K_HC2R: {
+      const double *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
+      double ra, ia, rb, ib;
+      if (k == 0) { ra = Ha[0]; ia = 0.0; rb = Hb[0]; ib = 0.0; }
+      else if (2 * k < n) { ra = Ha[k]; ia = Ha[n - k]; rb = Hb[k]; ib = Hb[n - k]; }
+      else if (2 * k == n) { ra = Ha[k]; ia = 0.0; rb = Hb[k]; ib = 0.0; }
+      else { ra = Ha[n - k]; ia = -Ha[k]; rb = Hb[n - k]; ib = -Hb[k]; }
+      z = make_double2(ra - ib, ia + rb);
+      break;
+    }
+    case K_REDFT01:
+    case K_RODFT01: {
+      const double *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
+      double xa, xna, xb, xnb;
+      if (kind == K_REDFT01) {
+        xa = Ha[k]; xb = Hb[k];
+        xna = (k == 0) ? 0.0 : Ha[n - k];
+        xnb = (k == 0) ? 0.0 : Hb[n - k];
+      } else {  // input reversed
+        xa = Ha[n - 1 - k]; xb = Hb[n - 1 - k];
+        xna = (k == 0) ? 0.0 : Ha[k - 1];
+        xnb = (k == 0) ? 0.0 : Hb[k - 1];
+      }
+      const double cs = q[k].x, sn = q[k].y;
+      const double rca = xa * cs + xna * sn, ica = xa * sn - xna * cs;
+      const double rcb = xb * cs + xnb * sn, icb = xb * sn - xnb * cs;
+      z = make_double2(rca - icb, ica + rcb);
+      break;
+    }
+    default: z = make_double2(0.0, 0.0);
+  }
+  return z;
+}
+
+// real output k of line g from the transformed packed lines in cur (length m each)
+__device__ inline double post_value(int kind, const double2 *__restrict__ cur, int g, int k, int n, int m,
+                                  const double2 *__restrict__ q, const double2 *__restrict__ t2) {
+  double val;
+  switch (kind) {
+    case K_R2HC: {
+      const double2 *Z = cur + (long)(g >> 1) * m;
+      const int kk = (2 * k <= n) ? k : n - k;
+      const double2 zk = Z[kk], znk = Z[(m - kk) % m];
+      double re, im;
+      if ((g & 1) == 0) { re = 0.5 * (zk.x + znk.x); im = 0.5 * (zk.y - znk.y); }
+      else { re = 0.5 * (zk.y + znk.y); im = 0.5 * (znk.x - zk.x); }
+      val = (2 * k <= n) ? re : im;
+      break;
+    }
+    case K_REDFT10:
+    case K_RODFT10: {
+      const double2 *Z = cur + (long)(g >> 1) * m;
+      const int kk = (kind == K_REDFT10) ? k : n - 1 - k;
+      const double2 zk = Z[kk], znk = Z[(m - kk) % m];
+      double re, im;
+      if ((g & 1) == 0) { re = 0.5 * (zk.x + znk.x); im = 0.5 * (zk.y - znk.y); }
+      else { re = 0.5 * (zk.y + znk.y); im = 0.5 * (znk.x - zk.x); }
+      val = 2.0 * (q[kk].x * re + q[kk].y * im);
+      break;
+    }
+    case K_REDFT11:
+    case K_RODFT11: {
+      const int kk = (kind == K_REDFT11) ? k : n - 1 - k;
+      const double2 zk = cur[(long)g * m + kk];
+      val = 2.0 * (t2[kk].x * zk.x + t2[kk].y * zk.y);
+      break;
+    }
+    case K_HC2R: {
+      const double2 zk = cur[(long)(g >> 1) * m + k];
+      val = (g & 1) ? zk.y : zk.x;
+      break;
+    }
+    case K_REDFT01:
+    case K_RODFT01: {
+      const int srcm = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
+      const double2 zk = cur[(long)(g >> 1) * m + srcm];
+      val = (g & 1) ? zk.y : zk.x;
+      if (kind == K_RODFT01 && (k & 1)) val = -val;
+      break;
+    }
+    default: val = 0.0;
+  }
+  return val;
 }
 
 // Batched r2r line transform: G lines per block, all in LDS.
@@ -117,59 +231,7 @@ __global__ void __launch_bounds__(256) k_r2r(const double *__restrict__ src, Lin
   const int totc = nfft * m;
   for (int idx = threadIdx.x; idx < totc; idx += blockDim.x) {
     const int f = idx / m, k = idx - f * m;
-    double2 z;
-    switch (kind) {
-      case K_R2HC: z = make_double2(Rw[(2 * f) * n + k], Rw[(2 * f + 1) * n + k]); break;
-      case K_REDFT10:
-      case K_RODFT10: {
-        const int e = (k <= (n - 1) / 2) ? 2 * k : 2 * (n - 1 - k) + 1;
-        const double sg = (kind == K_RODFT10 && (e & 1)) ? -1.0 : 1.0;
-        z = make_double2(sg * Rw[(2 * f) * n + e], sg * Rw[(2 * f + 1) * n + e]);
-        break;
-      }
-      case K_REDFT11:
-      case K_RODFT11: {
-        if (k < n) {
-          const double sg = (kind == K_RODFT11 && (k & 1)) ? -1.0 : 1.0;
-          const double x = sg * Rw[f * n + k];
-          z = make_double2(x * q[k].x, -x * q[k].y);
-        } else {
-          z = make_double2(0.0, 0.0);
-        }
-        break;
-      }
-      case K_HC2R: {
-        const double *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
-        double ra, ia, rb, ib;
-        if (k == 0) { ra = Ha[0]; ia = 0.0; rb = Hb[0]; ib = 0.0; }
-        else if (2 * k < n) { ra = Ha[k]; ia = Ha[n - k]; rb = Hb[k]; ib = Hb[n - k]; }
-        else if (2 * k == n) { ra = Ha[k]; ia = 0.0; rb = Hb[k]; ib = 0.0; }
-        else { ra = Ha[n - k]; ia = -Ha[k]; rb = Hb[n - k]; ib = -Hb[k]; }
-        z = make_double2(ra - ib, ia + rb);
-        break;
-      }
-      case K_REDFT01:
-      case K_RODFT01: {
-        const double *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
-        double xa, xna, xb, xnb;
-        if (kind == K_REDFT01) {
-          xa = Ha[k]; xb = Hb[k];
-          xna = (k == 0) ? 0.0 : Ha[n - k];
-          xnb = (k == 0) ? 0.0 : Hb[n - k];
-        } else {  // input reversed
-          xa = Ha[n - 1 - k]; xb = Hb[n - 1 - k];
-          xna = (k == 0) ? 0.0 : Ha[k - 1];
-          xnb = (k == 0) ? 0.0 : Hb[k - 1];
-        }
-        const double cs = q[k].x, sn = q[k].y;
-        const double rca = xa * cs + xna * sn, ica = xa * sn - xna * cs;
-        const double rcb = xb * cs + xnb * sn, icb = xb * sn - xnb * cs;
-        z = make_double2(rca - icb, ica + rcb);
-        break;
-      }
-      default: z = make_double2(0.0, 0.0);
-    }
-    A[idx] = z;
+    A[idx] = pre_value(kind, Rw, f, k, n, q);
   }
   __syncthreads();
   // 3. complex FFT passes
@@ -187,52 +249,7 @@ __global__ void __launch_bounds__(256) k_r2r(const double *__restrict__ src, Lin
   double *O = (double *)oth;
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int g = idx / n, k = idx - g * n;
-    double val;
-    switch (kind) {
-      case K_R2HC: {
-        const double2 *Z = cur + (long)(g >> 1) * m;
-        const int kk = (2 * k <= n) ? k : n - k;
-        const double2 zk = Z[kk], znk = Z[(m - kk) % m];
-        double re, im;
-        if ((g & 1) == 0) { re = 0.5 * (zk.x + znk.x); im = 0.5 * (zk.y - znk.y); }
-        else { re = 0.5 * (zk.y + znk.y); im = 0.5 * (znk.x - zk.x); }
-        val = (2 * k <= n) ? re : im;
-        break;
-      }
-      case K_REDFT10:
-      case K_RODFT10: {
-        const double2 *Z = cur + (long)(g >> 1) * m;
-        const int kk = (kind == K_REDFT10) ? k : n - 1 - k;
-        const double2 zk = Z[kk], znk = Z[(m - kk) % m];
-        double re, im;
-        if ((g & 1) == 0) { re = 0.5 * (zk.x + znk.x); im = 0.5 * (zk.y - znk.y); }
-        else { re = 0.5 * (zk.y + znk.y); im = 0.5 * (znk.x - zk.x); }
-        val = 2.0 * (q[kk].x * re + q[kk].y * im);
-        break;
-      }
-      case K_REDFT11:
-      case K_RODFT11: {
-        const int kk = (kind == K_REDFT11) ? k : n - 1 - k;
-        const double2 zk = cur[(long)g * m + kk];
-        val = 2.0 * (t2[kk].x * zk.x + t2[kk].y * zk.y);
-        break;
-      }
-      case K_HC2R: {
-        const double2 zk = cur[(long)(g >> 1) * m + k];
-        val = (g & 1) ? zk.y : zk.x;
-        break;
-      }
-      case K_REDFT01:
-      case K_RODFT01: {
-        const int srcm = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
-        const double2 zk = cur[(long)(g >> 1) * m + srcm];
-        val = (g & 1) ? zk.y : zk.x;
-        if (kind == K_RODFT01 && (k & 1)) val = -val;
-        break;
-      }
-      default: val = 0.0;
-    }
-    O[idx] = val;
+    O[idx] = post_value(kind, cur, g, k, n, m, q, t2);
   }
   __syncthreads();
   // 5. store
@@ -241,6 +258,154 @@ __global__ void __launch_bounds__(256) k_r2r(const double *__restrict__ src, Lin
     if (out.se == 1) { g = idx / n; e = idx - g * n; }
     else { e = idx / G; g = idx - e * G; }
     if (g < gcount) dst[out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se] = O[g * n + e] * scale;
+  }
+}
+
+// Power-of-two line transforms (complex length M = 2^p, 16 <= M <= 2048).
+// The same Stockham factorisation as k_r2r (radix 4 passes, then one radix 2
+// pass when p is odd) with identical butterfly and twiddle arithmetic, so the
+// results are bitwise those of k_r2r; here each thread keeps its four points
+// in registers and the passes exchange through one LDS buffer, with all
+// index arithmetic resolved at compile time. G real lines per block, M/4
+// threads per complex line.
+template <int M, bool QUARTER>
+struct P2Shape {
+  static constexpr int LG = __builtin_ctz(M);
+  static constexpr int NP4 = LG / 2;           // radix-4 passes
+  static constexpr bool HAS2 = (LG & 1) != 0;  // closing radix-2 pass
+  static constexpr int N = QUARTER ? M / 2 : M;  // real line length
+  static constexpr int G = QUARTER ? (2048 / M < 64 ? 2048 / M : 64) : (4096 / M < 64 ? (4096 / M > 2 ? 4096 / M : 2) : 64);
+  static constexpr int NFFT = QUARTER ? G : G / 2;
+  static constexpr int NT = NFFT * (M / 4);
+  static constexpr size_t LDS = (size_t)NFFT * M * sizeof(double2);
+};
+
+template <int M, bool QUARTER>
+__global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, LineMap in, double *__restrict__ dst,
+                                                LineMap out, int L1, int kind, double scale,
+                                                const double2 *__restrict__ tw, const double2 *__restrict__ q,
+                                                const double2 *__restrict__ t2) {
+  using S = P2Shape<M, QUARTER>;
+  constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4;
+  extern __shared__ double2 smem[];
+  double *Rw = (double *)smem;  // G*N staged reals, overlaid by the complex lines
+  const int tid = threadIdx.x;
+  const int l1_0 = blockIdx.x * G;
+  const long l2 = blockIdx.y;
+  const int gcount = min(G, L1 - l1_0);
+  // 1. coalesced load of the G lines
+#pragma unroll
+  for (int r = 0; r < G * N / NT; r++) {
+    const int idx = tid + r * NT;
+    int g, e;
+    if (in.se == 1) { g = idx / N; e = idx - g * N; }
+    else { e = idx / G; g = idx - e * G; }
+    Rw[g * N + e] = (g < gcount) ? src[in.base + (long)(l1_0 + g) * in.s1 + l2 * in.s2 + (long)e * in.se] : 0.0;
+  }
+  __syncthreads();
+  const int f = tid / Q4, b = tid - f * Q4;
+  double2 v[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) v[t] = pre_value(kind, Rw, f, b + t * Q4, N, q);
+  __syncthreads();
+  const double sign = kind_is_forward(kind) ? -1.0 : 1.0;
+  double2 *Z = smem + (long)f * M;
+  // 2. radix-4 passes
+#pragma unroll
+  for (int s = 0; s < S::NP4; s++) {
+    const int Ns = 1 << (2 * s);
+    if (s > 0) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) v[t] = Z[b + t * Q4];
+      __syncthreads();
+    }
+    const int k = b & (Ns - 1);
+    const int base = (b - k) * 4 + k;
+    const int tstride = M / (Ns * 4);
+    if (k > 0) {
+#pragma unroll
+      for (int t = 1; t < 4; t++) {
+        double2 w = tw[t * k * tstride];
+        if (sign > 0) w.y = -w.y;
+        v[t] = cmul(v[t], w);
+      }
+    }
+    bfly4(v, sign);
+#pragma unroll
+    for (int t = 0; t < 4; t++) Z[base + t * Ns] = v[t];
+    __syncthreads();
+  }
+  // 3. closing radix-2 pass (Ns = M/2): butterflies b and b + M/4
+  if (S::HAS2) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) v[t] = Z[b + t * Q4];
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int k = b + h * Q4;
+      double2 x0 = v[h], x1 = v[h + 2];
+      if (k > 0) {
+        double2 w = tw[k];
+        if (sign > 0) w.y = -w.y;
+        x1 = cmul(x1, w);
+      }
+      Z[k] = cadd(x0, x1);
+      Z[k + M / 2] = csub(x0, x1);
+    }
+    __syncthreads();
+  }
+  // 4. post-processing and store
+  constexpr int PER = G * N / NT;
+  double val[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int idx = tid + r * NT;
+    const int g = idx / N, k = idx - g * N;
+    val[r] = post_value(kind, smem, g, k, N, M, q, t2);
+  }
+  if (out.se == 1) {
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int idx = tid + r * NT;
+      const int g = idx / N, e = idx - g * N;
+      if (g < gcount) dst[out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + e] = val[r] * scale;
+    }
+    return;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; r++) Rw[tid + r * NT] = val[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int idx = tid + r * NT;
+    const int e = idx / G, g = idx - e * G;
+    if (g < gcount) dst[out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se] = Rw[g * N + e] * scale;
+  }
+}
+
+template <int M, bool QUARTER>
+static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
+                      double *dst, LineMap out, int L1, int L2, double scale) {
+  using S = P2Shape<M, QUARTER>;
+  dim3 grid((L1 + S::G - 1) / S::G, L2, 1);
+  FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind, scale, P.tw,
+            P.q, P.t2);
+}
+
+template <bool QUARTER>
+static bool dispatch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
+                        double *dst, LineMap out, int L1, int L2, double scale) {
+  switch (P.m) {
+    case 16: launch_p2<16, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    case 32: launch_p2<32, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    case 64: launch_p2<64, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    case 128: launch_p2<128, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    case 256: launch_p2<256, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    case 512: launch_p2<512, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    case 1024: launch_p2<1024, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    case 2048: launch_p2<2048, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    default: return false;
   }
 }
 
@@ -270,13 +435,61 @@ __device__ inline void thomas_col(double *__restrict__ col, double *__restrict__
   }
 }
 
-__global__ void k_thomas(double *__restrict__ W, double *__restrict__ D, const double *__restrict__ lambdaxy,
-                         const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ c,
-                         int n1, int n2, int n3) {
+// The same recurrence as thomas_col, in chunks of TK rows: each chunk's
+// loads are issued together so one wave keeps TK loads in flight (one
+// column per lane leaves few waves per CU to hide latency otherwise).
+template <int TK>
+__global__ void __launch_bounds__(64) k_thomas(double *__restrict__ W, double *__restrict__ D,
+                                               const double *__restrict__ lambdaxy, const double *__restrict__ a,
+                                               const double *__restrict__ b, const double *__restrict__ c, int n1,
+                                               int n2, int n3) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)n1 * n2) return;
   const long ps = (long)n1 * n2;
-  thomas_col(W + idx, D + idx, ps, n3, a, b, c, lambdaxy[idx]);
+  double *col = W + idx, *dcol = D + idx;
+  const double lam = lambdaxy[idx];
+  double d = 0.0, pprev = 0.0;
+  for (int l0 = 0; l0 < n3; l0 += TK) {
+    double r[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < n3) r[t] = col[(long)(l0 + t) * ps];
+#pragma unroll
+    for (int t = 0; t < TK; t++) {
+      const int l = l0 + t;
+      if (l < n3) {
+        const double bb = b[l] + lam;
+        double z;
+        if (l == 0) {
+          z = 1.0 / (bb + FZ_EPS);
+          d = c[0] * z;
+          pprev = r[t] * z;
+        } else {
+          z = 1.0 / (bb - a[l] * d + FZ_EPS);
+          d = c[l] * z;
+          pprev = (r[t] - a[l] * pprev) * z;
+        }
+        dcol[(long)l * ps] = d;
+        col[(long)l * ps] = pprev;
+      }
+    }
+  }
+  double pnext = pprev;
+  for (int h0 = n3 - 2; h0 >= 0; h0 -= TK) {
+    double r[TK], dd[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        r[t] = col[(long)(h0 - t) * ps];
+        dd[t] = dcol[(long)(h0 - t) * ps];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        pnext = r[t] - dd[t] * pnext;
+        col[(long)(h0 - t) * ps] = pnext;
+      }
+  }
 }
 
 // gaussel_periodic, src/solver.f90:107-149
@@ -494,6 +707,14 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
 static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
                       double *dst, LineMap out, int L1, int L2, double scale) {
   const bool quarter = kind_is_quarter(kind);
+  if (P.m >= 16 && P.m <= 2048 && (P.m & (P.m - 1)) == 0) {
+    const bool done = quarter ? dispatch_p2<true>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale)
+                              : dispatch_p2<false>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale);
+    if (done) {
+      FZ_LAUNCHED();
+      return FZ_OK;
+    }
+  }
   int G;
   size_t lds;
   if (quarter) {
@@ -532,8 +753,8 @@ int fz_solver_dev(fz_ctx *ctx, double *p) {
   const int tpb = 256;
   const int nb = (int)((ps + tpb - 1) / tpb);
   if (!P->periodic_z) {
-    FZ_LAUNCH(ctx, "thomas", k_thomas, dim3(nb), dim3(tpb), 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
-                       n3);
+    FZ_LAUNCH(ctx, "thomas", k_thomas<16>, dim3((int)((ps + 63) / 64)), dim3(64), 0, W, P->piv, P->lambdaxy, P->a,
+              P->b, P->c, n1, n2, n3);
   } else {
     FZ_LAUNCH(ctx, "thomas_periodic", k_thomas_periodic, dim3(nb), dim3(tpb), 0, W, P->piv, P->aux, P->lambdaxy, P->a,
                        P->b, P->c, n1, n2, n3);

@@ -13,18 +13,32 @@
 #include "fz_internal.h"
 #include "fz_math.h"
 
+// block coordinates after the XCD remap (x fastest, then y, then z)
+#define FZ_BLOCK_XYZ(bx, by, bz)                                                        \
+  unsigned bx, by, bz;                                                                  \
+  {                                                                                     \
+    const unsigned gxy_ = gridDim.x * gridDim.y;                                        \
+    const unsigned L_ = fz_xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), \
+                                     gxy_ * gridDim.z);                                 \
+    bz = L_ / gxy_;                                                                     \
+    by = (L_ - bz * gxy_) / gridDim.x;                                                  \
+    bx = L_ - bz * gxy_ - by * gridDim.x;                                               \
+  }
+
 #define CELL_IJK(d)                                   \
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1; \
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1; \
-  const int k = blockIdx.z + 1;                       \
+  FZ_BLOCK_XYZ(bx_, by_, bz_);                        \
+  const int i = bx_ * blockDim.x + threadIdx.x + 1;   \
+  const int j = by_ * blockDim.y + threadIdx.y + 1;   \
+  const int k = bz_ + 1;                              \
   if (i > (d).n1 || j > (d).n2) return;               \
   const long q = i + (d).sy * j + (d).sz * k;         \
   const long r = (i - 1) + (long)(d).n1 * ((j - 1) + (long)(d).n2 * (k - 1));
 
 #define HALO_IJK(d)                                   \
-  const int i = blockIdx.x * blockDim.x + threadIdx.x; \
-  const int j = blockIdx.y * blockDim.y + threadIdx.y; \
-  const int k = blockIdx.z;                           \
+  FZ_BLOCK_XYZ(bx_, by_, bz_);                        \
+  const int i = bx_ * blockDim.x + threadIdx.x;       \
+  const int j = by_ * blockDim.y + threadIdx.y;       \
+  const int k = bz_;                                  \
   if (i > (d).n1 + 1 || j > (d).n2 + 1) return;       \
   const long q = i + (d).sy * j + (d).sz * k;
 
@@ -68,20 +82,16 @@ __global__ void k_add_face(fz_dims d, int dir, int plane, const double *__restri
 }
 
 // ---------------------------------------------------------------------------
-// acdi_transport_pf, src/acdi.f90:102-224 (SDF-normal sharpening)
-__global__ void __launch_bounds__(256) k_acdi_transport(fz_dims d, const double *__restrict__ dzci,
-                                                        const double *__restrict__ dzfi, double dxi, double dyi,
-                                                        const double *__restrict__ d_gam, double gam_v, double seps,
-                                                        const double *__restrict__ u, const double *__restrict__ v,
-                                                        const double *__restrict__ w, const double *__restrict__ normx,
-                                                        const double *__restrict__ normy,
-                                                        const double *__restrict__ normz,
-                                                        const double *__restrict__ phi, const double *__restrict__ psi,
-                                                        double *__restrict__ dpsidt, double *__restrict__ flux_x,
-                                                        double *__restrict__ flux_y, double *__restrict__ flux_z) {
-  CELL_IJK(d);
+// acdi_transport_pf at one cell, src/acdi.f90:102-224 (SDF-normal sharpening):
+// the right-hand side R and the three face fluxes of the cell's + faces
+__device__ __forceinline__ void acdi_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
+                                          const double *__restrict__ dzfi, double dxi, double dyi, double gam,
+                                          double seps, const double *__restrict__ u, const double *__restrict__ v,
+                                          const double *__restrict__ w, const double *__restrict__ normx,
+                                          const double *__restrict__ normy, const double *__restrict__ normz,
+                                          const double *__restrict__ phi, const double *__restrict__ psi, double &R,
+                                          double &fx, double &fy, double &fz) {
   const long sy = d.sy, sz = d.sz;
-  const double gam = d_gam ? *d_gam : gam_v;
   const double dzfi_c = dzfi[k], dzci_c = dzci[k], dzci_m = dzci[k - 1];
   const double psi_ccm = psi[q - sz], psi_cmc = psi[q - sy], psi_mcc = psi[q - 1], psi_ccc = psi[q],
                psi_pcc = psi[q + 1], psi_cpc = psi[q + sy], psi_ccp = psi[q + sz];
@@ -124,10 +134,70 @@ __global__ void __launch_bounds__(256) k_acdi_transport(fz_dims d, const double 
   const double rglrym = dpsidym - sharpym, rglryp = dpsidyp - sharpyp;
   const double rglrzm = dpsidzm - sharpzm, rglrzp = dpsidzp - sharpzp;
   const double rglr = (rglrxp - rglrxm) * dxi + (rglryp - rglrym) * dyi + (rglrzp - rglrzm) * dzfi_c;
-  dpsidt[r] = -adv + rglr;
-  flux_x[q] = upsiip - rglrxp;
-  flux_y[q] = vpsijp - rglryp;
-  flux_z[q] = wpsikp - rglrzp;
+  R = -adv + rglr;
+  fx = upsiip - rglrxp;
+  fy = vpsijp - rglryp;
+  fz = wpsikp - rglrzp;
+}
+
+__global__ void __launch_bounds__(256) k_acdi_transport(fz_dims d, const double *__restrict__ dzci,
+                                                        const double *__restrict__ dzfi, double dxi, double dyi,
+                                                        const double *__restrict__ d_gam, double gam_v, double seps,
+                                                        const double *__restrict__ u, const double *__restrict__ v,
+                                                        const double *__restrict__ w, const double *__restrict__ normx,
+                                                        const double *__restrict__ normy,
+                                                        const double *__restrict__ normz,
+                                                        const double *__restrict__ phi, const double *__restrict__ psi,
+                                                        double *__restrict__ dpsidt, double *__restrict__ flux_x,
+                                                        double *__restrict__ flux_y, double *__restrict__ flux_z) {
+  CELL_IJK(d);
+  const double gam = d_gam ? *d_gam : gam_v;
+  double R, fx, fy, fz;
+  acdi_cell(d, q, k, dzci, dzfi, dxi, dyi, gam, seps, u, v, w, normx, normy, normz, phi, psi, R, fx, fy, fz);
+  dpsidt[r] = R;
+  flux_x[q] = fx;
+  flux_y[q] = fy;
+  flux_z[q] = fz;
+}
+
+// rk_2fl's phase-field part in one pass (src/rk.f90:226-252): acdi_transport_pf,
+// the AB2 update psi = psio + f1*R + f2*R_old and the clip to [0,1] (the
+// reference also clips the halos, which the boundary update that follows
+// overwrites from the interior). The history dpsio is read and replaced by
+// R in place; on the first step the history is R itself.
+__global__ void __launch_bounds__(256) k_acdi_step(fz_dims d, const double *__restrict__ dzci,
+                                                   const double *__restrict__ dzfi, double dxi, double dyi,
+                                                   const double *__restrict__ d_gam, double gam_v, double seps,
+                                                   double factor1, double factor2, int first,
+                                                   const double *__restrict__ u, const double *__restrict__ v,
+                                                   const double *__restrict__ w, const double *__restrict__ normx,
+                                                   const double *__restrict__ normy, const double *__restrict__ normz,
+                                                   const double *__restrict__ phi, const double *__restrict__ psio,
+                                                   double *__restrict__ dpsio, double *__restrict__ psi_out,
+                                                   double *__restrict__ flux_x, double *__restrict__ flux_y,
+                                                   double *__restrict__ flux_z) {
+  CELL_IJK(d);
+  const double gam = d_gam ? *d_gam : gam_v;
+  double R, fx, fy, fz;
+  acdi_cell(d, q, k, dzci, dzfi, dxi, dyi, gam, seps, u, v, w, normx, normy, normz, phi, psio, R, fx, fy, fz);
+  const double Ro = first ? R : dpsio[r];
+  dpsio[r] = R;
+  flux_x[q] = fx;
+  flux_y[q] = fy;
+  flux_z[q] = fz;
+  const double pn = psio[q] + factor1 * R + factor2 * Ro;
+  psi_out[q] = fmin(fmax(0.0, pn), 1.0);
+}
+
+int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
+                        int first, const double *u, const double *v, const double *w, const double *normx,
+                        const double *normy, const double *normz, const double *phi, const double *psio,
+                        double *dpsio, double *psi_out, double *flux_x, double *flux_y, double *flux_z) {
+  FZ_LAUNCH(c, "acdi_step", k_acdi_step, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
+            c->dli[1], d_gam, gam, seps, factor1, factor2, first, u, v, w, normx, normy, normz, phi, psio, dpsio,
+            psi_out, flux_x, flux_y, flux_z);
+  FZ_LAUNCHED();
+  return FZ_OK;
 }
 
 // acdi_cmpt_phi, src/acdi.f90:239-247 (halo-inclusive)
@@ -247,17 +317,15 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
 }
 
 // ---------------------------------------------------------------------------
-// mom_xyz_ad, src/mom.f90:545-850 (non-split viscous diffusion)
-__global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restrict__ dzci,
-                                                const double *__restrict__ dzfi, double dxi, double dyi, double rho,
-                                                double drho, double mu, double dmu,
-                                                const double *__restrict__ psiflx_x,
-                                                const double *__restrict__ psiflx_y,
-                                                const double *__restrict__ psiflx_z, const double *__restrict__ u,
-                                                const double *__restrict__ v, const double *__restrict__ w,
-                                                const double *__restrict__ psi, double *__restrict__ dudt,
-                                                double *__restrict__ dvdt, double *__restrict__ dwdt) {
-  CELL_IJK(d);
+// mom_xyz_ad at one cell (linear index q): advection with the phase-field mass
+// flux and variable-viscosity (non-split) diffusion, src/mom.f90:545-850
+__device__ __forceinline__ void mom_ad_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
+                                            const double *__restrict__ dzfi, double dxi, double dyi, double rho,
+                                            double drho, double mu, double dmu, const double *__restrict__ psiflx_x,
+                                            const double *__restrict__ psiflx_y,
+                                            const double *__restrict__ psiflx_z, const double *__restrict__ u,
+                                            const double *__restrict__ v, const double *__restrict__ w,
+                                            const double *__restrict__ psi, double &ru, double &rv, double &rw) {
   const long sy = d.sy, sz = d.sz;
 #define S_(f, a, b, c) f[q + (a) + (b)*sy + (c)*sz]
   const double u_ccm = S_(u, 0, 0, -1), u_cmc = S_(u, 0, -1, 0), u_mcc = S_(u, -1, 0, 0), u_ccc = S_(u, 0, 0, 0),
@@ -364,21 +432,37 @@ __global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restr
                dyi * ((dwdyp + dvdzp) * muyp - (dwdym + dvdzm) * muym) +
                dzci_c * ((dwdzp + dwdzp) * muzp - (dwdzm + dwdzm) * muzm);
   }
-  dudt[r] = dudt_aux;
-  dvdt[r] = dvdt_aux;
-  dwdt[r] = dwdt_aux;
+  ru = dudt_aux;
+  rv = dvdt_aux;
+  rw = dwdt_aux;
 }
 
-// mom_xyz_oth, src/mom.f90:912-1038 (constant-coefficient split, Brackbill)
-__global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__restrict__ dzci, double dxi, double dyi,
-                                                 double dt_r, double rho, double drho, double surf_factor,
-                                                 double rho0, double bfx, double bfy, double bfz, double gx, double gy,
-                                                 double gz, double rax, double ray, double raz,
-                                                 const double *__restrict__ p, const double *__restrict__ psi,
-                                                 const double *__restrict__ kappa, const double *__restrict__ pn,
-                                                 const double *__restrict__ po, double *__restrict__ dudt,
-                                                 double *__restrict__ dvdt, double *__restrict__ dwdt) {
+__global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restrict__ dzci,
+                                                const double *__restrict__ dzfi, double dxi, double dyi, double rho,
+                                                double drho, double mu, double dmu,
+                                                const double *__restrict__ psiflx_x,
+                                                const double *__restrict__ psiflx_y,
+                                                const double *__restrict__ psiflx_z, const double *__restrict__ u,
+                                                const double *__restrict__ v, const double *__restrict__ w,
+                                                const double *__restrict__ psi, double *__restrict__ dudt,
+                                                double *__restrict__ dvdt, double *__restrict__ dwdt) {
   CELL_IJK(d);
+  double ru, rv, rw;
+  mom_ad_cell(d, q, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, psiflx_x, psiflx_y, psiflx_z, u, v, w, psi, ru, rv,
+              rw);
+  dudt[r] = ru;
+  dvdt[r] = rv;
+  dwdt[r] = rw;
+}
+
+// mom_xyz_oth at one cell, src/mom.f90:912-1038 (constant-coefficient split, Brackbill)
+__device__ __forceinline__ void mom_oth_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
+                                             double dxi, double dyi, double dt_r, double rho, double drho,
+                                             double surf_factor, double rho0, double bfx, double bfy, double bfz,
+                                             double gx, double gy, double gz, double rax, double ray, double raz,
+                                             const double *__restrict__ p, const double *__restrict__ psi,
+                                             const double *__restrict__ kappa, const double *__restrict__ pn,
+                                             const double *__restrict__ po, double &ru, double &rv, double &rw) {
   const long sy = d.sy, sz = d.sz;
   const double dzci_c = dzci[k];
   const double p_ccc = p[q], p_pcc = p[q + 1], p_cpc = p[q + sy], p_ccp = p[q + sz];
@@ -408,9 +492,85 @@ __global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__rest
   dudt_aux = dudt_aux - (dpdx_e + rhoxp / rho0 * (dpdx - dpdx_e));
   dvdt_aux = dvdt_aux - (dpdy_e + rhoyp / rho0 * (dpdy - dpdy_e));
   dwdt_aux = dwdt_aux - (dpdz_e + rhozp / rho0 * (dpdz - dpdz_e));
-  dudt[r] = dudt_aux;
-  dvdt[r] = dvdt_aux;
-  dwdt[r] = dwdt_aux;
+  ru = dudt_aux;
+  rv = dvdt_aux;
+  rw = dwdt_aux;
+}
+
+__global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__restrict__ dzci, double dxi, double dyi,
+                                                 double dt_r, double rho, double drho, double surf_factor,
+                                                 double rho0, double bfx, double bfy, double bfz, double gx, double gy,
+                                                 double gz, double rax, double ray, double raz,
+                                                 const double *__restrict__ p, const double *__restrict__ psi,
+                                                 const double *__restrict__ kappa, const double *__restrict__ pn,
+                                                 const double *__restrict__ po, double *__restrict__ dudt,
+                                                 double *__restrict__ dvdt, double *__restrict__ dwdt) {
+  CELL_IJK(d);
+  double ru, rv, rw;
+  mom_oth_cell(d, q, k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz,
+               p, psi, kappa, pn, po, ru, rv, rw);
+  dudt[r] = ru;
+  dvdt[r] = rv;
+  dwdt[r] = rw;
+}
+
+// The whole momentum part of rk (src/rk.f90:72-130) in one pass: mom_xyz_ad,
+// the AB2 update with the density rescale, mom_xyz_oth (pn = p) and the
+// second update, with the same expressions as k_mom_ad, k_rk_mom_ab2,
+// k_mom_oth and k_rk_mom_oth in turn (no intermediate is rounded
+// differently: each was a double in memory and is a double in a register
+// here). The new velocity goes to un/vn/wn (the stencil reads the old one);
+// the AB2 history duo/dvo/dwo is read and replaced by this step's
+// advection-diffusion term in place (on the first step the history is the
+// term itself, src/rk.f90:76-83).
+__global__ void __launch_bounds__(256) k_mom_fused(
+    fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
+    double drho, double mu, double dmu, double factor1, double factor2, int first, double factor12, double dt_r,
+    double surf_factor, double rho0, double bfx, double bfy, double bfz, double gx, double gy, double gz, double rax,
+    double ray, double raz, const double *__restrict__ psiflx_x, const double *__restrict__ psiflx_y,
+    const double *__restrict__ psiflx_z, const double *__restrict__ u, const double *__restrict__ v,
+    const double *__restrict__ w, const double *__restrict__ psi, const double *__restrict__ psio,
+    const double *__restrict__ p, const double *__restrict__ po, const double *__restrict__ kappa,
+    double *__restrict__ duo, double *__restrict__ dvo, double *__restrict__ dwo, double *__restrict__ un,
+    double *__restrict__ vn, double *__restrict__ wn) {
+  CELL_IJK(d);
+  const long sy = d.sy, sz = d.sz;
+  double ru, rv, rw;
+  mom_ad_cell(d, q, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, psiflx_x, psiflx_y, psiflx_z, u, v, w, psi, ru, rv,
+              rw);
+  const double ruo = first ? ru : duo[r], rvo = first ? rv : dvo[r], rwo = first ? rw : dwo[r];
+  duo[r] = ru;
+  dvo[r] = rv;
+  dwo[r] = rw;
+  const double rhox_n = rho + drho * 0.5 * (psio[q] + psio[q + 1]);
+  const double rhoy_n = rho + drho * 0.5 * (psio[q] + psio[q + sy]);
+  const double rhoz_n = rho + drho * 0.5 * (psio[q] + psio[q + sz]);
+  const double rhox_p = rho + drho * 0.5 * (psi[q] + psi[q + 1]);
+  const double rhoy_p = rho + drho * 0.5 * (psi[q] + psi[q + sy]);
+  const double rhoz_p = rho + drho * 0.5 * (psi[q] + psi[q + sz]);
+  double uu = u[q] * rhox_n / rhox_p + (factor1 * ru + factor2 * ruo) / rhox_p;
+  double vv = v[q] * rhoy_n / rhoy_p + (factor1 * rv + factor2 * rvo) / rhoy_p;
+  double ww = w[q] * rhoz_n / rhoz_p + (factor1 * rw + factor2 * rwo) / rhoz_p;
+  mom_oth_cell(d, q, k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz,
+               p, psi, kappa, p, po, ru, rv, rw);
+  un[q] = uu + factor12 * ru / rhox_p;
+  vn[q] = vv + factor12 * rv / rhoy_p;
+  wn[q] = ww + factor12 * rw / rhoz_p;
+}
+
+int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], double factor1, double factor2,
+                        int first, double factor12, double dt_r, double sigma, double rho0, const double bforce[3],
+                        const double gacc[3], const double rho_av[3], const double *fx, const double *fy,
+                        const double *fz, const double *u, const double *v, const double *w, const double *psi,
+                        const double *psio, const double *p, const double *po, const double *kappa, double *duo,
+                        double *dvo, double *dwo, double *un, double *vn, double *wn) {
+  const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
+  FZ_LAUNCH(c, "mom_fused", k_mom_fused, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
+            c->dli[1], rho12[1], rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12,
+            dt_r, surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1],
+            rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);
+  FZ_LAUNCHED();
+  return FZ_OK;
 }
 
 // AB2 momentum update with the density rescale, src/rk.f90:86-101

@@ -3,9 +3,9 @@
  *
  * Every entry point replaces one routine of the reference's Fortran module
  * interface; the reference file:line it stands in for is given next to each
- * declaration. INTEGRATION.md shows the iso_c_binding interfaces a Fortran
- * caller binds (the modules under cans-fizzy_amd/fortran/ are those
- * bindings, with the reference's module and routine names).
+ * declaration. The Fortran modules under cans-fizzy_amd/fortran/ bind these
+ * entry points through iso_c_binding under the reference's module and
+ * routine names.
  *
  * Conventions
  *  - Fields are device pointers (hipMalloc'd, e.g. through fz_malloc) to

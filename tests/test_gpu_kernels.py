@@ -237,7 +237,7 @@ def test_r2r_lines_bitwise(kind, rng):
     """fz_r2r (the fft routine, src/fft.f90:176-187) against the oracle's r2r, line by line."""
     ctx = F.Context([4, 4, 4], [1, 1, 1], "PPPPPP")
     try:
-        for n in [1, 2, 3, 5, 6, 7, 8, 10, 12, 13, 16, 18, 36, 64, 96, 128, 256, 512, 1024]:
+        for n in [1, 2, 3, 5, 6, 7, 8, 10, 12, 13, 16, 18, 32, 36, 64, 96, 128, 256, 512, 1024, 2048]:
             if KINDS[kind] in (6, 10) and 2 * n > 2048:
                 continue
             howmany = 7
@@ -255,7 +255,8 @@ def test_r2r_lines_bitwise(kind, rng):
 
 POISSON = [([16, 12, 10], "PPPPNN"), ([64, 64, 32], "PPPPNN"), ([30, 18, 7], "NNPPNN"), ([12, 10, 9], "DDNNDD"),
            ([2, 64, 64], "PPNNNN"), ([64, 2, 128], "NNPPNN"), ([20, 14, 8], "NDDNPP"), ([24, 16, 16], "PPPPPP"),
-           ([36, 10, 12], "DNNDDN"), ([128, 128, 64], "PPPPNN")]
+           ([36, 10, 12], "DNNDDN"), ([128, 128, 64], "PPPPNN"), ([256, 256, 8], "PPPPNN"),
+           ([512, 40, 4], "PPNNNN"), ([24, 512, 5], "NNPPDD"), ([16, 64, 8], "PPNDNN"), ([1024, 6, 3], "DNPPNN")]
 
 
 @pytest.mark.parametrize("n,cbc", POISSON, ids=[f"{'x'.join(map(str, a))}-{b}" for a, b in POISSON])

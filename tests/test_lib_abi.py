@@ -5,6 +5,7 @@ import os
 
 import fizzy_amd as F
 from fizzy_amd import _lib
+from conftest import ROOT
 
 
 def test_library_exports_every_declared_symbol():
@@ -38,3 +39,17 @@ def test_host_only_entry_points_run_without_gpu():
     dzc, dzf, zc, zf = F.initgrid(1, 8, 0.0, 2.0)
     assert np.allclose(dzf[1:-1], 0.25) and np.isclose(zc[1], 0.125)
     assert os.path.exists(F.LIB_PATH)
+
+
+def test_fortran_modules_bind_the_library():
+    """The drop-in Fortran modules and their driver are built and link libfizzy."""
+    import subprocess
+    drv = os.path.join(ROOT, "cans-fizzy_amd", "fortran", "bin", "fz_drive")
+    assert os.path.exists(drv), "run __graft_entry__.build()"
+    out = subprocess.run(["ldd", drv], capture_output=True, text=True).stdout
+    assert "libfizzy.so" in out and "not found" not in out
+    mods = os.listdir(os.path.join(ROOT, "cans-fizzy_amd", "fortran", "bin"))
+    for m in ["mod_solver", "mod_initsolver", "mod_fillps", "mod_correc", "mod_updatep", "mod_acdi",
+              "mod_vof_thinc_qq", "mod_two_fluid", "mod_mom", "mod_rk", "mod_bound", "mod_chkdiv", "mod_chkdt",
+              "mod_forcing", "mod_utils"]:
+        assert m + ".mod" in mods, m
