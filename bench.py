@@ -36,12 +36,16 @@ ALG_BYTES = {
     "mom_fused": 160,
     # fused phase-field step: reads u,v,w,normals x3,phi,psio,history (9), writes psi,history,flux x3 (5)
     "acdi_step": 112,
+    # fillps evaluated in the load of the forward x transform: reads u,v,w, writes the spectral array
+    "fillps_fft_x_fwd": 32,
+    # correc + updatep + gamma speeds over the halo-inclusive grid: reads u,v,w,dp,p, writes u,v,w,p
+    "correc_step": 72,
     "fft_x_fwd": 16, "fft_y_fwd": 16, "fft_y_bwd": 16, "fft_x_bwd": 16, "thomas": 16, "thomas_periodic": 16,
     "mom_ad": 80, "acdi_transport": 96, "youngs": 40, "mom_oth": 56, "rk_mom_ab2": 112, "rk_mom_oth": 80,
     "rk_psi_oop": 32, "acdi_phi": 16, "clip": 16, "fillps": 32, "correc": 56, "updatep_oop": 24,
     "red_gamma": 24, "red_div": 24, "red_dt": 24, "vof_flux": 88, "vof_div": 32, "vof_phi": 16, "forcing": 32,
 }
-HALO_KERNELS = {"acdi_phi", "clip", "updatep_oop", "vof_phi", "correc"}
+HALO_KERNELS = {"acdi_phi", "clip", "updatep_oop", "vof_phi", "correc", "correc_step"}
 # minimal per-step dataflow of the whole step (stage model S1-S9): 61 passes of 8 B
 STEP_ALG_BYTES_PER_CELL = 488
 

@@ -8,7 +8,10 @@
 // x-pencil), then y and z where they are not periodic. Each direction is one
 // launch over its full halo-inclusive planes; the sub-operations of one
 // direction run in order in the thread that owns a plane position.
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <unordered_map>
 
 #include "fz_internal.h"
 
@@ -99,60 +102,60 @@ static void set_bc_ops(FieldOps &fo, char ctype, int ib, int nn, bool centered, 
   }
 }
 
-static int launch_dir(fz_ctx *c, int dir, double *f0, double *f1, double *f2, const DirOps &ops) {
+static int launch_dir(fz_ctx *c, int dir, double *f0, double *f1, double *f2, const DirOps &ops, bool periodic_copy) {
   const fz_dims &d = c->d;
   int na, nb;
   if (dir == 0) { na = d.n2 + 2; nb = d.n3 + 2; }
   else if (dir == 1) { na = d.n1 + 2; nb = d.n3 + 2; }
   else { na = d.n1 + 2; nb = d.n2 + 2; }
-  FZ_LAUNCH(c, "plane_ops", k_plane_ops, dim3((na + 63) / 64, nb), dim3(64), 0, d, dir, f0, f1, f2, ops);
+  static const char *names[2][3] = {{"bc_x", "bc_y", "bc_z"}, {"bc_per_x", "bc_per_y", "bc_per_z"}};
+  FZ_LAUNCH(c, names[periodic_copy ? 1 : 0][dir], k_plane_ops, dim3((na + 63) / 64, nb), dim3(64), 0, d, dir, f0, f1,
+            f2, ops);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 
-static int periodic_halos(fz_ctx *c, double *f0, double *f1, double *f2) {
+// the ordered phases of one boundp / bounduvw call
+struct Phase {
+  int dir;
+  bool periodic_copy;
+  DirOps ops;
+};
+
+static void periodic_phases(fz_ctx *c, int nfield, std::vector<Phase> &ph) {
   const int nn[3] = {c->d.n1, c->d.n2, c->d.n3};
   for (int dir = 1; dir <= 2; dir++) {
     if (!c->per[dir]) continue;
-    DirOps ops;
-    memset(&ops, 0, sizeof(ops));
-    for (int fi = 0; fi < 3; fi++) {
-      push(ops.f[fi], 1, nn[dir] + 1, 1, 0.0, 0.0);
-      push(ops.f[fi], 1, 0, nn[dir], 0.0, 0.0);
+    Phase P;
+    memset(&P, 0, sizeof(P));
+    P.dir = dir;
+    P.periodic_copy = true;
+    for (int fi = 0; fi < nfield; fi++) {
+      push(P.ops.f[fi], 1, nn[dir] + 1, 1, 0.0, 0.0);
+      push(P.ops.f[fi], 1, 0, nn[dir], 0.0, 0.0);
     }
-    FZ_TRY(launch_dir(c, dir, f0, f1, f2, ops));
+    ph.push_back(P);
   }
-  return FZ_OK;
 }
 
-static bool valid_codes(const char *cbc, int n) {
-  for (int q = 0; q < n; q++)
-    if (cbc[q] != 'P' && cbc[q] != 'D' && cbc[q] != 'N') return false;
-  return true;
-}
-
-extern "C" int fz_boundp(fz_ctx *c, const char cbc[6], const double bc[6], double *p) {
-  if (!c || !cbc || !bc || !p) return fz_set_error(FZ_ERR_ARG, "fz_boundp: null argument");
-  if (!valid_codes(cbc, 6)) return fz_set_error(FZ_ERR_ARG, "fz_boundp: bad boundary code");
-  FZ_TRY(periodic_halos(c, p, nullptr, nullptr));
+static void boundp_phases(fz_ctx *c, const char cbc[6], const double bc[6], std::vector<Phase> &ph) {
+  periodic_phases(c, 1, ph);
   const int nn[3] = {c->d.n1, c->d.n2, c->d.n3};
   const double dr[3][2] = {{c->dl[0], c->dl[0]}, {c->dl[1], c->dl[1]}, {c->h_dzc[0], c->h_dzc[c->d.n3]}};
   for (int dir = 0; dir < 3; dir++) {
     if (dir > 0 && c->per[dir]) continue;
-    DirOps ops;
-    memset(&ops, 0, sizeof(ops));
+    Phase P;
+    memset(&P, 0, sizeof(P));
+    P.dir = dir;
     for (int ib = 0; ib < 2; ib++)
-      set_bc_ops(ops.f[0], cbc[2 * dir + ib], ib, nn[dir], true, bc[2 * dir + ib], dr[dir][ib]);
-    FZ_TRY(launch_dir(c, dir, p, nullptr, nullptr, ops));
+      set_bc_ops(P.ops.f[0], cbc[2 * dir + ib], ib, nn[dir], true, bc[2 * dir + ib], dr[dir][ib]);
+    ph.push_back(P);
   }
-  return FZ_OK;
 }
 
-extern "C" int fz_bounduvw(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, double *u, double *v,
-                           double *w) {
-  if (!c || !cbc || !bc || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: null argument");
-  if (!valid_codes(cbc, 18)) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: bad boundary code");
-  FZ_TRY(periodic_halos(c, u, v, w));
+static void bounduvw_phases(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc,
+                            std::vector<Phase> &ph) {
+  periodic_phases(c, 3, ph);
   const int nn[3] = {c->d.n1, c->d.n2, c->d.n3};
   const double *dzc = c->h_dzc.data(), *dzf = c->h_dzf.data();
 #define CB(b, dd, cc) cbc[(b) + 2 * ((dd) + 3 * (cc))]
@@ -162,8 +165,9 @@ extern "C" int fz_bounduvw(fz_ctx *c, const char cbc[18], const double bc[18], i
     // the face-normal component is skipped on the correction call unless
     // periodic, src/bound.f90:46,57,68
     const bool impose = (!is_correc) || (CB(0, dir, dir) == 'P' && CB(1, dir, dir) == 'P');
-    DirOps ops;
-    memset(&ops, 0, sizeof(ops));
+    Phase P;
+    memset(&P, 0, sizeof(P));
+    P.dir = dir;
     for (int ib = 0; ib < 2; ib++) {
       for (int comp = 0; comp < 3; comp++) {
         const bool normal = (comp == dir);
@@ -171,12 +175,254 @@ extern "C" int fz_bounduvw(fz_ctx *c, const char cbc[18], const double bc[18], i
         double dr;
         if (dir < 2) dr = c->dl[dir];
         else dr = normal ? (ib == 0 ? dzf[0] : dzf[nn[2]]) : (ib == 0 ? dzc[0] : dzc[nn[2]]);
-        set_bc_ops(ops.f[comp], CB(ib, dir, comp), ib, nn[dir], !normal, BV(ib, dir, comp), dr);
+        set_bc_ops(P.ops.f[comp], CB(ib, dir, comp), ib, nn[dir], !normal, BV(ib, dir, comp), dr);
       }
     }
-    FZ_TRY(launch_dir(c, dir, u, v, w, ops));
+    ph.push_back(P);
   }
 #undef CB
 #undef BV
+}
+
+static int run_phases(fz_ctx *c, const std::vector<Phase> &ph, double *f0, double *f1, double *f2) {
+  for (const Phase &P : ph) FZ_TRY(launch_dir(c, P.dir, f0, f1, f2, P.ops, P.periodic_copy));
   return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The phases composed into one gather per call. Every operation writes one
+// cell from one other cell (copy or affine) or from a constant, so the final
+// value of every written cell is a chain of at most a few such operations
+// applied to one original value. Replaying the phases symbolically on the
+// host gives, per written cell, its source cell and its chain; one launch
+// then evaluates the chains with the same arithmetic as the planes kernel
+// (so the result is bit for bit that of the ordered phases). When some
+// source cell is itself written by the call the gather would race, and the
+// ordered phases are used instead.
+// ---------------------------------------------------------------------------
+#define BC_MAXOPS 6
+struct BcChain {
+  int nops;
+  int mode[BC_MAXOPS];
+  double factor[BC_MAXOPS], sgn[BC_MAXOPS];
+};
+struct fz_bcmap {
+  bool ok = false;  // false: use the ordered phases
+  int n = 0;
+  int *dst = nullptr, *src = nullptr;
+  unsigned char *cid = nullptr;
+  BcChain *chains = nullptr;
+};
+struct BcMapDev {
+  const int *dst, *src;
+  const unsigned char *cid;
+  const BcChain *chains;
+  int n;
+};
+
+__global__ void k_bc_gather(BcMapDev m0, BcMapDev m1, BcMapDev m2, double *__restrict__ f0, double *__restrict__ f1,
+                            double *__restrict__ f2) {
+  const BcMapDev m = blockIdx.y == 0 ? m0 : (blockIdx.y == 1 ? m1 : m2);
+  double *f = blockIdx.y == 0 ? f0 : (blockIdx.y == 1 ? f1 : f2);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.n; t += gridDim.x * blockDim.x) {
+    const BcChain &ch = m.chains[m.cid[t]];
+    const int sidx = m.src[t];
+    double v = sidx >= 0 ? f[sidx] : 0.0;
+    for (int o = 0; o < ch.nops; o++) v = (ch.mode[o] == 2) ? ch.factor[o] + ch.sgn[o] * v : ch.factor[o];
+    f[m.dst[t]] = v;
+  }
+}
+
+namespace {
+struct Expr {
+  long src;  // original cell, or -1 when the chain starts from a constant
+  int nops;
+  SubOp ops[BC_MAXOPS];
+};
+}  // namespace
+
+static void plane_geometry(const fz_dims &d, int dir, int &na, int &nb) {
+  if (dir == 0) { na = d.n2 + 2; nb = d.n3 + 2; }
+  else if (dir == 1) { na = d.n1 + 2; nb = d.n3 + 2; }
+  else { na = d.n1 + 2; nb = d.n2 + 2; }
+}
+
+static fz_bcmap *compose(fz_ctx *c, const std::vector<Phase> &ph, int fi) {
+  fz_bcmap *m = new fz_bcmap();
+  const fz_dims &d = c->d;
+  if (c->nh >= 2147483647L) return m;
+  std::unordered_map<long, Expr> E;
+  E.reserve(8 * (size_t)((d.n1 + 2) * (d.n2 + 2) + (d.n1 + 2) * (d.n3 + 2) + (d.n2 + 2) * (d.n3 + 2)));
+  for (const Phase &P : ph) {
+    const FieldOps &fo = P.ops.f[fi];
+    if (fo.nops == 0) continue;
+    int na, nb;
+    plane_geometry(d, P.dir, na, nb);
+    for (int b = 0; b < nb; b++)
+      for (int a = 0; a < na; a++) {
+        long base, st;
+        if (P.dir == 0) { base = d.sy * a + d.sz * b; st = 1; }
+        else if (P.dir == 1) { base = a + d.sz * b; st = d.sy; }
+        else { base = a + d.sy * b; st = d.sz; }
+        for (int s = 0; s < fo.nops; s++) {
+          const SubOp &o = fo.op[s];
+          const long dst = base + st * o.dst, src = base + st * o.src;
+          Expr e;
+          if (o.mode == 3) {
+            e.src = -1;
+            e.nops = 1;
+            e.ops[0] = o;
+          } else {
+            auto it = E.find(src);
+            if (it != E.end()) e = it->second;
+            else { e.src = src; e.nops = 0; }
+            if (o.mode == 2) {
+              if (e.nops == BC_MAXOPS) return m;
+              e.ops[e.nops++] = o;
+            }
+          }
+          E[dst] = e;
+        }
+      }
+  }
+  std::vector<std::pair<long, const Expr *>> recs;
+  recs.reserve(E.size());
+  for (const auto &kv : E) {
+    if (kv.second.src >= 0 && E.count(kv.second.src)) return m;  // a written cell is also read: not a gather
+    recs.push_back({kv.first, &kv.second});
+  }
+  std::sort(recs.begin(), recs.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  std::vector<int> hdst(recs.size()), hsrc(recs.size());
+  std::vector<unsigned char> hcid(recs.size());
+  std::vector<BcChain> chains;
+  std::map<std::string, int> chain_id;
+  for (size_t r = 0; r < recs.size(); r++) {
+    const Expr &e = *recs[r].second;
+    BcChain ch;
+    memset(&ch, 0, sizeof(ch));
+    ch.nops = e.nops;
+    for (int o = 0; o < e.nops; o++) {
+      ch.mode[o] = e.ops[o].mode;
+      ch.factor[o] = e.ops[o].factor;
+      ch.sgn[o] = e.ops[o].sgn;
+    }
+    const std::string key((const char *)&ch, sizeof(ch));
+    auto it = chain_id.find(key);
+    int id;
+    if (it == chain_id.end()) {
+      id = (int)chains.size();
+      if (id > 255) return m;
+      chains.push_back(ch);
+      chain_id[key] = id;
+    } else {
+      id = it->second;
+    }
+    hdst[r] = (int)recs[r].first;
+    hsrc[r] = (int)e.src;
+    hcid[r] = (unsigned char)id;
+  }
+  const int n = (int)recs.size();
+  bool good = hipMalloc((void **)&m->dst, sizeof(int) * (n ? n : 1)) == hipSuccess &&
+              hipMalloc((void **)&m->src, sizeof(int) * (n ? n : 1)) == hipSuccess &&
+              hipMalloc((void **)&m->cid, (n ? n : 1)) == hipSuccess &&
+              hipMalloc((void **)&m->chains, sizeof(BcChain) * (chains.empty() ? 1 : chains.size())) == hipSuccess;
+  if (good && n) {
+    good = hipMemcpy(m->dst, hdst.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(m->src, hsrc.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(m->cid, hcid.data(), n, hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(m->chains, chains.data(), sizeof(BcChain) * chains.size(), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  m->n = n;
+  m->ok = good;
+  return m;
+}
+
+static void bcmap_free(fz_bcmap *m) {
+  if (!m) return;
+  if (m->dst) (void)hipFree(m->dst);
+  if (m->src) (void)hipFree(m->src);
+  if (m->cid) (void)hipFree(m->cid);
+  if (m->chains) (void)hipFree(m->chains);
+  delete m;
+}
+
+void fz_bcmaps_free(fz_ctx *c) {
+  for (auto &kv : c->bcmaps) bcmap_free(kv.second);
+  c->bcmaps.clear();
+}
+
+static fz_bcmap *get_map(fz_ctx *c, const std::string &key, const std::vector<Phase> &ph, int fi) {
+  auto it = c->bcmaps.find(key);
+  if (it != c->bcmaps.end()) return it->second;
+  fz_bcmap *m = compose(c, ph, fi);
+  c->bcmaps[key] = m;
+  return m;
+}
+
+static std::string signature(const char *kind, const char *cbc, int ncbc, const double *bc, int is_correc, int fi) {
+  std::string k(kind);
+  k.append(cbc, ncbc);
+  k.append((const char *)bc, sizeof(double) * ncbc);
+  k.push_back((char)('0' + is_correc));
+  k.push_back((char)('0' + fi));
+  return k;
+}
+
+static BcMapDev devmap(const fz_bcmap *m) {
+  BcMapDev d;
+  d.dst = m ? m->dst : nullptr;
+  d.src = m ? m->src : nullptr;
+  d.cid = m ? m->cid : nullptr;
+  d.chains = m ? m->chains : nullptr;
+  d.n = m ? m->n : 0;
+  return d;
+}
+
+static int launch_gather(fz_ctx *c, int nf, const fz_bcmap *const *maps, double *f0, double *f1, double *f2) {
+  int nmax = 0;
+  for (int q = 0; q < nf; q++) nmax = std::max(nmax, maps[q]->n);
+  if (nmax == 0) return FZ_OK;
+  const int nb = std::min((nmax + 255) / 256, 4096);
+  FZ_LAUNCH(c, "bc_gather", k_bc_gather, dim3(nb, nf), dim3(256), 0, devmap(maps[0]),
+            devmap(nf > 1 ? maps[1] : nullptr), devmap(nf > 2 ? maps[2] : nullptr), f0, f1, f2);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+static bool valid_codes(const char *cbc, int n) {
+  for (int q = 0; q < n; q++)
+    if (cbc[q] != 'P' && cbc[q] != 'D' && cbc[q] != 'N') return false;
+  return true;
+}
+
+static bool g_gather_off = getenv("FZ_BC_PHASES") != nullptr;  // the ordered phases, for comparison
+
+extern "C" int fz_boundp(fz_ctx *c, const char cbc[6], const double bc[6], double *p) {
+  if (!c || !cbc || !bc || !p) return fz_set_error(FZ_ERR_ARG, "fz_boundp: null argument");
+  if (!valid_codes(cbc, 6)) return fz_set_error(FZ_ERR_ARG, "fz_boundp: bad boundary code");
+  std::vector<Phase> ph;
+  boundp_phases(c, cbc, bc, ph);
+  if (!g_gather_off) {
+    fz_bcmap *m = get_map(c, signature("p", cbc, 6, bc, 0, 0), ph, 0);
+    if (m->ok) return launch_gather(c, 1, &m, p, nullptr, nullptr);
+  }
+  return run_phases(c, ph, p, nullptr, nullptr);
+}
+
+extern "C" int fz_bounduvw(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, double *u, double *v,
+                           double *w) {
+  if (!c || !cbc || !bc || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: null argument");
+  if (!valid_codes(cbc, 18)) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: bad boundary code");
+  std::vector<Phase> ph;
+  bounduvw_phases(c, cbc, bc, is_correc ? 1 : 0, ph);
+  if (!g_gather_off) {
+    fz_bcmap *m[3];
+    bool ok = true;
+    for (int fi = 0; fi < 3; fi++) {
+      m[fi] = get_map(c, signature("uvw", cbc, 18, bc, is_correc ? 1 : 0, fi), ph, fi);
+      ok = ok && m[fi]->ok;
+    }
+    if (ok) return launch_gather(c, 3, m, u, v, w);
+  }
+  return run_phases(c, ph, u, v, w);
 }

@@ -12,6 +12,7 @@
 //    mom_xyz_oth reads it).
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "fz_internal.h"
 
@@ -28,6 +29,11 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
                         const double *psio, const double *p, const double *po, const double *kappa, double *duo,
                         double *dvo, double *dwo, double *un, double *vn, double *wn);
 int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
+int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *dp, const double *p, const double *u,
+                          const double *v, const double *w, double *un, double *vn, double *wn, double *p_out,
+                          double *part);
+int fz_launch_gamma_fin(fz_ctx *c, int nb, const double *part, double factor, double gmin, double *out);
+long fz_correc_step_blocks(const fz_ctx *c);
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
                         int first, const double *u, const double *v, const double *w, const double *normx,
                         const double *normy, const double *normz, const double *phi, const double *psio,
@@ -51,13 +57,6 @@ __global__ void k_rk_psi_oop(fz_dims d, double f1, double f2, const double *__re
   psi_out[q] = psi_in[q] + f1 * dpsidt[r] + f2 * dpsidto[r];
 }
 
-// out-of-place updatep: p_out = p + dp over the whole array, src/updatep.f90:21-23
-__global__ void k_updatep_oop(long nh, const double *__restrict__ dp, const double *__restrict__ p,
-                              double *__restrict__ p_out) {
-  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
-    p_out[q] = dp[q] + p[q];
-}
-
 enum { F_U, F_V, F_W, F_P, F_PN, F_PO, F_PSI, F_PHI, F_KAPPA, F_NX, F_NY, F_NZ, F_FX, F_FY, F_FZ, F_DPSIO, F_DUO,
        F_DVO, F_DWO, F_N };
 enum { S_TIME, S_DT, S_DTO, S_GAM, S_SEPS, S_RHO0, S_DIVTOT, S_DIVMAX, S_DTCFL, S_ISTEP, S_FIRST, S_N };
@@ -69,6 +68,7 @@ struct fz_case {
   double *psio = nullptr, *dpsidt = nullptr, *dp = nullptr;
   double *un = nullptr, *vn = nullptr, *wn = nullptr;  // the next velocity (rotated with u, v, w)
   double *dsc = nullptr;  // device scalars: 0 gam, 1-2 div, 3 adv dt, 4-6 rho_av
+  double *gpart = nullptr;  // per-block maxima of the squared cell speed (gamma)
   double *gvr_c = nullptr, *gvr_f = nullptr;
   double sc[12] = {0};
   char cbccur[6];
@@ -94,6 +94,7 @@ static int case_alloc(fz_case *cs) {
     FZ_HIP(hipMalloc((void **)itr[q], sizeof(double) * c->ni));
     FZ_HIP(hipMemsetAsync(*itr[q], 0, sizeof(double) * c->ni, c->st));
   }
+  FZ_HIP(hipMalloc((void **)&cs->gpart, sizeof(double) * fz_correc_step_blocks(c)));
   FZ_HIP(hipMalloc((void **)&cs->dsc, sizeof(double) * 16));
   FZ_HIP(hipMemsetAsync(cs->dsc, 0, sizeof(double) * 16, c->st));
   const int nz = c->d.n3 + 2;
@@ -115,8 +116,8 @@ extern "C" int fz_case_destroy(fz_case *cs) {
   if (cs->ctx) (void)hipStreamSynchronize(cs->ctx->st);
   for (int q = 0; q < F_N; q++)
     if (cs->f[q]) (void)hipFree(cs->f[q]);
-  double *o[9] = {cs->psio, cs->dpsidt, cs->un, cs->vn, cs->wn, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f};
-  for (int q = 0; q < 9; q++)
+  double *o[10] = {cs->psio, cs->dpsidt, cs->un, cs->vn, cs->wn, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f, cs->gpart};
+  for (int q = 0; q < 10; q++)
     if (o[q]) (void)hipFree(o[q]);
   if (cs->ctx) fz_ctx_destroy(cs->ctx);
   delete cs;
@@ -228,19 +229,15 @@ static int new_dt(fz_case *cs, double *dt, int *kill) {
   return FZ_OK;
 }
 
-extern "C" int fz_case_init(fz_case *cs) {
-  if (!cs) return fz_set_error(FZ_ERR_ARG, "fz_case_init: null argument");
+// the initialisation after the fields are set, src/main.f90:333-407:
+// boundary conditions, pn = po = p, phi/normals/curvature, gamma and dt
+static int finish_init(fz_case *cs) {
   fz_ctx *c = cs->ctx;
   const fz_params &P = cs->prm;
   double **f = cs->f;
-  double seps;
-  FZ_TRY(fz_acdi_set_epsilon(c, P.psi_thickness_factor, &seps));  // src/main.f90:301
-  cs->sc[S_SEPS] = seps;
-  for (int q = F_U; q <= F_P; q++) FZ_TRY(fz_launch_fill(c, c->nh, 0.0, f[q]));  // initflow 'zer'
-  FZ_TRY(fz_init2fl(c, &P, seps, f[F_PSI]));
   FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
   FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));
-  FZ_HIP(hipMemcpyAsync(f[F_PO], f[F_P], sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
+  FZ_TRY(fz_copy(c, f[F_PO], f[F_P], sizeof(double) * c->nh));
   FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
   FZ_TRY(phase_geometry(cs));
   if (!P.vof) {
@@ -249,13 +246,60 @@ extern "C" int fz_case_init(fz_case *cs) {
   }
   double dt;
   FZ_TRY(new_dt(cs, &dt, nullptr));
-  cs->sc[S_TIME] = 0.0;
   cs->sc[S_DT] = dt;
   cs->sc[S_DTO] = dt;
-  cs->sc[S_ISTEP] = 0.0;
-  cs->sc[S_FIRST] = 1.0;
+  cs->sc[S_FIRST] = 1.0;  // the AB2 histories start over (src/rk.f90:76-83)
   FZ_HIP(hipStreamSynchronize(c->st));
   return FZ_OK;
+}
+
+static int set_epsilon(fz_case *cs) {
+  double seps;
+  FZ_TRY(fz_acdi_set_epsilon(cs->ctx, cs->prm.psi_thickness_factor, &seps));  // src/main.f90:301
+  cs->sc[S_SEPS] = seps;
+  return FZ_OK;
+}
+
+extern "C" int fz_case_init(fz_case *cs) {
+  if (!cs) return fz_set_error(FZ_ERR_ARG, "fz_case_init: null argument");
+  fz_ctx *c = cs->ctx;
+  double **f = cs->f;
+  FZ_TRY(set_epsilon(cs));
+  for (int q = F_U; q <= F_P; q++) FZ_TRY(fz_launch_fill(c, c->nh, 0.0, f[q]));  // initflow 'zer'
+  FZ_TRY(fz_init2fl(c, &cs->prm, cs->sc[S_SEPS], f[F_PSI]));
+  cs->sc[S_TIME] = 0.0;
+  cs->sc[S_ISTEP] = 0.0;
+  return finish_init(cs);
+}
+
+static const char *kFieldExt[5] = {"u", "v", "w", "p", "psi"};
+static const int kFieldId[5] = {F_U, F_V, F_W, F_P, F_PSI};
+
+int fz_load_one(fz_ctx *c, char io, const char *filename, double *p, double *time, int *istep);
+
+extern "C" int fz_case_save(fz_case *cs, const char *prefix) {
+  if (!cs || !prefix) return fz_set_error(FZ_ERR_ARG, "fz_case_save: null argument");
+  double time = cs->sc[S_TIME];
+  int istep = (int)cs->sc[S_ISTEP];
+  for (int q = 0; q < 5; q++) {
+    const std::string name = std::string(prefix) + "_" + kFieldExt[q] + ".bin";
+    FZ_TRY(fz_load_one(cs->ctx, 'w', name.c_str(), cs->f[kFieldId[q]], &time, &istep));
+  }
+  return FZ_OK;
+}
+
+extern "C" int fz_case_restart(fz_case *cs, const char *prefix) {
+  if (!cs || !prefix) return fz_set_error(FZ_ERR_ARG, "fz_case_restart: null argument");
+  FZ_TRY(set_epsilon(cs));
+  double time = 0.0;
+  int istep = 0;
+  for (int q = 0; q < 5; q++) {
+    const std::string name = std::string(prefix) + "_" + kFieldExt[q] + ".bin";
+    FZ_TRY(fz_load_one(cs->ctx, 'r', name.c_str(), cs->f[kFieldId[q]], &time, &istep));
+  }
+  cs->sc[S_TIME] = time;
+  cs->sc[S_ISTEP] = (double)istep;
+  return finish_init(cs);
 }
 
 template <typename T>
@@ -329,16 +373,17 @@ extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
       if (P.is_forced_hit) FZ_TRY(fz_lscale_forcing(c, 2, 0.5, dt, f[F_U], f[F_V], f[F_W]));
       FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
       // projection on the correction buffer dp, src/main.f90:488-502
-      FZ_TRY(fz_fillps(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp));
-      FZ_TRY(fz_updt_rhs_b(c, cs->dp));
-      FZ_TRY(fz_solver_dev(c, cs->dp));
+      FZ_TRY(fz_projection_rhs_solve(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp));
       FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, cs->dp));
-      FZ_TRY(fz_correc(c, rho0, P.rho12, dt, cs->dp, f[F_PSI], f[F_U], f[F_V], f[F_W]));
-      FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 1, f[F_U], f[F_V], f[F_W]));
-      // updatep into the buffer of the previous pressure, then rotate
-      FZ_LAUNCH(c, "updatep_oop", k_updatep_oop, dim3((int)std::min<long>((c->nh + 255) / 256, 8192)), dim3(256), 0, c->nh, cs->dp, f[F_P], f[F_PO]);
-      FZ_LAUNCHED();
+      // correc into the spare velocity buffers, updatep into the buffer of the
+      // previous pressure and the speeds of acdi_set_gamma, in one pass
+      FZ_TRY(fz_launch_correc_step(c, rho0, dt, cs->dp, f[F_P], f[F_U], f[F_V], f[F_W], cs->un, cs->vn, cs->wn,
+                                   f[F_PO], P.vof ? nullptr : cs->gpart));
+      swap_ptr(f[F_U], cs->un);
+      swap_ptr(f[F_V], cs->vn);
+      swap_ptr(f[F_W], cs->wn);
       swap_ptr(f[F_P], f[F_PO]);  // p <- p + dp, po <- p of the step before (= pn)
+      FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 1, f[F_U], f[F_V], f[F_W]));
       FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));
     } else {
       FZ_HIP(hipMemcpyAsync(f[F_PO], f[F_P], sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
@@ -346,7 +391,10 @@ extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
     cs->sc[S_FIRST] = 0.0;
     cs->sc[S_DTO] = dt;  // dto = dt, src/main.f90:523
     if (!P.vof) {
-      FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
+      if (P.is_solve_ns)
+        FZ_TRY(fz_launch_gamma_fin(c, (int)fz_correc_step_blocks(c), cs->gpart, P.acdi_gam_factor, P.acdi_gam_min,
+                                   cs->dsc));
+      else FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
       cs->gam_on_device = true;
     }
     const int istep = (int)cs->sc[S_ISTEP];

@@ -1,5 +1,6 @@
 // Context, memory and grid set-up of libfizzy.
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 
@@ -70,6 +71,8 @@ extern "C" int fz_ctx_create(fz_ctx **out, const int n[3], const double l[3], co
   c->d.n3 = n[2];
   c->d.sy = n[0] + 2;
   c->d.sz = (long)(n[0] + 2) * (n[1] + 2);
+  c->d.tile_y = 8;
+  if (const char *e = getenv("FZ_TILE_Y")) c->d.tile_y = atoi(e);  // block-order tuning
   c->nh = c->d.sz * (n[2] + 2);
   c->ni = (long)n[0] * n[1] * n[2];
   for (int d = 0; d < 3; d++) {
@@ -128,6 +131,7 @@ extern "C" int fz_ctx_destroy(fz_ctx *c) {
   for (int q = 0; q < 6; q++)
     if (dev[q]) (void)hipFree(dev[q]);
   if (c->pois) fz_poisson_destroy(c->pois);
+  fz_bcmaps_free(c);
   for (const fz_prof_rec &r : c->prof_recs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);

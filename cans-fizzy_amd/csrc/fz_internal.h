@@ -4,6 +4,8 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "../../include/fizzy.h"
@@ -30,6 +32,7 @@ int fz_set_error(int code, const char *fmt, ...);
 struct fz_dims {
   int n1, n2, n3;
   long sy, sz;
+  int tile_y;  // rows of blocks per z-marching pencil in the stencil block order (0: plain order)
 };
 
 // Batched line transform plan along one axis (r2r kinds as FFTW's).
@@ -54,6 +57,7 @@ struct fz_poisson {
   double *piv = nullptr;    // Thomas pivots, n1*n2*n3
   double *aux = nullptr;    // periodic-z second solve, n1*n2*n3
   int periodic_z = 0;
+  int rhsb_zero = 0;  // all boundary right-hand sides vanish (homogeneous pressure BCs)
   char cbc[6];
 };
 
@@ -80,6 +84,7 @@ struct fz_ctx {
   double *red = nullptr;      // per-block partial results of reductions
   size_t red_cap = 0;
   double *h_small = nullptr;  // pinned host landing zone for reduction results
+  std::map<std::string, struct fz_bcmap *> bcmaps;  // composed boundary maps by signature
 };
 
 // brackets one launch with events when profiling is on
@@ -104,6 +109,29 @@ __device__ inline unsigned fz_xcd_block(unsigned P, unsigned nb) {
   return (nb & 7u) ? P : (P & 7u) * (nb >> 3) + (P >> 3);
 }
 
+// Block coordinates of a 3-D stencil grid (x blocks, y blocks, one z plane
+// per block) in pencil order: after the XCD remap, consecutive blocks cover
+// one pencil of ty rows of blocks (full x extent) and march it through z, so
+// the k-1/k/k+1 planes an XCD's resident blocks read stay in its L2.
+__device__ inline void fz_stencil_block(unsigned ty, unsigned &bx, unsigned &by, unsigned &bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const unsigned nb = gx * gy * gz;
+  const unsigned L = fz_xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+  if (ty == 0 || gy % ty != 0) {
+    bz = L / (gx * gy);
+    by = (L - bz * gx * gy) / gx;
+    bx = L - bz * gx * gy - by * gx;
+    return;
+  }
+  const unsigned per_plane = gx * ty;      // blocks of one pencil in one plane
+  const unsigned pencil = L / (per_plane * gz);
+  const unsigned rem = L - pencil * per_plane * gz;
+  bz = rem / per_plane;
+  const unsigned in = rem - bz * per_plane;
+  by = pencil * ty + in / gx;
+  bx = in - (in / gx) * gx;
+}
+
 // ---- launch helpers ----
 static inline dim3 fz_block3() { return dim3(64, 4, 1); }
 static inline dim3 fz_grid_interior(const fz_dims &d) {
@@ -115,8 +143,11 @@ static inline dim3 fz_grid_halo(const fz_dims &d) {
 
 // internal entry points shared between translation units
 int fz_reduce_ensure(fz_ctx *ctx, size_t nblocks);
+void fz_bcmaps_free(fz_ctx *ctx);
 int fz_poisson_destroy(fz_poisson *p);
 int fz_solver_dev(fz_ctx *ctx, double *p);
+int fz_projection_rhs_solve(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v, const double *w,
+                            double *p);
 int fz_acdi_gamma_dev(fz_ctx *ctx, double gam_factor, double gam_min, const double *u, const double *v,
                       const double *w, double *d_gam);
 int fz_acdi_transport_dev(fz_ctx *ctx, const double *d_gam, double gam, double seps, const double *u,

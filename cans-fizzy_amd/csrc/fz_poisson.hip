@@ -22,6 +22,15 @@ struct RadList {
 struct LineMap {  // element e of line (l1,l2) sits at base + l1*s1 + l2*s2 + e*se
   long base, s1, s2, se;
 };
+// fillps (src/fillps.f90:33-50) evaluated in the load of the forward x
+// transform: the line element is the divergence at the cell instead of a
+// value read from memory (u, v, w halo'd fields; k = l2 + 1)
+struct FillSrc {
+  const double *u, *v, *w, *dzfi;
+  double rho0, dti, dtidxi, dtidyi;
+  long sy, sz;
+  int on;
+};
 
 static __host__ __device__ inline bool kind_is_quarter(int kind) { return kind == K_REDFT11 || kind == K_RODFT11; }
 static __host__ __device__ inline bool kind_is_forward(int kind) {
@@ -284,7 +293,7 @@ template <int M, bool QUARTER>
 __global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, LineMap in, double *__restrict__ dst,
                                                 LineMap out, int L1, int kind, double scale,
                                                 const double2 *__restrict__ tw, const double2 *__restrict__ q,
-                                                const double2 *__restrict__ t2) {
+                                                const double2 *__restrict__ t2, FillSrc fs) {
   using S = P2Shape<M, QUARTER>;
   constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4;
   extern __shared__ double2 smem[];
@@ -300,7 +309,18 @@ __global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, 
     int g, e;
     if (in.se == 1) { g = idx / N; e = idx - g * N; }
     else { e = idx / G; g = idx - e * G; }
-    Rw[g * N + e] = (g < gcount) ? src[in.base + (long)(l1_0 + g) * in.s1 + l2 * in.s2 + (long)e * in.se] : 0.0;
+    double val = 0.0;
+    if (g < gcount) {
+      const long a = in.base + (long)(l1_0 + g) * in.s1 + l2 * in.s2 + (long)e * in.se;
+      if (fs.on) {
+        const int k = (int)l2 + 1;
+        val = fs.rho0 * ((fs.w[a] - fs.w[a - fs.sz]) * fs.dti * fs.dzfi[k] + (fs.v[a] - fs.v[a - fs.sy]) * fs.dtidyi +
+                         (fs.u[a] - fs.u[a - 1]) * fs.dtidxi);
+      } else {
+        val = src[a];
+      }
+    }
+    Rw[g * N + e] = val;
   }
   __syncthreads();
   const int f = tid / Q4, b = tid - f * Q4;
@@ -386,25 +406,27 @@ __global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, 
 
 template <int M, bool QUARTER>
 static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
-                      double *dst, LineMap out, int L1, int L2, double scale) {
+                      double *dst, LineMap out, int L1, int L2, double scale, const FillSrc &fs) {
   using S = P2Shape<M, QUARTER>;
   dim3 grid((L1 + S::G - 1) / S::G, L2, 1);
   FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind, scale, P.tw,
-            P.q, P.t2);
+            P.q, P.t2, fs);
 }
+
+static bool p2_length(int m) { return m >= 16 && m <= 2048 && (m & (m - 1)) == 0; }
 
 template <bool QUARTER>
 static bool dispatch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
-                        double *dst, LineMap out, int L1, int L2, double scale) {
+                        double *dst, LineMap out, int L1, int L2, double scale, const FillSrc &fs) {
   switch (P.m) {
-    case 16: launch_p2<16, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
-    case 32: launch_p2<32, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
-    case 64: launch_p2<64, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
-    case 128: launch_p2<128, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
-    case 256: launch_p2<256, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
-    case 512: launch_p2<512, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
-    case 1024: launch_p2<1024, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
-    case 2048: launch_p2<2048, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale); return true;
+    case 16: launch_p2<16, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
+    case 32: launch_p2<32, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
+    case 64: launch_p2<64, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
+    case 128: launch_p2<128, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
+    case 256: launch_p2<256, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
+    case 512: launch_p2<512, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
+    case 1024: launch_p2<1024, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
+    case 2048: launch_p2<2048, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
     default: return false;
   }
 }
@@ -670,12 +692,17 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
     fz_poisson_destroy(ctx->pois);
     ctx->pois = nullptr;
   }
+  bool zero_b = true;
+  for (double x : rbx) zero_b = zero_b && (x == 0.0);
+  for (double x : rby) zero_b = zero_b && (x == 0.0);
+  for (double x : rbz) zero_b = zero_b && (x == 0.0);
   if (cbc[4] == 'P' && cbc[5] == 'P' && n3 < 2)
     return fz_set_error(FZ_ERR_UNSUPPORTED, "fz_initsolver: periodic z needs n3 >= 2 (gaussel_periodic)");
   fz_poisson *P = new fz_poisson();
   memcpy(P->cbc, cbc, 6);
   P->normfft = normfft;
   P->periodic_z = (cbc[4] == 'P' && cbc[5] == 'P');
+  P->rhsb_zero = zero_b ? 1 : 0;
   int rc = fftplan_init(&P->px, cbc[0], cbc[1], n1);
   if (rc == FZ_OK) rc = fftplan_init(&P->py, cbc[2], cbc[3], n2);
   if (rc != FZ_OK) {
@@ -707,9 +734,11 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
 static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
                       double *dst, LineMap out, int L1, int L2, double scale) {
   const bool quarter = kind_is_quarter(kind);
-  if (P.m >= 16 && P.m <= 2048 && (P.m & (P.m - 1)) == 0) {
-    const bool done = quarter ? dispatch_p2<true>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale)
-                              : dispatch_p2<false>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale);
+  if (p2_length(P.m)) {
+    FillSrc off;
+    memset(&off, 0, sizeof(off));
+    const bool done = quarter ? dispatch_p2<true>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, off)
+                              : dispatch_p2<false>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, off);
     if (done) {
       FZ_LAUNCHED();
       return FZ_OK;
@@ -738,17 +767,69 @@ static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int ki
   return FZ_OK;
 }
 
+// everything after the forward x transform: y transform, z solve, inverses
+static int solver_rest(fz_ctx *ctx, double *p);
+
 int fz_solver_dev(fz_ctx *ctx, double *p) {
   fz_poisson *P = ctx->pois;
   if (!P) return fz_set_error(FZ_ERR_STATE, "fz_solver: fz_initsolver has not been called");
   const int n1 = ctx->d.n1, n2 = ctx->d.n2, n3 = ctx->d.n3;
   const long sy = ctx->d.sy, sz = ctx->d.sz;
   const long ps = (long)n1 * n2;
-  double *W = P->work;
   const LineMap p_x = {1 + sy + sz, sy, sz, 1};  // x-lines of the halo'd field, (j,k) -> line
   const LineMap w_x = {0, n1, ps, 1};
+  FZ_TRY(launch_r2r(ctx, "fft_x_fwd", P->px, P->px.kind_f, p, p_x, P->work, w_x, n2, n3, 1.0));
+  return solver_rest(ctx, p);
+}
+
+int fz_fillps(fz_ctx *c, double dti, double rho0, const double *u, const double *v, const double *w, double *p);
+int fz_updt_rhs_b(fz_ctx *c, double *p);
+
+// fillps + updt_rhs_b + solver (src/main.f90:492-496). When every boundary
+// right-hand side is zero, updt_rhs_b adds exact zeros and is skipped, and
+// fillps is evaluated inside the load of the forward x transform (power-of-two
+// x lengths); p then receives the correction pressure only. Otherwise the
+// three routines run as they are.
+int fz_projection_rhs_solve(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v, const double *w,
+                            double *p) {
+  fz_poisson *P = ctx->pois;
+  if (!P) return fz_set_error(FZ_ERR_STATE, "fz_solver: fz_initsolver has not been called");
+  if (!P->rhsb_zero || !p2_length(P->px.m) || kind_is_quarter(P->px.kind_f)) {
+    FZ_TRY(fz_fillps(ctx, dti, rho0, u, v, w, p));
+    FZ_TRY(fz_updt_rhs_b(ctx, p));
+    return fz_solver_dev(ctx, p);
+  }
+  const int n1 = ctx->d.n1, n2 = ctx->d.n2, n3 = ctx->d.n3;
+  const long sy = ctx->d.sy, sz = ctx->d.sz;
+  const long ps = (long)n1 * n2;
+  const LineMap p_x = {1 + sy + sz, sy, sz, 1};
+  const LineMap w_x = {0, n1, ps, 1};
+  FillSrc fs;
+  fs.u = u;
+  fs.v = v;
+  fs.w = w;
+  fs.dzfi = ctx->dzfi;
+  fs.rho0 = rho0;
+  fs.dti = dti;
+  fs.dtidxi = dti * ctx->dli[0];  // the scalars of fz_fillps
+  fs.dtidyi = dti * ctx->dli[1];
+  fs.sy = sy;
+  fs.sz = sz;
+  fs.on = 1;
+  dispatch_p2<false>(ctx, "fillps_fft_x_fwd", P->px, P->px.kind_f, nullptr, p_x, P->work, w_x, n2, n3, 1.0, fs);
+  FZ_LAUNCHED();
+  return solver_rest(ctx, p);
+}
+
+static int solver_rest(fz_ctx *ctx, double *p) {
+  fz_poisson *P = ctx->pois;
+  const int n1 = ctx->d.n1, n2 = ctx->d.n2, n3 = ctx->d.n3;
+  const long sy = ctx->d.sy, sz = ctx->d.sz;
+  const long ps = (long)n1 * n2;
+  double *W = P->work;
+  const LineMap p_x = {1 + sy + sz, sy, sz, 1};
+  const LineMap w_x = {0, n1, ps, 1};
   const LineMap w_y = {0, 1, ps, n1};
-  FZ_TRY(launch_r2r(ctx, "fft_x_fwd", P->px, P->px.kind_f, p, p_x, W, w_x, n2, n3, 1.0));
   FZ_TRY(launch_r2r(ctx, "fft_y_fwd", P->py, P->py.kind_f, W, w_y, W, w_y, n1, n3, 1.0));
   const int tpb = 256;
   const int nb = (int)((ps + tpb - 1) / tpb);

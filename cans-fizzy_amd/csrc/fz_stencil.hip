@@ -13,20 +13,13 @@
 #include "fz_internal.h"
 #include "fz_math.h"
 
-// block coordinates after the XCD remap (x fastest, then y, then z)
-#define FZ_BLOCK_XYZ(bx, by, bz)                                                        \
-  unsigned bx, by, bz;                                                                  \
-  {                                                                                     \
-    const unsigned gxy_ = gridDim.x * gridDim.y;                                        \
-    const unsigned L_ = fz_xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), \
-                                     gxy_ * gridDim.z);                                 \
-    bz = L_ / gxy_;                                                                     \
-    by = (L_ - bz * gxy_) / gridDim.x;                                                  \
-    bx = L_ - bz * gxy_ - by * gridDim.x;                                               \
-  }
+// block coordinates in the XCD-aware pencil order
+#define FZ_BLOCK_XYZ(d, bx, by, bz) \
+  unsigned bx, by, bz;              \
+  fz_stencil_block((d).tile_y, bx, by, bz);
 
 #define CELL_IJK(d)                                   \
-  FZ_BLOCK_XYZ(bx_, by_, bz_);                        \
+  FZ_BLOCK_XYZ(d, bx_, by_, bz_);                     \
   const int i = bx_ * blockDim.x + threadIdx.x + 1;   \
   const int j = by_ * blockDim.y + threadIdx.y + 1;   \
   const int k = bz_ + 1;                              \
@@ -35,7 +28,7 @@
   const long r = (i - 1) + (long)(d).n1 * ((j - 1) + (long)(d).n2 * (k - 1));
 
 #define HALO_IJK(d)                                   \
-  FZ_BLOCK_XYZ(bx_, by_, bz_);                        \
+  FZ_BLOCK_XYZ(d, bx_, by_, bz_);                     \
   const int i = bx_ * blockDim.x + threadIdx.x;       \
   const int j = by_ * blockDim.y + threadIdx.y;       \
   const int k = bz_;                                  \
@@ -61,6 +54,96 @@ __global__ void k_correc(fz_dims d, const double *__restrict__ dzci, double fact
   if (i <= d.n1) u[q] = u[q] - factori * (p[q + 1] - p[q]) / rhop;
   if (j <= d.n2) v[q] = v[q] - factorj * (p[q + d.sy] - p[q]) / rhop;
   if (k <= d.n3) w[q] = w[q] - dt * dzci[k] * (p[q + d.sz] - p[q]) / rhop;
+}
+
+// The end of the projection in one pass over the halo-inclusive grid:
+// correc (src/correc.f90:34-66) into the spare velocity buffers un/vn/wn,
+// updatep (src/updatep.f90:21-23) into the buffer of the previous pressure,
+// and the cell-centred speeds of acdi_set_gamma (src/acdi.f90:32-62) over
+// the interior. Faces that correc leaves alone (u at i = n1+1, v at
+// j = n2+1, w at k = n3+1) are carried over. An interior thread recomputes
+// the corrected velocity of its lower faces with the same expression as
+// the thread that owns them, so the speeds are those of the corrected
+// field bit for bit; each block writes the maximum squared speed of its
+// cells to part[block] (fmax skips NaN speeds, as the reduction kernel
+// does), and k_gamma_fin reduces the partials.
+__global__ void __launch_bounds__(256) k_correc_step(fz_dims d, const double *__restrict__ dzci, double factori,
+                                                     double factorj, double dt, double rhop,
+                                                     const double *__restrict__ dp, const double *__restrict__ p,
+                                                     const double *__restrict__ u, const double *__restrict__ v,
+                                                     const double *__restrict__ w, double *__restrict__ un,
+                                                     double *__restrict__ vn, double *__restrict__ wn,
+                                                     double *__restrict__ p_out, double *__restrict__ part) {
+  FZ_BLOCK_XYZ(d, bx, by, bz);
+  const int i = bx * blockDim.x + threadIdx.x, j = by * blockDim.y + threadIdx.y, k = bz;
+  double m = 0.0;
+  if (i <= d.n1 + 1 && j <= d.n2 + 1) {
+    const long sy = d.sy, sz = d.sz;
+    const long q = i + sy * j + sz * k;
+    const double uq = (i <= d.n1) ? u[q] - factori * (dp[q + 1] - dp[q]) / rhop : u[q];
+    const double vq = (j <= d.n2) ? v[q] - factorj * (dp[q + sy] - dp[q]) / rhop : v[q];
+    const double wq = (k <= d.n3) ? w[q] - dt * dzci[k] * (dp[q + sz] - dp[q]) / rhop : w[q];
+    un[q] = uq;
+    vn[q] = vq;
+    wn[q] = wq;
+    p_out[q] = dp[q] + p[q];
+    if (part && i >= 1 && i <= d.n1 && j >= 1 && j <= d.n2 && k >= 1 && k <= d.n3) {
+      const double um = u[q - 1] - factori * (dp[q] - dp[q - 1]) / rhop;
+      const double vm = v[q - sy] - factorj * (dp[q] - dp[q - sy]) / rhop;
+      const double wm = w[q - sz] - dt * dzci[k - 1] * (dp[q] - dp[q - sz]) / rhop;
+      const double uc = 0.5 * (uq + um), vc = 0.5 * (vq + vm), wc = 0.5 * (wq + wm);
+      m = fmax(m, uc * uc + vc * vc + wc * wc);
+    }
+  }
+  if (!part) return;  // uniform across the grid
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  __shared__ double sh[16];
+  const int t = threadIdx.x + blockDim.x * threadIdx.y;
+  const int nw = (int)((blockDim.x * blockDim.y + 63) >> 6);
+  if ((t & 63) == 0) sh[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    double b = sh[0];
+    for (int q = 1; q < nw; q++) b = fmax(b, sh[q]);
+    part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = b;
+  }
+}
+
+// gam = max(sqrt(velmax)*gam_factor, gam_min) (src/acdi.f90:57-60, src/main.f90:526-527)
+// from the per-block maxima of k_correc_step
+__global__ void __launch_bounds__(1024) k_gamma_fin(int nb, const double *__restrict__ part, double factor,
+                                                    double gmin, double *__restrict__ out) {
+  double m = 0.0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) m = fmax(m, part[q]);
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  __shared__ double sh[16];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmax(m, sh[q]);
+    out[0] = fmax(sqrt(m) * factor, gmin);
+  }
+}
+
+// number of partials k_correc_step writes
+long fz_correc_step_blocks(const fz_ctx *c) {
+  const dim3 g = fz_grid_halo(c->d);
+  return (long)g.x * g.y * g.z;
+}
+
+int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *dp, const double *p, const double *u,
+                          const double *v, const double *w, double *un, double *vn, double *wn, double *p_out,
+                          double *part) {
+  FZ_LAUNCH(c, "correc_step", k_correc_step, fz_grid_halo(c->d), fz_block3(), 0, c->d, c->dzci, dt * c->dli[0],
+            dt * c->dli[1], dt, rho0, dp, p, u, v, w, un, vn, wn, p_out, part);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+int fz_launch_gamma_fin(fz_ctx *c, int nb, const double *part, double factor, double gmin, double *out) {
+  FZ_LAUNCH(c, "gamma_fin", k_gamma_fin, dim3(1), dim3(1024), 0, nb, part, factor, gmin, out);
+  FZ_LAUNCHED();
+  return FZ_OK;
 }
 
 // updatep, src/updatep.f90:21-23, whole array

@@ -87,6 +87,11 @@ SIGNATURES = {
     "fz_case_set_scalars": (C.c_int, [vp, dp]),
     "fz_case_init": (C.c_int, [vp]),
     "fz_case_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+    "fz_case_save": (C.c_int, [vp, C.c_char_p]),
+    "fz_case_restart": (C.c_int, [vp, C.c_char_p]),
+    "fz_load_one": (C.c_int, [vp, C.c_char, C.c_char_p, vp, dp, C.POINTER(C.c_int)]),
+    "fz_out3d": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_int), vp]),
+    "fz_write_grid": (C.c_int, [vp, C.c_char_p]),
 }
 
 _lib = None

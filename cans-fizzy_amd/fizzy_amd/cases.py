@@ -242,3 +242,18 @@ class DeviceCase:
 
     def sync(self):
         call("fz_ctx_sync", self.ctx)
+
+    def save(self, prefix):
+        """Checkpoint: prefix_u.bin, _v, _w, _p, _psi in the load_one format."""
+        call("fz_case_save", self._h, prefix.encode())
+
+    def restart(self, prefix):
+        """Restart from save()'s files (src/main.f90:312-407 with restart = T)."""
+        call("fz_case_restart", self._h, prefix.encode())
+
+    def write_grid(self, datadir):
+        call("fz_write_grid", self.ctx, datadir.encode())
+
+    def out3d(self, filename, name, nskip=(1, 1, 1)):
+        ptr, _ = self.field_ptr(name)
+        call("fz_out3d", self.ctx, filename.encode(), (C.c_int * 3)(*nskip), vp(ptr))

@@ -226,5 +226,19 @@ module mod_fz_capi
       real(c_double), value :: sigma, gam, seps
       real(c_double), intent(out) :: dtmax
     end function
+    integer(c_int) function fz_load_one(ctx, io, filename, p, time, istep) bind(C, name='fz_load_one')
+      import :: c_int, c_double, c_char, c_ptr
+      type(c_ptr), value :: ctx, p
+      character(kind=c_char), value :: io
+      character(kind=c_char), intent(in) :: filename(*)
+      real(c_double), intent(inout) :: time
+      integer(c_int), intent(inout) :: istep
+    end function
+    integer(c_int) function fz_out3d(ctx, filename, nskip, p) bind(C, name='fz_out3d')
+      import :: c_int, c_char, c_ptr
+      type(c_ptr), value :: ctx, p
+      character(kind=c_char), intent(in) :: filename(*)
+      integer(c_int), intent(in) :: nskip(3)
+    end function
   end interface
 end module mod_fz_capi

@@ -679,3 +679,74 @@ contains
     arr2 => tmp
   end subroutine swap
 end module mod_utils
+
+module mod_load
+  use, intrinsic :: iso_c_binding
+  use mod_types
+  use mod_fz
+  use mod_fz_capi, only: fz_load_one
+  implicit none
+  private
+  public load_one
+contains
+  ! src/load.f90:19-116: one restart file (the interior, then [time, istep]);
+  ! comm is accepted for the interface and unused (one device holds the field)
+  subroutine load_one(io, filename, comm, ng, nh, lo, hi, p, time, istep)
+    character(len=1), intent(in) :: io
+    character(len=*), intent(in) :: filename
+    integer         , intent(in) :: comm
+    integer , intent(in), dimension(3) :: ng, nh, lo, hi
+    real(rp), intent(inout), dimension(lo(1)-nh(1):,lo(2)-nh(2):,lo(3)-nh(3):), target, contiguous :: p
+    real(rp), intent(inout) :: time
+    integer , intent(inout) :: istep
+    integer(c_int) :: is
+    if (any(ng /= fz_ng()) .or. any(nh /= 1)) error stop 'load_one: grid or halo differs from fz_setup'
+    is = int(istep, c_int)
+    call fz_check(fz_load_one(fz_ctx_ptr(), io, trim(filename)//c_null_char, c_loc(p), time, is), 'load_one')
+    istep = int(is)
+  end subroutine load_one
+end module mod_load
+
+module mod_output
+  use, intrinsic :: iso_c_binding
+  use mod_types
+  use mod_fz
+  use mod_fz_capi, only: fz_out3d
+  implicit none
+  private
+  public out3d, write_log_output, write_visu_3d
+contains
+  ! src/output.f90:185-236: the interior every nskip points, no header; p is
+  ! the halo'd device field
+  subroutine out3d(fname, nskip, p)
+    character(len=*), intent(in) :: fname
+    integer , intent(in), dimension(3) :: nskip
+    real(rp), intent(in), dimension(0:,0:,0:), target, contiguous :: p
+    call fz_check(fz_out3d(fz_ctx_ptr(), trim(fname)//c_null_char, int(nskip, c_int), c_loc(p)), 'out3d')
+  end subroutine out3d
+
+  ! src/output.f90:238-266: one line per saved file, for the xdmf generator
+  subroutine write_log_output(fname, fname_fld, varname, nmin, nmax, nskip, time, istep)
+    character(len=*), intent(in) :: fname, fname_fld, varname
+    integer , intent(in), dimension(3) :: nmin, nmax, nskip
+    real(rp), intent(in)               :: time
+    integer , intent(in)               :: istep
+    integer :: iunit
+    open(newunit=iunit, file=fname, position='append')
+    write(iunit, '(A,A,A,9i5,E16.7e3,i8)') trim(fname_fld), ' ', trim(varname), nmin, nmax, nskip, time, istep
+    close(iunit)
+  end subroutine write_log_output
+
+  ! src/output.f90:268-281; p is the whole halo'd device field (the
+  ! reference passes the interior section of its host copy)
+  subroutine write_visu_3d(datadir, fname_bin, fname_log, varname, nmin, nmax, nskip, time, istep, p)
+    character(len=*), intent(in)       :: datadir, fname_bin, fname_log, varname
+    integer , intent(in), dimension(3) :: nmin, nmax, nskip
+    real(rp), intent(in)               :: time
+    integer , intent(in)               :: istep
+    real(rp), intent(in), dimension(0:,0:,0:), target, contiguous :: p
+    call out3d(trim(datadir)//trim(fname_bin), nskip, p)
+    call write_log_output(trim(datadir)//trim(fname_log), trim(fname_bin), trim(varname), nmin, nmax, nskip, time, &
+                          istep)
+  end subroutine write_visu_3d
+end module mod_output

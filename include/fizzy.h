@@ -191,6 +191,18 @@ int fz_chkdt(fz_ctx *ctx, int is_solve_ns, int is_track_interface, int vof, cons
              const double rho12[2], double sigma, const double gacc[3], const double *u, const double *v,
              const double *w, double gam, double seps, double *dtmax);
 
+/* ---- field files ---- */
+/* load_one / io_field, src/load.f90:19-168: io 'w' writes the interior of the
+ * halo'd device field p followed by [time, istep]; io 'r' reads such a file
+ * into the interior of p (a file of the wrong size is refused, as the
+ * reference aborts on it, src/load.f90:41-51) */
+int fz_load_one(fz_ctx *ctx, char io, const char *filename, double *p, double *time, int *istep);
+/* out3d, src/output.f90:185-236: the interior every nskip points from index 1 */
+int fz_out3d(fz_ctx *ctx, const char *filename, const int nskip[3], const double *p);
+/* grid.bin, grid.out and geometry.out of src/main.f90:230-243 into datadir
+ * (a prefix, e.g. "data/") */
+int fz_write_grid(fz_ctx *ctx, const char *datadir);
+
 /* ---- a whole case: the time loop of src/main.f90:414-562 on the device ---- */
 /* field ids: 0 u,1 v,2 w,3 p,4 pn,5 po,6 psi,7 phi,8 kappa,9 normx,10 normy,11 normz,
  * 12 psiflx_x,13 psiflx_y,14 psiflx_z (halo'd); 15 dpsidt_old,16 dudt_old,17 dvdt_old,
@@ -208,6 +220,13 @@ int fz_case_init(fz_case *cs);
 /* nsteps time steps; *killed = 1 when a divergence or time-step check
  * stopped the run as src/main.f90:530-562 does */
 int fz_case_step(fz_case *cs, int nsteps, int *killed);
+/* the checkpoint of src/main.f90:597-631: prefix_u.bin, _v, _w, _p, _psi
+ * in the load_one format, with the case's time and step */
+int fz_case_save(fz_case *cs, const char *prefix);
+/* restart, src/main.f90:312-407 with restart = T: the five fields, time and
+ * step from such files, then the same boundary conditions, phi/normals,
+ * gamma and dt as a fresh start; the AB2 histories start over */
+int fz_case_restart(fz_case *cs, const char *prefix);
 
 #ifdef __cplusplus
 }

@@ -1483,6 +1483,18 @@ void or_case_init_fields(or_case *c) {
   /* initflow 'zer', src/initflow.f90:69-70 */
   for (int q = 0; q < 4; q++) memset(f[q], 0, sizeof(double) * NH);
   or_init2fl(P, c->sc[4], c->zc, f[6]);
+  c->sc[0] = 0.0;
+  c->sc[9] = 0.0;
+  or_case_finish_init(c);
+}
+
+/* the initialisation after the fields are set (fresh start or restart,
+ * src/main.f90:333-407): BCs, pn = po = p, phi/normals, gamma, dt */
+void or_case_finish_init(or_case *c) {
+  const or_params *P = &c->prm;
+  double **f = c->f;
+  const long NH = (long)(c->n[0] + 2) * (c->n[1] + 2) * (c->n[2] + 2);
+  c->sc[4] = or_acdi_set_epsilon(c->dl, c->dzfi, c->n[2], P->psi_thickness_factor);
   or_bounduvw(c->n, P->cbcpre, P->cbcvel, P->bcvel, 0, c->dl, c->dzc, c->dzf, f[0], f[1], f[2]);
   or_boundp(c->n, P->cbcpre, P->cbcpre, P->bcpre, c->dl, c->dzc, f[3]);
   memcpy(f[4], f[3], sizeof(double) * NH);
@@ -1494,10 +1506,8 @@ void or_case_init_fields(or_case *c) {
     c->sc[3] = dmax(g, P->acdi_gam_min);
   }
   const double dt = new_dt(c, NULL);
-  c->sc[0] = 0.0;
   c->sc[1] = dt;
   c->sc[2] = dt;
-  c->sc[9] = 0.0;
   c->sc[10] = 1.0;
 }
 

@@ -147,6 +147,7 @@ double *or_case_field(or_case *c, int id);
 /* scalars: 0 time,1 dt,2 dto,3 gam,4 seps,5 rho0,6 divtot,7 divmax,8 dt_cfl,9 istep,10 is_first */
 double *or_case_scalars(or_case *c);
 void or_case_init_fields(or_case *c);       /* initflow 'zer' + init2fl + BCs + phi/normals + gamma + dt */
+void or_case_finish_init(or_case *c);       /* the same from the fields as they are (restart) */
 int or_case_step(or_case *c, int nsteps);   /* returns 0, or 1 when a divergence/dt check killed the run */
 
 #ifdef __cplusplus

@@ -226,5 +226,9 @@ class Case:
     def init_fields(self):
         lib().or_case_init_fields(C.c_void_p(self._h))
 
+    def finish_init(self):
+        """The initialisation after the fields are set, as on a restart."""
+        lib().or_case_finish_init(C.c_void_p(self._h))
+
     def step(self, nsteps=1):
         return lib().or_case_step(C.c_void_p(self._h), C.c_int(nsteps))
