@@ -50,6 +50,25 @@ HALO_KERNELS = {"acdi_phi", "clip", "updatep_oop", "vof_phi", "correc", "correc_
 STEP_ALG_BYTES_PER_CELL = 488
 
 
+WORKLOAD_NAMES = {"c2": "acdi_rising_bubble_256x256x512", "c0": "acdi_rising_bubble_64x64x128"}
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"step_kernels_{WORKLOAD_NAMES[workload]}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = line.split()
+        if len(cols) >= 6 and cols[0] == "k_" + kernel:
+            try:
+                return round((float(cols[4]) + float(cols[5])) * 1e6)
+            except ValueError:
+                return None
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -114,12 +133,65 @@ def cpu_baseline(spec, budget_s=15.0, max_steps=20):
                       f"(OpenMP, {threads} threads), {dt:.1f} s"}
 
 
+def main_serving(args):
+    """The serving mix: Poisson arrivals (mean 50 ms) of 2-D/3-D rising bubbles of
+    uneven grids and density ratios, 200 steps each, admitted by HBM footprint
+    and stepped together; p50/p99 per-step latency, aggregate throughput, and
+    the same requests run one after another for comparison."""
+    import ctypes as C
+
+    import numpy as np
+
+    import fizzy_amd as F
+    from fizzy_amd import serve
+
+    reqs = serve.serving_requests(args.requests, mean_interarrival=0.05, nsteps=args.case_steps)
+    cells = [int(np.prod(spec["ng"])) * n for _, spec, n in reqs]
+    f, t = C.c_size_t(), C.c_size_t()
+    F.call("fz_device_memory", C.byref(f), C.byref(t))
+    srv = serve.Server(int(0.5 * f.value))
+    try:
+        wall, ids = serve.run_stream(srv, reqs)
+        lat = srv.step_times(0)
+    finally:
+        srv.close()
+    log(f"serving: {len(reqs)} requests, {sum(cells) / 1e6:.1f} Mcell-steps in {wall:.2f} s, "
+        f"p50 {np.percentile(lat, 50):.3f} ms, p99 {np.percentile(lat, 99):.3f} ms")
+    t0 = time.perf_counter()
+    for _, spec, n in reqs:  # sequential: one case at a time, same work
+        dc = F.DeviceCase(spec)
+        try:
+            dc.init()
+            dc.step(n)
+            dc.sync()
+        finally:
+            dc.close()
+    seq = time.perf_counter() - t0
+    thr, seq_thr = sum(cells) / wall / 1e6, sum(cells) / seq / 1e6
+    log(f"sequential: {seq:.2f} s ({seq_thr:.1f} Mcell-steps/s); served {thr:.1f} Mcell-steps/s")
+    out = {
+        "metric": "ensemble p50/p99 step latency",
+        "value": round(float(np.percentile(lat, 99)), 4), "unit": "ms (p99 per-case step)",
+        "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+        "throughput_Mcell_steps_s": round(thr, 2), "sequential_Mcell_steps_s": round(seq_thr, 2),
+        "speedup_vs_sequential": round(thr / seq_thr, 3), "wall_s": round(wall, 3), "n_gpus": 1,
+        "higher_is_better": False, "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "serving_mix", "requests": len(reqs), "steps_per_case": args.case_steps,
+                   "arrivals": "Poisson, mean 50 ms", "grids": "64^2, 128^2, 256^2 (2-D), 64^2x128, 128^2x256",
+                   "density_ratio": "uniform [2, 1000]"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["c2", "c0"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c0", "c4"], default="c2",
+                    help="c2: 256x256x512 bubble (default); c0: 64x64x128; c4: the serving mix")
+    ap.add_argument("--requests", type=int, default=24, help="c4: number of requests")
+    ap.add_argument("--case-steps", type=int, default=200, help="c4: steps per request")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-prof", action="store_true", help="time without per-launch events")
     args = ap.parse_args()
@@ -128,6 +200,9 @@ def main():
         print(f"bench.py measures one MI355X: this build runs the hot path on a single GPU "
               f"(asked for --gpus {args.gpus}, WORLD_SIZE {world}); nothing measured.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c4":
+        return main_serving(args)
 
     import fizzy_amd as F
     from fizzy_amd import cases
@@ -168,7 +243,8 @@ def main():
         avg_s = ms / calls / 1e3
         achieved = bytes_per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, args.workload),
+                "kernel": dom,
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bytes_per_launch,
                 "kernel_time_share": round(ms / total_ms, 4)}
     step_bw = STEP_ALG_BYTES_PER_CELL * cells / (elapsed / args.steps) / 1e9
@@ -191,8 +267,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: the reference's rising-bubble initial condition (sphere, fluid at rest)",
-        "config": {"workload": ("acdi_rising_bubble_256x256x512" if args.workload == "c2"
-                                else "acdi_rising_bubble_64x64x128"),
+        "config": {"workload": WORKLOAD_NAMES[args.workload],
                    "grid": n, "density_ratio": spec["rho12"][0] / spec["rho12"][1],
                    "viscosity_ratio": spec["mu12"][0] / spec["mu12"][1], "precision": "fp64",
                    "step": "AB2 (one reference time step)"},

@@ -31,9 +31,8 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
 int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
 int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *dp, const double *p, const double *u,
                           const double *v, const double *w, double *un, double *vn, double *wn, double *p_out,
-                          double *part);
-int fz_launch_gamma_fin(fz_ctx *c, int nb, const double *part, double factor, double gmin, double *out);
-long fz_correc_step_blocks(const fz_ctx *c);
+                          unsigned long long *slots);
+int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out);
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
                         int first, const double *u, const double *v, const double *w, const double *normx,
                         const double *normy, const double *normz, const double *phi, const double *psio,
@@ -68,7 +67,7 @@ struct fz_case {
   double *psio = nullptr, *dpsidt = nullptr, *dp = nullptr;
   double *un = nullptr, *vn = nullptr, *wn = nullptr;  // the next velocity (rotated with u, v, w)
   double *dsc = nullptr;  // device scalars: 0 gam, 1-2 div, 3 adv dt, 4-6 rho_av
-  double *gpart = nullptr;  // per-block maxima of the squared cell speed (gamma)
+  unsigned long long *gslots = nullptr;  // maxima of the squared cell speed (gamma), zero between steps
   double *gvr_c = nullptr, *gvr_f = nullptr;
   double sc[12] = {0};
   char cbccur[6];
@@ -94,7 +93,8 @@ static int case_alloc(fz_case *cs) {
     FZ_HIP(hipMalloc((void **)itr[q], sizeof(double) * c->ni));
     FZ_HIP(hipMemsetAsync(*itr[q], 0, sizeof(double) * c->ni, c->st));
   }
-  FZ_HIP(hipMalloc((void **)&cs->gpart, sizeof(double) * fz_correc_step_blocks(c)));
+  FZ_HIP(hipMalloc((void **)&cs->gslots, sizeof(unsigned long long) * FZ_GAMMA_SLOTS));
+  FZ_HIP(hipMemsetAsync(cs->gslots, 0, sizeof(unsigned long long) * FZ_GAMMA_SLOTS, c->st));
   FZ_HIP(hipMalloc((void **)&cs->dsc, sizeof(double) * 16));
   FZ_HIP(hipMemsetAsync(cs->dsc, 0, sizeof(double) * 16, c->st));
   const int nz = c->d.n3 + 2;
@@ -116,7 +116,8 @@ extern "C" int fz_case_destroy(fz_case *cs) {
   if (cs->ctx) (void)hipStreamSynchronize(cs->ctx->st);
   for (int q = 0; q < F_N; q++)
     if (cs->f[q]) (void)hipFree(cs->f[q]);
-  double *o[10] = {cs->psio, cs->dpsidt, cs->un, cs->vn, cs->wn, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f, cs->gpart};
+  double *o[10] = {cs->psio, cs->dpsidt, cs->un, cs->vn, cs->wn, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f,
+                   (double *)cs->gslots};
   for (int q = 0; q < 10; q++)
     if (o[q]) (void)hipFree(o[q]);
   if (cs->ctx) fz_ctx_destroy(cs->ctx);
@@ -378,7 +379,7 @@ extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
       // correc into the spare velocity buffers, updatep into the buffer of the
       // previous pressure and the speeds of acdi_set_gamma, in one pass
       FZ_TRY(fz_launch_correc_step(c, rho0, dt, cs->dp, f[F_P], f[F_U], f[F_V], f[F_W], cs->un, cs->vn, cs->wn,
-                                   f[F_PO], P.vof ? nullptr : cs->gpart));
+                                   f[F_PO], P.vof ? nullptr : cs->gslots));
       swap_ptr(f[F_U], cs->un);
       swap_ptr(f[F_V], cs->vn);
       swap_ptr(f[F_W], cs->wn);
@@ -392,8 +393,7 @@ extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
     cs->sc[S_DTO] = dt;  // dto = dt, src/main.f90:523
     if (!P.vof) {
       if (P.is_solve_ns)
-        FZ_TRY(fz_launch_gamma_fin(c, (int)fz_correc_step_blocks(c), cs->gpart, P.acdi_gam_factor, P.acdi_gam_min,
-                                   cs->dsc));
+        FZ_TRY(fz_launch_gamma_fin(c, cs->gslots, P.acdi_gam_factor, P.acdi_gam_min, cs->dsc));
       else FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
       cs->gam_on_device = true;
     }

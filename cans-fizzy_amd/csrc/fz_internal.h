@@ -11,6 +11,7 @@
 #include "../../include/fizzy.h"
 
 #define FZ_EPS 2.220446049250313080847e-16  // epsilon(1._rp)
+#define FZ_GAMMA_SLOTS 256                   // atomic-max slots of the fused gamma reduction
 #define FZ_PI 3.141592653589793115998        // acos(-1._rp)
 
 int fz_set_error(int code, const char *fmt, ...);

@@ -6,9 +6,10 @@
 // to an O(eps) change of phi, hence of the Youngs normals and curvature and
 // of the surface-tension force one step later. Two builds whose tanh/log
 // differ by one ulp therefore drift apart at O(1e-1) within a few steps. These
-// routines are written out with +,-,*,/ only (Cody-Waite reduction and
-// Taylor/atanh series, within about two ulps), so every build that uses
-// them rounds identically; the CPU oracle carries its own copy of the same
+// routines are written out with +,-,*,/ and fused multiply-add (Cody-Waite
+// reduction, Taylor/atanh series in Horner form, within about two ulps);
+// every operation is correctly rounded, so every build that uses them
+// rounds identically. The CPU oracle carries its own copy of the same
 // method, and tests pin both against the C library.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -38,65 +39,77 @@ constexpr double LN2_HI = 0.6931467056274414;      // ln 2 with 32 trailing zero
 constexpr double LN2_LO = 4.7493250390316726e-07;  // ln 2 - LN2_HI
 constexpr double INV_LN2 = 1.4426950408889634;
 
-// exp(r) - 1 for |r| <= ln2/2: Taylor series to r^14
+constexpr double LN2_HALF = 0.34657359027997264;  // ln 2 / 2
+
+// exp(r) - 1 for |r| <= ln2/2: Taylor series to r^14, Horner with fma
 FZ_HD double expm1_kernel(double r) {
   double p = 1.1470745597729725e-11;
-  p = 1.6059043836821613e-10 + r * p;
-  p = 2.08767569878681e-09 + r * p;
-  p = 2.505210838544172e-08 + r * p;
-  p = 2.755731922398589e-07 + r * p;
-  p = 2.7557319223985893e-06 + r * p;
-  p = 2.48015873015873e-05 + r * p;
-  p = 0.0001984126984126984 + r * p;
-  p = 0.001388888888888889 + r * p;
-  p = 0.008333333333333333 + r * p;
-  p = 0.041666666666666664 + r * p;
-  p = 0.16666666666666666 + r * p;
-  p = 0.5 + r * p;
-  return r + (r * r) * p;
+  p = __builtin_fma(r, p, 1.6059043836821613e-10);
+  p = __builtin_fma(r, p, 2.08767569878681e-09);
+  p = __builtin_fma(r, p, 2.505210838544172e-08);
+  p = __builtin_fma(r, p, 2.755731922398589e-07);
+  p = __builtin_fma(r, p, 2.7557319223985893e-06);
+  p = __builtin_fma(r, p, 2.48015873015873e-05);
+  p = __builtin_fma(r, p, 0.0001984126984126984);
+  p = __builtin_fma(r, p, 0.001388888888888889);
+  p = __builtin_fma(r, p, 0.008333333333333333);
+  p = __builtin_fma(r, p, 0.041666666666666664);
+  p = __builtin_fma(r, p, 0.16666666666666666);
+  p = __builtin_fma(r, p, 0.5);
+  return __builtin_fma(r * r, p, r);
 }
 
-/* exp(r) - 1 for |r| <= 1.05: Taylor series to r^20 */
+// exp(r) - 1 for |r| <= 1.05: Taylor series to r^20, Horner with fma (tanh's small arguments)
 FZ_HD double expm1_kernel20(double r) {
   double p = 4.110317623312165e-19;
-  p = 8.22063524662433e-18 + r * p;
-  p = 1.5619206968586225e-16 + r * p;
-  p = 2.8114572543455206e-15 + r * p;
-  p = 4.779477332387385e-14 + r * p;
-  p = 7.647163731819816e-13 + r * p;
-  p = 1.1470745597729725e-11 + r * p;
-  p = 1.6059043836821613e-10 + r * p;
-  p = 2.08767569878681e-09 + r * p;
-  p = 2.505210838544172e-08 + r * p;
-  p = 2.755731922398589e-07 + r * p;
-  p = 2.7557319223985893e-06 + r * p;
-  p = 2.48015873015873e-05 + r * p;
-  p = 0.0001984126984126984 + r * p;
-  p = 0.001388888888888889 + r * p;
-  p = 0.008333333333333333 + r * p;
-  p = 0.041666666666666664 + r * p;
-  p = 0.16666666666666666 + r * p;
-  p = 0.5 + r * p;
-  return r + (r * r) * p;
+  p = __builtin_fma(r, p, 8.22063524662433e-18);
+  p = __builtin_fma(r, p, 1.5619206968586225e-16);
+  p = __builtin_fma(r, p, 2.8114572543455206e-15);
+  p = __builtin_fma(r, p, 4.779477332387385e-14);
+  p = __builtin_fma(r, p, 7.647163731819816e-13);
+  p = __builtin_fma(r, p, 1.1470745597729725e-11);
+  p = __builtin_fma(r, p, 1.6059043836821613e-10);
+  p = __builtin_fma(r, p, 2.08767569878681e-09);
+  p = __builtin_fma(r, p, 2.505210838544172e-08);
+  p = __builtin_fma(r, p, 2.755731922398589e-07);
+  p = __builtin_fma(r, p, 2.7557319223985893e-06);
+  p = __builtin_fma(r, p, 2.48015873015873e-05);
+  p = __builtin_fma(r, p, 0.0001984126984126984);
+  p = __builtin_fma(r, p, 0.001388888888888889);
+  p = __builtin_fma(r, p, 0.008333333333333333);
+  p = __builtin_fma(r, p, 0.041666666666666664);
+  p = __builtin_fma(r, p, 0.16666666666666666);
+  p = __builtin_fma(r, p, 0.5);
+  return __builtin_fma(r * r, p, r);
+}
+
+// x = k ln2 + r, |r| <= ln2/2 (k*LN2_HI is exact)
+FZ_HD double reduce_ln2(double x, int *k) {
+  const double kf = __builtin_floor(x * INV_LN2 + 0.5);
+  *k = (int)kf;
+  return __builtin_fma(-kf, LN2_LO, x - kf * LN2_HI);
 }
 
 FZ_HD double exp(double x) {
   if (x != x) return x;
   if (x > 709.782712893384) return __builtin_huge_val();
   if (x < -745.1332191019412) return 0.0;
-  const double kf = __builtin_floor(x * INV_LN2 + 0.5);
-  const int k = (int)kf;
-  const double r = (x - kf * LN2_HI) - kf * LN2_LO;
+  int k;
+  const double r = reduce_ln2(x, &k);
   return scale2(1.0 + expm1_kernel(r), k);
 }
 
+// expm1(x) = 2^k (1 + expm1(r)) - 1 = 2^k expm1(r) + (2^k - 1); both terms
+// are exact scalings or exact differences, so one rounding remains
 FZ_HD double expm1(double x) {
   if (x != x) return x;
   const double a = x < 0.0 ? -x : x;
-  if (a <= 1.04) return expm1_kernel20(x);
+  if (a <= LN2_HALF) return expm1_kernel(x);
   if (x > 709.782712893384) return __builtin_huge_val();
   if (x < -40.0) return -1.0;
-  return exp(x) - 1.0;
+  int k;
+  const double e = expm1_kernel(reduce_ln2(x, &k));
+  return scale2(e, k) + (scale2(1.0, k) - 1.0);
 }
 
 // log(x): x = 2^k m, m in [sqrt(2)/2, sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1)
@@ -121,19 +134,19 @@ FZ_HD double log(double x) {
   const double s = f / (2.0 + f);
   const double z = s * s;
   double p = 0.08695652173913043;
-  p = 0.09523809523809523 + z * p;
-  p = 0.10526315789473684 + z * p;
-  p = 0.11764705882352941 + z * p;
-  p = 0.13333333333333333 + z * p;
-  p = 0.15384615384615385 + z * p;
-  p = 0.18181818181818182 + z * p;
-  p = 0.2222222222222222 + z * p;
-  p = 0.2857142857142857 + z * p;
-  p = 0.4 + z * p;
-  p = 0.6666666666666666 + z * p;
-  const double lm = 2.0 * s + (s * z) * p;
+  p = __builtin_fma(z, p, 0.09523809523809523);
+  p = __builtin_fma(z, p, 0.10526315789473684);
+  p = __builtin_fma(z, p, 0.11764705882352941);
+  p = __builtin_fma(z, p, 0.13333333333333333);
+  p = __builtin_fma(z, p, 0.15384615384615385);
+  p = __builtin_fma(z, p, 0.18181818181818182);
+  p = __builtin_fma(z, p, 0.2222222222222222);
+  p = __builtin_fma(z, p, 0.2857142857142857);
+  p = __builtin_fma(z, p, 0.4);
+  p = __builtin_fma(z, p, 0.6666666666666666);
+  const double lm = __builtin_fma(s * z, p, 2.0 * s);
   const double kd = (double)k;
-  return kd * LN2_HI + (lm + kd * LN2_LO);
+  return __builtin_fma(kd, LN2_HI, __builtin_fma(kd, LN2_LO, lm));
 }
 
 FZ_HD double tanh(double x) {
@@ -159,30 +172,30 @@ constexpr double TWO_OVER_PI = 0.6366197723675814;
 FZ_HD double sin_kernel(double r) {  // |r| <= pi/4
   const double z = r * r;
   double p = -7.647163731819816e-13;
-  p = 1.6059043836821613e-10 + z * p;
-  p = -2.505210838544172e-08 + z * p;
-  p = 2.7557319223985893e-06 + z * p;
-  p = -0.0001984126984126984 + z * p;
-  p = 0.008333333333333333 + z * p;
-  p = -0.16666666666666666 + z * p;
-  return r + (r * z) * p;
+  p = __builtin_fma(z, p, 1.6059043836821613e-10);
+  p = __builtin_fma(z, p, -2.505210838544172e-08);
+  p = __builtin_fma(z, p, 2.7557319223985893e-06);
+  p = __builtin_fma(z, p, -0.0001984126984126984);
+  p = __builtin_fma(z, p, 0.008333333333333333);
+  p = __builtin_fma(z, p, -0.16666666666666666);
+  return __builtin_fma(r * z, p, r);
 }
 FZ_HD double cos_kernel(double r) {
   const double z = r * r;
   double p = 4.779477332387385e-14;
-  p = -1.1470745597729725e-11 + z * p;
-  p = 2.08767569878681e-09 + z * p;
-  p = -2.755731922398589e-07 + z * p;
-  p = 2.48015873015873e-05 + z * p;
-  p = -0.001388888888888889 + z * p;
-  p = 0.041666666666666664 + z * p;
-  return (1.0 - 0.5 * z) + (z * z) * p;
+  p = __builtin_fma(z, p, -1.1470745597729725e-11);
+  p = __builtin_fma(z, p, 2.08767569878681e-09);
+  p = __builtin_fma(z, p, -2.755731922398589e-07);
+  p = __builtin_fma(z, p, 2.48015873015873e-05);
+  p = __builtin_fma(z, p, -0.001388888888888889);
+  p = __builtin_fma(z, p, 0.041666666666666664);
+  return __builtin_fma(z * z, p, 1.0 - 0.5 * z);
 }
-// quadrant reduction for |x| < 2^20
+// quadrant reduction for |x| < 2^20 (kf * PIO2_1 is exact)
 FZ_HD double reduce_pio2(double x, int *q) {
   const double kf = __builtin_floor(x * TWO_OVER_PI + 0.5);
   *q = ((int)kf) & 3;
-  return ((x - kf * PIO2_1) - kf * PIO2_2) - kf * PIO2_3;
+  return __builtin_fma(-kf, PIO2_3, __builtin_fma(-kf, PIO2_2, x - kf * PIO2_1));
 }
 FZ_HD double sin(double x) {
   int q;

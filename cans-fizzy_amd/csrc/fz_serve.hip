@@ -1,0 +1,239 @@
+// Serving: many independent cases on one device, admitted by HBM footprint
+// and stepped together (continuous batching). No reference counterpart: the
+// reference runs one case per job. The footprint model follows the idea of
+// device_memory_footprint (src/utils.f90:155-219) for this library's own
+// buffers.
+//
+// Each admitted case owns its context, stream and every buffer, and its
+// kernels are deterministic, so a case's fields are the same whether it runs
+// alone or in a mix. A scheduling round admits queued requests in arrival
+// order while the footprint budget allows (head-of-line: a request that
+// does not fit waits, and the ones behind it wait too), enqueues one time
+// step of every active case on its own stream (the steps of different cases
+// overlap on the device), then waits for each case's step and records its
+// duration from HIP events on that case's stream. A case that has done its
+// steps leaves the active set; its memory is returned when the caller
+// releases it.
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "fz_internal.h"
+
+// bytes of device memory fz_case_create + fz_case_init allocate for prm
+extern "C" int fz_case_footprint(const fz_params *prm, size_t *bytes) {
+  if (!prm || !bytes) return fz_set_error(FZ_ERR_ARG, "fz_case_footprint: null argument");
+  const double n1 = prm->ng[0], n2 = prm->ng[1], n3 = prm->ng[2];
+  const double nh = (n1 + 2) * (n2 + 2) * (n3 + 2), ni = n1 * n2 * n3;
+  const bool periodic_z = prm->cbcpre[4] == 'P' && prm->cbcpre[5] == 'P';
+  double b = 0.0;
+  b += 8.0 * nh * (14 + 5);  // case fields (pn aliases p) + psio, dp, three spare velocities
+  b += 8.0 * ni * 5;         // four AB2 histories + the phase-field right-hand side
+  b += 8.0 * ni * (periodic_z ? 3 : 2);  // Poisson work array, Thomas pivots (+ periodic-z second solve)
+  b += 8.0 * (n1 * n2 + 3 * n3 + 2 * (n2 * n3 + n1 * n3 + n1 * n2));  // eigenvalues, tridiagonal, boundary terms
+  const double planes = 2.0 * ((n1 + 2) * (n2 + 2) + (n1 + 2) * (n3 + 2) + (n2 + 2) * (n3 + 2));
+  b += 9.0 * planes * 12;    // composed boundary maps (about 12 signatures per case)
+  b += 8.0 * ni / 64.0 * 2;  // reduction partials
+  b += 4.0 * 1024 * 1024;    // grids, plans, scalars, allocation rounding
+  *bytes = (size_t)b;
+  return FZ_OK;
+}
+
+namespace {
+struct Request {
+  int id;
+  fz_params prm;
+  int nsteps;
+  size_t bytes;
+};
+struct Running {
+  int id;
+  fz_case *cs;
+  fz_ctx *ctx;
+  int left;
+  size_t bytes;
+  hipEvent_t a, b;
+  bool killed;
+};
+struct Finished {
+  fz_case *cs;
+  size_t bytes;
+  int killed;
+};
+}  // namespace
+
+struct fz_server {
+  size_t budget = 0, in_use = 0;
+  int next_id = 1;
+  std::deque<Request> queue;
+  std::vector<Running> active;
+  std::map<int, Finished> done;
+  std::map<int, std::vector<float>> step_ms;  // per case, every step's duration
+  std::vector<float> all_ms;
+};
+
+extern "C" int fz_server_create(fz_server **out, size_t hbm_budget) {
+  if (!out || hbm_budget == 0) return fz_set_error(FZ_ERR_ARG, "fz_server_create: bad argument");
+  fz_server *s = new fz_server();
+  s->budget = hbm_budget;
+  *out = s;
+  return FZ_OK;
+}
+
+static void drop_running(Running &r) {
+  if (r.a) (void)hipEventDestroy(r.a);
+  if (r.b) (void)hipEventDestroy(r.b);
+  fz_case_destroy(r.cs);
+}
+
+extern "C" int fz_server_destroy(fz_server *s) {
+  if (!s) return FZ_OK;
+  for (Running &r : s->active) drop_running(r);
+  for (auto &kv : s->done) fz_case_destroy(kv.second.cs);
+  delete s;
+  return FZ_OK;
+}
+
+extern "C" int fz_server_submit(fz_server *s, const fz_params *prm, int nsteps, int *id) {
+  if (!s || !prm || nsteps < 0 || !id) return fz_set_error(FZ_ERR_ARG, "fz_server_submit: bad argument");
+  Request q;
+  q.id = s->next_id++;
+  q.prm = *prm;
+  q.nsteps = nsteps;
+  FZ_TRY(fz_case_footprint(prm, &q.bytes));
+  if (q.bytes > s->budget)
+    return fz_set_error(FZ_ERR_ARG, "fz_server_submit: the case needs %zu bytes, more than the whole budget %zu",
+                        q.bytes, s->budget);
+  s->queue.push_back(q);
+  *id = q.id;
+  return FZ_OK;
+}
+
+static int admit(fz_server *s) {
+  while (!s->queue.empty() && s->in_use + s->queue.front().bytes <= s->budget) {
+    Request q = s->queue.front();
+    s->queue.pop_front();
+    Running r;
+    memset(&r, 0, sizeof(r));
+    r.id = q.id;
+    r.left = q.nsteps;
+    r.bytes = q.bytes;
+    FZ_TRY(fz_case_create(&r.cs, &q.prm, nullptr));
+    int rc = fz_case_ctx(r.cs, &r.ctx);
+    if (rc == FZ_OK) rc = fz_case_init(r.cs);
+    if (rc == FZ_OK && hipEventCreate(&r.a) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
+    if (rc == FZ_OK && hipEventCreate(&r.b) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
+    if (rc != FZ_OK) {
+      drop_running(r);
+      return rc;
+    }
+    s->in_use += r.bytes;
+    s->active.push_back(r);
+  }
+  return FZ_OK;
+}
+
+extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
+  if (!s) return fz_set_error(FZ_ERR_ARG, "fz_server_round: null server");
+  FZ_TRY(admit(s));
+  // one step of every active case, each on its own stream
+  for (Running &r : s->active) {
+    if (r.left <= 0) continue;
+    FZ_HIP(hipEventRecord(r.a, r.ctx->st));
+    int killed = 0;
+    FZ_TRY(fz_case_step(r.cs, 1, &killed));
+    FZ_HIP(hipEventRecord(r.b, r.ctx->st));
+    r.killed = r.killed || killed;
+    r.left = killed ? 0 : r.left - 1;
+  }
+  std::vector<Running> still;
+  for (Running &r : s->active) {
+    FZ_HIP(hipEventSynchronize(r.b));
+    float ms = 0.0f;
+    FZ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    s->step_ms[r.id].push_back(ms);
+    s->all_ms.push_back(ms);
+    if (r.left <= 0) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+      s->done[r.id] = Finished{r.cs, r.bytes, r.killed ? 1 : 0};
+    } else {
+      still.push_back(r);
+    }
+  }
+  s->active.swap(still);
+  if (n_active) *n_active = (int)s->active.size();
+  if (n_queued) *n_queued = (int)s->queue.size();
+  return FZ_OK;
+}
+
+// state of a request: 0 queued, 1 running, 2 finished (killed set when a
+// check stopped it), -1 unknown or released
+extern "C" int fz_server_status(fz_server *s, int id, int *state, int *killed) {
+  if (!s || !state) return fz_set_error(FZ_ERR_ARG, "fz_server_status: null argument");
+  *state = -1;
+  if (killed) *killed = 0;
+  for (const Request &q : s->queue)
+    if (q.id == id) *state = 0;
+  for (const Running &r : s->active)
+    if (r.id == id) *state = 1;
+  auto it = s->done.find(id);
+  if (it != s->done.end()) {
+    *state = 2;
+    if (killed) *killed = it->second.killed;
+  }
+  return FZ_OK;
+}
+
+// the case of a finished request (valid until fz_server_release)
+extern "C" int fz_server_case(fz_server *s, int id, fz_case **cs) {
+  if (!s || !cs) return fz_set_error(FZ_ERR_ARG, "fz_server_case: null argument");
+  auto it = s->done.find(id);
+  if (it == s->done.end()) return fz_set_error(FZ_ERR_STATE, "fz_server_case: request %d has not finished", id);
+  *cs = it->second.cs;
+  return FZ_OK;
+}
+
+extern "C" int fz_server_release(fz_server *s, int id) {
+  if (!s) return fz_set_error(FZ_ERR_ARG, "fz_server_release: null server");
+  auto it = s->done.find(id);
+  if (it == s->done.end()) return fz_set_error(FZ_ERR_STATE, "fz_server_release: request %d has not finished", id);
+  fz_case_destroy(it->second.cs);
+  s->in_use -= it->second.bytes;
+  s->done.erase(it);
+  return FZ_OK;
+}
+
+// step durations in ms: of request id, or of every step so far (id = 0);
+// *count receives the total, at most cap are copied
+extern "C" int fz_server_step_times(fz_server *s, int id, float *ms, int cap, int *count) {
+  if (!s || !count) return fz_set_error(FZ_ERR_ARG, "fz_server_step_times: null argument");
+  const std::vector<float> *v = &s->all_ms;
+  std::vector<float> none;
+  if (id != 0) {
+    auto it = s->step_ms.find(id);
+    v = it == s->step_ms.end() ? &none : &it->second;
+  }
+  *count = (int)v->size();
+  if (ms)
+    for (int q = 0; q < std::min(cap, *count); q++) ms[q] = (*v)[q];
+  return FZ_OK;
+}
+
+extern "C" int fz_server_memory(fz_server *s, size_t *in_use, size_t *budget) {
+  if (!s) return fz_set_error(FZ_ERR_ARG, "fz_server_memory: null server");
+  if (in_use) *in_use = s->in_use;
+  if (budget) *budget = s->budget;
+  return FZ_OK;
+}
+
+// free and total device memory of the current device (for choosing a budget)
+extern "C" int fz_device_memory(size_t *free_bytes, size_t *total_bytes) {
+  size_t f = 0, t = 0;
+  FZ_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return FZ_OK;
+}

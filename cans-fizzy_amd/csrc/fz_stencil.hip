@@ -9,6 +9,7 @@
 // without FMA contraction) so that, transcendental functions aside, results
 // round exactly as the reference's own CPU build does.
 #include <cmath>
+#include <cstdlib>
 
 #include "fz_internal.h"
 #include "fz_math.h"
@@ -64,16 +65,20 @@ __global__ void k_correc(fz_dims d, const double *__restrict__ dzci, double fact
 // j = n2+1, w at k = n3+1) are carried over. An interior thread recomputes
 // the corrected velocity of its lower faces with the same expression as
 // the thread that owns them, so the speeds are those of the corrected
-// field bit for bit; each block writes the maximum squared speed of its
-// cells to part[block] (fmax skips NaN speeds, as the reduction kernel
-// does), and k_gamma_fin reduces the partials.
+// field bit for bit. Each block folds the maximum squared speed of its
+// cells into one of GAMMA_SLOTS slots with a 64-bit atomic max on the bit
+// pattern (which orders non-negative doubles like their values; NaN speeds
+// are skipped, as fmax does in the reduction kernel); spreading the blocks
+// over the slots keeps same-address atomics from serialising, and
+// k_gamma_fin reduces and clears the slots.
 __global__ void __launch_bounds__(256) k_correc_step(fz_dims d, const double *__restrict__ dzci, double factori,
                                                      double factorj, double dt, double rhop,
                                                      const double *__restrict__ dp, const double *__restrict__ p,
                                                      const double *__restrict__ u, const double *__restrict__ v,
                                                      const double *__restrict__ w, double *__restrict__ un,
                                                      double *__restrict__ vn, double *__restrict__ wn,
-                                                     double *__restrict__ p_out, double *__restrict__ part) {
+                                                     double *__restrict__ p_out,
+                                                     unsigned long long *__restrict__ slots) {
   FZ_BLOCK_XYZ(d, bx, by, bz);
   const int i = bx * blockDim.x + threadIdx.x, j = by * blockDim.y + threadIdx.y, k = bz;
   double m = 0.0;
@@ -87,7 +92,7 @@ __global__ void __launch_bounds__(256) k_correc_step(fz_dims d, const double *__
     vn[q] = vq;
     wn[q] = wq;
     p_out[q] = dp[q] + p[q];
-    if (part && i >= 1 && i <= d.n1 && j >= 1 && j <= d.n2 && k >= 1 && k <= d.n3) {
+    if (slots && i >= 1 && i <= d.n1 && j >= 1 && j <= d.n2 && k >= 1 && k <= d.n3) {
       const double um = u[q - 1] - factori * (dp[q] - dp[q - 1]) / rhop;
       const double vm = v[q - sy] - factorj * (dp[q] - dp[q - sy]) / rhop;
       const double wm = w[q - sz] - dt * dzci[k - 1] * (dp[q] - dp[q - sz]) / rhop;
@@ -95,7 +100,7 @@ __global__ void __launch_bounds__(256) k_correc_step(fz_dims d, const double *__
       m = fmax(m, uc * uc + vc * vc + wc * wc);
     }
   }
-  if (!part) return;  // uniform across the grid
+  if (!slots) return;  // uniform across the grid
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
   __shared__ double sh[16];
   const int t = threadIdx.x + blockDim.x * threadIdx.y;
@@ -105,43 +110,39 @@ __global__ void __launch_bounds__(256) k_correc_step(fz_dims d, const double *__
   if (t == 0) {
     double b = sh[0];
     for (int q = 1; q < nw; q++) b = fmax(b, sh[q]);
-    part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = b;
+    if (b > 0.0)
+      atomicMax(slots + ((blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) & (FZ_GAMMA_SLOTS - 1)),
+                (unsigned long long)__double_as_longlong(b));
   }
 }
 
 // gam = max(sqrt(velmax)*gam_factor, gam_min) (src/acdi.f90:57-60, src/main.f90:526-527)
-// from the per-block maxima of k_correc_step
-__global__ void __launch_bounds__(1024) k_gamma_fin(int nb, const double *__restrict__ part, double factor,
-                                                    double gmin, double *__restrict__ out) {
-  double m = 0.0;
-  for (int q = threadIdx.x; q < nb; q += blockDim.x) m = fmax(m, part[q]);
+// from the slots of k_correc_step, which are cleared for the next step
+__global__ void __launch_bounds__(FZ_GAMMA_SLOTS) k_gamma_fin(unsigned long long *__restrict__ slots, double factor,
+                                                              double gmin, double *__restrict__ out) {
+  double m = __longlong_as_double((long long)slots[threadIdx.x]);
+  slots[threadIdx.x] = 0ull;
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
-  __shared__ double sh[16];
+  __shared__ double sh[FZ_GAMMA_SLOTS / 64];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmax(m, sh[q]);
+    for (int q = 1; q < FZ_GAMMA_SLOTS / 64; q++) m = fmax(m, sh[q]);
     out[0] = fmax(sqrt(m) * factor, gmin);
   }
 }
 
-// number of partials k_correc_step writes
-long fz_correc_step_blocks(const fz_ctx *c) {
-  const dim3 g = fz_grid_halo(c->d);
-  return (long)g.x * g.y * g.z;
-}
-
 int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *dp, const double *p, const double *u,
                           const double *v, const double *w, double *un, double *vn, double *wn, double *p_out,
-                          double *part) {
+                          unsigned long long *slots) {
   FZ_LAUNCH(c, "correc_step", k_correc_step, fz_grid_halo(c->d), fz_block3(), 0, c->d, c->dzci, dt * c->dli[0],
-            dt * c->dli[1], dt, rho0, dp, p, u, v, w, un, vn, wn, p_out, part);
+            dt * c->dli[1], dt, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 
-int fz_launch_gamma_fin(fz_ctx *c, int nb, const double *part, double factor, double gmin, double *out) {
-  FZ_LAUNCH(c, "gamma_fin", k_gamma_fin, dim3(1), dim3(1024), 0, nb, part, factor, gmin, out);
+int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out) {
+  FZ_LAUNCH(c, "gamma_fin", k_gamma_fin, dim3(1), dim3(FZ_GAMMA_SLOTS), 0, slots, factor, gmin, out);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -606,7 +607,9 @@ __global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__rest
 // the AB2 history duo/dvo/dwo is read and replaced by this step's
 // advection-diffusion term in place (on the first step the history is the
 // term itself, src/rk.f90:76-83).
-__global__ void __launch_bounds__(256) k_mom_fused(
+// the body, shared by occupancy variants of the kernel
+__device__ __forceinline__ void mom_fused_body(
+
     fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
     double drho, double mu, double dmu, double factor1, double factor2, int first, double factor12, double dt_r,
     double surf_factor, double rho0, double bfx, double bfy, double bfz, double gx, double gy, double gz, double rax,
@@ -641,6 +644,14 @@ __global__ void __launch_bounds__(256) k_mom_fused(
   wn[q] = ww + factor12 * rw / rhoz_p;
 }
 
+#define FZ_MOM_FUSED_KERNEL(NAME, ATTR)                                                                 \
+  __global__ void __launch_bounds__(256) ATTR NAME(    fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,    double drho, double mu, double dmu, double factor1, double factor2, int first, double factor12, double dt_r,    double surf_factor, double rho0, double bfx, double bfy, double bfz, double gx, double gy, double gz, double rax,    double ray, double raz, const double *__restrict__ psiflx_x, const double *__restrict__ psiflx_y,    const double *__restrict__ psiflx_z, const double *__restrict__ u, const double *__restrict__ v,    const double *__restrict__ w, const double *__restrict__ psi, const double *__restrict__ psio,    const double *__restrict__ p, const double *__restrict__ po, const double *__restrict__ kappa,    double *__restrict__ duo, double *__restrict__ dvo, double *__restrict__ dwo, double *__restrict__ un,    double *__restrict__ vn, double *__restrict__ wn) {                \
+    mom_fused_body(d, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, factor1, factor2, first, factor12, dt_r, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz, psiflx_x, psiflx_y, psiflx_z, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);                                                                          \
+  }
+FZ_MOM_FUSED_KERNEL(k_mom_fused, )
+FZ_MOM_FUSED_KERNEL(k_mom_fused_w4, __attribute__((amdgpu_waves_per_eu(4))))
+FZ_MOM_FUSED_KERNEL(k_mom_fused_w5, __attribute__((amdgpu_waves_per_eu(5))))
+
 int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], double factor1, double factor2,
                         int first, double factor12, double dt_r, double sigma, double rho0, const double bforce[3],
                         const double gacc[3], const double rho_av[3], const double *fx, const double *fy,
@@ -648,7 +659,9 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
                         const double *psio, const double *p, const double *po, const double *kappa, double *duo,
                         double *dvo, double *dwo, double *un, double *vn, double *wn) {
   const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
-  FZ_LAUNCH(c, "mom_fused", k_mom_fused, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
+  static const int variant = getenv("FZ_MOM_WPE") ? atoi(getenv("FZ_MOM_WPE")) : 0;  // occupancy tuning
+  auto kern = variant == 4 ? k_mom_fused_w4 : (variant == 5 ? k_mom_fused_w5 : k_mom_fused);
+  FZ_LAUNCH(c, "mom_fused", kern, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
             c->dli[1], rho12[1], rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12,
             dt_r, surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1],
             rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);

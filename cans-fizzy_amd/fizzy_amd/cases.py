@@ -184,9 +184,24 @@ class DeviceCase:
         call("fz_case_ctx", self._h, C.byref(c))
         self.ctx = c
 
+    @classmethod
+    def view(cls, handle, case):
+        """A DeviceCase over a case another owner created (e.g. the server); not destroyed by close()."""
+        self = cls.__new__(cls)
+        self.spec = case
+        self.prm = make_params(case)
+        self.n = [int(x) for x in case["ng"]]
+        self._h = handle if isinstance(handle, vp) else vp(handle)
+        self._owned = False
+        c = vp()
+        call("fz_case_ctx", self._h, C.byref(c))
+        self.ctx = c
+        return self
+
     def close(self):
         if getattr(self, "_h", None):
-            lib().fz_case_destroy(self._h)
+            if getattr(self, "_owned", True):
+                lib().fz_case_destroy(self._h)
             self._h = None
 
     def __del__(self):
