@@ -61,7 +61,8 @@ def profiled_traffic(kernel, workload):
         return None
     for line in open(path):
         cols = line.split()
-        if len(cols) >= 6 and cols[0] == "k_" + kernel:
+        # kernel names carry their template arguments (the FP64 instances here)
+        if len(cols) >= 6 and cols[0].split("<")[0] == "k_" + kernel and "float" not in cols[0]:
             try:
                 return round((float(cols[4]) + float(cols[5])) * 1e6)
             except ValueError:
@@ -145,7 +146,8 @@ def main_serving(args):
     import fizzy_amd as F
     from fizzy_amd import serve
 
-    reqs = serve.serving_requests(args.requests, mean_interarrival=0.05, nsteps=args.case_steps)
+    reqs = serve.serving_requests(args.requests, mean_interarrival=0.05, nsteps=args.case_steps,
+                                  precision=args.precision)
     cells = [int(np.prod(spec["ng"])) * n for _, spec, n in reqs]
     f, t = C.c_size_t(), C.c_size_t()
     F.call("fz_device_memory", C.byref(f), C.byref(t))
@@ -158,10 +160,11 @@ def main_serving(args):
     log(f"serving: {len(reqs)} requests, {sum(cells) / 1e6:.1f} Mcell-steps in {wall:.2f} s, "
         f"p50 {np.percentile(lat, 50):.3f} ms, p99 {np.percentile(lat, 99):.3f} ms")
     t0 = time.perf_counter()
-    for _, spec, n in reqs:  # sequential: one case at a time, same work
+    for _, spec, n in reqs:  # sequential: one case at a time, same work (graphs as in the server)
         dc = F.DeviceCase(spec)
         try:
             dc.init()
+            F.call("fz_case_set_graphs", dc._h, 1)
             dc.step(n)
             dc.sync()
         finally:
@@ -175,7 +178,8 @@ def main_serving(args):
         "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "throughput_Mcell_steps_s": round(thr, 2), "sequential_Mcell_steps_s": round(seq_thr, 2),
         "speedup_vs_sequential": round(thr / seq_thr, 3), "wall_s": round(wall, 3), "n_gpus": 1,
-        "higher_is_better": False, "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "higher_is_better": False, "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "mixed",
+        "data": "synthetic",
         "config": {"workload": "serving_mix", "requests": len(reqs), "steps_per_case": args.case_steps,
                    "arrivals": "Poisson, mean 50 ms", "grids": "64^2, 128^2, 256^2 (2-D), 64^2x128, 128^2x256",
                    "density_ratio": "uniform [2, 1000]"},
@@ -183,17 +187,127 @@ def main_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def main_poisson(args):
+    """The standalone Poisson solve: 512^3, periodic x/y, Neumann z, a
+    right-hand side seeded uniform in [-1, 1] with its (volume-weighted)
+    mean removed; the FP64 solver against FP32 transforms with FP64-residual
+    refinement to a 1e-12 relative residual, each timed over the solves."""
+    import ctypes as C
+
+    import numpy as np
+
+    import fizzy_amd as F
+
+    n = [512, 512, 512]
+    nsolve = args.steps
+    cells = n[0] * n[1] * n[2]
+    ctx = F.Context(n, [1.0, 1.0, 1.0], "PPPPNN")
+    try:
+        F.call("fz_initsolver", ctx.h, None, None, None, None, None, None, None, None, None)
+        rng = np.random.default_rng(20261202)
+        f = np.zeros((n[0] + 2, n[1] + 2, n[2] + 2), order="F")
+        fi = f[1:-1, 1:-1, 1:-1]
+        fi[...] = rng.uniform(-1.0, 1.0, n)
+        wz = ctx.dzf[1:-1]
+        fi -= (fi * wz).sum() / (wz.sum() * n[0] * n[1])
+        b = ctx.halo(f)
+        x = ctx.halo()
+        res = {}
+        for mode in ("fp64", "mixed"):
+            its, rels = [], []
+            for rep in range(args.warmup + nsolve):
+                if rep == args.warmup:
+                    ctx.sync()
+                    t0 = time.perf_counter()
+                F.call("fz_copy", ctx.h, x.ptr, b.ptr, C.c_size_t(x.nbytes))
+                if mode == "fp64":
+                    F.call("fz_solver", ctx.h, x.ptr)
+                else:
+                    it, rr = C.c_int(), C.c_double()
+                    F.call("fz_solver_mixed", ctx.h, x.ptr, 1e-12, 10, C.byref(it), C.byref(rr))
+                    its.append(it.value)
+                    rels.append(rr.value)
+            ctx.sync()
+            t1 = time.perf_counter()
+            # the residual of the last solution, by the reference's chk_poisson
+            zero = (C.c_double * 6)()
+            F.call("fz_boundp", ctx.h, b"PPPPNN", zero, x.ptr)
+            dm = C.c_double()
+            F.call("fz_chk_poisson", ctx.h, b.ptr, x.ptr, C.byref(dm))
+            ms = 1e3 * (t1 - t0) / nsolve
+            res[mode] = {"ms_per_solve": round(ms, 3), "Mcell_solves_s": round(cells / ms / 1e3, 1),
+                         "relres": dm.value / np.abs(fi).max()}
+            if its:
+                res[mode]["refinements"] = sorted(set(its))
+            log(f"{mode}: {ms:.3f} ms/solve (incl. the rhs copy), relres {res[mode]['relres']:.2e}"
+                + (f", refinements {sorted(set(its))}" if its else ""))
+    finally:
+        ctx.close()
+    out = {
+        "metric": "standalone Poisson solve, FP64 vs FP32 + FP64-residual refinement",
+        "value": res["fp64"]["Mcell_solves_s"], "unit": "Mcell-solves/s (FP64)", "n_gpus": 1, "steps": nsolve,
+        "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: uniform [-1,1] right-hand side, seeded, volume-weighted mean removed",
+        "config": {"workload": "poisson_512x512x512_PPPPNN", "grid": n}, "fp64": res["fp64"],
+        "mixed": res["mixed"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_mixed(F, spec, args, cells, fp64_s_per_step):
+    """The same workload in the mixed-precision mode: timed the same way as
+    the FP64 run (same steps, warm-up and per-launch events)."""
+    ms = dict(spec, precision="mixed", refine=args.refine)
+    dc = F.DeviceCase(ms)
+    try:
+        dc.init()
+        if args.graphs:
+            F.call("fz_case_set_graphs", dc._h, 1)
+        if args.warmup:
+            dc.step(args.warmup)
+        dc.sync()
+        F.call("fz_prof_enable", dc.ctx, 0 if args.no_prof else 1)
+        prof_report(F, dc.ctx)
+        t0 = time.perf_counter()
+        killed = dc.step(args.steps)
+        dc.sync()
+        t1 = time.perf_counter()
+        F.call("fz_prof_enable", dc.ctx, 0)
+        rows = {} if args.no_prof else prof_report(F, dc.ctx)
+        sc = dc.scalars()
+    finally:
+        dc.close()
+    s_per_step = (t1 - t0) / args.steps
+    value = cells / s_per_step / 1e6
+    log(f"mixed ({args.refine} refinements): {1e3 * s_per_step:.3f} ms/step, {value:.1f} Mcell-steps/s, "
+        f"divmax {sc[7]:.2e}, killed {killed}")
+    if rows:
+        total_ms = sum(m for _, m in rows.values())
+        for name, (calls, m) in sorted(rows.items(), key=lambda kv: -kv[1][1])[:12]:
+            log(f"  {name:22s} {calls:5d} launches {m / calls * 1e3:9.1f} us/launch {100 * m / total_ms:5.1f}%")
+    return {"value": round(value, 2), "ms_per_step": round(1e3 * s_per_step, 4),
+            "speedup_vs_fp64": round(fp64_s_per_step / s_per_step, 3), "refinements": args.refine,
+            "divmax": float(sc[7]),
+            "dtype": "f32 phase field, normals, curvature, fluxes, AB2 histories and transforms; "
+                     "f64 velocity, pressure, momentum arithmetic, pivots, residuals and sums"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["c2", "c0", "c4"], default="c2",
-                    help="c2: 256x256x512 bubble (default); c0: 64x64x128; c4: the serving mix")
+    ap.add_argument("--workload", choices=["c2", "c0", "c1", "c4"], default="c2",
+                    help="c2: 256x256x512 bubble (default); c0: 64x64x128; c1: the standalone 512^3 Poisson "
+                         "solve; c4: the serving mix")
     ap.add_argument("--requests", type=int, default=24, help="c4: number of requests")
     ap.add_argument("--case-steps", type=int, default=200, help="c4: steps per request")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-prof", action="store_true", help="time without per-launch events")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run of the same workload")
+    ap.add_argument("--refine", type=int, default=1, help="mixed precision: FP64-residual refinements per solve")
+    ap.add_argument("--graphs", action="store_true", help="replay the steps from captured HIP graphs")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="c4: precision of the cases")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -203,6 +317,8 @@ def main():
 
     if args.workload == "c4":
         return main_serving(args)
+    if args.workload == "c1":
+        return main_poisson(args)
 
     import fizzy_amd as F
     from fizzy_amd import cases
@@ -213,6 +329,8 @@ def main():
     log(f"workload {args.workload}: {n[0]}x{n[1]}x{n[2]} = {cells} cells, warmup {args.warmup}, steps {args.steps}")
     dc = F.DeviceCase(spec)
     dc.init()
+    if args.graphs:
+        F.call("fz_case_set_graphs", dc._h, 1)
     if args.warmup:
         dc.step(args.warmup)
     dc.sync()
@@ -251,6 +369,11 @@ def main():
     copy_gbs = copy_bandwidth(F, dc.ctx, 8 * dc_count(dc))
     log(f"plain copy kernel over one field ({8 * dc_count(dc) / 1e6:.0f} MB): {copy_gbs:.0f} GB/s")
 
+    dc.close()
+    mixed = None
+    if not args.no_mixed:
+        mixed = run_mixed(F, spec, args, cells, elapsed / args.steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec)
@@ -276,6 +399,7 @@ def main():
                           "frac": round(step_bw / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "hbm_copy_GBs": round(copy_gbs, 1),
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
 

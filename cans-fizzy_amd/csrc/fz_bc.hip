@@ -30,8 +30,11 @@ struct DirOps {
 };
 }  // namespace
 
-__global__ void k_plane_ops(fz_dims d, int dir, double *__restrict__ f0, double *__restrict__ f1,
-                            double *__restrict__ f2, DirOps ops) {
+// T = double: the reference's FP64 fields; T = float: the FP32 phase fields
+// of the mixed-precision step (the operations then round in float)
+template <typename T>
+__global__ void k_plane_ops(fz_dims d, int dir, T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2,
+                            DirOps ops) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   long base, st;
@@ -48,17 +51,17 @@ __global__ void k_plane_ops(fz_dims d, int dir, double *__restrict__ f0, double 
     base = a + d.sy * b;
     st = d.sz;
   }
-  double *fs[3] = {f0, f1, f2};
+  T *fs[3] = {f0, f1, f2};
   for (int fi = 0; fi < 3; fi++) {
-    double *p = fs[fi];
+    T *p = fs[fi];
     if (!p) continue;
     const FieldOps &fo = ops.f[fi];
     for (int s = 0; s < fo.nops; s++) {
       const SubOp &o = fo.op[s];
-      double val;
+      T val;
       if (o.mode == 1) val = p[base + st * o.src];
-      else if (o.mode == 2) val = o.factor + o.sgn * p[base + st * o.src];
-      else val = o.factor;
+      else if (o.mode == 2) val = (T)o.factor + (T)o.sgn * p[base + st * o.src];
+      else val = (T)o.factor;
       p[base + st * o.dst] = val;
     }
   }
@@ -102,15 +105,16 @@ static void set_bc_ops(FieldOps &fo, char ctype, int ib, int nn, bool centered, 
   }
 }
 
-static int launch_dir(fz_ctx *c, int dir, double *f0, double *f1, double *f2, const DirOps &ops, bool periodic_copy) {
+template <typename T>
+static int launch_dir(fz_ctx *c, int dir, T *f0, T *f1, T *f2, const DirOps &ops, bool periodic_copy) {
   const fz_dims &d = c->d;
   int na, nb;
   if (dir == 0) { na = d.n2 + 2; nb = d.n3 + 2; }
   else if (dir == 1) { na = d.n1 + 2; nb = d.n3 + 2; }
   else { na = d.n1 + 2; nb = d.n2 + 2; }
   static const char *names[2][3] = {{"bc_x", "bc_y", "bc_z"}, {"bc_per_x", "bc_per_y", "bc_per_z"}};
-  FZ_LAUNCH(c, names[periodic_copy ? 1 : 0][dir], k_plane_ops, dim3((na + 63) / 64, nb), dim3(64), 0, d, dir, f0, f1,
-            f2, ops);
+  FZ_LAUNCH(c, names[periodic_copy ? 1 : 0][dir], k_plane_ops<T>, dim3((na + 63) / 64, nb), dim3(64), 0, d, dir, f0,
+            f1, f2, ops);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -184,7 +188,8 @@ static void bounduvw_phases(fz_ctx *c, const char cbc[18], const double bc[18], 
 #undef BV
 }
 
-static int run_phases(fz_ctx *c, const std::vector<Phase> &ph, double *f0, double *f1, double *f2) {
+template <typename T>
+static int run_phases(fz_ctx *c, const std::vector<Phase> &ph, T *f0, T *f1, T *f2) {
   for (const Phase &P : ph) FZ_TRY(launch_dir(c, P.dir, f0, f1, f2, P.ops, P.periodic_copy));
   return FZ_OK;
 }
@@ -220,15 +225,16 @@ struct BcMapDev {
   int n;
 };
 
-__global__ void k_bc_gather(BcMapDev m0, BcMapDev m1, BcMapDev m2, double *__restrict__ f0, double *__restrict__ f1,
-                            double *__restrict__ f2) {
+template <typename T>
+__global__ void k_bc_gather(BcMapDev m0, BcMapDev m1, BcMapDev m2, T *__restrict__ f0, T *__restrict__ f1,
+                            T *__restrict__ f2) {
   const BcMapDev m = blockIdx.y == 0 ? m0 : (blockIdx.y == 1 ? m1 : m2);
-  double *f = blockIdx.y == 0 ? f0 : (blockIdx.y == 1 ? f1 : f2);
+  T *f = blockIdx.y == 0 ? f0 : (blockIdx.y == 1 ? f1 : f2);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.n; t += gridDim.x * blockDim.x) {
     const BcChain &ch = m.chains[m.cid[t]];
     const int sidx = m.src[t];
-    double v = sidx >= 0 ? f[sidx] : 0.0;
-    for (int o = 0; o < ch.nops; o++) v = (ch.mode[o] == 2) ? ch.factor[o] + ch.sgn[o] * v : ch.factor[o];
+    T v = sidx >= 0 ? f[sidx] : (T)0;
+    for (int o = 0; o < ch.nops; o++) v = (ch.mode[o] == 2) ? (T)ch.factor[o] + (T)ch.sgn[o] * v : (T)ch.factor[o];
     f[m.dst[t]] = v;
   }
 }
@@ -378,12 +384,13 @@ static BcMapDev devmap(const fz_bcmap *m) {
   return d;
 }
 
-static int launch_gather(fz_ctx *c, int nf, const fz_bcmap *const *maps, double *f0, double *f1, double *f2) {
+template <typename T>
+static int launch_gather(fz_ctx *c, int nf, const fz_bcmap *const *maps, T *f0, T *f1, T *f2) {
   int nmax = 0;
   for (int q = 0; q < nf; q++) nmax = std::max(nmax, maps[q]->n);
   if (nmax == 0) return FZ_OK;
   const int nb = std::min((nmax + 255) / 256, 4096);
-  FZ_LAUNCH(c, "bc_gather", k_bc_gather, dim3(nb, nf), dim3(256), 0, devmap(maps[0]),
+  FZ_LAUNCH(c, "bc_gather", k_bc_gather<T>, dim3(nb, nf), dim3(256), 0, devmap(maps[0]),
             devmap(nf > 1 ? maps[1] : nullptr), devmap(nf > 2 ? maps[2] : nullptr), f0, f1, f2);
   FZ_LAUNCHED();
   return FZ_OK;
@@ -397,22 +404,19 @@ static bool valid_codes(const char *cbc, int n) {
 
 static bool g_gather_off = getenv("FZ_BC_PHASES") != nullptr;  // the ordered phases, for comparison
 
-extern "C" int fz_boundp(fz_ctx *c, const char cbc[6], const double bc[6], double *p) {
-  if (!c || !cbc || !bc || !p) return fz_set_error(FZ_ERR_ARG, "fz_boundp: null argument");
-  if (!valid_codes(cbc, 6)) return fz_set_error(FZ_ERR_ARG, "fz_boundp: bad boundary code");
+template <typename T>
+static int boundp_t(fz_ctx *c, const char cbc[6], const double bc[6], T *p) {
   std::vector<Phase> ph;
   boundp_phases(c, cbc, bc, ph);
   if (!g_gather_off) {
     fz_bcmap *m = get_map(c, signature("p", cbc, 6, bc, 0, 0), ph, 0);
-    if (m->ok) return launch_gather(c, 1, &m, p, nullptr, nullptr);
+    if (m->ok) return launch_gather<T>(c, 1, &m, p, nullptr, nullptr);
   }
-  return run_phases(c, ph, p, nullptr, nullptr);
+  return run_phases<T>(c, ph, p, nullptr, nullptr);
 }
 
-extern "C" int fz_bounduvw(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, double *u, double *v,
-                           double *w) {
-  if (!c || !cbc || !bc || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: null argument");
-  if (!valid_codes(cbc, 18)) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: bad boundary code");
+template <typename T>
+static int bounduvw_t(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, T *u, T *v, T *w) {
   std::vector<Phase> ph;
   bounduvw_phases(c, cbc, bc, is_correc ? 1 : 0, ph);
   if (!g_gather_off) {
@@ -422,7 +426,32 @@ extern "C" int fz_bounduvw(fz_ctx *c, const char cbc[18], const double bc[18], i
       m[fi] = get_map(c, signature("uvw", cbc, 18, bc, is_correc ? 1 : 0, fi), ph, fi);
       ok = ok && m[fi]->ok;
     }
-    if (ok) return launch_gather(c, 3, m, u, v, w);
+    if (ok) return launch_gather<T>(c, 3, m, u, v, w);
   }
-  return run_phases(c, ph, u, v, w);
+  return run_phases<T>(c, ph, u, v, w);
+}
+
+extern "C" int fz_boundp(fz_ctx *c, const char cbc[6], const double bc[6], double *p) {
+  if (!c || !cbc || !bc || !p) return fz_set_error(FZ_ERR_ARG, "fz_boundp: null argument");
+  if (!valid_codes(cbc, 6)) return fz_set_error(FZ_ERR_ARG, "fz_boundp: bad boundary code");
+  return boundp_t<double>(c, cbc, bc, p);
+}
+
+extern "C" int fz_bounduvw(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, double *u, double *v,
+                           double *w) {
+  if (!c || !cbc || !bc || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: null argument");
+  if (!valid_codes(cbc, 18)) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: bad boundary code");
+  return bounduvw_t<double>(c, cbc, bc, is_correc, u, v, w);
+}
+
+// FP32 fields of the mixed-precision step (same sequence and maps)
+int fz_boundp_f32(fz_ctx *c, const char cbc[6], const double bc[6], float *p) {
+  if (!valid_codes(cbc, 6)) return fz_set_error(FZ_ERR_ARG, "fz_boundp: bad boundary code");
+  return boundp_t<float>(c, cbc, bc, p);
+}
+
+int fz_bounduvw_f32(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, float *u, float *v,
+                    float *w) {
+  if (!valid_codes(cbc, 18)) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: bad boundary code");
+  return bounduvw_t<float>(c, cbc, bc, is_correc, u, v, w);
 }

@@ -22,27 +22,35 @@ int fz_launch_rk_mom(fz_ctx *c, double f1, double f2, const double rho12[2], con
                      const double *dwo, double *u, double *v, double *w);
 int fz_launch_rk_oth(fz_ctx *c, double f12, const double rho12[2], const double *psi, const double *du,
                      const double *dv, const double *dw, double *u, double *v, double *w);
+template <typename PT, typename HT>
 int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], double factor1, double factor2,
-                        int first, double factor12, double dt_r, double sigma, double rho0, const double bforce[3],
-                        const double gacc[3], const double rho_av[3], const double *fx, const double *fy,
-                        const double *fz, const double *u, const double *v, const double *w, const double *psi,
-                        const double *psio, const double *p, const double *po, const double *kappa, double *duo,
-                        double *dvo, double *dwo, double *un, double *vn, double *wn);
+                        int first, double factor12, double dt_r, const double *stc, double sigma, double rho0,
+                        const double bforce[3],
+                        const double gacc[3], const double rho_av[3], const PT *fx, const PT *fy, const PT *fz,
+                        const double *u, const double *v, const double *w, const PT *psi, const PT *psio,
+                        const double *p, const double *po, const PT *kappa, HT *duo, HT *dvo, HT *dwo, double *un,
+                        double *vn, double *wn);
 int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
-int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *dp, const double *p, const double *u,
-                          const double *v, const double *w, double *un, double *vn, double *wn, double *p_out,
-                          unsigned long long *slots);
+int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
+                          const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
+                          double *p_out, unsigned long long *slots);
 int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out);
+template <typename T>
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
-                        int first, const double *u, const double *v, const double *w, const double *normx,
-                        const double *normy, const double *normz, const double *phi, const double *psio,
-                        double *dpsio, double *psi_out, double *flux_x, double *flux_y, double *flux_z);
+                        const double *stc, int first, const double *u, const double *v, const double *w,
+                        const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
+                        T *psi_out, T *flux_x, T *flux_y, T *flux_z);
 int fz_chkdiv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
 int fz_chkdt_adv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
 double fz_chkdt_finish(const fz_ctx *c, double dti, int is_solve_ns, int is_track_interface, int vof,
                        const double mu12[2], const double rho12[2], double sigma, const double gacc[3], double gam,
                        double seps);
-int fz_bulk_mean_dev(fz_ctx *c, int idir, const double *psi, const double p12[2], const double *gvr, double *d_out);
+template <typename PT>
+int fz_bulk_mean_dev(fz_ctx *c, int idir, const PT *psi, const double p12[2], const double *gvr, double *d_out);
+int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi);
+int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float *normz, float *kappa);
+int fz_boundp_f32(fz_ctx *c, const char cbc[6], const double bc[6], float *p);
+int fz_bounduvw_f32(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, float *u, float *v, float *w);
 
 // out-of-place AB2 phase-field update (psi_out may alias psi_in)
 __global__ void k_rk_psi_oop(fz_dims d, double f1, double f2, const double *__restrict__ dpsidt,
@@ -73,28 +81,76 @@ struct fz_case {
   char cbccur[6];
   double bccur[6];
   bool gam_on_device = false;
+  // mixed precision: the phase field, its geometry, fluxes and the AB2
+  // histories in FP32 (g[id], f[id] null); velocity, pressure and the
+  // projection stay FP64
+  int precision = FZ_PREC_FP64;
+  int nref = 1;
+  float *g[F_N] = {nullptr};
+  float *psio32 = nullptr;
+  // per-step scalars in device memory and the captured step graphs
+  double *dstc = nullptr;
+  double stc_host[FZ_STC_N] = {0};
+  bool stc_valid = false;
+  bool use_graphs = false;
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    const void *key[2] = {nullptr, nullptr};
+  } graph[2];
 };
+
+// fields held in FP32 by the mixed-precision mode
+static bool is_f32_field(int id) {
+  return id == F_PSI || id == F_PHI || id == F_KAPPA || (id >= F_NX && id <= F_FZ) || id >= F_DPSIO;
+}
+
+__global__ void k_to_f32(long n, const double *__restrict__ a, float *__restrict__ b) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) b[q] = (float)a[q];
+}
+__global__ void k_to_f64(long n, const float *__restrict__ a, double *__restrict__ b) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) b[q] = (double)a[q];
+}
+static int to_f32(fz_ctx *c, long n, const double *a, float *b) {
+  FZ_LAUNCH(c, "to_f32", k_to_f32, dim3((unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096)), dim3(256), 0, n, a, b);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+static int to_f64(fz_ctx *c, long n, const float *a, double *b) {
+  FZ_LAUNCH(c, "to_f64", k_to_f64, dim3((unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096)), dim3(256), 0, n, a, b);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
 
 static int case_alloc(fz_case *cs) {
   fz_ctx *c = cs->ctx;
+  const bool mixed = cs->precision == FZ_PREC_MIXED;
   for (int q = 0; q < F_N; q++) {
     if (q == F_PN) continue;  // pn is p (see above)
     const long n = q < F_DPSIO ? c->nh : c->ni;
-    FZ_HIP(hipMalloc((void **)&cs->f[q], sizeof(double) * n));
-    FZ_HIP(hipMemsetAsync(cs->f[q], 0, sizeof(double) * n, c->st));
+    if (mixed && is_f32_field(q)) {
+      FZ_HIP(hipMalloc((void **)&cs->g[q], sizeof(float) * n));
+      FZ_HIP(hipMemsetAsync(cs->g[q], 0, sizeof(float) * n, c->st));
+    } else {
+      FZ_HIP(hipMalloc((void **)&cs->f[q], sizeof(double) * n));
+      FZ_HIP(hipMemsetAsync(cs->f[q], 0, sizeof(double) * n, c->st));
+    }
   }
-  double **hal[5] = {&cs->psio, &cs->dp, &cs->un, &cs->vn, &cs->wn};
+  double **hal[5] = {mixed ? nullptr : &cs->psio, &cs->dp, &cs->un, &cs->vn, &cs->wn};
   for (int q = 0; q < 5; q++) {
+    if (!hal[q]) continue;
     FZ_HIP(hipMalloc((void **)hal[q], sizeof(double) * c->nh));
     FZ_HIP(hipMemsetAsync(*hal[q], 0, sizeof(double) * c->nh, c->st));
   }
-  double **itr[1] = {&cs->dpsidt};
-  for (int q = 0; q < 1; q++) {
-    FZ_HIP(hipMalloc((void **)itr[q], sizeof(double) * c->ni));
-    FZ_HIP(hipMemsetAsync(*itr[q], 0, sizeof(double) * c->ni, c->st));
+  if (mixed) {
+    FZ_HIP(hipMalloc((void **)&cs->psio32, sizeof(float) * c->nh));
+    FZ_HIP(hipMemsetAsync(cs->psio32, 0, sizeof(float) * c->nh, c->st));
+  } else {
+    FZ_HIP(hipMalloc((void **)&cs->dpsidt, sizeof(double) * c->ni));
+    FZ_HIP(hipMemsetAsync(cs->dpsidt, 0, sizeof(double) * c->ni, c->st));
   }
   FZ_HIP(hipMalloc((void **)&cs->gslots, sizeof(unsigned long long) * FZ_GAMMA_SLOTS));
   FZ_HIP(hipMemsetAsync(cs->gslots, 0, sizeof(unsigned long long) * FZ_GAMMA_SLOTS, c->st));
+  FZ_HIP(hipMalloc((void **)&cs->dstc, sizeof(double) * FZ_STC_N));
   FZ_HIP(hipMalloc((void **)&cs->dsc, sizeof(double) * 16));
   FZ_HIP(hipMemsetAsync(cs->dsc, 0, sizeof(double) * 16, c->st));
   const int nz = c->d.n3 + 2;
@@ -114,8 +170,14 @@ static int case_alloc(fz_case *cs) {
 extern "C" int fz_case_destroy(fz_case *cs) {
   if (!cs) return FZ_OK;
   if (cs->ctx) (void)hipStreamSynchronize(cs->ctx->st);
-  for (int q = 0; q < F_N; q++)
+  for (int q = 0; q < F_N; q++) {
     if (cs->f[q]) (void)hipFree(cs->f[q]);
+    if (cs->g[q]) (void)hipFree(cs->g[q]);
+  }
+  if (cs->psio32) (void)hipFree(cs->psio32);
+  if (cs->dstc) (void)hipFree(cs->dstc);
+  for (int q = 0; q < 2; q++)
+    if (cs->graph[q].exec) (void)hipGraphExecDestroy(cs->graph[q].exec);
   double *o[10] = {cs->psio, cs->dpsidt, cs->un, cs->vn, cs->wn, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f,
                    (double *)cs->gslots};
   for (int q = 0; q < 10; q++)
@@ -126,7 +188,20 @@ extern "C" int fz_case_destroy(fz_case *cs) {
 }
 
 extern "C" int fz_case_create(fz_case **out, const fz_params *prm, void *hip_stream) {
+  return fz_case_create_ex(out, prm, hip_stream, FZ_PREC_FP64, 0);
+}
+
+extern "C" int fz_case_create_ex(fz_case **out, const fz_params *prm, void *hip_stream, int precision, int nref) {
   if (!out || !prm) return fz_set_error(FZ_ERR_ARG, "fz_case_create: null argument");
+  if (precision != FZ_PREC_FP64 && precision != FZ_PREC_MIXED)
+    return fz_set_error(FZ_ERR_ARG, "fz_case_create: unknown precision %d", precision);
+  if (precision == FZ_PREC_MIXED) {
+    if (nref < 0 || nref > 8) return fz_set_error(FZ_ERR_ARG, "fz_case_create: refinements %d not in [0, 8]", nref);
+    if (prm->vof || !prm->is_track_interface || !prm->is_solve_ns)
+      return fz_set_error(FZ_ERR_UNSUPPORTED,
+                          "mixed precision is implemented for the ACDI two-fluid step (vof = F, interface and "
+                          "Navier-Stokes on)");
+  }
   const char *codes[4] = {prm->cbcvel, prm->cbcpre, prm->cbcpsi, prm->cbcnor};
   const int lens[4] = {18, 6, 6, 18};
   for (int a = 0; a < 4; a++)
@@ -135,6 +210,8 @@ extern "C" int fz_case_create(fz_case **out, const fz_params *prm, void *hip_str
         return fz_set_error(FZ_ERR_ARG, "fz_case_create: bad boundary code '%c'", codes[a][b]);
   fz_case *cs = new fz_case();
   cs->prm = *prm;
+  cs->precision = precision;
+  cs->nref = nref;
   const int n3 = prm->ng[2];
   std::vector<double> dzc(n3 + 2), dzf(n3 + 2), zc(n3 + 2), zf(n3 + 2);
   int rc = fz_initgrid(prm->gtype, n3, prm->gr, prm->l[2], dzc.data(), dzf.data(), zc.data(), zf.data());
@@ -142,6 +219,8 @@ extern "C" int fz_case_create(fz_case **out, const fz_params *prm, void *hip_str
                                       prm->cbcpre, hip_stream);
   if (rc == FZ_OK) rc = fz_initsolver(cs->ctx, prm->bcpre, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                       nullptr, nullptr);
+  if (rc == FZ_OK && precision == FZ_PREC_MIXED && !cs->ctx->pois->rhsb_zero)
+    rc = fz_set_error(FZ_ERR_UNSUPPORTED, "mixed precision needs homogeneous pressure boundary values");
   if (rc == FZ_OK) rc = case_alloc(cs);
   if (rc != FZ_OK) {
     fz_case_destroy(cs);
@@ -163,10 +242,29 @@ extern "C" int fz_case_ctx(fz_case *cs, fz_ctx **ctx) {
 
 extern "C" int fz_case_field(fz_case *cs, int id, double **dptr, size_t *count) {
   if (!cs || !dptr || id < 0 || id >= F_N) return fz_set_error(FZ_ERR_ARG, "fz_case_field: bad argument");
+  if (cs->precision == FZ_PREC_MIXED && is_f32_field(id))
+    return fz_set_error(FZ_ERR_ARG, "fz_case_field: field %d is FP32 in this case (fz_case_field_f32)", id);
   *dptr = (id == F_PN) ? cs->f[F_P] : cs->f[id];
   if (count) *count = (size_t)(id < F_DPSIO ? cs->ctx->nh : cs->ctx->ni);
   return FZ_OK;
 }
+
+extern "C" int fz_case_field_f32(fz_case *cs, int id, float **dptr, size_t *count) {
+  if (!cs || !dptr || id < 0 || id >= F_N) return fz_set_error(FZ_ERR_ARG, "fz_case_field_f32: bad argument");
+  if (!cs->g[id]) return fz_set_error(FZ_ERR_ARG, "fz_case_field_f32: field %d is not FP32 in this case", id);
+  *dptr = cs->g[id];
+  if (count) *count = (size_t)(id < F_DPSIO ? cs->ctx->nh : cs->ctx->ni);
+  return FZ_OK;
+}
+
+extern "C" int fz_case_precision(fz_case *cs, int *precision, int *nref) {
+  if (!cs || !precision) return fz_set_error(FZ_ERR_ARG, "fz_case_precision: null argument");
+  *precision = cs->precision;
+  if (nref) *nref = cs->nref;
+  return FZ_OK;
+}
+
+static void drop_graphs(fz_case *cs);
 
 static int sync_gam(fz_case *cs) {
   if (!cs->gam_on_device) return FZ_OK;
@@ -187,6 +285,7 @@ extern "C" int fz_case_get_scalars(fz_case *cs, double out[12]) {
 extern "C" int fz_case_set_scalars(fz_case *cs, const double in[12]) {
   if (!cs || !in) return fz_set_error(FZ_ERR_ARG, "fz_case_set_scalars: null argument");
   memcpy(cs->sc, in, sizeof(cs->sc));
+  drop_graphs(cs);  // seps is an argument of the captured kernels
   fz_ctx *c = cs->ctx;
   c->h_small[0] = cs->sc[S_GAM];
   FZ_HIP(hipMemcpyAsync(cs->dsc, c->h_small, sizeof(double), hipMemcpyHostToDevice, c->st));
@@ -199,6 +298,16 @@ static int phase_geometry(fz_case *cs) {
   fz_ctx *c = cs->ctx;
   const fz_params &P = cs->prm;
   double **f = cs->f;
+  if (cs->precision == FZ_PREC_MIXED) {
+    float **g = cs->g;
+    FZ_TRY(fz_acdi_cmpt_phi_f32(c, cs->sc[S_SEPS], g[F_PSI], g[F_PHI]));
+    FZ_TRY(fz_youngs_f32(c, g[F_PHI], g[F_NX], g[F_NY], g[F_NZ], g[F_KAPPA]));
+    FZ_TRY(fz_boundp_f32(c, cs->cbccur, cs->bccur, g[F_KAPPA]));
+    FZ_TRY(fz_boundp_f32(c, P.cbcnor + 0, P.bcnor + 0, g[F_NX]));
+    FZ_TRY(fz_boundp_f32(c, P.cbcnor + 6, P.bcnor + 6, g[F_NY]));
+    FZ_TRY(fz_boundp_f32(c, P.cbcnor + 12, P.bcnor + 12, g[F_NZ]));
+    return FZ_OK;
+  }
   if (!P.vof) FZ_TRY(fz_acdi_cmpt_phi(c, cs->sc[S_SEPS], f[F_PSI], f[F_PHI]));
   else FZ_TRY(fz_vof_thinc_cmpt_phi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_PSI], f[F_PHI]));
   FZ_TRY(fz_cmpt_norm_curv_youngs(c, f[F_PHI], f[F_NX], f[F_NY], f[F_NZ], f[F_KAPPA]));
@@ -239,7 +348,8 @@ static int finish_init(fz_case *cs) {
   FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
   FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));
   FZ_TRY(fz_copy(c, f[F_PO], f[F_P], sizeof(double) * c->nh));
-  FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
+  if (cs->precision == FZ_PREC_MIXED) FZ_TRY(fz_boundp_f32(c, P.cbcpsi, P.bcpsi, cs->g[F_PSI]));
+  else FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
   FZ_TRY(phase_geometry(cs));
   if (!P.vof) {
     FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
@@ -255,6 +365,7 @@ static int finish_init(fz_case *cs) {
 }
 
 static int set_epsilon(fz_case *cs) {
+  drop_graphs(cs);
   double seps;
   FZ_TRY(fz_acdi_set_epsilon(cs->ctx, cs->prm.psi_thickness_factor, &seps));  // src/main.f90:301
   cs->sc[S_SEPS] = seps;
@@ -267,7 +378,12 @@ extern "C" int fz_case_init(fz_case *cs) {
   double **f = cs->f;
   FZ_TRY(set_epsilon(cs));
   for (int q = F_U; q <= F_P; q++) FZ_TRY(fz_launch_fill(c, c->nh, 0.0, f[q]));  // initflow 'zer'
-  FZ_TRY(fz_init2fl(c, &cs->prm, cs->sc[S_SEPS], f[F_PSI]));
+  if (cs->precision == FZ_PREC_MIXED) {  // the FP64 initial field, rounded
+    FZ_TRY(fz_init2fl(c, &cs->prm, cs->sc[S_SEPS], cs->dp));
+    FZ_TRY(to_f32(c, c->nh, cs->dp, cs->g[F_PSI]));
+  } else {
+    FZ_TRY(fz_init2fl(c, &cs->prm, cs->sc[S_SEPS], f[F_PSI]));
+  }
   cs->sc[S_TIME] = 0.0;
   cs->sc[S_ISTEP] = 0.0;
   return finish_init(cs);
@@ -284,7 +400,12 @@ extern "C" int fz_case_save(fz_case *cs, const char *prefix) {
   int istep = (int)cs->sc[S_ISTEP];
   for (int q = 0; q < 5; q++) {
     const std::string name = std::string(prefix) + "_" + kFieldExt[q] + ".bin";
-    FZ_TRY(fz_load_one(cs->ctx, 'w', name.c_str(), cs->f[kFieldId[q]], &time, &istep));
+    double *src = cs->f[kFieldId[q]];
+    if (!src) {  // an FP32 field, written in the file's FP64 format
+      FZ_TRY(to_f64(cs->ctx, cs->ctx->nh, cs->g[kFieldId[q]], cs->dp));
+      src = cs->dp;
+    }
+    FZ_TRY(fz_load_one(cs->ctx, 'w', name.c_str(), src, &time, &istep));
   }
   return FZ_OK;
 }
@@ -296,7 +417,9 @@ extern "C" int fz_case_restart(fz_case *cs, const char *prefix) {
   int istep = 0;
   for (int q = 0; q < 5; q++) {
     const std::string name = std::string(prefix) + "_" + kFieldExt[q] + ".bin";
-    FZ_TRY(fz_load_one(cs->ctx, 'r', name.c_str(), cs->f[kFieldId[q]], &time, &istep));
+    double *dst = cs->f[kFieldId[q]] ? cs->f[kFieldId[q]] : cs->dp;
+    FZ_TRY(fz_load_one(cs->ctx, 'r', name.c_str(), dst, &time, &istep));
+    if (!cs->f[kFieldId[q]]) FZ_TRY(to_f32(cs->ctx, cs->ctx->nh, cs->dp, cs->g[kFieldId[q]]));
   }
   cs->sc[S_TIME] = time;
   cs->sc[S_ISTEP] = (double)istep;
@@ -310,93 +433,239 @@ static void swap_ptr(T *&a, T *&b) {
   b = t;
 }
 
+// The device work of one time step (src/main.f90:414-529); pointer
+// rotations happen on the host as the work is enqueued. Host
+// synchronisation only for a bulk mean that the momentum needs by value.
+struct StepArgs {
+  double dt, factor1, factor2, factor12, dt_r;
+  bool first;
+  const double *stc;  // the same scalars in device memory, or null
+};
+
+static int enqueue_step(fz_case *cs, const StepArgs &a) {
+  fz_ctx *c = cs->ctx;
+  const fz_params &P = cs->prm;
+  double **f = cs->f;
+  float **g = cs->g;
+  const bool mixed = cs->precision == FZ_PREC_MIXED;
+  const double rho0 = cs->sc[S_RHO0];
+  const long bi = sizeof(double) * c->ni;
+  const double dt = a.dt, dt_r = a.dt_r, factor1 = a.factor1, factor2 = a.factor2, factor12 = a.factor12;
+  const bool is_first = a.first;
+  // psio <- psi by rotation: the update writes into the older buffer
+  swap_ptr(cs->psio, f[F_PSI]);
+  swap_ptr(cs->psio32, g[F_PSI]);
+  if (mixed) {  // the same sequence on the FP32 phase field
+    FZ_TRY(fz_launch_acdi_step<float>(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS],
+                                      factor1, factor2, a.stc, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], g[F_NX], g[F_NY],
+                                      g[F_NZ], g[F_PHI], cs->psio32, g[F_DPSIO], g[F_PSI], g[F_FX], g[F_FY],
+                                      g[F_FZ]));
+    FZ_TRY(fz_bounduvw_f32(c, P.cbcvel, P.bcvel, 0, g[F_FX], g[F_FY], g[F_FZ]));
+    FZ_TRY(fz_boundp_f32(c, P.cbcpsi, P.bcpsi, g[F_PSI]));
+    FZ_TRY(phase_geometry(cs));
+  } else if (P.is_track_interface) {
+    if (!P.vof) {
+      // transport, AB2 update and clip in one pass; the history is updated in place
+      FZ_TRY(fz_launch_acdi_step<double>(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS],
+                                           factor1, factor2, a.stc, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ],
+                                 f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ]));
+    } else {
+      FZ_TRY(fz_vof_thinc_transport_psi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_U], f[F_V], f[F_W], f[F_NX],
+                                        f[F_NY], f[F_NZ], f[F_PHI], cs->psio, cs->dpsidt, f[F_FX], f[F_FY],
+                                        f[F_FZ]));
+      if (is_first) FZ_HIP(hipMemcpyAsync(f[F_DPSIO], cs->dpsidt, bi, hipMemcpyDeviceToDevice, c->st));
+      FZ_LAUNCH(c, "rk_psi_oop", k_rk_psi_oop, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1, factor2,
+                cs->dpsidt, f[F_DPSIO], cs->psio, f[F_PSI]);
+      FZ_LAUNCHED();
+      const double mm[2] = {0.0, 1.0};
+      FZ_TRY(fz_clip_field(c, mm, f[F_PSI]));  // src/rk.f90:248
+      swap_ptr(cs->dpsidt, f[F_DPSIO]);         // src/rk.f90:252
+    }
+    FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_FX], f[F_FY], f[F_FZ]));
+    FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
+    FZ_TRY(phase_geometry(cs));
+  } else {
+    FZ_HIP(hipMemcpyAsync(f[F_PSI], cs->psio, sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
+    for (int q = F_FX; q <= F_FZ; q++) FZ_TRY(fz_launch_fill(c, c->nh, 0.0, f[q]));
+  }
+  if (P.is_solve_ns) {
+    double rho_av[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; d++) {
+      if (fabs(P.gacc[d]) > 0.0 && c->per[d]) {  // src/main.f90:470-477
+        if (mixed)
+          FZ_TRY(fz_bulk_mean_dev(c, d + 1, g[F_PSI], P.rho12, d < 2 ? cs->gvr_c : cs->gvr_f, cs->dsc + 4 + d));
+        else
+          FZ_TRY(fz_bulk_mean_dev(c, d + 1, f[F_PSI], P.rho12, d < 2 ? cs->gvr_c : cs->gvr_f, cs->dsc + 4 + d));
+        FZ_HIP(hipMemcpyAsync(c->h_small + 8, cs->dsc + 4 + d, sizeof(double), hipMemcpyDeviceToHost, c->st));
+        FZ_HIP(hipStreamSynchronize(c->st));
+        rho_av[d] = c->h_small[8];
+      }
+    }
+    // rk, src/rk.f90:72-130, in one pass into the spare velocity buffers
+    if (mixed)
+      FZ_TRY((fz_launch_mom_fused<float, float>(c, P.rho12, P.mu12, factor1, factor2, is_first ? 1 : 0, factor12,
+                                               dt_r, a.stc, P.sigma, rho0, P.bforce, P.gacc, rho_av, g[F_FX], g[F_FY],
+                                               g[F_FZ], f[F_U], f[F_V], f[F_W], g[F_PSI], cs->psio32, f[F_P],
+                                               f[F_PO], g[F_KAPPA], g[F_DUO], g[F_DVO], g[F_DWO], cs->un, cs->vn,
+                                               cs->wn)));
+    else
+      FZ_TRY((fz_launch_mom_fused<double, double>(c, P.rho12, P.mu12, factor1, factor2, is_first ? 1 : 0, factor12,
+                                                 dt_r, a.stc, P.sigma, rho0, P.bforce, P.gacc, rho_av, f[F_FX], f[F_FY],
+                                                 f[F_FZ], f[F_U], f[F_V], f[F_W], f[F_PSI], cs->psio, f[F_P],
+                                                 f[F_PO], f[F_KAPPA], f[F_DUO], f[F_DVO], f[F_DWO], cs->un, cs->vn,
+                                                 cs->wn)));
+    swap_ptr(f[F_U], cs->un);
+    swap_ptr(f[F_V], cs->vn);
+    swap_ptr(f[F_W], cs->wn);
+    if (P.is_forced_hit) FZ_TRY(fz_lscale_forcing(c, 2, 0.5, dt, f[F_U], f[F_V], f[F_W]));
+    FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
+    // projection on the correction buffer dp, src/main.f90:488-502
+    if (mixed) FZ_TRY(fz_projection_rhs_solve_mixed(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp, cs->nref,
+                                                      a.stc));
+    else FZ_TRY(fz_projection_rhs_solve(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp, a.stc));
+    FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, cs->dp));
+    // correc into the spare velocity buffers, updatep into the buffer of the
+    // previous pressure and the speeds of acdi_set_gamma, in one pass
+    FZ_TRY(fz_launch_correc_step(c, rho0, dt, a.stc, cs->dp, f[F_P], f[F_U], f[F_V], f[F_W], cs->un, cs->vn, cs->wn,
+                                 f[F_PO], P.vof ? nullptr : cs->gslots));
+    swap_ptr(f[F_U], cs->un);
+    swap_ptr(f[F_V], cs->vn);
+    swap_ptr(f[F_W], cs->wn);
+    swap_ptr(f[F_P], f[F_PO]);  // p <- p + dp, po <- p of the step before (= pn)
+    FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 1, f[F_U], f[F_V], f[F_W]));
+    FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));
+  } else {
+    FZ_HIP(hipMemcpyAsync(f[F_PO], f[F_P], sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
+  }
+  if (!P.vof) {
+    if (P.is_solve_ns)
+      FZ_TRY(fz_launch_gamma_fin(c, cs->gslots, P.acdi_gam_factor, P.acdi_gam_min, cs->dsc));
+    else FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
+  }
+  return FZ_OK;
+}
+
+// Steps after the first replay a captured graph of enqueue_step when the
+// case qualifies: every kernel argument that changes between steps is read
+// from device memory (gamma, and dt with its AB2 factors), the step has no
+// host synchronisation and no memory-copy nodes, and per-launch profiling
+// is off. The pointer rotations give two graphs, one per parity, keyed by
+// the buffers that hold p and psi when the step starts.
+static bool graph_ok(const fz_case *cs) {
+  const fz_params &P = cs->prm;
+  if (!cs->use_graphs || cs->ctx->prof_on) return false;
+  if (!P.is_track_interface || P.vof || !P.is_solve_ns || P.is_forced_hit) return false;
+  for (int d = 0; d < 3; d++)
+    if (fabs(P.gacc[d]) > 0.0 && cs->ctx->per[d]) return false;  // bulk mean read back by the host
+  return fz_projection_device_scalars_ok(cs->ctx, cs->precision == FZ_PREC_MIXED) != 0;
+}
+
+__global__ void k_set_stc(double *__restrict__ stc, double dt, double dti, double f1, double f2, double f12,
+                          double dtr) {
+  stc[FZ_STC_DT] = dt;
+  stc[FZ_STC_DTI] = dti;
+  stc[FZ_STC_F1] = f1;
+  stc[FZ_STC_F2] = f2;
+  stc[FZ_STC_F12] = f12;
+  stc[FZ_STC_DTR] = dtr;
+}
+
+static int set_stc(fz_case *cs, const StepArgs &a) {
+  const double v[FZ_STC_N] = {a.dt, 1.0 / a.dt, a.factor1, a.factor2, a.factor12, a.dt_r};
+  if (cs->stc_valid && memcmp(v, cs->stc_host, sizeof(v)) == 0) return FZ_OK;
+  FZ_LAUNCH(cs->ctx, "set_stc", k_set_stc, dim3(1), dim3(1), 0, cs->dstc, v[0], v[1], v[2], v[3], v[4], v[5]);
+  FZ_LAUNCHED();
+  memcpy(cs->stc_host, v, sizeof(v));
+  cs->stc_valid = true;
+  return FZ_OK;
+}
+
+static void drop_graphs(fz_case *cs) {
+  for (int q = 0; q < 2; q++) {
+    if (cs->graph[q].exec) (void)hipGraphExecDestroy(cs->graph[q].exec);
+    cs->graph[q] = fz_case::StepGraph();
+  }
+}
+
+// one step through a captured graph (captured on the first use per parity)
+static int graph_step(fz_case *cs, const StepArgs &a) {
+  fz_ctx *c = cs->ctx;
+  const void *k0 = cs->f[F_P];
+  const void *k1 = cs->precision == FZ_PREC_MIXED ? (const void *)cs->g[F_PSI] : (const void *)cs->f[F_PSI];
+  int slot = -1;
+  for (int q = 0; q < 2; q++)
+    if (cs->graph[q].exec && cs->graph[q].key[0] == k0 && cs->graph[q].key[1] == k1) slot = q;
+  if (slot < 0) {
+    slot = cs->graph[0].exec ? 1 : 0;
+    if (cs->graph[slot].exec) drop_graphs(cs), slot = 0;
+    // capture from a copy of the host-side pointer state; the launches
+    // then rotate the real state exactly as the replay will
+    FZ_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(cs, a);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->st, &gr);
+    if (rc != FZ_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return fz_set_error(FZ_ERR_HIP, "step graph capture: %s", hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ei != hipSuccess) return fz_set_error(FZ_ERR_HIP, "step graph instantiate: %s", hipGetErrorString(ei));
+    cs->graph[slot].exec = ex;
+    cs->graph[slot].key[0] = k0;
+    cs->graph[slot].key[1] = k1;
+    // the capture recorded the work without running it
+    FZ_HIP(hipGraphLaunch(ex, c->st));
+    return FZ_OK;
+  }
+  // replay: rotate the host-side pointers as enqueue_step does
+  FZ_HIP(hipGraphLaunch(cs->graph[slot].exec, c->st));
+  double **f = cs->f;
+  swap_ptr(cs->psio, f[F_PSI]);
+  swap_ptr(cs->psio32, cs->g[F_PSI]);
+  swap_ptr(f[F_P], f[F_PO]);
+  return FZ_OK;
+}
+
+extern "C" int fz_case_set_graphs(fz_case *cs, int on) {
+  if (!cs) return fz_set_error(FZ_ERR_ARG, "fz_case_set_graphs: null case");
+  if (!on) drop_graphs(cs);
+  cs->use_graphs = on != 0;
+  return FZ_OK;
+}
+
 extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
   if (!cs || nsteps < 0) return fz_set_error(FZ_ERR_ARG, "fz_case_step: bad argument");
   fz_ctx *c = cs->ctx;
   const fz_params &P = cs->prm;
   double **f = cs->f;
-  const double rho0 = cs->sc[S_RHO0];
-  const long bi = sizeof(double) * c->ni;
   int kill = 0;
   for (int s = 0; s < nsteps && !kill; s++) {
-    const double dt = cs->sc[S_DT], dto = cs->sc[S_DTO];
-    const bool is_first = cs->sc[S_FIRST] != 0.0;
+    StepArgs a;
+    a.dt = cs->sc[S_DT];
+    const double dto = cs->sc[S_DTO];
+    a.first = cs->sc[S_FIRST] != 0.0;
     cs->sc[S_ISTEP] += 1.0;
-    cs->sc[S_TIME] += dt;
-    const double dt_r = dt / dto;  // src/main.f90:422-424, src/rk.f90:53-55
-    const double tm1 = 1.0 + 0.5 * dt_r, tm2 = -0.5 * dt_r;
-    const double factor1 = tm1 * dt, factor2 = tm2 * dt, factor12 = factor1 + factor2;
-    // psio <- psi by rotation: the update writes into the older buffer
-    swap_ptr(cs->psio, f[F_PSI]);
-    if (P.is_track_interface) {
-      if (!P.vof) {
-        // transport, AB2 update and clip in one pass; the history is updated in place
-        FZ_TRY(fz_launch_acdi_step(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS], factor1,
-                                   factor2, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ],
-                                   f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ]));
-      } else {
-        FZ_TRY(fz_vof_thinc_transport_psi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_U], f[F_V], f[F_W], f[F_NX],
-                                          f[F_NY], f[F_NZ], f[F_PHI], cs->psio, cs->dpsidt, f[F_FX], f[F_FY],
-                                          f[F_FZ]));
-        if (is_first) FZ_HIP(hipMemcpyAsync(f[F_DPSIO], cs->dpsidt, bi, hipMemcpyDeviceToDevice, c->st));
-        FZ_LAUNCH(c, "rk_psi_oop", k_rk_psi_oop, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1, factor2,
-                  cs->dpsidt, f[F_DPSIO], cs->psio, f[F_PSI]);
-        FZ_LAUNCHED();
-        const double mm[2] = {0.0, 1.0};
-        FZ_TRY(fz_clip_field(c, mm, f[F_PSI]));  // src/rk.f90:248
-        swap_ptr(cs->dpsidt, f[F_DPSIO]);         // src/rk.f90:252
-      }
-      FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_FX], f[F_FY], f[F_FZ]));
-      FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
-      FZ_TRY(phase_geometry(cs));
+    cs->sc[S_TIME] += a.dt;
+    a.dt_r = a.dt / dto;  // src/main.f90:422-424, src/rk.f90:53-55
+    const double tm1 = 1.0 + 0.5 * a.dt_r, tm2 = -0.5 * a.dt_r;
+    a.factor1 = tm1 * a.dt;
+    a.factor2 = tm2 * a.dt;
+    a.factor12 = a.factor1 + a.factor2;
+    const bool use_graph = !a.first && graph_ok(cs);
+    a.stc = nullptr;
+    if (use_graph) {
+      a.stc = cs->dstc;
+      FZ_TRY(set_stc(cs, a));
+      FZ_TRY(graph_step(cs, a));
     } else {
-      FZ_HIP(hipMemcpyAsync(f[F_PSI], cs->psio, sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
-      for (int q = F_FX; q <= F_FZ; q++) FZ_TRY(fz_launch_fill(c, c->nh, 0.0, f[q]));
-    }
-    if (P.is_solve_ns) {
-      double rho_av[3] = {0.0, 0.0, 0.0};
-      for (int d = 0; d < 3; d++) {
-        if (fabs(P.gacc[d]) > 0.0 && c->per[d]) {  // src/main.f90:470-477
-          FZ_TRY(fz_bulk_mean_dev(c, d + 1, f[F_PSI], P.rho12, d < 2 ? cs->gvr_c : cs->gvr_f, cs->dsc + 4 + d));
-          FZ_HIP(hipMemcpyAsync(c->h_small + 8, cs->dsc + 4 + d, sizeof(double), hipMemcpyDeviceToHost, c->st));
-          FZ_HIP(hipStreamSynchronize(c->st));
-          rho_av[d] = c->h_small[8];
-        }
-      }
-      // rk, src/rk.f90:72-130, in one pass into the spare velocity buffers
-      FZ_TRY(fz_launch_mom_fused(c, P.rho12, P.mu12, factor1, factor2, is_first ? 1 : 0, factor12, dt_r, P.sigma,
-                                 rho0, P.bforce, P.gacc, rho_av, f[F_FX], f[F_FY], f[F_FZ], f[F_U], f[F_V], f[F_W],
-                                 f[F_PSI], cs->psio, f[F_P], f[F_PO], f[F_KAPPA], f[F_DUO], f[F_DVO], f[F_DWO],
-                                 cs->un, cs->vn, cs->wn));
-      swap_ptr(f[F_U], cs->un);
-      swap_ptr(f[F_V], cs->vn);
-      swap_ptr(f[F_W], cs->wn);
-      if (P.is_forced_hit) FZ_TRY(fz_lscale_forcing(c, 2, 0.5, dt, f[F_U], f[F_V], f[F_W]));
-      FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
-      // projection on the correction buffer dp, src/main.f90:488-502
-      FZ_TRY(fz_projection_rhs_solve(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp));
-      FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, cs->dp));
-      // correc into the spare velocity buffers, updatep into the buffer of the
-      // previous pressure and the speeds of acdi_set_gamma, in one pass
-      FZ_TRY(fz_launch_correc_step(c, rho0, dt, cs->dp, f[F_P], f[F_U], f[F_V], f[F_W], cs->un, cs->vn, cs->wn,
-                                   f[F_PO], P.vof ? nullptr : cs->gslots));
-      swap_ptr(f[F_U], cs->un);
-      swap_ptr(f[F_V], cs->vn);
-      swap_ptr(f[F_W], cs->wn);
-      swap_ptr(f[F_P], f[F_PO]);  // p <- p + dp, po <- p of the step before (= pn)
-      FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 1, f[F_U], f[F_V], f[F_W]));
-      FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));
-    } else {
-      FZ_HIP(hipMemcpyAsync(f[F_PO], f[F_P], sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
+      FZ_TRY(enqueue_step(cs, a));
     }
     cs->sc[S_FIRST] = 0.0;
-    cs->sc[S_DTO] = dt;  // dto = dt, src/main.f90:523
-    if (!P.vof) {
-      if (P.is_solve_ns)
-        FZ_TRY(fz_launch_gamma_fin(c, cs->gslots, P.acdi_gam_factor, P.acdi_gam_min, cs->dsc));
-      else FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
-      cs->gam_on_device = true;
-    }
+    cs->sc[S_DTO] = a.dt;  // dto = dt, src/main.f90:523
+    if (!P.vof) cs->gam_on_device = true;
     const int istep = (int)cs->sc[S_ISTEP];
     if (P.icheck > 0 && istep % P.icheck == 0) {  // src/main.f90:530-562
       double dtn;

@@ -46,6 +46,7 @@ struct fz_fftplan {
   double2 *tw = nullptr;  // exp(-2 pi i j / m), j < m
   double2 *q = nullptr;   // (cos, sin)(pi k / 2n), k <= n
   double2 *t2 = nullptr;  // (cos, sin)(pi (2k+1) / 4n), k < n
+  float2 *tw32 = nullptr, *q32 = nullptr, *t232 = nullptr;  // the same tables rounded to FP32
 };
 
 struct fz_poisson {
@@ -59,6 +60,10 @@ struct fz_poisson {
   double *aux = nullptr;    // periodic-z second solve, n1*n2*n3
   int periodic_z = 0;
   int rhsb_zero = 0;  // all boundary right-hand sides vanish (homogeneous pressure BCs)
+  long null_col = -1;  // spectral column of the operator's null mode, or -1
+  float *work32 = nullptr;              // FP32 spectral buffer of the mixed-precision solve
+  double *bcopy = nullptr;              // right-hand side kept for the residuals (halo'd layout)
+  unsigned long long *rslots = nullptr; // max |residual| slots (FP64 bits)
   char cbc[6];
 };
 
@@ -147,8 +152,14 @@ int fz_reduce_ensure(fz_ctx *ctx, size_t nblocks);
 void fz_bcmaps_free(fz_ctx *ctx);
 int fz_poisson_destroy(fz_poisson *p);
 int fz_solver_dev(fz_ctx *ctx, double *p);
+// a case's per-step scalars in device memory (kernels read them when given
+// a pointer, so a captured step graph serves any dt)
+enum { FZ_STC_DT = 0, FZ_STC_DTI, FZ_STC_F1, FZ_STC_F2, FZ_STC_F12, FZ_STC_DTR, FZ_STC_N };
 int fz_projection_rhs_solve(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v, const double *w,
-                            double *p);
+                            double *p, const double *stc = nullptr);
+int fz_projection_rhs_solve_mixed(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v,
+                                  const double *w, double *p, int nref, const double *stc = nullptr);
+int fz_projection_device_scalars_ok(fz_ctx *ctx, int mixed);
 int fz_acdi_gamma_dev(fz_ctx *ctx, double gam_factor, double gam_min, const double *u, const double *v,
                       const double *w, double *d_gam);
 int fz_acdi_transport_dev(fz_ctx *ctx, const double *d_gam, double gam, double seps, const double *u,

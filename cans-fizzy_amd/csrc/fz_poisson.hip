@@ -22,30 +22,88 @@ struct RadList {
 struct LineMap {  // element e of line (l1,l2) sits at base + l1*s1 + l2*s2 + e*se
   long base, s1, s2, se;
 };
-// fillps (src/fillps.f90:33-50) evaluated in the load of the forward x
-// transform: the line element is the divergence at the cell instead of a
-// value read from memory (u, v, w halo'd fields; k = l2 + 1)
+// The line source of a forward x transform when it is not a plain array:
+// fillps (src/fillps.f90:33-50) evaluated in the load (u, v, w halo'd
+// fields; k = l2 + 1), or, for the residual iterations of the mixed-precision
+// solve, a right-hand side minus the discrete operator applied to the current
+// solution x (FP64). The optional slots take the maximum |value| (FP64 bits).
+enum { SRC_ARRAY = 0, SRC_FILLPS = 1, SRC_FILLPS_RES = 2, SRC_B_RES = 3, SRC_B = 4 };
 struct FillSrc {
   const double *u, *v, *w, *dzfi;
   double rho0, dti, dtidxi, dtidyi;
   long sy, sz;
   int on;
+  const double *x, *b, *dzci;  // current solution (halo'd), right-hand side (halo'd layout)
+  double dxi2, dyi2;
+  unsigned long long *rmax;    // FZ_GAMMA_SLOTS slots, or null
+  const double *stc;           // the case's device scalars (1/dt from there), or null
+  double dxi, dyi;
 };
+
+// the FP64 value of the source at halo index a (z index k)
+__device__ inline double src_value(const FillSrc &fs, long a, int k) {
+  double val;
+  if (fs.on == SRC_FILLPS || fs.on == SRC_FILLPS_RES) {
+    const double dti = fs.stc ? fs.stc[FZ_STC_DTI] : fs.dti;
+    const double dtidxi = fs.stc ? dti * fs.dxi : fs.dtidxi, dtidyi = fs.stc ? dti * fs.dyi : fs.dtidyi;
+    val = fs.rho0 * ((fs.w[a] - fs.w[a - fs.sz]) * dti * fs.dzfi[k] + (fs.v[a] - fs.v[a - fs.sy]) * dtidyi +
+                     (fs.u[a] - fs.u[a - 1]) * dtidxi);
+  } else {
+    val = fs.b[a];
+  }
+  if (fs.on == SRC_FILLPS_RES || fs.on == SRC_B_RES) {
+    // the operator the transforms invert: 3-point second differences in x
+    // and y, and in z the tridiagonal of src/initsolver.f90:153-182 (the
+    // boundary rows follow from the halo values of homogeneous conditions)
+    const double *x = fs.x;
+    const double xc = x[a];
+    const double lap = ((x[a + 1] - xc) - (xc - x[a - 1])) * fs.dxi2 +
+                       ((x[a + fs.sy] - xc) - (xc - x[a - fs.sy])) * fs.dyi2 +
+                       ((x[a + fs.sz] - xc) * fs.dzci[k] - (xc - x[a - fs.sz]) * fs.dzci[k - 1]) * fs.dzfi[k];
+    val -= lap;
+  }
+  return val;
+}
+
+// maximum of m over the wave, one atomic per wave into the slots
+__device__ inline void slots_max(unsigned long long *slots, double m, unsigned wave_id) {
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(slots + (wave_id & (FZ_GAMMA_SLOTS - 1)), (unsigned long long)__double_as_longlong(m));
+}
 
 static __host__ __device__ inline bool kind_is_quarter(int kind) { return kind == K_REDFT11 || kind == K_RODFT11; }
 static __host__ __device__ inline bool kind_is_forward(int kind) {
   return kind == K_R2HC || kind == K_REDFT10 || kind == K_RODFT10 || kind == K_REDFT11 || kind == K_RODFT11;
 }
 
+// Transforms are written once for the real type R: double for the plain
+// solver, float for the mixed-precision one.
+template <typename R>
+struct CT;
+template <>
+struct CT<double> {
+  using C = double2;
+};
+template <>
+struct CT<float> {
+  using C = float2;
+};
+__device__ inline double2 mkc(double x, double y) { return make_double2(x, y); }
+__device__ inline float2 mkc(float x, float y) { return make_float2(x, y); }
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
 // radix-4 butterfly in place on twiddled inputs; sign -1 forward, +1 inverse
-__device__ inline void bfly4(double2 *v, double sign) {
-  const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-  const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-  const double2 r = make_double2(-sign * d13.y, sign * d13.x);  // i*sign*d13
+template <typename C, typename R>
+__device__ inline void bfly4(C *v, R sign) {
+  const C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+  const C s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+  const C r = mkc(-sign * d13.y, sign * d13.x);  // i*sign*d13
   v[0] = cadd(s02, s13);
   v[1] = cadd(d02, r);
   v[2] = csub(s02, s13);
@@ -53,22 +111,24 @@ __device__ inline void bfly4(double2 *v, double sign) {
 }
 
 // One Stockham pass of radix R over nfft complex lines of length m (LDS to LDS).
-__device__ inline void stockham_pass(const double2 *__restrict__ X, double2 *__restrict__ Y, int nfft, int m, int R,
-                                     int Ns, double sign, const double2 *__restrict__ tw) {
+template <typename T>
+__device__ inline void stockham_pass(const typename CT<T>::C *__restrict__ X, typename CT<T>::C *__restrict__ Y,
+                                     int nfft, int m, int R, int Ns, T sign, const typename CT<T>::C *__restrict__ tw) {
+  using C = typename CT<T>::C;
   const int nb = m / R;
   const int total = nfft * nb;
   const int tstride = m / (Ns * R);
   for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
     const int f = idx / nb, b = idx - f * nb;
-    const double2 *x = X + (long)f * m;
-    double2 *y = Y + (long)f * m;
+    const C *x = X + (long)f * m;
+    C *y = Y + (long)f * m;
     const int k = b % Ns;
     const int base = (b / Ns) * Ns * R + k;
-    double2 v[16];
+    C v[16];
     for (int t = 0; t < R; t++) {
-      double2 xv = x[b + t * nb];
+      C xv = x[b + t * nb];
       if (t > 0 && k > 0) {
-        double2 w = tw[t * k * tstride];  // exp(-2 pi i j/m); the inverse uses its conjugate
+        C w = tw[t * k * tstride];  // exp(-2 pi i j/m); the inverse uses its conjugate
         if (sign > 0) w.y = -w.y;
         xv = cmul(xv, w);
       }
@@ -81,21 +141,21 @@ __device__ inline void stockham_pass(const double2 *__restrict__ X, double2 *__r
       bfly4(v, sign);
       for (int t = 0; t < 4; t++) y[base + t * Ns] = v[t];
     } else if (R == 3) {
-      const double c3 = 0.86602540378443864676;  // sqrt(3)/2
-      const double2 t1 = cadd(v[1], v[2]);
-      const double2 t2 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
-      const double2 d = csub(v[1], v[2]);
-      const double2 t3 = make_double2(-sign * c3 * d.y, sign * c3 * d.x);
+      const T c3 = (T)0.86602540378443864676;  // sqrt(3)/2
+      const C t1 = cadd(v[1], v[2]);
+      const C t2 = mkc(v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y);
+      const C d = csub(v[1], v[2]);
+      const C t3 = mkc(-sign * c3 * d.y, sign * c3 * d.x);
       y[base] = cadd(v[0], t1);
       y[base + Ns] = cadd(t2, t3);
       y[base + 2 * Ns] = csub(t2, t3);
     } else {
       const int st = m / R;
       for (int s = 0; s < R; s++) {
-        double2 acc = v[0];
+        C acc = v[0];
         for (int t = 1; t < R; t++) {
-          const double2 w0 = tw[((t * s) % R) * st];
-          const double2 w = make_double2(w0.x, sign > 0 ? -w0.y : w0.y);
+          const C w0 = tw[((t * s) % R) * st];
+          const C w = mkc(w0.x, sign > 0 ? -w0.y : w0.y);
           acc = cadd(acc, cmul(v[t], w));
         }
         y[base + s * Ns] = acc;
@@ -105,105 +165,109 @@ __device__ inline void stockham_pass(const double2 *__restrict__ X, double2 *__r
 }
 
 // complex input k of packed line f, from the real lines staged in Rw (n values each)
-__device__ inline double2 pre_value(int kind, const double *__restrict__ Rw, int f, int k, int n,
-                                  const double2 *__restrict__ q) {
-  double2 z;
+template <typename R>
+__device__ inline typename CT<R>::C pre_value(int kind, const R *__restrict__ Rw, int f, int k, int n,
+                                              const typename CT<R>::C *__restrict__ q) {
+  using C = typename CT<R>::C;
+  C z;
   switch (kind) {
-    case K_R2HC: z = make_double2(Rw[(2 * f) * n + k], Rw[(2 * f + 1) * n + k]); break;
+    case K_R2HC: z = mkc(Rw[(2 * f) * n + k], Rw[(2 * f + 1) * n + k]); break;
     case K_REDFT10:
     case K_RODFT10: {
       const int e = (k <= (n - 1) / 2) ? 2 * k : 2 * (n - 1 - k) + 1;
-      const double sg = (kind == K_RODFT10 && (e & 1)) ? -1.0 : 1.0;
-      z = make_double2(sg * Rw[(2 * f) * n + e], sg * Rw[(2 * f + 1) * n + e]);
+      const R sg = (kind == K_RODFT10 && (e & 1)) ? (R)-1.0 : (R)1.0;
+      z = mkc(sg * Rw[(2 * f) * n + e], sg * Rw[(2 * f + 1) * n + e]);
       break;
     }
     case K_REDFT11:
     case K_RODFT11: {
       if (k < n) {
-        const double sg = (kind == K_RODFT11 && (k & 1)) ? -1.0 : 1.0;
-        const double x = sg * Rw[f * n + k];
-        z = make_double2(x * q[k].x, -x * q[k].y);
+        const R sg = (kind == K_RODFT11 && (k & 1)) ? (R)-1.0 : (R)1.0;
+        const R x = sg * Rw[f * n + k];
+        z = mkc(x * q[k].x, -x * q[k].y);
       } else {
-        z = make_double2(0.0, 0.0);
+        z = mkc((R)0.0, (R)0.0);
       }
       break;
     }
     case K_HC2R: {
-      const double *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
-      double ra, ia, rb, ib;
+      const R *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
+      R ra, ia, rb, ib;
       if (k == 0) { ra = Ha[0]; ia = 0.0; rb = Hb[0]; ib = 0.0; }
       else if (2 * k < n) { ra = Ha[k]; ia = Ha[n - k]; rb = Hb[k]; ib = Hb[n - k]; }
       else if (2 * k == n) { ra = Ha[k]; ia = 0.0; rb = Hb[k]; ib = 0.0; }
       else { ra = Ha[n - k]; ia = -Ha[k]; rb = Hb[n - k]; ib = -Hb[k]; }
-      z = make_double2(ra - ib, ia + rb);
+      z = mkc(ra - ib, ia + rb);
       break;
     }
     case K_REDFT01:
     case K_RODFT01: {
-      const double *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
-      double xa, xna, xb, xnb;
+      const R *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
+      R xa, xna, xb, xnb;
       if (kind == K_REDFT01) {
         xa = Ha[k]; xb = Hb[k];
-        xna = (k == 0) ? 0.0 : Ha[n - k];
-        xnb = (k == 0) ? 0.0 : Hb[n - k];
+        xna = (k == 0) ? (R)0.0 : Ha[n - k];
+        xnb = (k == 0) ? (R)0.0 : Hb[n - k];
       } else {  // input reversed
         xa = Ha[n - 1 - k]; xb = Hb[n - 1 - k];
-        xna = (k == 0) ? 0.0 : Ha[k - 1];
-        xnb = (k == 0) ? 0.0 : Hb[k - 1];
+        xna = (k == 0) ? (R)0.0 : Ha[k - 1];
+        xnb = (k == 0) ? (R)0.0 : Hb[k - 1];
       }
-      const double cs = q[k].x, sn = q[k].y;
-      const double rca = xa * cs + xna * sn, ica = xa * sn - xna * cs;
-      const double rcb = xb * cs + xnb * sn, icb = xb * sn - xnb * cs;
-      z = make_double2(rca - icb, ica + rcb);
+      const R cs = q[k].x, sn = q[k].y;
+      const R rca = xa * cs + xna * sn, ica = xa * sn - xna * cs;
+      const R rcb = xb * cs + xnb * sn, icb = xb * sn - xnb * cs;
+      z = mkc(rca - icb, ica + rcb);
       break;
     }
-    default: z = make_double2(0.0, 0.0);
+    default: z = mkc((R)0.0, (R)0.0);
   }
   return z;
 }
 
 // real output k of line g from the transformed packed lines in cur (length m each)
-__device__ inline double post_value(int kind, const double2 *__restrict__ cur, int g, int k, int n, int m,
-                                  const double2 *__restrict__ q, const double2 *__restrict__ t2) {
-  double val;
+template <typename R>
+__device__ inline R post_value(int kind, const typename CT<R>::C *__restrict__ cur, int g, int k, int n, int m,
+                               const typename CT<R>::C *__restrict__ q, const typename CT<R>::C *__restrict__ t2) {
+  using C = typename CT<R>::C;
+  R val;
   switch (kind) {
     case K_R2HC: {
-      const double2 *Z = cur + (long)(g >> 1) * m;
+      const C *Z = cur + (long)(g >> 1) * m;
       const int kk = (2 * k <= n) ? k : n - k;
-      const double2 zk = Z[kk], znk = Z[(m - kk) % m];
-      double re, im;
-      if ((g & 1) == 0) { re = 0.5 * (zk.x + znk.x); im = 0.5 * (zk.y - znk.y); }
-      else { re = 0.5 * (zk.y + znk.y); im = 0.5 * (znk.x - zk.x); }
+      const C zk = Z[kk], znk = Z[(m - kk) % m];
+      R re, im;
+      if ((g & 1) == 0) { re = (R)0.5 * (zk.x + znk.x); im = (R)0.5 * (zk.y - znk.y); }
+      else { re = (R)0.5 * (zk.y + znk.y); im = (R)0.5 * (znk.x - zk.x); }
       val = (2 * k <= n) ? re : im;
       break;
     }
     case K_REDFT10:
     case K_RODFT10: {
-      const double2 *Z = cur + (long)(g >> 1) * m;
+      const C *Z = cur + (long)(g >> 1) * m;
       const int kk = (kind == K_REDFT10) ? k : n - 1 - k;
-      const double2 zk = Z[kk], znk = Z[(m - kk) % m];
-      double re, im;
-      if ((g & 1) == 0) { re = 0.5 * (zk.x + znk.x); im = 0.5 * (zk.y - znk.y); }
-      else { re = 0.5 * (zk.y + znk.y); im = 0.5 * (znk.x - zk.x); }
-      val = 2.0 * (q[kk].x * re + q[kk].y * im);
+      const C zk = Z[kk], znk = Z[(m - kk) % m];
+      R re, im;
+      if ((g & 1) == 0) { re = (R)0.5 * (zk.x + znk.x); im = (R)0.5 * (zk.y - znk.y); }
+      else { re = (R)0.5 * (zk.y + znk.y); im = (R)0.5 * (znk.x - zk.x); }
+      val = (R)2.0 * (q[kk].x * re + q[kk].y * im);
       break;
     }
     case K_REDFT11:
     case K_RODFT11: {
       const int kk = (kind == K_REDFT11) ? k : n - 1 - k;
-      const double2 zk = cur[(long)g * m + kk];
-      val = 2.0 * (t2[kk].x * zk.x + t2[kk].y * zk.y);
+      const C zk = cur[(long)g * m + kk];
+      val = (R)2.0 * (t2[kk].x * zk.x + t2[kk].y * zk.y);
       break;
     }
     case K_HC2R: {
-      const double2 zk = cur[(long)(g >> 1) * m + k];
+      const C zk = cur[(long)(g >> 1) * m + k];
       val = (g & 1) ? zk.y : zk.x;
       break;
     }
     case K_REDFT01:
     case K_RODFT01: {
       const int srcm = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
-      const double2 zk = cur[(long)(g >> 1) * m + srcm];
+      const C zk = cur[(long)(g >> 1) * m + srcm];
       val = (g & 1) ? zk.y : zk.x;
       if (kind == K_RODFT01 && (k & 1)) val = -val;
       break;
@@ -213,17 +277,31 @@ __device__ inline double post_value(int kind, const double2 *__restrict__ cur, i
   return val;
 }
 
-// Batched r2r line transform: G lines per block, all in LDS.
-__global__ void __launch_bounds__(256) k_r2r(const double *__restrict__ src, LineMap in, double *__restrict__ dst,
+// the stored value of an output element: scaled (in FP64 when the output is
+// FP64), or added to what the output holds (accumulating corrections)
+template <typename TO, typename R>
+__device__ inline void store_out(TO *dst, long a, R val, double scale, int accum) {
+  const double s = (double)val * scale;
+  if (accum) dst[a] = (TO)((double)dst[a] + s);
+  else dst[a] = (TO)s;
+}
+
+// Batched r2r line transform: G lines per block, all in LDS. R is the
+// arithmetic type, TI/TO the element types of the input and output arrays.
+template <typename R, typename TI, typename TO>
+__global__ void __launch_bounds__(256) k_r2r(const TI *__restrict__ src, LineMap in, TO *__restrict__ dst,
                                              LineMap out, int L1, int n, int m, int kind, int G, double scale,
-                                             const double2 *__restrict__ tw, const double2 *__restrict__ q,
-                                             const double2 *__restrict__ t2, int nrad, RadList rl) {
-  extern __shared__ double2 smem[];
+                                             int accum, const typename CT<R>::C *__restrict__ tw,
+                                             const typename CT<R>::C *__restrict__ q,
+                                             const typename CT<R>::C *__restrict__ t2, int nrad, RadList rl) {
+  using C = typename CT<R>::C;
+  extern __shared__ double2 smem_raw[];
+  C *smem = (C *)smem_raw;
   const bool quarter = kind_is_quarter(kind);
   const int nfft = quarter ? G : G / 2;
-  double2 *A = smem;
-  double2 *B = smem + (long)nfft * m;
-  double *Rw = (double *)B;  // raw staging, G*n doubles
+  C *A = smem;
+  C *B = smem + (long)nfft * m;
+  R *Rw = (R *)B;  // raw staging, G*n values
   const int l1_0 = blockIdx.x * G;
   const long l2 = blockIdx.y;
   const int gcount = min(G, L1 - l1_0);
@@ -233,32 +311,32 @@ __global__ void __launch_bounds__(256) k_r2r(const double *__restrict__ src, Lin
     int g, e;
     if (in.se == 1) { g = idx / n; e = idx - g * n; }
     else { e = idx / G; g = idx - e * G; }
-    Rw[g * n + e] = (g < gcount) ? src[in.base + (long)(l1_0 + g) * in.s1 + l2 * in.s2 + (long)e * in.se] : 0.0;
+    Rw[g * n + e] = (g < gcount) ? (R)src[in.base + (long)(l1_0 + g) * in.s1 + l2 * in.s2 + (long)e * in.se] : (R)0.0;
   }
   __syncthreads();
   // 2. pre-processing into complex lines
   const int totc = nfft * m;
   for (int idx = threadIdx.x; idx < totc; idx += blockDim.x) {
     const int f = idx / m, k = idx - f * m;
-    A[idx] = pre_value(kind, Rw, f, k, n, q);
+    A[idx] = pre_value<R>(kind, Rw, f, k, n, q);
   }
   __syncthreads();
   // 3. complex FFT passes
-  const double sign = kind_is_forward(kind) ? -1.0 : 1.0;
-  double2 *cur = A, *oth = B;
+  const R sign = kind_is_forward(kind) ? (R)-1.0 : (R)1.0;
+  C *cur = A, *oth = B;
   int Ns = 1;
   for (int s = 0; s < nrad; s++) {
-    const int R = rl.r[s];
-    stockham_pass(cur, oth, nfft, m, R, Ns, sign, tw);
+    const int Rd = rl.r[s];
+    stockham_pass<R>(cur, oth, nfft, m, Rd, Ns, sign, tw);
     __syncthreads();
-    double2 *t = cur; cur = oth; oth = t;
-    Ns *= R;
+    C *t = cur; cur = oth; oth = t;
+    Ns *= Rd;
   }
   // 4. post-processing into real lines
-  double *O = (double *)oth;
+  R *O = (R *)oth;
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int g = idx / n, k = idx - g * n;
-    O[idx] = post_value(kind, cur, g, k, n, m, q, t2);
+    O[idx] = post_value<R>(kind, cur, g, k, n, m, q, t2);
   }
   __syncthreads();
   // 5. store
@@ -266,7 +344,8 @@ __global__ void __launch_bounds__(256) k_r2r(const double *__restrict__ src, Lin
     int g, e;
     if (out.se == 1) { g = idx / n; e = idx - g * n; }
     else { e = idx / G; g = idx - e * G; }
-    if (g < gcount) dst[out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se] = O[g * n + e] * scale;
+    if (g < gcount) store_out(dst, out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se, O[g * n + e],
+                              scale, accum);
   }
 }
 
@@ -289,47 +368,52 @@ struct P2Shape {
   static constexpr size_t LDS = (size_t)NFFT * M * sizeof(double2);
 };
 
-template <int M, bool QUARTER>
-__global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, LineMap in, double *__restrict__ dst,
-                                                LineMap out, int L1, int kind, double scale,
-                                                const double2 *__restrict__ tw, const double2 *__restrict__ q,
-                                                const double2 *__restrict__ t2, FillSrc fs) {
+template <int M, bool QUARTER, typename R, typename TI, typename TO>
+__global__ void __launch_bounds__(512) k_r2r_p2(const TI *__restrict__ src, LineMap in, TO *__restrict__ dst,
+                                                LineMap out, int L1, int kind, double scale, int accum,
+                                                const typename CT<R>::C *__restrict__ tw,
+                                                const typename CT<R>::C *__restrict__ q,
+                                                const typename CT<R>::C *__restrict__ t2, FillSrc fs) {
   using S = P2Shape<M, QUARTER>;
+  using C = typename CT<R>::C;
   constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4;
-  extern __shared__ double2 smem[];
-  double *Rw = (double *)smem;  // G*N staged reals, overlaid by the complex lines
+  extern __shared__ double2 smem_raw[];
+  C *smem = (C *)smem_raw;
+  R *Rw = (R *)smem;  // G*N staged reals, overlaid by the complex lines
   const int tid = threadIdx.x;
   const int l1_0 = blockIdx.x * G;
   const long l2 = blockIdx.y;
   const int gcount = min(G, L1 - l1_0);
   // 1. coalesced load of the G lines
+  double vmax = 0.0;
 #pragma unroll
   for (int r = 0; r < G * N / NT; r++) {
     const int idx = tid + r * NT;
     int g, e;
     if (in.se == 1) { g = idx / N; e = idx - g * N; }
     else { e = idx / G; g = idx - e * G; }
-    double val = 0.0;
+    R val = 0.0;
     if (g < gcount) {
       const long a = in.base + (long)(l1_0 + g) * in.s1 + l2 * in.s2 + (long)e * in.se;
       if (fs.on) {
-        const int k = (int)l2 + 1;
-        val = fs.rho0 * ((fs.w[a] - fs.w[a - fs.sz]) * fs.dti * fs.dzfi[k] + (fs.v[a] - fs.v[a - fs.sy]) * fs.dtidyi +
-                         (fs.u[a] - fs.u[a - 1]) * fs.dtidxi);
+        const double sv = src_value(fs, a, (int)l2 + 1);
+        vmax = fmax(vmax, fabs(sv));
+        val = (R)sv;
       } else {
-        val = src[a];
+        val = (R)src[a];
       }
     }
     Rw[g * N + e] = val;
   }
+  if (fs.rmax) slots_max(fs.rmax, vmax, (blockIdx.x + gridDim.x * blockIdx.y) * (NT / 64) + tid / 64);
   __syncthreads();
   const int f = tid / Q4, b = tid - f * Q4;
-  double2 v[4];
+  C v[4];
 #pragma unroll
-  for (int t = 0; t < 4; t++) v[t] = pre_value(kind, Rw, f, b + t * Q4, N, q);
+  for (int t = 0; t < 4; t++) v[t] = pre_value<R>(kind, Rw, f, b + t * Q4, N, q);
   __syncthreads();
-  const double sign = kind_is_forward(kind) ? -1.0 : 1.0;
-  double2 *Z = smem + (long)f * M;
+  const R sign = kind_is_forward(kind) ? (R)-1.0 : (R)1.0;
+  C *Z = smem + (long)f * M;
   // 2. radix-4 passes
 #pragma unroll
   for (int s = 0; s < S::NP4; s++) {
@@ -345,7 +429,7 @@ __global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, 
     if (k > 0) {
 #pragma unroll
       for (int t = 1; t < 4; t++) {
-        double2 w = tw[t * k * tstride];
+        C w = tw[t * k * tstride];
         if (sign > 0) w.y = -w.y;
         v[t] = cmul(v[t], w);
       }
@@ -363,9 +447,9 @@ __global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, 
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int k = b + h * Q4;
-      double2 x0 = v[h], x1 = v[h + 2];
+      C x0 = v[h], x1 = v[h + 2];
       if (k > 0) {
-        double2 w = tw[k];
+        C w = tw[k];
         if (sign > 0) w.y = -w.y;
         x1 = cmul(x1, w);
       }
@@ -376,19 +460,19 @@ __global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, 
   }
   // 4. post-processing and store
   constexpr int PER = G * N / NT;
-  double val[PER];
+  R val[PER];
 #pragma unroll
   for (int r = 0; r < PER; r++) {
     const int idx = tid + r * NT;
     const int g = idx / N, k = idx - g * N;
-    val[r] = post_value(kind, smem, g, k, N, M, q, t2);
+    val[r] = post_value<R>(kind, smem, g, k, N, M, q, t2);
   }
   if (out.se == 1) {
 #pragma unroll
     for (int r = 0; r < PER; r++) {
       const int idx = tid + r * NT;
       const int g = idx / N, e = idx - g * N;
-      if (g < gcount) dst[out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + e] = val[r] * scale;
+      if (g < gcount) store_out(dst, out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + e, val[r], scale, accum);
     }
     return;
   }
@@ -400,82 +484,102 @@ __global__ void __launch_bounds__(512) k_r2r_p2(const double *__restrict__ src, 
   for (int r = 0; r < PER; r++) {
     const int idx = tid + r * NT;
     const int e = idx / G, g = idx - e * G;
-    if (g < gcount) dst[out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se] = Rw[g * N + e] * scale;
+    if (g < gcount)
+      store_out(dst, out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se, Rw[g * N + e], scale, accum);
   }
 }
 
-template <int M, bool QUARTER>
-static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
-                      double *dst, LineMap out, int L1, int L2, double scale, const FillSrc &fs) {
+// the transform tables in the arithmetic type R
+template <typename R>
+struct PlanTabs {
+  const typename CT<R>::C *tw, *q, *t2;
+};
+static PlanTabs<double> tabs_of(const fz_fftplan &P, double) { return {P.tw, P.q, P.t2}; }
+static PlanTabs<float> tabs_of(const fz_fftplan &P, float) { return {P.tw32, P.q32, P.t232}; }
+
+template <int M, bool QUARTER, typename R, typename TI, typename TO>
+static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
+                      TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs) {
   using S = P2Shape<M, QUARTER>;
   dim3 grid((L1 + S::G - 1) / S::G, L2, 1);
-  FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind, scale, P.tw,
-            P.q, P.t2, fs);
+  const PlanTabs<R> t = tabs_of(P, R());
+  FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind,
+            scale, accum, t.tw, t.q, t.t2, fs);
 }
 
 static bool p2_length(int m) { return m >= 16 && m <= 2048 && (m & (m - 1)) == 0; }
 
-template <bool QUARTER>
-static bool dispatch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
-                        double *dst, LineMap out, int L1, int L2, double scale, const FillSrc &fs) {
+template <bool QUARTER, typename R, typename TI, typename TO>
+static bool dispatch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
+                        TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs) {
+#define FZ_P2_CASE(MM) \
+  case MM: launch_p2<MM, QUARTER, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, fs); return true;
   switch (P.m) {
-    case 16: launch_p2<16, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
-    case 32: launch_p2<32, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
-    case 64: launch_p2<64, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
-    case 128: launch_p2<128, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
-    case 256: launch_p2<256, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
-    case 512: launch_p2<512, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
-    case 1024: launch_p2<1024, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
-    case 2048: launch_p2<2048, QUARTER>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, fs); return true;
+    FZ_P2_CASE(16)
+    FZ_P2_CASE(32)
+    FZ_P2_CASE(64)
+    FZ_P2_CASE(128)
+    FZ_P2_CASE(256)
+    FZ_P2_CASE(512)
+    FZ_P2_CASE(1024)
+    FZ_P2_CASE(2048)
     default: return false;
   }
+#undef FZ_P2_CASE
 }
 
 // Thomas recurrence of dgtsv_homebrewed (src/solver.f90:151-177) on one
-// column of the spectral buffer; pivots go to piv (same layout).
-__device__ inline void thomas_col(double *__restrict__ col, double *__restrict__ dcol, long ps, int n,
+// column of the spectral buffer; pivots go to piv (same layout, FP64). The
+// recurrence runs in FP64 whatever the column's element type.
+template <typename T>
+__device__ inline void thomas_col(T *__restrict__ col, double *__restrict__ dcol, long ps, int n,
                                   const double *__restrict__ a, const double *__restrict__ b,
                                   const double *__restrict__ c, double lam) {
   double bb = b[0] + lam;
   double z = 1.0 / (bb + FZ_EPS);
   double d = c[0] * z;
-  double pprev = col[0] * z;
+  double pprev = (double)col[0] * z;
   dcol[0] = d;
-  col[0] = pprev;
+  col[0] = (T)pprev;
   for (int l = 1; l < n; l++) {
     bb = b[l] + lam;
     z = 1.0 / (bb - a[l] * d + FZ_EPS);
     d = c[l] * z;
-    pprev = (col[l * ps] - a[l] * pprev) * z;
+    pprev = ((double)col[l * ps] - a[l] * pprev) * z;
     dcol[l * ps] = d;
-    col[l * ps] = pprev;
+    col[l * ps] = (T)pprev;
   }
   double pnext = pprev;
   for (int l = n - 2; l >= 0; l--) {
-    pnext = col[l * ps] - dcol[l * ps] * pnext;
-    col[l * ps] = pnext;
+    pnext = (double)col[l * ps] - dcol[l * ps] * pnext;
+    col[l * ps] = (T)pnext;
   }
 }
 
 // The same recurrence as thomas_col, in chunks of TK rows: each chunk's
 // loads are issued together so one wave keeps TK loads in flight (one
 // column per lane leaves few waves per CU to hide latency otherwise).
-template <int TK>
-__global__ void __launch_bounds__(64) k_thomas(double *__restrict__ W, double *__restrict__ D,
+// Column `gauge` (the singular mode of an all-Neumann/periodic operator, or
+// -1) has its last unknown set to zero instead of divided by the eps-shifted
+// zero pivot: the mixed-precision solve fixes the null mode this way, since
+// the O(1/eps) constant of the reference's pivots would swamp FP32 values.
+template <int TK, typename T>
+__global__ void __launch_bounds__(64) k_thomas(T *__restrict__ W, double *__restrict__ D,
                                                const double *__restrict__ lambdaxy, const double *__restrict__ a,
                                                const double *__restrict__ b, const double *__restrict__ c, int n1,
-                                               int n2, int n3) {
+                                               int n2, int n3, long gauge) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)n1 * n2) return;
   const long ps = (long)n1 * n2;
-  double *col = W + idx, *dcol = D + idx;
+  T *col = W + idx;
+  double *dcol = D + idx;
   const double lam = lambdaxy[idx];
   double d = 0.0, pprev = 0.0;
   for (int l0 = 0; l0 < n3; l0 += TK) {
     double r[TK];
 #pragma unroll
     for (int t = 0; t < TK; t++)
-      if (l0 + t < n3) r[t] = col[(long)(l0 + t) * ps];
+      if (l0 + t < n3) r[t] = (double)col[(long)(l0 + t) * ps];
 #pragma unroll
     for (int t = 0; t < TK; t++) {
       const int l = l0 + t;
@@ -491,8 +595,9 @@ __global__ void __launch_bounds__(64) k_thomas(double *__restrict__ W, double *_
           d = c[l] * z;
           pprev = (r[t] - a[l] * pprev) * z;
         }
+        if (idx == gauge && l == n3 - 1) pprev = 0.0;
         dcol[(long)l * ps] = d;
-        col[(long)l * ps] = pprev;
+        col[(long)l * ps] = (T)pprev;
       }
     }
   }
@@ -502,39 +607,65 @@ __global__ void __launch_bounds__(64) k_thomas(double *__restrict__ W, double *_
 #pragma unroll
     for (int t = 0; t < TK; t++)
       if (h0 - t >= 0) {
-        r[t] = col[(long)(h0 - t) * ps];
+        r[t] = (double)col[(long)(h0 - t) * ps];
         dd[t] = dcol[(long)(h0 - t) * ps];
       }
 #pragma unroll
     for (int t = 0; t < TK; t++)
       if (h0 - t >= 0) {
         pnext = r[t] - dd[t] * pnext;
-        col[(long)(h0 - t) * ps] = pnext;
+        col[(long)(h0 - t) * ps] = (T)pnext;
       }
   }
 }
 
-// gaussel_periodic, src/solver.f90:107-149
-__global__ void k_thomas_periodic(double *__restrict__ W, double *__restrict__ D, double *__restrict__ P2,
+// gaussel_periodic, src/solver.f90:107-149 (P2 and the pivots in FP64; the
+// gauge column as in k_thomas: the cyclic unknown is set to zero)
+template <typename T>
+__global__ void k_thomas_periodic(T *__restrict__ W, double *__restrict__ D, double *__restrict__ P2,
                                   const double *__restrict__ lambdaxy, const double *__restrict__ a,
                                   const double *__restrict__ b, const double *__restrict__ c, int n1, int n2,
-                                  int n3) {
+                                  int n3, long gauge) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)n1 * n2) return;
   const long ps = (long)n1 * n2;
   const double lam = lambdaxy[idx];
   const int m = n3 - 1;
-  double *col = W + idx, *p2 = P2 + idx;
+  T *col = W + idx;
+  double *p2 = P2 + idx;
   thomas_col(col, D + idx, ps, m, a, b, c, lam);
   for (int l = 0; l < m; l++) p2[l * ps] = 0.0;
   p2[0] = -a[0];
   p2[(m - 1) * ps] = -c[m - 1];
   thomas_col(p2, D + idx, ps, m, a, b, c, lam);
   const double bbm = b[m] + lam;
-  const double pn = (col[m * ps] - c[m] * col[0] - a[m] * col[(m - 1) * ps]) /
-                    (bbm + c[m] * p2[0] + a[m] * p2[(m - 1) * ps] + FZ_EPS);
-  col[m * ps] = pn;
-  for (int l = 0; l < m; l++) col[l * ps] = col[l * ps] + p2[l * ps] * pn;
+  double pn = ((double)col[m * ps] - c[m] * (double)col[0] - a[m] * (double)col[(m - 1) * ps]) /
+              (bbm + c[m] * p2[0] + a[m] * p2[(m - 1) * ps] + FZ_EPS);
+  if (idx == gauge) pn = 0.0;
+  col[m * ps] = (T)pn;
+  for (int l = 0; l < m; l++) col[l * ps] = (T)((double)col[l * ps] + p2[l * ps] * pn);
+}
+
+// the source of a forward x transform evaluated into x lines of W (the
+// general path for lengths k_r2r_p2 does not cover)
+template <typename T>
+__global__ void k_src_lines(fz_dims d, FillSrc fs, T *__restrict__ W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  double v = 0.0;
+  if (i <= d.n1 && j <= d.n2) {
+    v = src_value(fs, i + d.sy * j + d.sz * k, k);
+    W[(i - 1) + (long)d.n1 * ((j - 1) + (long)d.n2 * (k - 1))] = (T)v;
+  }
+  if (fs.rmax) {
+    const unsigned t = threadIdx.x + blockDim.x * threadIdx.y;
+    const unsigned blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    double m = fabs(v);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((t & 63) == 0)
+      atomicMax(fs.rmax + ((blk * 4 + t / 64) & (FZ_GAMMA_SLOTS - 1)), (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -597,6 +728,18 @@ static int fftplan_init_kinds(fz_fftplan *P, int kind_f, int kind_b, int n) {
     q[k] = make_double2(fzm::cos(FZ_PI * (double)k / (2.0 * n)), fzm::sin(FZ_PI * (double)k / (2.0 * n)));
     t2[k] = make_double2(fzm::cos(FZ_PI * (double)(2 * k + 1) / (4.0 * n)), fzm::sin(FZ_PI * (double)(2 * k + 1) / (4.0 * n)));
   }
+  std::vector<float2> tw32(P->m), q32(n + 1), t232(n + 1);
+  for (int j = 0; j < P->m; j++) tw32[j] = make_float2((float)tw[j].x, (float)tw[j].y);
+  for (int k = 0; k <= n; k++) {
+    q32[k] = make_float2((float)q[k].x, (float)q[k].y);
+    t232[k] = make_float2((float)t2[k].x, (float)t2[k].y);
+  }
+  FZ_HIP(hipMalloc((void **)&P->tw32, sizeof(float2) * P->m));
+  FZ_HIP(hipMalloc((void **)&P->q32, sizeof(float2) * (n + 1)));
+  FZ_HIP(hipMalloc((void **)&P->t232, sizeof(float2) * (n + 1)));
+  FZ_HIP(hipMemcpy(P->tw32, tw32.data(), sizeof(float2) * P->m, hipMemcpyHostToDevice));
+  FZ_HIP(hipMemcpy(P->q32, q32.data(), sizeof(float2) * (n + 1), hipMemcpyHostToDevice));
+  FZ_HIP(hipMemcpy(P->t232, t232.data(), sizeof(float2) * (n + 1), hipMemcpyHostToDevice));
   FZ_HIP(hipMalloc((void **)&P->tw, sizeof(double2) * P->m));
   FZ_HIP(hipMalloc((void **)&P->q, sizeof(double2) * (n + 1)));
   FZ_HIP(hipMalloc((void **)&P->t2, sizeof(double2) * (n + 1)));
@@ -618,6 +761,10 @@ static void fftplan_free(fz_fftplan *P) {
   if (P->q) (void)hipFree(P->q);
   if (P->t2) (void)hipFree(P->t2);
   P->tw = P->q = P->t2 = nullptr;
+  if (P->tw32) (void)hipFree(P->tw32);
+  if (P->q32) (void)hipFree(P->q32);
+  if (P->t232) (void)hipFree(P->t232);
+  P->tw32 = P->q32 = P->t232 = nullptr;
 }
 
 int fz_poisson_destroy(fz_poisson *p) {
@@ -628,6 +775,9 @@ int fz_poisson_destroy(fz_poisson *p) {
   for (int q = 0; q < 9; q++)
     if (bufs[q]) (void)hipFree(bufs[q]);
   if (p->aux) (void)hipFree(p->aux);
+  if (p->work32) (void)hipFree(p->work32);
+  if (p->bcopy) (void)hipFree(p->bcopy);
+  if (p->rslots) (void)hipFree(p->rslots);
   delete p;
   return FZ_OK;
 }
@@ -703,6 +853,8 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
   P->normfft = normfft;
   P->periodic_z = (cbc[4] == 'P' && cbc[5] == 'P');
   P->rhsb_zero = zero_b ? 1 : 0;
+  const bool z_singular = (cbc[4] == 'N' && cbc[5] == 'N') || P->periodic_z;
+  P->null_col = (lxy[0] == 0.0 && z_singular) ? 0 : -1;
   int rc = fftplan_init(&P->px, cbc[0], cbc[1], n1);
   if (rc == FZ_OK) rc = fftplan_init(&P->py, cbc[2], cbc[3], n2);
   if (rc != FZ_OK) {
@@ -731,14 +883,16 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
   return FZ_OK;
 }
 
-static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const double *src, LineMap in,
-                      double *dst, LineMap out, int L1, int L2, double scale) {
+template <typename R, typename TI, typename TO>
+static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
+                      TO *dst, LineMap out, int L1, int L2, double scale, int accum = 0) {
   const bool quarter = kind_is_quarter(kind);
   if (p2_length(P.m)) {
     FillSrc off;
     memset(&off, 0, sizeof(off));
-    const bool done = quarter ? dispatch_p2<true>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, off)
-                              : dispatch_p2<false>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, off);
+    const bool done =
+        quarter ? dispatch_p2<true, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, off)
+                : dispatch_p2<false, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, off);
     if (done) {
       FZ_LAUNCHED();
       return FZ_OK;
@@ -750,25 +904,29 @@ static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int ki
     G = 2048 / P.m;
     if (G < 1) G = 1;
     if (G > 64) G = 64;
-    lds = (size_t)G * P.m * sizeof(double2) * 2;
+    lds = (size_t)G * P.m * sizeof(typename CT<R>::C) * 2;
   } else {
     G = 4096 / P.m;
     if (G < 2) G = 2;
     if (G > 64) G = 64;
     G &= ~1;
-    lds = (size_t)(G / 2) * P.m * sizeof(double2) * 2;
+    lds = (size_t)(G / 2) * P.m * sizeof(typename CT<R>::C) * 2;
   }
   RadList rl;
   for (int s = 0; s < 24; s++) rl.r[s] = P.rad[s];
   dim3 grid((L1 + G - 1) / G, L2, 1);
-  FZ_LAUNCH(ctx, name, k_r2r, grid, dim3(256), lds, src, in, dst, out, L1, P.n, P.m, kind, G, scale, P.tw,
-                     P.q, P.t2, P.nrad, rl);
+  const PlanTabs<R> t = tabs_of(P, R());
+  FZ_LAUNCH(ctx, name, (k_r2r<R, TI, TO>), grid, dim3(256), lds, src, in, dst, out, L1, P.n, P.m, kind, G, scale,
+            accum, t.tw, t.q, t.t2, P.nrad, rl);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 
 // everything after the forward x transform: y transform, z solve, inverses
-static int solver_rest(fz_ctx *ctx, double *p);
+// (T = double: the reference's solver; T = float: one FP32 solve of the
+// mixed-precision form, whose inverse x transform writes or adds into FP64 p)
+template <typename T>
+static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum);
 
 int fz_solver_dev(fz_ctx *ctx, double *p) {
   fz_poisson *P = ctx->pois;
@@ -778,12 +936,35 @@ int fz_solver_dev(fz_ctx *ctx, double *p) {
   const long ps = (long)n1 * n2;
   const LineMap p_x = {1 + sy + sz, sy, sz, 1};  // x-lines of the halo'd field, (j,k) -> line
   const LineMap w_x = {0, n1, ps, 1};
-  FZ_TRY(launch_r2r(ctx, "fft_x_fwd", P->px, P->px.kind_f, p, p_x, P->work, w_x, n2, n3, 1.0));
-  return solver_rest(ctx, p);
+  FZ_TRY((launch_r2r<double, double, double>(ctx, "fft_x_fwd", P->px, P->px.kind_f, p, p_x, P->work, w_x, n2, n3,
+                                             1.0)));
+  return solver_rest<double>(ctx, P->work, p, 0);
 }
 
 int fz_fillps(fz_ctx *c, double dti, double rho0, const double *u, const double *v, const double *w, double *p);
 int fz_updt_rhs_b(fz_ctx *c, double *p);
+
+static FillSrc fill_src(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v, const double *w) {
+  FillSrc fs;
+  memset(&fs, 0, sizeof(fs));
+  fs.u = u;
+  fs.v = v;
+  fs.w = w;
+  fs.dzfi = ctx->dzfi;
+  fs.dzci = ctx->dzci;
+  fs.rho0 = rho0;
+  fs.dti = dti;
+  fs.dtidxi = dti * ctx->dli[0];  // the scalars of fz_fillps
+  fs.dtidyi = dti * ctx->dli[1];
+  fs.dxi2 = ctx->dli[0] * ctx->dli[0];
+  fs.dyi2 = ctx->dli[1] * ctx->dli[1];
+  fs.dxi = ctx->dli[0];
+  fs.dyi = ctx->dli[1];
+  fs.sy = ctx->d.sy;
+  fs.sz = ctx->d.sz;
+  fs.on = SRC_FILLPS;
+  return fs;
+}
 
 // fillps + updt_rhs_b + solver (src/main.f90:492-496). When every boundary
 // right-hand side is zero, updt_rhs_b adds exact zeros and is skipped, and
@@ -791,10 +972,11 @@ int fz_updt_rhs_b(fz_ctx *c, double *p);
 // x lengths); p then receives the correction pressure only. Otherwise the
 // three routines run as they are.
 int fz_projection_rhs_solve(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v, const double *w,
-                            double *p) {
+                            double *p, const double *stc) {
   fz_poisson *P = ctx->pois;
   if (!P) return fz_set_error(FZ_ERR_STATE, "fz_solver: fz_initsolver has not been called");
   if (!P->rhsb_zero || !p2_length(P->px.m) || kind_is_quarter(P->px.kind_f)) {
+    if (stc) return fz_set_error(FZ_ERR_STATE, "fz_projection_rhs_solve: device scalars need the fused path");
     FZ_TRY(fz_fillps(ctx, dti, rho0, u, v, w, p));
     FZ_TRY(fz_updt_rhs_b(ctx, p));
     return fz_solver_dev(ctx, p);
@@ -804,51 +986,215 @@ int fz_projection_rhs_solve(fz_ctx *ctx, double dti, double rho0, const double *
   const long ps = (long)n1 * n2;
   const LineMap p_x = {1 + sy + sz, sy, sz, 1};
   const LineMap w_x = {0, n1, ps, 1};
-  FillSrc fs;
-  fs.u = u;
-  fs.v = v;
-  fs.w = w;
-  fs.dzfi = ctx->dzfi;
-  fs.rho0 = rho0;
-  fs.dti = dti;
-  fs.dtidxi = dti * ctx->dli[0];  // the scalars of fz_fillps
-  fs.dtidyi = dti * ctx->dli[1];
-  fs.sy = sy;
-  fs.sz = sz;
-  fs.on = 1;
-  dispatch_p2<false>(ctx, "fillps_fft_x_fwd", P->px, P->px.kind_f, nullptr, p_x, P->work, w_x, n2, n3, 1.0, fs);
+  FillSrc fs = fill_src(ctx, dti, rho0, u, v, w);
+  fs.stc = stc;
+  dispatch_p2<false, double, double, double>(ctx, "fillps_fft_x_fwd", P->px, P->px.kind_f, (const double *)nullptr,
+                                             p_x, P->work, w_x, n2, n3, 1.0, 0, fs);
   FZ_LAUNCHED();
-  return solver_rest(ctx, p);
+  return solver_rest<double>(ctx, P->work, p, 0);
 }
 
-static int solver_rest(fz_ctx *ctx, double *p) {
+template <typename T>
+static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum) {
   fz_poisson *P = ctx->pois;
   const int n1 = ctx->d.n1, n2 = ctx->d.n2, n3 = ctx->d.n3;
   const long sy = ctx->d.sy, sz = ctx->d.sz;
   const long ps = (long)n1 * n2;
-  double *W = P->work;
+  const bool mixed = sizeof(T) == sizeof(float);
+  const long gauge = mixed ? P->null_col : -1;
   const LineMap p_x = {1 + sy + sz, sy, sz, 1};
   const LineMap w_x = {0, n1, ps, 1};
   const LineMap w_y = {0, 1, ps, n1};
-  FZ_TRY(launch_r2r(ctx, "fft_y_fwd", P->py, P->py.kind_f, W, w_y, W, w_y, n1, n3, 1.0));
+  FZ_TRY((launch_r2r<T, T, T>(ctx, mixed ? "fft_y_fwd_f32" : "fft_y_fwd", P->py, P->py.kind_f, W, w_y, W, w_y, n1, n3,
+                              1.0)));
   const int tpb = 256;
   const int nb = (int)((ps + tpb - 1) / tpb);
   if (!P->periodic_z) {
-    FZ_LAUNCH(ctx, "thomas", k_thomas<16>, dim3((int)((ps + 63) / 64)), dim3(64), 0, W, P->piv, P->lambdaxy, P->a,
-              P->b, P->c, n1, n2, n3);
+    // FP32 columns move half the bytes per load: twice the rows in flight
+    FZ_LAUNCH(ctx, mixed ? "thomas_f32" : "thomas", (k_thomas<sizeof(T) == 4 ? 32 : 16, T>), dim3((int)((ps + 63) / 64)),
+              dim3(64), 0, W,
+              P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3, gauge);
   } else {
-    FZ_LAUNCH(ctx, "thomas_periodic", k_thomas_periodic, dim3(nb), dim3(tpb), 0, W, P->piv, P->aux, P->lambdaxy, P->a,
-                       P->b, P->c, n1, n2, n3);
+    FZ_LAUNCH(ctx, mixed ? "thomas_periodic_f32" : "thomas_periodic", k_thomas_periodic<T>, dim3(nb), dim3(tpb), 0, W,
+              P->piv, P->aux, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3, gauge);
   }
   FZ_LAUNCHED();
-  FZ_TRY(launch_r2r(ctx, "fft_y_bwd", P->py, P->py.kind_b, W, w_y, W, w_y, n1, n3, 1.0));
-  FZ_TRY(launch_r2r(ctx, "fft_x_bwd", P->px, P->px.kind_b, W, w_x, p, p_x, n2, n3, P->normfft));
+  FZ_TRY((launch_r2r<T, T, T>(ctx, mixed ? "fft_y_bwd_f32" : "fft_y_bwd", P->py, P->py.kind_b, W, w_y, W, w_y, n1, n3,
+                              1.0)));
+  FZ_TRY((launch_r2r<T, T, double>(ctx, mixed ? "fft_x_bwd_f32" : "fft_x_bwd", P->px, P->px.kind_b, W, w_x, p, p_x,
+                                   n2, n3, P->normfft, accum)));
   return FZ_OK;
 }
 
 extern "C" int fz_solver(fz_ctx *ctx, double *p) {
   if (!ctx || !p) return fz_set_error(FZ_ERR_ARG, "fz_solver: null argument");
   return fz_solver_dev(ctx, p);
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision: FP32 transforms and line solves, FP64 residual refinement
+// ---------------------------------------------------------------------------
+static int mixed_buffers(fz_ctx *ctx) {
+  fz_poisson *P = ctx->pois;
+  if (!P->work32) FZ_HIP(hipMalloc((void **)&P->work32, sizeof(float) * ctx->ni));
+  if (!P->rslots) {
+    FZ_HIP(hipMalloc((void **)&P->rslots, sizeof(unsigned long long) * FZ_GAMMA_SLOTS));
+    FZ_HIP(hipMemsetAsync(P->rslots, 0, sizeof(unsigned long long) * FZ_GAMMA_SLOTS, ctx->st));
+  }
+  return FZ_OK;
+}
+
+// one FP32 solve of the source fs: forward x transform of the source (fused
+// into the load for power-of-two lengths), the rest in FP32; the result is
+// written into p (accum 0) or added to it (accum 1)
+static int mixed_pass(fz_ctx *ctx, const FillSrc &fs, double *p, int accum) {
+  fz_poisson *P = ctx->pois;
+  const int n1 = ctx->d.n1, n2 = ctx->d.n2, n3 = ctx->d.n3;
+  const long sy = ctx->d.sy, sz = ctx->d.sz;
+  const long ps = (long)n1 * n2;
+  const LineMap p_x = {1 + sy + sz, sy, sz, 1};
+  const LineMap w_x = {0, n1, ps, 1};
+  float *W = P->work32;
+  const char *name = fs.on == SRC_FILLPS ? "fillps_fft_x_fwd_f32" : (fs.on == SRC_B ? "fft_x_fwd_f32" : "residual_fft_x_fwd_f32");
+  bool fused = false;
+  if (p2_length(P->px.m)) {
+    fused = kind_is_quarter(P->px.kind_f)
+                ? dispatch_p2<true, float, double, float>(ctx, name, P->px, P->px.kind_f, (const double *)nullptr, p_x,
+                                                          W, w_x, n2, n3, 1.0, 0, fs)
+                : dispatch_p2<false, float, double, float>(ctx, name, P->px, P->px.kind_f, (const double *)nullptr,
+                                                           p_x, W, w_x, n2, n3, 1.0, 0, fs);
+    FZ_LAUNCHED();
+  }
+  if (!fused) {
+    FZ_LAUNCH(ctx, "src_lines_f32", k_src_lines<float>, fz_grid_interior(ctx->d), fz_block3(), 0, ctx->d, fs, W);
+    FZ_LAUNCHED();
+    FZ_TRY((launch_r2r<float, float, float>(ctx, "fft_x_fwd_f32", P->px, P->px.kind_f, W, w_x, W, w_x, n2, n3, 1.0)));
+  }
+  return solver_rest<float>(ctx, W, p, accum);
+}
+
+// the homogeneous pressure conditions on the halo of x (the residual's
+// operator is the solver's, whose boundary values sit in the right-hand side)
+static int halo_homogeneous(fz_ctx *ctx, double *x) {
+  const double zero[6] = {0, 0, 0, 0, 0, 0};
+  return fz_boundp(ctx, ctx->pois->cbc, zero, x);
+}
+
+// x = solve(fs) with FP32 transforms, then nref refinements
+// x += solve32(fs_b - A x) whose residual is formed in FP64
+static int mixed_solve(fz_ctx *ctx, FillSrc fs, double *x, int nref) {
+  FZ_TRY(mixed_buffers(ctx));
+  fs.rmax = nullptr;
+  FZ_TRY(mixed_pass(ctx, fs, x, 0));
+  FillSrc fr = fs;
+  fr.on = fs.on == SRC_FILLPS ? SRC_FILLPS_RES : SRC_B_RES;
+  fr.x = x;
+  for (int it = 0; it < nref; it++) {
+    FZ_TRY(halo_homogeneous(ctx, x));
+    FZ_TRY(mixed_pass(ctx, fr, x, 1));
+  }
+  return FZ_OK;
+}
+
+// fz_projection_rhs_solve in mixed precision: fillps in FP64 evaluated in the
+// load of the first FP32 transform, nref FP64-residual refinements
+int fz_projection_rhs_solve_mixed(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v,
+                                  const double *w, double *p, int nref, const double *stc) {
+  fz_poisson *P = ctx->pois;
+  if (!P) return fz_set_error(FZ_ERR_STATE, "fz_solver: fz_initsolver has not been called");
+  if (!P->rhsb_zero)
+    return fz_set_error(FZ_ERR_UNSUPPORTED, "mixed-precision projection: nonzero boundary pressure values");
+  FillSrc fs = fill_src(ctx, dti, rho0, u, v, w);
+  fs.stc = stc;
+  return mixed_solve(ctx, fs, p, nref);
+}
+
+// whether the projection can take 1/dt from device memory (the fused
+// forward transform; graph capture of the step depends on it)
+int fz_projection_device_scalars_ok(fz_ctx *ctx, int mixed) {
+  fz_poisson *P = ctx->pois;
+  if (!P) return 0;
+  if (mixed) return 1;
+  return P->rhsb_zero && p2_length(P->px.m) && !kind_is_quarter(P->px.kind_f);
+}
+
+__global__ void k_slots_max(unsigned long long *__restrict__ slots, double *__restrict__ out) {
+  __shared__ unsigned long long s[FZ_GAMMA_SLOTS];
+  const int t = threadIdx.x;
+  s[t] = slots[t];
+  slots[t] = 0ull;
+  __syncthreads();
+  for (int o = FZ_GAMMA_SLOTS / 2; o > 0; o >>= 1) {
+    if (t < o) s[t] = s[t] > s[t + o] ? s[t] : s[t + o];
+    __syncthreads();
+  }
+  if (t == 0) *out = __longlong_as_double((long long)s[0]);
+}
+
+// max |value| of the slots (cleared for the next use), synchronising
+static int read_slots(fz_ctx *ctx, double *out) {
+  fz_poisson *P = ctx->pois;
+  FZ_LAUNCH(ctx, "slots_max", k_slots_max, dim3(1), dim3(FZ_GAMMA_SLOTS), 0, P->rslots, ctx->red);
+  FZ_LAUNCHED();
+  FZ_HIP(hipMemcpyAsync(ctx->h_small + 12, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  FZ_HIP(hipStreamSynchronize(ctx->st));
+  *out = ctx->h_small[12];
+  return FZ_OK;
+}
+
+// max |b - A x| over the interior (x's halo receives the homogeneous
+// conditions); b in the halo'd layout
+static int residual_max(fz_ctx *ctx, const double *b, double *x, double *res) {
+  FZ_TRY(halo_homogeneous(ctx, x));
+  FillSrc fr = fill_src(ctx, 0.0, 0.0, nullptr, nullptr, nullptr);
+  fr.on = SRC_B_RES;
+  fr.b = b;
+  fr.x = x;
+  fr.rmax = ctx->pois->rslots;
+  FZ_LAUNCH(ctx, "residual_max", k_src_lines<float>, fz_grid_interior(ctx->d), fz_block3(), 0, ctx->d, fr,
+            ctx->pois->work32);
+  FZ_LAUNCHED();
+  return read_slots(ctx, res);
+}
+
+// fz_solver with FP32 transforms and FP64-residual iterative refinement:
+// p holds the right-hand side on entry and the solution on exit (interior);
+// refinements continue until max|b - A x| <= tol * max|b| or maxit of them
+extern "C" int fz_solver_mixed(fz_ctx *ctx, double *p, double tol, int maxit, int *iters, double *relres) {
+  if (!ctx || !p || maxit < 0) return fz_set_error(FZ_ERR_ARG, "fz_solver_mixed: bad argument");
+  fz_poisson *P = ctx->pois;
+  if (!P) return fz_set_error(FZ_ERR_STATE, "fz_solver_mixed: fz_initsolver has not been called");
+  FZ_TRY(mixed_buffers(ctx));
+  FZ_TRY(fz_reduce_ensure(ctx, 1));
+  if (!P->bcopy) FZ_HIP(hipMalloc((void **)&P->bcopy, sizeof(double) * ctx->nh));
+  FZ_HIP(hipMemcpyAsync(P->bcopy, p, sizeof(double) * ctx->nh, hipMemcpyDeviceToDevice, ctx->st));
+  FillSrc fb = fill_src(ctx, 0.0, 0.0, nullptr, nullptr, nullptr);
+  fb.on = SRC_B;
+  fb.b = P->bcopy;
+  fb.rmax = P->rslots;
+  FZ_TRY(mixed_pass(ctx, fb, p, 0));
+  double bmax;
+  FZ_TRY(read_slots(ctx, &bmax));
+  FillSrc fr = fb;
+  fr.on = SRC_B_RES;
+  fr.x = p;
+  int it = 0;
+  double res = bmax, prev = 0.0;
+  while (it < maxit) {
+    FZ_TRY(halo_homogeneous(ctx, p));
+    FZ_TRY(mixed_pass(ctx, fr, p, 1));  // the residual of the current x, then its correction
+    it++;
+    FZ_TRY(read_slots(ctx, &res));
+    if (res <= tol * bmax) break;
+    if (it > 1 && res > 0.5 * prev) break;  // at the FP64 rounding floor of the residual
+    prev = res;
+  }
+  if (relres) {
+    FZ_TRY(residual_max(ctx, P->bcopy, p, &res));
+    *relres = bmax > 0.0 ? res / bmax : res;
+  }
+  if (iters) *iters = it;
+  return FZ_OK;
 }
 
 // fft, src/fft.f90:176-187: one r2r kind over `howmany` contiguous lines of
@@ -863,7 +1209,7 @@ extern "C" int fz_r2r(fz_ctx *ctx, int kind, int n, int howmany, double *x) {
   fz_fftplan P;
   FZ_TRY(fftplan_init_kinds(&P, kind, kind, n));
   const LineMap lines = {0, n, 0, 1};
-  int rc = launch_r2r(ctx, "fft_lines", P, kind, x, lines, x, lines, howmany, 1, 1.0);
+  int rc = launch_r2r<double, double, double>(ctx, "fft_lines", P, kind, x, lines, x, lines, howmany, 1, 1.0);
   if (rc == FZ_OK && hipStreamSynchronize(ctx->st) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "fz_r2r: sync");
   fftplan_free(&P);
   return rc;

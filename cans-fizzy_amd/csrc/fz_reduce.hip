@@ -144,11 +144,12 @@ __global__ void k_fin_max(int nb, const double *__restrict__ part, double *__res
   if (threadIdx.x == 0) out[0] = m;
 }
 
+template <typename PT>  // FP64 sums also over the FP32 phase field of the mixed-precision step
 __global__ void k_red_bulk(fz_dims d, long off, const double *__restrict__ gvr, double p1, double p2,
-                           const double *__restrict__ psi, double *__restrict__ part) {
+                           const PT *__restrict__ psi, double *__restrict__ part) {
   double s = 0.0;
   FOR_CELLS(d)
-  const double pl = 0.5 * (psi[q + off] + psi[q]);
+  const double pl = 0.5 * ((double)psi[q + off] + (double)psi[q]);
   s = s + (pl * p1 + (1.0 - pl) * p2) * gvr[k];
 }
 s = block_sum(s);
@@ -293,16 +294,21 @@ extern "C" int fz_chkdt(fz_ctx *c, int is_solve_ns, int is_track_interface, int 
   return FZ_OK;
 }
 
-int fz_bulk_mean_dev(fz_ctx *c, int idir, const double *psi, const double p12[2], const double *gvr, double *d_out) {
+template <typename PT>
+int fz_bulk_mean_dev(fz_ctx *c, int idir, const PT *psi, const double p12[2], const double *gvr, double *d_out) {
   const int nb = red_blocks(c);
   FZ_TRY(fz_reduce_ensure(c, nb + 8));
   const long off = idir == 1 ? 1 : (idir == 2 ? c->d.sy : c->d.sz);
-  FZ_LAUNCH(c, "red_bulk", k_red_bulk, dim3(nb), dim3(RED_THREADS), 0, c->d, off, gvr, p12[0], p12[1], psi, c->red);
+  FZ_LAUNCH(c, "red_bulk", k_red_bulk<PT>, dim3(nb), dim3(RED_THREADS), 0, c->d, off, gvr, p12[0], p12[1], psi,
+            c->red);
   FZ_LAUNCHED();
   FZ_LAUNCH(c, "fin_sum", k_fin_sum, dim3(1), dim3(RED_THREADS), 0, nb, c->red, d_out);
   FZ_LAUNCHED();
   return FZ_OK;
 }
+
+template int fz_bulk_mean_dev<double>(fz_ctx *, int, const double *, const double *, const double *, double *);
+template int fz_bulk_mean_dev<float>(fz_ctx *, int, const float *, const double *, const double *, double *);
 
 extern "C" int fz_bulk_mean_12_stag(fz_ctx *c, int idir, const double *psi, const double p12[2], double *mean) {
   if (!c || !psi || !p12 || !mean || idir < 1 || idir > 3)

@@ -15,6 +15,7 @@
 // steps leaves the active set; its memory is returned when the caller
 // releases it.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -22,16 +23,35 @@
 
 #include "fz_internal.h"
 
-// bytes of device memory fz_case_create + fz_case_init allocate for prm
-extern "C" int fz_case_footprint(const fz_params *prm, size_t *bytes) {
+// Device bytes of `count` buffers of `b` bytes each: allocations of 2 MiB
+// or more are made in 2 MiB units; smaller ones are packed into 2 MiB pages
+// (as many whole buffers, 4 KiB aligned, as fit in one page).
+static double alloc_bytes(double b, double count) {
+  const double page = 2.0 * 1024 * 1024;
+  if (b >= page) return count * page * std::ceil(b / page);
+  const double aligned = 4096.0 * std::ceil(b / 4096.0);
+  const double per_page = std::floor(page / aligned);
+  return page * std::ceil(count / per_page);
+}
+
+// bytes of device memory fz_case_create_ex + fz_case_init allocate for prm
+extern "C" int fz_case_footprint_ex(const fz_params *prm, int precision, size_t *bytes) {
   if (!prm || !bytes) return fz_set_error(FZ_ERR_ARG, "fz_case_footprint: null argument");
   const double n1 = prm->ng[0], n2 = prm->ng[1], n3 = prm->ng[2];
   const double nh = (n1 + 2) * (n2 + 2) * (n3 + 2), ni = n1 * n2 * n3;
   const bool periodic_z = prm->cbcpre[4] == 'P' && prm->cbcpre[5] == 'P';
   double b = 0.0;
-  b += 8.0 * nh * (14 + 5);  // case fields (pn aliases p) + psio, dp, three spare velocities
-  b += 8.0 * ni * 5;         // four AB2 histories + the phase-field right-hand side
-  b += 8.0 * ni * (periodic_z ? 3 : 2);  // Poisson work array, Thomas pivots (+ periodic-z second solve)
+  if (precision == FZ_PREC_MIXED) {
+    b += alloc_bytes(8.0 * nh, 5 + 4);  // u, v, w, p, po + dp, three spare velocities
+    b += alloc_bytes(4.0 * nh, 9 + 1);  // psi, phi, kappa, normals, fluxes + psio (FP32)
+    b += alloc_bytes(4.0 * ni, 4 + 1);  // four AB2 histories + the FP32 spectral buffer
+    b += alloc_bytes(8.0 * ni, periodic_z ? 3 : 2);  // FP64 Poisson work array and pivots (+ periodic z)
+  } else {
+    b += alloc_bytes(8.0 * nh, 14 + 5);  // case fields (pn aliases p) + psio, dp, three spare velocities
+    // four AB2 histories, the phase-field right-hand side, the Poisson work
+    // array and Thomas pivots (+ the periodic-z second solve)
+    b += alloc_bytes(8.0 * ni, 5 + (periodic_z ? 3 : 2));
+  }
   b += 8.0 * (n1 * n2 + 3 * n3 + 2 * (n2 * n3 + n1 * n3 + n1 * n2));  // eigenvalues, tridiagonal, boundary terms
   const double planes = 2.0 * ((n1 + 2) * (n2 + 2) + (n1 + 2) * (n3 + 2) + (n2 + 2) * (n3 + 2));
   b += 9.0 * planes * 12;    // composed boundary maps (about 12 signatures per case)
@@ -41,11 +61,16 @@ extern "C" int fz_case_footprint(const fz_params *prm, size_t *bytes) {
   return FZ_OK;
 }
 
+extern "C" int fz_case_footprint(const fz_params *prm, size_t *bytes) {
+  return fz_case_footprint_ex(prm, FZ_PREC_FP64, bytes);
+}
+
 namespace {
 struct Request {
   int id;
   fz_params prm;
   int nsteps;
+  int precision, nref;
   size_t bytes;
 };
 struct Running {
@@ -97,12 +122,19 @@ extern "C" int fz_server_destroy(fz_server *s) {
 }
 
 extern "C" int fz_server_submit(fz_server *s, const fz_params *prm, int nsteps, int *id) {
+  return fz_server_submit_ex(s, prm, nsteps, FZ_PREC_FP64, 0, id);
+}
+
+extern "C" int fz_server_submit_ex(fz_server *s, const fz_params *prm, int nsteps, int precision, int nref,
+                                   int *id) {
   if (!s || !prm || nsteps < 0 || !id) return fz_set_error(FZ_ERR_ARG, "fz_server_submit: bad argument");
   Request q;
   q.id = s->next_id++;
   q.prm = *prm;
   q.nsteps = nsteps;
-  FZ_TRY(fz_case_footprint(prm, &q.bytes));
+  q.precision = precision;
+  q.nref = nref;
+  FZ_TRY(fz_case_footprint_ex(prm, precision, &q.bytes));
   if (q.bytes > s->budget)
     return fz_set_error(FZ_ERR_ARG, "fz_server_submit: the case needs %zu bytes, more than the whole budget %zu",
                         q.bytes, s->budget);
@@ -120,9 +152,10 @@ static int admit(fz_server *s) {
     r.id = q.id;
     r.left = q.nsteps;
     r.bytes = q.bytes;
-    FZ_TRY(fz_case_create(&r.cs, &q.prm, nullptr));
+    FZ_TRY(fz_case_create_ex(&r.cs, &q.prm, nullptr, q.precision, q.nref));
     int rc = fz_case_ctx(r.cs, &r.ctx);
     if (rc == FZ_OK) rc = fz_case_init(r.cs);
+    if (rc == FZ_OK) rc = fz_case_set_graphs(r.cs, 1);  // steps after the first replay captured graphs
     if (rc == FZ_OK && hipEventCreate(&r.a) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
     if (rc == FZ_OK && hipEventCreate(&r.b) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
     if (rc != FZ_OK) {

@@ -71,14 +71,19 @@ __global__ void k_correc(fz_dims d, const double *__restrict__ dzci, double fact
 // are skipped, as fmax does in the reduction kernel); spreading the blocks
 // over the slots keeps same-address atomics from serialising, and
 // k_gamma_fin reduces and clears the slots.
-__global__ void __launch_bounds__(256) k_correc_step(fz_dims d, const double *__restrict__ dzci, double factori,
-                                                     double factorj, double dt, double rhop,
+__global__ void __launch_bounds__(256) k_correc_step(fz_dims d, const double *__restrict__ dzci, double factori_v,
+                                                     double factorj_v, double dt_v, double dxi, double dyi,
+                                                     const double *__restrict__ stc, double rhop,
                                                      const double *__restrict__ dp, const double *__restrict__ p,
                                                      const double *__restrict__ u, const double *__restrict__ v,
                                                      const double *__restrict__ w, double *__restrict__ un,
                                                      double *__restrict__ vn, double *__restrict__ wn,
                                                      double *__restrict__ p_out,
                                                      unsigned long long *__restrict__ slots) {
+  // dt from the case's device scalars when given (the products are the
+  // host's, operand for operand)
+  const double dt = stc ? stc[FZ_STC_DT] : dt_v;
+  const double factori = stc ? dt * dxi : factori_v, factorj = stc ? dt * dyi : factorj_v;
   FZ_BLOCK_XYZ(d, bx, by, bz);
   const int i = bx * blockDim.x + threadIdx.x, j = by * blockDim.y + threadIdx.y, k = bz;
   double m = 0.0;
@@ -132,11 +137,11 @@ __global__ void __launch_bounds__(FZ_GAMMA_SLOTS) k_gamma_fin(unsigned long long
   }
 }
 
-int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *dp, const double *p, const double *u,
-                          const double *v, const double *w, double *un, double *vn, double *wn, double *p_out,
-                          unsigned long long *slots) {
+int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
+                          const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
+                          double *p_out, unsigned long long *slots) {
   FZ_LAUNCH(c, "correc_step", k_correc_step, fz_grid_halo(c->d), fz_block3(), 0, c->d, c->dzci, dt * c->dli[0],
-            dt * c->dli[1], dt, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots);
+            dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -165,59 +170,71 @@ __global__ void k_add_face(fz_dims d, int dir, int plane, const double *__restri
   p[i + d.sy * j + d.sz * k] += rhs[a + (long)na * b];
 }
 
+// transcendentals by phase-field type: the shared FP64 routines (identical
+// on the device and in the oracle), or the device library's FP32 ones
+__device__ inline double fz_tanh(double x) { return fzm::tanh(x); }
+__device__ inline float fz_tanh(float x) { return tanhf(x); }
+__device__ inline double fz_log(double x) { return fzm::log(x); }
+__device__ inline float fz_log(float x) { return logf(x); }
+
 // ---------------------------------------------------------------------------
 // acdi_transport_pf at one cell, src/acdi.f90:102-224 (SDF-normal sharpening):
 // the right-hand side R and the three face fluxes of the cell's + faces
+// T is the type of the phase fields and of the arithmetic: double, or float
+// for the mixed-precision step (the FP64 velocities and grid are rounded to T)
+template <typename T>
 __device__ __forceinline__ void acdi_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
-                                          const double *__restrict__ dzfi, double dxi, double dyi, double gam,
-                                          double seps, const double *__restrict__ u, const double *__restrict__ v,
-                                          const double *__restrict__ w, const double *__restrict__ normx,
-                                          const double *__restrict__ normy, const double *__restrict__ normz,
-                                          const double *__restrict__ phi, const double *__restrict__ psi, double &R,
-                                          double &fx, double &fy, double &fz) {
+                                          const double *__restrict__ dzfi, double dxi_d, double dyi_d, double gam_d,
+                                          double seps_d, const double *__restrict__ u, const double *__restrict__ v,
+                                          const double *__restrict__ w, const T *__restrict__ normx,
+                                          const T *__restrict__ normy, const T *__restrict__ normz,
+                                          const T *__restrict__ phi, const T *__restrict__ psi, T &R, T &fx, T &fy,
+                                          T &fz) {
   const long sy = d.sy, sz = d.sz;
-  const double dzfi_c = dzfi[k], dzci_c = dzci[k], dzci_m = dzci[k - 1];
-  const double psi_ccm = psi[q - sz], psi_cmc = psi[q - sy], psi_mcc = psi[q - 1], psi_ccc = psi[q],
-               psi_pcc = psi[q + 1], psi_cpc = psi[q + sy], psi_ccp = psi[q + sz];
-  const double phi_ccm = phi[q - sz], phi_cmc = phi[q - sy], phi_mcc = phi[q - 1], phi_ccc = phi[q],
-               phi_pcc = phi[q + 1], phi_cpc = phi[q + sy], phi_ccp = phi[q + sz];
-  const double u_mcc = u[q - 1], u_ccc = u[q], v_cmc = v[q - sy], v_ccc = v[q], w_ccm = w[q - sz], w_ccc = w[q];
-  const double nxc = normx[q], nyc = normy[q], nzc = normz[q];
-  const double nx_xm = 0.5 * (normx[q - 1] + nxc), nx_xp = 0.5 * (normx[q + 1] + nxc);
-  const double nx_ym = 0.5 * (normx[q - sy] + nxc), nx_yp = 0.5 * (normx[q + sy] + nxc);
-  const double nx_zm = 0.5 * (normx[q - sz] + nxc), nx_zp = 0.5 * (normx[q + sz] + nxc);
-  const double ny_xm = 0.5 * (normy[q - 1] + nyc), ny_xp = 0.5 * (normy[q + 1] + nyc);
-  const double ny_ym = 0.5 * (normy[q - sy] + nyc), ny_yp = 0.5 * (normy[q + sy] + nyc);
-  const double ny_zm = 0.5 * (normy[q - sz] + nyc), ny_zp = 0.5 * (normy[q + sz] + nyc);
-  const double nz_xm = 0.5 * (normz[q - 1] + nzc), nz_xp = 0.5 * (normz[q + 1] + nzc);
-  const double nz_ym = 0.5 * (normz[q - sy] + nzc), nz_yp = 0.5 * (normz[q + sy] + nzc);
-  const double nz_zm = 0.5 * (normz[q - sz] + nzc), nz_zp = 0.5 * (normz[q + sz] + nzc);
-  const double upsiim = 0.5 * (psi_mcc + psi_ccc) * u_mcc, upsiip = 0.5 * (psi_pcc + psi_ccc) * u_ccc;
-  const double vpsijm = 0.5 * (psi_cmc + psi_ccc) * v_cmc, vpsijp = 0.5 * (psi_cpc + psi_ccc) * v_ccc;
-  const double wpsikm = 0.5 * (psi_ccm + psi_ccc) * w_ccm, wpsikp = 0.5 * (psi_ccp + psi_ccc) * w_ccc;
-  const double adv = dxi * (upsiip - upsiim) + dyi * (vpsijp - vpsijm) + dzfi_c * (wpsikp - wpsikm);
-  const double gs = gam * seps;
-  const double dpsidxm = gs * (psi_ccc - psi_mcc) * dxi, dpsidxp = gs * (psi_pcc - psi_ccc) * dxi;
-  const double dpsidym = gs * (psi_ccc - psi_cmc) * dyi, dpsidyp = gs * (psi_cpc - psi_ccc) * dyi;
-  const double dpsidzm = gs * (psi_ccc - psi_ccm) * dzci_m, dpsidzp = gs * (psi_ccp - psi_ccc) * dzci_c;
-  const double rn_01 = nx_xm / (sqrt(nx_xm * nx_xm + ny_xm * ny_xm + nz_xm * nz_xm) + FZ_EPS);
-  const double rn_11 = nx_xp / (sqrt(nx_xp * nx_xp + ny_xp * ny_xp + nz_xp * nz_xp) + FZ_EPS);
-  const double rn_02 = ny_ym / (sqrt(nx_ym * nx_ym + ny_ym * ny_ym + nz_ym * nz_ym) + FZ_EPS);
-  const double rn_12 = ny_yp / (sqrt(nx_yp * nx_yp + ny_yp * ny_yp + nz_yp * nz_yp) + FZ_EPS);
-  const double rn_03 = nz_zm / (sqrt(nx_zm * nx_zm + ny_zm * ny_zm + nz_zm * nz_zm) + FZ_EPS);
-  const double rn_13 = nz_zp / (sqrt(nx_zp * nx_zp + ny_zp * ny_zp + nz_zp * nz_zp) + FZ_EPS);
-  const double g4 = 0.25 * gam;
-  double t;
-  t = fzm::tanh(0.25 * (phi_ccc + phi_mcc) / seps); const double sharpxm = g4 * ((1.0 - t * t) * rn_01);
-  t = fzm::tanh(0.25 * (phi_pcc + phi_ccc) / seps); const double sharpxp = g4 * ((1.0 - t * t) * rn_11);
-  t = fzm::tanh(0.25 * (phi_ccc + phi_cmc) / seps); const double sharpym = g4 * ((1.0 - t * t) * rn_02);
-  t = fzm::tanh(0.25 * (phi_cpc + phi_ccc) / seps); const double sharpyp = g4 * ((1.0 - t * t) * rn_12);
-  t = fzm::tanh(0.25 * (phi_ccc + phi_ccm) / seps); const double sharpzm = g4 * ((1.0 - t * t) * rn_03);
-  t = fzm::tanh(0.25 * (phi_ccp + phi_ccc) / seps); const double sharpzp = g4 * ((1.0 - t * t) * rn_13);
-  const double rglrxm = dpsidxm - sharpxm, rglrxp = dpsidxp - sharpxp;
-  const double rglrym = dpsidym - sharpym, rglryp = dpsidyp - sharpyp;
-  const double rglrzm = dpsidzm - sharpzm, rglrzp = dpsidzp - sharpzp;
-  const double rglr = (rglrxp - rglrxm) * dxi + (rglryp - rglrym) * dyi + (rglrzp - rglrzm) * dzfi_c;
+  const T dxi = (T)dxi_d, dyi = (T)dyi_d, gam = (T)gam_d, seps = (T)seps_d;
+  const T dzfi_c = (T)dzfi[k], dzci_c = (T)dzci[k], dzci_m = (T)dzci[k - 1];
+  const T psi_ccm = psi[q - sz], psi_cmc = psi[q - sy], psi_mcc = psi[q - 1], psi_ccc = psi[q],
+          psi_pcc = psi[q + 1], psi_cpc = psi[q + sy], psi_ccp = psi[q + sz];
+  const T phi_ccm = phi[q - sz], phi_cmc = phi[q - sy], phi_mcc = phi[q - 1], phi_ccc = phi[q],
+          phi_pcc = phi[q + 1], phi_cpc = phi[q + sy], phi_ccp = phi[q + sz];
+  const T u_mcc = (T)u[q - 1], u_ccc = (T)u[q], v_cmc = (T)v[q - sy], v_ccc = (T)v[q], w_ccm = (T)w[q - sz],
+          w_ccc = (T)w[q];
+  const T nxc = normx[q], nyc = normy[q], nzc = normz[q];
+  const T nx_xm = (T)0.5 * (normx[q - 1] + nxc), nx_xp = (T)0.5 * (normx[q + 1] + nxc);
+  const T nx_ym = (T)0.5 * (normx[q - sy] + nxc), nx_yp = (T)0.5 * (normx[q + sy] + nxc);
+  const T nx_zm = (T)0.5 * (normx[q - sz] + nxc), nx_zp = (T)0.5 * (normx[q + sz] + nxc);
+  const T ny_xm = (T)0.5 * (normy[q - 1] + nyc), ny_xp = (T)0.5 * (normy[q + 1] + nyc);
+  const T ny_ym = (T)0.5 * (normy[q - sy] + nyc), ny_yp = (T)0.5 * (normy[q + sy] + nyc);
+  const T ny_zm = (T)0.5 * (normy[q - sz] + nyc), ny_zp = (T)0.5 * (normy[q + sz] + nyc);
+  const T nz_xm = (T)0.5 * (normz[q - 1] + nzc), nz_xp = (T)0.5 * (normz[q + 1] + nzc);
+  const T nz_ym = (T)0.5 * (normz[q - sy] + nzc), nz_yp = (T)0.5 * (normz[q + sy] + nzc);
+  const T nz_zm = (T)0.5 * (normz[q - sz] + nzc), nz_zp = (T)0.5 * (normz[q + sz] + nzc);
+  const T upsiim = (T)0.5 * (psi_mcc + psi_ccc) * u_mcc, upsiip = (T)0.5 * (psi_pcc + psi_ccc) * u_ccc;
+  const T vpsijm = (T)0.5 * (psi_cmc + psi_ccc) * v_cmc, vpsijp = (T)0.5 * (psi_cpc + psi_ccc) * v_ccc;
+  const T wpsikm = (T)0.5 * (psi_ccm + psi_ccc) * w_ccm, wpsikp = (T)0.5 * (psi_ccp + psi_ccc) * w_ccc;
+  const T adv = dxi * (upsiip - upsiim) + dyi * (vpsijp - vpsijm) + dzfi_c * (wpsikp - wpsikm);
+  const T gs = gam * seps;
+  const T dpsidxm = gs * (psi_ccc - psi_mcc) * dxi, dpsidxp = gs * (psi_pcc - psi_ccc) * dxi;
+  const T dpsidym = gs * (psi_ccc - psi_cmc) * dyi, dpsidyp = gs * (psi_cpc - psi_ccc) * dyi;
+  const T dpsidzm = gs * (psi_ccc - psi_ccm) * dzci_m, dpsidzp = gs * (psi_ccp - psi_ccc) * dzci_c;
+  const T rn_01 = nx_xm / (sqrt(nx_xm * nx_xm + ny_xm * ny_xm + nz_xm * nz_xm) + (T)FZ_EPS);
+  const T rn_11 = nx_xp / (sqrt(nx_xp * nx_xp + ny_xp * ny_xp + nz_xp * nz_xp) + (T)FZ_EPS);
+  const T rn_02 = ny_ym / (sqrt(nx_ym * nx_ym + ny_ym * ny_ym + nz_ym * nz_ym) + (T)FZ_EPS);
+  const T rn_12 = ny_yp / (sqrt(nx_yp * nx_yp + ny_yp * ny_yp + nz_yp * nz_yp) + (T)FZ_EPS);
+  const T rn_03 = nz_zm / (sqrt(nx_zm * nx_zm + ny_zm * ny_zm + nz_zm * nz_zm) + (T)FZ_EPS);
+  const T rn_13 = nz_zp / (sqrt(nx_zp * nx_zp + ny_zp * ny_zp + nz_zp * nz_zp) + (T)FZ_EPS);
+  const T g4 = (T)0.25 * gam;
+  T t;
+  t = fz_tanh((T)0.25 * (phi_ccc + phi_mcc) / seps); const T sharpxm = g4 * (((T)1.0 - t * t) * rn_01);
+  t = fz_tanh((T)0.25 * (phi_pcc + phi_ccc) / seps); const T sharpxp = g4 * (((T)1.0 - t * t) * rn_11);
+  t = fz_tanh((T)0.25 * (phi_ccc + phi_cmc) / seps); const T sharpym = g4 * (((T)1.0 - t * t) * rn_02);
+  t = fz_tanh((T)0.25 * (phi_cpc + phi_ccc) / seps); const T sharpyp = g4 * (((T)1.0 - t * t) * rn_12);
+  t = fz_tanh((T)0.25 * (phi_ccc + phi_ccm) / seps); const T sharpzm = g4 * (((T)1.0 - t * t) * rn_03);
+  t = fz_tanh((T)0.25 * (phi_ccp + phi_ccc) / seps); const T sharpzp = g4 * (((T)1.0 - t * t) * rn_13);
+  const T rglrxm = dpsidxm - sharpxm, rglrxp = dpsidxp - sharpxp;
+  const T rglrym = dpsidym - sharpym, rglryp = dpsidyp - sharpyp;
+  const T rglrzm = dpsidzm - sharpzm, rglrzp = dpsidzp - sharpzp;
+  const T rglr = (rglrxp - rglrxm) * dxi + (rglryp - rglrym) * dyi + (rglrzp - rglrzm) * dzfi_c;
   R = -adv + rglr;
   fx = upsiip - rglrxp;
   fy = vpsijp - rglryp;
@@ -249,45 +266,59 @@ __global__ void __launch_bounds__(256) k_acdi_transport(fz_dims d, const double 
 // reference also clips the halos, which the boundary update that follows
 // overwrites from the interior). The history dpsio is read and replaced by
 // R in place; on the first step the history is R itself.
+template <typename T>
 __global__ void __launch_bounds__(256) k_acdi_step(fz_dims d, const double *__restrict__ dzci,
                                                    const double *__restrict__ dzfi, double dxi, double dyi,
                                                    const double *__restrict__ d_gam, double gam_v, double seps,
-                                                   double factor1, double factor2, int first,
+                                                   double factor1_v, double factor2_v,
+                                                   const double *__restrict__ stc, int first,
                                                    const double *__restrict__ u, const double *__restrict__ v,
-                                                   const double *__restrict__ w, const double *__restrict__ normx,
-                                                   const double *__restrict__ normy, const double *__restrict__ normz,
-                                                   const double *__restrict__ phi, const double *__restrict__ psio,
-                                                   double *__restrict__ dpsio, double *__restrict__ psi_out,
-                                                   double *__restrict__ flux_x, double *__restrict__ flux_y,
-                                                   double *__restrict__ flux_z) {
+                                                   const double *__restrict__ w, const T *__restrict__ normx,
+                                                   const T *__restrict__ normy, const T *__restrict__ normz,
+                                                   const T *__restrict__ phi, const T *__restrict__ psio,
+                                                   T *__restrict__ dpsio, T *__restrict__ psi_out,
+                                                   T *__restrict__ flux_x, T *__restrict__ flux_y,
+                                                   T *__restrict__ flux_z) {
   CELL_IJK(d);
   const double gam = d_gam ? *d_gam : gam_v;
-  double R, fx, fy, fz;
-  acdi_cell(d, q, k, dzci, dzfi, dxi, dyi, gam, seps, u, v, w, normx, normy, normz, phi, psio, R, fx, fy, fz);
-  const double Ro = first ? R : dpsio[r];
+  const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
+  T R, fx, fy, fz;
+  acdi_cell<T>(d, q, k, dzci, dzfi, dxi, dyi, gam, seps, u, v, w, normx, normy, normz, phi, psio, R, fx, fy, fz);
+  const T Ro = first ? R : dpsio[r];
   dpsio[r] = R;
   flux_x[q] = fx;
   flux_y[q] = fy;
   flux_z[q] = fz;
-  const double pn = psio[q] + factor1 * R + factor2 * Ro;
-  psi_out[q] = fmin(fmax(0.0, pn), 1.0);
+  const T pn = psio[q] + (T)factor1 * R + (T)factor2 * Ro;
+  psi_out[q] = fmin(fmax((T)0.0, pn), (T)1.0);
 }
 
+template <typename T>
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
-                        int first, const double *u, const double *v, const double *w, const double *normx,
-                        const double *normy, const double *normz, const double *phi, const double *psio,
-                        double *dpsio, double *psi_out, double *flux_x, double *flux_y, double *flux_z) {
-  FZ_LAUNCH(c, "acdi_step", k_acdi_step, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
-            c->dli[1], d_gam, gam, seps, factor1, factor2, first, u, v, w, normx, normy, normz, phi, psio, dpsio,
-            psi_out, flux_x, flux_y, flux_z);
+                        const double *stc, int first, const double *u, const double *v, const double *w,
+                        const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
+                        T *psi_out, T *flux_x, T *flux_y, T *flux_z) {
+  FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", k_acdi_step<T>, fz_grid_interior(c->d), fz_block3(),
+            0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc, first, u, v, w,
+            normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
   FZ_LAUNCHED();
   return FZ_OK;
 }
+template int fz_launch_acdi_step<double>(fz_ctx *, const double *, double, double, double, double, const double *,
+                                         int, const double *, const double *, const double *, const double *,
+                                         const double *, const double *, const double *, const double *, double *,
+                                         double *, double *, double *, double *);
+template int fz_launch_acdi_step<float>(fz_ctx *, const double *, double, double, double, double, const double *,
+                                        int, const double *, const double *, const double *, const float *,
+                                        const float *, const float *, const float *, const float *, float *, float *,
+                                        float *, float *, float *);
 
 // acdi_cmpt_phi, src/acdi.f90:239-247 (halo-inclusive)
-__global__ void k_acdi_phi(long nh, double seps, const double *__restrict__ psi, double *__restrict__ phi) {
+template <typename T>
+__global__ void k_acdi_phi(long nh, double seps_d, const T *__restrict__ psi, T *__restrict__ phi) {
+  const T seps = (T)seps_d;
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
-    phi[q] = seps * fzm::log((psi[q] + FZ_EPS) / (1.0 - psi[q] + FZ_EPS));
+    phi[q] = seps * fz_log((psi[q] + (T)FZ_EPS) / ((T)1.0 - psi[q] + (T)FZ_EPS));
 }
 
 // vof_thinc_cmpt_phi, src/vof_thinc_qq.f90:148-156
@@ -339,77 +370,83 @@ __global__ void k_clip(long nh, double rmin, double rmax, double *__restrict__ p
 }
 
 // cmpt_norm_curv_youngs, src/two_fluid.f90:81-174
+// (T = float: the mixed-precision step, FP32 fields and arithmetic)
+template <typename T>
 __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restrict__ dzci,
-                                                const double *__restrict__ dzfi, double dxi, double dyi,
-                                                const double *__restrict__ psi, double *__restrict__ normx,
-                                                double *__restrict__ normy, double *__restrict__ normz,
-                                                double *__restrict__ kappa) {
+                                                const double *__restrict__ dzfi, double dxi_d, double dyi_d,
+                                                const T *__restrict__ psi, T *__restrict__ normx,
+                                                T *__restrict__ normy, T *__restrict__ normz,
+                                                T *__restrict__ kappa) {
   CELL_IJK(d);
   (void)r;
   const long sy = d.sy, sz = d.sz;
+  const T dxi = (T)dxi_d, dyi = (T)dyi_d;
 #define P_(a, b, c) psi[q + (a) + (b)*sy + (c)*sz]
-  const double mmm = P_(-1, -1, -1), mcm = P_(-1, 0, -1), mpm = P_(-1, 1, -1), cmm = P_(0, -1, -1), ccm = P_(0, 0, -1),
+  const T mmm = P_(-1, -1, -1), mcm = P_(-1, 0, -1), mpm = P_(-1, 1, -1), cmm = P_(0, -1, -1), ccm = P_(0, 0, -1),
                cpm = P_(0, 1, -1), pmm = P_(1, -1, -1), pcm = P_(1, 0, -1), ppm = P_(1, 1, -1);
-  const double mmc = P_(-1, -1, 0), mcc = P_(-1, 0, 0), mpc = P_(-1, 1, 0), cmc = P_(0, -1, 0), ccc = P_(0, 0, 0),
+  const T mmc = P_(-1, -1, 0), mcc = P_(-1, 0, 0), mpc = P_(-1, 1, 0), cmc = P_(0, -1, 0), ccc = P_(0, 0, 0),
                cpc = P_(0, 1, 0), pmc = P_(1, -1, 0), pcc = P_(1, 0, 0), ppc = P_(1, 1, 0);
-  const double mmp = P_(-1, -1, 1), mcp = P_(-1, 0, 1), mpp = P_(-1, 1, 1), cmp = P_(0, -1, 1), ccp = P_(0, 0, 1),
+  const T mmp = P_(-1, -1, 1), mcp = P_(-1, 0, 1), mpp = P_(-1, 1, 1), cmp = P_(0, -1, 1), ccp = P_(0, 0, 1),
                cpp = P_(0, 1, 1), pmp = P_(1, -1, 1), pcp = P_(1, 0, 1), ppp = P_(1, 1, 1);
 #undef P_
-  const double zc0 = dzci[k], zcm = dzci[k - 1];
-  double mx[8], my[8], mz[8];
-  mx[0] = 0.25 * ((pcc + ppc + pcp + ppp) - (ccc + cpc + ccp + cpp)) * dxi;
-  mx[1] = 0.25 * ((pcc + pmc + pcp + pmp) - (ccc + cmc + ccp + cmp)) * dxi;
-  mx[2] = 0.25 * ((pcc + ppc + pcm + ppm) - (ccc + cpc + ccm + cpm)) * dxi;
-  mx[3] = 0.25 * ((pcc + pmc + pcm + pmm) - (ccc + cmc + ccm + cmm)) * dxi;
-  mx[4] = 0.25 * ((ccc + cpc + ccp + cpp) - (mcc + mpc + mcp + mpp)) * dxi;
-  mx[5] = 0.25 * ((ccc + cmc + ccp + cmp) - (mcc + mmc + mcp + mmp)) * dxi;
-  mx[6] = 0.25 * ((ccc + cpc + ccm + cpm) - (mcc + mpc + mcm + mpm)) * dxi;
-  mx[7] = 0.25 * ((ccc + cmc + ccm + cmm) - (mcc + mmc + mcm + mmm)) * dxi;
-  my[0] = 0.25 * ((cpc + ppc + cpp + ppp) - (ccc + pcc + ccp + pcp)) * dyi;
-  my[1] = 0.25 * ((ccc + pcc + ccp + pcp) - (cmc + pmc + cmp + pmp)) * dyi;
-  my[2] = 0.25 * ((cpc + ppc + cpm + ppm) - (ccc + pcc + ccm + pcm)) * dyi;
-  my[3] = 0.25 * ((ccc + pcc + ccm + pcm) - (cmc + pmc + cmm + pmm)) * dyi;
-  my[4] = 0.25 * ((cpc + mpc + cpp + mpp) - (ccc + mcc + ccp + mcp)) * dyi;
-  my[5] = 0.25 * ((ccc + mcc + ccp + mcp) - (cmc + mmc + cmp + mmp)) * dyi;
-  my[6] = 0.25 * ((cpc + mpc + cpm + mpm) - (ccc + mcc + ccm + mcm)) * dyi;
-  my[7] = 0.25 * ((ccc + mcc + ccm + mcm) - (cmc + mmc + cmm + mmm)) * dyi;
-  mz[0] = 0.25 * ((ccp + pcp + cpp + ppp) - (ccc + pcc + cpc + ppc)) * zc0;
-  mz[1] = 0.25 * ((ccp + pcp + cmp + pmp) - (ccc + pcc + cmc + pmc)) * zc0;
-  mz[2] = 0.25 * ((ccc + pcc + cpc + ppc) - (ccm + pcm + cpm + ppm)) * zcm;
-  mz[3] = 0.25 * ((ccc + pcc + cmc + pmc) - (ccm + pcm + cmm + pmm)) * zcm;
-  mz[4] = 0.25 * ((ccp + mcp + cpp + mpp) - (ccc + mcc + cpc + mpc)) * zc0;
-  mz[5] = 0.25 * ((ccp + mcp + cmp + mmp) - (ccc + mcc + cmc + mmc)) * zc0;
-  mz[6] = 0.25 * ((ccc + mcc + cpc + mpc) - (ccm + mcm + cpm + mpm)) * zcm;
-  mz[7] = 0.25 * ((ccc + mcc + cmc + mmc) - (ccm + mcm + cmm + mmm)) * zcm;
+  const T zc0 = (T)dzci[k], zcm = (T)dzci[k - 1];
+  T mx[8], my[8], mz[8];
+  mx[0] = (T)0.25 * ((pcc + ppc + pcp + ppp) - (ccc + cpc + ccp + cpp)) * dxi;
+  mx[1] = (T)0.25 * ((pcc + pmc + pcp + pmp) - (ccc + cmc + ccp + cmp)) * dxi;
+  mx[2] = (T)0.25 * ((pcc + ppc + pcm + ppm) - (ccc + cpc + ccm + cpm)) * dxi;
+  mx[3] = (T)0.25 * ((pcc + pmc + pcm + pmm) - (ccc + cmc + ccm + cmm)) * dxi;
+  mx[4] = (T)0.25 * ((ccc + cpc + ccp + cpp) - (mcc + mpc + mcp + mpp)) * dxi;
+  mx[5] = (T)0.25 * ((ccc + cmc + ccp + cmp) - (mcc + mmc + mcp + mmp)) * dxi;
+  mx[6] = (T)0.25 * ((ccc + cpc + ccm + cpm) - (mcc + mpc + mcm + mpm)) * dxi;
+  mx[7] = (T)0.25 * ((ccc + cmc + ccm + cmm) - (mcc + mmc + mcm + mmm)) * dxi;
+  my[0] = (T)0.25 * ((cpc + ppc + cpp + ppp) - (ccc + pcc + ccp + pcp)) * dyi;
+  my[1] = (T)0.25 * ((ccc + pcc + ccp + pcp) - (cmc + pmc + cmp + pmp)) * dyi;
+  my[2] = (T)0.25 * ((cpc + ppc + cpm + ppm) - (ccc + pcc + ccm + pcm)) * dyi;
+  my[3] = (T)0.25 * ((ccc + pcc + ccm + pcm) - (cmc + pmc + cmm + pmm)) * dyi;
+  my[4] = (T)0.25 * ((cpc + mpc + cpp + mpp) - (ccc + mcc + ccp + mcp)) * dyi;
+  my[5] = (T)0.25 * ((ccc + mcc + ccp + mcp) - (cmc + mmc + cmp + mmp)) * dyi;
+  my[6] = (T)0.25 * ((cpc + mpc + cpm + mpm) - (ccc + mcc + ccm + mcm)) * dyi;
+  my[7] = (T)0.25 * ((ccc + mcc + ccm + mcm) - (cmc + mmc + cmm + mmm)) * dyi;
+  mz[0] = (T)0.25 * ((ccp + pcp + cpp + ppp) - (ccc + pcc + cpc + ppc)) * zc0;
+  mz[1] = (T)0.25 * ((ccp + pcp + cmp + pmp) - (ccc + pcc + cmc + pmc)) * zc0;
+  mz[2] = (T)0.25 * ((ccc + pcc + cpc + ppc) - (ccm + pcm + cpm + ppm)) * zcm;
+  mz[3] = (T)0.25 * ((ccc + pcc + cmc + pmc) - (ccm + pcm + cmm + pmm)) * zcm;
+  mz[4] = (T)0.25 * ((ccp + mcp + cpp + mpp) - (ccc + mcc + cpc + mpc)) * zc0;
+  mz[5] = (T)0.25 * ((ccp + mcp + cmp + mmp) - (ccc + mcc + cmc + mmc)) * zc0;
+  mz[6] = (T)0.25 * ((ccc + mcc + cpc + mpc) - (ccm + mcm + cpm + mpm)) * zcm;
+  mz[7] = (T)0.25 * ((ccc + mcc + cmc + mmc) - (ccm + mcm + cmm + mmm)) * zcm;
 #pragma unroll
   for (int m = 0; m < 8; m++) {
-    const double nrm = sqrt(mx[m] * mx[m] + my[m] * my[m] + mz[m] * mz[m]) + FZ_EPS;
+    const T nrm = sqrt(mx[m] * mx[m] + my[m] * my[m] + mz[m] * mz[m]) + (T)FZ_EPS;
     mx[m] = mx[m] / nrm;
     my[m] = my[m] / nrm;
     mz[m] = mz[m] / nrm;
   }
-  const double nx = .125 * (mx[0] + mx[1] + mx[2] + mx[3] + mx[4] + mx[5] + mx[6] + mx[7]);
-  const double ny = .125 * (my[0] + my[1] + my[2] + my[3] + my[4] + my[5] + my[6] + my[7]);
-  const double nz = .125 * (mz[0] + mz[1] + mz[2] + mz[3] + mz[4] + mz[5] + mz[6] + mz[7]);
-  const double nrm = sqrt(nx * nx + ny * ny + nz * nz) + FZ_EPS;
+  const T nx = (T).125 * (mx[0] + mx[1] + mx[2] + mx[3] + mx[4] + mx[5] + mx[6] + mx[7]);
+  const T ny = (T).125 * (my[0] + my[1] + my[2] + my[3] + my[4] + my[5] + my[6] + my[7]);
+  const T nz = (T).125 * (mz[0] + mz[1] + mz[2] + mz[3] + mz[4] + mz[5] + mz[6] + mz[7]);
+  const T nrm = sqrt(nx * nx + ny * ny + nz * nz) + (T)FZ_EPS;
   normx[q] = nx / nrm;
   normy[q] = ny / nrm;
   normz[q] = nz / nrm;
-  kappa[q] = -(0.25 * ((mx[0] + mx[1] + mx[2] + mx[3]) - (mx[4] + mx[5] + mx[6] + mx[7])) * dxi +
-               0.25 * ((my[0] + my[2] + my[4] + my[6]) - (my[1] + my[3] + my[5] + my[7])) * dyi +
-               0.25 * ((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7])) * dzfi[k]);
+  kappa[q] = -((T)0.25 * ((mx[0] + mx[1] + mx[2] + mx[3]) - (mx[4] + mx[5] + mx[6] + mx[7])) * dxi +
+               (T)0.25 * ((my[0] + my[2] + my[4] + my[6]) - (my[1] + my[3] + my[5] + my[7])) * dyi +
+               (T)0.25 * ((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7])) * (T)dzfi[k]);
 }
 
 // ---------------------------------------------------------------------------
 // mom_xyz_ad at one cell (linear index q): advection with the phase-field mass
 // flux and variable-viscosity (non-split) diffusion, src/mom.f90:545-850
+// (PT: element type of the phase fields, FP32 in the mixed-precision step;
+// the arithmetic is FP64 either way)
+template <typename PT>
 __device__ __forceinline__ void mom_ad_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
                                             const double *__restrict__ dzfi, double dxi, double dyi, double rho,
-                                            double drho, double mu, double dmu, const double *__restrict__ psiflx_x,
-                                            const double *__restrict__ psiflx_y,
-                                            const double *__restrict__ psiflx_z, const double *__restrict__ u,
-                                            const double *__restrict__ v, const double *__restrict__ w,
-                                            const double *__restrict__ psi, double &ru, double &rv, double &rw) {
+                                            double drho, double mu, double dmu, const PT *__restrict__ psiflx_x,
+                                            const PT *__restrict__ psiflx_y, const PT *__restrict__ psiflx_z,
+                                            const double *__restrict__ u, const double *__restrict__ v,
+                                            const double *__restrict__ w, const PT *__restrict__ psi, double &ru,
+                                            double &rv, double &rw) {
   const long sy = d.sy, sz = d.sz;
 #define S_(f, a, b, c) f[q + (a) + (b)*sy + (c)*sz]
   const double u_ccm = S_(u, 0, 0, -1), u_cmc = S_(u, 0, -1, 0), u_mcc = S_(u, -1, 0, 0), u_ccc = S_(u, 0, 0, 0),
@@ -540,12 +577,13 @@ __global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restr
 }
 
 // mom_xyz_oth at one cell, src/mom.f90:912-1038 (constant-coefficient split, Brackbill)
+template <typename PT>
 __device__ __forceinline__ void mom_oth_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
                                              double dxi, double dyi, double dt_r, double rho, double drho,
                                              double surf_factor, double rho0, double bfx, double bfy, double bfz,
                                              double gx, double gy, double gz, double rax, double ray, double raz,
-                                             const double *__restrict__ p, const double *__restrict__ psi,
-                                             const double *__restrict__ kappa, const double *__restrict__ pn,
+                                             const double *__restrict__ p, const PT *__restrict__ psi,
+                                             const PT *__restrict__ kappa, const double *__restrict__ pn,
                                              const double *__restrict__ po, double &ru, double &rv, double &rw) {
   const long sy = d.sy, sz = d.sz;
   const double dzci_c = dzci[k];
@@ -608,16 +646,17 @@ __global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__rest
 // advection-diffusion term in place (on the first step the history is the
 // term itself, src/rk.f90:76-83).
 // the body, shared by occupancy variants of the kernel
+// (PT: phase fields, HT: the AB2 history; both FP32 in the mixed-precision step)
+template <typename PT, typename HT>
 __device__ __forceinline__ void mom_fused_body(
-
     fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
     double drho, double mu, double dmu, double factor1, double factor2, int first, double factor12, double dt_r,
     double surf_factor, double rho0, double bfx, double bfy, double bfz, double gx, double gy, double gz, double rax,
-    double ray, double raz, const double *__restrict__ psiflx_x, const double *__restrict__ psiflx_y,
-    const double *__restrict__ psiflx_z, const double *__restrict__ u, const double *__restrict__ v,
-    const double *__restrict__ w, const double *__restrict__ psi, const double *__restrict__ psio,
-    const double *__restrict__ p, const double *__restrict__ po, const double *__restrict__ kappa,
-    double *__restrict__ duo, double *__restrict__ dvo, double *__restrict__ dwo, double *__restrict__ un,
+    double ray, double raz, const PT *__restrict__ psiflx_x, const PT *__restrict__ psiflx_y,
+    const PT *__restrict__ psiflx_z, const double *__restrict__ u, const double *__restrict__ v,
+    const double *__restrict__ w, const PT *__restrict__ psi, const PT *__restrict__ psio,
+    const double *__restrict__ p, const double *__restrict__ po, const PT *__restrict__ kappa,
+    HT *__restrict__ duo, HT *__restrict__ dvo, HT *__restrict__ dwo, double *__restrict__ un,
     double *__restrict__ vn, double *__restrict__ wn) {
   CELL_IJK(d);
   const long sy = d.sy, sz = d.sz;
@@ -644,30 +683,62 @@ __device__ __forceinline__ void mom_fused_body(
   wn[q] = ww + factor12 * rw / rhoz_p;
 }
 
-#define FZ_MOM_FUSED_KERNEL(NAME, ATTR)                                                                 \
-  __global__ void __launch_bounds__(256) ATTR NAME(    fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,    double drho, double mu, double dmu, double factor1, double factor2, int first, double factor12, double dt_r,    double surf_factor, double rho0, double bfx, double bfy, double bfz, double gx, double gy, double gz, double rax,    double ray, double raz, const double *__restrict__ psiflx_x, const double *__restrict__ psiflx_y,    const double *__restrict__ psiflx_z, const double *__restrict__ u, const double *__restrict__ v,    const double *__restrict__ w, const double *__restrict__ psi, const double *__restrict__ psio,    const double *__restrict__ p, const double *__restrict__ po, const double *__restrict__ kappa,    double *__restrict__ duo, double *__restrict__ dvo, double *__restrict__ dwo, double *__restrict__ un,    double *__restrict__ vn, double *__restrict__ wn) {                \
-    mom_fused_body(d, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, factor1, factor2, first, factor12, dt_r, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz, psiflx_x, psiflx_y, psiflx_z, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);                                                                          \
+#define FZ_MOM_FUSED_KERNEL(NAME, ATTR)                                                                          \
+  template <typename PT, typename HT>                                                                            \
+  __global__ void __launch_bounds__(256) ATTR NAME(                                                              \
+      fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi,       \
+      double rho, double drho, double mu, double dmu, double factor1_v, double factor2_v, int first,            \
+      double factor12_v, double dt_r_v, const double *__restrict__ stc, double surf_factor, double rho0,         \
+      double bfx, double bfy, double bfz, double gx, double gy,                                                   \
+      double gz, double rax, double ray, double raz, const PT *__restrict__ psiflx_x,                            \
+      const PT *__restrict__ psiflx_y, const PT *__restrict__ psiflx_z, const double *__restrict__ u,            \
+      const double *__restrict__ v, const double *__restrict__ w, const PT *__restrict__ psi,                    \
+      const PT *__restrict__ psio, const double *__restrict__ p, const double *__restrict__ po,                  \
+      const PT *__restrict__ kappa, HT *__restrict__ duo, HT *__restrict__ dvo, HT *__restrict__ dwo,            \
+      double *__restrict__ un, double *__restrict__ vn, double *__restrict__ wn) {                               \
+    const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;      \
+    const double factor12 = stc ? stc[FZ_STC_F12] : factor12_v, dt_r = stc ? stc[FZ_STC_DTR] : dt_r_v;         \
+    mom_fused_body<PT, HT>(d, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, factor1, factor2, first, factor12, dt_r, \
+                           surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz, psiflx_x, psiflx_y,      \
+                           psiflx_z, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);               \
   }
 FZ_MOM_FUSED_KERNEL(k_mom_fused, )
 FZ_MOM_FUSED_KERNEL(k_mom_fused_w4, __attribute__((amdgpu_waves_per_eu(4))))
 FZ_MOM_FUSED_KERNEL(k_mom_fused_w5, __attribute__((amdgpu_waves_per_eu(5))))
 
+template <typename PT, typename HT>
 int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], double factor1, double factor2,
-                        int first, double factor12, double dt_r, double sigma, double rho0, const double bforce[3],
-                        const double gacc[3], const double rho_av[3], const double *fx, const double *fy,
-                        const double *fz, const double *u, const double *v, const double *w, const double *psi,
-                        const double *psio, const double *p, const double *po, const double *kappa, double *duo,
-                        double *dvo, double *dwo, double *un, double *vn, double *wn) {
+                        int first, double factor12, double dt_r, const double *stc, double sigma, double rho0,
+                        const double bforce[3],
+                        const double gacc[3], const double rho_av[3], const PT *fx, const PT *fy, const PT *fz,
+                        const double *u, const double *v, const double *w, const PT *psi, const PT *psio,
+                        const double *p, const double *po, const PT *kappa, HT *duo, HT *dvo, HT *dwo, double *un,
+                        double *vn, double *wn) {
   const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
   static const int variant = getenv("FZ_MOM_WPE") ? atoi(getenv("FZ_MOM_WPE")) : 0;  // occupancy tuning
-  auto kern = variant == 4 ? k_mom_fused_w4 : (variant == 5 ? k_mom_fused_w5 : k_mom_fused);
-  FZ_LAUNCH(c, "mom_fused", kern, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
-            c->dli[1], rho12[1], rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12,
-            dt_r, surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1],
-            rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);
+  auto kern = variant == 4 ? k_mom_fused_w4<PT, HT> : (variant == 5 ? k_mom_fused_w5<PT, HT> : k_mom_fused<PT, HT>);
+  FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", kern, fz_grid_interior(c->d), fz_block3(), 0, c->d,
+            c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1], rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1,
+            factor2, first, factor12, dt_r, stc, surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1],
+            gacc[2], rho_av[0], rho_av[1], rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un,
+            vn, wn);
   FZ_LAUNCHED();
   return FZ_OK;
 }
+template int fz_launch_mom_fused<double, double>(fz_ctx *, const double *, const double *, double, double, int, double,
+                                                 double, const double *, double, double, const double *,
+                                                 const double *,
+                                                 const double *, const double *, const double *, const double *,
+                                                 const double *, const double *, const double *, const double *,
+                                                 const double *, const double *, const double *, const double *,
+                                                 double *, double *, double *, double *, double *, double *);
+template int fz_launch_mom_fused<float, float>(fz_ctx *, const double *, const double *, double, double, int, double,
+                                               double, const double *, double, double, const double *,
+                                               const double *, const double *,
+                                               const float *, const float *, const float *, const double *,
+                                               const double *, const double *, const float *, const float *,
+                                               const double *, const double *, const float *, float *, float *,
+                                               float *, double *, double *, double *);
 
 // AB2 momentum update with the density rescale, src/rk.f90:86-101
 __global__ void k_rk_mom_ab2(fz_dims d, double factor1, double factor2, double rho, double drho,
@@ -850,7 +921,21 @@ extern "C" int fz_acdi_transport_pf(fz_ctx *c, double gam, double seps, const do
 
 extern "C" int fz_acdi_cmpt_phi(fz_ctx *c, double seps, const double *psi, double *phi) {
   if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_acdi_cmpt_phi: null argument");
-  FZ_LAUNCH(c, "acdi_phi", k_acdi_phi, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi);
+  FZ_LAUNCH(c, "acdi_phi", k_acdi_phi<double>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+// FP32 phase field of the mixed-precision step
+int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi) {
+  FZ_LAUNCH(c, "acdi_phi_f32", k_acdi_phi<float>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float *normz, float *kappa) {
+  FZ_LAUNCH(c, "youngs_f32", k_youngs<float>, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi,
+            c->dli[0], c->dli[1], psi, normx, normy, normz, kappa);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -891,7 +976,7 @@ extern "C" int fz_cmpt_norm_curv_youngs(fz_ctx *c, const double *psi, double *no
                                         double *kappa) {
   if (!c || !psi || !normx || !normy || !normz || !kappa)
     return fz_set_error(FZ_ERR_ARG, "fz_cmpt_norm_curv_youngs: null argument");
-  FZ_LAUNCH(c, "youngs", k_youngs, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
+  FZ_LAUNCH(c, "youngs", k_youngs<double>, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
                      c->dli[1], psi, normx, normy, normz, kappa);
   FZ_LAUNCHED();
   return FZ_OK;

@@ -50,6 +50,7 @@ SIGNATURES = {
     "fz_initgrid": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp, dp]),
     "fz_initsolver": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp]),
     "fz_solver": (C.c_int, [vp, vp]),
+    "fz_solver_mixed": (C.c_int, [vp, vp, C.c_double, C.c_int, C.POINTER(C.c_int), dp]),
     "fz_r2r": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
     "fz_fillps": (C.c_int, [vp, C.c_double, C.c_double, vp, vp, vp, vp]),
     "fz_updt_rhs_b": (C.c_int, [vp, vp]),
@@ -80,6 +81,10 @@ SIGNATURES = {
     "fz_chkdt": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, dp, vp, vp, vp, C.c_double,
                            C.c_double, dp]),
     "fz_case_create": (C.c_int, [C.POINTER(vp), C.POINTER(FzParams), vp]),
+    "fz_case_create_ex": (C.c_int, [C.POINTER(vp), C.POINTER(FzParams), vp, C.c_int, C.c_int]),
+    "fz_case_precision": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fz_case_field_f32": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_size_t)]),
+    "fz_case_set_graphs": (C.c_int, [vp, C.c_int]),
     "fz_case_destroy": (C.c_int, [vp]),
     "fz_case_ctx": (C.c_int, [vp, C.POINTER(vp)]),
     "fz_case_field": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_size_t)]),
@@ -92,6 +97,19 @@ SIGNATURES = {
     "fz_load_one": (C.c_int, [vp, C.c_char, C.c_char_p, vp, dp, C.POINTER(C.c_int)]),
     "fz_out3d": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_int), vp]),
     "fz_write_grid": (C.c_int, [vp, C.c_char_p]),
+    "fz_case_footprint": (C.c_int, [C.POINTER(FzParams), C.POINTER(C.c_size_t)]),
+    "fz_case_footprint_ex": (C.c_int, [C.POINTER(FzParams), C.c_int, C.POINTER(C.c_size_t)]),
+    "fz_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "fz_server_create": (C.c_int, [C.POINTER(vp), C.c_size_t]),
+    "fz_server_destroy": (C.c_int, [vp]),
+    "fz_server_submit": (C.c_int, [vp, C.POINTER(FzParams), C.c_int, C.POINTER(C.c_int)]),
+    "fz_server_submit_ex": (C.c_int, [vp, C.POINTER(FzParams), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "fz_server_round": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fz_server_status": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fz_server_case": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+    "fz_server_release": (C.c_int, [vp, C.c_int]),
+    "fz_server_step_times": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
+    "fz_server_memory": (C.c_int, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
 }
 
 _lib = None

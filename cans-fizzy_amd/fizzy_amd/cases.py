@@ -170,6 +170,12 @@ def make_params(case):
     return fill_params(FzParams(), case)
 
 
+PRECISIONS = {"fp64": 0, "mixed": 1}
+# fields a mixed-precision case holds in FP32
+F32_FIELDS = {"psi", "phi", "kappa", "normx", "normy", "normz", "psiflx_x", "psiflx_y", "psiflx_z", "dpsidt_old",
+              "dudt_old", "dvdt_old", "dwdt_old"}
+
+
 class DeviceCase:
     """The device-resident case (fz_case_* of include/fizzy.h)."""
 
@@ -178,7 +184,9 @@ class DeviceCase:
         self.prm = make_params(case)
         self.n = [int(x) for x in case["ng"]]
         h = vp()
-        call("fz_case_create", C.byref(h), C.byref(self.prm), vp(stream) if stream else None)
+        prec = PRECISIONS[case.get("precision", "fp64")]
+        call("fz_case_create_ex", C.byref(h), C.byref(self.prm), vp(stream) if stream else None, prec,
+             int(case.get("refine", 1)))
         self._h = h
         c = vp()
         call("fz_case_ctx", self._h, C.byref(c))
@@ -214,26 +222,39 @@ class DeviceCase:
         n = self.n
         return (n[0] + 2, n[1] + 2, n[2] + 2) if fid < 15 else (n[0], n[1], n[2])
 
+    def precision(self):
+        """('fp64' | 'mixed', refinements)."""
+        pr, nr = C.c_int(), C.c_int()
+        call("fz_case_precision", self._h, C.byref(pr), C.byref(nr))
+        return {v: k for k, v in PRECISIONS.items()}[pr.value], nr.value
+
+    def field_dtype(self, name):
+        mixed = self.spec.get("precision", "fp64") == "mixed"
+        return np.float32 if mixed and name in F32_FIELDS else np.float64
+
     def field_ptr(self, name):
         fid = FIELD_IDS[name]
         p = vp()
         cnt = C.c_size_t()
-        call("fz_case_field", self._h, fid, C.byref(p), C.byref(cnt))
+        fn = "fz_case_field_f32" if self.field_dtype(name) == np.float32 else "fz_case_field"
+        call(fn, self._h, fid, C.byref(p), C.byref(cnt))
         return p.value, cnt.value
 
     def get(self, name):
+        """The field as float64 (FP32 fields of a mixed case are widened)."""
         fid = FIELD_IDS[name]
         ptr, cnt = self.field_ptr(name)
-        out = np.empty(self._fshape(fid), dtype=np.float64, order="F")
-        call("fz_download", self.ctx, out.ctypes.data_as(vp), vp(ptr), C.c_size_t(cnt * 8))
-        return out
+        dt = self.field_dtype(name)
+        out = np.empty(self._fshape(fid), dtype=dt, order="F")
+        call("fz_download", self.ctx, out.ctypes.data_as(vp), vp(ptr), C.c_size_t(cnt * out.itemsize))
+        return out.astype(np.float64, order="F")
 
     def set(self, name, arr):
         fid = FIELD_IDS[name]
         ptr, cnt = self.field_ptr(name)
-        a = np.asfortranarray(arr, dtype=np.float64)
+        a = np.asfortranarray(arr, dtype=self.field_dtype(name))
         assert a.shape == self._fshape(fid) and a.size == cnt
-        call("fz_upload", self.ctx, vp(ptr), a.ctypes.data_as(vp), C.c_size_t(cnt * 8))
+        call("fz_upload", self.ctx, vp(ptr), a.ctypes.data_as(vp), C.c_size_t(cnt * a.itemsize))
 
     def scalars(self):
         out = (C.c_double * 12)()

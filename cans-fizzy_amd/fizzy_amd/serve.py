@@ -14,14 +14,14 @@ import time
 import numpy as np
 
 from ._lib import FzParams, call, vp
-from .cases import DeviceCase, fill_params, splitmix64
+from .cases import PRECISIONS, DeviceCase, fill_params, splitmix64
 
 
 def footprint(spec):
-    """Device bytes a case needs (fz_case_footprint)."""
+    """Device bytes a case needs (fz_case_footprint_ex, in the spec's precision)."""
     prm = fill_params(FzParams(), spec)
     b = C.c_size_t()
-    call("fz_case_footprint", C.byref(prm), C.byref(b))
+    call("fz_case_footprint_ex", C.byref(prm), PRECISIONS[spec.get("precision", "fp64")], C.byref(b))
     return b.value
 
 
@@ -46,7 +46,8 @@ class Server:
     def submit(self, spec, nsteps):
         prm = fill_params(FzParams(), spec)
         rid = C.c_int()
-        call("fz_server_submit", self._h, C.byref(prm), int(nsteps), C.byref(rid))
+        call("fz_server_submit_ex", self._h, C.byref(prm), int(nsteps), PRECISIONS[spec.get("precision", "fp64")],
+             int(spec.get("refine", 1)), C.byref(rid))
         self.specs[rid.value] = spec
         return rid.value
 
@@ -136,10 +137,10 @@ def bubble_case(n, nz=None, two_d=False, density_ratio=10.0, mu12=(10.0, 1.0)):
 GRIDS = [(64, None, True), (128, None, True), (256, None, True), (64, 128, False), (128, 256, False)]
 
 
-def serving_requests(n, mean_interarrival=0.05, nsteps=200, seed=20261202):
+def serving_requests(n, mean_interarrival=0.05, nsteps=200, seed=20261202, precision="fp64"):
     """The serving mix: Poisson arrivals (exponential gaps, SplitMix64), grids
     uniform over {64^2, 128^2, 256^2, 64^2x128, 128^2x256}, density ratio
-    uniform in [2, 1000], nsteps steps each."""
+    uniform in [2, 1000], nsteps steps each, in the given precision mode."""
     g = splitmix64(seed)
     t, out = 0.0, []
     for _ in range(n):
@@ -147,5 +148,8 @@ def serving_requests(n, mean_interarrival=0.05, nsteps=200, seed=20261202):
         gi = min(int(next(g) * len(GRIDS)), len(GRIDS) - 1)
         ratio = 2.0 + 998.0 * next(g)
         nxy, nz, two_d = GRIDS[gi]
-        out.append((t, bubble_case(nxy, nz, two_d, ratio), nsteps))
+        spec = bubble_case(nxy, nz, two_d, ratio)
+        if precision != "fp64":
+            spec = dict(spec, precision=precision, refine=1)
+        out.append((t, spec, nsteps))
     return out

@@ -103,6 +103,15 @@ int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdaxy, double *
                   double *normfft, double *rhsbx, double *rhsby, double *rhsbz);
 /* solver / solver_gpu, src/solver.f90:19-78, src/solver_gpu.f90:31-163 */
 int fz_solver(fz_ctx *ctx, double *p);
+/* solver with FP32 transforms and line solves (FP64 pivots) wrapped in
+ * FP64-residual iterative refinement; p holds the right-hand side on entry
+ * and the solution on exit, as for fz_solver. Refinements stop when
+ * max|b - A x| <= tol*max|b|, after maxit of them, or when the residual
+ * stops falling; *iters receives the refinements done and *relres (if not
+ * null) the final max|b - A x|/max|b|. The null mode of an all-Neumann or
+ * periodic operator is fixed by pinning one unknown of the mean column
+ * (compare with the mean removed). No reference counterpart. */
+int fz_solver_mixed(fz_ctx *ctx, double *p, double tol, int maxit, int *iters, double *relres);
 /* fft, src/fft.f90:176-187: an r2r transform (FFTW kind codes 0 R2HC, 1 HC2R,
  * 4 REDFT01, 5 REDFT10, 6 REDFT11, 8 RODFT01, 9 RODFT10, 10 RODFT11) over
  * `howmany` contiguous lines of length n, in place and unnormalised */
@@ -209,6 +218,20 @@ int fz_write_grid(fz_ctx *ctx, const char *datadir);
  * 18 dwdt_old (interior) */
 /* scalars: 0 time,1 dt,2 dto,3 gam,4 seps,5 rho0,6 divtot,7 divmax,8 dt_cfl,9 istep,10 is_first */
 int fz_case_create(fz_case **cs, const fz_params *prm, void *hip_stream);
+/* precision modes of a case (no reference counterpart: the reference builds
+ * all-FP64 or, with -D_SINGLE_PRECISION, all-FP32) */
+#define FZ_PREC_FP64 0
+/* FP32 phase field (psi, phi, normals, curvature, phase fluxes) and FP32 AB2
+ * histories with FP32 arithmetic in the phase-field kernels; velocity,
+ * pressure, the momentum arithmetic and the global sums in FP64; the
+ * projection solves with FP32 transforms and line solves (FP64 pivots)
+ * followed by nref FP64-residual refinements. ACDI two-fluid cases with
+ * homogeneous pressure boundary values. */
+#define FZ_PREC_MIXED 1
+int fz_case_create_ex(fz_case **cs, const fz_params *prm, void *hip_stream, int precision, int nref);
+int fz_case_precision(fz_case *cs, int *precision, int *nref);
+/* the FP32 fields of a mixed-precision case (fz_case_field refuses them) */
+int fz_case_field_f32(fz_case *cs, int id, float **dptr, size_t *count);
 int fz_case_destroy(fz_case *cs);
 int fz_case_ctx(fz_case *cs, fz_ctx **ctx);
 int fz_case_field(fz_case *cs, int id, double **dptr, size_t *count);
@@ -220,6 +243,13 @@ int fz_case_init(fz_case *cs);
 /* nsteps time steps; *killed = 1 when a divergence or time-step check
  * stopped the run as src/main.f90:530-562 does */
 int fz_case_step(fz_case *cs, int nsteps, int *killed);
+/* replay each time step after the first from a captured HIP graph (one per
+ * parity of the buffer rotation; dt, its AB2 factors and gamma are read
+ * from device memory, so the graphs serve every step). Applies to ACDI
+ * two-fluid cases whose step needs no host read-back; other cases, and
+ * steps with per-launch profiling on, run as before. Results are the same
+ * bit for bit. No reference counterpart. */
+int fz_case_set_graphs(fz_case *cs, int on);
 /* the checkpoint of src/main.f90:597-631: prefix_u.bin, _v, _w, _p, _psi
  * in the load_one format, with the case's time and step */
 int fz_case_save(fz_case *cs, const char *prefix);
@@ -227,6 +257,35 @@ int fz_case_save(fz_case *cs, const char *prefix);
  * step from such files, then the same boundary conditions, phi/normals,
  * gamma and dt as a fresh start; the AB2 histories start over */
 int fz_case_restart(fz_case *cs, const char *prefix);
+
+/* ---- serving: many cases on one device (no reference counterpart; the
+ * footprint model follows device_memory_footprint, src/utils.f90:155-219,
+ * for this library's buffers) ---- */
+typedef struct fz_server fz_server;
+/* device bytes fz_case_create + fz_case_init allocate for prm */
+int fz_case_footprint(const fz_params *prm, size_t *bytes);
+/* the same for a precision mode of fz_case_create_ex */
+int fz_case_footprint_ex(const fz_params *prm, int precision, size_t *bytes);
+/* free and total memory of the current device */
+int fz_device_memory(size_t *free_bytes, size_t *total_bytes);
+int fz_server_create(fz_server **s, size_t hbm_budget);
+int fz_server_destroy(fz_server *s);
+/* queue a case for nsteps time steps; its footprint must fit the budget */
+int fz_server_submit(fz_server *s, const fz_params *prm, int nsteps, int *id);
+/* the same in a precision mode of fz_case_create_ex */
+int fz_server_submit_ex(fz_server *s, const fz_params *prm, int nsteps, int precision, int nref, int *id);
+/* admit queued cases in arrival order while their footprints fit, then one
+ * time step of every running case on its own stream; waits for the round */
+int fz_server_round(fz_server *s, int *n_active, int *n_queued);
+/* 0 queued, 1 running, 2 finished (killed: a check stopped it), -1 unknown */
+int fz_server_status(fz_server *s, int id, int *state, int *killed);
+/* a finished case (fields via fz_case_field) until it is released */
+int fz_server_case(fz_server *s, int id, fz_case **cs);
+int fz_server_release(fz_server *s, int id);
+/* durations (ms, HIP events on the case's stream) of the steps of request
+ * id, or of every step (id 0); *count is the total, at most cap copied */
+int fz_server_step_times(fz_server *s, int id, float *ms, int cap, int *count);
+int fz_server_memory(fz_server *s, size_t *in_use, size_t *budget);
 
 #ifdef __cplusplus
 }

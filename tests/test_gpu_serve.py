@@ -1,0 +1,108 @@
+"""Serving on one device (fz_server_*): a case's result is the same alone and
+in a mix of cases that arrive while others run, admission respects the
+footprint budget in arrival order, and the footprint model matches what a
+case actually allocates."""
+import numpy as np
+import pytest
+
+import fizzy_amd as F
+from fizzy_amd import cases, serve
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["u", "v", "w", "p", "psi", "phi", "kappa"]
+
+
+def mix():
+    return [(serve.bubble_case(32, two_d=True, density_ratio=500.0), 23),
+            (cases.rising_bubble_3d(ng=(24, 24, 48)), 17),
+            (serve.bubble_case(16, 32, density_ratio=3.0), 30),
+            (cases.hysing_2d(), 12),
+            (dict(serve.bubble_case(32, 32, density_ratio=20.0), precision="mixed", refine=1), 15)]
+
+
+def alone(spec, nsteps):
+    dc = F.DeviceCase(spec)
+    try:
+        dc.init()
+        dc.step(nsteps)
+        return {k: dc.get(k) for k in FIELDS}, dc.scalars()
+    finally:
+        dc.close()
+
+
+def test_results_are_the_same_alone_and_in_the_mix():
+    reqs = mix()
+    ref = [alone(s, n) for s, n in reqs]
+    free, _ = serve_memory()
+    srv = serve.Server(int(0.5 * free))
+    try:
+        ids = [srv.submit(*reqs[0]), srv.submit(*reqs[1])]
+        for _ in range(5):
+            srv.round()
+        ids.append(srv.submit(*reqs[2]))  # arrives while the first two run
+        for _ in range(3):
+            srv.round()
+        ids.append(srv.submit(*reqs[3]))
+        ids.append(srv.submit(*reqs[4]))
+        while any(srv.status(i)[0] != 2 for i in ids):
+            srv.round()
+        for i, (fields, sc) in zip(ids, ref):
+            dc = srv.case(i)
+            for k in FIELDS:
+                assert np.array_equal(dc.get(k), fields[k]), (i, k)
+            assert np.array_equal(dc.scalars(), sc)
+            assert len(srv.step_times(i)) == reqs[ids.index(i)][1]
+            srv.release(i)
+        assert srv.memory()[0] == 0
+    finally:
+        srv.close()
+
+
+def serve_memory():
+    import ctypes as C
+    f, t = C.c_size_t(), C.c_size_t()
+    F.call("fz_device_memory", C.byref(f), C.byref(t))
+    return f.value, t.value
+
+
+def test_admission_in_arrival_order_within_the_budget():
+    specs = [(cases.rising_bubble_3d(ng=(32, 32, 64)), 8), (serve.bubble_case(24, 48), 3),
+             (serve.bubble_case(24, 48, density_ratio=50.0), 2)]
+    b = [serve.footprint(s) for s, _ in specs]
+    assert b[1] == b[2]
+    srv = serve.Server(b[0] + b[1] + b[2] // 2)  # the first two fit together, the third once the second leaves
+    try:
+        ids = [srv.submit(s, n) for s, n in specs]
+        srv.round()
+        assert [srv.status(i)[0] for i in ids] == [1, 1, 0]
+        while srv.status(ids[1])[0] != 2:
+            srv.round()
+        assert srv.status(ids[2])[0] == 0  # finished cases hold their memory until released
+        srv.release(ids[1])
+        srv.round()
+        assert srv.status(ids[2])[0] == 1
+        used, budget = srv.memory()
+        assert used == b[0] + b[2] <= budget
+    finally:
+        srv.close()
+
+
+@pytest.mark.parametrize("spec", [cases.rising_bubble_3d(ng=(64, 64, 128)), serve.bubble_case(128, two_d=True),
+                                  cases.vof_droplets(ng=(48, 48, 48)),
+                                  dict(cases.rising_bubble_3d(ng=(64, 64, 128)), precision="mixed", refine=1)],
+                         ids=["bubble64x64x128", "bubble2d128", "vof48", "mixed64x64x128"])
+def test_footprint_model_matches_the_allocation(spec):
+    before, _ = serve_memory()
+    dc = F.DeviceCase(spec)
+    try:
+        dc.init()
+        dc.step(11)  # every boundary map and reduction buffer exists after a check step
+        dc.sync()
+        after, _ = serve_memory()
+        used = before - after
+        est = serve.footprint(spec)
+        print(spec["ng"], "allocated", used, "model", est)
+        assert 0.9 * used <= est <= 1.25 * used + (8 << 20)
+    finally:
+        dc.close()

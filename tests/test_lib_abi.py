@@ -53,3 +53,20 @@ def test_fortran_modules_bind_the_library():
               "mod_vof_thinc_qq", "mod_two_fluid", "mod_mom", "mod_rk", "mod_bound", "mod_chkdiv", "mod_chkdt",
               "mod_forcing", "mod_utils"]:
         assert m + ".mod" in mods, m
+
+
+def test_mixed_precision_refuses_what_it_does_not_cover():
+    """Argument checks of fz_case_create_ex run on the host before any device
+    allocation: unknown precision, a refinement count out of range and the
+    VOF step in mixed precision are refused with a message."""
+    import ctypes as C
+    from fizzy_amd import cases
+    from fizzy_amd._lib import FzParams, FZ_OK
+    L = F.lib()
+    for spec, prec, nref, msg in [(cases.rising_bubble_3d(ng=(16, 16, 16)), 7, 1, b"unknown precision"),
+                                  (cases.rising_bubble_3d(ng=(16, 16, 16)), 1, 9, b"refinements"),
+                                  (cases.vof_droplets(ng=(16, 16, 16)), 1, 1, b"ACDI")]:
+        prm = cases.fill_params(FzParams(), spec)
+        h = C.c_void_p()
+        rc = L.fz_case_create_ex(C.byref(h), C.byref(prm), None, prec, nref)
+        assert rc != FZ_OK and msg in L.fz_last_error(), L.fz_last_error()
