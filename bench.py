@@ -261,6 +261,8 @@ def run_mixed(F, spec, args, cells, fp64_s_per_step):
     dc = F.DeviceCase(ms)
     try:
         dc.init()
+        if not args.no_autotune:
+            log(f"autotune (mixed): {dc.autotune(3)}")
         if args.graphs:
             F.call("fz_case_set_graphs", dc._h, 1)
         if args.warmup:
@@ -307,6 +309,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run of the same workload")
     ap.add_argument("--refine", type=int, default=1, help="mixed precision: FP64-residual refinements per solve")
     ap.add_argument("--graphs", action="store_true", help="replay the steps from captured HIP graphs")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="c4: precision of the cases")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -329,6 +332,10 @@ def main():
     log(f"workload {args.workload}: {n[0]}x{n[1]}x{n[2]} = {cells} cells, warmup {args.warmup}, steps {args.steps}")
     dc = F.DeviceCase(spec)
     dc.init()
+    tuning = None
+    if not args.no_autotune:
+        tuning = dc.autotune(3)
+        log(f"autotune: {tuning}")
     if args.graphs:
         F.call("fz_case_set_graphs", dc._h, 1)
     if args.warmup:
@@ -393,7 +400,7 @@ def main():
         "config": {"workload": WORKLOAD_NAMES[args.workload],
                    "grid": n, "density_ratio": spec["rho12"][0] / spec["rho12"][1],
                    "viscosity_ratio": spec["mu12"][0] / spec["mu12"][1], "precision": "fp64",
-                   "step": "AB2 (one reference time step)"},
+                   "step": "AB2 (one reference time step)", "launch_parameters": tuning},
         "roofline": roof,
         "step_roofline": {"alg_bytes_per_cell": STEP_ALG_BYTES_PER_CELL, "achieved_GBs": round(step_bw, 1),
                           "frac": round(step_bw / HBM_PEAK_GBS, 4)},

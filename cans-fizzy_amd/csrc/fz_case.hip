@@ -629,6 +629,9 @@ static int graph_step(fz_case *cs, const StepArgs &a) {
   return FZ_OK;
 }
 
+const fz_params *fz_case_prm(const fz_case *cs) { return &cs->prm; }
+void fz_case_drop_graphs(fz_case *cs) { drop_graphs(cs); }
+
 extern "C" int fz_case_set_graphs(fz_case *cs, int on) {
   if (!cs) return fz_set_error(FZ_ERR_ARG, "fz_case_set_graphs: null case");
   if (!on) drop_graphs(cs);

@@ -91,6 +91,10 @@ struct fz_ctx {
   size_t red_cap = 0;
   double *h_small = nullptr;  // pinned host landing zone for reduction results
   std::map<std::string, struct fz_bcmap *> bcmaps;  // composed boundary maps by signature
+  // launch parameters (defaults, environment overrides, or fz_case_autotune)
+  int mom_variant = 0;  // k_mom_fused variant: 0, 1 (staged loads), 4 or 5 (waves per EU)
+  int tk64 = 16;        // rows per chunk of the FP64 Thomas kernel: 8, 16 or 32
+  int tk32 = 32;        // the same for the FP32 one
 };
 
 // brackets one launch with events when profiling is on
@@ -160,6 +164,8 @@ int fz_projection_rhs_solve(fz_ctx *ctx, double dti, double rho0, const double *
 int fz_projection_rhs_solve_mixed(fz_ctx *ctx, double dti, double rho0, const double *u, const double *v,
                                   const double *w, double *p, int nref, const double *stc = nullptr);
 int fz_projection_device_scalars_ok(fz_ctx *ctx, int mixed);
+const fz_params *fz_case_prm(const fz_case *cs);
+void fz_case_drop_graphs(fz_case *cs);
 int fz_acdi_gamma_dev(fz_ctx *ctx, double gam_factor, double gam_min, const double *u, const double *v,
                       const double *w, double *d_gam);
 int fz_acdi_transport_dev(fz_ctx *ctx, const double *d_gam, double gam, double seps, const double *u,

@@ -1010,10 +1010,17 @@ static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum) {
   const int tpb = 256;
   const int nb = (int)((ps + tpb - 1) / tpb);
   if (!P->periodic_z) {
-    // FP32 columns move half the bytes per load: twice the rows in flight
-    FZ_LAUNCH(ctx, mixed ? "thomas_f32" : "thomas", (k_thomas<sizeof(T) == 4 ? 32 : 16, T>), dim3((int)((ps + 63) / 64)),
-              dim3(64), 0, W,
-              P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3, gauge);
+    // rows per chunk: the context's choice (FP32 columns move half the
+    // bytes per load, so their default keeps twice the rows in flight)
+    const int tk = mixed ? ctx->tk32 : ctx->tk64;
+    const dim3 grid((int)((ps + 63) / 64)), blk(64);
+    const char *name = mixed ? "thomas_f32" : "thomas";
+    if (tk == 8) FZ_LAUNCH(ctx, name, (k_thomas<8, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
+                           n3, gauge);
+    else if (tk == 32) FZ_LAUNCH(ctx, name, (k_thomas<32, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c,
+                                 n1, n2, n3, gauge);
+    else FZ_LAUNCH(ctx, name, (k_thomas<16, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
+                   gauge);
   } else {
     FZ_LAUNCH(ctx, mixed ? "thomas_periodic_f32" : "thomas_periodic", k_thomas_periodic<T>, dim3(nb), dim3(tpb), 0, W,
               P->piv, P->aux, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3, gauge);

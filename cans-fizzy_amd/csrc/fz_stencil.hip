@@ -170,6 +170,56 @@ __global__ void k_add_face(fz_dims d, int dir, int plane, const double *__restri
   p[i + d.sy * j + d.sz * k] += rhs[a + (long)na * b];
 }
 
+// Stencil access to a halo'd field around one cell, (a,b,c) the neighbour
+// offsets in x, y, z. PtrAt indexes through a 64-bit address per load;
+// BufAt addresses through a buffer resource: one 32-bit byte offset per
+// cell in a vector register (the (-1,-1,-1) neighbour), the y/z strides as
+// scalar offsets and the x offset as an immediate, which spares the vector
+// registers and instructions of a 64-bit address per field and plane. The
+// values read are the same; a read outside the field returns 0 instead of
+// faulting (the kernels never make one).
+template <typename T>
+struct PtrAt {
+  const T *__restrict__ p;
+  long q, sy, sz;
+  __device__ __forceinline__ T operator()(int a, int b, int c) const { return p[q + a + b * sy + c * sz]; }
+};
+template <typename T>
+struct BufAt {
+  __amdgpu_buffer_rsrc_t r;
+  unsigned v0, sy, sz;  // bytes
+  __device__ __forceinline__ T operator()(int a, int b, int c) const {
+    const unsigned so = (unsigned)(b + 1) * sy + (unsigned)(c + 1) * sz;
+    const unsigned vo = v0 + (unsigned)((a + 1) * (int)sizeof(T));
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+  }
+};
+template <typename T>
+__device__ __forceinline__ PtrAt<T> ptr_at(const T *p, const fz_dims &d, long q) {
+  return PtrAt<T>{p, q, d.sy, d.sz};
+}
+// (fields below 4 GiB: the launchers check fits_buffers on the host)
+template <typename T>
+__device__ __forceinline__ BufAt<T> buf_at(const T *p, const fz_dims &d, long q) {
+  const unsigned sy = (unsigned)d.sy, sz = (unsigned)d.sz;
+  const unsigned bytes = (unsigned)((unsigned long long)d.sz * (unsigned)(d.n3 + 2) * sizeof(T));
+  BufAt<T> b;
+  b.r = __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+  b.v0 = ((unsigned)q - 1u - sy - sz) * (unsigned)sizeof(T);
+  b.sy = sy * (unsigned)sizeof(T);
+  b.sz = sz * (unsigned)sizeof(T);
+  return b;
+}
+
+// whether a halo'd FP64 field is small enough for 32-bit buffer offsets
+static int fits_buffers(const fz_ctx *c, const char *who) {
+  if ((unsigned long long)c->nh * sizeof(double) < (1ull << 32)) return FZ_OK;
+  return fz_set_error(FZ_ERR_UNSUPPORTED, "%s: halo'd fields above 4 GiB are not supported", who);
+}
+
 // transcendentals by phase-field type: the shared FP64 routines (identical
 // on the device and in the oracle), or the device library's FP32 ones
 __device__ inline double fz_tanh(double x) { return fzm::tanh(x); }
@@ -182,33 +232,30 @@ __device__ inline float fz_log(float x) { return logf(x); }
 // the right-hand side R and the three face fluxes of the cell's + faces
 // T is the type of the phase fields and of the arithmetic: double, or float
 // for the mixed-precision step (the FP64 velocities and grid are rounded to T)
-template <typename T>
-__device__ __forceinline__ void acdi_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
-                                          const double *__restrict__ dzfi, double dxi_d, double dyi_d, double gam_d,
-                                          double seps_d, const double *__restrict__ u, const double *__restrict__ v,
-                                          const double *__restrict__ w, const T *__restrict__ normx,
-                                          const T *__restrict__ normy, const T *__restrict__ normz,
-                                          const T *__restrict__ phi, const T *__restrict__ psi, T &R, T &fx, T &fy,
+template <typename T, class AD, class AP>
+__device__ __forceinline__ void acdi_cell(int k, const double *__restrict__ dzci, const double *__restrict__ dzfi,
+                                          double dxi_d, double dyi_d, double gam_d, double seps_d, const AD &u,
+                                          const AD &v, const AD &w, const AP &normx, const AP &normy,
+                                          const AP &normz, const AP &phi, const AP &psi, T &R, T &fx, T &fy,
                                           T &fz) {
-  const long sy = d.sy, sz = d.sz;
   const T dxi = (T)dxi_d, dyi = (T)dyi_d, gam = (T)gam_d, seps = (T)seps_d;
   const T dzfi_c = (T)dzfi[k], dzci_c = (T)dzci[k], dzci_m = (T)dzci[k - 1];
-  const T psi_ccm = psi[q - sz], psi_cmc = psi[q - sy], psi_mcc = psi[q - 1], psi_ccc = psi[q],
-          psi_pcc = psi[q + 1], psi_cpc = psi[q + sy], psi_ccp = psi[q + sz];
-  const T phi_ccm = phi[q - sz], phi_cmc = phi[q - sy], phi_mcc = phi[q - 1], phi_ccc = phi[q],
-          phi_pcc = phi[q + 1], phi_cpc = phi[q + sy], phi_ccp = phi[q + sz];
-  const T u_mcc = (T)u[q - 1], u_ccc = (T)u[q], v_cmc = (T)v[q - sy], v_ccc = (T)v[q], w_ccm = (T)w[q - sz],
-          w_ccc = (T)w[q];
-  const T nxc = normx[q], nyc = normy[q], nzc = normz[q];
-  const T nx_xm = (T)0.5 * (normx[q - 1] + nxc), nx_xp = (T)0.5 * (normx[q + 1] + nxc);
-  const T nx_ym = (T)0.5 * (normx[q - sy] + nxc), nx_yp = (T)0.5 * (normx[q + sy] + nxc);
-  const T nx_zm = (T)0.5 * (normx[q - sz] + nxc), nx_zp = (T)0.5 * (normx[q + sz] + nxc);
-  const T ny_xm = (T)0.5 * (normy[q - 1] + nyc), ny_xp = (T)0.5 * (normy[q + 1] + nyc);
-  const T ny_ym = (T)0.5 * (normy[q - sy] + nyc), ny_yp = (T)0.5 * (normy[q + sy] + nyc);
-  const T ny_zm = (T)0.5 * (normy[q - sz] + nyc), ny_zp = (T)0.5 * (normy[q + sz] + nyc);
-  const T nz_xm = (T)0.5 * (normz[q - 1] + nzc), nz_xp = (T)0.5 * (normz[q + 1] + nzc);
-  const T nz_ym = (T)0.5 * (normz[q - sy] + nzc), nz_yp = (T)0.5 * (normz[q + sy] + nzc);
-  const T nz_zm = (T)0.5 * (normz[q - sz] + nzc), nz_zp = (T)0.5 * (normz[q + sz] + nzc);
+  const T psi_ccm = psi(0, 0, -1), psi_cmc = psi(0, -1, 0), psi_mcc = psi(-1, 0, 0), psi_ccc = psi(0, 0, 0),
+          psi_pcc = psi(1, 0, 0), psi_cpc = psi(0, 1, 0), psi_ccp = psi(0, 0, 1);
+  const T phi_ccm = phi(0, 0, -1), phi_cmc = phi(0, -1, 0), phi_mcc = phi(-1, 0, 0), phi_ccc = phi(0, 0, 0),
+          phi_pcc = phi(1, 0, 0), phi_cpc = phi(0, 1, 0), phi_ccp = phi(0, 0, 1);
+  const T u_mcc = (T)u(-1, 0, 0), u_ccc = (T)u(0, 0, 0), v_cmc = (T)v(0, -1, 0), v_ccc = (T)v(0, 0, 0),
+          w_ccm = (T)w(0, 0, -1), w_ccc = (T)w(0, 0, 0);
+  const T nxc = normx(0, 0, 0), nyc = normy(0, 0, 0), nzc = normz(0, 0, 0);
+  const T nx_xm = (T)0.5 * (normx(-1, 0, 0) + nxc), nx_xp = (T)0.5 * (normx(1, 0, 0) + nxc);
+  const T nx_ym = (T)0.5 * (normx(0, -1, 0) + nxc), nx_yp = (T)0.5 * (normx(0, 1, 0) + nxc);
+  const T nx_zm = (T)0.5 * (normx(0, 0, -1) + nxc), nx_zp = (T)0.5 * (normx(0, 0, 1) + nxc);
+  const T ny_xm = (T)0.5 * (normy(-1, 0, 0) + nyc), ny_xp = (T)0.5 * (normy(1, 0, 0) + nyc);
+  const T ny_ym = (T)0.5 * (normy(0, -1, 0) + nyc), ny_yp = (T)0.5 * (normy(0, 1, 0) + nyc);
+  const T ny_zm = (T)0.5 * (normy(0, 0, -1) + nyc), ny_zp = (T)0.5 * (normy(0, 0, 1) + nyc);
+  const T nz_xm = (T)0.5 * (normz(-1, 0, 0) + nzc), nz_xp = (T)0.5 * (normz(1, 0, 0) + nzc);
+  const T nz_ym = (T)0.5 * (normz(0, -1, 0) + nzc), nz_yp = (T)0.5 * (normz(0, 1, 0) + nzc);
+  const T nz_zm = (T)0.5 * (normz(0, 0, -1) + nzc), nz_zp = (T)0.5 * (normz(0, 0, 1) + nzc);
   const T upsiim = (T)0.5 * (psi_mcc + psi_ccc) * u_mcc, upsiip = (T)0.5 * (psi_pcc + psi_ccc) * u_ccc;
   const T vpsijm = (T)0.5 * (psi_cmc + psi_ccc) * v_cmc, vpsijp = (T)0.5 * (psi_cpc + psi_ccc) * v_ccc;
   const T wpsikm = (T)0.5 * (psi_ccm + psi_ccc) * w_ccm, wpsikp = (T)0.5 * (psi_ccp + psi_ccc) * w_ccc;
@@ -254,7 +301,9 @@ __global__ void __launch_bounds__(256) k_acdi_transport(fz_dims d, const double 
   CELL_IJK(d);
   const double gam = d_gam ? *d_gam : gam_v;
   double R, fx, fy, fz;
-  acdi_cell(d, q, k, dzci, dzfi, dxi, dyi, gam, seps, u, v, w, normx, normy, normz, phi, psi, R, fx, fy, fz);
+  acdi_cell<double>(k, dzci, dzfi, dxi, dyi, gam, seps, ptr_at(u, d, q), ptr_at(v, d, q), ptr_at(w, d, q),
+                    ptr_at(normx, d, q), ptr_at(normy, d, q), ptr_at(normz, d, q), ptr_at(phi, d, q),
+                    ptr_at(psi, d, q), R, fx, fy, fz);
   dpsidt[r] = R;
   flux_x[q] = fx;
   flux_y[q] = fy;
@@ -283,13 +332,15 @@ __global__ void __launch_bounds__(256) k_acdi_step(fz_dims d, const double *__re
   const double gam = d_gam ? *d_gam : gam_v;
   const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
   T R, fx, fy, fz;
-  acdi_cell<T>(d, q, k, dzci, dzfi, dxi, dyi, gam, seps, u, v, w, normx, normy, normz, phi, psio, R, fx, fy, fz);
+  const BufAt<T> PSIO = buf_at(psio, d, q);
+  acdi_cell<T>(k, dzci, dzfi, dxi, dyi, gam, seps, buf_at(u, d, q), buf_at(v, d, q), buf_at(w, d, q),
+               buf_at(normx, d, q), buf_at(normy, d, q), buf_at(normz, d, q), buf_at(phi, d, q), PSIO, R, fx, fy, fz);
   const T Ro = first ? R : dpsio[r];
   dpsio[r] = R;
   flux_x[q] = fx;
   flux_y[q] = fy;
   flux_z[q] = fz;
-  const T pn = psio[q] + (T)factor1 * R + (T)factor2 * Ro;
+  const T pn = PSIO(0, 0, 0) + (T)factor1 * R + (T)factor2 * Ro;
   psi_out[q] = fmin(fmax((T)0.0, pn), (T)1.0);
 }
 
@@ -298,6 +349,7 @@ int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps,
                         const double *stc, int first, const double *u, const double *v, const double *w,
                         const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
                         T *psi_out, T *flux_x, T *flux_y, T *flux_z) {
+  FZ_TRY(fits_buffers(c, "acdi_step"));
   FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", k_acdi_step<T>, fz_grid_interior(c->d), fz_block3(),
             0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc, first, u, v, w,
             normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
@@ -379,9 +431,9 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
                                                 T *__restrict__ kappa) {
   CELL_IJK(d);
   (void)r;
-  const long sy = d.sy, sz = d.sz;
   const T dxi = (T)dxi_d, dyi = (T)dyi_d;
-#define P_(a, b, c) psi[q + (a) + (b)*sy + (c)*sz]
+  const BufAt<T> PS = buf_at(psi, d, q);
+#define P_(a, b, c) PS(a, b, c)
   const T mmm = P_(-1, -1, -1), mcm = P_(-1, 0, -1), mpm = P_(-1, 1, -1), cmm = P_(0, -1, -1), ccm = P_(0, 0, -1),
                cpm = P_(0, 1, -1), pmm = P_(1, -1, -1), pcm = P_(1, 0, -1), ppm = P_(1, 1, -1);
   const T mmc = P_(-1, -1, 0), mcc = P_(-1, 0, 0), mpc = P_(-1, 1, 0), cmc = P_(0, -1, 0), ccc = P_(0, 0, 0),
@@ -439,16 +491,15 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
 // flux and variable-viscosity (non-split) diffusion, src/mom.f90:545-850
 // (PT: element type of the phase fields, FP32 in the mixed-precision step;
 // the arithmetic is FP64 either way)
-template <typename PT>
-__device__ __forceinline__ void mom_ad_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
+// (AD: accessor of the FP64 velocity, AP: of the phase fields, PtrAt or BufAt)
+template <int MASK = 7, class AD, class AP>  // MASK: components computed (1 x, 2 y, 4 z)
+__device__ __forceinline__ void mom_ad_cell(const fz_dims &d, int k, const double *__restrict__ dzci,
                                             const double *__restrict__ dzfi, double dxi, double dyi, double rho,
-                                            double drho, double mu, double dmu, const PT *__restrict__ psiflx_x,
-                                            const PT *__restrict__ psiflx_y, const PT *__restrict__ psiflx_z,
-                                            const double *__restrict__ u, const double *__restrict__ v,
-                                            const double *__restrict__ w, const PT *__restrict__ psi, double &ru,
-                                            double &rv, double &rw) {
-  const long sy = d.sy, sz = d.sz;
-#define S_(f, a, b, c) f[q + (a) + (b)*sy + (c)*sz]
+                                            double drho, double mu, double dmu, const AP &psiflx_x,
+                                            const AP &psiflx_y, const AP &psiflx_z, const AD &u, const AD &v,
+                                            const AD &w, const AP &psi, double &ru, double &rv, double &rw) {
+  (void)d;
+#define S_(f, a, b, c) ((double)f(a, b, c))
   const double u_ccm = S_(u, 0, 0, -1), u_cmc = S_(u, 0, -1, 0), u_mcc = S_(u, -1, 0, 0), u_ccc = S_(u, 0, 0, 0),
                u_pcc = S_(u, 1, 0, 0), u_mpc = S_(u, -1, 1, 0), u_cpc = S_(u, 0, 1, 0), u_mcp = S_(u, -1, 0, 1),
                u_ccp = S_(u, 0, 0, 1);
@@ -487,28 +538,35 @@ __device__ __forceinline__ void mom_ad_cell(const fz_dims &d, long q, int k, con
   const double flx_z_cpc = rho * w_cpc + drho * S_(psiflx_z, 0, 1, 0);
 #undef S_
   const double dzci_c = dzci[k], dzci_m = dzci[k - 1], dzfi_c = dzfi[k], dzfi_p = dzfi[k + 1];
-  const double uuip = 0.25 * (flx_x_pcc + flx_x_ccc) * (u_ccc + u_pcc);
-  const double uuim = 0.25 * (flx_x_mcc + flx_x_ccc) * (u_ccc + u_mcc);
-  const double vujp = 0.25 * (flx_y_pcc + flx_y_ccc) * (u_ccc + u_cpc);
-  const double vujm = 0.25 * (flx_y_pmc + flx_y_cmc) * (u_ccc + u_cmc);
-  const double wukp = 0.25 * (flx_z_pcc + flx_z_ccc) * (u_ccc + u_ccp);
-  const double wukm = 0.25 * (flx_z_pcm + flx_z_ccm) * (u_ccc + u_ccm);
-  double dudt_aux = dxi * (-uuip + uuim) + dyi * (-vujp + vujm) + dzfi_c * (-wukp + wukm);
-  const double uvip = 0.25 * (flx_x_ccc + flx_x_cpc) * (v_ccc + v_pcc);
-  const double uvim = 0.25 * (flx_x_mcc + flx_x_mpc) * (v_ccc + v_mcc);
-  const double vvjp = 0.25 * (flx_y_ccc + flx_y_cpc) * (v_ccc + v_cpc);
-  const double vvjm = 0.25 * (flx_y_ccc + flx_y_cmc) * (v_ccc + v_cmc);
-  const double wvkp = 0.25 * (flx_z_ccc + flx_z_cpc) * (v_ccc + v_ccp);
-  const double wvkm = 0.25 * (flx_z_ccm + flx_z_cpm) * (v_ccc + v_ccm);
-  double dvdt_aux = dxi * (-uvip + uvim) + dyi * (-vvjp + vvjm) + dzfi_c * (-wvkp + wvkm);
-  const double uwip = 0.25 * (flx_x_ccc + flx_x_ccp) * (w_ccc + w_pcc);
-  const double uwim = 0.25 * (flx_x_mcc + flx_x_mcp) * (w_ccc + w_mcc);
-  const double vwjp = 0.25 * (flx_y_ccc + flx_y_ccp) * (w_ccc + w_cpc);
-  const double vwjm = 0.25 * (flx_y_cmc + flx_y_cmp) * (w_ccc + w_cmc);
-  const double wwkp = 0.25 * (flx_z_ccc + flx_z_ccp) * (w_ccc + w_ccp);
-  const double wwkm = 0.25 * (flx_z_ccc + flx_z_ccm) * (w_ccc + w_ccm);
-  double dwdt_aux = dxi * (-uwip + uwim) + dyi * (-vwjp + vwjm) + dzci_c * (-wwkp + wwkm);
-  {
+  double dudt_aux = 0.0, dvdt_aux = 0.0, dwdt_aux = 0.0;
+  if constexpr ((MASK & 1) != 0) {
+    const double uuip = 0.25 * (flx_x_pcc + flx_x_ccc) * (u_ccc + u_pcc);
+    const double uuim = 0.25 * (flx_x_mcc + flx_x_ccc) * (u_ccc + u_mcc);
+    const double vujp = 0.25 * (flx_y_pcc + flx_y_ccc) * (u_ccc + u_cpc);
+    const double vujm = 0.25 * (flx_y_pmc + flx_y_cmc) * (u_ccc + u_cmc);
+    const double wukp = 0.25 * (flx_z_pcc + flx_z_ccc) * (u_ccc + u_ccp);
+    const double wukm = 0.25 * (flx_z_pcm + flx_z_ccm) * (u_ccc + u_ccm);
+    dudt_aux = dxi * (-uuip + uuim) + dyi * (-vujp + vujm) + dzfi_c * (-wukp + wukm);
+  }
+  if constexpr ((MASK & 2) != 0) {
+    const double uvip = 0.25 * (flx_x_ccc + flx_x_cpc) * (v_ccc + v_pcc);
+    const double uvim = 0.25 * (flx_x_mcc + flx_x_mpc) * (v_ccc + v_mcc);
+    const double vvjp = 0.25 * (flx_y_ccc + flx_y_cpc) * (v_ccc + v_cpc);
+    const double vvjm = 0.25 * (flx_y_ccc + flx_y_cmc) * (v_ccc + v_cmc);
+    const double wvkp = 0.25 * (flx_z_ccc + flx_z_cpc) * (v_ccc + v_ccp);
+    const double wvkm = 0.25 * (flx_z_ccm + flx_z_cpm) * (v_ccc + v_ccm);
+    dvdt_aux = dxi * (-uvip + uvim) + dyi * (-vvjp + vvjm) + dzfi_c * (-wvkp + wvkm);
+  }
+  if constexpr ((MASK & 4) != 0) {
+    const double uwip = 0.25 * (flx_x_ccc + flx_x_ccp) * (w_ccc + w_pcc);
+    const double uwim = 0.25 * (flx_x_mcc + flx_x_mcp) * (w_ccc + w_mcc);
+    const double vwjp = 0.25 * (flx_y_ccc + flx_y_ccp) * (w_ccc + w_cpc);
+    const double vwjm = 0.25 * (flx_y_cmc + flx_y_cmp) * (w_ccc + w_cmc);
+    const double wwkp = 0.25 * (flx_z_ccc + flx_z_ccp) * (w_ccc + w_ccp);
+    const double wwkm = 0.25 * (flx_z_ccc + flx_z_ccm) * (w_ccc + w_ccm);
+    dwdt_aux = dxi * (-uwip + uwim) + dyi * (-vwjp + vwjm) + dzci_c * (-wwkp + wwkm);
+  }
+  if constexpr ((MASK & 1) != 0) {
     const double dudxp = (u_pcc - u_ccc) * dxi, dudxm = (u_ccc - u_mcc) * dxi;
     const double dvdxp = (v_pcc - v_ccc) * dxi, dvdxm = (v_pmc - v_cmc) * dxi;
     const double dudyp = (u_cpc - u_ccc) * dyi, dudym = (u_ccc - u_cmc) * dyi;
@@ -523,7 +581,7 @@ __device__ __forceinline__ void mom_ad_cell(const fz_dims &d, long q, int k, con
                dyi * ((dudyp + dvdxp) * muyp - (dudym + dvdxm) * muym) +
                dzfi_c * ((dudzp + dwdxp) * muzp - (dudzm + dwdxm) * muzm);
   }
-  {
+  if constexpr ((MASK & 2) != 0) {
     const double dvdxp = (v_pcc - v_ccc) * dxi, dvdxm = (v_ccc - v_mcc) * dxi;
     const double dudyp = (u_cpc - u_ccc) * dyi, dudym = (u_mpc - u_mcc) * dyi;
     const double dvdyp = (v_cpc - v_ccc) * dyi, dvdym = (v_ccc - v_cmc) * dyi;
@@ -538,7 +596,7 @@ __device__ __forceinline__ void mom_ad_cell(const fz_dims &d, long q, int k, con
                dyi * ((dvdyp + dvdyp) * muyp - (dvdym + dvdym) * muym) +
                dzfi_c * ((dvdzp + dwdyp) * muzp - (dvdzm + dwdym) * muzm);
   }
-  {
+  if constexpr ((MASK & 4) != 0) {
     const double dwdxp = (w_pcc - w_ccc) * dxi, dwdxm = (w_ccc - w_mcc) * dxi;
     const double dudzp = (u_ccp - u_ccc) * dzci_c, dudzm = (u_mcp - u_mcc) * dzci_c;
     const double dwdyp = (w_cpc - w_ccc) * dyi, dwdym = (w_ccc - w_cmc) * dyi;
@@ -569,7 +627,8 @@ __global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restr
                                                 double *__restrict__ dvdt, double *__restrict__ dwdt) {
   CELL_IJK(d);
   double ru, rv, rw;
-  mom_ad_cell(d, q, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, psiflx_x, psiflx_y, psiflx_z, u, v, w, psi, ru, rv,
+  mom_ad_cell(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, ptr_at(psiflx_x, d, q), ptr_at(psiflx_y, d, q),
+              ptr_at(psiflx_z, d, q), ptr_at(u, d, q), ptr_at(v, d, q), ptr_at(w, d, q), ptr_at(psi, d, q), ru, rv,
               rw);
   dudt[r] = ru;
   dvdt[r] = rv;
@@ -577,23 +636,21 @@ __global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restr
 }
 
 // mom_xyz_oth at one cell, src/mom.f90:912-1038 (constant-coefficient split, Brackbill)
-template <typename PT>
-__device__ __forceinline__ void mom_oth_cell(const fz_dims &d, long q, int k, const double *__restrict__ dzci,
-                                             double dxi, double dyi, double dt_r, double rho, double drho,
-                                             double surf_factor, double rho0, double bfx, double bfy, double bfz,
-                                             double gx, double gy, double gz, double rax, double ray, double raz,
-                                             const double *__restrict__ p, const PT *__restrict__ psi,
-                                             const PT *__restrict__ kappa, const double *__restrict__ pn,
-                                             const double *__restrict__ po, double &ru, double &rv, double &rw) {
-  const long sy = d.sy, sz = d.sz;
+template <class AD, class AP>
+__device__ __forceinline__ void mom_oth_cell(int k, const double *__restrict__ dzci, double dxi, double dyi,
+                                             double dt_r, double rho, double drho, double surf_factor, double rho0,
+                                             double bfx, double bfy, double bfz, double gx, double gy, double gz,
+                                             double rax, double ray, double raz, const AD &p, const AP &psi,
+                                             const AP &kappa, const AD &pn, const AD &po, double &ru, double &rv,
+                                             double &rw) {
   const double dzci_c = dzci[k];
-  const double p_ccc = p[q], p_pcc = p[q + 1], p_cpc = p[q + sy], p_ccp = p[q + sz];
-  const double q_ccc = (1.0 + dt_r) * pn[q] - dt_r * po[q];
-  const double q_pcc = (1.0 + dt_r) * pn[q + 1] - dt_r * po[q + 1];
-  const double q_cpc = (1.0 + dt_r) * pn[q + sy] - dt_r * po[q + sy];
-  const double q_ccp = (1.0 + dt_r) * pn[q + sz] - dt_r * po[q + sz];
-  const double k_ccc = kappa[q], k_pcc = kappa[q + 1], k_cpc = kappa[q + sy], k_ccp = kappa[q + sz];
-  const double c_ccc = psi[q], c_pcc = psi[q + 1], c_cpc = psi[q + sy], c_ccp = psi[q + sz];
+  const double p_ccc = p(0, 0, 0), p_pcc = p(1, 0, 0), p_cpc = p(0, 1, 0), p_ccp = p(0, 0, 1);
+  const double q_ccc = (1.0 + dt_r) * pn(0, 0, 0) - dt_r * po(0, 0, 0);
+  const double q_pcc = (1.0 + dt_r) * pn(1, 0, 0) - dt_r * po(1, 0, 0);
+  const double q_cpc = (1.0 + dt_r) * pn(0, 1, 0) - dt_r * po(0, 1, 0);
+  const double q_ccp = (1.0 + dt_r) * pn(0, 0, 1) - dt_r * po(0, 0, 1);
+  const double k_ccc = kappa(0, 0, 0), k_pcc = kappa(1, 0, 0), k_cpc = kappa(0, 1, 0), k_ccp = kappa(0, 0, 1);
+  const double c_ccc = psi(0, 0, 0), c_pcc = psi(1, 0, 0), c_cpc = psi(0, 1, 0), c_ccp = psi(0, 0, 1);
   const double psixp = 0.5 * (c_pcc + c_ccc), psiyp = 0.5 * (c_cpc + c_ccc), psizp = 0.5 * (c_ccp + c_ccc);
   const double rhoxp = rho + drho * psixp, rhoyp = rho + drho * psiyp, rhozp = rho + drho * psizp;
   const double dpdx = (p_pcc - p_ccc) * dxi, dpdy = (p_cpc - p_ccc) * dyi, dpdz = (p_ccp - p_ccc) * dzci_c;
@@ -629,8 +686,9 @@ __global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__rest
                                                  double *__restrict__ dvdt, double *__restrict__ dwdt) {
   CELL_IJK(d);
   double ru, rv, rw;
-  mom_oth_cell(d, q, k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz,
-               p, psi, kappa, pn, po, ru, rv, rw);
+  mom_oth_cell(k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz,
+               ptr_at(p, d, q), ptr_at(psi, d, q), ptr_at(kappa, d, q), ptr_at(pn, d, q), ptr_at(po, d, q), ru, rv,
+               rw);
   dudt[r] = ru;
   dvdt[r] = rv;
   dwdt[r] = rw;
@@ -646,8 +704,11 @@ __global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__rest
 // advection-diffusion term in place (on the first step the history is the
 // term itself, src/rk.f90:76-83).
 // the body, shared by occupancy variants of the kernel
-// (PT: phase fields, HT: the AB2 history; both FP32 in the mixed-precision step)
-template <typename PT, typename HT>
+// (PT: phase fields, HT: the AB2 history; both FP32 in the mixed-precision
+// step. STAGED: the three advection-diffusion components one after another,
+// each with its own loads, behind scheduling barriers, so fewer values are
+// live at once; the arithmetic is the same.)
+template <typename PT, typename HT, bool STAGED>
 __device__ __forceinline__ void mom_fused_body(
     fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
     double drho, double mu, double dmu, double factor1, double factor2, int first, double factor12, double dt_r,
@@ -659,31 +720,47 @@ __device__ __forceinline__ void mom_fused_body(
     HT *__restrict__ duo, HT *__restrict__ dvo, HT *__restrict__ dwo, double *__restrict__ un,
     double *__restrict__ vn, double *__restrict__ wn) {
   CELL_IJK(d);
-  const long sy = d.sy, sz = d.sz;
+  const BufAt<double> U = buf_at(u, d, q), V = buf_at(v, d, q), W = buf_at(w, d, q);
+  const BufAt<PT> PSI = buf_at(psi, d, q), FX = buf_at(psiflx_x, d, q), FY = buf_at(psiflx_y, d, q),
+                  FZ = buf_at(psiflx_z, d, q);
   double ru, rv, rw;
-  mom_ad_cell(d, q, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, psiflx_x, psiflx_y, psiflx_z, u, v, w, psi, ru, rv,
-              rw);
+  if constexpr (STAGED) {
+    double t0, t1;
+    mom_ad_cell<1>(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, ru, t0, t1);
+    asm volatile("" ::: "memory");  // each stage loads its own values
+    __builtin_amdgcn_sched_barrier(0);
+    mom_ad_cell<2>(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, t0, rv, t1);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    mom_ad_cell<4>(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, t0, t1, rw);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+  } else {
+    mom_ad_cell(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, ru, rv, rw);
+  }
   const double ruo = first ? ru : duo[r], rvo = first ? rv : dvo[r], rwo = first ? rw : dwo[r];
   duo[r] = ru;
   dvo[r] = rv;
   dwo[r] = rw;
-  const double rhox_n = rho + drho * 0.5 * (psio[q] + psio[q + 1]);
-  const double rhoy_n = rho + drho * 0.5 * (psio[q] + psio[q + sy]);
-  const double rhoz_n = rho + drho * 0.5 * (psio[q] + psio[q + sz]);
-  const double rhox_p = rho + drho * 0.5 * (psi[q] + psi[q + 1]);
-  const double rhoy_p = rho + drho * 0.5 * (psi[q] + psi[q + sy]);
-  const double rhoz_p = rho + drho * 0.5 * (psi[q] + psi[q + sz]);
-  double uu = u[q] * rhox_n / rhox_p + (factor1 * ru + factor2 * ruo) / rhox_p;
-  double vv = v[q] * rhoy_n / rhoy_p + (factor1 * rv + factor2 * rvo) / rhoy_p;
-  double ww = w[q] * rhoz_n / rhoz_p + (factor1 * rw + factor2 * rwo) / rhoz_p;
-  mom_oth_cell(d, q, k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz,
-               p, psi, kappa, p, po, ru, rv, rw);
+  const BufAt<PT> PSIO = buf_at(psio, d, q), KAP = buf_at(kappa, d, q);
+  const BufAt<double> P = buf_at(p, d, q), PO = buf_at(po, d, q);
+  const double rhox_n = rho + drho * 0.5 * ((double)PSIO(0, 0, 0) + (double)PSIO(1, 0, 0));
+  const double rhoy_n = rho + drho * 0.5 * ((double)PSIO(0, 0, 0) + (double)PSIO(0, 1, 0));
+  const double rhoz_n = rho + drho * 0.5 * ((double)PSIO(0, 0, 0) + (double)PSIO(0, 0, 1));
+  const double rhox_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(1, 0, 0));
+  const double rhoy_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(0, 1, 0));
+  const double rhoz_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(0, 0, 1));
+  double uu = U(0, 0, 0) * rhox_n / rhox_p + (factor1 * ru + factor2 * ruo) / rhox_p;
+  double vv = V(0, 0, 0) * rhoy_n / rhoy_p + (factor1 * rv + factor2 * rvo) / rhoy_p;
+  double ww = W(0, 0, 0) * rhoz_n / rhoz_p + (factor1 * rw + factor2 * rwo) / rhoz_p;
+  mom_oth_cell(k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz, P, PSI,
+               KAP, P, PO, ru, rv, rw);
   un[q] = uu + factor12 * ru / rhox_p;
   vn[q] = vv + factor12 * rv / rhoy_p;
   wn[q] = ww + factor12 * rw / rhoz_p;
 }
 
-#define FZ_MOM_FUSED_KERNEL(NAME, ATTR)                                                                          \
+#define FZ_MOM_FUSED_KERNEL(NAME, ATTR, STG)                                                                     \
   template <typename PT, typename HT>                                                                            \
   __global__ void __launch_bounds__(256) ATTR NAME(                                                              \
       fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi,       \
@@ -698,13 +775,15 @@ __device__ __forceinline__ void mom_fused_body(
       double *__restrict__ un, double *__restrict__ vn, double *__restrict__ wn) {                               \
     const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;      \
     const double factor12 = stc ? stc[FZ_STC_F12] : factor12_v, dt_r = stc ? stc[FZ_STC_DTR] : dt_r_v;         \
-    mom_fused_body<PT, HT>(d, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, factor1, factor2, first, factor12, dt_r, \
+    mom_fused_body<PT, HT, STG>(d, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, factor1, factor2, first, factor12,    \
+                                dt_r,                                                                            \
                            surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz, psiflx_x, psiflx_y,      \
                            psiflx_z, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);               \
   }
-FZ_MOM_FUSED_KERNEL(k_mom_fused, )
-FZ_MOM_FUSED_KERNEL(k_mom_fused_w4, __attribute__((amdgpu_waves_per_eu(4))))
-FZ_MOM_FUSED_KERNEL(k_mom_fused_w5, __attribute__((amdgpu_waves_per_eu(5))))
+FZ_MOM_FUSED_KERNEL(k_mom_fused, , false)
+FZ_MOM_FUSED_KERNEL(k_mom_fused_st, __attribute__((amdgpu_waves_per_eu(4))), true)
+FZ_MOM_FUSED_KERNEL(k_mom_fused_w4, __attribute__((amdgpu_waves_per_eu(4))), false)
+FZ_MOM_FUSED_KERNEL(k_mom_fused_w5, __attribute__((amdgpu_waves_per_eu(5))), false)
 
 template <typename PT, typename HT>
 int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], double factor1, double factor2,
@@ -714,9 +793,13 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
                         const double *u, const double *v, const double *w, const PT *psi, const PT *psio,
                         const double *p, const double *po, const PT *kappa, HT *duo, HT *dvo, HT *dwo, double *un,
                         double *vn, double *wn) {
+  FZ_TRY(fits_buffers(c, "mom_fused"));
   const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
-  static const int variant = getenv("FZ_MOM_WPE") ? atoi(getenv("FZ_MOM_WPE")) : 0;  // occupancy tuning
-  auto kern = variant == 4 ? k_mom_fused_w4<PT, HT> : (variant == 5 ? k_mom_fused_w5<PT, HT> : k_mom_fused<PT, HT>);
+  const int variant = c->mom_variant;  // occupancy variant (the context's tuned choice)
+  auto kern = variant == 4   ? k_mom_fused_w4<PT, HT>
+              : variant == 5 ? k_mom_fused_w5<PT, HT>
+              : variant == 1 ? k_mom_fused_st<PT, HT>
+                             : k_mom_fused<PT, HT>;
   FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", kern, fz_grid_interior(c->d), fz_block3(), 0, c->d,
             c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1], rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1,
             factor2, first, factor12, dt_r, stc, surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1],
@@ -934,6 +1017,7 @@ int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi) {
 }
 
 int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float *normz, float *kappa) {
+  FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
   FZ_LAUNCH(c, "youngs_f32", k_youngs<float>, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi,
             c->dli[0], c->dli[1], psi, normx, normy, normz, kappa);
   FZ_LAUNCHED();
@@ -976,6 +1060,7 @@ extern "C" int fz_cmpt_norm_curv_youngs(fz_ctx *c, const double *psi, double *no
                                         double *kappa) {
   if (!c || !psi || !normx || !normy || !normz || !kappa)
     return fz_set_error(FZ_ERR_ARG, "fz_cmpt_norm_curv_youngs: null argument");
+  FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
   FZ_LAUNCH(c, "youngs", k_youngs<double>, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
                      c->dli[1], psi, normx, normy, normz, kappa);
   FZ_LAUNCHED();

@@ -222,6 +222,15 @@ class DeviceCase:
         n = self.n
         return (n[0] + 2, n[1] + 2, n[2] + 2) if fid < 15 else (n[0], n[1], n[2])
 
+    def autotune(self, trial_steps=3):
+        """fz_case_autotune: launch parameters picked by timing; returns the report."""
+        buf = C.create_string_buffer(256)
+        call("fz_case_autotune", self._h, int(trial_steps), buf, len(buf))
+        return buf.value.decode()
+
+    def set_graphs(self, on=True):
+        call("fz_case_set_graphs", self._h, 1 if on else 0)
+
     def precision(self):
         """('fp64' | 'mixed', refinements)."""
         pr, nr = C.c_int(), C.c_int()

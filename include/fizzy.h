@@ -250,6 +250,13 @@ int fz_case_step(fz_case *cs, int nsteps, int *killed);
  * steps with per-launch profiling on, run as before. Results are the same
  * bit for bit. No reference counterpart. */
 int fz_case_set_graphs(fz_case *cs, int on);
+/* pick the case's launch parameters (stencil block order, momentum-kernel
+ * occupancy, Thomas rows per chunk) by timing trial_steps steps per
+ * candidate on a scratch case of the same parameters; applies the choice to
+ * the case and caches it per shape for later cases. Results are the same
+ * bit for bit under every choice. report (if not null) receives the choice
+ * and its time. No reference counterpart. */
+int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size_t cap);
 /* the checkpoint of src/main.f90:597-631: prefix_u.bin, _v, _w, _p, _psi
  * in the load_one format, with the case's time and step */
 int fz_case_save(fz_case *cs, const char *prefix);
