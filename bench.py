@@ -50,7 +50,8 @@ HALO_KERNELS = {"acdi_phi", "clip", "updatep_oop", "vof_phi", "correc", "correc_
 STEP_ALG_BYTES_PER_CELL = 488
 
 
-WORKLOAD_NAMES = {"c2": "acdi_rising_bubble_256x256x512", "c0": "acdi_rising_bubble_64x64x128"}
+WORKLOAD_NAMES = {"c2": "acdi_rising_bubble_256x256x512", "c0": "acdi_rising_bubble_64x64x128",
+                  "c3": "vof_droplets_forced_256x256x256"}
 
 
 def profiled_traffic(kernel, workload):
@@ -299,9 +300,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["c2", "c0", "c1", "c4"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c0", "c1", "c3", "c4"], default="c2",
                     help="c2: 256x256x512 bubble (default); c0: 64x64x128; c1: the standalone 512^3 Poisson "
-                         "solve; c4: the serving mix")
+                         "solve; c3: THINC/QQ droplets in forced turbulence, 256^3; c4: the serving mix")
     ap.add_argument("--requests", type=int, default=24, help="c4: number of requests")
     ap.add_argument("--case-steps", type=int, default=200, help="c4: steps per request")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -326,7 +327,7 @@ def main():
     import fizzy_amd as F
     from fizzy_amd import cases
 
-    spec = cases.config2() if args.workload == "c2" else cases.config0()
+    spec = {"c2": cases.config2, "c0": cases.config0, "c3": cases.config3}[args.workload]()
     n = spec["ng"]
     cells = n[0] * n[1] * n[2]
     log(f"workload {args.workload}: {n[0]}x{n[1]}x{n[2]} = {cells} cells, warmup {args.warmup}, steps {args.steps}")
@@ -396,8 +397,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: the reference's rising-bubble initial condition (sphere, fluid at rest)",
-        "config": {"workload": WORKLOAD_NAMES[args.workload],
+        "data": ("synthetic: seeded droplets (SplitMix64 centres), fluid at rest, Taylor-Green forcing"
+                 if spec.get("vof") else
+                 "synthetic: the reference's rising-bubble initial condition (sphere, fluid at rest)"),
+        "config": {"workload": WORKLOAD_NAMES[args.workload], "interface": "THINC/QQ" if spec.get("vof") else "ACDI",
                    "grid": n, "density_ratio": spec["rho12"][0] / spec["rho12"][1],
                    "viscosity_ratio": spec["mu12"][0] / spec["mu12"][1], "precision": "fp64",
                    "step": "AB2 (one reference time step)", "launch_parameters": tuning},

@@ -48,20 +48,26 @@ double fz_chkdt_finish(const fz_ctx *c, double dti, int is_solve_ns, int is_trac
 template <typename PT>
 int fz_bulk_mean_dev(fz_ctx *c, int idir, const PT *psi, const double p12[2], const double *gvr, double *d_out);
 int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi);
+int fz_vof_transport_f32(fz_ctx *c, double beta, const double *u, const double *v, const double *w,
+                         const float *normx, const float *normy, const float *normz, const float *phi, float *dpsidt,
+                         float *flux_x, float *flux_y, float *flux_z);
+int fz_vof_phi_f32(fz_ctx *c, double beta, const float *psi, float *phi);
+int fz_clip_f32(fz_ctx *c, double rmin, double rmax, float *p);
 int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float *normz, float *kappa);
 int fz_boundp_f32(fz_ctx *c, const char cbc[6], const double bc[6], float *p);
 int fz_bounduvw_f32(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, float *u, float *v, float *w);
 
 // out-of-place AB2 phase-field update (psi_out may alias psi_in)
-__global__ void k_rk_psi_oop(fz_dims d, double f1, double f2, const double *__restrict__ dpsidt,
-                             const double *__restrict__ dpsidto, const double *psi_in, double *psi_out) {
+template <typename T>
+__global__ void k_rk_psi_oop(fz_dims d, double f1, double f2, const T *__restrict__ dpsidt,
+                             const T *__restrict__ dpsidto, const T *psi_in, T *psi_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   const int k = blockIdx.z + 1;
   if (i > d.n1 || j > d.n2) return;
   const long q = i + d.sy * j + d.sz * k;
   const long r = (i - 1) + (long)d.n1 * ((j - 1) + (long)d.n2 * (k - 1));
-  psi_out[q] = psi_in[q] + f1 * dpsidt[r] + f2 * dpsidto[r];
+  psi_out[q] = psi_in[q] + (T)f1 * dpsidt[r] + (T)f2 * dpsidto[r];
 }
 
 enum { F_U, F_V, F_W, F_P, F_PN, F_PO, F_PSI, F_PHI, F_KAPPA, F_NX, F_NY, F_NZ, F_FX, F_FY, F_FZ, F_DPSIO, F_DUO,
@@ -87,7 +93,7 @@ struct fz_case {
   int precision = FZ_PREC_FP64;
   int nref = 1;
   float *g[F_N] = {nullptr};
-  float *psio32 = nullptr;
+  float *psio32 = nullptr, *dpsidt32 = nullptr;
   // per-step scalars in device memory and the captured step graphs
   double *dstc = nullptr;
   double stc_host[FZ_STC_N] = {0};
@@ -144,6 +150,10 @@ static int case_alloc(fz_case *cs) {
   if (mixed) {
     FZ_HIP(hipMalloc((void **)&cs->psio32, sizeof(float) * c->nh));
     FZ_HIP(hipMemsetAsync(cs->psio32, 0, sizeof(float) * c->nh, c->st));
+    if (cs->prm.vof) {
+      FZ_HIP(hipMalloc((void **)&cs->dpsidt32, sizeof(float) * c->ni));
+      FZ_HIP(hipMemsetAsync(cs->dpsidt32, 0, sizeof(float) * c->ni, c->st));
+    }
   } else {
     FZ_HIP(hipMalloc((void **)&cs->dpsidt, sizeof(double) * c->ni));
     FZ_HIP(hipMemsetAsync(cs->dpsidt, 0, sizeof(double) * c->ni, c->st));
@@ -175,6 +185,7 @@ extern "C" int fz_case_destroy(fz_case *cs) {
     if (cs->g[q]) (void)hipFree(cs->g[q]);
   }
   if (cs->psio32) (void)hipFree(cs->psio32);
+  if (cs->dpsidt32) (void)hipFree(cs->dpsidt32);
   if (cs->dstc) (void)hipFree(cs->dstc);
   for (int q = 0; q < 2; q++)
     if (cs->graph[q].exec) (void)hipGraphExecDestroy(cs->graph[q].exec);
@@ -197,10 +208,9 @@ extern "C" int fz_case_create_ex(fz_case **out, const fz_params *prm, void *hip_
     return fz_set_error(FZ_ERR_ARG, "fz_case_create: unknown precision %d", precision);
   if (precision == FZ_PREC_MIXED) {
     if (nref < 0 || nref > 8) return fz_set_error(FZ_ERR_ARG, "fz_case_create: refinements %d not in [0, 8]", nref);
-    if (prm->vof || !prm->is_track_interface || !prm->is_solve_ns)
+    if (!prm->is_track_interface || !prm->is_solve_ns)
       return fz_set_error(FZ_ERR_UNSUPPORTED,
-                          "mixed precision is implemented for the ACDI two-fluid step (vof = F, interface and "
-                          "Navier-Stokes on)");
+                          "mixed precision is implemented for the two-fluid step (interface and Navier-Stokes on)");
   }
   const char *codes[4] = {prm->cbcvel, prm->cbcpre, prm->cbcpsi, prm->cbcnor};
   const int lens[4] = {18, 6, 6, 18};
@@ -300,7 +310,8 @@ static int phase_geometry(fz_case *cs) {
   double **f = cs->f;
   if (cs->precision == FZ_PREC_MIXED) {
     float **g = cs->g;
-    FZ_TRY(fz_acdi_cmpt_phi_f32(c, cs->sc[S_SEPS], g[F_PSI], g[F_PHI]));
+    if (P.vof) FZ_TRY(fz_vof_phi_f32(c, 1.0 / (2.0 * P.psi_thickness_factor), g[F_PSI], g[F_PHI]));
+    else FZ_TRY(fz_acdi_cmpt_phi_f32(c, cs->sc[S_SEPS], g[F_PSI], g[F_PHI]));
     FZ_TRY(fz_youngs_f32(c, g[F_PHI], g[F_NX], g[F_NY], g[F_NZ], g[F_KAPPA]));
     FZ_TRY(fz_boundp_f32(c, cs->cbccur, cs->bccur, g[F_KAPPA]));
     FZ_TRY(fz_boundp_f32(c, P.cbcnor + 0, P.bcnor + 0, g[F_NX]));
@@ -455,7 +466,20 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
   // psio <- psi by rotation: the update writes into the older buffer
   swap_ptr(cs->psio, f[F_PSI]);
   swap_ptr(cs->psio32, g[F_PSI]);
-  if (mixed) {  // the same sequence on the FP32 phase field
+  if (mixed && P.vof) {  // the VOF sequence below on the FP32 phase field
+    FZ_TRY(fz_vof_transport_f32(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_U], f[F_V], f[F_W], g[F_NX], g[F_NY],
+                                g[F_NZ], g[F_PHI], cs->dpsidt32, g[F_FX], g[F_FY], g[F_FZ]));
+    if (is_first)
+      FZ_HIP(hipMemcpyAsync(g[F_DPSIO], cs->dpsidt32, sizeof(float) * c->ni, hipMemcpyDeviceToDevice, c->st));
+    FZ_LAUNCH(c, "rk_psi_oop_f32", k_rk_psi_oop<float>, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1,
+              factor2, cs->dpsidt32, g[F_DPSIO], cs->psio32, g[F_PSI]);
+    FZ_LAUNCHED();
+    FZ_TRY(fz_clip_f32(c, 0.0, 1.0, g[F_PSI]));
+    swap_ptr(cs->dpsidt32, g[F_DPSIO]);
+    FZ_TRY(fz_bounduvw_f32(c, P.cbcvel, P.bcvel, 0, g[F_FX], g[F_FY], g[F_FZ]));
+    FZ_TRY(fz_boundp_f32(c, P.cbcpsi, P.bcpsi, g[F_PSI]));
+    FZ_TRY(phase_geometry(cs));
+  } else if (mixed) {  // the same sequence on the FP32 phase field
     FZ_TRY(fz_launch_acdi_step<float>(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS],
                                       factor1, factor2, a.stc, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], g[F_NX], g[F_NY],
                                       g[F_NZ], g[F_PHI], cs->psio32, g[F_DPSIO], g[F_PSI], g[F_FX], g[F_FY],
@@ -474,7 +498,7 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
                                         f[F_NY], f[F_NZ], f[F_PHI], cs->psio, cs->dpsidt, f[F_FX], f[F_FY],
                                         f[F_FZ]));
       if (is_first) FZ_HIP(hipMemcpyAsync(f[F_DPSIO], cs->dpsidt, bi, hipMemcpyDeviceToDevice, c->st));
-      FZ_LAUNCH(c, "rk_psi_oop", k_rk_psi_oop, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1, factor2,
+      FZ_LAUNCH(c, "rk_psi_oop", k_rk_psi_oop<double>, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1, factor2,
                 cs->dpsidt, f[F_DPSIO], cs->psio, f[F_PSI]);
       FZ_LAUNCHED();
       const double mm[2] = {0.0, 1.0};

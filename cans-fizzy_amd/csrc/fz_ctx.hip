@@ -74,6 +74,7 @@ extern "C" int fz_ctx_create(fz_ctx **out, const int n[3], const double l[3], co
   c->d.tile_y = 8;
   if (const char *e = getenv("FZ_TILE_Y")) c->d.tile_y = atoi(e);  // block-order tuning
   if (const char *e = getenv("FZ_MOM_WPE")) c->mom_variant = atoi(e);
+  if (const char *e = getenv("FZ_THOMAS_ROWS32")) c->tk32 = atoi(e);
   c->nh = c->d.sz * (n[2] + 2);
   c->ni = (long)n[0] * n[1] * n[2];
   for (int d = 0; d < 3; d++) {

@@ -94,7 +94,7 @@ struct fz_ctx {
   // launch parameters (defaults, environment overrides, or fz_case_autotune)
   int mom_variant = 0;  // k_mom_fused variant: 0, 1 (staged loads), 4 or 5 (waves per EU)
   int tk64 = 16;        // rows per chunk of the FP64 Thomas kernel: 8, 16 or 32
-  int tk32 = 32;        // the same for the FP32 one
+  int tk32 = 32;        // the same for the FP32 one; negative: two columns per lane, -8 or -16
 };
 
 // brackets one launch with events when profiling is on

@@ -356,25 +356,30 @@ __global__ void __launch_bounds__(256) k_r2r(const TI *__restrict__ src, LineMap
 // in registers and the passes exchange through one LDS buffer, with all
 // index arithmetic resolved at compile time. G real lines per block, M/4
 // threads per complex line.
-template <int M, bool QUARTER>
+// (R: the arithmetic type, which sets the LDS bytes. FP32 blocks keep the
+// FP64 line counts: twice the lines per block, at 1024 threads, measured
+// slower for both the x and the y transforms.)
+template <int M, bool QUARTER, typename R = double>
 struct P2Shape {
   static constexpr int LG = __builtin_ctz(M);
   static constexpr int NP4 = LG / 2;           // radix-4 passes
   static constexpr bool HAS2 = (LG & 1) != 0;  // closing radix-2 pass
   static constexpr int N = QUARTER ? M / 2 : M;  // real line length
-  static constexpr int G = QUARTER ? (2048 / M < 64 ? 2048 / M : 64) : (4096 / M < 64 ? (4096 / M > 2 ? 4096 / M : 2) : 64);
+  static constexpr int G64 = QUARTER ? (2048 / M < 64 ? 2048 / M : 64) : (4096 / M < 64 ? (4096 / M > 2 ? 4096 / M : 2) : 64);
+  static constexpr int G = G64;
   static constexpr int NFFT = QUARTER ? G : G / 2;
   static constexpr int NT = NFFT * (M / 4);
-  static constexpr size_t LDS = (size_t)NFFT * M * sizeof(double2);
+  static constexpr size_t LDS = (size_t)NFFT * M * sizeof(typename CT<R>::C);
 };
 
 template <int M, bool QUARTER, typename R, typename TI, typename TO>
-__global__ void __launch_bounds__(512) k_r2r_p2(const TI *__restrict__ src, LineMap in, TO *__restrict__ dst,
+__global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
+    k_r2r_p2(const TI *__restrict__ src, LineMap in, TO *__restrict__ dst,
                                                 LineMap out, int L1, int kind, double scale, int accum,
                                                 const typename CT<R>::C *__restrict__ tw,
                                                 const typename CT<R>::C *__restrict__ q,
                                                 const typename CT<R>::C *__restrict__ t2, FillSrc fs) {
-  using S = P2Shape<M, QUARTER>;
+  using S = P2Shape<M, QUARTER, R>;
   using C = typename CT<R>::C;
   constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4;
   extern __shared__ double2 smem_raw[];
@@ -500,7 +505,7 @@ static PlanTabs<float> tabs_of(const fz_fftplan &P, float) { return {P.tw32, P.q
 template <int M, bool QUARTER, typename R, typename TI, typename TO>
 static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
                       TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs) {
-  using S = P2Shape<M, QUARTER>;
+  using S = P2Shape<M, QUARTER, R>;
   dim3 grid((L1 + S::G - 1) / S::G, L2, 1);
   const PlanTabs<R> t = tabs_of(P, R());
   FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind,
@@ -615,6 +620,78 @@ __global__ void __launch_bounds__(64) k_thomas(T *__restrict__ W, double *__rest
       if (h0 - t >= 0) {
         pnext = r[t] - dd[t] * pnext;
         col[(long)(h0 - t) * ps] = (T)pnext;
+      }
+  }
+}
+
+// k_thomas for FP32 columns, two adjacent columns per lane: each load moves
+// 8 bytes of W and 16 of pivots per lane, as the FP64 kernel does with one
+// column. The per-column arithmetic is k_thomas<TK, float>'s exactly.
+// Needs an even column count n1*n2.
+template <int TK>
+__global__ void __launch_bounds__(64) k_thomas_x2(float *__restrict__ W, double *__restrict__ D,
+                                                  const double *__restrict__ lambdaxy, const double *__restrict__ a,
+                                                  const double *__restrict__ b, const double *__restrict__ c, int n1,
+                                                  int n2, int n3, long gauge) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;  // column pair
+  const long ps2 = (long)n1 * n2 / 2;
+  if (idx >= ps2) return;
+  float2 *col = (float2 *)W + idx;
+  double2 *dcol = (double2 *)D + idx;
+  const double lam0 = lambdaxy[2 * idx], lam1 = lambdaxy[2 * idx + 1];
+  const bool g0 = 2 * idx == gauge, g1 = 2 * idx + 1 == gauge;
+  double d0 = 0.0, d1 = 0.0, p0 = 0.0, p1 = 0.0;
+  for (int l0 = 0; l0 < n3; l0 += TK) {
+    float2 r[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < n3) r[t] = col[(long)(l0 + t) * ps2];
+#pragma unroll
+    for (int t = 0; t < TK; t++) {
+      const int l = l0 + t;
+      if (l < n3) {
+        const double bb0 = b[l] + lam0, bb1 = b[l] + lam1;
+        double z0, z1;
+        if (l == 0) {
+          z0 = 1.0 / (bb0 + FZ_EPS);
+          z1 = 1.0 / (bb1 + FZ_EPS);
+          d0 = c[0] * z0;
+          d1 = c[0] * z1;
+          p0 = (double)r[t].x * z0;
+          p1 = (double)r[t].y * z1;
+        } else {
+          z0 = 1.0 / (bb0 - a[l] * d0 + FZ_EPS);
+          z1 = 1.0 / (bb1 - a[l] * d1 + FZ_EPS);
+          d0 = c[l] * z0;
+          d1 = c[l] * z1;
+          p0 = ((double)r[t].x - a[l] * p0) * z0;
+          p1 = ((double)r[t].y - a[l] * p1) * z1;
+        }
+        if (l == n3 - 1) {
+          if (g0) p0 = 0.0;
+          if (g1) p1 = 0.0;
+        }
+        dcol[(long)l * ps2] = make_double2(d0, d1);
+        col[(long)l * ps2] = make_float2((float)p0, (float)p1);
+      }
+    }
+  }
+  double q0 = p0, q1 = p1;
+  for (int h0 = n3 - 2; h0 >= 0; h0 -= TK) {
+    float2 r[TK];
+    double2 dd[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        r[t] = col[(long)(h0 - t) * ps2];
+        dd[t] = dcol[(long)(h0 - t) * ps2];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        q0 = (double)r[t].x - dd[t].x * q0;
+        q1 = (double)r[t].y - dd[t].y * q1;
+        col[(long)(h0 - t) * ps2] = make_float2((float)q0, (float)q1);
       }
   }
 }
@@ -1015,7 +1092,14 @@ static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum) {
     const int tk = mixed ? ctx->tk32 : ctx->tk64;
     const dim3 grid((int)((ps + 63) / 64)), blk(64);
     const char *name = mixed ? "thomas_f32" : "thomas";
-    if (tk == 8) FZ_LAUNCH(ctx, name, (k_thomas<8, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
+    if (mixed && tk < 0 && ps % 2 == 0) {  // two FP32 columns per lane (tk = -rows per chunk)
+      const dim3 g2((int)((ps / 2 + 63) / 64));
+      float *W32 = (float *)(void *)W;
+      if (tk == -8) FZ_LAUNCH(ctx, name, k_thomas_x2<8>, g2, blk, 0, W32, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
+                              n3, gauge);
+      else FZ_LAUNCH(ctx, name, k_thomas_x2<16>, g2, blk, 0, W32, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
+                     gauge);
+    } else if (tk == 8) FZ_LAUNCH(ctx, name, (k_thomas<8, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
                            n3, gauge);
     else if (tk == 32) FZ_LAUNCH(ctx, name, (k_thomas<32, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c,
                                  n1, n2, n3, gauge);

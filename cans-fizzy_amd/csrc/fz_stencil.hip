@@ -373,52 +373,67 @@ __global__ void k_acdi_phi(long nh, double seps_d, const T *__restrict__ psi, T 
     phi[q] = seps * fz_log((psi[q] + (T)FZ_EPS) / ((T)1.0 - psi[q] + (T)FZ_EPS));
 }
 
-// vof_thinc_cmpt_phi, src/vof_thinc_qq.f90:148-156
-__global__ void k_vof_phi(long nh, double beta, const double *__restrict__ psi, double *__restrict__ phi) {
+// epsilon(1._rp) of the phase-field type (the reference's single-precision
+// build uses epsilon(1._sp) where its double build uses epsilon(1._dp))
+template <typename T>
+__device__ __forceinline__ T fz_eps_of() {
+  return sizeof(T) == 4 ? (T)1.1920928955078125e-07 : (T)FZ_EPS;
+}
+__device__ inline double fz_exp(double x) { return fzm::exp(x); }
+__device__ inline float fz_exp(float x) { return expf(x); }
+
+// vof_thinc_cmpt_phi, src/vof_thinc_qq.f90:148-156 (T = float: the mixed step)
+template <typename T>
+__global__ void k_vof_phi(long nh, double beta_d, const T *__restrict__ psi, T *__restrict__ phi) {
+  const T beta = (T)beta_d, eps = fz_eps_of<T>();
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x) {
-    const double pa = fmin(fmax(psi[q], FZ_EPS), 1.0 - FZ_EPS);
-    phi[q] = 0.5 / beta * fzm::log(pa / (1.0 - pa));
+    const T pa = fmin(fmax(psi[q], eps), (T)1.0 - eps);
+    phi[q] = (T)0.5 / beta * fz_log(pa / ((T)1.0 - pa));
   }
 }
 
 // vof_thinc_transport_psi fluxes, src/vof_thinc_qq.f90:66-121 (faces 0..n)
-__global__ void k_vof_flux(fz_dims d, double beta, const double *__restrict__ u, const double *__restrict__ v,
-                           const double *__restrict__ w, const double *__restrict__ normx,
-                           const double *__restrict__ normy, const double *__restrict__ normz,
-                           const double *__restrict__ phi, double *__restrict__ flux_x, double *__restrict__ flux_y,
-                           double *__restrict__ flux_z) {
+template <typename T>
+__global__ void k_vof_flux(fz_dims d, double beta_d, const double *__restrict__ u, const double *__restrict__ v,
+                           const double *__restrict__ w, const T *__restrict__ normx, const T *__restrict__ normy,
+                           const T *__restrict__ normz, const T *__restrict__ phi, T *__restrict__ flux_x,
+                           T *__restrict__ flux_y, T *__restrict__ flux_z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
   if (i > d.n1 || j > d.n2) return;
+  const T beta = (T)beta_d;
   const long q = i + d.sy * j + d.sz * k;
-  const double vel_x = u[q], vel_y = v[q], vel_z = w[q];
+  const T vel_x = (T)u[q], vel_y = (T)v[q], vel_z = (T)w[q];
   long qx, qy, qz;
-  double xx, yy, zz;
-  if (vel_x > 0.0) { qx = q; xx = 0.5; } else { qx = q + 1; xx = -0.5; }
-  if (vel_y > 0.0) { qy = q; yy = 0.5; } else { qy = q + d.sy; yy = -0.5; }
-  if (vel_z > 0.0) { qz = q; zz = 0.5; } else { qz = q + d.sz; zz = -0.5; }
-  const double lfx = 1.0 / (1.0 + fzm::exp(-2.0 * beta * (normx[qx] * xx + phi[qx])));
-  const double lfy = 1.0 / (1.0 + fzm::exp(-2.0 * beta * (normy[qy] * yy + phi[qy])));
-  const double lfz = 1.0 / (1.0 + fzm::exp(-2.0 * beta * (normz[qz] * zz + phi[qz])));
+  T xx, yy, zz;
+  if (vel_x > (T)0.0) { qx = q; xx = (T)0.5; } else { qx = q + 1; xx = (T)-0.5; }
+  if (vel_y > (T)0.0) { qy = q; yy = (T)0.5; } else { qy = q + d.sy; yy = (T)-0.5; }
+  if (vel_z > (T)0.0) { qz = q; zz = (T)0.5; } else { qz = q + d.sz; zz = (T)-0.5; }
+  const T lfx = (T)1.0 / ((T)1.0 + fz_exp((T)-2.0 * beta * (normx[qx] * xx + phi[qx])));
+  const T lfy = (T)1.0 / ((T)1.0 + fz_exp((T)-2.0 * beta * (normy[qy] * yy + phi[qy])));
+  const T lfz = (T)1.0 / ((T)1.0 + fz_exp((T)-2.0 * beta * (normz[qz] * zz + phi[qz])));
   flux_x[q] = vel_x * lfx;
   flux_y[q] = vel_y * lfy;
   flux_z[q] = vel_z * lfz;
 }
 
 // flux divergence, src/vof_thinc_qq.f90:123-132
-__global__ void k_vof_div(fz_dims d, const double *__restrict__ dzfi, double dxi, double dyi,
-                          const double *__restrict__ flux_x, const double *__restrict__ flux_y,
-                          const double *__restrict__ flux_z, double *__restrict__ dpsidt) {
+template <typename T>
+__global__ void k_vof_div(fz_dims d, const double *__restrict__ dzfi, double dxi_d, double dyi_d,
+                          const T *__restrict__ flux_x, const T *__restrict__ flux_y, const T *__restrict__ flux_z,
+                          T *__restrict__ dpsidt) {
   CELL_IJK(d);
+  const T dxi = (T)dxi_d, dyi = (T)dyi_d;
   dpsidt[r] = -(dxi * (flux_x[q] - flux_x[q - 1]) + dyi * (flux_y[q] - flux_y[q - d.sy]) +
-                dzfi[k] * (flux_z[q] - flux_z[q - d.sz]));
+                (T)dzfi[k] * (flux_z[q] - flux_z[q - d.sz]));
 }
 
 // clip_field, src/two_fluid.f90:54-57
-__global__ void k_clip(long nh, double rmin, double rmax, double *__restrict__ p) {
+template <typename T>
+__global__ void k_clip(long nh, double rmin, double rmax, T *__restrict__ p) {
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
-    p[q] = fmin(fmax(rmin, p[q]), rmax);
+    p[q] = fmin(fmax((T)rmin, p[q]), (T)rmax);
 }
 
 // cmpt_norm_curv_youngs, src/two_fluid.f90:81-174
@@ -1026,7 +1041,22 @@ int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float
 
 extern "C" int fz_vof_thinc_cmpt_phi(fz_ctx *c, double beta, const double *psi, double *phi) {
   if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_vof_thinc_cmpt_phi: null argument");
-  FZ_LAUNCH(c, "vof_phi", k_vof_phi, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, beta, psi, phi);
+  FZ_LAUNCH(c, "vof_phi", k_vof_phi<double>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, beta, psi, phi);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+template <typename T>
+static int vof_transport(fz_ctx *c, double beta, const double *u, const double *v, const double *w,
+                         const T *normx, const T *normy, const T *normz, const T *phi, T *dpsidt, T *flux_x,
+                         T *flux_y, T *flux_z) {
+  const fz_dims &d = c->d;
+  dim3 g((d.n1 + 1 + 63) / 64, (d.n2 + 1 + 3) / 4, d.n3 + 1);
+  FZ_LAUNCH(c, sizeof(T) == 4 ? "vof_flux_f32" : "vof_flux", k_vof_flux<T>, g, fz_block3(), 0, d, beta, u, v, w,
+            normx, normy, normz, phi, flux_x, flux_y, flux_z);
+  FZ_LAUNCHED();
+  FZ_LAUNCH(c, sizeof(T) == 4 ? "vof_div_f32" : "vof_div", k_vof_div<T>, fz_grid_interior(d), fz_block3(), 0, d,
+            c->dzfi, c->dli[0], c->dli[1], flux_x, flux_y, flux_z, dpsidt);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -1038,20 +1068,31 @@ extern "C" int fz_vof_thinc_transport_psi(fz_ctx *c, double beta, const double *
   (void)psi;
   if (!c || !u || !v || !w || !normx || !normy || !normz || !phi || !dpsidt || !flux_x || !flux_y || !flux_z)
     return fz_set_error(FZ_ERR_ARG, "fz_vof_thinc_transport_psi: null argument");
-  const fz_dims &d = c->d;
-  dim3 g((d.n1 + 1 + 63) / 64, (d.n2 + 1 + 3) / 4, d.n3 + 1);
-  FZ_LAUNCH(c, "vof_flux", k_vof_flux, g, fz_block3(), 0, d, beta, u, v, w, normx, normy, normz, phi, flux_x, flux_y,
-                     flux_z);
+  return vof_transport<double>(c, beta, u, v, w, normx, normy, normz, phi, dpsidt, flux_x, flux_y, flux_z);
+}
+
+// the FP32 phase field of the mixed-precision step
+int fz_vof_transport_f32(fz_ctx *c, double beta, const double *u, const double *v, const double *w,
+                         const float *normx, const float *normy, const float *normz, const float *phi, float *dpsidt,
+                         float *flux_x, float *flux_y, float *flux_z) {
+  return vof_transport<float>(c, beta, u, v, w, normx, normy, normz, phi, dpsidt, flux_x, flux_y, flux_z);
+}
+
+int fz_vof_phi_f32(fz_ctx *c, double beta, const float *psi, float *phi) {
+  FZ_LAUNCH(c, "vof_phi_f32", k_vof_phi<float>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, beta, psi, phi);
   FZ_LAUNCHED();
-  FZ_LAUNCH(c, "vof_div", k_vof_div, fz_grid_interior(d), fz_block3(), 0, d, c->dzfi, c->dli[0], c->dli[1], flux_x,
-                     flux_y, flux_z, dpsidt);
+  return FZ_OK;
+}
+
+int fz_clip_f32(fz_ctx *c, double rmin, double rmax, float *p) {
+  FZ_LAUNCH(c, "clip_f32", k_clip<float>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, rmin, rmax, p);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 
 extern "C" int fz_clip_field(fz_ctx *c, const double minmax[2], double *p) {
   if (!c || !minmax || !p) return fz_set_error(FZ_ERR_ARG, "fz_clip_field: null argument");
-  FZ_LAUNCH(c, "clip", k_clip, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, minmax[0], minmax[1], p);
+  FZ_LAUNCH(c, "clip", k_clip<double>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, minmax[0], minmax[1], p);
   FZ_LAUNCHED();
   return FZ_OK;
 }

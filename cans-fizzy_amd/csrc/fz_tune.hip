@@ -4,10 +4,12 @@
 // arithmetic, so every choice gives the same fields bit for bit:
 //  - tile_y: rows of blocks per z-marching pencil in the stencil block order;
 //  - the variant of the fused momentum kernel (occupancy, staged loads);
-//  - rows per chunk of the Thomas kernel (FP64, or FP32 in mixed cases).
+//  - rows per chunk of the Thomas kernel (FP64, or FP32 in mixed cases,
+//    where two columns per lane are a candidate too).
 // A scratch case with the same parameters (dt checks off, so every trial
 // times the same work) is stepped once per candidate to warm up and then
-// `trial_steps` times between HIP events on its stream; the parameters are
+// twice `trial_steps` times between HIP events on its stream (the faster
+// of the two counts); the parameters are
 // chosen one after another (coordinate descent), each keeping the best time
 // so far. The result is applied to the case's context and cached by grid,
 // boundary conditions and precision, so cases of the same shape reuse it.
@@ -38,18 +40,22 @@ void apply(fz_ctx *c, const Tuned &t, bool mixed) {
   else c->tk64 = t.tk;
 }
 
-// ms per step of the scratch case under t
+// ms per step of the scratch case under t: the better of two timed runs
 int trial(fz_case *sc, fz_ctx *c, const Tuned &t, bool mixed, int steps, hipEvent_t a, hipEvent_t b, float *ms) {
   apply(c, t, mixed);
   int killed = 0;
   FZ_TRY(fz_case_step(sc, 1, &killed));
-  FZ_HIP(hipEventRecord(a, c->st));
-  FZ_TRY(fz_case_step(sc, steps, &killed));
-  FZ_HIP(hipEventRecord(b, c->st));
-  FZ_HIP(hipEventSynchronize(b));
-  float e = 0.0f;
-  FZ_HIP(hipEventElapsedTime(&e, a, b));
-  *ms = e / (float)steps;
+  float best = 0.0f;
+  for (int rep = 0; rep < 2; rep++) {
+    FZ_HIP(hipEventRecord(a, c->st));
+    FZ_TRY(fz_case_step(sc, steps, &killed));
+    FZ_HIP(hipEventRecord(b, c->st));
+    FZ_HIP(hipEventSynchronize(b));
+    float e = 0.0f;
+    FZ_HIP(hipEventElapsedTime(&e, a, b));
+    if (rep == 0 || e < best) best = e;
+  }
+  *ms = best / (float)steps;
   return FZ_OK;
 }
 }  // namespace
@@ -82,10 +88,10 @@ extern "C" int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size
     if (rc == FZ_OK) rc = trial(sc, c, best, mixed, trial_steps, a, b, &best.ms);
     const int tiles[] = {0, 2, 4, 8, 16};
     const int moms[] = {0, 1, 4, 5};
-    const int tks[] = {8, 16, 32};
+    const int tks[] = {8, 16, 32, -8, -16};  // negative: two FP32 columns per lane (mixed cases)
     for (int dim = 0; dim < 3 && rc == FZ_OK; dim++) {
       const int *cand = dim == 0 ? tiles : (dim == 1 ? moms : tks);
-      const int ncand = dim == 0 ? 5 : (dim == 1 ? 4 : 3);
+      const int ncand = dim == 0 ? 5 : (dim == 1 ? 4 : (mixed ? 5 : 3));
       for (int q = 0; q < ncand && rc == FZ_OK; q++) {
         Tuned t = best;
         int &field = dim == 0 ? t.tile_y : (dim == 1 ? t.mom_variant : t.tk);
@@ -105,7 +111,8 @@ extern "C" int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size
   apply(ctx, best, mixed);
   fz_case_drop_graphs(cs);  // captured launches carry the old parameters
   if (report && cap > 0)
-    snprintf(report, cap, "tile_y=%d mom_variant=%d thomas_rows=%d ms_per_step=%.4f%s", best.tile_y,
-             best.mom_variant, best.tk, best.ms, cached ? " (cached)" : "");
+    snprintf(report, cap, "tile_y=%d mom_variant=%d thomas_rows=%d%s ms_per_step=%.4f%s", best.tile_y,
+             best.mom_variant, best.tk < 0 ? -best.tk : best.tk, best.tk < 0 ? "x2" : "", best.ms,
+             cached ? " (cached)" : "");
   return FZ_OK;
 }

@@ -81,6 +81,20 @@ def vof_droplets(ng=(64, 64, 64), ndrop=4, r=0.3, seed=20261202, rho12=(50.0, 1.
         inipsi="drp3", spheres=[(x, y, z, r) for (x, y, z) in cents], vof=True, psi_thickness_factor=0.5)
 
 
+def config3(ng=256, mixed=False):
+    """THINC/QQ droplets in forced turbulence: triply periodic box of side
+    2*pi, 32 droplets of radius 0.3 at seeded non-overlapping centres, density
+    ratio 50, Weber number 5 on the carrier density, a unit velocity and the
+    droplet diameter (sigma = 1*1*0.6/5), the reference's Taylor-Green
+    large-scale forcing (src/forcing.f90; the reference has no random
+    solenoidal forcing), fluid initially at rest."""
+    spec = vof_droplets(ng=(ng, ng, ng), ndrop=32, r=0.3, rho12=(50.0, 1.0), mu12=(0.02, 0.002),
+                        sigma=1.0 * 1.0 * 0.6 / 5.0)
+    if mixed:
+        spec = dict(spec, precision="mixed", refine=1)
+    return spec
+
+
 def splitmix64(seed):
     """SplitMix64 generator of 53-bit uniforms in [0,1)."""
     s = seed & 0xFFFFFFFFFFFFFFFF

@@ -75,6 +75,22 @@ def test_mixed_solver_refines_to_fp64(n, cbc, gr, rng):
     assert err <= max(1e-10, fp64_acc)
 
 
+@pytest.mark.parametrize("rows", ["8", "16", "-8", "-16"])
+def test_thomas_variants_agree_bitwise(rows, monkeypatch):
+    """Every FP32 Thomas variant (rows per chunk, one or two columns per
+    lane; a context reads FZ_THOMAS_ROWS32 when it is created) gives the
+    default variant's solution bit for bit."""
+    out = []
+    for env in (None, rows):
+        if env is None:
+            monkeypatch.delenv("FZ_THOMAS_ROWS32", raising=False)
+        else:
+            monkeypatch.setenv("FZ_THOMAS_ROWS32", env)
+        it, rr, res, err, acc = solve_pair([64, 64, 32], "PPPPNN", 0.6, np.random.default_rng(7), 1e-13, 2)
+        out.append((it, rr, err))
+    assert out[0] == out[1]
+
+
 @pytest.mark.parametrize("n,cbc,gr", CASES[:4], ids=[f"{'x'.join(map(str, a))}-{b}" for a, b, g in CASES[:4]])
 def test_single_fp32_solve_is_fp32_accurate(n, cbc, gr, rng):
     it, rr, res, err, fp64_acc = solve_pair(n, cbc, gr, rng, 1e-13, 0)
@@ -123,15 +139,16 @@ def field_errors(got, ref):
 
 
 MCASES = [("bubble3d-32x32x64", cases.rising_bubble_3d(ng=(32, 32, 64), rho12=(1000.0, 1.0), mu12=(10.0, 0.1))),
-          ("hysing2d", cases.hysing_2d())]
+          ("hysing2d", cases.hysing_2d()),
+          ("vof-droplets-32", cases.vof_droplets(ng=(32, 32, 32)))]
 
 
 @pytest.mark.parametrize("name,spec", MCASES, ids=[m[0] for m in MCASES])
 def test_mixed_case_against_fp64(name, spec):
     """40 steps (checks every 10) from the same initial state. Stated bounds
     with one refinement: chkdiv divmax <= 1e-10; velocity <= 1e-2, pressure
-    <= 1e-3, psi <= 1e-5 and interface curvature <= 1e-2 of their scales.
-    Without refinement the FP32 solve leaves divmax at O(1e-6)."""
+    <= 1e-3, psi <= 1e-5 (THINC/QQ: 1e-3) and interface curvature <= 1e-2 of
+    their scales. Without refinement the FP32 solve leaves divmax at O(1e-6)."""
     nsteps = 40
     ref, sref, _ = run_case(spec, nsteps)
     out = {}
@@ -146,7 +163,8 @@ def test_mixed_case_against_fp64(name, spec):
     assert divmax <= 1e-10
     assert out[2][1] <= 1e-10
     assert out[0][1] > 1e-9  # the refinement is what brings the divergence down
-    bounds = {"u": 1e-2, "v": 1e-2, "w": 1e-2, "p": 1e-3, "psi": 1e-5, "kappa": 1e-2}
+    # THINC's sharp profile (an FP32 exp per face) moves psi more than ACDI's
+    bounds = {"u": 1e-2, "v": 1e-2, "w": 1e-2, "p": 1e-3, "psi": 1e-3 if spec.get("vof") else 1e-5, "kappa": 1e-2}
     for k in CASE_FIELDS:
         assert errs[k] <= bounds[k], (k, errs[k])
     assert drift <= 1e-5

@@ -57,15 +57,16 @@ def test_fortran_modules_bind_the_library():
 
 def test_mixed_precision_refuses_what_it_does_not_cover():
     """Argument checks of fz_case_create_ex run on the host before any device
-    allocation: unknown precision, a refinement count out of range and the
-    VOF step in mixed precision are refused with a message."""
+    allocation: unknown precision, a refinement count out of range and a
+    single-phase case in mixed precision are refused with a message."""
     import ctypes as C
     from fizzy_amd import cases
     from fizzy_amd._lib import FzParams, FZ_OK
     L = F.lib()
     for spec, prec, nref, msg in [(cases.rising_bubble_3d(ng=(16, 16, 16)), 7, 1, b"unknown precision"),
                                   (cases.rising_bubble_3d(ng=(16, 16, 16)), 1, 9, b"refinements"),
-                                  (cases.vof_droplets(ng=(16, 16, 16)), 1, 1, b"ACDI")]:
+                                  (dict(cases.rising_bubble_3d(ng=(16, 16, 16)), is_track_interface=False), 1, 1,
+                                   b"two-fluid")]:
         prm = cases.fill_params(FzParams(), spec)
         h = C.c_void_p()
         rc = L.fz_case_create_ex(C.byref(h), C.byref(prm), None, prec, nref)
