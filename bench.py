@@ -188,6 +188,42 @@ def main_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def batched_poisson(F, np, k=16, n=64, rounds=20):
+    """k same-shaped n^3 solves: one fz_solver per context against one
+    fz_solver_batched call for all of them."""
+    import ctypes as C
+    ctxs = [F.Context([n, n, n], [1.0, 1.0, 1.0], "PPPPNN") for _ in range(k)]
+    try:
+        rng = np.random.default_rng(5)
+        fields = []
+        for c in ctxs:
+            F.call("fz_initsolver", c.h, None, None, None, None, None, None, None, None, None)
+            fields.append(c.halo(rng.uniform(-1.0, 1.0, (n + 2, n + 2, n + 2))))
+        cv = (C.c_void_p * k)(*[c.h.value for c in ctxs])
+        pv = (C.c_void_p * k)(*[f.ptr.value for f in fields])
+        t = {}
+        for mode in ("single", "batched"):
+            for rep in range(rounds + 2):
+                if rep == 2:
+                    for c in ctxs:
+                        c.sync()
+                    t0 = time.perf_counter()
+                if mode == "single":
+                    for c, f in zip(ctxs, fields):
+                        F.call("fz_solver", c.h, f.ptr)
+                else:
+                    F.call("fz_solver_batched", cv, pv, k)
+            for c in ctxs:
+                c.sync()
+            t[mode] = (time.perf_counter() - t0) / rounds * 1e3
+        log(f"{k} solves of {n}^3: one per context {t['single']:.3f} ms, batched {t['batched']:.3f} ms")
+        return {"solves": k, "grid": [n, n, n], "ms_one_per_context": round(t["single"], 4),
+                "ms_batched": round(t["batched"], 4), "speedup": round(t["single"] / t["batched"], 3)}
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def main_poisson(args):
     """The standalone Poisson solve: 512^3, periodic x/y, Neumann z, a
     right-hand side seeded uniform in [-1, 1] with its (volume-weighted)
@@ -244,13 +280,14 @@ def main_poisson(args):
                 + (f", refinements {sorted(set(its))}" if its else ""))
     finally:
         ctx.close()
+    res["batched"] = batched_poisson(F, np)
     out = {
         "metric": "standalone Poisson solve, FP64 vs FP32 + FP64-residual refinement",
         "value": res["fp64"]["Mcell_solves_s"], "unit": "Mcell-solves/s (FP64)", "n_gpus": 1, "steps": nsolve,
         "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: uniform [-1,1] right-hand side, seeded, volume-weighted mean removed",
         "config": {"workload": "poisson_512x512x512_PPPPNN", "grid": n}, "fp64": res["fp64"],
-        "mixed": res["mixed"],
+        "mixed": res["mixed"], "batched": res["batched"],
     }
     print(json.dumps(out), flush=True)
 

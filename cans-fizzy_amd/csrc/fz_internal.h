@@ -57,13 +57,15 @@ struct fz_poisson {
   double *rhsbx = nullptr, *rhsby = nullptr, *rhsbz = nullptr;  // device
   double *work = nullptr;   // spectral buffer, n1*n2*n3
   double *piv = nullptr;    // Thomas pivots, n1*n2*n3
-  double *aux = nullptr;    // periodic-z second solve, n1*n2*n3
+  double *aux = nullptr;    // periodic z: gaussel_periodic's second solve, made at set-up, n1*n2*n3
   int periodic_z = 0;
   int rhsb_zero = 0;  // all boundary right-hand sides vanish (homogeneous pressure BCs)
   long null_col = -1;  // spectral column of the operator's null mode, or -1
   float *work32 = nullptr;              // FP32 spectral buffer of the mixed-precision solve
   double *bcopy = nullptr;              // right-hand side kept for the residuals (halo'd layout)
   unsigned long long *rslots = nullptr; // max |residual| slots (FP64 bits)
+  void **batch_ptrs = nullptr;          // device pointer arrays of fz_solver_batched
+  size_t batch_cap = 0;
   char cbc[6];
 };
 

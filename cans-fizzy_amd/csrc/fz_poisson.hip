@@ -22,6 +22,13 @@ struct RadList {
 struct LineMap {  // element e of line (l1,l2) sits at base + l1*s1 + l2*s2 + e*se
   long base, s1, s2, se;
 };
+// the arrays of a batch of same-shaped solves, one per blockIdx.z (device
+// arrays of n pointers; src null: no batch)
+struct Batch {
+  const void *const *src;
+  void *const *dst;
+  int n;
+};
 // The line source of a forward x transform when it is not a plain array:
 // fillps (src/fillps.f90:33-50) evaluated in the load (u, v, w halo'd
 // fields; k = l2 + 1), or, for the residual iterations of the mixed-precision
@@ -293,8 +300,13 @@ __global__ void __launch_bounds__(256) k_r2r(const TI *__restrict__ src, LineMap
                                              LineMap out, int L1, int n, int m, int kind, int G, double scale,
                                              int accum, const typename CT<R>::C *__restrict__ tw,
                                              const typename CT<R>::C *__restrict__ q,
-                                             const typename CT<R>::C *__restrict__ t2, int nrad, RadList rl) {
+                                             const typename CT<R>::C *__restrict__ t2, int nrad, RadList rl,
+                                             Batch bt) {
   using C = typename CT<R>::C;
+  if (bt.src) {
+    src = (const TI *)bt.src[blockIdx.z];
+    dst = (TO *)bt.dst[blockIdx.z];
+  }
   extern __shared__ double2 smem_raw[];
   C *smem = (C *)smem_raw;
   const bool quarter = kind_is_quarter(kind);
@@ -378,10 +390,15 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
                                                 LineMap out, int L1, int kind, double scale, int accum,
                                                 const typename CT<R>::C *__restrict__ tw,
                                                 const typename CT<R>::C *__restrict__ q,
-                                                const typename CT<R>::C *__restrict__ t2, FillSrc fs) {
+                                                const typename CT<R>::C *__restrict__ t2, FillSrc fs,
+                                                Batch bt) {
   using S = P2Shape<M, QUARTER, R>;
   using C = typename CT<R>::C;
   constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4;
+  if (bt.src) {  // case blockIdx.z of a batch of same-shaped solves
+    src = (const TI *)bt.src[blockIdx.z];
+    dst = (TO *)bt.dst[blockIdx.z];
+  }
   extern __shared__ double2 smem_raw[];
   C *smem = (C *)smem_raw;
   R *Rw = (R *)smem;  // G*N staged reals, overlaid by the complex lines
@@ -504,21 +521,25 @@ static PlanTabs<float> tabs_of(const fz_fftplan &P, float) { return {P.tw32, P.q
 
 template <int M, bool QUARTER, typename R, typename TI, typename TO>
 static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
-                      TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs) {
+                      TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs,
+                      const Batch &bt) {
   using S = P2Shape<M, QUARTER, R>;
-  dim3 grid((L1 + S::G - 1) / S::G, L2, 1);
+  dim3 grid((L1 + S::G - 1) / S::G, L2, bt.src ? bt.n : 1);
   const PlanTabs<R> t = tabs_of(P, R());
   FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind,
-            scale, accum, t.tw, t.q, t.t2, fs);
+            scale, accum, t.tw, t.q, t.t2, fs, bt);
 }
+
+static const Batch kNoBatch = {nullptr, nullptr, 0};
 
 static bool p2_length(int m) { return m >= 16 && m <= 2048 && (m & (m - 1)) == 0; }
 
 template <bool QUARTER, typename R, typename TI, typename TO>
 static bool dispatch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
-                        TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs) {
+                        TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs,
+                        const Batch &bt = kNoBatch) {
 #define FZ_P2_CASE(MM) \
-  case MM: launch_p2<MM, QUARTER, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, fs); return true;
+  case MM: launch_p2<MM, QUARTER, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, fs, bt); return true;
   switch (P.m) {
     FZ_P2_CASE(16)
     FZ_P2_CASE(32)
@@ -561,24 +582,13 @@ __device__ inline void thomas_col(T *__restrict__ col, double *__restrict__ dcol
   }
 }
 
-// The same recurrence as thomas_col, in chunks of TK rows: each chunk's
-// loads are issued together so one wave keeps TK loads in flight (one
-// column per lane leaves few waves per CU to hide latency otherwise).
-// Column `gauge` (the singular mode of an all-Neumann/periodic operator, or
-// -1) has its last unknown set to zero instead of divided by the eps-shifted
-// zero pivot: the mixed-precision solve fixes the null mode this way, since
-// the O(1/eps) constant of the reference's pivots would swamp FP32 values.
+// The chunked recurrence of one column over n rows (pin: the last unknown
+// set to zero, for the gauge column of the mixed-precision solve)
 template <int TK, typename T>
-__global__ void __launch_bounds__(64) k_thomas(T *__restrict__ W, double *__restrict__ D,
-                                               const double *__restrict__ lambdaxy, const double *__restrict__ a,
-                                               const double *__restrict__ b, const double *__restrict__ c, int n1,
-                                               int n2, int n3, long gauge) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)n1 * n2) return;
-  const long ps = (long)n1 * n2;
-  T *col = W + idx;
-  double *dcol = D + idx;
-  const double lam = lambdaxy[idx];
+__device__ __forceinline__ void thomas_chunked(T *__restrict__ col, double *__restrict__ dcol, long ps, int n3,
+                                               double lam, const double *__restrict__ a,
+                                               const double *__restrict__ b, const double *__restrict__ c,
+                                               bool pin) {
   double d = 0.0, pprev = 0.0;
   for (int l0 = 0; l0 < n3; l0 += TK) {
     double r[TK];
@@ -600,7 +610,7 @@ __global__ void __launch_bounds__(64) k_thomas(T *__restrict__ W, double *__rest
           d = c[l] * z;
           pprev = (r[t] - a[l] * pprev) * z;
         }
-        if (idx == gauge && l == n3 - 1) pprev = 0.0;
+        if (pin && l == n3 - 1) pprev = 0.0;
         dcol[(long)l * ps] = d;
         col[(long)l * ps] = (T)pprev;
       }
@@ -622,6 +632,28 @@ __global__ void __launch_bounds__(64) k_thomas(T *__restrict__ W, double *__rest
         col[(long)(h0 - t) * ps] = (T)pnext;
       }
   }
+}
+
+// The same recurrence as thomas_col, in chunks of TK rows: each chunk's
+// loads are issued together so one wave keeps TK loads in flight (one
+// column per lane leaves few waves per CU to hide latency otherwise).
+// Column `gauge` (the singular mode of an all-Neumann/periodic operator, or
+// -1) has its last unknown set to zero instead of divided by the eps-shifted
+// zero pivot: the mixed-precision solve fixes the null mode this way, since
+// the O(1/eps) constant of the reference's pivots would swamp FP32 values.
+template <int TK, typename T>
+__global__ void __launch_bounds__(64) k_thomas(T *__restrict__ W, double *__restrict__ D,
+                                               const double *__restrict__ lambdaxy, const double *__restrict__ a,
+                                               const double *__restrict__ b, const double *__restrict__ c, int n1,
+                                               int n2, int n3, long gauge, Batch bt) {
+  if (bt.src) {  // case blockIdx.y of a batch: its spectral array and pivots
+    W = (T *)bt.src[blockIdx.y];
+    D = (double *)bt.dst[blockIdx.y];
+  }
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n1 * n2) return;
+  const long ps = (long)n1 * n2;
+  thomas_chunked<TK, T>(W + idx, D + idx, ps, n3, lambdaxy[idx], a, b, c, idx == gauge);
 }
 
 // k_thomas for FP32 columns, two adjacent columns per lane: each load moves
@@ -696,31 +728,58 @@ __global__ void __launch_bounds__(64) k_thomas_x2(float *__restrict__ W, double 
   }
 }
 
-// gaussel_periodic, src/solver.f90:107-149 (P2 and the pivots in FP64; the
-// gauge column as in k_thomas: the cyclic unknown is set to zero)
-template <typename T>
-__global__ void k_thomas_periodic(T *__restrict__ W, double *__restrict__ D, double *__restrict__ P2,
-                                  const double *__restrict__ lambdaxy, const double *__restrict__ a,
-                                  const double *__restrict__ b, const double *__restrict__ c, int n1, int n2,
-                                  int n3, long gauge) {
+// the second solve of gaussel_periodic (src/solver.f90:107-149): it
+// depends on the operator only, so it is made once per solver set-up, with
+// the operations of the per-solve form (pivots scratch in D)
+__global__ void k_periodic_p2(double *__restrict__ D, double *__restrict__ P2, const double *__restrict__ lambdaxy,
+                              const double *__restrict__ a, const double *__restrict__ b,
+                              const double *__restrict__ c, int n1, int n2, int n3) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n1 * n2) return;
+  const long ps = (long)n1 * n2;
+  const int m = n3 - 1;
+  double *p2 = P2 + idx;
+  for (int l = 0; l < m; l++) p2[l * ps] = 0.0;
+  p2[0] = -a[0];
+  p2[(m - 1) * ps] = -c[m - 1];
+  thomas_col(p2, D + idx, ps, m, a, b, c, lambdaxy[idx]);
+}
+
+// gaussel_periodic with the second solve P2 made at set-up: the chunked
+// recurrence on the first n3-1 rows, then the cyclic unknown and the update
+// (the gauge column as in k_thomas: the cyclic unknown is set to zero)
+template <int TK, typename T>
+__global__ void __launch_bounds__(64) k_thomas_periodic_pc(T *__restrict__ W, double *__restrict__ D,
+                                                           const double *__restrict__ P2,
+                                                           const double *__restrict__ lambdaxy,
+                                                           const double *__restrict__ a, const double *__restrict__ b,
+                                                           const double *__restrict__ c, int n1, int n2, int n3,
+                                                           long gauge) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)n1 * n2) return;
   const long ps = (long)n1 * n2;
   const double lam = lambdaxy[idx];
   const int m = n3 - 1;
   T *col = W + idx;
-  double *p2 = P2 + idx;
-  thomas_col(col, D + idx, ps, m, a, b, c, lam);
-  for (int l = 0; l < m; l++) p2[l * ps] = 0.0;
-  p2[0] = -a[0];
-  p2[(m - 1) * ps] = -c[m - 1];
-  thomas_col(p2, D + idx, ps, m, a, b, c, lam);
+  const double *p2 = P2 + idx;
+  thomas_chunked<TK, T>(col, D + idx, ps, m, lam, a, b, c, false);
   const double bbm = b[m] + lam;
   double pn = ((double)col[m * ps] - c[m] * (double)col[0] - a[m] * (double)col[(m - 1) * ps]) /
               (bbm + c[m] * p2[0] + a[m] * p2[(m - 1) * ps] + FZ_EPS);
   if (idx == gauge) pn = 0.0;
   col[m * ps] = (T)pn;
-  for (int l = 0; l < m; l++) col[l * ps] = (T)((double)col[l * ps] + p2[l * ps] * pn);
+  for (int l0 = 0; l0 < m; l0 += TK) {
+    double r[TK], q[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < m) {
+        r[t] = (double)col[(long)(l0 + t) * ps];
+        q[t] = p2[(long)(l0 + t) * ps];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < m) col[(long)(l0 + t) * ps] = (T)(r[t] + q[t] * pn);
+  }
 }
 
 // the source of a forward x transform evaluated into x lines of W (the
@@ -855,6 +914,7 @@ int fz_poisson_destroy(fz_poisson *p) {
   if (p->work32) (void)hipFree(p->work32);
   if (p->bcopy) (void)hipFree(p->bcopy);
   if (p->rslots) (void)hipFree(p->rslots);
+  if (p->batch_ptrs) (void)hipFree(p->batch_ptrs);
   delete p;
   return FZ_OK;
 }
@@ -957,19 +1017,26 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
     return fz_set_error(FZ_ERR_HIP, "fz_initsolver: %s", hipGetErrorString(e));
   }
   ctx->pois = P;
+  if (P->periodic_z) {  // the operator's second solve of gaussel_periodic, once
+    const long ps = (long)n1 * n2;
+    FZ_LAUNCH(ctx, "periodic_p2", k_periodic_p2, dim3((int)((ps + 255) / 256)), dim3(256), 0, P->piv, P->aux,
+              P->lambdaxy, P->a, P->b, P->c, n1, n2, n3);
+    FZ_LAUNCHED();
+  }
   return FZ_OK;
 }
 
 template <typename R, typename TI, typename TO>
 static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
-                      TO *dst, LineMap out, int L1, int L2, double scale, int accum = 0) {
+                      TO *dst, LineMap out, int L1, int L2, double scale, int accum = 0,
+                      const Batch &bt = kNoBatch) {
   const bool quarter = kind_is_quarter(kind);
   if (p2_length(P.m)) {
     FillSrc off;
     memset(&off, 0, sizeof(off));
     const bool done =
-        quarter ? dispatch_p2<true, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, off)
-                : dispatch_p2<false, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, off);
+        quarter ? dispatch_p2<true, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, off, bt)
+                : dispatch_p2<false, R, TI, TO>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, off, bt);
     if (done) {
       FZ_LAUNCHED();
       return FZ_OK;
@@ -991,10 +1058,10 @@ static int launch_r2r(fz_ctx *ctx, const char *name, const fz_fftplan &P, int ki
   }
   RadList rl;
   for (int s = 0; s < 24; s++) rl.r[s] = P.rad[s];
-  dim3 grid((L1 + G - 1) / G, L2, 1);
+  dim3 grid((L1 + G - 1) / G, L2, bt.src ? bt.n : 1);
   const PlanTabs<R> t = tabs_of(P, R());
   FZ_LAUNCH(ctx, name, (k_r2r<R, TI, TO>), grid, dim3(256), lds, src, in, dst, out, L1, P.n, P.m, kind, G, scale,
-            accum, t.tw, t.q, t.t2, P.nrad, rl);
+            accum, t.tw, t.q, t.t2, P.nrad, rl, bt);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -1100,14 +1167,17 @@ static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum) {
       else FZ_LAUNCH(ctx, name, k_thomas_x2<16>, g2, blk, 0, W32, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
                      gauge);
     } else if (tk == 8) FZ_LAUNCH(ctx, name, (k_thomas<8, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
-                           n3, gauge);
+                           n3, gauge, kNoBatch);
     else if (tk == 32) FZ_LAUNCH(ctx, name, (k_thomas<32, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c,
-                                 n1, n2, n3, gauge);
+                                 n1, n2, n3, gauge, kNoBatch);
     else FZ_LAUNCH(ctx, name, (k_thomas<16, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
-                   gauge);
+                   gauge, kNoBatch);
   } else {
-    FZ_LAUNCH(ctx, mixed ? "thomas_periodic_f32" : "thomas_periodic", k_thomas_periodic<T>, dim3(nb), dim3(tpb), 0, W,
-              P->piv, P->aux, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3, gauge);
+    (void)nb;
+    (void)tpb;
+    FZ_LAUNCH(ctx, mixed ? "thomas_periodic_f32" : "thomas_periodic", (k_thomas_periodic_pc<16, T>),
+              dim3((int)((ps + 63) / 64)), dim3(64), 0, W, P->piv, P->aux, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
+              gauge);
   }
   FZ_LAUNCHED();
   FZ_TRY((launch_r2r<T, T, T>(ctx, mixed ? "fft_y_bwd_f32" : "fft_y_bwd", P->py, P->py.kind_b, W, w_y, W, w_y, n1, n3,
@@ -1285,6 +1355,72 @@ extern "C" int fz_solver_mixed(fz_ctx *ctx, double *p, double tol, int maxit, in
     *relres = bmax > 0.0 ? res / bmax : res;
   }
   if (iters) *iters = it;
+  return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched solves: the solvers of n contexts of one shape in one launch per stage
+// ---------------------------------------------------------------------------
+static bool same_operator(const fz_ctx *a, const fz_ctx *b) {
+  if (a->d.n1 != b->d.n1 || a->d.n2 != b->d.n2 || a->d.n3 != b->d.n3) return false;
+  if (memcmp(a->pois->cbc, b->pois->cbc, 6) != 0) return false;
+  if (a->dli[0] != b->dli[0] || a->dli[1] != b->dli[1]) return false;
+  return a->h_dzc == b->h_dzc && a->h_dzf == b->h_dzf;
+}
+
+extern "C" int fz_solver_batched(fz_ctx *const *ctxs, double *const *ps, int n) {
+  if (!ctxs || !ps || n < 0) return fz_set_error(FZ_ERR_ARG, "fz_solver_batched: bad argument");
+  if (n == 0) return FZ_OK;
+  fz_ctx *c0 = ctxs[0];
+  for (int q = 0; q < n; q++) {
+    if (!ctxs[q] || !ps[q]) return fz_set_error(FZ_ERR_ARG, "fz_solver_batched: null context or field %d", q);
+    if (!ctxs[q]->pois)
+      return fz_set_error(FZ_ERR_STATE, "fz_solver_batched: fz_initsolver has not been called on context %d", q);
+    if (!same_operator(c0, ctxs[q]))
+      return fz_set_error(FZ_ERR_ARG, "fz_solver_batched: context %d differs in grid or pressure conditions", q);
+  }
+  fz_poisson *P = c0->pois;
+  if (P->periodic_z) return fz_set_error(FZ_ERR_UNSUPPORTED, "fz_solver_batched: periodic z is not batched");
+  if (n > 65535) return fz_set_error(FZ_ERR_ARG, "fz_solver_batched: at most 65535 solves per call");
+  // each context's pending work on its fields first
+  for (int q = 0; q < n; q++) FZ_HIP(hipStreamSynchronize(ctxs[q]->st));
+  // device arrays: p, work, pivots of every case
+  std::vector<void *> h(3 * (size_t)n);
+  for (int q = 0; q < n; q++) {
+    h[q] = ps[q];
+    h[n + q] = ctxs[q]->pois->work;
+    h[2 * n + q] = ctxs[q]->pois->piv;
+  }
+  if (P->batch_cap < 3 * (size_t)n) {
+    if (P->batch_ptrs) FZ_HIP(hipFree(P->batch_ptrs));
+    P->batch_ptrs = nullptr;
+    FZ_HIP(hipMalloc((void **)&P->batch_ptrs, sizeof(void *) * 3 * (size_t)n));
+    P->batch_cap = 3 * (size_t)n;
+  }
+  FZ_HIP(hipMemcpy(P->batch_ptrs, h.data(), sizeof(void *) * 3 * (size_t)n, hipMemcpyHostToDevice));
+  void **dp = P->batch_ptrs;
+  const Batch p_w = {(const void *const *)dp, dp + n, n};              // p -> work
+  const Batch w_w = {(const void *const *)(dp + n), dp + n, n};        // work in place
+  const Batch w_p = {(const void *const *)(dp + n), dp, n};            // work -> p
+  const Batch w_d = {(const void *const *)(dp + n), dp + 2 * n, n};    // Thomas: work, pivots
+  const int n1 = c0->d.n1, n2 = c0->d.n2, n3 = c0->d.n3;
+  const long sy = c0->d.sy, sz = c0->d.sz, ps1 = (long)n1 * n2;
+  const LineMap p_x = {1 + sy + sz, sy, sz, 1};
+  const LineMap w_x = {0, n1, ps1, 1};
+  const LineMap w_y = {0, 1, ps1, n1};
+  FZ_TRY((launch_r2r<double, double, double>(c0, "fft_x_fwd_batch", P->px, P->px.kind_f, (const double *)nullptr,
+                                             p_x, (double *)nullptr, w_x, n2, n3, 1.0, 0, p_w)));
+  FZ_TRY((launch_r2r<double, double, double>(c0, "fft_y_fwd_batch", P->py, P->py.kind_f, (const double *)nullptr,
+                                             w_y, (double *)nullptr, w_y, n1, n3, 1.0, 0, w_w)));
+  const dim3 grid((int)((ps1 + 63) / 64), n), blk(64);
+  FZ_LAUNCH(c0, "thomas_batch", (k_thomas<16, double>), grid, blk, 0, (double *)nullptr, (double *)nullptr,
+            P->lambdaxy, P->a, P->b, P->c, n1, n2, n3, -1L, w_d);
+  FZ_LAUNCHED();
+  FZ_TRY((launch_r2r<double, double, double>(c0, "fft_y_bwd_batch", P->py, P->py.kind_b, (const double *)nullptr,
+                                             w_y, (double *)nullptr, w_y, n1, n3, 1.0, 0, w_w)));
+  FZ_TRY((launch_r2r<double, double, double>(c0, "fft_x_bwd_batch", P->px, P->px.kind_b, (const double *)nullptr,
+                                             w_x, (double *)nullptr, p_x, n2, n3, P->normfft, 0, w_p)));
+  FZ_HIP(hipStreamSynchronize(c0->st));
   return FZ_OK;
 }
 

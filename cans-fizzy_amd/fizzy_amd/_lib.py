@@ -51,6 +51,7 @@ SIGNATURES = {
     "fz_initsolver": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp]),
     "fz_solver": (C.c_int, [vp, vp]),
     "fz_solver_mixed": (C.c_int, [vp, vp, C.c_double, C.c_int, C.POINTER(C.c_int), dp]),
+    "fz_solver_batched": (C.c_int, [C.POINTER(vp), C.POINTER(vp), C.c_int]),
     "fz_r2r": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
     "fz_fillps": (C.c_int, [vp, C.c_double, C.c_double, vp, vp, vp, vp]),
     "fz_updt_rhs_b": (C.c_int, [vp, vp]),

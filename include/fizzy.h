@@ -112,6 +112,13 @@ int fz_solver(fz_ctx *ctx, double *p);
  * periodic operator is fixed by pinning one unknown of the mean column
  * (compare with the mean removed). No reference counterpart. */
 int fz_solver_mixed(fz_ctx *ctx, double *p, double tol, int maxit, int *iters, double *relres);
+/* fz_solver on n contexts of one shape (grid, spacing and pressure
+ * conditions; each after fz_initsolver) with one launch per stage for all of
+ * them: ps[q] is context q's field, solved in place as fz_solver does, bit
+ * for bit. Runs on ctxs[0]'s stream after every context's stream has
+ * drained, and returns when the solves are done. Periodic z is not
+ * batched. No reference counterpart. */
+int fz_solver_batched(fz_ctx *const *ctxs, double *const *ps, int n);
 /* fft, src/fft.f90:176-187: an r2r transform (FFTW kind codes 0 R2HC, 1 HC2R,
  * 4 REDFT01, 5 REDFT10, 6 REDFT11, 8 RODFT01, 9 RODFT10, 10 RODFT11) over
  * `howmany` contiguous lines of length n, in place and unnormalised */
