@@ -225,11 +225,20 @@ struct BcMapDev {
   int n;
 };
 
+// up to FZ_GATHER_MAX (map, field) pairs, one per blockIdx.y: the boundary
+// updates of several fields in one launch
+#define FZ_GATHER_MAX 4
 template <typename T>
-__global__ void k_bc_gather(BcMapDev m0, BcMapDev m1, BcMapDev m2, T *__restrict__ f0, T *__restrict__ f1,
-                            T *__restrict__ f2) {
-  const BcMapDev m = blockIdx.y == 0 ? m0 : (blockIdx.y == 1 ? m1 : m2);
-  T *f = blockIdx.y == 0 ? f0 : (blockIdx.y == 1 ? f1 : f2);
+struct GatherSet {
+  BcMapDev m[FZ_GATHER_MAX];
+  T *f[FZ_GATHER_MAX];
+};
+
+template <typename T>
+__global__ void k_bc_gather(GatherSet<T> g) {
+  const unsigned y = blockIdx.y;
+  const BcMapDev m = y == 0 ? g.m[0] : (y == 1 ? g.m[1] : (y == 2 ? g.m[2] : g.m[3]));
+  T *f = y == 0 ? g.f[0] : (y == 1 ? g.f[1] : (y == 2 ? g.f[2] : g.f[3]));
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.n; t += gridDim.x * blockDim.x) {
     const BcChain &ch = m.chains[m.cid[t]];
     const int sidx = m.src[t];
@@ -384,15 +393,23 @@ static BcMapDev devmap(const fz_bcmap *m) {
   return d;
 }
 
+// nf (map, field) pairs, FZ_GATHER_MAX per launch
 template <typename T>
-static int launch_gather(fz_ctx *c, int nf, const fz_bcmap *const *maps, T *f0, T *f1, T *f2) {
-  int nmax = 0;
-  for (int q = 0; q < nf; q++) nmax = std::max(nmax, maps[q]->n);
-  if (nmax == 0) return FZ_OK;
-  const int nb = std::min((nmax + 255) / 256, 4096);
-  FZ_LAUNCH(c, "bc_gather", k_bc_gather<T>, dim3(nb, nf), dim3(256), 0, devmap(maps[0]),
-            devmap(nf > 1 ? maps[1] : nullptr), devmap(nf > 2 ? maps[2] : nullptr), f0, f1, f2);
-  FZ_LAUNCHED();
+static int launch_gather(fz_ctx *c, int nf, const fz_bcmap *const *maps, T *const *fields) {
+  for (int q0 = 0; q0 < nf; q0 += FZ_GATHER_MAX) {
+    const int k = std::min(FZ_GATHER_MAX, nf - q0);
+    GatherSet<T> g;
+    int nmax = 0;
+    for (int q = 0; q < FZ_GATHER_MAX; q++) {
+      g.m[q] = devmap(q < k ? maps[q0 + q] : nullptr);
+      g.f[q] = q < k ? fields[q0 + q] : nullptr;
+      if (q < k) nmax = std::max(nmax, maps[q0 + q]->n);
+    }
+    if (nmax == 0) continue;
+    const int nb = std::min((nmax + 255) / 256, 4096);
+    FZ_LAUNCH(c, "bc_gather", k_bc_gather<T>, dim3(nb, k), dim3(256), 0, g);
+    FZ_LAUNCHED();
+  }
   return FZ_OK;
 }
 
@@ -410,7 +427,7 @@ static int boundp_t(fz_ctx *c, const char cbc[6], const double bc[6], T *p) {
   boundp_phases(c, cbc, bc, ph);
   if (!g_gather_off) {
     fz_bcmap *m = get_map(c, signature("p", cbc, 6, bc, 0, 0), ph, 0);
-    if (m->ok) return launch_gather<T>(c, 1, &m, p, nullptr, nullptr);
+    if (m->ok) return launch_gather<T>(c, 1, &m, &p);
   }
   return run_phases<T>(c, ph, p, nullptr, nullptr);
 }
@@ -426,7 +443,8 @@ static int bounduvw_t(fz_ctx *c, const char cbc[18], const double bc[18], int is
       m[fi] = get_map(c, signature("uvw", cbc, 18, bc, is_correc ? 1 : 0, fi), ph, fi);
       ok = ok && m[fi]->ok;
     }
-    if (ok) return launch_gather<T>(c, 3, m, u, v, w);
+    T *const f[3] = {u, v, w};
+    if (ok) return launch_gather<T>(c, 3, m, f);
   }
   return run_phases<T>(c, ph, u, v, w);
 }
@@ -443,6 +461,44 @@ extern "C" int fz_bounduvw(fz_ctx *c, const char cbc[18], const double bc[18], i
   if (!valid_codes(cbc, 18)) return fz_set_error(FZ_ERR_ARG, "fz_bounduvw: bad boundary code");
   return bounduvw_t<double>(c, cbc, bc, is_correc, u, v, w);
 }
+
+// Several boundary updates in as few launches as the gather allows: items
+// of boundp (one field) or bounduvw (three fields) on distinct fields, each
+// result as its own call's. Items whose composed map is unavailable run as
+// their own calls.
+template <typename T>
+int fz_bounds_group(fz_ctx *c, int n, const FzBcItem *items) {
+  std::vector<fz_bcmap *> maps;
+  std::vector<T *> fields;
+  for (int q = 0; q < n; q++) {
+    const FzBcItem &it = items[q];
+    const int ncbc = it.kind == 0 ? 6 : 18;
+    if (!valid_codes(it.cbc, ncbc)) return fz_set_error(FZ_ERR_ARG, "boundary update: bad boundary code");
+    std::vector<Phase> ph;
+    if (it.kind == 0) boundp_phases(c, it.cbc, it.bc, ph);
+    else bounduvw_phases(c, it.cbc, it.bc, it.is_correc ? 1 : 0, ph);
+    const int nf = it.kind == 0 ? 1 : 3;
+    bool ok = !g_gather_off;
+    fz_bcmap *m[3] = {nullptr, nullptr, nullptr};
+    for (int fi = 0; fi < nf && ok; fi++) {
+      m[fi] = get_map(c, signature(it.kind == 0 ? "p" : "uvw", it.cbc, ncbc, it.bc, it.is_correc ? 1 : 0, fi), ph,
+                      fi);
+      ok = ok && m[fi]->ok;
+    }
+    if (!ok) {
+      FZ_TRY(run_phases<T>(c, ph, (T *)it.f[0], (T *)it.f[1], (T *)it.f[2]));
+      continue;
+    }
+    for (int fi = 0; fi < nf; fi++) {
+      maps.push_back(m[fi]);
+      fields.push_back((T *)it.f[fi]);
+    }
+  }
+  if (maps.empty()) return FZ_OK;
+  return launch_gather<T>(c, (int)maps.size(), maps.data(), fields.data());
+}
+template int fz_bounds_group<double>(fz_ctx *, int, const FzBcItem *);
+template int fz_bounds_group<float>(fz_ctx *, int, const FzBcItem *);
 
 // FP32 fields of the mixed-precision step (same sequence and maps)
 int fz_boundp_f32(fz_ctx *c, const char cbc[6], const double bc[6], float *p) {

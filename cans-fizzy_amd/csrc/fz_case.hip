@@ -110,6 +110,23 @@ static bool is_f32_field(int id) {
   return id == F_PSI || id == F_PHI || id == F_KAPPA || (id >= F_NX && id <= F_FZ) || id >= F_DPSIO;
 }
 
+// grouped boundary updates of the step (one gather launch per group of up
+// to four fields; each field's values as its own boundp/bounduvw call gives)
+template <typename T>
+static int bc_fluxes_psi(fz_ctx *c, const fz_params &P, T *fx, T *fy, T *fz, T *psi) {
+  const FzBcItem it[2] = {{1, P.cbcvel, P.bcvel, 0, {fx, fy, fz}}, {0, P.cbcpsi, P.bcpsi, 0, {psi, nullptr, nullptr}}};
+  return fz_bounds_group<T>(c, 2, it);
+}
+template <typename T>
+static int bc_geometry(fz_ctx *c, const fz_params &P, const char *cbccur, const double *bccur, T *kappa, T *nx, T *ny,
+                       T *nz) {
+  const FzBcItem it[4] = {{0, cbccur, bccur, 0, {kappa, nullptr, nullptr}},
+                          {0, P.cbcnor + 0, P.bcnor + 0, 0, {nx, nullptr, nullptr}},
+                          {0, P.cbcnor + 6, P.bcnor + 6, 0, {ny, nullptr, nullptr}},
+                          {0, P.cbcnor + 12, P.bcnor + 12, 0, {nz, nullptr, nullptr}}};
+  return fz_bounds_group<T>(c, 4, it);
+}
+
 __global__ void k_to_f32(long n, const double *__restrict__ a, float *__restrict__ b) {
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) b[q] = (float)a[q];
 }
@@ -313,20 +330,12 @@ static int phase_geometry(fz_case *cs) {
     if (P.vof) FZ_TRY(fz_vof_phi_f32(c, 1.0 / (2.0 * P.psi_thickness_factor), g[F_PSI], g[F_PHI]));
     else FZ_TRY(fz_acdi_cmpt_phi_f32(c, cs->sc[S_SEPS], g[F_PSI], g[F_PHI]));
     FZ_TRY(fz_youngs_f32(c, g[F_PHI], g[F_NX], g[F_NY], g[F_NZ], g[F_KAPPA]));
-    FZ_TRY(fz_boundp_f32(c, cs->cbccur, cs->bccur, g[F_KAPPA]));
-    FZ_TRY(fz_boundp_f32(c, P.cbcnor + 0, P.bcnor + 0, g[F_NX]));
-    FZ_TRY(fz_boundp_f32(c, P.cbcnor + 6, P.bcnor + 6, g[F_NY]));
-    FZ_TRY(fz_boundp_f32(c, P.cbcnor + 12, P.bcnor + 12, g[F_NZ]));
-    return FZ_OK;
+    return bc_geometry<float>(c, P, cs->cbccur, cs->bccur, g[F_KAPPA], g[F_NX], g[F_NY], g[F_NZ]);
   }
   if (!P.vof) FZ_TRY(fz_acdi_cmpt_phi(c, cs->sc[S_SEPS], f[F_PSI], f[F_PHI]));
   else FZ_TRY(fz_vof_thinc_cmpt_phi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_PSI], f[F_PHI]));
   FZ_TRY(fz_cmpt_norm_curv_youngs(c, f[F_PHI], f[F_NX], f[F_NY], f[F_NZ], f[F_KAPPA]));
-  FZ_TRY(fz_boundp(c, cs->cbccur, cs->bccur, f[F_KAPPA]));
-  FZ_TRY(fz_boundp(c, P.cbcnor + 0, P.bcnor + 0, f[F_NX]));
-  FZ_TRY(fz_boundp(c, P.cbcnor + 6, P.bcnor + 6, f[F_NY]));
-  FZ_TRY(fz_boundp(c, P.cbcnor + 12, P.bcnor + 12, f[F_NZ]));
-  return FZ_OK;
+  return bc_geometry<double>(c, P, cs->cbccur, cs->bccur, f[F_KAPPA], f[F_NX], f[F_NY], f[F_NZ]);
 }
 
 // chkdt + the dt selection of src/main.f90:396-404 and :532-550
@@ -476,16 +485,14 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
     FZ_LAUNCHED();
     FZ_TRY(fz_clip_f32(c, 0.0, 1.0, g[F_PSI]));
     swap_ptr(cs->dpsidt32, g[F_DPSIO]);
-    FZ_TRY(fz_bounduvw_f32(c, P.cbcvel, P.bcvel, 0, g[F_FX], g[F_FY], g[F_FZ]));
-    FZ_TRY(fz_boundp_f32(c, P.cbcpsi, P.bcpsi, g[F_PSI]));
+    FZ_TRY(bc_fluxes_psi<float>(c, P, g[F_FX], g[F_FY], g[F_FZ], g[F_PSI]));
     FZ_TRY(phase_geometry(cs));
   } else if (mixed) {  // the same sequence on the FP32 phase field
     FZ_TRY(fz_launch_acdi_step<float>(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS],
                                       factor1, factor2, a.stc, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], g[F_NX], g[F_NY],
                                       g[F_NZ], g[F_PHI], cs->psio32, g[F_DPSIO], g[F_PSI], g[F_FX], g[F_FY],
                                       g[F_FZ]));
-    FZ_TRY(fz_bounduvw_f32(c, P.cbcvel, P.bcvel, 0, g[F_FX], g[F_FY], g[F_FZ]));
-    FZ_TRY(fz_boundp_f32(c, P.cbcpsi, P.bcpsi, g[F_PSI]));
+    FZ_TRY(bc_fluxes_psi<float>(c, P, g[F_FX], g[F_FY], g[F_FZ], g[F_PSI]));
     FZ_TRY(phase_geometry(cs));
   } else if (P.is_track_interface) {
     if (!P.vof) {
@@ -505,8 +512,7 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
       FZ_TRY(fz_clip_field(c, mm, f[F_PSI]));  // src/rk.f90:248
       swap_ptr(cs->dpsidt, f[F_DPSIO]);         // src/rk.f90:252
     }
-    FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_FX], f[F_FY], f[F_FZ]));
-    FZ_TRY(fz_boundp(c, P.cbcpsi, P.bcpsi, f[F_PSI]));
+    FZ_TRY(bc_fluxes_psi<double>(c, P, f[F_FX], f[F_FY], f[F_FZ], f[F_PSI]));
     FZ_TRY(phase_geometry(cs));
   } else {
     FZ_HIP(hipMemcpyAsync(f[F_PSI], cs->psio, sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
@@ -556,8 +562,11 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
     swap_ptr(f[F_V], cs->vn);
     swap_ptr(f[F_W], cs->wn);
     swap_ptr(f[F_P], f[F_PO]);  // p <- p + dp, po <- p of the step before (= pn)
-    FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 1, f[F_U], f[F_V], f[F_W]));
-    FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, f[F_P]));
+    {
+      const FzBcItem it[2] = {{1, P.cbcvel, P.bcvel, 1, {f[F_U], f[F_V], f[F_W]}},
+                              {0, P.cbcpre, P.bcpre, 0, {f[F_P], nullptr, nullptr}}};
+      FZ_TRY(fz_bounds_group<double>(c, 2, it));
+    }
   } else {
     FZ_HIP(hipMemcpyAsync(f[F_PO], f[F_P], sizeof(double) * c->nh, hipMemcpyDeviceToDevice, c->st));
   }
