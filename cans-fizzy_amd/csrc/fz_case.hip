@@ -48,27 +48,16 @@ double fz_chkdt_finish(const fz_ctx *c, double dti, int is_solve_ns, int is_trac
 template <typename PT>
 int fz_bulk_mean_dev(fz_ctx *c, int idir, const PT *psi, const double p12[2], const double *gvr, double *d_out);
 int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi);
-int fz_vof_transport_f32(fz_ctx *c, double beta, const double *u, const double *v, const double *w,
-                         const float *normx, const float *normy, const float *normz, const float *phi, float *dpsidt,
-                         float *flux_x, float *flux_y, float *flux_z);
 int fz_vof_phi_f32(fz_ctx *c, double beta, const float *psi, float *phi);
-int fz_clip_f32(fz_ctx *c, double rmin, double rmax, float *p);
+template <typename T>
+int fz_launch_vof_step(fz_ctx *c, double beta, double factor1, double factor2, int first, const double *u,
+                       const double *v, const double *w, const T *normx, const T *normy, const T *normz,
+                       const T *phi, const T *psio, T *dpsio, T *psi_out, T *flux_x, T *flux_y, T *flux_z);
 int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float *normz, float *kappa);
 int fz_boundp_f32(fz_ctx *c, const char cbc[6], const double bc[6], float *p);
 int fz_bounduvw_f32(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, float *u, float *v, float *w);
 
 // out-of-place AB2 phase-field update (psi_out may alias psi_in)
-template <typename T>
-__global__ void k_rk_psi_oop(fz_dims d, double f1, double f2, const T *__restrict__ dpsidt,
-                             const T *__restrict__ dpsidto, const T *psi_in, T *psi_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > d.n1 || j > d.n2) return;
-  const long q = i + d.sy * j + d.sz * k;
-  const long r = (i - 1) + (long)d.n1 * ((j - 1) + (long)d.n2 * (k - 1));
-  psi_out[q] = psi_in[q] + (T)f1 * dpsidt[r] + (T)f2 * dpsidto[r];
-}
 
 enum { F_U, F_V, F_W, F_P, F_PN, F_PO, F_PSI, F_PHI, F_KAPPA, F_NX, F_NY, F_NZ, F_FX, F_FY, F_FZ, F_DPSIO, F_DUO,
        F_DVO, F_DWO, F_N };
@@ -78,7 +67,7 @@ struct fz_case {
   fz_params prm;
   fz_ctx *ctx = nullptr;
   double *f[F_N] = {nullptr};
-  double *psio = nullptr, *dpsidt = nullptr, *dp = nullptr;
+  double *psio = nullptr, *dp = nullptr;
   double *un = nullptr, *vn = nullptr, *wn = nullptr;  // the next velocity (rotated with u, v, w)
   double *dsc = nullptr;  // device scalars: 0 gam, 1-2 div, 3 adv dt, 4-6 rho_av
   unsigned long long *gslots = nullptr;  // maxima of the squared cell speed (gamma), zero between steps
@@ -93,7 +82,7 @@ struct fz_case {
   int precision = FZ_PREC_FP64;
   int nref = 1;
   float *g[F_N] = {nullptr};
-  float *psio32 = nullptr, *dpsidt32 = nullptr;
+  float *psio32 = nullptr;
   // per-step scalars in device memory and the captured step graphs
   double *dstc = nullptr;
   double stc_host[FZ_STC_N] = {0};
@@ -167,13 +156,6 @@ static int case_alloc(fz_case *cs) {
   if (mixed) {
     FZ_HIP(hipMalloc((void **)&cs->psio32, sizeof(float) * c->nh));
     FZ_HIP(hipMemsetAsync(cs->psio32, 0, sizeof(float) * c->nh, c->st));
-    if (cs->prm.vof) {
-      FZ_HIP(hipMalloc((void **)&cs->dpsidt32, sizeof(float) * c->ni));
-      FZ_HIP(hipMemsetAsync(cs->dpsidt32, 0, sizeof(float) * c->ni, c->st));
-    }
-  } else {
-    FZ_HIP(hipMalloc((void **)&cs->dpsidt, sizeof(double) * c->ni));
-    FZ_HIP(hipMemsetAsync(cs->dpsidt, 0, sizeof(double) * c->ni, c->st));
   }
   FZ_HIP(hipMalloc((void **)&cs->gslots, sizeof(unsigned long long) * FZ_GAMMA_SLOTS));
   FZ_HIP(hipMemsetAsync(cs->gslots, 0, sizeof(unsigned long long) * FZ_GAMMA_SLOTS, c->st));
@@ -202,11 +184,10 @@ extern "C" int fz_case_destroy(fz_case *cs) {
     if (cs->g[q]) (void)hipFree(cs->g[q]);
   }
   if (cs->psio32) (void)hipFree(cs->psio32);
-  if (cs->dpsidt32) (void)hipFree(cs->dpsidt32);
   if (cs->dstc) (void)hipFree(cs->dstc);
   for (int q = 0; q < 2; q++)
     if (cs->graph[q].exec) (void)hipGraphExecDestroy(cs->graph[q].exec);
-  double *o[10] = {cs->psio, cs->dpsidt, cs->un, cs->vn, cs->wn, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f,
+  double *o[10] = {cs->psio, nullptr, cs->un, cs->vn, cs->wn, cs->dp, cs->dsc, cs->gvr_c, cs->gvr_f,
                    (double *)cs->gslots};
   for (int q = 0; q < 10; q++)
     if (o[q]) (void)hipFree(o[q]);
@@ -469,22 +450,15 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
   float **g = cs->g;
   const bool mixed = cs->precision == FZ_PREC_MIXED;
   const double rho0 = cs->sc[S_RHO0];
-  const long bi = sizeof(double) * c->ni;
   const double dt = a.dt, dt_r = a.dt_r, factor1 = a.factor1, factor2 = a.factor2, factor12 = a.factor12;
   const bool is_first = a.first;
   // psio <- psi by rotation: the update writes into the older buffer
   swap_ptr(cs->psio, f[F_PSI]);
   swap_ptr(cs->psio32, g[F_PSI]);
   if (mixed && P.vof) {  // the VOF sequence below on the FP32 phase field
-    FZ_TRY(fz_vof_transport_f32(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_U], f[F_V], f[F_W], g[F_NX], g[F_NY],
-                                g[F_NZ], g[F_PHI], cs->dpsidt32, g[F_FX], g[F_FY], g[F_FZ]));
-    if (is_first)
-      FZ_HIP(hipMemcpyAsync(g[F_DPSIO], cs->dpsidt32, sizeof(float) * c->ni, hipMemcpyDeviceToDevice, c->st));
-    FZ_LAUNCH(c, "rk_psi_oop_f32", k_rk_psi_oop<float>, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1,
-              factor2, cs->dpsidt32, g[F_DPSIO], cs->psio32, g[F_PSI]);
-    FZ_LAUNCHED();
-    FZ_TRY(fz_clip_f32(c, 0.0, 1.0, g[F_PSI]));
-    swap_ptr(cs->dpsidt32, g[F_DPSIO]);
+    FZ_TRY(fz_launch_vof_step<float>(c, 1.0 / (2.0 * P.psi_thickness_factor), factor1, factor2, is_first ? 1 : 0,
+                                     f[F_U], f[F_V], f[F_W], g[F_NX], g[F_NY], g[F_NZ], g[F_PHI], cs->psio32,
+                                     g[F_DPSIO], g[F_PSI], g[F_FX], g[F_FY], g[F_FZ]));
     FZ_TRY(bc_fluxes_psi<float>(c, P, g[F_FX], g[F_FY], g[F_FZ], g[F_PSI]));
     FZ_TRY(phase_geometry(cs));
   } else if (mixed) {  // the same sequence on the FP32 phase field
@@ -501,16 +475,10 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
                                            factor1, factor2, a.stc, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ],
                                  f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ]));
     } else {
-      FZ_TRY(fz_vof_thinc_transport_psi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_U], f[F_V], f[F_W], f[F_NX],
-                                        f[F_NY], f[F_NZ], f[F_PHI], cs->psio, cs->dpsidt, f[F_FX], f[F_FY],
-                                        f[F_FZ]));
-      if (is_first) FZ_HIP(hipMemcpyAsync(f[F_DPSIO], cs->dpsidt, bi, hipMemcpyDeviceToDevice, c->st));
-      FZ_LAUNCH(c, "rk_psi_oop", k_rk_psi_oop<double>, fz_grid_interior(c->d), fz_block3(), 0, c->d, factor1, factor2,
-                cs->dpsidt, f[F_DPSIO], cs->psio, f[F_PSI]);
-      FZ_LAUNCHED();
-      const double mm[2] = {0.0, 1.0};
-      FZ_TRY(fz_clip_field(c, mm, f[F_PSI]));  // src/rk.f90:248
-      swap_ptr(cs->dpsidt, f[F_DPSIO]);         // src/rk.f90:252
+      // vof_thinc_transport_psi, the AB2 update and clip in one pass (src/rk.f90:226-252)
+      FZ_TRY(fz_launch_vof_step<double>(c, 1.0 / (2.0 * P.psi_thickness_factor), factor1, factor2,
+                                        is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ],
+                                        f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ]));
     }
     FZ_TRY(bc_fluxes_psi<double>(c, P, f[F_FX], f[F_FY], f[F_FZ], f[F_PSI]));
     FZ_TRY(phase_geometry(cs));

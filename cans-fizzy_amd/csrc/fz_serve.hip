@@ -44,13 +44,13 @@ extern "C" int fz_case_footprint_ex(const fz_params *prm, int precision, size_t 
   if (precision == FZ_PREC_MIXED) {
     b += alloc_bytes(8.0 * nh, 5 + 4);  // u, v, w, p, po + dp, three spare velocities
     b += alloc_bytes(4.0 * nh, 9 + 1);  // psi, phi, kappa, normals, fluxes + psio (FP32)
-    b += alloc_bytes(4.0 * ni, 4 + 1 + (prm->vof ? 1 : 0));  // AB2 histories, FP32 spectral buffer (+ VOF rhs)
+    b += alloc_bytes(4.0 * ni, 4 + 1);  // four AB2 histories + the FP32 spectral buffer
     b += alloc_bytes(8.0 * ni, periodic_z ? 3 : 2);  // FP64 Poisson work array and pivots (+ periodic z)
   } else {
     b += alloc_bytes(8.0 * nh, 14 + 5);  // case fields (pn aliases p) + psio, dp, three spare velocities
-    // four AB2 histories, the phase-field right-hand side, the Poisson work
-    // array and Thomas pivots (+ the periodic-z second solve)
-    b += alloc_bytes(8.0 * ni, 5 + (periodic_z ? 3 : 2));
+    // four AB2 histories, the Poisson work array and Thomas pivots (+ the
+    // periodic-z second solve)
+    b += alloc_bytes(8.0 * ni, 4 + (periodic_z ? 3 : 2));
   }
   b += 8.0 * (n1 * n2 + 3 * n3 + 2 * (n2 * n3 + n1 * n3 + n1 * n2));  // eigenvalues, tridiagonal, boundary terms
   const double planes = 2.0 * ((n1 + 2) * (n2 + 2) + (n1 + 2) * (n3 + 2) + (n2 + 2) * (n3 + 2));

@@ -418,6 +418,73 @@ __global__ void k_vof_flux(fz_dims d, double beta_d, const double *__restrict__ 
   flux_z[q] = vel_z * lfz;
 }
 
+// THINC/QQ face flux at face index f of the x (dir 0), y (1) or z (2)
+// direction, the expression of k_vof_flux
+template <typename T>
+__device__ __forceinline__ T vof_face_flux(const fz_dims &d, int dir, long f, T beta, const double *__restrict__ vel,
+                                           const T *__restrict__ nrm, const T *__restrict__ phi) {
+  const T vv = (T)vel[f];
+  const long up = dir == 0 ? 1 : (dir == 1 ? d.sy : d.sz);
+  long qq;
+  T hh;
+  if (vv > (T)0.0) { qq = f; hh = (T)0.5; } else { qq = f + up; hh = (T)-0.5; }
+  const T lf = (T)1.0 / ((T)1.0 + fz_exp((T)-2.0 * beta * (nrm[qq] * hh + phi[qq])));
+  return vv * lf;
+}
+
+// rk_2fl's THINC/QQ phase-field part in one pass (src/rk.f90:226-252 with
+// vof_thinc_transport_psi, src/vof_thinc_qq.f90:39-132): the three face
+// fluxes of the cell's + faces (stored) and - faces (recomputed with the
+// same expressions as the neighbour that stores them), the divergence, the
+// AB2 update psi = psio + f1*R + f2*R_old and the clip to [0,1]. The
+// history dpsio is read and replaced by R in place (on the first step the
+// history is R itself). Fluxes of the faces at index 0 are left to the
+// boundary update that follows, which sets every halo face.
+template <typename T>
+__global__ void __launch_bounds__(256) k_vof_step(fz_dims d, const double *__restrict__ dzfi, double dxi_d,
+                                                  double dyi_d, double beta_d, double factor1, double factor2,
+                                                  int first, const double *__restrict__ u,
+                                                  const double *__restrict__ v, const double *__restrict__ w,
+                                                  const T *__restrict__ normx, const T *__restrict__ normy,
+                                                  const T *__restrict__ normz, const T *__restrict__ phi,
+                                                  const T *__restrict__ psio, T *__restrict__ dpsio,
+                                                  T *__restrict__ psi_out, T *__restrict__ flux_x,
+                                                  T *__restrict__ flux_y, T *__restrict__ flux_z) {
+  CELL_IJK(d);
+  const T beta = (T)beta_d, dxi = (T)dxi_d, dyi = (T)dyi_d;
+  const T fxp = vof_face_flux<T>(d, 0, q, beta, u, normx, phi), fxm = vof_face_flux<T>(d, 0, q - 1, beta, u, normx, phi);
+  const T fyp = vof_face_flux<T>(d, 1, q, beta, v, normy, phi),
+          fym = vof_face_flux<T>(d, 1, q - d.sy, beta, v, normy, phi);
+  const T fzp = vof_face_flux<T>(d, 2, q, beta, w, normz, phi),
+          fzm = vof_face_flux<T>(d, 2, q - d.sz, beta, w, normz, phi);
+  flux_x[q] = fxp;
+  flux_y[q] = fyp;
+  flux_z[q] = fzp;
+  const T R = -(dxi * (fxp - fxm) + dyi * (fyp - fym) + (T)dzfi[k] * (fzp - fzm));
+  const T Ro = first ? R : dpsio[r];
+  dpsio[r] = R;
+  const T pn = psio[q] + (T)factor1 * R + (T)factor2 * Ro;
+  psi_out[q] = fmin(fmax((T)0.0, pn), (T)1.0);
+}
+
+template <typename T>
+int fz_launch_vof_step(fz_ctx *c, double beta, double factor1, double factor2, int first, const double *u,
+                       const double *v, const double *w, const T *normx, const T *normy, const T *normz,
+                       const T *phi, const T *psio, T *dpsio, T *psi_out, T *flux_x, T *flux_y, T *flux_z) {
+  FZ_LAUNCH(c, sizeof(T) == 4 ? "vof_step_f32" : "vof_step", k_vof_step<T>, fz_grid_interior(c->d), fz_block3(), 0,
+            c->d, c->dzfi, c->dli[0], c->dli[1], beta, factor1, factor2, first, u, v, w, normx, normy, normz, phi,
+            psio, dpsio, psi_out, flux_x, flux_y, flux_z);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+template int fz_launch_vof_step<double>(fz_ctx *, double, double, double, int, const double *, const double *,
+                                        const double *, const double *, const double *, const double *,
+                                        const double *, const double *, double *, double *, double *, double *,
+                                        double *);
+template int fz_launch_vof_step<float>(fz_ctx *, double, double, double, int, const double *, const double *,
+                                       const double *, const float *, const float *, const float *, const float *,
+                                       const float *, float *, float *, float *, float *, float *);
+
 // flux divergence, src/vof_thinc_qq.f90:123-132
 template <typename T>
 __global__ void k_vof_div(fz_dims d, const double *__restrict__ dzfi, double dxi_d, double dyi_d,
@@ -1071,24 +1138,12 @@ extern "C" int fz_vof_thinc_transport_psi(fz_ctx *c, double beta, const double *
   return vof_transport<double>(c, beta, u, v, w, normx, normy, normz, phi, dpsidt, flux_x, flux_y, flux_z);
 }
 
-// the FP32 phase field of the mixed-precision step
-int fz_vof_transport_f32(fz_ctx *c, double beta, const double *u, const double *v, const double *w,
-                         const float *normx, const float *normy, const float *normz, const float *phi, float *dpsidt,
-                         float *flux_x, float *flux_y, float *flux_z) {
-  return vof_transport<float>(c, beta, u, v, w, normx, normy, normz, phi, dpsidt, flux_x, flux_y, flux_z);
-}
-
 int fz_vof_phi_f32(fz_ctx *c, double beta, const float *psi, float *phi) {
   FZ_LAUNCH(c, "vof_phi_f32", k_vof_phi<float>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, beta, psi, phi);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 
-int fz_clip_f32(fz_ctx *c, double rmin, double rmax, float *p) {
-  FZ_LAUNCH(c, "clip_f32", k_clip<float>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, rmin, rmax, p);
-  FZ_LAUNCHED();
-  return FZ_OK;
-}
 
 extern "C" int fz_clip_field(fz_ctx *c, const double minmax[2], double *p) {
   if (!c || !minmax || !p) return fz_set_error(FZ_ERR_ARG, "fz_clip_field: null argument");
