@@ -224,6 +224,20 @@ def batched_poisson(F, np, k=16, n=64, rounds=20):
             c.close()
 
 
+def cpu_baseline_one_core(workload):
+    """The same oracle timing on one thread (a separate process, since the
+    OpenMP thread count is fixed at start-up), over one AB2 step."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload", workload],
+                           env=env, capture_output=True, text=True, timeout=300)
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception as e:  # reported, not fatal: the GPU figures stand without it
+        log(f"one-core baseline not measured: {e}")
+        return None
+
+
 def main_poisson(args):
     """The standalone Poisson solve: 512^3, periodic x/y, Neumann z, a
     right-hand side seeded uniform in [-1, 1] with its (volume-weighted)
@@ -349,6 +363,7 @@ def main():
     ap.add_argument("--graphs", action="store_true", help="replay the steps from captured HIP graphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="c4: precision of the cases")
+    ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -356,6 +371,11 @@ def main():
               f"(asked for --gpus {args.gpus}, WORLD_SIZE {world}); nothing measured.", file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_baseline_only:  # the one-core oracle timing, run as a child process
+        from fizzy_amd import cases
+        spec = {"c2": cases.config2, "c0": cases.config0, "c3": cases.config3}[args.workload]()
+        print(json.dumps(cpu_baseline(spec, budget_s=10.0, max_steps=1)), flush=True)
+        return
     if args.workload == "c4":
         return main_serving(args)
     if args.workload == "c1":
@@ -422,6 +442,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec)
+        cpu["one_core"] = cpu_baseline_one_core(args.workload)
     out = {
         "metric": "Mcell-steps/s per RK3 step, FP64 vs mixed; ensemble p50/p99 step latency",
         "value": round(value, 2),

@@ -804,6 +804,99 @@ __global__ void k_src_lines(fz_dims d, FillSrc fs, T *__restrict__ W) {
   }
 }
 
+// k_thomas_periodic_pc for FP32 columns, two adjacent columns per lane (the
+// per-column arithmetic of k_thomas_periodic_pc<TK, float>; even n1*n2)
+template <int TK>
+__global__ void __launch_bounds__(64) k_thomas_periodic_x2(float *__restrict__ W, double *__restrict__ D,
+                                                           const double *__restrict__ P2,
+                                                           const double *__restrict__ lambdaxy,
+                                                           const double *__restrict__ a, const double *__restrict__ b,
+                                                           const double *__restrict__ c, int n1, int n2, int n3,
+                                                           long gauge) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;  // column pair
+  const long ps2 = (long)n1 * n2 / 2;
+  if (idx >= ps2) return;
+  float2 *col = (float2 *)W + idx;
+  double2 *dcol = (double2 *)D + idx;
+  const double2 *p2 = (const double2 *)P2 + idx;
+  const double lam0 = lambdaxy[2 * idx], lam1 = lambdaxy[2 * idx + 1];
+  const int m = n3 - 1;
+  double d0 = 0.0, d1 = 0.0, p0 = 0.0, p1 = 0.0;
+  for (int l0 = 0; l0 < m; l0 += TK) {
+    float2 r[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < m) r[t] = col[(long)(l0 + t) * ps2];
+#pragma unroll
+    for (int t = 0; t < TK; t++) {
+      const int l = l0 + t;
+      if (l < m) {
+        const double bb0 = b[l] + lam0, bb1 = b[l] + lam1;
+        double z0, z1;
+        if (l == 0) {
+          z0 = 1.0 / (bb0 + FZ_EPS);
+          z1 = 1.0 / (bb1 + FZ_EPS);
+          d0 = c[0] * z0;
+          d1 = c[0] * z1;
+          p0 = (double)r[t].x * z0;
+          p1 = (double)r[t].y * z1;
+        } else {
+          z0 = 1.0 / (bb0 - a[l] * d0 + FZ_EPS);
+          z1 = 1.0 / (bb1 - a[l] * d1 + FZ_EPS);
+          d0 = c[l] * z0;
+          d1 = c[l] * z1;
+          p0 = ((double)r[t].x - a[l] * p0) * z0;
+          p1 = ((double)r[t].y - a[l] * p1) * z1;
+        }
+        dcol[(long)l * ps2] = make_double2(d0, d1);
+        col[(long)l * ps2] = make_float2((float)p0, (float)p1);
+      }
+    }
+  }
+  double q0 = p0, q1 = p1;
+  for (int h0 = m - 2; h0 >= 0; h0 -= TK) {
+    float2 r[TK];
+    double2 dd[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        r[t] = col[(long)(h0 - t) * ps2];
+        dd[t] = dcol[(long)(h0 - t) * ps2];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        q0 = (double)r[t].x - dd[t].x * q0;
+        q1 = (double)r[t].y - dd[t].y * q1;
+        col[(long)(h0 - t) * ps2] = make_float2((float)q0, (float)q1);
+      }
+  }
+  const float2 cm = col[(long)m * ps2], c0 = col[0], cm1 = col[(long)(m - 1) * ps2];
+  const double2 p2f = p2[0], p2l = p2[(long)(m - 1) * ps2];
+  double pn0 = ((double)cm.x - c[m] * (double)c0.x - a[m] * (double)cm1.x) /
+               (b[m] + lam0 + c[m] * p2f.x + a[m] * p2l.x + FZ_EPS);
+  double pn1 = ((double)cm.y - c[m] * (double)c0.y - a[m] * (double)cm1.y) /
+               (b[m] + lam1 + c[m] * p2f.y + a[m] * p2l.y + FZ_EPS);
+  if (2 * idx == gauge) pn0 = 0.0;
+  if (2 * idx + 1 == gauge) pn1 = 0.0;
+  col[(long)m * ps2] = make_float2((float)pn0, (float)pn1);
+  for (int l0 = 0; l0 < m; l0 += TK) {
+    float2 r[TK];
+    double2 q[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < m) {
+        r[t] = col[(long)(l0 + t) * ps2];
+        q[t] = p2[(long)(l0 + t) * ps2];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < m)
+        col[(long)(l0 + t) * ps2] = make_float2((float)((double)r[t].x + q[t].x * pn0),
+                                                (float)((double)r[t].y + q[t].y * pn1));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side: operator set-up
 // ---------------------------------------------------------------------------
@@ -1175,9 +1268,13 @@ static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum) {
   } else {
     (void)nb;
     (void)tpb;
-    FZ_LAUNCH(ctx, mixed ? "thomas_periodic_f32" : "thomas_periodic", (k_thomas_periodic_pc<16, T>),
-              dim3((int)((ps + 63) / 64)), dim3(64), 0, W, P->piv, P->aux, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
-              gauge);
+    if (mixed && ctx->tk32 < 0 && ps % 2 == 0)  // two FP32 columns per lane
+      FZ_LAUNCH(ctx, "thomas_periodic_f32", k_thomas_periodic_x2<16>, dim3((int)((ps / 2 + 63) / 64)), dim3(64), 0,
+                (float *)(void *)W, P->piv, P->aux, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3, gauge);
+    else
+      FZ_LAUNCH(ctx, mixed ? "thomas_periodic_f32" : "thomas_periodic", (k_thomas_periodic_pc<16, T>),
+                dim3((int)((ps + 63) / 64)), dim3(64), 0, W, P->piv, P->aux, P->lambdaxy, P->a, P->b, P->c, n1, n2,
+                n3, gauge);
   }
   FZ_LAUNCHED();
   FZ_TRY((launch_r2r<T, T, T>(ctx, mixed ? "fft_y_bwd_f32" : "fft_y_bwd", P->py, P->py.kind_b, W, w_y, W, w_y, n1, n3,

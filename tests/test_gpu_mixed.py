@@ -75,18 +75,20 @@ def test_mixed_solver_refines_to_fp64(n, cbc, gr, rng):
     assert err <= max(1e-10, fp64_acc)
 
 
+@pytest.mark.parametrize("cbc", ["PPPPNN", "PPPPPP"])
 @pytest.mark.parametrize("rows", ["8", "16", "-8", "-16"])
-def test_thomas_variants_agree_bitwise(rows, monkeypatch):
+def test_thomas_variants_agree_bitwise(rows, cbc, monkeypatch):
     """Every FP32 Thomas variant (rows per chunk, one or two columns per
-    lane; a context reads FZ_THOMAS_ROWS32 when it is created) gives the
-    default variant's solution bit for bit."""
+    lane; a context reads FZ_THOMAS_ROWS32 when it is created), for Neumann
+    and periodic z, gives the default variant's solution bit for bit."""
     out = []
     for env in (None, rows):
         if env is None:
             monkeypatch.delenv("FZ_THOMAS_ROWS32", raising=False)
         else:
             monkeypatch.setenv("FZ_THOMAS_ROWS32", env)
-        it, rr, res, err, acc = solve_pair([64, 64, 32], "PPPPNN", 0.6, np.random.default_rng(7), 1e-13, 2)
+        it, rr, res, err, acc = solve_pair([64, 64, 32], cbc, 0.6 if cbc == "PPPPNN" else 0.0,
+                                           np.random.default_rng(7), 1e-13, 2)
         out.append((it, rr, err))
     assert out[0] == out[1]
 
