@@ -171,26 +171,27 @@ __device__ inline void stockham_pass(const typename CT<T>::C *__restrict__ X, ty
   }
 }
 
-// complex input k of packed line f, from the real lines staged in Rw (n values each)
+// complex input k of packed line f, from the real lines staged in Rw (n values
+// each, rows rs apart)
 template <typename R>
 __device__ inline typename CT<R>::C pre_value(int kind, const R *__restrict__ Rw, int f, int k, int n,
-                                              const typename CT<R>::C *__restrict__ q) {
+                                              const typename CT<R>::C *__restrict__ q, int rs) {
   using C = typename CT<R>::C;
   C z;
   switch (kind) {
-    case K_R2HC: z = mkc(Rw[(2 * f) * n + k], Rw[(2 * f + 1) * n + k]); break;
+    case K_R2HC: z = mkc(Rw[(2 * f) * rs + k], Rw[(2 * f + 1) * rs + k]); break;
     case K_REDFT10:
     case K_RODFT10: {
       const int e = (k <= (n - 1) / 2) ? 2 * k : 2 * (n - 1 - k) + 1;
       const R sg = (kind == K_RODFT10 && (e & 1)) ? (R)-1.0 : (R)1.0;
-      z = mkc(sg * Rw[(2 * f) * n + e], sg * Rw[(2 * f + 1) * n + e]);
+      z = mkc(sg * Rw[(2 * f) * rs + e], sg * Rw[(2 * f + 1) * rs + e]);
       break;
     }
     case K_REDFT11:
     case K_RODFT11: {
       if (k < n) {
         const R sg = (kind == K_RODFT11 && (k & 1)) ? (R)-1.0 : (R)1.0;
-        const R x = sg * Rw[f * n + k];
+        const R x = sg * Rw[f * rs + k];
         z = mkc(x * q[k].x, -x * q[k].y);
       } else {
         z = mkc((R)0.0, (R)0.0);
@@ -198,7 +199,7 @@ __device__ inline typename CT<R>::C pre_value(int kind, const R *__restrict__ Rw
       break;
     }
     case K_HC2R: {
-      const R *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
+      const R *Ha = Rw + (2 * f) * rs, *Hb = Rw + (2 * f + 1) * rs;
       R ra, ia, rb, ib;
       if (k == 0) { ra = Ha[0]; ia = 0.0; rb = Hb[0]; ib = 0.0; }
       else if (2 * k < n) { ra = Ha[k]; ia = Ha[n - k]; rb = Hb[k]; ib = Hb[n - k]; }
@@ -209,7 +210,7 @@ __device__ inline typename CT<R>::C pre_value(int kind, const R *__restrict__ Rw
     }
     case K_REDFT01:
     case K_RODFT01: {
-      const R *Ha = Rw + (2 * f) * n, *Hb = Rw + (2 * f + 1) * n;
+      const R *Ha = Rw + (2 * f) * rs, *Hb = Rw + (2 * f + 1) * rs;
       R xa, xna, xb, xnb;
       if (kind == K_REDFT01) {
         xa = Ha[k]; xb = Hb[k];
@@ -231,17 +232,17 @@ __device__ inline typename CT<R>::C pre_value(int kind, const R *__restrict__ Rw
   return z;
 }
 
-// real output k of line g from the transformed packed lines in cur (length m each)
-template <typename R>
-__device__ inline R post_value(int kind, const typename CT<R>::C *__restrict__ cur, int g, int k, int n, int m,
+// real output k of line g from the transformed packed lines (length m each;
+// Z(row, c) is complex c of row `row`)
+template <typename R, class ZA>
+__device__ inline R post_value(int kind, const ZA &Z, int g, int k, int n, int m,
                                const typename CT<R>::C *__restrict__ q, const typename CT<R>::C *__restrict__ t2) {
   using C = typename CT<R>::C;
   R val;
   switch (kind) {
     case K_R2HC: {
-      const C *Z = cur + (long)(g >> 1) * m;
       const int kk = (2 * k <= n) ? k : n - k;
-      const C zk = Z[kk], znk = Z[(m - kk) % m];
+      const C zk = Z(g >> 1, kk), znk = Z(g >> 1, (m - kk) % m);
       R re, im;
       if ((g & 1) == 0) { re = (R)0.5 * (zk.x + znk.x); im = (R)0.5 * (zk.y - znk.y); }
       else { re = (R)0.5 * (zk.y + znk.y); im = (R)0.5 * (znk.x - zk.x); }
@@ -250,9 +251,8 @@ __device__ inline R post_value(int kind, const typename CT<R>::C *__restrict__ c
     }
     case K_REDFT10:
     case K_RODFT10: {
-      const C *Z = cur + (long)(g >> 1) * m;
       const int kk = (kind == K_REDFT10) ? k : n - 1 - k;
-      const C zk = Z[kk], znk = Z[(m - kk) % m];
+      const C zk = Z(g >> 1, kk), znk = Z(g >> 1, (m - kk) % m);
       R re, im;
       if ((g & 1) == 0) { re = (R)0.5 * (zk.x + znk.x); im = (R)0.5 * (zk.y - znk.y); }
       else { re = (R)0.5 * (zk.y + znk.y); im = (R)0.5 * (znk.x - zk.x); }
@@ -262,19 +262,19 @@ __device__ inline R post_value(int kind, const typename CT<R>::C *__restrict__ c
     case K_REDFT11:
     case K_RODFT11: {
       const int kk = (kind == K_REDFT11) ? k : n - 1 - k;
-      const C zk = cur[(long)g * m + kk];
+      const C zk = Z(g, kk);
       val = (R)2.0 * (t2[kk].x * zk.x + t2[kk].y * zk.y);
       break;
     }
     case K_HC2R: {
-      const C zk = cur[(long)(g >> 1) * m + k];
+      const C zk = Z(g >> 1, k);
       val = (g & 1) ? zk.y : zk.x;
       break;
     }
     case K_REDFT01:
     case K_RODFT01: {
       const int srcm = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
-      const C zk = cur[(long)(g >> 1) * m + srcm];
+      const C zk = Z(g >> 1, srcm);
       val = (g & 1) ? zk.y : zk.x;
       if (kind == K_RODFT01 && (k & 1)) val = -val;
       break;
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(256) k_r2r(const TI *__restrict__ src, LineMap
   const int totc = nfft * m;
   for (int idx = threadIdx.x; idx < totc; idx += blockDim.x) {
     const int f = idx / m, k = idx - f * m;
-    A[idx] = pre_value<R>(kind, Rw, f, k, n, q);
+    A[idx] = pre_value<R>(kind, Rw, f, k, n, q, n);
   }
   __syncthreads();
   // 3. complex FFT passes
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(256) k_r2r(const TI *__restrict__ src, LineMap
   R *O = (R *)oth;
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int g = idx / n, k = idx - g * n;
-    O[idx] = post_value<R>(kind, cur, g, k, n, m, q, t2);
+    O[idx] = post_value<R>(kind, [&](int row, int c) { return cur[(long)row * m + c]; }, g, k, n, m, q, t2);
   }
   __syncthreads();
   // 5. store
@@ -381,11 +381,27 @@ struct P2Shape {
   static constexpr int G = G64;
   static constexpr int NFFT = QUARTER ? G : G / 2;
   static constexpr int NT = NFFT * (M / 4);
-  static constexpr size_t LDS = (size_t)NFFT * M * sizeof(typename CT<R>::C);
+  // LDS rows, padded against bank conflicts: one complex every 128 bytes
+  // (SH: log2 of the complex values per 128 bytes), and one real per staged
+  // line, so that strided lines stage without bank conflicts
+  static constexpr int SH = sizeof(typename CT<R>::C) == 16 ? 3 : 4;
+  static constexpr int MP = M + (M >> SH);  // complex row stride
+  static constexpr int RS = N + 1;          // real row stride
+  static constexpr size_t LDSC = (size_t)NFFT * MP * sizeof(typename CT<R>::C);
+  static constexpr size_t LDSR = (size_t)G * RS * sizeof(R);
+  static constexpr size_t LDS = LDSC > LDSR ? LDSC : LDSR;
 };
+// padded position of complex c in an LDS row
+template <int SH>
+__device__ __forceinline__ int lds_pad(int c) {
+  return c + (c >> SH);
+}
 
-template <int M, bool QUARTER, typename R, typename TI, typename TO>
-__global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
+// (WPE: the waves per SIMD its registers must allow; 8 for array sources, a
+// level the LDS allows too, 1 (no cap) for computed sources, whose loads
+// need more registers)
+template <int M, bool QUARTER, typename R, typename TI, typename TO, int WPE>
+__global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT)) __attribute__((amdgpu_waves_per_eu(WPE)))
     k_r2r_p2(const TI *__restrict__ src, LineMap in, TO *__restrict__ dst,
                                                 LineMap out, int L1, int kind, double scale, int accum,
                                                 const typename CT<R>::C *__restrict__ tw,
@@ -394,14 +410,14 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
                                                 Batch bt) {
   using S = P2Shape<M, QUARTER, R>;
   using C = typename CT<R>::C;
-  constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4;
+  constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4, SH = S::SH, MP = S::MP, RS = S::RS;
   if (bt.src) {  // case blockIdx.z of a batch of same-shaped solves
     src = (const TI *)bt.src[blockIdx.z];
     dst = (TO *)bt.dst[blockIdx.z];
   }
   extern __shared__ double2 smem_raw[];
   C *smem = (C *)smem_raw;
-  R *Rw = (R *)smem;  // G*N staged reals, overlaid by the complex lines
+  R *Rw = (R *)smem;  // G staged real lines (RS apart), overlaid by the complex lines (MP apart)
   const int tid = threadIdx.x;
   const int l1_0 = blockIdx.x * G;
   const long l2 = blockIdx.y;
@@ -425,24 +441,24 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
         val = (R)src[a];
       }
     }
-    Rw[g * N + e] = val;
+    Rw[g * RS + e] = val;
   }
   if (fs.rmax) slots_max(fs.rmax, vmax, (blockIdx.x + gridDim.x * blockIdx.y) * (NT / 64) + tid / 64);
   __syncthreads();
   const int f = tid / Q4, b = tid - f * Q4;
   C v[4];
 #pragma unroll
-  for (int t = 0; t < 4; t++) v[t] = pre_value<R>(kind, Rw, f, b + t * Q4, N, q);
+  for (int t = 0; t < 4; t++) v[t] = pre_value<R>(kind, Rw, f, b + t * Q4, N, q, RS);
   __syncthreads();
   const R sign = kind_is_forward(kind) ? (R)-1.0 : (R)1.0;
-  C *Z = smem + (long)f * M;
+  C *Z = smem + (long)f * MP;
   // 2. radix-4 passes
 #pragma unroll
   for (int s = 0; s < S::NP4; s++) {
     const int Ns = 1 << (2 * s);
     if (s > 0) {
 #pragma unroll
-      for (int t = 0; t < 4; t++) v[t] = Z[b + t * Q4];
+      for (int t = 0; t < 4; t++) v[t] = Z[lds_pad<SH>(b + t * Q4)];
       __syncthreads();
     }
     const int k = b & (Ns - 1);
@@ -458,13 +474,13 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
     }
     bfly4(v, sign);
 #pragma unroll
-    for (int t = 0; t < 4; t++) Z[base + t * Ns] = v[t];
+    for (int t = 0; t < 4; t++) Z[lds_pad<SH>(base + t * Ns)] = v[t];
     __syncthreads();
   }
   // 3. closing radix-2 pass (Ns = M/2): butterflies b and b + M/4
   if (S::HAS2) {
 #pragma unroll
-    for (int t = 0; t < 4; t++) v[t] = Z[b + t * Q4];
+    for (int t = 0; t < 4; t++) v[t] = Z[lds_pad<SH>(b + t * Q4)];
     __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -475,8 +491,8 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
         if (sign > 0) w.y = -w.y;
         x1 = cmul(x1, w);
       }
-      Z[k] = cadd(x0, x1);
-      Z[k + M / 2] = csub(x0, x1);
+      Z[lds_pad<SH>(k)] = cadd(x0, x1);
+      Z[lds_pad<SH>(k + M / 2)] = csub(x0, x1);
     }
     __syncthreads();
   }
@@ -487,7 +503,8 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
   for (int r = 0; r < PER; r++) {
     const int idx = tid + r * NT;
     const int g = idx / N, k = idx - g * N;
-    val[r] = post_value<R>(kind, smem, g, k, N, M, q, t2);
+    val[r] = post_value<R>(kind, [&](int row, int c) { return smem[row * MP + lds_pad<SH>(c)]; }, g, k, N, M, q,
+                           t2);
   }
   if (out.se == 1) {
 #pragma unroll
@@ -500,14 +517,19 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT))
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < PER; r++) Rw[tid + r * NT] = val[r];
+  for (int r = 0; r < PER; r++) {
+    const int idx = tid + r * NT;
+    const int g = idx / N, k = idx - g * N;
+    Rw[g * RS + k] = val[r];
+  }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < PER; r++) {
     const int idx = tid + r * NT;
     const int e = idx / G, g = idx - e * G;
     if (g < gcount)
-      store_out(dst, out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se, Rw[g * N + e], scale, accum);
+      store_out(dst, out.base + (long)(l1_0 + g) * out.s1 + l2 * out.s2 + (long)e * out.se, Rw[g * RS + e], scale,
+                accum);
   }
 }
 
@@ -526,8 +548,12 @@ static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int ki
   using S = P2Shape<M, QUARTER, R>;
   dim3 grid((L1 + S::G - 1) / S::G, L2, bt.src ? bt.n : 1);
   const PlanTabs<R> t = tabs_of(P, R());
-  FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind,
-            scale, accum, t.tw, t.q, t.t2, fs, bt);
+  if (fs.on)
+    FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, 1>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind,
+              scale, accum, t.tw, t.q, t.t2, fs, bt);
+  else
+    FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, 8>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind,
+              scale, accum, t.tw, t.q, t.t2, fs, bt);
 }
 
 static const Batch kNoBatch = {nullptr, nullptr, 0};
@@ -654,6 +680,108 @@ __global__ void __launch_bounds__(64) k_thomas(T *__restrict__ W, double *__rest
   if (idx >= (long)n1 * n2) return;
   const long ps = (long)n1 * n2;
   thomas_chunked<TK, T>(W + idx, D + idx, ps, n3, lambdaxy[idx], a, b, c, idx == gauge);
+}
+
+// k_thomas without a stored pivot array. A first sweep runs the forward
+// elimination and keeps only its state (the last pivot d and eliminated
+// value) at the end of every chunk of TK rows, in CK (two FP64 values per
+// chunk and column). The backward sweep takes the chunks from the last:
+// it re-runs a chunk's elimination from the saved state, with the same
+// operations and so the same values, holds that chunk's pivots and
+// eliminated values in registers and substitutes back. A column is read
+// twice and written once (about 25 bytes per FP64 cell), against three
+// reads and three writes (48 bytes) with a pivot array. The values are
+// k_thomas<TK, T>'s bit for bit: the eliminated values are rounded to T
+// where k_thomas stores and reloads them, and the last row keeps its FP64
+// value as the start of the substitution, as there.
+template <int TK, typename T>
+__global__ void __launch_bounds__(64) k_thomas_rc(T *__restrict__ W, double *__restrict__ CK,
+                                                  const double *__restrict__ lambdaxy, const double *__restrict__ a,
+                                                  const double *__restrict__ b, const double *__restrict__ c, int n1,
+                                                  int n2, int n3, long gauge, Batch bt) {
+  if (bt.src) {  // case blockIdx.y of a batch: its spectral array and checkpoint area
+    W = (T *)bt.src[blockIdx.y];
+    CK = (double *)bt.dst[blockIdx.y];
+  }
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n1 * n2) return;
+  const long ps = (long)n1 * n2;
+  T *col = W + idx;
+  double *ck = CK + idx;
+  const double lam = lambdaxy[idx];
+  const bool pin = idx == gauge;
+  const int nch = (n3 + TK - 1) / TK;
+  double d = 0.0, pprev = 0.0;
+  for (int ch = 0; ch < nch; ch++) {
+    const int l0 = ch * TK;
+    double r[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < n3) r[t] = (double)col[(long)(l0 + t) * ps];
+#pragma unroll
+    for (int t = 0; t < TK; t++) {
+      const int l = l0 + t;
+      if (l < n3) {
+        const double bb = b[l] + lam;
+        double z;
+        if (l == 0) {
+          z = 1.0 / (bb + FZ_EPS);
+          d = c[0] * z;
+          pprev = r[t] * z;
+        } else {
+          z = 1.0 / (bb - a[l] * d + FZ_EPS);
+          d = c[l] * z;
+          pprev = (r[t] - a[l] * pprev) * z;
+        }
+        if (pin && l == n3 - 1) pprev = 0.0;
+      }
+    }
+    if (ch + 1 < nch) {
+      ck[(long)(2 * ch) * ps] = d;
+      ck[(long)(2 * ch + 1) * ps] = pprev;
+    }
+  }
+  double pnext = 0.0;
+  for (int ch = nch - 1; ch >= 0; ch--) {
+    const int l0 = ch * TK;
+    double dd = 0.0, pp = 0.0;
+    if (ch > 0) {
+      dd = ck[(long)(2 * ch - 2) * ps];
+      pp = ck[(long)(2 * ch - 1) * ps];
+    }
+    double r[TK], dv[TK], pv[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < n3) r[t] = (double)col[(long)(l0 + t) * ps];
+#pragma unroll
+    for (int t = 0; t < TK; t++) {
+      const int l = l0 + t;
+      if (l < n3) {
+        const double bb = b[l] + lam;
+        double z;
+        if (l == 0) {
+          z = 1.0 / (bb + FZ_EPS);
+          dd = c[0] * z;
+          pp = r[t] * z;
+        } else {
+          z = 1.0 / (bb - a[l] * dd + FZ_EPS);
+          dd = c[l] * z;
+          pp = (r[t] - a[l] * pp) * z;
+        }
+        if (pin && l == n3 - 1) pp = 0.0;
+        dv[t] = dd;
+        pv[t] = l == n3 - 1 ? pp : (double)(T)pp;
+      }
+    }
+#pragma unroll
+    for (int t = TK - 1; t >= 0; t--) {
+      const int l = l0 + t;
+      if (l < n3) {
+        pnext = l == n3 - 1 ? pv[t] : pv[t] - dv[t] * pnext;
+        col[(long)l * ps] = (T)pnext;
+      }
+    }
+  }
 }
 
 // k_thomas for FP32 columns, two adjacent columns per lane: each load moves
@@ -1252,7 +1380,12 @@ static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum) {
     const int tk = mixed ? ctx->tk32 : ctx->tk64;
     const dim3 grid((int)((ps + 63) / 64)), blk(64);
     const char *name = mixed ? "thomas_f32" : "thomas";
-    if (mixed && tk < 0 && ps % 2 == 0) {  // two FP32 columns per lane (tk = -rows per chunk)
+    if (ctx->thomas_rc && tk > 0) {  // no pivot array: the elimination is re-run per chunk
+      if (tk == 8) FZ_LAUNCH(ctx, name, (k_thomas_rc<8, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c,
+                             n1, n2, n3, gauge, kNoBatch);
+      else FZ_LAUNCH(ctx, name, (k_thomas_rc<16, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
+                     n3, gauge, kNoBatch);
+    } else if (mixed && tk < 0 && ps % 2 == 0) {  // two FP32 columns per lane (tk = -rows per chunk)
       const dim3 g2((int)((ps / 2 + 63) / 64));
       float *W32 = (float *)(void *)W;
       if (tk == -8) FZ_LAUNCH(ctx, name, k_thomas_x2<8>, g2, blk, 0, W32, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,

@@ -5,7 +5,9 @@
 //  - tile_y: rows of blocks per z-marching pencil in the stencil block order;
 //  - the variant of the fused momentum kernel (occupancy, staged loads);
 //  - rows per chunk of the Thomas kernel (FP64, or FP32 in mixed cases,
-//    where two columns per lane are a candidate too).
+//    where two columns per lane are a candidate too);
+//  - for a non-periodic z, the Thomas kernel without a pivot array (it
+//    reads a column twice instead of storing and reloading pivots).
 // A scratch case with the same parameters (dt checks off, so every trial
 // times the same work) is stepped once per candidate to warm up and then
 // twice `trial_steps` times between HIP events on its stream (the faster
@@ -22,7 +24,7 @@
 
 namespace {
 struct Tuned {
-  int tile_y, mom_variant, tk;
+  int tile_y, mom_variant, tk, rc;
   float ms;
 };
 std::map<std::string, Tuned> g_tuned;
@@ -38,6 +40,7 @@ void apply(fz_ctx *c, const Tuned &t, bool mixed) {
   c->mom_variant = t.mom_variant;
   if (mixed) c->tk32 = t.tk;
   else c->tk64 = t.tk;
+  c->thomas_rc = t.rc;
 }
 
 // ms per step of the scratch case under t: the better of two timed runs
@@ -84,17 +87,21 @@ extern "C" int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size
     if (rc == FZ_OK) rc = fz_case_init(sc);
     if (rc == FZ_OK && hipEventCreate(&a) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
     if (rc == FZ_OK && hipEventCreate(&b) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
-    best = {c ? c->d.tile_y : 8, c ? c->mom_variant : 0, mixed ? (c ? c->tk32 : 32) : (c ? c->tk64 : 16), 0.0f};
+    best = {c ? c->d.tile_y : 8, c ? c->mom_variant : 0, mixed ? (c ? c->tk32 : 32) : (c ? c->tk64 : 16),
+            c ? c->thomas_rc : 0, 0.0f};
     if (rc == FZ_OK) rc = trial(sc, c, best, mixed, trial_steps, a, b, &best.ms);
     const int tiles[] = {0, 2, 4, 8, 16};
     const int moms[] = {0, 1, 4, 5};
     const int tks[] = {8, 16, 32, -8, -16};  // negative: two FP32 columns per lane (mixed cases)
-    for (int dim = 0; dim < 3 && rc == FZ_OK; dim++) {
-      const int *cand = dim == 0 ? tiles : (dim == 1 ? moms : tks);
-      const int ncand = dim == 0 ? 5 : (dim == 1 ? 4 : (mixed ? 5 : 3));
+    const int rcs[] = {0, 1};
+    const bool periodic_z = prm.cbcpre[4] == 'P';
+    for (int dim = 0; dim < 4 && rc == FZ_OK; dim++) {
+      const int *cand = dim == 0 ? tiles : (dim == 1 ? moms : (dim == 2 ? tks : rcs));
+      int ncand = dim == 0 ? 5 : (dim == 1 ? 4 : (dim == 2 ? (mixed ? 5 : 3) : 2));
+      if (dim == 3 && (periodic_z || best.tk < 0)) ncand = 0;  // no such kernel there
       for (int q = 0; q < ncand && rc == FZ_OK; q++) {
         Tuned t = best;
-        int &field = dim == 0 ? t.tile_y : (dim == 1 ? t.mom_variant : t.tk);
+        int &field = dim == 0 ? t.tile_y : (dim == 1 ? t.mom_variant : (dim == 2 ? t.tk : t.rc));
         if (field == cand[q]) continue;
         field = cand[q];
         rc = trial(sc, c, t, mixed, trial_steps, a, b, &t.ms);
@@ -112,7 +119,7 @@ extern "C" int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size
   fz_case_drop_graphs(cs);  // captured launches carry the old parameters
   if (report && cap > 0)
     snprintf(report, cap, "tile_y=%d mom_variant=%d thomas_rows=%d%s ms_per_step=%.4f%s", best.tile_y,
-             best.mom_variant, best.tk < 0 ? -best.tk : best.tk, best.tk < 0 ? "x2" : "", best.ms,
+             best.mom_variant, best.tk < 0 ? -best.tk : best.tk, best.tk < 0 ? "x2" : (best.rc ? "rc" : ""), best.ms,
              cached ? " (cached)" : "");
   return FZ_OK;
 }

@@ -261,6 +261,24 @@ POISSON = [([16, 12, 10], "PPPPNN"), ([64, 64, 32], "PPPPNN"), ([30, 18, 7], "NN
 
 @pytest.mark.parametrize("n,cbc", POISSON, ids=[f"{'x'.join(map(str, a))}-{b}" for a, b in POISSON])
 def test_poisson_solve_bitwise(n, cbc, rng):
+    poisson_solve_bitwise(n, cbc, rng)
+
+
+NOT_PERIODIC_Z = [(n, cbc) for n, cbc in POISSON if cbc[4:] != "PP"]
+
+
+@pytest.mark.parametrize("rows", ["8", "16"])
+@pytest.mark.parametrize("n,cbc", NOT_PERIODIC_Z, ids=[f"{'x'.join(map(str, a))}-{b}" for a, b in NOT_PERIODIC_Z])
+def test_poisson_solve_bitwise_without_pivot_array(n, cbc, rows, rng, monkeypatch):
+    """The Thomas kernel that re-runs the elimination per chunk instead of
+    storing pivots (a context reads FZ_THOMAS_RC and FZ_THOMAS_ROWS64 when
+    it is created) matches the oracle bit for bit as the default one does."""
+    monkeypatch.setenv("FZ_THOMAS_RC", "1")
+    monkeypatch.setenv("FZ_THOMAS_ROWS64", rows)
+    poisson_solve_bitwise(n, cbc, rng)
+
+
+def poisson_solve_bitwise(n, cbc, rng):
     pr = Pair(n, [1.0, 0.9, 1.7], cbc, gr=0.6 if cbc[4:] != "PP" else 0.0)
     try:
         a, b, c = np.zeros(n[2]), np.zeros(n[2]), np.zeros(n[2])

@@ -76,17 +76,21 @@ def test_mixed_solver_refines_to_fp64(n, cbc, gr, rng):
 
 
 @pytest.mark.parametrize("cbc", ["PPPPNN", "PPPPPP"])
-@pytest.mark.parametrize("rows", ["8", "16", "-8", "-16"])
+@pytest.mark.parametrize("rows", ["8", "16", "-8", "-16", "rc8", "rc16"])
 def test_thomas_variants_agree_bitwise(rows, cbc, monkeypatch):
     """Every FP32 Thomas variant (rows per chunk, one or two columns per
-    lane; a context reads FZ_THOMAS_ROWS32 when it is created), for Neumann
-    and periodic z, gives the default variant's solution bit for bit."""
+    lane, rc: no pivot array; a context reads FZ_THOMAS_ROWS32 and
+    FZ_THOMAS_RC when it is created), for Neumann and periodic z, gives the
+    default variant's solution bit for bit."""
     out = []
     for env in (None, rows):
         if env is None:
             monkeypatch.delenv("FZ_THOMAS_ROWS32", raising=False)
+            monkeypatch.delenv("FZ_THOMAS_RC", raising=False)
         else:
-            monkeypatch.setenv("FZ_THOMAS_ROWS32", env)
+            monkeypatch.setenv("FZ_THOMAS_ROWS32", env.lstrip("rc"))
+            if env.startswith("rc"):
+                monkeypatch.setenv("FZ_THOMAS_RC", "1")
         it, rr, res, err, acc = solve_pair([64, 64, 32], cbc, 0.6 if cbc == "PPPPNN" else 0.0,
                                            np.random.default_rng(7), 1e-13, 2)
         out.append((it, rr, err))
