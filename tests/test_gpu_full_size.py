@@ -1,0 +1,129 @@
+"""Properties at the full sizes of the benchmarked workloads, where the oracle
+is too slow to run: the C1 Poisson solve (512^3, PPPPNN) against a
+manufactured solution, and the C2 step (256x256x512 ACDI rising bubble,
+density ratio 1000) for determinism, the projection's divergence, the phase
+field's bounds and volume, and the mixed-precision step against the FP64
+one (the bounds the smaller mixed-precision tests state)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import fizzy_amd as F
+from fizzy_amd import cases
+from test_gpu_mixed import field_errors, run_case
+
+pytestmark = pytest.mark.gpu
+
+
+def apply_operator(p, dxi, dyi, dzci, dzfi):
+    """The discrete operator the solver inverts, on the interior of the
+    halo'd p: 3-point second differences in periodic x and y, and in z the
+    flux form with Neumann walls (src/initsolver.f90:153-182), cell by cell
+    as fz_chk_poisson evaluates it."""
+    x = p[1:-1, 1:-1, 1:-1]
+    out = (np.roll(x, -1, 0) - 2.0 * x + np.roll(x, 1, 0)) * dxi ** 2
+    out += (np.roll(x, -1, 1) - 2.0 * x + np.roll(x, 1, 1)) * dyi ** 2
+    xz = np.concatenate([x[:, :, :1], x, x[:, :, -1:]], axis=2)  # Neumann ghosts
+    flux = (xz[:, :, 1:] - xz[:, :, :-1]) * dzci[None, None, :-1]  # faces k-1/2, k = 1 .. n3+1
+    out += (flux[:, :, 1:] - flux[:, :, :-1]) * dzfi[None, None, 1:-1]
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c1_poisson_recovers_a_manufactured_solution(precision):
+    """A smooth zero-mean p on the 512^3 grid (modes 3, 5, 7 in x, y, z plus
+    a product mode), b = A p with the solver's own discrete operator, then
+    the solve of b gives p back (mean removed: the operator is singular):
+    to 1e-12 of its maximum in mixed precision, whose refinement pins the
+    null mode, and to 1e-9 in FP64, where the reference's eps-shifted pivot
+    (src/solver.f90:163-167) turns the rounding left in b's compatibility
+    (a sum over 1.3e8 cells) into a large constant whose own rounding is
+    what remains (3.2e-10 measured; mixed: 2.5e-14)."""
+    n = [512, 512, 512]
+    ctx = F.Context(n, [1.0, 1.0, 1.0], "PPPPNN")
+    try:
+        dxi, dyi = ctx.dli[0], ctx.dli[1]
+        dzci, dzfi = 1.0 / ctx.dzc, 1.0 / ctx.dzf
+        F.call("fz_initsolver", ctx.h, None, None, None, None, None, None, None, None, None)
+        x = (np.arange(n[0]) + 0.5) / n[0]
+        y = (np.arange(n[1]) + 0.5) / n[1]
+        z = ctx.zc[1:-1] / ctx.l[2]
+        p = np.zeros([m + 2 for m in n], order="F")
+        pi = np.cos(2 * np.pi * 3 * x)[:, None, None] + np.sin(2 * np.pi * 5 * y)[None, :, None] \
+            + np.cos(np.pi * 7 * z)[None, None, :] \
+            + np.cos(2 * np.pi * x)[:, None, None] * np.sin(2 * np.pi * 2 * y)[None, :, None] \
+            * np.cos(np.pi * 3 * z)[None, None, :]
+        p[1:-1, 1:-1, 1:-1] = pi
+        b = np.zeros_like(p)
+        b[1:-1, 1:-1, 1:-1] = apply_operator(p, dxi, dyi, dzci, dzfi)
+        # exact compatibility with the operator's null vector (the cell
+        # volumes dzf, nonuniform at 1e-7 in the reference's grid)
+        wz = ctx.dzf[1:-1]
+        bi = b[1:-1, 1:-1, 1:-1]
+        bi -= (bi * wz).sum() / (wz.sum() * n[0] * n[1])
+        d = ctx.halo().upload(b)
+        if precision == "fp64":
+            F.call("fz_solver", ctx.h, d.ptr)
+        else:
+            it, rr = C.c_int(), C.c_double()
+            F.call("fz_solver_mixed", ctx.h, d.ptr, 1e-13, 8, C.byref(it), C.byref(rr))
+            print("refinements", it.value, "relres", rr.value)
+        got = d.download()[1:-1, 1:-1, 1:-1]
+        d.free()
+        got = got - got.mean()
+        want = pi - pi.mean()
+        err = float(np.abs(got - want).max() / np.abs(want).max())
+        print(precision, "max error / max |p|", err)
+        assert err <= (1e-9 if precision == "fp64" else 1e-12)
+    finally:
+        ctx.close()
+
+
+def c2(**kw):
+    return dict(cases.config2(), icheck=5, **kw)
+
+
+def run_c2(spec, nsteps):
+    dc = F.DeviceCase(spec)
+    try:
+        dc.init()
+        vol0 = float(dc.get("psi")[1:-1, 1:-1, 1:-1].sum())
+        dc.step(nsteps)
+        out = {k: dc.get(k) for k in ("u", "v", "w", "p", "psi")}
+        return out, dc.scalars(), vol0
+    finally:
+        dc.close()
+
+
+def test_c2_step_is_deterministic_solenoidal_and_bounded():
+    """Two runs of 5 steps give the same fields bit for bit; chkdiv's divmax
+    after the 5th step is below 1e-10; psi stays in [0, 1] and its volume
+    (the ACDI fluxes are conservative) moves by less than 1e-12 (relative)
+    in 5 steps."""
+    a, sa, vol0 = run_c2(c2(), 5)
+    b, sb, _ = run_c2(c2(), 5)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(sa, sb)
+    divmax = float(sa[7])
+    psi = a["psi"][1:-1, 1:-1, 1:-1]
+    drift = abs(float(psi.sum()) - vol0) / vol0
+    print("divmax", divmax, "psi range", psi.min(), psi.max(), "volume drift", drift)
+    assert divmax <= 1e-10
+    assert psi.min() >= 0.0 and psi.max() <= 1.0
+    assert drift <= 1e-12
+
+
+def test_c2_mixed_step_against_fp64():
+    """5 steps of the mixed-precision C2 step with one refinement against the
+    FP64 step: divmax <= 1e-10, velocity <= 1e-2, pressure <= 1e-3, psi <=
+    1e-5 and interface curvature <= 1e-2 of their scales."""
+    ref, _, _ = run_case(c2(), 5)
+    got, sc, _ = run_case(c2(precision="mixed", refine=1), 5)
+    errs = field_errors(got, ref)
+    print("errors vs fp64", errs, "divmax", sc[7])
+    assert sc[7] <= 1e-10
+    bounds = {"u": 1e-2, "v": 1e-2, "w": 1e-2, "p": 1e-3, "psi": 1e-5, "kappa": 1e-2}
+    for k, bnd in bounds.items():
+        assert errs[k] <= bnd, (k, errs[k])
