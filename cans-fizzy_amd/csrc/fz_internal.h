@@ -97,6 +97,7 @@ struct fz_ctx {
   int mom_variant = 0;  // k_mom_fused variant: 0, 1 (staged loads), 4 or 5 (waves per EU)
   int tk64 = 16;        // rows per chunk of the FP64 Thomas kernel: 8, 16 or 32
   int tk32 = 32;        // the same for the FP32 one; negative: two columns per lane, -8 or -16
+  int correc_kz = 8;    // k_correc_step: 0 one cell per thread, 8 or 16 cells per thread marching in z
   int thomas_rc = 0;    // 1: Thomas kernels without a pivot array (the elimination re-run per chunk);
                         // set at creation from the column count
 };
