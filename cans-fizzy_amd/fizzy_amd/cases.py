@@ -65,7 +65,8 @@ def lid_driven_cavity(ng=(2, 64, 64)):
     return dict(
         ng=list(ng), l=[0.03125 * ng[0] / 2.0, 1.0, 1.0], cfl=0.95, dtmax=1e9, dt_f=-1.0, icheck=10,
         cbcvel=bc3("PP", "DD", "DD") * 3, bcvel=bcvel, cbcpre=bc3("PP", "NN", "NN"),
-        cbcpsi=bc3("PP", "NN", "NN"), cbcnor=bc3("PP", "NN", "NN") * 3,
+        cbcpsi=bc3("PP", "NN", "NN"),
+        cbcnor=bc3("PP", "NN", "NN") + bc3("PP", "DD", "NN") + bc3("PP", "NN", "DD"),
         gacc=[0.0, 0.0, 0.0], rho12=[1.0, 1.0], mu12=[0.01, 0.01], sigma=0.0, inipsi="uni", spheres=[], vof=False)
 
 

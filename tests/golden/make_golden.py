@@ -8,7 +8,9 @@ R2HC/HC2R halfcomplex). scipy's norm=None conventions equal FFTW's.
 The reference-test fixtures c1g2l3s.txt (Hysing et al. 2009 bubble contour,
 tests/rising_bubble), ghiau.txt and ghiav.txt (Ghia et al. 1982 cavity
 profiles, tests/lid_driven_cavity) are data files copied from the reference's
-own tests by copy_reference_fixtures().
+own tests by copy_reference_fixtures(), and so are the input files of those
+two tests (input.nml, and spheres.in of the rising bubble), under the names
+rising_bubble.nml, rising_bubble_spheres.in and lid_driven_cavity.nml.
 """
 import os
 import shutil
@@ -61,11 +63,14 @@ def make_r2r():
 
 
 def copy_reference_fixtures(ref="/root/reference"):
-    for rel in ["tests/rising_bubble/c1g2l3s.txt", "tests/lid_driven_cavity/ghiau.txt",
-                "tests/lid_driven_cavity/ghiav.txt"]:
+    for rel, name in [("tests/rising_bubble/c1g2l3s.txt", None), ("tests/lid_driven_cavity/ghiau.txt", None),
+                      ("tests/lid_driven_cavity/ghiav.txt", None),
+                      ("tests/rising_bubble/input.nml", "rising_bubble.nml"),
+                      ("tests/rising_bubble/spheres.in", "rising_bubble_spheres.in"),
+                      ("tests/lid_driven_cavity/input.nml", "lid_driven_cavity.nml")]:
         src = os.path.join(ref, rel)
         if os.path.exists(src):
-            shutil.copyfile(src, os.path.join(HERE, os.path.basename(rel)))
+            shutil.copyfile(src, os.path.join(HERE, name or os.path.basename(rel)))
 
 
 if __name__ == "__main__":
