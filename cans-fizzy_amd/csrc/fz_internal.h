@@ -94,7 +94,8 @@ struct fz_ctx {
   double *h_small = nullptr;  // pinned host landing zone for reduction results
   std::map<std::string, struct fz_bcmap *> bcmaps;  // composed boundary maps by signature
   // launch parameters (defaults, environment overrides, or fz_case_autotune)
-  int mom_variant = 0;  // k_mom_fused variant: 0, 1 (staged loads), 4 or 5 (waves per EU)
+  int mom_variant = 0;  // momentum kernel: 0 by precision (8 for FP64, 2 for FP32 phase fields), 1 staged
+                        // loads, 2 plain, 4 or 5 waves per EU, 8 stencil fields in LDS (k_mom_lds)
   int tk64 = 16;        // rows per chunk of the FP64 Thomas kernel: 8, 16 or 32
   int tk32 = 32;        // the same for the FP32 one; negative: two columns per lane, -8 or -16
   int correc_kz = 8;    // k_correc_step: 0 one cell per thread, 8 or 16 cells per thread marching in z

@@ -287,6 +287,24 @@ __device__ __forceinline__ BufAt<T> buf_at(const T *p, const fz_dims &d, long q)
   return b;
 }
 
+// buffer loads at element e of a halo'd field (byte offsets in 32 bits, a
+// uniform extra byte offset in soffset; past the field's end they read 0)
+template <typename T>
+struct BufFld {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ T operator()(unsigned e, unsigned so_bytes) const {
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, e * 8u, so_bytes, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, e * 4u, so_bytes, 0));
+  }
+};
+template <typename T>
+__device__ __forceinline__ BufFld<T> buf_fld(const T *p, const fz_dims &d) {
+  const unsigned bytes = (unsigned)((unsigned long long)d.sz * (unsigned)(d.n3 + 2) * sizeof(T));
+  return BufFld<T>{__builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000)};
+}
+
 // whether a halo'd FP64 field is small enough for 32-bit buffer offsets
 static int fits_buffers(const fz_ctx *c, const char *who) {
   if ((unsigned long long)c->nh * sizeof(double) < (1ull << 32)) return FZ_OK;
@@ -791,12 +809,12 @@ __global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restr
 }
 
 // mom_xyz_oth at one cell, src/mom.f90:912-1038 (constant-coefficient split, Brackbill)
-template <class AD, class AP>
+template <class AD, class AP, class AK = AP>
 __device__ __forceinline__ void mom_oth_cell(int k, const double *__restrict__ dzci, double dxi, double dyi,
                                              double dt_r, double rho, double drho, double surf_factor, double rho0,
                                              double bfx, double bfy, double bfz, double gx, double gy, double gz,
                                              double rax, double ray, double raz, const AD &p, const AP &psi,
-                                             const AP &kappa, const AD &pn, const AD &po, double &ru, double &rv,
+                                             const AK &kappa, const AD &pn, const AD &po, double &ru, double &rv,
                                              double &rw) {
   const double dzci_c = dzci[k];
   const double p_ccc = p(0, 0, 0), p_pcc = p(1, 0, 0), p_cpc = p(0, 1, 0), p_ccp = p(0, 0, 1);
@@ -940,6 +958,143 @@ FZ_MOM_FUSED_KERNEL(k_mom_fused_st, __attribute__((amdgpu_waves_per_eu(4))), tru
 FZ_MOM_FUSED_KERNEL(k_mom_fused_w4, __attribute__((amdgpu_waves_per_eu(4))), false)
 FZ_MOM_FUSED_KERNEL(k_mom_fused_w5, __attribute__((amdgpu_waves_per_eu(5))), false)
 
+// ---------------------------------------------------------------------------
+// k_mom_fused with its stencil fields staged in LDS. The advection-diffusion
+// stencil reads u, v, w, psi and the three psi fluxes at up to 16 neighbours
+// each; read from HBM/L1 one by one, those loads keep the texture-address
+// unit busy 94 % of the kernel (rocprofv3 TA_TA_BUSY at 256x256x512) while
+// HBM moves 3 TB/s. Here a block of MX x MY cells marches up KZ planes in z
+// and holds, for each of the seven fields, the planes k-1, k, k+1 of its
+// tile plus a one-cell x/y frame in LDS (a ring of three plane slots); each
+// plane is read from HBM once, in rows, the plane k+2 being loaded into
+// registers while level k is computed. The cell arithmetic is
+// mom_fused_body's through an LDS accessor (LdsAt), so the results are the
+// same bit for bit.
+constexpr int MOM_MX = 32, MOM_MY = 8, MOM_PX = MOM_MX + 2, MOM_PL = MOM_PX * (MOM_MY + 2);
+
+// a field's three plane slots in LDS, seen from one cell
+template <typename T>
+struct LdsAt {
+  const T *base;
+  int c;           // the cell's position in a plane
+  int s0, s1, s2;  // offsets of the planes k-1, k, k+1
+  __device__ __forceinline__ T operator()(int a, int b, int cc) const {
+    const int so = cc < 0 ? s0 : (cc == 0 ? s1 : s2);
+    return base[so + c + a + b * MOM_PX];
+  }
+};
+
+template <typename PT, typename HT, int KZ>
+__global__ void __launch_bounds__(MOM_MX *MOM_MY) k_mom_lds(
+    fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
+    double drho, double mu, double dmu, double factor1_v, double factor2_v, int first, double factor12_v,
+    double dt_r_v, const double *__restrict__ stc, double surf_factor, double rho0, double bfx, double bfy,
+    double bfz, double gx, double gy, double gz, double rax, double ray, double raz, const PT *__restrict__ psiflx_x,
+    const PT *__restrict__ psiflx_y, const PT *__restrict__ psiflx_z, const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ w, const PT *__restrict__ psi,
+    const PT *__restrict__ psio, const double *__restrict__ p, const double *__restrict__ po,
+    const PT *__restrict__ kappa, HT *__restrict__ duo, HT *__restrict__ dvo, HT *__restrict__ dwo,
+    double *__restrict__ un, double *__restrict__ vn, double *__restrict__ wn) {
+  constexpr int NT = MOM_MX * MOM_MY, PL = MOM_PL;
+  constexpr int NPT = (PL + NT - 1) / NT;  // plane positions per thread
+  __shared__ double s_u[3 * PL], s_v[3 * PL], s_w[3 * PL];
+  __shared__ PT s_psi[3 * PL], s_fx[3 * PL], s_fy[3 * PL], s_fz[3 * PL];
+  const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
+  const double factor12 = stc ? stc[FZ_STC_F12] : factor12_v, dt_r = stc ? stc[FZ_STC_DTR] : dt_r_v;
+  FZ_BLOCK_XYZ(d, bx, by, bz);
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + MOM_MX * ty;
+  const int i0 = (int)bx * MOM_MX + 1, j0 = (int)by * MOM_MY + 1, k0 = (int)bz * KZ + 1;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool act = i <= d.n1 && j <= d.n2;
+  const int kend = min(k0 + KZ - 1, d.n3);
+  const int c = (ty + 1) * MOM_PX + (tx + 1);
+  const BufFld<double> GU = buf_fld(u, d), GV = buf_fld(v, d), GW = buf_fld(w, d);
+  const BufFld<PT> GPSI = buf_fld(psi, d), GFX = buf_fld(psiflx_x, d), GFY = buf_fld(psiflx_y, d),
+                   GFZ = buf_fld(psiflx_z, d);
+  // halo'd index of plane position pos (a frame cell past the grid's halo
+  // reads another row of the array, or zero past its end; no cell uses it)
+  unsigned gpos[NPT];
+#pragma unroll
+  for (int r = 0; r < NPT; r++) {
+    const int pos = min(t + r * NT, PL - 1), py = pos / MOM_PX, px = pos - py * MOM_PX;
+    gpos[r] = (unsigned)(i0 - 1 + px) + (unsigned)d.sy * (unsigned)(j0 - 1 + py);
+  }
+  auto slot = [](int kp) { return (kp % 3) * PL; };
+  double qu[NPT], qv[NPT], qw[NPT];
+  PT qp[NPT], qx[NPT], qy[NPT], qz[NPT];
+  auto load_plane = [&](int kp) {
+    const unsigned zo = (unsigned)d.sz * (unsigned)kp;
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+      const unsigned e = gpos[r] + zo;
+      qu[r] = GU(e, 0);
+      qv[r] = GV(e, 0);
+      qw[r] = GW(e, 0);
+      qp[r] = GPSI(e, 0);
+      qx[r] = GFX(e, 0);
+      qy[r] = GFY(e, 0);
+      qz[r] = GFZ(e, 0);
+    }
+  };
+  auto store_plane = [&](int kp) {
+    const int so = slot(kp);
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+      const int pos = t + r * NT;
+      if (pos < PL) {
+        s_u[so + pos] = qu[r];
+        s_v[so + pos] = qv[r];
+        s_w[so + pos] = qw[r];
+        s_psi[so + pos] = qp[r];
+        s_fx[so + pos] = qx[r];
+        s_fy[so + pos] = qy[r];
+        s_fz[so + pos] = qz[r];
+      }
+    }
+  };
+  for (int kp = k0 - 1; kp <= k0 + 1; kp++) {
+    load_plane(kp);
+    store_plane(kp);
+  }
+  __syncthreads();
+  for (int k = k0; k <= kend; k++) {
+    if (k < kend) load_plane(k + 2);  // in flight during this level
+    if (act) {
+      const long q = i + d.sy * j + d.sz * (long)k;
+      const long r = (i - 1) + (long)d.n1 * ((j - 1) + (long)d.n2 * (k - 1));
+      const int sa = slot(k - 1), sb = slot(k), sc = slot(k + 1);
+      const LdsAt<double> U{s_u, c, sa, sb, sc}, V{s_v, c, sa, sb, sc}, W{s_w, c, sa, sb, sc};
+      const LdsAt<PT> PSI{s_psi, c, sa, sb, sc}, FX{s_fx, c, sa, sb, sc}, FY{s_fy, c, sa, sb, sc},
+          FZ{s_fz, c, sa, sb, sc};
+      double ru, rv, rw;
+      mom_ad_cell(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, ru, rv, rw);
+      const double ruo = first ? ru : duo[r], rvo = first ? rv : dvo[r], rwo = first ? rw : dwo[r];
+      duo[r] = ru;
+      dvo[r] = rv;
+      dwo[r] = rw;
+      const BufAt<PT> PSIO = buf_at(psio, d, q), KAP = buf_at(kappa, d, q);
+      const BufAt<double> P = buf_at(p, d, q), PO = buf_at(po, d, q);
+      const double rhox_n = rho + drho * 0.5 * ((double)PSIO(0, 0, 0) + (double)PSIO(1, 0, 0));
+      const double rhoy_n = rho + drho * 0.5 * ((double)PSIO(0, 0, 0) + (double)PSIO(0, 1, 0));
+      const double rhoz_n = rho + drho * 0.5 * ((double)PSIO(0, 0, 0) + (double)PSIO(0, 0, 1));
+      const double rhox_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(1, 0, 0));
+      const double rhoy_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(0, 1, 0));
+      const double rhoz_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(0, 0, 1));
+      double uu = U(0, 0, 0) * rhox_n / rhox_p + (factor1 * ru + factor2 * ruo) / rhox_p;
+      double vv = V(0, 0, 0) * rhoy_n / rhoy_p + (factor1 * rv + factor2 * rvo) / rhoy_p;
+      double ww = W(0, 0, 0) * rhoz_n / rhoz_p + (factor1 * rw + factor2 * rwo) / rhoz_p;
+      mom_oth_cell(k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz,
+                   P, PSI, KAP, P, PO, ru, rv, rw);
+      un[q] = uu + factor12 * ru / rhox_p;
+      vn[q] = vv + factor12 * rv / rhoy_p;
+      wn[q] = ww + factor12 * rw / rhoz_p;
+    }
+    __syncthreads();  // the slot of plane k-1 is free
+    if (k < kend) store_plane(k + 2);
+    __syncthreads();
+  }
+}
+
 template <typename PT, typename HT>
 int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], double factor1, double factor2,
                         int first, double factor12, double dt_r, const double *stc, double sigma, double rho0,
@@ -950,7 +1105,22 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
                         double *vn, double *wn) {
   FZ_TRY(fits_buffers(c, "mom_fused"));
   const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
-  const int variant = c->mom_variant;  // occupancy variant (the context's tuned choice)
+  // the context's variant; 0: the faster one by precision at 256x256x512,
+  // LDS staging for FP64 (1386 against 1741 us), the plain kernel for FP32
+  // phase fields (1292 against 1431 us)
+  int variant = c->mom_variant;
+  if (variant == 0) variant = sizeof(PT) == 8 ? 8 : 2;
+  if (variant == 8) {  // stencil fields staged in LDS
+    constexpr int KZ = 16;
+    const dim3 grid((c->d.n1 + MOM_MX - 1) / MOM_MX, (c->d.n2 + MOM_MY - 1) / MOM_MY, (c->d.n3 + KZ - 1) / KZ);
+    FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", (k_mom_lds<PT, HT, KZ>), grid,
+              dim3(MOM_MX, MOM_MY), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1], rho12[0] - rho12[1],
+              mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12, dt_r, stc, surf_factor, rho0, bforce[0],
+              bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1], rho_av[2], fx, fy, fz, u, v, w,
+              psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);
+    FZ_LAUNCHED();
+    return FZ_OK;
+  }
   auto kern = variant == 4   ? k_mom_fused_w4<PT, HT>
               : variant == 5 ? k_mom_fused_w5<PT, HT>
               : variant == 1 ? k_mom_fused_st<PT, HT>

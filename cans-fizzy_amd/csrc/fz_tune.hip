@@ -3,7 +3,8 @@
 // The parameters change the order and shape of launches, never the
 // arithmetic, so every choice gives the same fields bit for bit:
 //  - tile_y: rows of blocks per z-marching pencil in the stencil block order;
-//  - the variant of the fused momentum kernel (occupancy, staged loads);
+//  - the variant of the fused momentum kernel (occupancy, staged loads,
+//    stencil fields staged in LDS);
 //  - rows per chunk of the Thomas kernel (FP64, or FP32 in mixed cases,
 //    where two columns per lane are a candidate too);
 //  - for a non-periodic z, the Thomas kernel without a pivot array (it
@@ -91,13 +92,13 @@ extern "C" int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size
             c ? c->thomas_rc : 0, 0.0f};
     if (rc == FZ_OK) rc = trial(sc, c, best, mixed, trial_steps, a, b, &best.ms);
     const int tiles[] = {0, 2, 4, 8, 16};
-    const int moms[] = {0, 1, 4, 5};
+    const int moms[] = {1, 2, 4, 5, 8};
     const int tks[] = {8, 16, 32, -8, -16};  // negative: two FP32 columns per lane (mixed cases)
     const int rcs[] = {0, 1};
     const bool periodic_z = prm.cbcpre[4] == 'P';
     for (int dim = 0; dim < 4 && rc == FZ_OK; dim++) {
       const int *cand = dim == 0 ? tiles : (dim == 1 ? moms : (dim == 2 ? tks : rcs));
-      int ncand = dim == 0 ? 5 : (dim == 1 ? 4 : (dim == 2 ? (mixed ? 5 : 3) : 2));
+      int ncand = dim == 0 ? 5 : (dim == 1 ? 5 : (dim == 2 ? (mixed ? 5 : 3) : 2));
       if (dim == 3 && (periodic_z || best.tk < 0)) ncand = 0;  // no such kernel there
       for (int q = 0; q < ncand && rc == FZ_OK; q++) {
         Tuned t = best;

@@ -1,7 +1,7 @@
-"""Every variant of the fused momentum kernel (staged loads, waves-per-SIMD
-caps; a context reads FZ_MOM_WPE when it is made) gives the default
-variant's fields bit for bit, in FP64 and in mixed precision, on a grid
-whose sizes are not multiples of the blocks."""
+"""Every variant of the momentum kernel (staged loads, plain, waves-per-SIMD
+caps, the stencil fields staged in LDS; a context reads FZ_MOM_WPE when it
+is made) gives the default variant's fields bit for bit, in FP64 and in
+mixed precision, on a grid whose sizes are not multiples of the blocks."""
 import numpy as np
 import pytest
 
@@ -30,7 +30,7 @@ def run(spec, nsteps, variant, monkeypatch):
         dc.close()
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 4, 5, 8])
 @pytest.mark.parametrize("name", list(SPECS))
 def test_momentum_variants_agree_bitwise(name, variant, monkeypatch):
     if name not in _reference:

@@ -33,7 +33,7 @@ def test_tuned_case_gives_the_same_fields(spec):
     b, sb, rep = run(spec, 25, True)
     print(rep)
     m = re.match(r"tile_y=(\d+) mom_variant=(\d+) thomas_rows=(\d+)(?:x2|rc)? ms_per_step=([0-9.]+)", rep)
-    assert m and int(m.group(1)) in (0, 2, 4, 8, 16) and int(m.group(2)) in (0, 1, 4, 5)
+    assert m and int(m.group(1)) in (0, 2, 4, 8, 16) and int(m.group(2)) in (0, 1, 2, 4, 5, 8)
     assert int(m.group(3)) in (8, 16, 32) and float(m.group(4)) > 0
     for k in FIELDS:
         assert np.array_equal(a[k], b[k]), k
