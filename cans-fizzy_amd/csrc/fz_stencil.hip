@@ -201,23 +201,6 @@ __global__ void __launch_bounds__(FZ_GAMMA_SLOTS) k_gamma_fin(unsigned long long
   }
 }
 
-int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
-                          const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
-                          double *p_out, unsigned long long *slots) {
-  if (c->correc_kz > 0) {
-    dim3 grid = fz_grid_halo(c->d);
-    const int kz = c->correc_kz == 8 ? 8 : 16;
-    grid.z = (c->d.n3 + 2 + kz - 1) / kz;
-    FZ_LAUNCH(c, "correc_step", kz == 8 ? k_correc_step_mz<8> : k_correc_step_mz<16>, grid, fz_block3(), 0, c->d,
-              c->dzci, dt * c->dli[0], dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn,
-              wn, p_out, slots);
-  } else {
-    FZ_LAUNCH(c, "correc_step", k_correc_step, fz_grid_halo(c->d), fz_block3(), 0, c->d, c->dzci, dt * c->dli[0],
-              dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots);
-  }
-  FZ_LAUNCHED();
-  return FZ_OK;
-}
 
 int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out) {
   FZ_LAUNCH(c, "gamma_fin", k_gamma_fin, dim3(1), dim3(FZ_GAMMA_SLOTS), 0, slots, factor, gmin, out);
@@ -310,6 +293,148 @@ static int fits_buffers(const fz_ctx *c, const char *who) {
   if ((unsigned long long)c->nh * sizeof(double) < (1ull << 32)) return FZ_OK;
   return fz_set_error(FZ_ERR_UNSUPPORTED, "%s: halo'd fields above 4 GiB are not supported", who);
 }
+
+// k_correc_step_mz with the correction dp and the velocities u, v staged in
+// LDS: a block of 64 x 4 cells (of the halo-inclusive range) marches up 16
+// planes and keeps, per field, the planes k and k+1 of its tile with a
+// one-cell x/y frame, so each value is read from HBM once instead of through
+// five (dp) or two (u, v) neighbour loads, whose address processing bounds
+// the one-cell-per-load kernel. Same expressions, same values.
+constexpr int COR_MX = 64, COR_MY = 4, COR_PX = COR_MX + 2, COR_PL = COR_PX * (COR_MY + 2);
+template <int KZ>
+__global__ void __launch_bounds__(COR_MX *COR_MY) k_correc_lds(fz_dims d, const double *__restrict__ dzci,
+                                                              double factori_v, double factorj_v, double dt_v,
+                                                              double dxi, double dyi, const double *__restrict__ stc,
+                                                              double rhop, const double *__restrict__ dp,
+                                                              const double *__restrict__ p,
+                                                              const double *__restrict__ u,
+                                                              const double *__restrict__ v,
+                                                              const double *__restrict__ w, double *__restrict__ un,
+                                                              double *__restrict__ vn, double *__restrict__ wn,
+                                                              double *__restrict__ p_out,
+                                                              unsigned long long *__restrict__ slots) {
+  constexpr int NT = COR_MX * COR_MY, PL = COR_PL, NPT = (PL + NT - 1) / NT;
+  __shared__ double s_dp[2 * PL], s_u[2 * PL], s_v[2 * PL];
+  const double dt = stc ? stc[FZ_STC_DT] : dt_v;
+  const double factori = stc ? dt * dxi : factori_v, factorj = stc ? dt * dyi : factorj_v;
+  FZ_BLOCK_XYZ(d, bx, by, bz);
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + COR_MX * ty;
+  const int i0 = (int)bx * COR_MX, j0 = (int)by * COR_MY;  // the range starts at the halo
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = (int)bz * KZ, kend = min(k0 + KZ - 1, d.n3 + 1), ktop = min(kend + 1, d.n3 + 1);
+  const bool act = i <= d.n1 + 1 && j <= d.n2 + 1;
+  const bool in_ij = i >= 1 && i <= d.n1 && j >= 1 && j <= d.n2;
+  const int c = (ty + 1) * COR_PX + (tx + 1);
+  const BufFld<double> GDP = buf_fld(dp, d), GU = buf_fld(u, d), GV = buf_fld(v, d);
+  // halo'd index of plane position pos (frame cells outside the array read
+  // another row, or zero past its end; no cell uses them)
+  unsigned gpos[NPT];
+#pragma unroll
+  for (int r = 0; r < NPT; r++) {
+    const int pos = min(t + r * NT, PL - 1), py = pos / COR_PX, px = pos - py * COR_PX;
+    gpos[r] = (unsigned)(i0 - 1 + px) + (unsigned)d.sy * (unsigned)(j0 - 1 + py);
+  }
+  double qd[NPT], qu[NPT], qv[NPT];
+  auto load_plane = [&](int kp) {
+    const unsigned zo = (unsigned)d.sz * (unsigned)kp;
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+      qd[r] = GDP(gpos[r] + zo, 0);
+      qu[r] = GU(gpos[r] + zo, 0);
+      qv[r] = GV(gpos[r] + zo, 0);
+    }
+  };
+  auto store_plane = [&](int kp) {
+    const int so = (kp & 1) * PL;
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+      const int pos = t + r * NT;
+      if (pos < PL) {
+        s_dp[so + pos] = qd[r];
+        s_u[so + pos] = qu[r];
+        s_v[so + pos] = qv[r];
+      }
+    }
+  };
+  load_plane(k0);
+  store_plane(k0);
+  if (k0 + 1 <= ktop) {
+    load_plane(k0 + 1);
+    store_plane(k0 + 1);
+  }
+  double wbelow = 0.0;  // the new w of the cell below (the lower face of this one)
+  if (slots && in_ij && k0 >= 1 && k0 <= d.n3) {
+    const long q = i + d.sy * j + d.sz * (long)k0;
+    wbelow = w[q - d.sz] - dt * dzci[k0 - 1] * (dp[q] - dp[q - d.sz]) / rhop;
+  }
+  __syncthreads();
+  double m = 0.0;
+  for (int k = k0; k <= kend; k++) {
+    if (k + 2 <= ktop) load_plane(k + 2);  // in flight during this level
+    if (act) {
+      const long q = i + d.sy * j + d.sz * (long)k;
+      const int sa = (k & 1) * PL, sb = ((k + 1) & 1) * PL;
+      const double dpc = s_dp[sa + c];
+      const double dpz = k <= d.n3 ? s_dp[sb + c] : 0.0;
+      const double uq = (i <= d.n1) ? s_u[sa + c] - factori * (s_dp[sa + c + 1] - dpc) / rhop : s_u[sa + c];
+      const double vq =
+          (j <= d.n2) ? s_v[sa + c] - factorj * (s_dp[sa + c + COR_PX] - dpc) / rhop : s_v[sa + c];
+      const double wq = (k <= d.n3) ? w[q] - dt * dzci[k] * (dpz - dpc) / rhop : w[q];
+      un[q] = uq;
+      vn[q] = vq;
+      wn[q] = wq;
+      p_out[q] = dpc + p[q];
+      if (slots && in_ij && k >= 1 && k <= d.n3) {
+        const double um = s_u[sa + c - 1] - factori * (dpc - s_dp[sa + c - 1]) / rhop;
+        const double vm = s_v[sa + c - COR_PX] - factorj * (dpc - s_dp[sa + c - COR_PX]) / rhop;
+        const double uc = 0.5 * (uq + um), vc = 0.5 * (vq + vm), wc = 0.5 * (wq + wbelow);
+        m = fmax(m, uc * uc + vc * vc + wc * wc);
+      }
+      wbelow = wq;
+    }
+    __syncthreads();  // the slot of plane k is free
+    if (k + 2 <= ktop) store_plane(k + 2);
+    __syncthreads();
+  }
+  if (!slots) return;  // uniform across the grid
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  __shared__ double sh[NT / 64];
+  if ((t & 63) == 0) sh[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    double b = sh[0];
+    for (int q = 1; q < NT / 64; q++) b = fmax(b, sh[q]);
+    if (b > 0.0)
+      atomicMax(slots + ((blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) & (FZ_GAMMA_SLOTS - 1)),
+                (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
+                          const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
+                          double *p_out, unsigned long long *slots) {
+  if (c->correc_kz == 1) {  // dp, u, v staged in LDS
+    FZ_TRY(fits_buffers(c, "correc_step"));
+    constexpr int KZ = 16;
+    const dim3 grid((c->d.n1 + 2 + COR_MX - 1) / COR_MX, (c->d.n2 + 2 + COR_MY - 1) / COR_MY,
+                    (c->d.n3 + 2 + KZ - 1) / KZ);
+    FZ_LAUNCH(c, "correc_step", k_correc_lds<KZ>, grid, dim3(COR_MX, COR_MY), 0, c->d, c->dzci, dt * c->dli[0],
+              dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots);
+  } else if (c->correc_kz > 0) {
+    dim3 grid = fz_grid_halo(c->d);
+    const int kz = c->correc_kz == 8 ? 8 : 16;
+    grid.z = (c->d.n3 + 2 + kz - 1) / kz;
+    FZ_LAUNCH(c, "correc_step", kz == 8 ? k_correc_step_mz<8> : k_correc_step_mz<16>, grid, fz_block3(), 0, c->d,
+              c->dzci, dt * c->dli[0], dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn,
+              wn, p_out, slots);
+  } else {
+    FZ_LAUNCH(c, "correc_step", k_correc_step, fz_grid_halo(c->d), fz_block3(), 0, c->d, c->dzci, dt * c->dli[0],
+              dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots);
+  }
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
 
 // transcendentals by phase-field type: the shared FP64 routines (identical
 // on the device and in the oracle), or the device library's FP32 ones
