@@ -98,6 +98,8 @@ struct fz_ctx {
                         // loads, 2 plain, 4 or 5 waves per EU, 8 stencil fields in LDS (k_mom_lds)
   int tk64 = 16;        // rows per chunk of the FP64 Thomas kernel: 8, 16 or 32
   int tk32 = 32;        // the same for the FP32 one; negative: two columns per lane, -8 or -16
+  int acdi_variant = 0;  // ACDI step: 0 by precision (1 for FP32, 2 for FP64), 1 fields staged in LDS
+                         // (k_acdi_lds), 2 one cell per thread
   int correc_kz = 1;    // correction step: 0 one cell per thread, 1 dp, u, v staged in LDS (k_correc_lds),
                         // 8 or 16 cells per thread marching in z (k_correc_step_mz)
   int thomas_rc = 0;    // 1: Thomas kernels without a pivot array (the elimination re-run per chunk);

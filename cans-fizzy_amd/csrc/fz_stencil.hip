@@ -560,15 +560,168 @@ __global__ void __launch_bounds__(256) k_acdi_step(fz_dims d, const double *__re
   psi_out[q] = fmin(fmax((T)0.0, pn), (T)1.0);
 }
 
+// k_acdi_step with its eight fields staged in LDS (the address unit, not the
+// arithmetic, was the busier of the two in the one-cell-per-thread kernel:
+// TA_TA_BUSY 84 % against 72 % for the VALU at 256x256x512). A block of
+// 64 x 4 cells marches up 16 planes and keeps, per field, the planes k and
+// k+1 of its tile with a one-cell x/y frame; the stencil reads below the
+// cell are the cell's own values of the level before, kept in registers,
+// and the reads above it come from plane k+1. The cell arithmetic is
+// acdi_cell's through the accessor LdsZ, so the results are the same bit for
+// bit.
+constexpr int ACD_MX = 64, ACD_MY = 4, ACD_PX = ACD_MX + 2, ACD_PL = ACD_PX * (ACD_MY + 2);
+
+// a field's planes k, k+1 in LDS seen from one cell, and its value below
+// the cell (acdi_cell reads only (0,0,-1) below and (0,0,+1) above)
+template <typename T>
+struct LdsZ {
+  const T *base;
+  int c, s1, s2;
+  T below;
+  __device__ __forceinline__ T operator()(int a, int b, int cc) const {
+    if (cc < 0) return below;
+    return base[(cc == 0 ? s1 : s2) + c + a + b * ACD_PX];
+  }
+};
+
+template <typename T, int KZ>
+__global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_lds(fz_dims d, const double *__restrict__ dzci,
+                                                            const double *__restrict__ dzfi, double dxi, double dyi,
+                                                            const double *__restrict__ d_gam, double gam_v,
+                                                            double seps, double factor1_v, double factor2_v,
+                                                            const double *__restrict__ stc, int first,
+                                                            const double *__restrict__ u, const double *__restrict__ v,
+                                                            const double *__restrict__ w, const T *__restrict__ normx,
+                                                            const T *__restrict__ normy, const T *__restrict__ normz,
+                                                            const T *__restrict__ phi, const T *__restrict__ psio,
+                                                            T *__restrict__ dpsio, T *__restrict__ psi_out,
+                                                            T *__restrict__ flux_x, T *__restrict__ flux_y,
+                                                            T *__restrict__ flux_z) {
+  constexpr int NT = ACD_MX * ACD_MY, PL = ACD_PL, NPT = (PL + NT - 1) / NT;
+  __shared__ double s_u[2 * PL], s_v[2 * PL], s_w[2 * PL];
+  __shared__ T s_nx[2 * PL], s_ny[2 * PL], s_nz[2 * PL], s_phi[2 * PL], s_psi[2 * PL];
+  const double gam = d_gam ? *d_gam : gam_v;
+  const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
+  FZ_BLOCK_XYZ(d, bx, by, bz);
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + ACD_MX * ty;
+  const int i0 = (int)bx * ACD_MX + 1, j0 = (int)by * ACD_MY + 1, k0 = (int)bz * KZ + 1;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool act = i <= d.n1 && j <= d.n2;
+  const int kend = min(k0 + KZ - 1, d.n3);
+  const int c = (ty + 1) * ACD_PX + (tx + 1);
+  const BufFld<double> GU = buf_fld(u, d), GV = buf_fld(v, d), GW = buf_fld(w, d);
+  const BufFld<T> GNX = buf_fld(normx, d), GNY = buf_fld(normy, d), GNZ = buf_fld(normz, d), GPH = buf_fld(phi, d),
+                  GPS = buf_fld(psio, d);
+  unsigned gpos[NPT];
+#pragma unroll
+  for (int r = 0; r < NPT; r++) {
+    const int pos = min(t + r * NT, PL - 1), py = pos / ACD_PX, px = pos - py * ACD_PX;
+    gpos[r] = (unsigned)(i0 - 1 + px) + (unsigned)d.sy * (unsigned)(j0 - 1 + py);
+  }
+  double qu[NPT], qv[NPT], qw[NPT];
+  T qx[NPT], qy[NPT], qz[NPT], qf[NPT], qs[NPT];
+  auto load_plane = [&](int kp) {
+    const unsigned zo = (unsigned)d.sz * (unsigned)kp;
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+      const unsigned e = gpos[r] + zo;
+      qu[r] = GU(e, 0);
+      qv[r] = GV(e, 0);
+      qw[r] = GW(e, 0);
+      qx[r] = GNX(e, 0);
+      qy[r] = GNY(e, 0);
+      qz[r] = GNZ(e, 0);
+      qf[r] = GPH(e, 0);
+      qs[r] = GPS(e, 0);
+    }
+  };
+  auto store_plane = [&](int kp) {
+    const int so = (kp & 1) * PL;
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+      const int pos = t + r * NT;
+      if (pos < PL) {
+        s_u[so + pos] = qu[r];
+        s_v[so + pos] = qv[r];
+        s_w[so + pos] = qw[r];
+        s_nx[so + pos] = qx[r];
+        s_ny[so + pos] = qy[r];
+        s_nz[so + pos] = qz[r];
+        s_phi[so + pos] = qf[r];
+        s_psi[so + pos] = qs[r];
+      }
+    }
+  };
+  // the values below the first level, then the planes k0 and k0+1
+  double wb = 0.0;
+  T xb = (T)0, yb = (T)0, zb = (T)0, fb = (T)0, sb_ = (T)0;
+  if (act) {
+    const unsigned e = (unsigned)i + (unsigned)d.sy * (unsigned)j + (unsigned)d.sz * (unsigned)(k0 - 1);
+    wb = GW(e, 0);
+    xb = GNX(e, 0);
+    yb = GNY(e, 0);
+    zb = GNZ(e, 0);
+    fb = GPH(e, 0);
+    sb_ = GPS(e, 0);
+  }
+  load_plane(k0);
+  store_plane(k0);
+  load_plane(k0 + 1);
+  store_plane(k0 + 1);
+  __syncthreads();
+  for (int k = k0; k <= kend; k++) {
+    if (k < kend) load_plane(k + 2);  // in flight during this level
+    const int s1 = (k & 1) * PL, s2 = ((k + 1) & 1) * PL;
+    if (act) {
+      const long q = i + d.sy * j + d.sz * (long)k;
+      const long r = (i - 1) + (long)d.n1 * ((j - 1) + (long)d.n2 * (k - 1));
+      const LdsZ<double> U{s_u, c, s1, s2, 0.0}, V{s_v, c, s1, s2, 0.0}, W{s_w, c, s1, s2, wb};
+      const LdsZ<T> NX{s_nx, c, s1, s2, xb}, NY{s_ny, c, s1, s2, yb}, NZ{s_nz, c, s1, s2, zb},
+          PH{s_phi, c, s1, s2, fb}, PSIO{s_psi, c, s1, s2, sb_};
+      T R, fx, fy, fz;
+      acdi_cell<T>(k, dzci, dzfi, dxi, dyi, gam, seps, U, V, W, NX, NY, NZ, PH, PSIO, R, fx, fy, fz);
+      const T Ro = first ? R : dpsio[r];
+      dpsio[r] = R;
+      flux_x[q] = fx;
+      flux_y[q] = fy;
+      flux_z[q] = fz;
+      const T pn = PSIO(0, 0, 0) + (T)factor1 * R + (T)factor2 * Ro;
+      psi_out[q] = fmin(fmax((T)0.0, pn), (T)1.0);
+      // this level's values: below the next one
+      wb = s_w[s1 + c];
+      xb = s_nx[s1 + c];
+      yb = s_ny[s1 + c];
+      zb = s_nz[s1 + c];
+      fb = s_phi[s1 + c];
+      sb_ = s_psi[s1 + c];
+    }
+    __syncthreads();  // the slot of plane k is free
+    if (k < kend) store_plane(k + 2);
+    __syncthreads();
+  }
+}
+
 template <typename T>
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
                         const double *stc, int first, const double *u, const double *v, const double *w,
                         const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
                         T *psi_out, T *flux_x, T *flux_y, T *flux_z) {
   FZ_TRY(fits_buffers(c, "acdi_step"));
-  FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", k_acdi_step<T>, fz_grid_interior(c->d), fz_block3(),
-            0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc, first, u, v, w,
-            normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
+  // 0: by precision, measured at 256x256x512: LDS staging for FP32 (619
+  // against 663 us), the one-cell kernel for FP64 (1230 against 1471 us: the
+  // staged kernel needs 182 VGPRs, two waves per SIMD, and is VALU-bound)
+  const int variant = c->acdi_variant ? c->acdi_variant : (sizeof(T) == 4 ? 1 : 2);
+  if (variant == 1) {  // fields staged in LDS
+    constexpr int KZ = 16;
+    const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
+    FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", (k_acdi_lds<T, KZ>), grid, dim3(ACD_MX, ACD_MY), 0,
+              c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc, first, u, v, w,
+              normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
+  } else {
+    FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", k_acdi_step<T>, fz_grid_interior(c->d),
+              fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc,
+              first, u, v, w, normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
+  }
   FZ_LAUNCHED();
   return FZ_OK;
 }

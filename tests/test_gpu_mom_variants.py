@@ -40,3 +40,19 @@ def test_momentum_variants_agree_bitwise(name, variant, monkeypatch):
     for k in FIELDS:
         assert np.array_equal(a[k], b[k]), k
     assert np.array_equal(sa, sb)
+
+
+@pytest.mark.parametrize("env", ["FZ_ACDI_VARIANT=1", "FZ_ACDI_VARIANT=2", "FZ_CORREC_KZ=0", "FZ_CORREC_KZ=8"])
+@pytest.mark.parametrize("name", list(SPECS))
+def test_acdi_and_correction_variants_agree_bitwise(name, env, monkeypatch):
+    """The ACDI step staged in LDS or one cell per thread, and the correction
+    step one cell per thread or marching in z, give the defaults' fields."""
+    if name not in _reference:
+        _reference[name] = run(SPECS[name], 8, 0, monkeypatch)
+    a, sa = _reference[name]
+    key, val = env.split("=")
+    monkeypatch.setenv(key, val)
+    b, sb = run(SPECS[name], 8, None, monkeypatch)
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(sa, sb)
