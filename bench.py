@@ -54,16 +54,23 @@ WORKLOAD_NAMES = {"c2": "acdi_rising_bubble_256x256x512", "c0": "acdi_rising_bub
                   "c3": "vof_droplets_forced_256x256x256"}
 
 
+KERNEL_SYMBOLS = {"mom_fused": ("k_mom_lds", "k_mom_fused"),
+                  "correc_step": ("k_correc_lds", "k_correc_step_mz", "k_correc_step"),
+                  "acdi_step": ("k_acdi_step", "k_acdi_lds")}
+
+
 def profiled_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
     path = os.path.join(ROOT, "profiles", f"step_kernels_{WORKLOAD_NAMES[workload]}.txt")
     if not os.path.exists(path):
         return None
+    # the kernels a launch name stands for (variants share one name)
+    symbols = KERNEL_SYMBOLS.get(kernel, ("k_" + kernel,))
     for line in open(path):
         cols = line.split()
         # kernel names carry their template arguments (the FP64 instances here)
-        if len(cols) >= 6 and cols[0].split("<")[0] == "k_" + kernel and "float" not in cols[0]:
+        if len(cols) >= 6 and cols[0].split("<")[0] in symbols and "float" not in cols[0]:
             try:
                 return round((float(cols[4]) + float(cols[5])) * 1e6)
             except ValueError:
