@@ -70,11 +70,11 @@ def profiled_traffic(kernel, workload):
     for line in open(path):
         cols = line.split()
         # kernel names carry their template arguments (the FP64 instances here)
-        if len(cols) >= 6 and cols[0].split("<")[0] in symbols and "float" not in cols[0]:
+        if len(cols) >= 6 and cols[0].split("<")[0] in symbols and "float" not in cols[0] and cols[1].isdigit():
             try:
                 return round((float(cols[4]) + float(cols[5])) * 1e6)
             except ValueError:
-                return None
+                continue  # not a table row
     return None
 
 
