@@ -97,7 +97,8 @@ struct fz_ctx {
   int mom_variant = 0;  // momentum kernel: 0 by precision (8 for FP64, 2 for FP32 phase fields), 1 staged
                         // loads, 2 plain, 4 or 5 waves per EU, 8 stencil fields in LDS (k_mom_lds)
   int tk64 = 16;        // rows per chunk of the FP64 Thomas kernel: 8, 16 or 32
-  int tk32 = 32;        // the same for the FP32 one; negative: two columns per lane, -8 or -16
+  int tk32 = -8;        // the same for the FP32 one; negative: two columns per lane, -8 or -16 (the default,
+                        // -8, is the tuner's choice at 256x256x512 and 256^3)
   int acdi_variant = 0;  // ACDI step: 0 by precision (1 for FP32, 2 for FP64), 1 fields staged in LDS
                          // (k_acdi_lds), 2 one cell per thread
   int correc_kz = 1;    // correction step: 0 one cell per thread, 1 dp, u, v staged in LDS (k_correc_lds),

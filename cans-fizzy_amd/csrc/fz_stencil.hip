@@ -1262,8 +1262,8 @@ struct LdsAt {
   }
 };
 
-template <typename PT, typename HT, int KZ>
-__global__ void __launch_bounds__(MOM_MX *MOM_MY) k_mom_lds(
+template <typename PT, typename HT, int KZ, int WPE>  // WPE: waves per SIMD the registers must allow
+__global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_per_eu(WPE))) k_mom_lds(
     fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
     double drho, double mu, double dmu, double factor1_v, double factor2_v, int first, double factor12_v,
     double dt_r_v, const double *__restrict__ stc, double surf_factor, double rho0, double bfx, double bfy,
@@ -1391,7 +1391,8 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
   if (variant == 8) {  // stencil fields staged in LDS
     constexpr int KZ = 16;
     const dim3 grid((c->d.n1 + MOM_MX - 1) / MOM_MX, (c->d.n2 + MOM_MY - 1) / MOM_MY, (c->d.n3 + KZ - 1) / KZ);
-    FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", (k_mom_lds<PT, HT, KZ>), grid,
+    // (3 waves per SIMD fit FP32 phase fields' registers, 169 -> 168 VGPRs)
+    FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", (k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1>), grid,
               dim3(MOM_MX, MOM_MY), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1], rho12[0] - rho12[1],
               mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12, dt_r, stc, surf_factor, rho0, bforce[0],
               bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1], rho_av[2], fx, fy, fz, u, v, w,

@@ -88,7 +88,7 @@ extern "C" int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size
     if (rc == FZ_OK) rc = fz_case_init(sc);
     if (rc == FZ_OK && hipEventCreate(&a) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
     if (rc == FZ_OK && hipEventCreate(&b) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
-    best = {c ? c->d.tile_y : 8, c ? c->mom_variant : 0, mixed ? (c ? c->tk32 : 32) : (c ? c->tk64 : 16),
+    best = {c ? c->d.tile_y : 8, c ? c->mom_variant : 0, mixed ? (c ? c->tk32 : -8) : (c ? c->tk64 : 16),
             c ? c->thomas_rc : 0, 0.0f};
     if (rc == FZ_OK) rc = trial(sc, c, best, mixed, trial_steps, a, b, &best.ms);
     const int tiles[] = {0, 2, 4, 8, 16};
