@@ -80,6 +80,7 @@ extern "C" int fz_ctx_create(fz_ctx **out, const int n[3], const double l[3], co
   // the device (measured: slower at 256x256 columns, faster at 512x512)
   c->thomas_rc = (long)n[0] * n[1] >= 262144 ? 1 : 0;
   if (const char *e = getenv("FZ_THOMAS_RC")) c->thomas_rc = atoi(e);
+  if (const char *e = getenv("FZ_THOMAS_PRE")) c->thomas_pre = atoi(e);
   if (const char *e = getenv("FZ_THOMAS_ROWS32")) c->tk32 = atoi(e);
   if (const char *e = getenv("FZ_THOMAS_ROWS64")) c->tk64 = atoi(e);
   c->nh = c->d.sz * (n[2] + 2);

@@ -58,6 +58,8 @@ struct fz_poisson {
   double *work = nullptr;   // spectral buffer, n1*n2*n3
   double *piv = nullptr;    // Thomas pivots, n1*n2*n3
   double *aux = nullptr;    // periodic z: gaussel_periodic's second solve, made at set-up, n1*n2*n3
+  double *zpre = nullptr;   // non-periodic z with precomputed pivots: 1/pivot and d of every row, n1*n2*n3
+  double *dpre = nullptr;   // each
   int periodic_z = 0;
   int rhsb_zero = 0;  // all boundary right-hand sides vanish (homogeneous pressure BCs)
   long null_col = -1;  // spectral column of the operator's null mode, or -1
@@ -103,6 +105,7 @@ struct fz_ctx {
                          // (k_acdi_lds), 2 one cell per thread
   int correc_kz = 1;    // correction step: 0 one cell per thread, 1 dp, u, v staged in LDS (k_correc_lds),
                         // 8 or 16 cells per thread marching in z (k_correc_step_mz)
+  int thomas_pre = 0;   // 1: Thomas solves with the pivots made once at set-up (k_thomas_pre)
   int thomas_rc = 0;    // 1: Thomas kernels without a pivot array (the elimination re-run per chunk);
                         // set at creation from the column count
 };

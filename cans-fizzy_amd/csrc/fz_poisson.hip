@@ -784,6 +784,76 @@ __global__ void __launch_bounds__(64) k_thomas_rc(T *__restrict__ W, double *__r
   }
 }
 
+// The pivots of dgtsv_homebrewed (src/solver.f90:151-177) depend on the
+// operator only: k_piv_pre makes them once, with thomas_chunked's operations
+// (Z: the reciprocal pivot z of every row, D: d = c z), and k_thomas_pre
+// solves with them. The values are k_thomas's bit for bit; the bytes moved
+// are the same (Z read in the forward sweep where k_thomas writes D), but no
+// division lies on the per-row dependency chain.
+__global__ void k_piv_pre(double *__restrict__ Z, double *__restrict__ D, const double *__restrict__ lambdaxy,
+                          const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ c,
+                          int n1, int n2, int n3) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ps = (long)n1 * n2;
+  if (idx >= ps) return;
+  const double lam = lambdaxy[idx];
+  double d = 0.0;
+  for (int l = 0; l < n3; l++) {
+    const double bb = b[l] + lam;
+    const double z = l == 0 ? 1.0 / (bb + FZ_EPS) : 1.0 / (bb - a[l] * d + FZ_EPS);
+    d = (l == 0 ? c[0] : c[l]) * z;
+    Z[idx + (long)l * ps] = z;
+    D[idx + (long)l * ps] = d;
+  }
+}
+
+template <int TK, typename T>
+__global__ void __launch_bounds__(64) k_thomas_pre(T *__restrict__ W, const double *__restrict__ Z,
+                                                   const double *__restrict__ D, const double *__restrict__ a,
+                                                   int n1, int n2, int n3, long gauge) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ps = (long)n1 * n2;
+  if (idx >= ps) return;
+  T *col = W + idx;
+  const double *zc = Z + idx, *dc = D + idx;
+  const bool pin = idx == gauge;
+  double pprev = 0.0;
+  for (int l0 = 0; l0 < n3; l0 += TK) {
+    double r[TK], z[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < n3) {
+        r[t] = (double)col[(long)(l0 + t) * ps];
+        z[t] = zc[(long)(l0 + t) * ps];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++) {
+      const int l = l0 + t;
+      if (l < n3) {
+        pprev = l == 0 ? r[t] * z[t] : (r[t] - a[l] * pprev) * z[t];
+        if (pin && l == n3 - 1) pprev = 0.0;
+        col[(long)l * ps] = (T)pprev;
+      }
+    }
+  }
+  double pnext = pprev;
+  for (int h0 = n3 - 2; h0 >= 0; h0 -= TK) {
+    double r[TK], dd[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        r[t] = (double)col[(long)(h0 - t) * ps];
+        dd[t] = dc[(long)(h0 - t) * ps];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        pnext = r[t] - dd[t] * pnext;
+        col[(long)(h0 - t) * ps] = (T)pnext;
+      }
+  }
+}
+
 // k_thomas for FP32 columns, two adjacent columns per lane: each load moves
 // 8 bytes of W and 16 of pivots per lane, as the FP64 kernel does with one
 // column. The per-column arithmetic is k_thomas<TK, float>'s exactly.
@@ -1132,6 +1202,8 @@ int fz_poisson_destroy(fz_poisson *p) {
   for (int q = 0; q < 9; q++)
     if (bufs[q]) (void)hipFree(bufs[q]);
   if (p->aux) (void)hipFree(p->aux);
+  if (p->zpre) (void)hipFree(p->zpre);
+  if (p->dpre) (void)hipFree(p->dpre);
   if (p->work32) (void)hipFree(p->work32);
   if (p->bcopy) (void)hipFree(p->bcopy);
   if (p->rslots) (void)hipFree(p->rslots);
@@ -1233,11 +1305,21 @@ extern "C" int fz_initsolver(fz_ctx *ctx, const double bcpre[6], double *lambdax
   if (e == hipSuccess) e = hipMalloc((void **)&P->work, sizeof(double) * N);
   if (e == hipSuccess) e = hipMalloc((void **)&P->piv, sizeof(double) * N);
   if (e == hipSuccess && P->periodic_z) e = hipMalloc((void **)&P->aux, sizeof(double) * N);
+  if (e == hipSuccess && !P->periodic_z && ctx->thomas_pre) {
+    e = hipMalloc((void **)&P->zpre, sizeof(double) * N);
+    if (e == hipSuccess) e = hipMalloc((void **)&P->dpre, sizeof(double) * N);
+  }
   if (e != hipSuccess) {
     fz_poisson_destroy(P);
     return fz_set_error(FZ_ERR_HIP, "fz_initsolver: %s", hipGetErrorString(e));
   }
   ctx->pois = P;
+  if (P->zpre) {  // the pivots, once
+    const long ps = (long)n1 * n2;
+    FZ_LAUNCH(ctx, "piv_pre", k_piv_pre, dim3((int)((ps + 255) / 256)), dim3(256), 0, P->zpre, P->dpre, P->lambdaxy,
+              P->a, P->b, P->c, n1, n2, n3);
+    FZ_LAUNCHED();
+  }
   if (P->periodic_z) {  // the operator's second solve of gaussel_periodic, once
     const long ps = (long)n1 * n2;
     FZ_LAUNCH(ctx, "periodic_p2", k_periodic_p2, dim3((int)((ps + 255) / 256)), dim3(256), 0, P->piv, P->aux,
@@ -1380,7 +1462,11 @@ static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum) {
     const int tk = mixed ? ctx->tk32 : ctx->tk64;
     const dim3 grid((int)((ps + 63) / 64)), blk(64);
     const char *name = mixed ? "thomas_f32" : "thomas";
-    if (ctx->thomas_rc && tk > 0) {  // no pivot array: the elimination is re-run per chunk
+    if (P->zpre && tk > 0) {  // pivots made at set-up
+      if (tk == 8) FZ_LAUNCH(ctx, name, (k_thomas_pre<8, T>), grid, blk, 0, W, P->zpre, P->dpre, P->a, n1, n2, n3,
+                             gauge);
+      else FZ_LAUNCH(ctx, name, (k_thomas_pre<16, T>), grid, blk, 0, W, P->zpre, P->dpre, P->a, n1, n2, n3, gauge);
+    } else if (ctx->thomas_rc && tk > 0) {  // no pivot array: the elimination is re-run per chunk
       if (tk == 8) FZ_LAUNCH(ctx, name, (k_thomas_rc<8, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c,
                              n1, n2, n3, gauge, kNoBatch);
       else FZ_LAUNCH(ctx, name, (k_thomas_rc<16, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,

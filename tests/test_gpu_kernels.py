@@ -278,6 +278,14 @@ def test_poisson_solve_bitwise_without_pivot_array(n, cbc, rows, rng, monkeypatc
     poisson_solve_bitwise(n, cbc, rng)
 
 
+@pytest.mark.parametrize("n,cbc", NOT_PERIODIC_Z, ids=[f"{'x'.join(map(str, a))}-{b}" for a, b in NOT_PERIODIC_Z])
+def test_poisson_solve_bitwise_with_pivots_made_at_set_up(n, cbc, rng, monkeypatch):
+    """The Thomas kernel reading pivots made once by fz_initsolver (a context
+    reads FZ_THOMAS_PRE when it is created) matches the oracle bit for bit."""
+    monkeypatch.setenv("FZ_THOMAS_PRE", "1")
+    poisson_solve_bitwise(n, cbc, rng)
+
+
 def poisson_solve_bitwise(n, cbc, rng):
     pr = Pair(n, [1.0, 0.9, 1.7], cbc, gr=0.6 if cbc[4:] != "PP" else 0.0)
     try:
