@@ -872,18 +872,13 @@ __global__ void k_clip(long nh, double rmin, double rmax, T *__restrict__ p) {
     p[q] = fmin(fmax((T)rmin, p[q]), (T)rmax);
 }
 
-// cmpt_norm_curv_youngs, src/two_fluid.f90:81-174
-// (T = float: the mixed-precision step, FP32 fields and arithmetic)
-template <typename T>
-__global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restrict__ dzci,
-                                                const double *__restrict__ dzfi, double dxi_d, double dyi_d,
-                                                const T *__restrict__ psi, T *__restrict__ normx,
-                                                T *__restrict__ normy, T *__restrict__ normz,
-                                                T *__restrict__ kappa) {
-  CELL_IJK(d);
-  (void)r;
+// cmpt_norm_curv_youngs, src/two_fluid.f90:81-174, at one cell: the 27-point
+// phase field through the accessor PS (T = float: the mixed-precision step,
+// FP32 fields and arithmetic)
+template <typename T, class A>
+__device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dyi_d, double dzci_k, double dzci_km,
+                                            double dzfi_k, T &onx, T &ony, T &onz, T &okap) {
   const T dxi = (T)dxi_d, dyi = (T)dyi_d;
-  const BufAt<T> PS = buf_at(psi, d, q);
 #define P_(a, b, c) PS(a, b, c)
   const T mmm = P_(-1, -1, -1), mcm = P_(-1, 0, -1), mpm = P_(-1, 1, -1), cmm = P_(0, -1, -1), ccm = P_(0, 0, -1),
                cpm = P_(0, 1, -1), pmm = P_(1, -1, -1), pcm = P_(1, 0, -1), ppm = P_(1, 1, -1);
@@ -892,7 +887,7 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
   const T mmp = P_(-1, -1, 1), mcp = P_(-1, 0, 1), mpp = P_(-1, 1, 1), cmp = P_(0, -1, 1), ccp = P_(0, 0, 1),
                cpp = P_(0, 1, 1), pmp = P_(1, -1, 1), pcp = P_(1, 0, 1), ppp = P_(1, 1, 1);
 #undef P_
-  const T zc0 = (T)dzci[k], zcm = (T)dzci[k - 1];
+  const T zc0 = (T)dzci_k, zcm = (T)dzci_km;
   T mx[8], my[8], mz[8];
   mx[0] = (T)0.25 * ((pcc + ppc + pcp + ppp) - (ccc + cpc + ccp + cpp)) * dxi;
   mx[1] = (T)0.25 * ((pcc + pmc + pcp + pmp) - (ccc + cmc + ccp + cmp)) * dxi;
@@ -929,12 +924,37 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
   const T ny = (T).125 * (my[0] + my[1] + my[2] + my[3] + my[4] + my[5] + my[6] + my[7]);
   const T nz = (T).125 * (mz[0] + mz[1] + mz[2] + mz[3] + mz[4] + mz[5] + mz[6] + mz[7]);
   const T nrm = sqrt(nx * nx + ny * ny + nz * nz) + (T)FZ_EPS;
-  normx[q] = nx / nrm;
-  normy[q] = ny / nrm;
-  normz[q] = nz / nrm;
-  kappa[q] = -((T)0.25 * ((mx[0] + mx[1] + mx[2] + mx[3]) - (mx[4] + mx[5] + mx[6] + mx[7])) * dxi +
+  onx = nx / nrm;
+  ony = ny / nrm;
+  onz = nz / nrm;
+  okap = -((T)0.25 * ((mx[0] + mx[1] + mx[2] + mx[3]) - (mx[4] + mx[5] + mx[6] + mx[7])) * dxi +
                (T)0.25 * ((my[0] + my[2] + my[4] + my[6]) - (my[1] + my[3] + my[5] + my[7])) * dyi +
-               (T)0.25 * ((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7])) * (T)dzfi[k]);
+               (T)0.25 * ((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7])) * (T)dzfi_k);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restrict__ dzci,
+                                                const double *__restrict__ dzfi, double dxi_d, double dyi_d,
+                                                const T *__restrict__ psi, T *__restrict__ normx,
+                                                T *__restrict__ normy, T *__restrict__ normz,
+                                                T *__restrict__ kappa) {
+  CELL_IJK(d);
+  (void)r;
+  T nx, ny, nz, ka;
+  youngs_cell<T>(buf_at(psi, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka);
+  normx[q] = nx;
+  normy[q] = ny;
+  normz[q] = nz;
+  kappa[q] = ka;
+}
+
+template <typename T>
+static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T *normy, T *normz, T *kappa) {
+  FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
+  FZ_LAUNCH(c, name, k_youngs<T>, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
+            c->dli[1], psi, normx, normy, normz, kappa);
+  FZ_LAUNCHED();
+  return FZ_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1621,11 +1641,7 @@ int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi) {
 }
 
 int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float *normz, float *kappa) {
-  FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
-  FZ_LAUNCH(c, "youngs_f32", k_youngs<float>, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi,
-            c->dli[0], c->dli[1], psi, normx, normy, normz, kappa);
-  FZ_LAUNCHED();
-  return FZ_OK;
+  return launch_youngs<float>(c, "youngs_f32", psi, normx, normy, normz, kappa);
 }
 
 extern "C" int fz_vof_thinc_cmpt_phi(fz_ctx *c, double beta, const double *psi, double *phi) {
@@ -1678,11 +1694,7 @@ extern "C" int fz_cmpt_norm_curv_youngs(fz_ctx *c, const double *psi, double *no
                                         double *kappa) {
   if (!c || !psi || !normx || !normy || !normz || !kappa)
     return fz_set_error(FZ_ERR_ARG, "fz_cmpt_norm_curv_youngs: null argument");
-  FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
-  FZ_LAUNCH(c, "youngs", k_youngs<double>, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
-                     c->dli[1], psi, normx, normy, normz, kappa);
-  FZ_LAUNCHED();
-  return FZ_OK;
+  return launch_youngs<double>(c, "youngs", psi, normx, normy, normz, kappa);
 }
 
 extern "C" int fz_mom_xyz_ad(fz_ctx *c, const double rho12[2], const double mu12[2], const double *psiflx_x,
