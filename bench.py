@@ -434,7 +434,7 @@ def main():
         achieved = bytes_per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, args.workload),
-                "kernel": dom,
+                "kernel": dom, "symbols": list(KERNEL_SYMBOLS.get(dom, ("k_" + dom,))),
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bytes_per_launch,
                 "kernel_time_share": round(ms / total_ms, 4)}
     step_bw = STEP_ALG_BYTES_PER_CELL * cells / (elapsed / args.steps) / 1e9
