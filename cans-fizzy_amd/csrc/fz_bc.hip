@@ -11,7 +11,6 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
-#include <unordered_map>
 
 #include "fz_internal.h"
 
@@ -249,10 +248,71 @@ __global__ void k_bc_gather(GatherSet<T> g) {
 }
 
 namespace {
-struct Expr {
-  long src;  // original cell, or -1 when the chain starts from a constant
-  int nops;
-  SubOp ops[BC_MAXOPS];
+// The cells a boundary operation can write: those with i, j or k in the
+// layers {0, n, n+1} of their direction (a staggered wall value sits on
+// layer n), numbered densely: the three z layers, then the three y rows of
+// the other layers, then the three x cells of the other rows.
+struct ShellIndex {
+  long sy, sz;
+  int n1, n2, n3;
+  long size;
+  explicit ShellIndex(const fz_dims &d) : sy(d.sy), sz(d.sz), n1(d.n1), n2(d.n2), n3(d.n3) {
+    size = 3 * sz + 3 * sy * (n3 - 1) + 3 * (long)(n2 - 1) * (n3 - 1);
+  }
+  static int layer(long x, int n) { return x == 0 ? 0 : (x == n ? 1 : (x == n + 1 ? 2 : -1)); }
+  long pos3(long i, long j, long k) const {  // -1 for a cell no operation can write
+    const int lz = layer(k, n3);
+    if (lz >= 0) return lz * sz + j * sy + i;
+    const long kk = k - 1;  // 0 .. n3-2
+    long base = 3 * sz;
+    const int ly = layer(j, n2);
+    if (ly >= 0) return base + (kk * 3 + ly) * sy + i;
+    base += 3 * sy * (n3 - 1);
+    const int lx = layer(i, n1);
+    if (lx >= 0) return base + (kk * (n2 - 1) + (j - 1)) * 3 + lx;
+    return -1;
+  }
+};
+
+// The distinct chains met while composing, each with its successors under
+// every operation (memo: chain x operation -> chain, -1 not yet made)
+struct ChainTable {
+  std::vector<BcChain> ch;
+  std::vector<int> memo;
+  int nop;
+  explicit ChainTable(int nop_) : nop(nop_) { add(BcChain()); }
+  int add(const BcChain &c) {
+    for (size_t q = 0; q < ch.size(); q++)
+      if (memcmp(&ch[q], &c, sizeof(c)) == 0) return (int)q;
+    ch.push_back(c);
+    memo.resize(ch.size() * nop, -1);
+    return (int)ch.size() - 1;
+  }
+  // chain a followed by operation o (number g); -2 when too long
+  int next(int a, const SubOp &o, int g) {
+    int &m = memo[(size_t)a * nop + g];
+    if (m != -1) return m;
+    BcChain c;
+    memset(&c, 0, sizeof(c));
+    if (o.mode == 3) {
+      c.nops = 1;
+      c.mode[0] = 3;
+      c.factor[0] = o.factor;
+      c.sgn[0] = o.sgn;
+    } else {
+      c = ch[a];
+      if (o.mode == 2) {
+        if (c.nops == BC_MAXOPS) return memo[(size_t)a * nop + g] = -2;
+        c.mode[c.nops] = 2;
+        c.factor[c.nops] = o.factor;
+        c.sgn[c.nops] = o.sgn;
+        c.nops++;
+      }
+    }
+    const int id = add(c);
+    memo[(size_t)a * nop + g] = id;  // add() may have moved the memo
+    return id;
+  }
 };
 }  // namespace
 
@@ -262,12 +322,21 @@ static void plane_geometry(const fz_dims &d, int dir, int &na, int &nb) {
   else { na = d.n1 + 2; nb = d.n2 + 2; }
 }
 
+// Each written cell holds (source cell or -1, chain): the phases are applied
+// in order, an operation's destination taking its source's pair extended by
+// the operation (a source not written so far is its own original value).
 static fz_bcmap *compose(fz_ctx *c, const std::vector<Phase> &ph, int fi) {
   fz_bcmap *m = new fz_bcmap();
   const fz_dims &d = c->d;
   if (c->nh >= 2147483647L) return m;
-  std::unordered_map<long, Expr> E;
-  E.reserve(8 * (size_t)((d.n1 + 2) * (d.n2 + 2) + (d.n1 + 2) * (d.n3 + 2) + (d.n2 + 2) * (d.n3 + 2)));
+  const ShellIndex sh(d);
+  std::vector<long> wsrc(sh.size), wsp(sh.size);  // source cell and its shell position (-1: none)
+  std::vector<int> wch(sh.size, -1);              // chain, -1: not written
+  int nop = 0;
+  for (const Phase &P : ph) nop += P.ops.f[fi].nops;
+  ChainTable tab(nop > 0 ? nop : 1);
+  int g0 = 0;
+  size_t nw = 0;  // cells written
   for (const Phase &P : ph) {
     const FieldOps &fo = P.ops.f[fi];
     if (fo.nops == 0) continue;
@@ -279,61 +348,68 @@ static fz_bcmap *compose(fz_ctx *c, const std::vector<Phase> &ph, int fi) {
         if (P.dir == 0) { base = d.sy * a + d.sz * b; st = 1; }
         else if (P.dir == 1) { base = a + d.sz * b; st = d.sy; }
         else { base = a + d.sy * b; st = d.sz; }
+        // the cell at offset x along the phase's direction, in shell numbering
+        auto spos = [&](long x) {
+          return P.dir == 0 ? sh.pos3(x, a, b) : (P.dir == 1 ? sh.pos3(a, x, b) : sh.pos3(a, b, x));
+        };
         for (int s = 0; s < fo.nops; s++) {
           const SubOp &o = fo.op[s];
-          const long dst = base + st * o.dst, src = base + st * o.src;
-          Expr e;
-          if (o.mode == 3) {
-            e.src = -1;
-            e.nops = 1;
-            e.ops[0] = o;
-          } else {
-            auto it = E.find(src);
-            if (it != E.end()) e = it->second;
-            else { e.src = src; e.nops = 0; }
-            if (o.mode == 2) {
-              if (e.nops == BC_MAXOPS) return m;
-              e.ops[e.nops++] = o;
-            }
+          const long src = base + st * o.src;
+          long esrc = -1, esp = -1;
+          int ech = 0;
+          if (o.mode != 3) {
+            const long ps = spos(o.src);
+            if (ps >= 0 && wch[ps] >= 0) { esrc = wsrc[ps]; esp = wsp[ps]; ech = wch[ps]; }
+            else { esrc = src; esp = ps; }
           }
-          E[dst] = e;
+          ech = tab.next(ech, o, g0 + s);
+          if (ech == -2) return m;  // longer than a chain holds: the ordered phases
+          const long pd = spos(o.dst);
+          if (pd < 0) return m;  // not a boundary layer: the ordered phases
+          nw += wch[pd] < 0;
+          wsrc[pd] = esrc;
+          wsp[pd] = esp;
+          wch[pd] = ech;
         }
       }
+    g0 += fo.nops;
   }
-  std::vector<std::pair<long, const Expr *>> recs;
-  recs.reserve(E.size());
-  for (const auto &kv : E) {
-    if (kv.second.src >= 0 && E.count(kv.second.src)) return m;  // a written cell is also read: not a gather
-    recs.push_back({kv.first, &kv.second});
+  // the written cells in increasing linear order
+  std::vector<std::pair<long, long>> recs;  // (cell, shell position)
+  recs.reserve(nw);
+  auto visit = [&](long i, long j, long k) {
+    const long p = sh.pos3(i, j, k);
+    if (wch[p] >= 0) recs.push_back({k * d.sz + j * d.sy + i, p});
+  };
+  for (long k = 0; k <= d.n3 + 1; k++)
+    for (long j = 0; j <= d.n2 + 1; j++) {
+      if (ShellIndex::layer(k, d.n3) >= 0 || ShellIndex::layer(j, d.n2) >= 0) {
+        for (long i = 0; i <= d.n1 + 1; i++) visit(i, j, k);
+      } else {
+        visit(0, j, k);
+        visit(d.n1, j, k);
+        visit(d.n1 + 1, j, k);
+      }
+    }
+  for (const auto &rp : recs) {
+    const long pe = wsp[rp.second];
+    if (pe >= 0 && wch[pe] >= 0) return m;  // a written cell is also read: not a gather
   }
-  std::sort(recs.begin(), recs.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  // chain numbers in order of first use over the sorted cells
+  std::vector<int> renum(tab.ch.size(), -1);
+  std::vector<BcChain> chains;
   std::vector<int> hdst(recs.size()), hsrc(recs.size());
   std::vector<unsigned char> hcid(recs.size());
-  std::vector<BcChain> chains;
-  std::map<std::string, int> chain_id;
   for (size_t r = 0; r < recs.size(); r++) {
-    const Expr &e = *recs[r].second;
-    BcChain ch;
-    memset(&ch, 0, sizeof(ch));
-    ch.nops = e.nops;
-    for (int o = 0; o < e.nops; o++) {
-      ch.mode[o] = e.ops[o].mode;
-      ch.factor[o] = e.ops[o].factor;
-      ch.sgn[o] = e.ops[o].sgn;
-    }
-    const std::string key((const char *)&ch, sizeof(ch));
-    auto it = chain_id.find(key);
-    int id;
-    if (it == chain_id.end()) {
+    const long p = recs[r].second;
+    int &id = renum[wch[p]];
+    if (id < 0) {
       id = (int)chains.size();
       if (id > 255) return m;
-      chains.push_back(ch);
-      chain_id[key] = id;
-    } else {
-      id = it->second;
+      chains.push_back(tab.ch[wch[p]]);
     }
     hdst[r] = (int)recs[r].first;
-    hsrc[r] = (int)e.src;
+    hsrc[r] = (int)wsrc[p];
     hcid[r] = (unsigned char)id;
   }
   const int n = (int)recs.size();

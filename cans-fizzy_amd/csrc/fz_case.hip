@@ -88,6 +88,7 @@ struct fz_case {
   double stc_host[FZ_STC_N] = {0};
   bool stc_valid = false;
   bool use_graphs = false;
+  bool check_pending = false;  // a step's check is enqueued and not yet read
   struct StepGraph {
     hipGraphExec_t exec = nullptr;
     const void *key[2] = {nullptr, nullptr};
@@ -640,49 +641,105 @@ extern "C" int fz_case_set_graphs(fz_case *cs, int on) {
   return FZ_OK;
 }
 
-extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
-  if (!cs || nsteps < 0) return fz_set_error(FZ_ERR_ARG, "fz_case_step: bad argument");
+// The checks of src/main.f90:530-562 at the end of a step whose number is a
+// multiple of icheck: chkdt and chkdiv enqueued with one read-back of their
+// device scalars (gamma, divergence sum and maximum, 1/dt) into h_small[16..19]
+static int check_enqueue(fz_case *cs) {
+  fz_ctx *c = cs->ctx;
+  double **f = cs->f;
+  FZ_TRY(fz_chkdt_adv_dev(c, f[F_U], f[F_V], f[F_W], cs->dsc + 3));
+  FZ_TRY(fz_chkdiv_dev(c, f[F_U], f[F_V], f[F_W], cs->dsc + 1));
+  FZ_HIP(hipMemcpyAsync(c->h_small + 16, cs->dsc, sizeof(double) * 4, hipMemcpyDeviceToHost, c->st));
+  cs->check_pending = true;
+  return FZ_OK;
+}
+
+// the host side of the checks once the read-back has landed: the new dt
+// (src/main.f90:532-550) and the stop conditions (small dt, divergence)
+static void check_finish(fz_case *cs, int *kill) {
   fz_ctx *c = cs->ctx;
   const fz_params &P = cs->prm;
-  double **f = cs->f;
+  const double *h = c->h_small + 16;
+  if (cs->gam_on_device) cs->sc[S_GAM] = h[0];
+  const double dt_cfl = fz_chkdt_finish(c, h[3], P.is_solve_ns, P.is_track_interface, P.vof, P.mu12, P.rho12,
+                                        P.sigma, P.gacc, cs->sc[S_GAM], cs->sc[S_SEPS]);
+  cs->sc[S_DTCFL] = dt_cfl;
+  double d = fmin(P.cfl * dt_cfl, P.dtmax);
+  if (P.dt_f > 0.0) {
+    if (P.dt_f > d) *kill = 1;
+    d = P.dt_f;
+  }
+  if (dt_cfl < FZ_EPS * 1e7) *kill = 1;  // small, src/param.f90:21
+  cs->sc[S_DT] = d;
+  cs->sc[S_DIVTOT] = h[1];
+  cs->sc[S_DIVMAX] = h[2];
+  if (cs->sc[S_DIVMAX] > FZ_EPS * 1e7 || std::isnan(cs->sc[S_DIVTOT])) *kill = 1;
+  cs->check_pending = false;
+}
+
+// one step enqueued (src/main.f90:414-529), with its checks when due
+static int step_enqueue(fz_case *cs) {
+  const fz_params &P = cs->prm;
+  StepArgs a;
+  a.dt = cs->sc[S_DT];
+  const double dto = cs->sc[S_DTO];
+  a.first = cs->sc[S_FIRST] != 0.0;
+  cs->sc[S_ISTEP] += 1.0;
+  cs->sc[S_TIME] += a.dt;
+  a.dt_r = a.dt / dto;  // src/main.f90:422-424, src/rk.f90:53-55
+  const double tm1 = 1.0 + 0.5 * a.dt_r, tm2 = -0.5 * a.dt_r;
+  a.factor1 = tm1 * a.dt;
+  a.factor2 = tm2 * a.dt;
+  a.factor12 = a.factor1 + a.factor2;
+  const bool use_graph = !a.first && graph_ok(cs);
+  a.stc = nullptr;
+  if (use_graph) {
+    a.stc = cs->dstc;
+    FZ_TRY(set_stc(cs, a));
+    FZ_TRY(graph_step(cs, a));
+  } else {
+    FZ_TRY(enqueue_step(cs, a));
+  }
+  cs->sc[S_FIRST] = 0.0;
+  cs->sc[S_DTO] = a.dt;  // dto = dt, src/main.f90:523
+  if (!P.vof) cs->gam_on_device = true;
+  const int istep = (int)cs->sc[S_ISTEP];
+  if (P.icheck > 0 && istep % P.icheck == 0) FZ_TRY(check_enqueue(cs));  // src/main.f90:530-562
+  return FZ_OK;
+}
+
+extern "C" int fz_case_step(fz_case *cs, int nsteps, int *killed) {
+  if (!cs || nsteps < 0) return fz_set_error(FZ_ERR_ARG, "fz_case_step: bad argument");
+  if (cs->check_pending) return fz_set_error(FZ_ERR_STATE, "fz_case_step: fz_case_step_end has not been called");
   int kill = 0;
   for (int s = 0; s < nsteps && !kill; s++) {
-    StepArgs a;
-    a.dt = cs->sc[S_DT];
-    const double dto = cs->sc[S_DTO];
-    a.first = cs->sc[S_FIRST] != 0.0;
-    cs->sc[S_ISTEP] += 1.0;
-    cs->sc[S_TIME] += a.dt;
-    a.dt_r = a.dt / dto;  // src/main.f90:422-424, src/rk.f90:53-55
-    const double tm1 = 1.0 + 0.5 * a.dt_r, tm2 = -0.5 * a.dt_r;
-    a.factor1 = tm1 * a.dt;
-    a.factor2 = tm2 * a.dt;
-    a.factor12 = a.factor1 + a.factor2;
-    const bool use_graph = !a.first && graph_ok(cs);
-    a.stc = nullptr;
-    if (use_graph) {
-      a.stc = cs->dstc;
-      FZ_TRY(set_stc(cs, a));
-      FZ_TRY(graph_step(cs, a));
-    } else {
-      FZ_TRY(enqueue_step(cs, a));
+    FZ_TRY(step_enqueue(cs));
+    if (cs->check_pending) {
+      FZ_HIP(hipStreamSynchronize(cs->ctx->st));
+      check_finish(cs, &kill);
     }
-    cs->sc[S_FIRST] = 0.0;
-    cs->sc[S_DTO] = a.dt;  // dto = dt, src/main.f90:523
-    if (!P.vof) cs->gam_on_device = true;
-    const int istep = (int)cs->sc[S_ISTEP];
-    if (P.icheck > 0 && istep % P.icheck == 0) {  // src/main.f90:530-562
-      double dtn;
-      FZ_TRY(new_dt(cs, &dtn, &kill));
-      if (cs->sc[S_DTCFL] < FZ_EPS * 1e7) kill = 1;  // small, src/param.f90:21
-      cs->sc[S_DT] = dtn;
-      FZ_TRY(fz_chkdiv_dev(c, f[F_U], f[F_V], f[F_W], cs->dsc + 1));
-      FZ_HIP(hipMemcpyAsync(c->h_small, cs->dsc + 1, sizeof(double) * 2, hipMemcpyDeviceToHost, c->st));
-      FZ_HIP(hipStreamSynchronize(c->st));
-      cs->sc[S_DIVTOT] = c->h_small[0];
-      cs->sc[S_DIVMAX] = c->h_small[1];
-      if (cs->sc[S_DIVMAX] > FZ_EPS * 1e7 || std::isnan(cs->sc[S_DIVTOT])) kill = 1;
-    }
+  }
+  if (killed) *killed = kill;
+  return FZ_OK;
+}
+
+// fz_case_step split for callers that step many cases: begin enqueues one
+// step and returns without waiting; end, once the step's work is done (or
+// after waiting for it), reads the check results that set the next dt.
+extern "C" int fz_case_step_begin(fz_case *cs, int *check_pending) {
+  if (!cs) return fz_set_error(FZ_ERR_ARG, "fz_case_step_begin: null case");
+  if (cs->check_pending) return fz_set_error(FZ_ERR_STATE, "fz_case_step_begin: the previous step has not ended");
+  FZ_TRY(step_enqueue(cs));
+  if (check_pending) *check_pending = cs->check_pending ? 1 : 0;
+  return FZ_OK;
+}
+
+extern "C" int fz_case_step_end(fz_case *cs, int *killed) {
+  if (!cs) return fz_set_error(FZ_ERR_ARG, "fz_case_step_end: null case");
+  int kill = 0;
+  if (cs->check_pending) {
+    FZ_HIP(hipStreamSynchronize(cs->ctx->st));
+    check_finish(cs, &kill);
   }
   if (killed) *killed = kill;
   return FZ_OK;

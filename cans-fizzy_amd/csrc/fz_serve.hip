@@ -171,23 +171,27 @@ static int admit(fz_server *s) {
 extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
   if (!s) return fz_set_error(FZ_ERR_ARG, "fz_server_round: null server");
   FZ_TRY(admit(s));
-  // one step of every active case, each on its own stream
+  // one step of every active case, each on its own stream: all are enqueued
+  // before any is waited for (a step's checks are read when it has ended)
   for (Running &r : s->active) {
     if (r.left <= 0) continue;
     FZ_HIP(hipEventRecord(r.a, r.ctx->st));
-    int killed = 0;
-    FZ_TRY(fz_case_step(r.cs, 1, &killed));
+    FZ_TRY(fz_case_step_begin(r.cs, nullptr));
     FZ_HIP(hipEventRecord(r.b, r.ctx->st));
-    r.killed = r.killed || killed;
-    r.left = killed ? 0 : r.left - 1;
   }
   std::vector<Running> still;
   for (Running &r : s->active) {
-    FZ_HIP(hipEventSynchronize(r.b));
-    float ms = 0.0f;
-    FZ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-    s->step_ms[r.id].push_back(ms);
-    s->all_ms.push_back(ms);
+    if (r.left > 0) {  // stepped in this round (a request of 0 steps is not)
+      FZ_HIP(hipEventSynchronize(r.b));
+      int killed = 0;
+      FZ_TRY(fz_case_step_end(r.cs, &killed));
+      r.killed = r.killed || killed;
+      r.left = killed ? 0 : r.left - 1;
+      float ms = 0.0f;
+      FZ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      s->step_ms[r.id].push_back(ms);
+      s->all_ms.push_back(ms);
+    }
     if (r.left <= 0) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);

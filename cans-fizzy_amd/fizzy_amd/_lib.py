@@ -94,6 +94,8 @@ SIGNATURES = {
     "fz_case_set_scalars": (C.c_int, [vp, dp]),
     "fz_case_init": (C.c_int, [vp]),
     "fz_case_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+    "fz_case_step_begin": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "fz_case_step_end": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "fz_case_save": (C.c_int, [vp, C.c_char_p]),
     "fz_case_restart": (C.c_int, [vp, C.c_char_p]),
     "fz_load_one": (C.c_int, [vp, C.c_char, C.c_char_p, vp, dp, C.POINTER(C.c_int)]),

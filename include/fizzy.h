@@ -250,6 +250,13 @@ int fz_case_init(fz_case *cs);
 /* nsteps time steps; *killed = 1 when a divergence or time-step check
  * stopped the run as src/main.f90:530-562 does */
 int fz_case_step(fz_case *cs, int nsteps, int *killed);
+/* fz_case_step(cs, 1, killed) in two halves, for a caller that steps many
+ * cases: begin enqueues the step (and its checks when the step number is a
+ * multiple of icheck, *check_pending = 1) without waiting; end waits for the
+ * checks, if any, and sets the next dt and *killed. The same results as
+ * fz_case_step; nothing else may be called on the case in between. */
+int fz_case_step_begin(fz_case *cs, int *check_pending);
+int fz_case_step_end(fz_case *cs, int *killed);
 /* replay each time step after the first from a captured HIP graph (one per
  * parity of the buffer rotation; dt, its AB2 factors and gamma are read
  * from device memory, so the graphs serve every step). Applies to ACDI

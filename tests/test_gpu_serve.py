@@ -106,3 +106,45 @@ def test_footprint_model_matches_the_allocation(spec):
         assert 0.9 * used <= est <= 1.25 * used + (8 << 20)
     finally:
         dc.close()
+
+
+def test_split_step_matches_the_whole_step():
+    """fz_case_step_begin/_end (the server's way of stepping) give the fields
+    and scalars of fz_case_step bit for bit across check steps (icheck = 4:
+    the next dt comes from the read-back that end() takes), and begin twice
+    without end() is refused."""
+    import ctypes as C
+    spec = dict(serve.bubble_case(24, 48, density_ratio=100.0), icheck=4)
+    ref = alone(spec, 11)
+    dc = F.DeviceCase(spec)
+    try:
+        dc.init()
+        pending, killed = C.c_int(), C.c_int()
+        seen = []
+        for _ in range(11):
+            F.call("fz_case_step_begin", dc._h, C.byref(pending))
+            seen.append(pending.value)
+            if pending.value:
+                with pytest.raises(F.FizzyError):
+                    F.call("fz_case_step_begin", dc._h, C.byref(pending))
+            F.call("fz_case_step_end", dc._h, C.byref(killed))
+            assert killed.value == 0
+        assert seen == [0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]
+        for k in FIELDS:
+            assert np.array_equal(dc.get(k), ref[0][k]), k
+        assert np.array_equal(dc.scalars(), ref[1])
+    finally:
+        dc.close()
+
+
+def test_a_request_of_zero_steps_finishes():
+    srv = serve.Server(2 * serve.footprint(serve.bubble_case(16, 32)))
+    try:
+        a = srv.submit(serve.bubble_case(16, 32), 0)
+        b = srv.submit(serve.bubble_case(16, 32), 2)
+        srv.round()
+        assert srv.status(a)[0] == 2 and len(srv.step_times(a)) == 0
+        srv.round()
+        assert srv.status(b)[0] == 2 and len(srv.step_times(b)) == 2
+    finally:
+        srv.close()
