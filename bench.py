@@ -154,7 +154,7 @@ def main_serving(args):
     import fizzy_amd as F
     from fizzy_amd import serve
 
-    reqs = serve.serving_requests(args.requests, mean_interarrival=0.05, nsteps=args.case_steps,
+    reqs = serve.serving_requests(args.requests, mean_interarrival=args.arrival_ms / 1e3, nsteps=args.case_steps,
                                   precision=args.precision)
     cells = [int(np.prod(spec["ng"])) * n for _, spec, n in reqs]
     f, t = C.c_size_t(), C.c_size_t()
@@ -185,11 +185,12 @@ def main_serving(args):
         "value": round(float(np.percentile(lat, 99)), 4), "unit": "ms (p99 per-case step)",
         "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "throughput_Mcell_steps_s": round(thr, 2), "sequential_Mcell_steps_s": round(seq_thr, 2),
-        "speedup_vs_sequential": round(thr / seq_thr, 3), "wall_s": round(wall, 3), "n_gpus": 1,
+        "speedup_vs_sequential": round(thr / seq_thr, 3), "wall_s": round(wall, 3),
+        "last_arrival_s": round(max(r[0] for r in reqs), 3), "n_gpus": 1,
         "higher_is_better": False, "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "mixed",
         "data": "synthetic",
         "config": {"workload": "serving_mix", "requests": len(reqs), "steps_per_case": args.case_steps,
-                   "arrivals": "Poisson, mean 50 ms", "grids": "64^2, 128^2, 256^2 (2-D), 64^2x128, 128^2x256",
+                   "arrivals": f"Poisson, mean {args.arrival_ms:g} ms", "grids": "64^2, 128^2, 256^2 (2-D), 64^2x128, 128^2x256",
                    "density_ratio": "uniform [2, 1000]"},
     }
     print(json.dumps(out), flush=True)
@@ -363,6 +364,8 @@ def main():
                          "solve; c3: THINC/QQ droplets in forced turbulence, 256^3; c4: the serving mix")
     ap.add_argument("--requests", type=int, default=24, help="c4: number of requests")
     ap.add_argument("--case-steps", type=int, default=200, help="c4: steps per request")
+    ap.add_argument("--arrival-ms", type=float, default=50.0,
+                    help="c4: mean gap between arrivals (0: every request waiting at the start)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-prof", action="store_true", help="time without per-launch events")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run of the same workload")

@@ -89,6 +89,7 @@ struct fz_case {
   bool stc_valid = false;
   bool use_graphs = false;
   bool check_pending = false;  // a step's check is enqueued and not yet read
+  bool capture_blocked = false;  // another thread may be making HIP calls: no graph capture now
   struct StepGraph {
     hipGraphExec_t exec = nullptr;
     const void *key[2] = {nullptr, nullptr};
@@ -597,6 +598,7 @@ static int graph_step(fz_case *cs, const StepArgs &a) {
   int slot = -1;
   for (int q = 0; q < 2; q++)
     if (cs->graph[q].exec && cs->graph[q].key[0] == k0 && cs->graph[q].key[1] == k1) slot = q;
+  if (slot < 0 && cs->capture_blocked) return enqueue_step(cs, a);  // the same work, launched directly
   if (slot < 0) {
     slot = cs->graph[0].exec ? 1 : 0;
     if (cs->graph[slot].exec) drop_graphs(cs), slot = 0;
@@ -632,6 +634,7 @@ static int graph_step(fz_case *cs, const StepArgs &a) {
 }
 
 const fz_params *fz_case_prm(const fz_case *cs) { return &cs->prm; }
+void fz_case_block_capture(fz_case *cs, bool on) { cs->capture_blocked = on; }
 void fz_case_drop_graphs(fz_case *cs) { drop_graphs(cs); }
 
 extern "C" int fz_case_set_graphs(fz_case *cs, int on) {

@@ -189,6 +189,9 @@ struct FzBcItem {
 template <typename T>
 int fz_bounds_group(fz_ctx *c, int n, const FzBcItem *items);
 void fz_case_drop_graphs(fz_case *cs);
+// while on, a step that would capture its graph runs its launches directly
+// (the server sets it while its set-up thread makes HIP calls)
+void fz_case_block_capture(fz_case *cs, bool on);
 int fz_acdi_gamma_dev(fz_ctx *ctx, double gam_factor, double gam_min, const double *u, const double *v,
                       const double *w, double *d_gam);
 int fz_acdi_transport_dev(fz_ctx *ctx, const double *d_gam, double gam, double seps, const double *u,

@@ -8,17 +8,26 @@
 // kernels are deterministic, so a case's fields are the same whether it runs
 // alone or in a mix. A scheduling round admits queued requests in arrival
 // order while the footprint budget allows (head-of-line: a request that
-// does not fit waits, and the ones behind it wait too), enqueues one time
-// step of every active case on its own stream (the steps of different cases
-// overlap on the device), then waits for each case's step and records its
-// duration from HIP events on that case's stream. A case that has done its
-// steps leaves the active set; its memory is returned when the caller
-// releases it.
+// does not fit waits, and the ones behind it wait too) and hands them to a
+// set-up thread, which creates and initialises each case (allocation, the
+// solver's set-up, the initial fields) on the case's own stream while the
+// rounds go on stepping the others. The round then takes the cases that are
+// ready (it waits for one only when nothing else is running),
+// enqueues one time step of every active case on its own stream (the steps
+// of different cases overlap on the device), then waits for each case's
+// step and records its duration from HIP events on that case's stream. A
+// case that has done its steps leaves the active set; its memory is returned
+// when the caller releases it.
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
+#include <set>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "fz_internal.h"
@@ -73,13 +82,18 @@ struct Request {
   int precision, nref;
   size_t bytes;
 };
+// a round gives each case up to FZ_SERVE_MAX_BATCH steps, about as many as
+// fit in the slowest running case's step time
+#define FZ_SERVE_MAX_BATCH 8
 struct Running {
   int id;
   fz_case *cs;
   fz_ctx *ctx;
   int left;
   size_t bytes;
-  hipEvent_t a, b;
+  hipEvent_t ev[2 * FZ_SERVE_MAX_BATCH];  // start and end of each step of the round
+  int nq;                                 // steps enqueued in this round
+  float last_ms;                          // duration of its latest step
   bool killed;
 };
 struct Finished {
@@ -87,34 +101,109 @@ struct Finished {
   size_t bytes;
   int killed;
 };
+
+struct Prepared {  // a set-up finished by the set-up thread
+  Running r;
+  int rc;
+  std::string err;
+};
 }  // namespace
 
 struct fz_server {
   size_t budget = 0, in_use = 0;
   int next_id = 1;
+  int device = 0;
   std::deque<Request> queue;
   std::vector<Running> active;
   std::map<int, Finished> done;
   std::map<int, std::vector<float>> step_ms;  // per case, every step's duration
   std::vector<float> all_ms;
+  // the set-up thread: requests admitted and not yet set up, the results
+  std::mutex mu;
+  std::condition_variable cv_work, cv_ready;
+  std::deque<Request> to_prepare;
+  std::deque<Prepared> ready;
+  std::set<int> preparing;  // ids admitted and not yet taken by a round
+  bool stop = false;
+  // held by the set-up thread for a whole set-up, and by a round while it
+  // enqueues steps that may capture graphs: a capture on one thread and
+  // allocations or copies on another have invalidated the capture
+  std::mutex hip_mu;
+  std::thread worker;
 };
+
+static void drop_running(Running &r) {
+  for (hipEvent_t &e : r.ev)
+    if (e) (void)hipEventDestroy(e);
+  fz_case_destroy(r.cs);
+}
+
+// create and initialise the case of request q (on its own stream)
+static int prepare(const Request &q, Running &r) {
+  memset(&r, 0, sizeof(r));
+  r.id = q.id;
+  r.left = q.nsteps;
+  r.bytes = q.bytes;
+  FZ_TRY(fz_case_create_ex(&r.cs, &q.prm, nullptr, q.precision, q.nref));
+  int rc = fz_case_ctx(r.cs, &r.ctx);
+  if (rc == FZ_OK) rc = fz_case_init(r.cs);
+  if (rc == FZ_OK) rc = fz_case_set_graphs(r.cs, 1);  // steps after the first replay captured graphs
+  for (hipEvent_t &e : r.ev)
+    if (rc == FZ_OK && hipEventCreate(&e) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
+  if (rc != FZ_OK) {
+    drop_running(r);
+    r.cs = nullptr;
+  }
+  return rc;
+}
+
+static void worker_loop(fz_server *s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    Request q;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [s] { return s->stop || !s->to_prepare.empty(); });
+      if (s->stop) return;
+      q = s->to_prepare.front();
+      s->to_prepare.pop_front();
+    }
+    Prepared p;
+    {
+      std::lock_guard<std::mutex> lk(s->hip_mu);
+      p.rc = prepare(q, p.r);
+    }
+    if (p.rc != FZ_OK) p.err = fz_last_error();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->ready.push_back(p);
+    }
+    s->cv_ready.notify_all();
+  }
+}
 
 extern "C" int fz_server_create(fz_server **out, size_t hbm_budget) {
   if (!out || hbm_budget == 0) return fz_set_error(FZ_ERR_ARG, "fz_server_create: bad argument");
+  int dev = 0;
+  FZ_HIP(hipGetDevice(&dev));
   fz_server *s = new fz_server();
   s->budget = hbm_budget;
+  s->device = dev;
+  s->worker = std::thread(worker_loop, s);
   *out = s;
   return FZ_OK;
 }
 
-static void drop_running(Running &r) {
-  if (r.a) (void)hipEventDestroy(r.a);
-  if (r.b) (void)hipEventDestroy(r.b);
-  fz_case_destroy(r.cs);
-}
-
 extern "C" int fz_server_destroy(fz_server *s) {
   if (!s) return FZ_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (Prepared &p : s->ready)
+    if (p.rc == FZ_OK) drop_running(p.r);
   for (Running &r : s->active) drop_running(r);
   for (auto &kv : s->done) fz_case_destroy(kv.second.cs);
   delete s;
@@ -143,58 +232,111 @@ extern "C" int fz_server_submit_ex(fz_server *s, const fz_params *prm, int nstep
   return FZ_OK;
 }
 
-static int admit(fz_server *s) {
-  while (!s->queue.empty() && s->in_use + s->queue.front().bytes <= s->budget) {
-    Request q = s->queue.front();
-    s->queue.pop_front();
-    Running r;
-    memset(&r, 0, sizeof(r));
-    r.id = q.id;
-    r.left = q.nsteps;
-    r.bytes = q.bytes;
-    FZ_TRY(fz_case_create_ex(&r.cs, &q.prm, nullptr, q.precision, q.nref));
-    int rc = fz_case_ctx(r.cs, &r.ctx);
-    if (rc == FZ_OK) rc = fz_case_init(r.cs);
-    if (rc == FZ_OK) rc = fz_case_set_graphs(r.cs, 1);  // steps after the first replay captured graphs
-    if (rc == FZ_OK && hipEventCreate(&r.a) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
-    if (rc == FZ_OK && hipEventCreate(&r.b) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventCreate");
-    if (rc != FZ_OK) {
-      drop_running(r);
-      return rc;
+// queued requests that fit, in arrival order, to the set-up thread
+static void admit(fz_server *s) {
+  bool any = false;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    while (!s->queue.empty() && s->in_use + s->queue.front().bytes <= s->budget) {
+      const Request q = s->queue.front();
+      s->queue.pop_front();
+      s->in_use += q.bytes;
+      s->preparing.insert(q.id);
+      s->to_prepare.push_back(q);
+      any = true;
     }
-    s->in_use += r.bytes;
-    s->active.push_back(r);
   }
+  if (any) s->cv_work.notify_one();
+}
+
+// the finished set-ups join the active set (a failed one is dropped, its
+// memory returned, and its error reported by this round)
+static int take_ready(fz_server *s, bool wait) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (wait) s->cv_ready.wait(lk, [s] { return !s->ready.empty() || s->preparing.empty(); });
+  int rc = FZ_OK;
+  std::string err;
+  while (!s->ready.empty()) {
+    Prepared p = s->ready.front();
+    s->ready.pop_front();
+    s->preparing.erase(p.r.id);
+    if (p.rc == FZ_OK) {
+      s->active.push_back(p.r);
+    } else {
+      s->in_use -= p.r.bytes;
+      if (rc == FZ_OK) {
+        rc = p.rc;
+        err = p.err;
+      }
+    }
+  }
+  lk.unlock();
+  if (rc != FZ_OK) return fz_set_error(rc, "fz_server_round: setting up a case failed: %s", err.c_str());
   return FZ_OK;
 }
 
 extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
   if (!s) return fz_set_error(FZ_ERR_ARG, "fz_server_round: null server");
-  FZ_TRY(admit(s));
-  // one step of every active case, each on its own stream: all are enqueued
-  // before any is waited for (a step's checks are read when it has ended)
-  for (Running &r : s->active) {
-    if (r.left <= 0) continue;
-    FZ_HIP(hipEventRecord(r.a, r.ctx->st));
-    FZ_TRY(fz_case_step_begin(r.cs, nullptr));
-    FZ_HIP(hipEventRecord(r.b, r.ctx->st));
+  admit(s);
+  FZ_TRY(take_ready(s, s->active.empty()));  // with nothing to step, wait for a set-up
+  // steps of every active case, each case on its own stream: all are
+  // enqueued before any is waited for. A case gets about as many steps as
+  // fit in the slowest case's step (at most FZ_SERVE_MAX_BATCH), so that
+  // small cases keep the device busy beside large ones; its steps stop at
+  // one whose checks must be read before the next step (its dt). Steps are
+  // enqueued step by step across the cases, which interleaves the streams'
+  // work on the hardware queues they share. A case whose graph is still to
+  // be captured runs the step's launches directly while the set-up thread
+  // is busy.
+  float tmax = 0.0f;
+  for (const Running &r : s->active) tmax = std::max(tmax, r.last_ms);
+  std::vector<int> want(s->active.size()), open(s->active.size());
+  for (size_t q = 0; q < s->active.size(); q++) {
+    Running &r = s->active[q];
+    int d = 1;
+    if (r.last_ms > 0.0f && tmax > 0.0f) d = std::max(1, std::min(FZ_SERVE_MAX_BATCH, (int)(tmax / r.last_ms)));
+    want[q] = std::min(d, std::max(r.left, 0));
+    open[q] = want[q] > 0;
+    r.nq = 0;
   }
+  const bool may_capture = s->hip_mu.try_lock();
+  int rc = FZ_OK;
+  for (int k = 0; k < FZ_SERVE_MAX_BATCH && rc == FZ_OK; k++) {
+    for (size_t q = 0; q < s->active.size() && rc == FZ_OK; q++) {
+      Running &r = s->active[q];
+      if (!open[q] || r.nq >= want[q]) continue;
+      fz_case_block_capture(r.cs, !may_capture);
+      int pending = 0;
+      if (hipEventRecord(r.ev[2 * r.nq], r.ctx->st) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventRecord");
+      if (rc == FZ_OK) rc = fz_case_step_begin(r.cs, &pending);
+      if (rc == FZ_OK && hipEventRecord(r.ev[2 * r.nq + 1], r.ctx->st) != hipSuccess)
+        rc = fz_set_error(FZ_ERR_HIP, "hipEventRecord");
+      if (rc != FZ_OK) break;
+      r.nq++;
+      if (pending) open[q] = 0;  // its checks set the next dt: no more steps this round
+    }
+  }
+  if (may_capture) s->hip_mu.unlock();
+  FZ_TRY(rc);
   std::vector<Running> still;
   for (Running &r : s->active) {
-    if (r.left > 0) {  // stepped in this round (a request of 0 steps is not)
-      FZ_HIP(hipEventSynchronize(r.b));
+    if (r.nq > 0) {  // stepped in this round (a request of 0 steps is not)
+      FZ_HIP(hipEventSynchronize(r.ev[2 * r.nq - 1]));
       int killed = 0;
       FZ_TRY(fz_case_step_end(r.cs, &killed));
+      for (int q = 0; q < r.nq; q++) {
+        float ms = 0.0f;
+        FZ_HIP(hipEventElapsedTime(&ms, r.ev[2 * q], r.ev[2 * q + 1]));
+        s->step_ms[r.id].push_back(ms);
+        s->all_ms.push_back(ms);
+        r.last_ms = ms;
+      }
       r.killed = r.killed || killed;
-      r.left = killed ? 0 : r.left - 1;
-      float ms = 0.0f;
-      FZ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-      s->step_ms[r.id].push_back(ms);
-      s->all_ms.push_back(ms);
+      r.left = killed ? 0 : r.left - r.nq;
     }
     if (r.left <= 0) {
-      (void)hipEventDestroy(r.a);
-      (void)hipEventDestroy(r.b);
+      for (hipEvent_t &e : r.ev) (void)hipEventDestroy(e);
+      fz_case_block_capture(r.cs, false);
       s->done[r.id] = Finished{r.cs, r.bytes, r.killed ? 1 : 0};
     } else {
       still.push_back(r);
@@ -202,7 +344,10 @@ extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
   }
   s->active.swap(still);
   if (n_active) *n_active = (int)s->active.size();
-  if (n_queued) *n_queued = (int)s->queue.size();
+  if (n_queued) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    *n_queued = (int)(s->queue.size() + s->preparing.size());
+  }
   return FZ_OK;
 }
 
@@ -214,6 +359,10 @@ extern "C" int fz_server_status(fz_server *s, int id, int *state, int *killed) {
   if (killed) *killed = 0;
   for (const Request &q : s->queue)
     if (q.id == id) *state = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->preparing.count(id)) *state = 0;  // admitted, being set up
+  }
   for (const Running &r : s->active)
     if (r.id == id) *state = 1;
   auto it = s->done.find(id);

@@ -67,20 +67,26 @@ def serve_memory():
 
 
 def test_admission_in_arrival_order_within_the_budget():
-    specs = [(cases.rising_bubble_3d(ng=(32, 32, 64)), 8), (serve.bubble_case(24, 48), 3),
+    specs = [(cases.rising_bubble_3d(ng=(32, 32, 64)), 400), (serve.bubble_case(24, 48), 3),
              (serve.bubble_case(24, 48, density_ratio=50.0), 2)]
     b = [serve.footprint(s) for s, _ in specs]
     assert b[1] == b[2]
     srv = serve.Server(b[0] + b[1] + b[2] // 2)  # the first two fit together, the third once the second leaves
     try:
         ids = [srv.submit(s, n) for s, n in specs]
-        srv.round()
+        for _ in range(50):  # both admitted; running once their set-ups are done
+            srv.round()
+            if srv.status(ids[1])[0] != 0:
+                break
         assert [srv.status(i)[0] for i in ids] == [1, 1, 0]
         while srv.status(ids[1])[0] != 2:
             srv.round()
         assert srv.status(ids[2])[0] == 0  # finished cases hold their memory until released
         srv.release(ids[1])
-        srv.round()
+        for _ in range(50):  # admitted now; running once its set-up (on the server's thread) is done
+            srv.round()
+            if srv.status(ids[2])[0] != 0:
+                break
         assert srv.status(ids[2])[0] == 1
         used, budget = srv.memory()
         assert used == b[0] + b[2] <= budget
@@ -142,9 +148,11 @@ def test_a_request_of_zero_steps_finishes():
     try:
         a = srv.submit(serve.bubble_case(16, 32), 0)
         b = srv.submit(serve.bubble_case(16, 32), 2)
-        srv.round()
+        for _ in range(50):
+            srv.round()
+            if srv.status(a)[0] == 2 and srv.status(b)[0] == 2:
+                break
         assert srv.status(a)[0] == 2 and len(srv.step_times(a)) == 0
-        srv.round()
         assert srv.status(b)[0] == 2 and len(srv.step_times(b)) == 2
     finally:
         srv.close()
