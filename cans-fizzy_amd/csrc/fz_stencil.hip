@@ -1295,8 +1295,10 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
     double *__restrict__ un, double *__restrict__ vn, double *__restrict__ wn) {
   constexpr int NT = MOM_MX * MOM_MY, PL = MOM_PL;
   constexpr int NPT = (PL + NT - 1) / NT;  // plane positions per thread
+  // u, v, w, psi and the z flux are read at k-1, k, k+1 (three plane slots);
+  // the x and y fluxes at k and k+1 only (two slots)
   __shared__ double s_u[3 * PL], s_v[3 * PL], s_w[3 * PL];
-  __shared__ PT s_psi[3 * PL], s_fx[3 * PL], s_fy[3 * PL], s_fz[3 * PL];
+  __shared__ PT s_psi[3 * PL], s_fx[2 * PL], s_fy[2 * PL], s_fz[3 * PL];
   const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
   const double factor12 = stc ? stc[FZ_STC_F12] : factor12_v, dt_r = stc ? stc[FZ_STC_DTR] : dt_r_v;
   FZ_BLOCK_XYZ(d, bx, by, bz);
@@ -1318,6 +1320,7 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
     gpos[r] = (unsigned)(i0 - 1 + px) + (unsigned)d.sy * (unsigned)(j0 - 1 + py);
   }
   auto slot = [](int kp) { return (kp % 3) * PL; };
+  auto slot2 = [](int kp) { return (kp % 2) * PL; };
   double qu[NPT], qv[NPT], qw[NPT];
   PT qp[NPT], qx[NPT], qy[NPT], qz[NPT];
   auto load_plane = [&](int kp) {
@@ -1335,7 +1338,7 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
     }
   };
   auto store_plane = [&](int kp) {
-    const int so = slot(kp);
+    const int so = slot(kp), so2 = slot2(kp);
 #pragma unroll
     for (int r = 0; r < NPT; r++) {
       const int pos = t + r * NT;
@@ -1344,8 +1347,8 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
         s_v[so + pos] = qv[r];
         s_w[so + pos] = qw[r];
         s_psi[so + pos] = qp[r];
-        s_fx[so + pos] = qx[r];
-        s_fy[so + pos] = qy[r];
+        s_fx[so2 + pos] = qx[r];
+        s_fy[so2 + pos] = qy[r];
         s_fz[so + pos] = qz[r];
       }
     }
@@ -1360,9 +1363,9 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
     if (act) {
       const long q = i + d.sy * j + d.sz * (long)k;
       const long r = (i - 1) + (long)d.n1 * ((j - 1) + (long)d.n2 * (k - 1));
-      const int sa = slot(k - 1), sb = slot(k), sc = slot(k + 1);
+      const int sa = slot(k - 1), sb = slot(k), sc = slot(k + 1), tb = slot2(k), tc = slot2(k + 1);
       const LdsAt<double> U{s_u, c, sa, sb, sc}, V{s_v, c, sa, sb, sc}, W{s_w, c, sa, sb, sc};
-      const LdsAt<PT> PSI{s_psi, c, sa, sb, sc}, FX{s_fx, c, sa, sb, sc}, FY{s_fy, c, sa, sb, sc},
+      const LdsAt<PT> PSI{s_psi, c, sa, sb, sc}, FX{s_fx, c, tb, tb, tc}, FY{s_fy, c, tb, tb, tc},
           FZ{s_fz, c, sa, sb, sc};
       double ru, rv, rw;
       mom_ad_cell(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, ru, rv, rw);
@@ -1411,12 +1414,13 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
   if (variant == 8) {  // stencil fields staged in LDS
     constexpr int KZ = 16;
     const dim3 grid((c->d.n1 + MOM_MX - 1) / MOM_MX, (c->d.n2 + MOM_MY - 1) / MOM_MY, (c->d.n3 + KZ - 1) / KZ);
-    // (3 waves per SIMD fit FP32 phase fields' registers, 169 -> 168 VGPRs)
-    FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", (k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1>), grid,
-              dim3(MOM_MX, MOM_MY), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1], rho12[0] - rho12[1],
-              mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12, dt_r, stc, surf_factor, rho0, bforce[0],
-              bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1], rho_av[2], fx, fy, fz, u, v, w,
-              psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);
+    // (3 waves per SIMD fit FP32 phase fields' registers, 169 -> 168 VGPRs;
+    // FP64 needs 188, and a 3-wave FP64 build spills 36 and runs 2.2x slower)
+    FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", (k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1>),
+              grid, dim3(MOM_MX, MOM_MY), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1],
+              rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12, dt_r, stc,
+              surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1],
+              rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);
     FZ_LAUNCHED();
     return FZ_OK;
   }
