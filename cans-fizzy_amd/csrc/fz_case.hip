@@ -207,7 +207,8 @@ extern "C" int fz_case_create_ex(fz_case **out, const fz_params *prm, void *hip_
   if (precision != FZ_PREC_FP64 && precision != FZ_PREC_MIXED)
     return fz_set_error(FZ_ERR_ARG, "fz_case_create: unknown precision %d", precision);
   if (precision == FZ_PREC_MIXED) {
-    if (nref < 0 || nref > 8) return fz_set_error(FZ_ERR_ARG, "fz_case_create: refinements %d not in [0, 8]", nref);
+    if (nref < -1 || nref > 8)
+      return fz_set_error(FZ_ERR_ARG, "fz_case_create: refinements %d not in [-1, 8]", nref);
     if (!prm->is_track_interface || !prm->is_solve_ns)
       return fz_set_error(FZ_ERR_UNSUPPORTED,
                           "mixed precision is implemented for the two-fluid step (interface and Navier-Stokes on)");
@@ -520,8 +521,9 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
     if (P.is_forced_hit) FZ_TRY(fz_lscale_forcing(c, 2, 0.5, dt, f[F_U], f[F_V], f[F_W]));
     FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
     // projection on the correction buffer dp, src/main.f90:488-502
-    if (mixed) FZ_TRY(fz_projection_rhs_solve_mixed(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp, cs->nref,
-                                                      a.stc));
+    // (nref = -1: the mixed step with the FP64 projection of the plain step)
+    if (mixed && cs->nref >= 0)
+      FZ_TRY(fz_projection_rhs_solve_mixed(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp, cs->nref, a.stc));
     else FZ_TRY(fz_projection_rhs_solve(c, 1.0 / dt, rho0, f[F_U], f[F_V], f[F_W], cs->dp, a.stc));
     FZ_TRY(fz_boundp(c, P.cbcpre, P.bcpre, cs->dp));
     // correc into the spare velocity buffers, updatep into the buffer of the
@@ -560,7 +562,7 @@ static bool graph_ok(const fz_case *cs) {
   if (!P.is_track_interface || P.vof || !P.is_solve_ns || P.is_forced_hit) return false;
   for (int d = 0; d < 3; d++)
     if (fabs(P.gacc[d]) > 0.0 && cs->ctx->per[d]) return false;  // bulk mean read back by the host
-  return fz_projection_device_scalars_ok(cs->ctx, cs->precision == FZ_PREC_MIXED) != 0;
+  return fz_projection_device_scalars_ok(cs->ctx, cs->precision == FZ_PREC_MIXED && cs->nref >= 0) != 0;
 }
 
 __global__ void k_set_stc(double *__restrict__ stc, double dt, double dti, double f1, double f2, double f12,

@@ -232,8 +232,9 @@ int fz_case_create(fz_case **cs, const fz_params *prm, void *hip_stream);
  * histories with FP32 arithmetic in the phase-field kernels; velocity,
  * pressure, the momentum arithmetic and the global sums in FP64; the
  * projection solves with FP32 transforms and line solves (FP64 pivots)
- * followed by nref FP64-residual refinements. Two-fluid cases (ACDI or
- * THINC/QQ) with homogeneous pressure boundary values. */
+ * followed by nref FP64-residual refinements (nref = -1: the plain
+ * step's FP64 projection instead). Two-fluid cases (ACDI or THINC/QQ) with
+ * homogeneous pressure boundary values. */
 #define FZ_PREC_MIXED 1
 int fz_case_create_ex(fz_case **cs, const fz_params *prm, void *hip_stream, int precision, int nref);
 int fz_case_precision(fz_case *cs, int *precision, int *nref);

@@ -154,11 +154,13 @@ def test_mixed_case_against_fp64(name, spec):
     """40 steps (checks every 10) from the same initial state. Stated bounds
     with one refinement: chkdiv divmax <= 1e-10; velocity <= 1e-2, pressure
     <= 1e-3, psi <= 1e-5 (THINC/QQ: 1e-3) and interface curvature <= 1e-2 of
-    their scales. Without refinement the FP32 solve leaves divmax at O(1e-6)."""
+    their scales. Without refinement the FP32 solve leaves divmax at O(1e-6).
+    With the plain step's FP64 projection (refine -1) divmax stays within
+    10x of the FP64 step's and the fields within the same bounds."""
     nsteps = 40
     ref, sref, _ = run_case(spec, nsteps)
     out = {}
-    for nref in (0, 1, 2):
+    for nref in (0, 1, 2, -1):
         got, sc, vol0 = run_case(dict(spec, precision="mixed", refine=nref), nsteps)
         errs = field_errors(got, ref)
         vol = float(got["psi"].sum())
@@ -174,10 +176,16 @@ def test_mixed_case_against_fp64(name, spec):
     for k in CASE_FIELDS:
         assert errs[k] <= bounds[k], (k, errs[k])
     assert drift <= 1e-5
+    errs64, divmax64, drift64 = out[-1]
+    assert divmax64 <= max(10 * float(sref[7]), 1e-13)
+    for k in CASE_FIELDS:
+        assert errs64[k] <= bounds[k], (k, errs64[k])
+    assert drift64 <= 1e-5
 
 
 GCASES = [("fp64", cases.rising_bubble_3d(ng=(32, 32, 64))),
           ("mixed", dict(cases.rising_bubble_3d(ng=(32, 32, 64)), precision="mixed", refine=1)),
+          ("mixed-fp64-projection", dict(cases.rising_bubble_3d(ng=(32, 32, 64)), precision="mixed", refine=-1)),
           ("hysing2d", cases.hysing_2d())]
 
 
