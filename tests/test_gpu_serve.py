@@ -156,3 +156,24 @@ def test_a_request_of_zero_steps_finishes():
         assert srv.status(b)[0] == 2 and len(srv.step_times(b)) == 2
     finally:
         srv.close()
+
+
+def test_close_while_cases_are_being_set_up():
+    """Closing the server with requests still queued for (or in) set-up on
+    its thread returns, and a new server works after it."""
+    specs = [serve.bubble_case(64, 128), serve.bubble_case(32, 64), serve.bubble_case(48, 96)]
+    srv = serve.Server(sum(serve.footprint(s) for s in specs))
+    for s in specs:
+        srv.submit(s, 5)
+    srv.round()  # waits for the first set-up only
+    srv.close()
+    srv = serve.Server(serve.footprint(specs[1]))
+    try:
+        rid = srv.submit(specs[1], 2)
+        for _ in range(50):
+            srv.round()
+            if srv.status(rid)[0] == 2:
+                break
+        assert srv.status(rid)[0] == 2
+    finally:
+        srv.close()
