@@ -298,10 +298,11 @@ int fz_server_submit(fz_server *s, const fz_params *prm, int nsteps, int *id);
 int fz_server_submit_ex(fz_server *s, const fz_params *prm, int nsteps, int precision, int nref, int *id);
 /* admit queued cases in arrival order while their footprints fit (the
  * server's set-up thread creates and initialises them while other cases
- * run; a round with no running case waits for one), then time steps of
- * every running case on its own stream: one for the slowest, up to 8 for
- * cases whose steps are shorter, fewer when a check step comes first;
- * waits for the round. n_queued counts the requests not yet running. */
+ * run; a round with no running case waits for one), then one time step
+ * of every running case on its own stream (with FZ_SERVE_BATCH=b in the
+ * environment, up to b steps for cases whose steps are shorter than the
+ * slowest one's, fewer when a check step comes first); waits for the
+ * round. n_queued counts the requests not yet running. */
 int fz_server_round(fz_server *s, int *n_active, int *n_queued);
 /* 0 queued, 1 running, 2 finished (killed: a check stopped it), -1 unknown */
 int fz_server_status(fz_server *s, int id, int *state, int *killed);

@@ -31,7 +31,10 @@ def alone(spec, nsteps):
         dc.close()
 
 
-def test_results_are_the_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("batch", ["1", "8"])
+def test_results_are_the_same_alone_and_in_the_mix(batch, monkeypatch):
+    """(batch: FZ_SERVE_BATCH, the most steps a short case gets per round)"""
+    monkeypatch.setenv("FZ_SERVE_BATCH", batch)
     reqs = mix()
     ref = [alone(s, n) for s, n in reqs]
     free, _ = serve_memory()
