@@ -336,8 +336,12 @@ static int new_dt(fz_case *cs, double *dt, int *kill) {
   cs->sc[S_DTCFL] = dt_cfl;
   double d = fmin(P.cfl * dt_cfl, P.dtmax);
   if (P.dt_f > 0.0) {
-    if (P.dt_f > d && kill) *kill = 1;
-    d = P.dt_f;
+    // at start-up a fixed dt above the stable one only warns (src/main.f90:397-403);
+    // in the loop it stops the run and keeps the stable dt (:534-541)
+    if (P.dt_f > d && kill)
+      *kill = 1;
+    else
+      d = P.dt_f;
   }
   *dt = d;
   return FZ_OK;
@@ -671,8 +675,11 @@ static void check_finish(fz_case *cs, int *kill) {
   cs->sc[S_DTCFL] = dt_cfl;
   double d = fmin(P.cfl * dt_cfl, P.dtmax);
   if (P.dt_f > 0.0) {
-    if (P.dt_f > d) *kill = 1;
-    d = P.dt_f;
+    // a fixed dt above the stable one stops the run and keeps the stable dt
+    if (P.dt_f > d)
+      *kill = 1;
+    else
+      d = P.dt_f;
   }
   if (dt_cfl < FZ_EPS * 1e7) *kill = 1;  // small, src/param.f90:21
   cs->sc[S_DT] = d;

@@ -98,11 +98,15 @@ struct Running {
   int nq;                                 // steps enqueued in this round
   float last_ms;                          // duration of its latest step
   bool killed;
+  int rc;           // the first error of this case in the round (FZ_OK if none)
+  std::string err;
 };
 struct Finished {
-  fz_case *cs;
+  fz_case *cs;       // null for a failed request (its case is gone, its memory returned)
   size_t bytes;
   int killed;
+  int failed;        // set-up or a step of the request returned an error
+  std::string err;   // that error's message
 };
 
 struct Prepared {  // a set-up finished by the set-up thread
@@ -144,7 +148,7 @@ static void drop_running(Running &r) {
 
 // create and initialise the case of request q (on its own stream)
 static int prepare(const Request &q, Running &r) {
-  memset(&r, 0, sizeof(r));
+  r = Running();
   r.id = q.id;
   r.left = q.nsteps;
   r.bytes = q.bytes;
@@ -210,7 +214,8 @@ extern "C" int fz_server_destroy(fz_server *s) {
   for (Prepared &p : s->ready)
     if (p.rc == FZ_OK) drop_running(p.r);
   for (Running &r : s->active) drop_running(r);
-  for (auto &kv : s->done) fz_case_destroy(kv.second.cs);
+  for (auto &kv : s->done)
+    if (kv.second.cs) fz_case_destroy(kv.second.cs);
   delete s;
   return FZ_OK;
 }
@@ -254,36 +259,30 @@ static void admit(fz_server *s) {
   if (any) s->cv_work.notify_one();
 }
 
-// the finished set-ups join the active set (a failed one is dropped, its
-// memory returned, and its error reported by this round)
-static int take_ready(fz_server *s, bool wait) {
+// a request that failed: it finishes with its error, and its memory returns
+static void fail_request(fz_server *s, int id, size_t bytes, const std::string &err) {
+  s->in_use -= bytes;
+  s->done[id] = Finished{nullptr, 0, 0, 1, err};
+}
+
+// the finished set-ups join the active set; a failed one finishes as failed
+// (fz_server_status, fz_server_error) without stopping the others
+static void take_ready(fz_server *s, bool wait) {
   std::unique_lock<std::mutex> lk(s->mu);
   if (wait) s->cv_ready.wait(lk, [s] { return !s->ready.empty() || s->preparing.empty(); });
-  int rc = FZ_OK;
-  std::string err;
   while (!s->ready.empty()) {
     Prepared p = s->ready.front();
     s->ready.pop_front();
     s->preparing.erase(p.r.id);
-    if (p.rc == FZ_OK) {
-      s->active.push_back(p.r);
-    } else {
-      s->in_use -= p.r.bytes;
-      if (rc == FZ_OK) {
-        rc = p.rc;
-        err = p.err;
-      }
-    }
+    if (p.rc == FZ_OK) s->active.push_back(p.r);
+    else fail_request(s, p.r.id, p.r.bytes, "set-up: " + p.err);
   }
-  lk.unlock();
-  if (rc != FZ_OK) return fz_set_error(rc, "fz_server_round: setting up a case failed: %s", err.c_str());
-  return FZ_OK;
 }
 
 extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
   if (!s) return fz_set_error(FZ_ERR_ARG, "fz_server_round: null server");
   admit(s);
-  FZ_TRY(take_ready(s, s->active.empty()));  // with nothing to step, wait for a set-up
+  take_ready(s, s->active.empty());  // with nothing to step, wait for a set-up
   // steps of every active case, each case on its own stream: all are
   // enqueued before any is waited for. A case gets about as many steps as
   // fit in the slowest case's step (at most max_batch), so that
@@ -303,46 +302,68 @@ extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
     want[q] = std::min(d, std::max(r.left, 0));
     open[q] = want[q] > 0;
     r.nq = 0;
+    r.rc = FZ_OK;
   }
+  // An error in one case's step stops that case only: the others' steps go
+  // on, and every step that was enqueued is waited for and ended below.
   const bool may_capture = s->hip_mu.try_lock();
-  int rc = FZ_OK;
-  for (int k = 0; k < s->max_batch && rc == FZ_OK; k++) {
-    for (size_t q = 0; q < s->active.size() && rc == FZ_OK; q++) {
+  for (int k = 0; k < s->max_batch; k++) {
+    for (size_t q = 0; q < s->active.size(); q++) {
       Running &r = s->active[q];
       if (!open[q] || r.nq >= want[q]) continue;
       fz_case_block_capture(r.cs, !may_capture);
-      int pending = 0;
+      int pending = 0, rc = FZ_OK;
       if (hipEventRecord(r.ev[2 * r.nq], r.ctx->st) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventRecord");
       if (rc == FZ_OK) rc = fz_case_step_begin(r.cs, &pending);
       if (rc == FZ_OK && hipEventRecord(r.ev[2 * r.nq + 1], r.ctx->st) != hipSuccess)
         rc = fz_set_error(FZ_ERR_HIP, "hipEventRecord");
-      if (rc != FZ_OK) break;
+      if (rc != FZ_OK) {
+        r.rc = rc;
+        r.err = std::string("step: ") + fz_last_error();
+        open[q] = 0;
+        continue;
+      }
       r.nq++;
       if (pending) open[q] = 0;  // its checks set the next dt: no more steps this round
     }
   }
   if (may_capture) s->hip_mu.unlock();
-  FZ_TRY(rc);
   std::vector<Running> still;
   for (Running &r : s->active) {
-    if (r.nq > 0) {  // stepped in this round (a request of 0 steps is not)
-      FZ_HIP(hipEventSynchronize(r.ev[2 * r.nq - 1]));
+    if (r.rc == FZ_OK && r.nq > 0) {  // stepped in this round (a request of 0 steps is not)
       int killed = 0;
-      FZ_TRY(fz_case_step_end(r.cs, &killed));
-      for (int q = 0; q < r.nq; q++) {
+      int rc = hipEventSynchronize(r.ev[2 * r.nq - 1]) == hipSuccess
+                   ? FZ_OK
+                   : fz_set_error(FZ_ERR_HIP, "hipEventSynchronize");
+      if (rc == FZ_OK) rc = fz_case_step_end(r.cs, &killed);
+      for (int q = 0; q < r.nq && rc == FZ_OK; q++) {
         float ms = 0.0f;
-        FZ_HIP(hipEventElapsedTime(&ms, r.ev[2 * q], r.ev[2 * q + 1]));
+        if (hipEventElapsedTime(&ms, r.ev[2 * q], r.ev[2 * q + 1]) != hipSuccess) {
+          rc = fz_set_error(FZ_ERR_HIP, "hipEventElapsedTime");
+          break;
+        }
         s->step_ms[r.id].push_back(ms);
         s->all_ms.push_back(ms);
         r.last_ms = ms;
       }
+      if (rc != FZ_OK) {
+        r.rc = rc;
+        r.err = std::string("step: ") + fz_last_error();
+      }
       r.killed = r.killed || killed;
       r.left = killed ? 0 : r.left - r.nq;
     }
-    if (r.left <= 0) {
+    if (r.rc != FZ_OK) {
+      // the failed case is dropped (its stream drained first), the request
+      // finishes as failed and its memory returns
+      (void)hipStreamSynchronize(r.ctx->st);
+      fz_case_block_capture(r.cs, false);
+      drop_running(r);
+      fail_request(s, r.id, r.bytes, r.err);
+    } else if (r.left <= 0) {
       for (hipEvent_t &e : r.ev) (void)hipEventDestroy(e);
       fz_case_block_capture(r.cs, false);
-      s->done[r.id] = Finished{r.cs, r.bytes, r.killed ? 1 : 0};
+      s->done[r.id] = Finished{r.cs, r.bytes, r.killed ? 1 : 0, 0, std::string()};
     } else {
       still.push_back(r);
     }
@@ -357,7 +378,8 @@ extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
 }
 
 // state of a request: 0 queued, 1 running, 2 finished (killed set when a
-// check stopped it), -1 unknown or released
+// check stopped it), 3 failed (fz_server_error has the message), -1 unknown
+// or released
 extern "C" int fz_server_status(fz_server *s, int id, int *state, int *killed) {
   if (!s || !state) return fz_set_error(FZ_ERR_ARG, "fz_server_status: null argument");
   *state = -1;
@@ -372,9 +394,18 @@ extern "C" int fz_server_status(fz_server *s, int id, int *state, int *killed) {
     if (r.id == id) *state = 1;
   auto it = s->done.find(id);
   if (it != s->done.end()) {
-    *state = 2;
+    *state = it->second.failed ? 3 : 2;
     if (killed) *killed = it->second.killed;
   }
+  return FZ_OK;
+}
+
+// the error message of a failed request (empty for any other)
+extern "C" int fz_server_error(fz_server *s, int id, char *buf, size_t cap) {
+  if (!s || !buf || cap == 0) return fz_set_error(FZ_ERR_ARG, "fz_server_error: bad argument");
+  buf[0] = 0;
+  auto it = s->done.find(id);
+  if (it != s->done.end() && it->second.failed) snprintf(buf, cap, "%s", it->second.err.c_str());
   return FZ_OK;
 }
 
@@ -383,6 +414,8 @@ extern "C" int fz_server_case(fz_server *s, int id, fz_case **cs) {
   if (!s || !cs) return fz_set_error(FZ_ERR_ARG, "fz_server_case: null argument");
   auto it = s->done.find(id);
   if (it == s->done.end()) return fz_set_error(FZ_ERR_STATE, "fz_server_case: request %d has not finished", id);
+  if (it->second.failed)
+    return fz_set_error(FZ_ERR_STATE, "fz_server_case: request %d failed: %s", id, it->second.err.c_str());
   *cs = it->second.cs;
   return FZ_OK;
 }
@@ -391,7 +424,7 @@ extern "C" int fz_server_release(fz_server *s, int id) {
   if (!s) return fz_set_error(FZ_ERR_ARG, "fz_server_release: null server");
   auto it = s->done.find(id);
   if (it == s->done.end()) return fz_set_error(FZ_ERR_STATE, "fz_server_release: request %d has not finished", id);
-  fz_case_destroy(it->second.cs);
+  if (it->second.cs) fz_case_destroy(it->second.cs);
   s->in_use -= it->second.bytes;
   s->done.erase(it);
   return FZ_OK;

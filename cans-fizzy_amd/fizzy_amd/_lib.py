@@ -110,6 +110,7 @@ SIGNATURES = {
     "fz_server_submit_ex": (C.c_int, [vp, C.POINTER(FzParams), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "fz_server_round": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fz_server_status": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fz_server_error": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_size_t]),
     "fz_server_case": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
     "fz_server_release": (C.c_int, [vp, C.c_int]),
     "fz_server_step_times": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),

@@ -57,11 +57,19 @@ class Server:
         call("fz_server_round", self._h, C.byref(na), C.byref(nq))
         return na.value, nq.value
 
+    QUEUED, RUNNING, FINISHED, FAILED = 0, 1, 2, 3
+
     def status(self, rid):
-        """(state, killed): state 0 queued, 1 running, 2 finished, -1 unknown/released."""
+        """(state, killed): state 0 queued, 1 running, 2 finished, 3 failed, -1 unknown/released."""
         st, k = C.c_int(), C.c_int()
         call("fz_server_status", self._h, int(rid), C.byref(st), C.byref(k))
         return st.value, k.value
+
+    def error(self, rid):
+        """The error message of a failed request ('' otherwise)."""
+        buf = C.create_string_buffer(1024)
+        call("fz_server_error", self._h, int(rid), buf, len(buf))
+        return buf.value.decode(errors="replace")
 
     def case(self, rid):
         """The finished case as a DeviceCase view (owned by the server)."""
@@ -89,7 +97,8 @@ class Server:
 def run_stream(server, requests, release=True, on_finish=None):
     """Submit (t_arrival_s, spec, nsteps) requests at their arrival times
     (relative to the start) while rounds run; returns the wall time and the
-    request ids in submission order."""
+    request ids in submission order. A request that fails (state 3) ends like
+    a finished one: on_finish sees it, and server.error(rid) gives why."""
     t0 = time.perf_counter()
     pending = sorted(requests, key=lambda r: r[0])
     ids, live = [], set()
@@ -107,7 +116,7 @@ def run_stream(server, requests, release=True, on_finish=None):
         server.round()
         for rid in list(live):
             st, _ = server.status(rid)
-            if st == 2:
+            if st in (Server.FINISHED, Server.FAILED):
                 live.discard(rid)
                 if on_finish:
                     on_finish(rid)

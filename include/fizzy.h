@@ -304,9 +304,14 @@ int fz_server_submit_ex(fz_server *s, const fz_params *prm, int nsteps, int prec
  * slowest one's, fewer when a check step comes first); waits for the
  * round. n_queued counts the requests not yet running. */
 int fz_server_round(fz_server *s, int *n_active, int *n_queued);
-/* 0 queued, 1 running, 2 finished (killed: a check stopped it), -1 unknown */
+/* 0 queued, 1 running, 2 finished (killed: a check stopped it), 3 failed
+ * (its set-up or a step returned an error: that case alone is dropped, its
+ * memory returned, and the other cases go on), -1 unknown */
 int fz_server_status(fz_server *s, int id, int *state, int *killed);
-/* a finished case (fields via fz_case_field) until it is released */
+/* the error message of a failed request (an empty string for any other) */
+int fz_server_error(fz_server *s, int id, char *buf, size_t cap);
+/* a finished case (fields via fz_case_field) until it is released; a
+ * failed request has none (FZ_ERR_STATE), and its release only forgets it */
 int fz_server_case(fz_server *s, int id, fz_case **cs);
 int fz_server_release(fz_server *s, int id);
 /* durations (ms, HIP events on the case's stream) of the steps of request

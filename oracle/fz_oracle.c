@@ -33,6 +33,41 @@
 #define EPS 2.220446049250313080847e-16 /* epsilon(1._rp), src/param.f90:17 */
 #define PI 3.141592653589793115998      /* acos(-1._rp), src/param.f90:15 */
 
+/* Planted mistakes. The pin tests show that each of their checks fails on a
+ * plausible slip in this restatement: a second build of this file,
+ * liboracle_planted.so (-DOR_PLANTED), carries numbered slips, and its
+ * global or_planted selects one of them (0: none). In liboracle.so every
+ * PLANTED(k) is the constant 0, so the restatement is unchanged. */
+#ifdef OR_PLANTED
+int or_planted = 0;
+#define PLANTED(k) (or_planted == (k))
+#else
+#define PLANTED(k) 0
+#endif
+enum {
+  PL_ACDI_SHARP_SIGN = 1,  /* sharpening added to the diffusive flux instead of subtracted */
+  PL_ACDI_QUARTER,         /* 0.5*gam instead of 0.25*gam in front of the sharpening */
+  PL_ACDI_TANH_ARG,        /* tanh(phi_face/seps) instead of tanh(phi_face/(2 seps)) */
+  PL_ACDI_NORMAL_COMP,     /* the y normal on the x+ face */
+  PL_ACDI_DZCI_M,          /* dzci(k) instead of dzci(k-1) on the z- face */
+  PL_THINC_UPWIND,         /* the downwind cell's normal and phi for u > 0 */
+  PL_THINC_OFFSET,         /* the face at -1/2 instead of +1/2 of the upwind cell for u > 0 */
+  PL_THINC_NORMAL_COMP,    /* the y normal on x faces */
+  PL_OTH_BRACKBILL,        /* the face density missing from the Brackbill surface tension */
+  PL_OTH_KAPPA_FACE,       /* the cell's own curvature instead of the face average */
+  PL_OTH_EXTRAP,           /* (1+dt_r)*po - dt_r*pn: the two pressure levels swapped */
+  PL_OTH_SPLIT,            /* rho0/rho_face instead of rho_face/rho0 */
+  PL_OTH_SURF_FACTOR,      /* sigma/(rho1+rho2) instead of 2 sigma/(rho1+rho2) */
+  PL_DT_VISC,              /* the factor 2 of the viscous limit dropped */
+  PL_DT_CAP_RHO,           /* the larger density in the capillary limit */
+  PL_DT_GRAV,              /* |g|/dlmin**2 for the gravity limit */
+  PL_DT_DLMIN,             /* dlmin over dx and dy only */
+  PL_DT_COMBINE,           /* the 0.5 of the combined limit dropped */
+  PL_INIT_WIDTH,           /* tanh(r/eps) instead of tanh(r/(2 eps)) in the smooth step */
+  PL_INIT_PERIODIC,        /* no periodic images of the spheres */
+  PL_INIT_FLM_INDEX,       /* the film's distance taken at zc(k-1) instead of zc(k) */
+};
+
 static inline long HX(const int *n, int i, int j, int k) {
   return i + (long)(n[0] + 2) * (j + (long)(n[1] + 2) * k);
 }
@@ -762,24 +797,28 @@ void or_acdi_transport_pf(const int n[3], const double dli[3], const double *dzc
         const double gs = gam * seps;
         const double dpsidxm = gs * (psi_ccc - psi_mcc) * dxi, dpsidxp = gs * (psi_pcc - psi_ccc) * dxi;
         const double dpsidym = gs * (psi_ccc - psi_cmc) * dyi, dpsidyp = gs * (psi_cpc - psi_ccc) * dyi;
-        const double dpsidzm = gs * (psi_ccc - psi_ccm) * dzci_m, dpsidzp = gs * (psi_ccp - psi_ccc) * dzci_c;
+        const double dpsidzm = gs * (psi_ccc - psi_ccm) * (PLANTED(PL_ACDI_DZCI_M) ? dzci_c : dzci_m),
+                     dpsidzp = gs * (psi_ccp - psi_ccc) * dzci_c;
         const double rn_01 = nx_xm / (sqrt(nx_xm * nx_xm + ny_xm * ny_xm + nz_xm * nz_xm) + EPS);
-        const double rn_11 = nx_xp / (sqrt(nx_xp * nx_xp + ny_xp * ny_xp + nz_xp * nz_xp) + EPS);
+        const double rn_11 = (PLANTED(PL_ACDI_NORMAL_COMP) ? ny_xp : nx_xp) /
+                             (sqrt(nx_xp * nx_xp + ny_xp * ny_xp + nz_xp * nz_xp) + EPS);
         const double rn_02 = ny_ym / (sqrt(nx_ym * nx_ym + ny_ym * ny_ym + nz_ym * nz_ym) + EPS);
         const double rn_12 = ny_yp / (sqrt(nx_yp * nx_yp + ny_yp * ny_yp + nz_yp * nz_yp) + EPS);
         const double rn_03 = nz_zm / (sqrt(nx_zm * nx_zm + ny_zm * ny_zm + nz_zm * nz_zm) + EPS);
         const double rn_13 = nz_zp / (sqrt(nx_zp * nx_zp + ny_zp * ny_zp + nz_zp * nz_zp) + EPS);
-        const double g4 = 0.25 * gam;
+        const double g4 = (PLANTED(PL_ACDI_QUARTER) ? 0.5 : 0.25) * gam;
+        const double ta = PLANTED(PL_ACDI_TANH_ARG) ? 0.5 : 0.25;
         double t;
-        t = om_tanh(0.25 * (phi_ccc + phi_mcc) / seps); const double sharpxm = g4 * ((1.0 - t * t) * rn_01);
-        t = om_tanh(0.25 * (phi_pcc + phi_ccc) / seps); const double sharpxp = g4 * ((1.0 - t * t) * rn_11);
-        t = om_tanh(0.25 * (phi_ccc + phi_cmc) / seps); const double sharpym = g4 * ((1.0 - t * t) * rn_02);
-        t = om_tanh(0.25 * (phi_cpc + phi_ccc) / seps); const double sharpyp = g4 * ((1.0 - t * t) * rn_12);
-        t = om_tanh(0.25 * (phi_ccc + phi_ccm) / seps); const double sharpzm = g4 * ((1.0 - t * t) * rn_03);
-        t = om_tanh(0.25 * (phi_ccp + phi_ccc) / seps); const double sharpzp = g4 * ((1.0 - t * t) * rn_13);
-        const double rglrxm = dpsidxm - sharpxm, rglrxp = dpsidxp - sharpxp;
-        const double rglrym = dpsidym - sharpym, rglryp = dpsidyp - sharpyp;
-        const double rglrzm = dpsidzm - sharpzm, rglrzp = dpsidzp - sharpzp;
+        t = om_tanh(ta * (phi_ccc + phi_mcc) / seps); const double sharpxm = g4 * ((1.0 - t * t) * rn_01);
+        t = om_tanh(ta * (phi_pcc + phi_ccc) / seps); const double sharpxp = g4 * ((1.0 - t * t) * rn_11);
+        t = om_tanh(ta * (phi_ccc + phi_cmc) / seps); const double sharpym = g4 * ((1.0 - t * t) * rn_02);
+        t = om_tanh(ta * (phi_cpc + phi_ccc) / seps); const double sharpyp = g4 * ((1.0 - t * t) * rn_12);
+        t = om_tanh(ta * (phi_ccc + phi_ccm) / seps); const double sharpzm = g4 * ((1.0 - t * t) * rn_03);
+        t = om_tanh(ta * (phi_ccp + phi_ccc) / seps); const double sharpzp = g4 * ((1.0 - t * t) * rn_13);
+        const double sg = PLANTED(PL_ACDI_SHARP_SIGN) ? -1.0 : 1.0;
+        const double rglrxm = dpsidxm - sg * sharpxm, rglrxp = dpsidxp - sg * sharpxp;
+        const double rglrym = dpsidym - sg * sharpym, rglryp = dpsidyp - sg * sharpyp;
+        const double rglrzm = dpsidzm - sg * sharpzm, rglrzp = dpsidzp - sg * sharpzp;
         const double rglr = (rglrxp - rglrxm) * dxi + (rglryp - rglrym) * dyi + (rglrzp - rglrzm) * dzfi_c;
         dpsidt[IX(n, i, j, k)] = -adv + rglr;
         flux_x[q] = upsiip - rglrxp;
@@ -832,10 +871,16 @@ void or_vof_thinc_transport_psi(const int n[3], const double dli[3], const doubl
         const double vel_x = u[q], vel_y = v[q], vel_z = w[q];
         long qx, qy, qz;
         double xx, yy, zz;
-        if (vel_x > 0.0) { qx = q; xx = 0.5; } else { qx = q + 1; xx = -0.5; }
+        if (vel_x > 0.0) {
+          qx = PLANTED(PL_THINC_UPWIND) ? q + 1 : q;
+          xx = PLANTED(PL_THINC_OFFSET) ? -0.5 : 0.5;
+        } else {
+          qx = q + 1;
+          xx = -0.5;
+        }
         if (vel_y > 0.0) { qy = q; yy = 0.5; } else { qy = q + sy; yy = -0.5; }
         if (vel_z > 0.0) { qz = q; zz = 0.5; } else { qz = q + sz; zz = -0.5; }
-        const double lnx = normx[qx], lny = normy[qy], lnz = normz[qz];
+        const double lnx = PLANTED(PL_THINC_NORMAL_COMP) ? normy[qx] : normx[qx], lny = normy[qy], lnz = normz[qz];
         const double d_x = phi[qx], d_y = phi[qy], d_z = phi[qz];
         const double lfx = 1.0 / (1.0 + om_exp(-2.0 * beta * (lnx * xx + d_x)));
         const double lfy = 1.0 / (1.0 + om_exp(-2.0 * beta * (lny * yy + d_y)));
@@ -1078,7 +1123,7 @@ void or_mom_xyz_oth(const int n[3], const double dli[3], const double *dzci, con
   (void)dzfi;
   const double rho = rho12[1], drho = rho12[0] - rho12[1];
   const double dxi = dli[0], dyi = dli[1];
-  const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
+  const double surf_factor = sigma * (PLANTED(PL_OTH_SURF_FACTOR) ? 1.0 : 2.0) / (rho12[0] + rho12[1]);
   const long sy = n[0] + 2, sz = (long)(n[0] + 2) * (n[1] + 2);
 #pragma omp parallel for schedule(static)
   for (int k = 1; k <= n[2]; k++)
@@ -1087,10 +1132,11 @@ void or_mom_xyz_oth(const int n[3], const double dli[3], const double *dzci, con
         const long q = HX(n, i, j, k);
         const double dzci_c = dzci[k];
         const double p_ccc = p[q], p_pcc = p[q + 1], p_cpc = p[q + sy], p_ccp = p[q + sz];
-        const double q_ccc = (1.0 + dt_r) * pn[q] - dt_r * po[q];
-        const double q_pcc = (1.0 + dt_r) * pn[q + 1] - dt_r * po[q + 1];
-        const double q_cpc = (1.0 + dt_r) * pn[q + sy] - dt_r * po[q + sy];
-        const double q_ccp = (1.0 + dt_r) * pn[q + sz] - dt_r * po[q + sz];
+        const double *pe1 = PLANTED(PL_OTH_EXTRAP) ? po : pn, *pe2 = PLANTED(PL_OTH_EXTRAP) ? pn : po;
+        const double q_ccc = (1.0 + dt_r) * pe1[q] - dt_r * pe2[q];
+        const double q_pcc = (1.0 + dt_r) * pe1[q + 1] - dt_r * pe2[q + 1];
+        const double q_cpc = (1.0 + dt_r) * pe1[q + sy] - dt_r * pe2[q + sy];
+        const double q_ccp = (1.0 + dt_r) * pe1[q + sz] - dt_r * pe2[q + sz];
         const double k_ccc = kappa[q], k_pcc = kappa[q + 1], k_cpc = kappa[q + sy], k_ccp = kappa[q + sz];
         const double c_ccc = psi[q], c_pcc = psi[q + 1], c_cpc = psi[q + sy], c_ccp = psi[q + sz];
         const double psixp = 0.5 * (c_pcc + c_ccc), psiyp = 0.5 * (c_cpc + c_ccc), psizp = 0.5 * (c_ccp + c_ccc);
@@ -1101,14 +1147,15 @@ void or_mom_xyz_oth(const int n[3], const double dli[3], const double *dzci, con
         double dwdt_aux = bforce[2] + (rhozp - rho_av[2]) * gacc[2];
         double skx = 0.5 * (k_pcc + k_ccc) * surf_factor, sky = 0.5 * (k_cpc + k_ccc) * surf_factor,
                skz = 0.5 * (k_ccp + k_ccc) * surf_factor;
-        skx = skx * rhoxp; sky = sky * rhoyp; skz = skz * rhozp;
+        if (PLANTED(PL_OTH_KAPPA_FACE)) skx = k_ccc * surf_factor;
+        if (!PLANTED(PL_OTH_BRACKBILL)) { skx = skx * rhoxp; sky = sky * rhoyp; skz = skz * rhozp; }
         const double surfx = skx * (c_pcc - c_ccc) * dxi, surfy = sky * (c_cpc - c_ccc) * dyi,
                      surfz = skz * (c_ccp - c_ccc) * dzci_c;
         dudt_aux = dudt_aux + surfx;
         dvdt_aux = dvdt_aux + surfy;
         dwdt_aux = dwdt_aux + surfz;
         const double dpdx_e = (q_pcc - q_ccc) * dxi, dpdy_e = (q_cpc - q_ccc) * dyi, dpdz_e = (q_ccp - q_ccc) * dzci_c;
-        dudt_aux = dudt_aux - (dpdx_e + rhoxp / rho0 * (dpdx - dpdx_e));
+        dudt_aux = dudt_aux - (dpdx_e + (PLANTED(PL_OTH_SPLIT) ? rho0 / rhoxp : rhoxp / rho0) * (dpdx - dpdx_e));
         dvdt_aux = dvdt_aux - (dpdy_e + rhoyp / rho0 * (dpdy - dpdy_e));
         dwdt_aux = dwdt_aux - (dpdz_e + rhozp / rho0 * (dpdz - dpdz_e));
         const long r = IX(n, i, j, k);
@@ -1256,7 +1303,7 @@ double or_chkdt(const int n[3], const double dl[3], const double *dzci, const do
   double dlmin = dmin(dl[0], dl[1]);
   double m = 1.0 / dzfi[0];
   for (int k = 1; k <= n[2] + 1; k++) m = dmin(m, 1.0 / dzfi[k]);
-  dlmin = dmin(dlmin, m);
+  if (!PLANTED(PL_DT_DLMIN)) dlmin = dmin(dlmin, m);
   double dti = 0.0;
   for (int k = 1; k <= n[2]; k++)
     for (int j = 1; j <= n[1]; j++)
@@ -1287,10 +1334,14 @@ double or_chkdt(const int n[3], const double dl[3], const double *dzci, const do
   if (!is_solve_ns) {
     dtmax = dmin(1.0 / dti, 1.0 / dtipsi);
   } else {
-    const double dtiv = dmax(mu12[0] / rho12[0], mu12[1] / rho12[1]) * 2.0 * (3.0 / (dlmin * dlmin));
-    const double dtik = sqrt(sigma / (dmin(rho12[0], rho12[1])) / (dlmin * dlmin * dlmin));
-    const double dtig = dmax(fabs(gacc[0]), dmax(fabs(gacc[1]), fabs(gacc[2]))) / dlmin;
-    dti = 0.5 * (dti + dtiv + sqrt((dti + dtiv) * (dti + dtiv) + 4.0 * (dtig * dtig + dtik * dtik)));
+    const double dtiv = dmax(mu12[0] / rho12[0], mu12[1] / rho12[1]) * (PLANTED(PL_DT_VISC) ? 1.0 : 2.0) *
+                        (3.0 / (dlmin * dlmin));
+    const double rmin = PLANTED(PL_DT_CAP_RHO) ? dmax(rho12[0], rho12[1]) : dmin(rho12[0], rho12[1]);
+    const double dtik = sqrt(sigma / (rmin) / (dlmin * dlmin * dlmin));
+    const double dtig = dmax(fabs(gacc[0]), dmax(fabs(gacc[1]), fabs(gacc[2]))) /
+                        (PLANTED(PL_DT_GRAV) ? dlmin * dlmin : dlmin);
+    dti = (PLANTED(PL_DT_COMBINE) ? 1.0 : 0.5) *
+          (dti + dtiv + sqrt((dti + dtiv) * (dti + dtiv) + 4.0 * (dtig * dtig + dtik * dtik)));
     if (dti == 0.0) dti = 1.0;
     dtmax = 1.0 / dti;
     if (is_track_interface) dtmax = dmin(dtmax, 1.0 / dtipsi);
@@ -1301,7 +1352,9 @@ double or_chkdt(const int n[3], const double dl[3], const double *dzci, const do
 /* ======================================================================
  * Initial phase field, src/two_fluid.f90:247-437 and :503-513
  * ====================================================================== */
-static double smooth_step_tanh(double r, double eps) { return .5 * (1.0 + om_tanh(r / (2.0 * eps))); }
+static double smooth_step_tanh(double r, double eps) {
+  return .5 * (1.0 + om_tanh(r / ((PLANTED(PL_INIT_WIDTH) ? 1.0 : 2.0) * eps)));
+}
 
 void or_init2fl(const or_params *prm, double seps, const double *zc, double *psi) {
   const int *n = prm->ng;
@@ -1322,7 +1375,8 @@ void or_init2fl(const or_params *prm, double seps, const double *zc, double *psi
   case 8:
     for (int k = 1; k <= n[2]; k++)
       for (int j = 1; j <= n[1]; j++)
-        for (int i = 1; i <= n[0]; i++) psi[HX(n, i, j, k)] = smooth_step_tanh(zc[k] - prm->l[2] / 2.0, seps);
+        for (int i = 1; i <= n[0]; i++)
+          psi[HX(n, i, j, k)] = smooth_step_tanh(zc[PLANTED(PL_INIT_FLM_INDEX) ? k - 1 : k] - prm->l[2] / 2.0, seps);
     break;
   default: break;
   }
@@ -1339,7 +1393,8 @@ void or_init2fl(const or_params *prm, double seps, const double *zc, double *psi
       nsph = 1;
     }
     int iperiod[3];
-    for (int d = 0; d < 3; d++) iperiod[d] = (prm->cbcpsi[2 * d] == 'P' && prm->cbcpsi[2 * d + 1] == 'P');
+    for (int d = 0; d < 3; d++)
+      iperiod[d] = (prm->cbcpsi[2 * d] == 'P' && prm->cbcpsi[2 * d + 1] == 'P') && !PLANTED(PL_INIT_PERIODIC);
     double lmax = 0.0;
     for (int d = 0; d < 3; d++) lmax = dmax(lmax, prm->l[d] * is_dim[d]);
     for (int s = 0; s < nsph; s++) {
@@ -1468,8 +1523,12 @@ static double new_dt(or_case *c, int *kill) {
   c->sc[8] = dt_cfl;
   double dt = dmin(P->cfl * dt_cfl, P->dtmax);
   if (P->dt_f > 0.0) {
-    if (P->dt_f > dt && kill) *kill = 1;
-    dt = P->dt_f;
+    /* in the loop (src/main.f90:534-541) a fixed dt above the stable one stops
+     * the run and keeps the stable dt; at start-up (:397-403) it only warns */
+    if (P->dt_f > dt && kill)
+      *kill = 1;
+    else
+      dt = P->dt_f;
   }
   return dt;
 }

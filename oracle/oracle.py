@@ -84,7 +84,28 @@ def fill_struct(p, case):
     p.inipsi = INIPSI[case.get("inipsi", "uni")]
 
 
+PLANTED_PATH = os.path.join(HERE, "liboracle_planted.so")
+# the numbered planted mistakes of liboracle_planted.so (the PL_* enum of fz_oracle.c)
+PLANTED = ["ACDI_SHARP_SIGN", "ACDI_QUARTER", "ACDI_TANH_ARG", "ACDI_NORMAL_COMP", "ACDI_DZCI_M", "THINC_UPWIND",
+           "THINC_OFFSET", "THINC_NORMAL_COMP", "OTH_BRACKBILL", "OTH_KAPPA_FACE", "OTH_EXTRAP", "OTH_SPLIT",
+           "OTH_SURF_FACTOR", "DT_VISC", "DT_CAP_RHO", "DT_GRAV", "DT_DLMIN", "DT_COMBINE", "INIT_WIDTH",
+           "INIT_PERIODIC", "INIT_FLM_INDEX"]
+
 _lib = None
+_planted = None
+
+
+def planted_lib(name):
+    """liboracle_planted.so with the planted mistake `name` selected (a name
+    of PLANTED): the restatement with one deliberate slip, for showing that a
+    pin test fails on it. Never the checker of anything."""
+    global _planted
+    if _planted is None:
+        if not os.path.exists(PLANTED_PATH):
+            raise RuntimeError(f"oracle library missing: {PLANTED_PATH} (run make -C oracle)")
+        _planted = _setup(C.CDLL(PLANTED_PATH))
+    C.c_int.in_dll(_planted, "or_planted").value = PLANTED.index(name) + 1
+    return _planted
 
 
 def lib():
@@ -92,23 +113,25 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
-        L = C.CDLL(LIB_PATH)
-        dp = C.POINTER(C.c_double)
-        L.or_case_create.restype = C.c_void_p
-        L.or_case_create.argtypes = [C.POINTER(OrParams)]
-        L.or_case_destroy.argtypes = [C.c_void_p]
-        L.or_case_field.restype = dp
-        L.or_case_field.argtypes = [C.c_void_p, C.c_int]
-        L.or_case_scalars.restype = dp
-        L.or_case_scalars.argtypes = [C.c_void_p]
-        L.or_case_init_fields.argtypes = [C.c_void_p]
-        L.or_case_step.argtypes = [C.c_void_p, C.c_int]
-        L.or_case_step.restype = C.c_int
-        for name in ["or_chk_poisson", "or_acdi_set_gamma", "or_acdi_set_epsilon", "or_chkdt",
-                     "or_bulk_mean_12_stag"]:
-            getattr(L, name).restype = C.c_double
-        _lib = L
+        _lib = _setup(C.CDLL(LIB_PATH))
     return _lib
+
+
+def _setup(L):
+    dp = C.POINTER(C.c_double)
+    L.or_case_create.restype = C.c_void_p
+    L.or_case_create.argtypes = [C.POINTER(OrParams)]
+    L.or_case_destroy.argtypes = [C.c_void_p]
+    L.or_case_field.restype = dp
+    L.or_case_field.argtypes = [C.c_void_p, C.c_int]
+    L.or_case_scalars.restype = dp
+    L.or_case_scalars.argtypes = [C.c_void_p]
+    L.or_case_init_fields.argtypes = [C.c_void_p]
+    L.or_case_step.argtypes = [C.c_void_p, C.c_int]
+    L.or_case_step.restype = C.c_int
+    for name in ["or_chk_poisson", "or_acdi_set_gamma", "or_acdi_set_epsilon", "or_chkdt", "or_bulk_mean_12_stag"]:
+        getattr(L, name).restype = C.c_double
+    return L
 
 
 def ptr(a):

@@ -309,3 +309,81 @@ def poisson_solve_bitwise(n, cbc, rng):
         assert np.array_equal(got, ref), rel_err(got, ref)
     finally:
         pr.close()
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s[0])))
+def test_checks_and_bulk_mean(shape, rng):
+    """chkdiv (src/chkdiv.f90:15-49), chkdt (src/chkdt.f90:15-94) and
+    bulk_mean_12_stag (src/utils.f90:68-136) through the C ABI against the
+    oracle. Maxima are order-independent and compared bit for bit; the sums
+    differ only in summation order (divtot: the round-off of a sum of N
+    terms of size divmax; the bulk mean: a sum of positive terms)."""
+    n, l, gr = shape
+    pr = Pair(n, l, "PPPPNN", gr)
+    try:
+        u, v, w = rand(rng, n), rand(rng, n), rand(rng, n)
+        du, dv, dw = pr.h().upload(u), pr.h().upload(v), pr.h().upload(w)
+        dt_, dm_ = C.c_double(), C.c_double()
+        F.call("fz_chkdiv", pr.ctx.h, du.ptr, dv.ptr, dw.ptr, C.byref(dt_), C.byref(dm_))
+        divtot, divmax = O.chkdiv(n, pr.dli, pr.dzfi, u, v, w)
+        assert dm_.value == divmax
+        ncell = n[0] * n[1] * n[2]
+        assert abs(dt_.value - divtot) <= 1e-13 * ncell * divmax
+        for prm in [(1, 1, 0, (0.3, 0.02), (50.0, 1.0), 0.07, (0.0, 0.0, -9.8), 0.7, 0.01),
+                    (1, 0, 0, (0.0, 0.0), (1.0, 1.0), 0.0, (0.0, 0.0, 0.0), 0.0, 0.0),
+                    (0, 1, 1, (0.1, 0.1), (2.0, 1.0), 0.5, (1.0, 0.0, 0.0), 0.0, 0.0)]:
+            solve_ns, track, vof, mu12, rho12, sigma, gacc, gam, seps = prm
+            got = C.c_double()
+            F.call("fz_chkdt", pr.ctx.h, solve_ns, track, vof, dvec(mu12), dvec(rho12), sigma, dvec(gacc), du.ptr,
+                   dv.ptr, dw.ptr, gam, seps, C.byref(got))
+            ref = L.or_chkdt(I(n), D(pr.dl), P(pr.dzci), P(pr.dzfi), solve_ns, track, vof, D(mu12), D(rho12),
+                             C.c_double(sigma), D(gacc), P(u), P(v), P(w), C.c_double(gam), C.c_double(seps))
+            assert got.value == ref, prm
+        psi = rand(rng, n, 0.0, 1.0)
+        dpsi = pr.h().upload(psi)
+        vol = l[0] * l[1] * l[2]
+        for idir in (1, 2, 3):
+            gvr = np.ascontiguousarray(pr.dl[0] * pr.dl[1] * (pr.dzc if idir < 3 else pr.dzf) / vol)
+            got = C.c_double()
+            F.call("fz_bulk_mean_12_stag", pr.ctx.h, idir, dpsi.ptr, dvec([40.0, 1.5]), C.byref(got))
+            ref = L.or_bulk_mean_12_stag(I(n), idir, P(gvr), P(psi), D([40.0, 1.5]))
+            assert abs(got.value - ref) <= 1e-14 * abs(ref), (idir, got.value, ref)
+    finally:
+        pr.close()
+
+
+RHSB = [([12, 10, 9], "DDNNDD"), ([20, 14, 8], "NDDNPP"), ([16, 12, 10], "PPPPND"), ([10, 8, 12], "DNNDDN")]
+
+
+@pytest.mark.parametrize("n,cbc", RHSB, ids=[f"{'x'.join(map(str, a))}-{b}" for a, b in RHSB])
+def test_updt_rhs_b_with_boundary_values(n, cbc, rng):
+    """updt_rhs_b (src/bound.f90:341-396) with non-zero Dirichlet and Neumann
+    pressure values: the boundary terms made by fz_initsolver (bc_rhs,
+    src/initsolver.f90:184-228) added to p, then the solve, bit for bit
+    against the oracle."""
+    pr = Pair(n, [1.0, 0.9, 1.7], cbc, gr=0.6 if cbc[4:] != "PP" else 0.0)
+    try:
+        bc = list(rng.uniform(-1, 1, 6))
+        a, b, c = np.zeros(n[2]), np.zeros(n[2]), np.zeros(n[2])
+        lam = np.zeros((n[0], n[1]), order="F")
+        rbx, rby, rbz = (np.zeros(s, order="F") for s in ((n[1], n[2], 2), (n[0], n[2], 2), (n[0], n[1], 2)))
+        nf = C.c_double()
+        dptr = C.POINTER(C.c_double)
+        F.call("fz_initsolver", pr.ctx.h, dvec(bc), lam.ctypes.data_as(dptr), a.ctypes.data_as(dptr),
+               b.ctypes.data_as(dptr), c.ctypes.data_as(dptr), C.byref(nf), rbx.ctypes.data_as(dptr),
+               rby.ctypes.data_as(dptr), rbz.ctypes.data_as(dptr))
+        sol = O.Solver(n, pr.dli, pr.dzci, pr.dzfi, list(cbc), bc=bc)
+        for got, ref in ((rbx, sol.rhsbx), (rby, sol.rhsby), (rbz, sol.rhsbz)):
+            assert np.array_equal(got, ref)
+        assert any(np.abs(r).max() > 0 for r in (rbx, rby, rbz))
+        f = rand(rng, n)
+        dpx = pr.h().upload(f)
+        F.call("fz_updt_rhs_b", pr.ctx.h, dpx.ptr)
+        ref = f.copy(order="F")
+        L.or_updt_rhs_b(I(n), S(cbc), P(sol.rhsbx), P(sol.rhsby), P(sol.rhsbz), P(ref))
+        assert np.array_equal(dpx.download(), ref)
+        F.call("fz_solver", pr.ctx.h, dpx.ptr)
+        sol.solve(ref)
+        assert np.array_equal(dpx.download(), ref)
+    finally:
+        pr.close()

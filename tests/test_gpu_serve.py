@@ -161,6 +161,39 @@ def test_a_request_of_zero_steps_finishes():
         srv.close()
 
 
+def test_a_failed_request_stops_only_itself():
+    """A request whose set-up fails (mixed precision with non-homogeneous
+    pressure boundary values is refused by fz_case_create_ex) ends in state 3
+    with its error message, its memory returns at once, and the requests
+    beside it finish with the fields they have alone."""
+    good = serve.bubble_case(16, 32, density_ratio=30.0)
+    bad = dict(serve.bubble_case(16, 32), precision="mixed", refine=1, cbcpre=list("PPPPND"),
+               bcpre=[0.0, 0.0, 0.0, 0.0, 0.0, 0.5])
+    ref = alone(good, 6)
+    srv = serve.Server(serve.footprint(good) + serve.footprint(bad))
+    try:
+        a = srv.submit(bad, 4)
+        b = srv.submit(good, 6)
+        for _ in range(100):
+            srv.round()
+            if srv.status(a)[0] in (2, 3) and srv.status(b)[0] in (2, 3):
+                break
+        assert srv.status(a)[0] == serve.Server.FAILED
+        assert "homogeneous" in srv.error(a)
+        assert srv.error(b) == ""
+        with pytest.raises(F.FizzyError):
+            srv.case(a)
+        assert srv.status(b)[0] == serve.Server.FINISHED
+        dc = srv.case(b)
+        for k in FIELDS:
+            assert np.array_equal(dc.get(k), ref[0][k]), k
+        srv.release(a)
+        srv.release(b)
+        assert srv.memory()[0] == 0
+    finally:
+        srv.close()
+
+
 def test_close_while_cases_are_being_set_up():
     """Closing the server with requests still queued for (or in) set-up on
     its thread returns, and a new server works after it."""
