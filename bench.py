@@ -10,9 +10,11 @@ momentum, projection with the FFT Poisson solve, gamma, and the dt/divergence
 checks every 10 steps. Fields stay resident in HBM; the timed region holds
 only device work and the checks' 8-byte read-backs.
 
-Prints one JSON line: metric Mcell-steps/s (cells x steps / seconds / 1e6),
-the dominant kernel's roofline from per-launch HIP events recorded over the
-timed region, and the CPU oracle timed on this host.
+Prints one JSON line: metric Mcell-steps/s (cells x steps / seconds / 1e6)
+over the timed region, where the steps replay their captured HIP graphs with
+no per-launch events; the dominant kernel's roofline from a second,
+instrumented pass over as many steps (direct launches, HIP events around
+every launch on the case's stream); and the CPU oracle timed on this host.
 """
 import argparse
 import ctypes as C
@@ -142,6 +144,61 @@ def cpu_baseline(spec, budget_s=15.0, max_steps=20):
                       f"(OpenMP, {threads} threads), {dt:.1f} s"}
 
 
+def cpu_baseline_poisson(n, f, budget_s=12.0, max_solves=5):
+    """The oracle's FP64 solve (plain C, OpenMP) of the same 512^3 right-hand
+    side on this host's cores: whole solves until about budget_s seconds or
+    max_solves solves (the plan's set-up not timed)."""
+    import numpy as np
+
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    dzc, dzf, zc, zf = O.initgrid(1, n[2], 0.0, 1.0)
+    dli = [n[0] / 1.0, n[1] / 1.0]
+    sol = O.Solver(n, dli + [float(n[2])], 1.0 / dzc, 1.0 / dzf, list("PPPPNN"))
+    p = np.empty_like(f)
+    solves, t = 0, 0.0
+    while solves < max_solves and t < budget_s:
+        p[...] = f
+        t0 = time.perf_counter()
+        sol.solve(p)
+        t += time.perf_counter() - t0
+        solves += 1
+    cells = n[0] * n[1] * n[2]
+    log(f"cpu oracle: {solves} solves of {n[0]}x{n[1]}x{n[2]} in {t:.2f} s on {threads} threads")
+    return {"value": round(cells * solves / t / 1e6, 2), "unit": "Mcell-solves/s", "cores": threads, "kind": "port",
+            "sample": f"{solves} FP64 solves of the same {n[0]}x{n[1]}x{n[2]} right-hand side by the C oracle "
+                      f"(OpenMP, {threads} threads), {t:.1f} s"}
+
+
+def cpu_baseline_serving(reqs, budget_s=15.0, max_steps=20):
+    """The oracle on this host's cores over the serving mix's requests in
+    arrival order: each request's case initialised and stepped (up to
+    max_steps of its steps, after its first), until about budget_s seconds of
+    stepping; Mcell-steps/s over the timed steps."""
+    import numpy as np
+
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    t, cell_steps, used = 0.0, 0, 0
+    for _, spec, nsteps in reqs:
+        if t >= budget_s:
+            break
+        oc = O.Case(spec)
+        oc.init_fields()
+        oc.step(1)
+        k = min(max_steps, max(nsteps - 1, 0))
+        t0 = time.perf_counter()
+        oc.step(k)
+        t += time.perf_counter() - t0
+        cell_steps += int(np.prod(spec["ng"])) * k
+        used += 1
+        del oc
+    log(f"cpu oracle: {used} requests, {cell_steps / 1e6:.1f} Mcell-steps in {t:.2f} s on {threads} threads")
+    return {"value": round(cell_steps / t / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"the first {used} requests of the mix in arrival order, steps 2-{max_steps + 1} of each "
+                      f"(FP64), by the C oracle (OpenMP, {threads} threads), {t:.1f} s"}
+
+
 def main_serving(args):
     """The serving mix: Poisson arrivals (mean 50 ms) of 2-D/3-D rising bubbles of
     uneven grids and density ratios, 200 steps each, admitted by HBM footprint
@@ -180,6 +237,10 @@ def main_serving(args):
     seq = time.perf_counter() - t0
     thr, seq_thr = sum(cells) / wall / 1e6, sum(cells) / seq / 1e6
     log(f"sequential: {seq:.2f} s ({seq_thr:.1f} Mcell-steps/s); served {thr:.1f} Mcell-steps/s")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_serving(reqs)
+        cpu["cpu_model"] = cpu_model()
     out = {
         "metric": "ensemble p50/p99 step latency",
         "value": round(float(np.percentile(lat, 99)), 4), "unit": "ms (p99 per-case step)",
@@ -192,6 +253,7 @@ def main_serving(args):
         "config": {"workload": "serving_mix", "requests": len(reqs), "steps_per_case": args.case_steps,
                    "arrivals": f"Poisson, mean {args.arrival_ms:g} ms", "grids": "64^2, 128^2, 256^2 (2-D), 64^2x128, 128^2x256",
                    "density_ratio": "uniform [2, 1000]"},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -303,60 +365,106 @@ def main_poisson(args):
     finally:
         ctx.close()
     res["batched"] = batched_poisson(F, np)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_poisson(n, f)
+        cpu["cpu_model"] = cpu_model()
     out = {
         "metric": "standalone Poisson solve, FP64 vs FP32 + FP64-residual refinement",
         "value": res["fp64"]["Mcell_solves_s"], "unit": "Mcell-solves/s (FP64)", "n_gpus": 1, "steps": nsolve,
         "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: uniform [-1,1] right-hand side, seeded, volume-weighted mean removed",
         "config": {"workload": "poisson_512x512x512_PPPPNN", "grid": n}, "fp64": res["fp64"],
-        "mixed": res["mixed"], "batched": res["batched"],
+        "mixed": res["mixed"], "batched": res["batched"], "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
-def run_mixed(F, spec, args, cells, fp64_s_per_step, refine=None):
-    """The same workload in the mixed-precision mode: timed the same way as
-    the FP64 run (same steps, warm-up and per-launch events). refine -1: the
-    FP32 phase field with the plain step's FP64 projection."""
-    refine = args.refine if refine is None else refine
-    ms = dict(spec, precision="mixed", refine=refine)
-    dc = F.DeviceCase(ms)
+def time_steps(F, dc, steps, graphs, prof):
+    """Wall time of `steps` steps of a case (stream synchronised on both sides):
+    replayed from captured graphs or launched directly, with per-launch HIP
+    events (prof) or without; returns (seconds, killed, per-launch rows)."""
+    F.call("fz_case_set_graphs", dc._h, 1 if graphs else 0)
+    F.call("fz_prof_enable", dc.ctx, 1 if prof else 0)
+    prof_report(F, dc.ctx)  # clears earlier records
+    dc.sync()
+    t0 = time.perf_counter()
+    killed = dc.step(steps)
+    dc.sync()
+    t1 = time.perf_counter()
+    F.call("fz_prof_enable", dc.ctx, 0)
+    rows = prof_report(F, dc.ctx) if prof else {}
+    return t1 - t0, killed, rows
+
+
+def log_rows(rows, width=22):
+    total_ms = sum(m for _, m in rows.values())
+    for name, (calls, m) in sorted(rows.items(), key=lambda kv: -kv[1][1])[:12]:
+        log(f"  {name:{width}s} {calls:5d} launches {m / calls * 1e3:9.1f} us/launch {100 * m / total_ms:5.1f}%")
+
+
+def run_case_timed(F, spec, args, label):
+    """Initialise, autotune and warm up a case (its step graphs are captured
+    in the warm-up), then: the timed region, `steps` steps replayed from the
+    graphs with no per-launch events (the headline); and an instrumented
+    pass over as many further steps, launched directly with HIP events
+    around every launch (the per-kernel table and the kernel roofline)."""
+    dc = F.DeviceCase(spec)
     try:
         dc.init()
+        tuning = None
         if not args.no_autotune:
-            log(f"autotune (mixed): {dc.autotune(3)}")
-        if args.graphs:
-            F.call("fz_case_set_graphs", dc._h, 1)
+            tuning = dc.autotune(3)
+            log(f"autotune ({label}): {tuning}")
+        graphs = not args.no_graphs
+        F.call("fz_case_set_graphs", dc._h, 1 if graphs else 0)
         if args.warmup:
             dc.step(args.warmup)
-        dc.sync()
-        F.call("fz_prof_enable", dc.ctx, 0 if args.no_prof else 1)
-        prof_report(F, dc.ctx)
-        t0 = time.perf_counter()
-        killed = dc.step(args.steps)
-        dc.sync()
-        t1 = time.perf_counter()
-        F.call("fz_prof_enable", dc.ctx, 0)
-        rows = {} if args.no_prof else prof_report(F, dc.ctx)
+        elapsed, killed, _ = time_steps(F, dc, args.steps, graphs, prof=False)
         sc = dc.scalars()
+        inst, ikilled, rows = (None, 0, {})
+        if not args.no_prof:
+            inst, ikilled, rows = time_steps(F, dc, args.steps, graphs=False, prof=True)
+        copy_gbs = None
+        if label == "fp64":
+            copy_gbs = copy_bandwidth(F, dc.ctx, 8 * dc_count(dc))
+        return dict(elapsed=elapsed, killed=killed or ikilled, divmax=float(sc[7]), rows=rows, inst=inst,
+                    tuning=tuning, copy_gbs=copy_gbs)
     finally:
         dc.close()
-    s_per_step = (t1 - t0) / args.steps
-    value = cells / s_per_step / 1e6
+
+
+def run_mixed(F, spec, args, cells, fp64_s_per_step, refine=None):
+    """The same workload in the mixed-precision mode, timed the same way as
+    the FP64 run. refine -1: the FP32 phase field with the plain step's FP64
+    projection."""
+    refine = args.refine if refine is None else refine
     what = "FP64 projection" if refine < 0 else f"{refine} refinements"
-    log(f"mixed ({what}): {1e3 * s_per_step:.3f} ms/step, {value:.1f} Mcell-steps/s, "
-        f"divmax {sc[7]:.2e}, killed {killed}")
-    if rows:
-        total_ms = sum(m for _, m in rows.values())
-        for name, (calls, m) in sorted(rows.items(), key=lambda kv: -kv[1][1])[:12]:
-            log(f"  {name:22s} {calls:5d} launches {m / calls * 1e3:9.1f} us/launch {100 * m / total_ms:5.1f}%")
+    r = run_case_timed(F, dict(spec, precision="mixed", refine=refine), args, "mixed")
+    s_per_step = r["elapsed"] / args.steps
+    value = cells / s_per_step / 1e6
+    log(f"mixed ({what}): {1e3 * s_per_step:.3f} ms/step, {value:.1f} Mcell-steps/s, divmax {r['divmax']:.2e}, "
+        f"killed {r['killed']}")
+    if r["rows"]:
+        log(f"  instrumented pass: {1e3 * r['inst'] / args.steps:.3f} ms/step")
+        log_rows(r["rows"])
     return {"value": round(value, 2), "ms_per_step": round(1e3 * s_per_step, 4),
             "speedup_vs_fp64": round(fp64_s_per_step / s_per_step, 3), "refinements": refine,
-            "divmax": float(sc[7]),
+            "divmax": r["divmax"],
             "dtype": ("f32 phase field, normals, curvature, fluxes and AB2 histories; f64 velocity, pressure, "
                       "momentum arithmetic and the whole projection" if refine < 0 else
                       "f32 phase field, normals, curvature, fluxes, AB2 histories and transforms; "
                       "f64 velocity, pressure, momentum arithmetic, pivots, residuals and sums")}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -372,10 +480,10 @@ def main():
     ap.add_argument("--arrival-ms", type=float, default=50.0,
                     help="c4: mean gap between arrivals (0: every request waiting at the start)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-prof", action="store_true", help="time without per-launch events")
+    ap.add_argument("--no-prof", action="store_true", help="skip the instrumented pass (per-launch events)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run of the same workload")
     ap.add_argument("--refine", type=int, default=1, help="mixed precision: FP64-residual refinements per solve")
-    ap.add_argument("--graphs", action="store_true", help="replay the steps from captured HIP graphs")
+    ap.add_argument("--no-graphs", action="store_true", help="time the steps launched directly, not from graphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="c4: precision of the cases")
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
@@ -403,37 +511,20 @@ def main():
     n = spec["ng"]
     cells = n[0] * n[1] * n[2]
     log(f"workload {args.workload}: {n[0]}x{n[1]}x{n[2]} = {cells} cells, warmup {args.warmup}, steps {args.steps}")
-    dc = F.DeviceCase(spec)
-    dc.init()
-    tuning = None
-    if not args.no_autotune:
-        tuning = dc.autotune(3)
-        log(f"autotune: {tuning}")
-    if args.graphs:
-        F.call("fz_case_set_graphs", dc._h, 1)
-    if args.warmup:
-        dc.step(args.warmup)
-    dc.sync()
-    F.call("fz_prof_enable", dc.ctx, 0 if args.no_prof else 1)
-    prof_report(F, dc.ctx)  # clear
-    t0 = time.perf_counter()
-    killed = dc.step(args.steps)
-    dc.sync()
-    t1 = time.perf_counter()
-    F.call("fz_prof_enable", dc.ctx, 0)
-    rows = {} if args.no_prof else prof_report(F, dc.ctx)
-    sc = dc.scalars()
-    elapsed = t1 - t0
+    r = run_case_timed(F, spec, args, "fp64")
+    elapsed, rows = r["elapsed"], r["rows"]
     ms_per_step = 1e3 * elapsed / args.steps
     value = cells * args.steps / elapsed / 1e6
-    log(f"{ms_per_step:.3f} ms/step, {value:.1f} Mcell-steps/s, divmax {sc[7]:.2e}, killed {killed}")
+    log(f"{ms_per_step:.3f} ms/step ({'graphs' if not args.no_graphs else 'direct launches'}, no per-launch events), "
+        f"{value:.1f} Mcell-steps/s, divmax {r['divmax']:.2e}, killed {r['killed']}")
 
     roof = None
     if rows:
+        inst_ms = 1e3 * r["inst"] / args.steps
+        log(f"instrumented pass (direct launches, HIP events around each): {inst_ms:.3f} ms/step")
+        log_rows(rows, 18)
         total_ms = sum(ms for _, ms in rows.values())
         ranked = sorted(rows.items(), key=lambda kv: -kv[1][1])
-        for name, (calls, ms) in ranked[:12]:
-            log(f"  {name:18s} {calls:5d} launches {ms / calls * 1e3:9.1f} us/launch {100 * ms / total_ms:5.1f}%")
         dom = next(name for name, _ in ranked if name in ALG_BYTES)
         calls, ms = rows[dom]
         units = (n[0] + 2) * (n[1] + 2) * (n[2] + 2) if dom in HALO_KERNELS else cells
@@ -444,12 +535,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom, args.workload),
                 "kernel": dom, "symbols": list(KERNEL_SYMBOLS.get(dom, ("k_" + dom,))),
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": bytes_per_launch,
-                "kernel_time_share": round(ms / total_ms, 4)}
+                "kernel_time_share": round(ms / total_ms, 4), "instrumented_ms_per_step": round(inst_ms, 4)}
     step_bw = STEP_ALG_BYTES_PER_CELL * cells / (elapsed / args.steps) / 1e9
-    copy_gbs = copy_bandwidth(F, dc.ctx, 8 * dc_count(dc))
-    log(f"plain copy kernel over one field ({8 * dc_count(dc) / 1e6:.0f} MB): {copy_gbs:.0f} GB/s")
+    log(f"plain copy kernel over one field ({8 * (n[0] + 2) * (n[1] + 2) * (n[2] + 2) / 1e6:.0f} MB): "
+        f"{r['copy_gbs']:.0f} GB/s")
 
-    dc.close()
     mixed = None
     if not args.no_mixed:
         mixed = run_mixed(F, spec, args, cells, elapsed / args.steps)
@@ -460,6 +550,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec)
         cpu["one_core"] = cpu_baseline_one_core(args.workload)
+        cpu["cpu_model"] = cpu_model()
     out = {
         "metric": "Mcell-steps/s per RK3 step, FP64 vs mixed; ensemble p50/p99 step latency",
         "value": round(value, 2),
@@ -478,12 +569,14 @@ def main():
         "config": {"workload": WORKLOAD_NAMES[args.workload], "interface": "THINC/QQ" if spec.get("vof") else "ACDI",
                    "grid": n, "density_ratio": spec["rho12"][0] / spec["rho12"][1],
                    "viscosity_ratio": spec["mu12"][0] / spec["mu12"][1], "precision": "fp64",
-                   "step": "AB2 (one reference time step)", "launch_parameters": tuning},
+                   "step": "AB2 (one reference time step)",
+                   "launch": "captured step graphs" if not args.no_graphs else "direct launches",
+                   "launch_parameters": r["tuning"]},
         "roofline": roof,
         "step_roofline": {"alg_bytes_per_cell": STEP_ALG_BYTES_PER_CELL, "achieved_GBs": round(step_bw, 1),
                           "frac": round(step_bw / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
-        "hbm_copy_GBs": round(copy_gbs, 1),
+        "hbm_copy_GBs": round(r["copy_gbs"], 1),
         "mixed": mixed,
     }
     print(json.dumps(out), flush=True)

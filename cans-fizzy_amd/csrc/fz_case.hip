@@ -50,9 +50,12 @@ int fz_bulk_mean_dev(fz_ctx *c, int idir, const PT *psi, const double p12[2], co
 int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi);
 int fz_vof_phi_f32(fz_ctx *c, double beta, const float *psi, float *phi);
 template <typename T>
-int fz_launch_vof_step(fz_ctx *c, double beta, double factor1, double factor2, int first, const double *u,
-                       const double *v, const double *w, const T *normx, const T *normy, const T *normz,
-                       const T *phi, const T *psio, T *dpsio, T *psi_out, T *flux_x, T *flux_y, T *flux_z);
+int fz_launch_vof_step(fz_ctx *c, double beta, double factor1, double factor2, const double *stc, int first,
+                       const double *u, const double *v, const double *w, const T *normx, const T *normy,
+                       const T *normz, const T *phi, const T *psio, T *dpsio, T *psi_out, T *flux_x, T *flux_y,
+                       T *flux_z);
+int fz_launch_forcing(fz_ctx *c, int ftype, double alpha, double dt, const double *stc, double *u, double *v,
+                      double *w);
 int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float *normz, float *kappa);
 int fz_boundp_f32(fz_ctx *c, const char cbc[6], const double bc[6], float *p);
 int fz_bounduvw_f32(fz_ctx *c, const char cbc[18], const double bc[18], int is_correc, float *u, float *v, float *w);
@@ -463,7 +466,8 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
   swap_ptr(cs->psio, f[F_PSI]);
   swap_ptr(cs->psio32, g[F_PSI]);
   if (mixed && P.vof) {  // the VOF sequence below on the FP32 phase field
-    FZ_TRY(fz_launch_vof_step<float>(c, 1.0 / (2.0 * P.psi_thickness_factor), factor1, factor2, is_first ? 1 : 0,
+    FZ_TRY(fz_launch_vof_step<float>(c, 1.0 / (2.0 * P.psi_thickness_factor), factor1, factor2, a.stc,
+                                     is_first ? 1 : 0,
                                      f[F_U], f[F_V], f[F_W], g[F_NX], g[F_NY], g[F_NZ], g[F_PHI], cs->psio32,
                                      g[F_DPSIO], g[F_PSI], g[F_FX], g[F_FY], g[F_FZ]));
     FZ_TRY(bc_fluxes_psi<float>(c, P, g[F_FX], g[F_FY], g[F_FZ], g[F_PSI]));
@@ -483,7 +487,7 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
                                  f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ]));
     } else {
       // vof_thinc_transport_psi, the AB2 update and clip in one pass (src/rk.f90:226-252)
-      FZ_TRY(fz_launch_vof_step<double>(c, 1.0 / (2.0 * P.psi_thickness_factor), factor1, factor2,
+      FZ_TRY(fz_launch_vof_step<double>(c, 1.0 / (2.0 * P.psi_thickness_factor), factor1, factor2, a.stc,
                                         is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ],
                                         f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ]));
     }
@@ -522,7 +526,7 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
     swap_ptr(f[F_U], cs->un);
     swap_ptr(f[F_V], cs->vn);
     swap_ptr(f[F_W], cs->wn);
-    if (P.is_forced_hit) FZ_TRY(fz_lscale_forcing(c, 2, 0.5, dt, f[F_U], f[F_V], f[F_W]));
+    if (P.is_forced_hit) FZ_TRY(fz_launch_forcing(c, 2, 0.5, dt, a.stc, f[F_U], f[F_V], f[F_W]));
     FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
     // projection on the correction buffer dp, src/main.f90:488-502
     // (nref = -1: the mixed step with the FP64 projection of the plain step)
@@ -556,14 +560,15 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
 
 // Steps after the first replay a captured graph of enqueue_step when the
 // case qualifies: every kernel argument that changes between steps is read
-// from device memory (gamma, and dt with its AB2 factors), the step has no
-// host synchronisation and no memory-copy nodes, and per-launch profiling
-// is off. The pointer rotations give two graphs, one per parity, keyed by
-// the buffers that hold p and psi when the step starts.
+// from device memory (gamma, and dt with its AB2 factors, also in the
+// THINC/QQ step and the forcing), the step has no host synchronisation and
+// no memory-copy nodes, and per-launch profiling is off. The pointer
+// rotations give two graphs, one per parity, keyed by the buffers that
+// hold p and psi when the step starts.
 static bool graph_ok(const fz_case *cs) {
   const fz_params &P = cs->prm;
   if (!cs->use_graphs || cs->ctx->prof_on) return false;
-  if (!P.is_track_interface || P.vof || !P.is_solve_ns || P.is_forced_hit) return false;
+  if (!P.is_track_interface || !P.is_solve_ns) return false;
   for (int d = 0; d < 3; d++)
     if (fabs(P.gacc[d]) > 0.0 && cs->ctx->per[d]) return false;  // bulk mean read back by the host
   return fz_projection_device_scalars_ok(cs->ctx, cs->precision == FZ_PREC_MIXED && cs->nref >= 0) != 0;

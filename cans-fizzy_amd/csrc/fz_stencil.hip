@@ -811,8 +811,9 @@ __device__ __forceinline__ T vof_face_flux(const fz_dims &d, int dir, long f, T 
 // boundary update that follows, which sets every halo face.
 template <typename T>
 __global__ void __launch_bounds__(256) k_vof_step(fz_dims d, const double *__restrict__ dzfi, double dxi_d,
-                                                  double dyi_d, double beta_d, double factor1, double factor2,
-                                                  int first, const double *__restrict__ u,
+                                                  double dyi_d, double beta_d, double factor1_v, double factor2_v,
+                                                  const double *__restrict__ stc, int first,
+                                                  const double *__restrict__ u,
                                                   const double *__restrict__ v, const double *__restrict__ w,
                                                   const T *__restrict__ normx, const T *__restrict__ normy,
                                                   const T *__restrict__ normz, const T *__restrict__ phi,
@@ -820,6 +821,7 @@ __global__ void __launch_bounds__(256) k_vof_step(fz_dims d, const double *__res
                                                   T *__restrict__ psi_out, T *__restrict__ flux_x,
                                                   T *__restrict__ flux_y, T *__restrict__ flux_z) {
   CELL_IJK(d);
+  const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
   const T beta = (T)beta_d, dxi = (T)dxi_d, dyi = (T)dyi_d;
   const T fxp = vof_face_flux<T>(d, 0, q, beta, u, normx, phi), fxm = vof_face_flux<T>(d, 0, q - 1, beta, u, normx, phi);
   const T fyp = vof_face_flux<T>(d, 1, q, beta, v, normy, phi),
@@ -837,22 +839,23 @@ __global__ void __launch_bounds__(256) k_vof_step(fz_dims d, const double *__res
 }
 
 template <typename T>
-int fz_launch_vof_step(fz_ctx *c, double beta, double factor1, double factor2, int first, const double *u,
-                       const double *v, const double *w, const T *normx, const T *normy, const T *normz,
-                       const T *phi, const T *psio, T *dpsio, T *psi_out, T *flux_x, T *flux_y, T *flux_z) {
+int fz_launch_vof_step(fz_ctx *c, double beta, double factor1, double factor2, const double *stc, int first,
+                       const double *u, const double *v, const double *w, const T *normx, const T *normy,
+                       const T *normz, const T *phi, const T *psio, T *dpsio, T *psi_out, T *flux_x, T *flux_y,
+                       T *flux_z) {
   FZ_LAUNCH(c, sizeof(T) == 4 ? "vof_step_f32" : "vof_step", k_vof_step<T>, fz_grid_interior(c->d), fz_block3(), 0,
-            c->d, c->dzfi, c->dli[0], c->dli[1], beta, factor1, factor2, first, u, v, w, normx, normy, normz, phi,
-            psio, dpsio, psi_out, flux_x, flux_y, flux_z);
+            c->d, c->dzfi, c->dli[0], c->dli[1], beta, factor1, factor2, stc, first, u, v, w, normx, normy, normz,
+            phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
   FZ_LAUNCHED();
   return FZ_OK;
 }
-template int fz_launch_vof_step<double>(fz_ctx *, double, double, double, int, const double *, const double *,
+template int fz_launch_vof_step<double>(fz_ctx *, double, double, double, const double *, int, const double *,
                                         const double *, const double *, const double *, const double *,
-                                        const double *, const double *, double *, double *, double *, double *,
-                                        double *);
-template int fz_launch_vof_step<float>(fz_ctx *, double, double, double, int, const double *, const double *,
-                                       const double *, const float *, const float *, const float *, const float *,
-                                       const float *, float *, float *, float *, float *, float *);
+                                        const double *, const double *, const double *, double *, double *, double *,
+                                        double *, double *);
+template int fz_launch_vof_step<float>(fz_ctx *, double, double, double, const double *, int, const double *,
+                                       const double *, const double *, const float *, const float *, const float *,
+                                       const float *, const float *, float *, float *, float *, float *, float *);
 
 // flux divergence, src/vof_thinc_qq.f90:123-132
 template <typename T>
@@ -1494,11 +1497,12 @@ __global__ void k_rk_psi_ab2(fz_dims d, double factor1, double factor2, const do
 }
 
 // lscale_forcing, src/forcing.f90:27-63
-__global__ void k_forcing(fz_dims d, int ftype, double f0, double factor, double l1, double l2, double l3, double dl1,
-                          double dl2, const double *__restrict__ zc, double *__restrict__ u, double *__restrict__ v,
-                          double *__restrict__ w) {
+__global__ void k_forcing(fz_dims d, int ftype, double f0, double factor_v, const double *__restrict__ stc, double l1,
+                          double l2, double l3, double dl1, double dl2, const double *__restrict__ zc,
+                          double *__restrict__ u, double *__restrict__ v, double *__restrict__ w) {
   CELL_IJK(d);
   (void)r;
+  const double factor = stc ? stc[FZ_STC_DT] : factor_v;  // the forcing is scaled by dt (src/main.f90:485)
   const double k0 = 2.0;
   const double xxc = (double)((float)i - 0.5f) * dl1 / l1;
   const double yyc = (double)((float)j - 0.5f) * dl2 / l2;
@@ -1729,13 +1733,19 @@ extern "C" int fz_mom_xyz_oth(fz_ctx *c, double dt_r, const double rho12[2], con
   return FZ_OK;
 }
 
-extern "C" int fz_lscale_forcing(fz_ctx *c, int ftype, double alpha, double dt, double *u, double *v, double *w) {
-  if (!c || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_lscale_forcing: null argument");
+// lscale_forcing with dt from device memory when stc is given (captured steps)
+int fz_launch_forcing(fz_ctx *c, int ftype, double alpha, double dt, const double *stc, double *u, double *v,
+                      double *w) {
   const double f0 = alpha * ((2.0 * FZ_PI / c->l[2]) * (2.0 * FZ_PI / c->l[2]));
-  FZ_LAUNCH(c, "forcing", k_forcing, fz_grid_interior(c->d), fz_block3(), 0, c->d, ftype, f0, dt, c->l[0], c->l[1],
-                     c->l[2], c->dl[0], c->dl[1], c->zc, u, v, w);
+  FZ_LAUNCH(c, "forcing", k_forcing, fz_grid_interior(c->d), fz_block3(), 0, c->d, ftype, f0, dt, stc, c->l[0],
+            c->l[1], c->l[2], c->dl[0], c->dl[1], c->zc, u, v, w);
   FZ_LAUNCHED();
   return FZ_OK;
+}
+
+extern "C" int fz_lscale_forcing(fz_ctx *c, int ftype, double alpha, double dt, double *u, double *v, double *w) {
+  if (!c || !u || !v || !w) return fz_set_error(FZ_ERR_ARG, "fz_lscale_forcing: null argument");
+  return fz_launch_forcing(c, ftype, alpha, dt, nullptr, u, v, w);
 }
 
 // rk_2fl / rk AB2 kernels for the case driver and fz_rk / fz_rk_2fl

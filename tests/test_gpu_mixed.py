@@ -186,7 +186,11 @@ def test_mixed_case_against_fp64(name, spec):
 GCASES = [("fp64", cases.rising_bubble_3d(ng=(32, 32, 64))),
           ("mixed", dict(cases.rising_bubble_3d(ng=(32, 32, 64)), precision="mixed", refine=1)),
           ("mixed-fp64-projection", dict(cases.rising_bubble_3d(ng=(32, 32, 64)), precision="mixed", refine=-1)),
-          ("hysing2d", cases.hysing_2d())]
+          ("hysing2d", cases.hysing_2d()),
+          # THINC/QQ with the Taylor-Green forcing (dt read from device memory)
+          ("vof-forced", cases.vof_droplets(ng=(32, 32, 32), ndrop=3, r=0.6)),
+          ("vof-forced-mixed", dict(cases.vof_droplets(ng=(32, 32, 32), ndrop=3, r=0.6), precision="mixed",
+                                    refine=1))]
 
 
 @pytest.mark.parametrize("name,spec", GCASES, ids=[g[0] for g in GCASES])

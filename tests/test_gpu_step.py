@@ -91,13 +91,20 @@ def test_per_step_parity(name):
             for k, v in errs.items():
                 worst[k] = max(worst.get(k, 0.0), v)
             ds, os_ = dc.scalars(), oc.scalars()
-            for k in ("time", "dt", "dto", "gam", "istep", "divmax"):
+            for k in ("time", "dt", "dto", "gam", "istep"):
                 i = O.SCALAR_IDS[k]
                 assert abs(ds[i] - os_[i]) <= 1e-12 * max(abs(os_[i]), 1e-300), (step, k, ds[i], os_[i])
-            # chkdiv's sum differs from the oracle's in summation order only
+            # chkdiv (src/chkdiv.f90:15-49): the divergence is the round-off left
+            # by cancelling terms of size |u|/dx, so its maximum and sum are
+            # held to the field bar relative to that scale (and divtot's sum
+            # differs from the oracle's in summation order besides)
+            vmax = max(np.abs(dc.get(c)).max() for c in ("u", "v", "w"))
+            scale = vmax * max(spec["ng"][d] / spec["l"][d] for d in range(3))  # uniform grids here
             ncell = int(np.prod(spec["ng"]))
             i, m = O.SCALAR_IDS["divtot"], O.SCALAR_IDS["divmax"]
-            assert abs(ds[i] - os_[i]) <= 1e-13 * ncell * max(os_[m], 1e-300), (step, ds[i], os_[i])
+            assert abs(ds[m] - os_[m]) <= TOL * scale, (step, "divmax", ds[m], os_[m])
+            assert abs(ds[i] - os_[i]) <= TOL * scale * ncell ** 0.5 + 1e-13 * ncell * os_[m], \
+                (step, "divtot", ds[i], os_[i])
         print(name, {k: f"{v:.1e}" for k, v in worst.items()})
         assert max(worst.values()) <= TOL, worst
     finally:
