@@ -130,6 +130,11 @@ extern "C" int fz_ctx_create(fz_ctx **out, const int n[3], const double l[3], co
     fz_ctx_destroy(c);
     return fz_set_error(FZ_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
   }
+  const int rs = fz_scratch_setup(c);
+  if (rs != FZ_OK) {
+    fz_ctx_destroy(c);
+    return rs;
+  }
   *out = c;
   return FZ_OK;
 }

@@ -94,6 +94,7 @@ struct fz_ctx {
   double *red = nullptr;      // per-block partial results of reductions
   size_t red_cap = 0;
   double *h_small = nullptr;  // pinned host landing zone for reduction results
+  double *scratch = nullptr;  // device address of this GPU's static reduction scratch (fz_scratch_setup)
   std::map<std::string, struct fz_bcmap *> bcmaps;  // composed boundary maps by signature
   // launch parameters (defaults, environment overrides, or fz_case_autotune)
   int mom_variant = 0;  // momentum kernel: 0 by precision (8 for FP64, 2 for FP32 phase fields), 1 staged
@@ -166,6 +167,7 @@ static inline dim3 fz_grid_halo(const fz_dims &d) {
 
 // internal entry points shared between translation units
 int fz_reduce_ensure(fz_ctx *ctx, size_t nblocks);
+int fz_scratch_setup(fz_ctx *ctx);
 void fz_bcmaps_free(fz_ctx *ctx);
 int fz_poisson_destroy(fz_poisson *p);
 int fz_solver_dev(fz_ctx *ctx, double *p);

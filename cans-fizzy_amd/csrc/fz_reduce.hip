@@ -6,6 +6,7 @@
 // fixed, so results do not depend on timing. Results stay on the device for
 // the case driver; the reference-style entry points copy them to the host.
 #include <cmath>
+#include <mutex>
 
 #include "fz_internal.h"
 
@@ -235,6 +236,55 @@ extern "C" int fz_chkdiv(fz_ctx *c, const double *u, const double *v, const doub
   FZ_TRY(fetch(c, res, 2, o));
   *divtot = o[0];
   *divmax = o[1];
+  return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// chkdiv without allocation. The per-block partials go to a file-scope device
+// array, `scratch`, instead of the context's buffer, and the two results to a
+// device buffer of the caller, so the call neither allocates nor synchronises.
+// Each GPU holds its own copy of the array until the process ends; every call
+// on one GPU shares it, so calls that overlap on one GPU race. Each call
+// writes the partials it reads before reading them, so nothing needs clearing.
+#define FZ_SCRATCH_BLOCKS RED_BLOCKS
+#define FZ_SCRATCH_DEVICES 64
+static __device__ __align__(256) double scratch[2 * FZ_SCRATCH_BLOCKS];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[FZ_SCRATCH_DEVICES];
+
+// Called by fz_ctx_create with the context's device current: looks up, once
+// per device, the device address of that device's `scratch` (in host code the
+// array's name is no device address) and gives it to the context.
+int fz_scratch_setup(fz_ctx *c) {
+  int dev = 0;
+  FZ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= FZ_SCRATCH_DEVICES)
+    return fz_set_error(FZ_ERR_UNSUPPORTED, "device %d is beyond the %d devices of the scratch table", dev,
+                        FZ_SCRATCH_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fz_set_error(FZ_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  c->scratch = g_scratch_ptr[dev];
+  return FZ_OK;
+}
+
+extern "C" int fz_chkdiv_noalloc(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out) {
+  if (!c || !u || !v || !w || !d_out) return fz_set_error(FZ_ERR_ARG, "fz_chkdiv_noalloc: null argument");
+  if (!c->scratch) return fz_set_error(FZ_ERR_STATE, "fz_chkdiv_noalloc: the context has no scratch address");
+  const int nb = red_blocks(c);  // the block count of fz_chkdiv, so the summation order is the same
+  if (nb > FZ_SCRATCH_BLOCKS)
+    return fz_set_error(FZ_ERR_ARG, "fz_chkdiv_noalloc: %d partial sums exceed the scratch maximum of %d", nb,
+                        FZ_SCRATCH_BLOCKS);
+  hipLaunchKernelGGL(k_red_div, dim3(nb), dim3(RED_THREADS), 0, c->st, c->d, c->dzfi, c->dli[0], c->dli[1], u, v, w,
+                     c->scratch);
+  FZ_LAUNCHED();
+  hipLaunchKernelGGL(k_fin_div, dim3(1), dim3(RED_THREADS), 0, c->st, nb, c->scratch, d_out);
+  FZ_LAUNCHED();
   return FZ_OK;
 }
 

@@ -202,6 +202,15 @@ int fz_rk_2fl(fz_ctx *ctx, fz_rk_state *st, const double rkpar[2], double dt, do
 /* ---- checks ---- */
 /* chkdiv, src/chkdiv.f90:15-49 (synchronises) */
 int fz_chkdiv(fz_ctx *ctx, const double *u, const double *v, const double *w, double *divtot, double *divmax);
+/* chkdiv into device memory without allocating: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops). The same kernels and summation order as
+ * fz_chkdiv; d_out[0] = divtot, d_out[1] = divmax, written asynchronously on
+ * the context's stream, with no synchronisation. The per-block partials live
+ * in a static device array of each GPU (1024 blocks x 2 doubles, 16 KiB,
+ * held until the process ends) that every call on that GPU shares: calls
+ * that overlap on one GPU race. */
+int fz_chkdiv_noalloc(fz_ctx *ctx, const double *u, const double *v, const double *w, double *d_out);
 /* chkdt, src/chkdt.f90:15-94 (synchronises) */
 int fz_chkdt(fz_ctx *ctx, int is_solve_ns, int is_track_interface, int vof, const double mu12[2],
              const double rho12[2], double sigma, const double gacc[3], const double *u, const double *v,
