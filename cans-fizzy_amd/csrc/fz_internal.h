@@ -102,11 +102,12 @@ struct fz_ctx {
   int tk64 = 16;        // rows per chunk of the FP64 Thomas kernel: 8, 16 or 32
   int tk32 = -8;        // the same for the FP32 one; negative: two columns per lane, -8 or -16 (the default,
                         // -8, is the tuner's choice at 256x256x512 and 256^3)
-  int acdi_variant = 0;  // ACDI step: 0 by precision (1 for FP32, 2 for FP64), 1 fields staged in LDS
+  int acdi_variant = 0;  // ACDI step: 0 the default (1), 1 fields staged in LDS
                          // (k_acdi_lds), 2 one cell per thread
   int correc_kz = 1;    // correction step: 0 one cell per thread, 1 dp, u, v staged in LDS (k_correc_lds),
                         // 8 or 16 cells per thread marching in z (k_correc_step_mz)
-  int thomas_pre = 0;   // 1: Thomas solves with the pivots made once at set-up (k_thomas_pre)
+  int thomas_pre = 0;   // 1: Thomas solves with the pivots made once at set-up (k_thomas_pre); set at
+                        // creation (1 unless thomas_rc), cleared for cases whose solves are FP32
   int thomas_rc = 0;    // 1: Thomas kernels without a pivot array (the elimination re-run per chunk);
                         // set at creation from the column count
 };

@@ -45,12 +45,16 @@ static double alloc_bytes(double b, double count) {
 }
 
 // bytes of device memory fz_case_create_ex + fz_case_init allocate for prm
-extern "C" int fz_case_footprint_ex(const fz_params *prm, int precision, size_t *bytes) {
+// nref: the refinements of a mixed case (-1: its projection is FP64)
+static int footprint(const fz_params *prm, int precision, int nref, size_t *bytes) {
   if (!prm || !bytes) return fz_set_error(FZ_ERR_ARG, "fz_case_footprint: null argument");
   const double n1 = prm->ng[0], n2 = prm->ng[1], n3 = prm->ng[2];
   const double nh = (n1 + 2) * (n2 + 2) * (n3 + 2), ni = n1 * n2 * n3;
   const bool periodic_z = prm->cbcpre[4] == 'P' && prm->cbcpre[5] == 'P';
   double b = 0.0;
+  // FP64 solves of a non-periodic z below 512x512 columns keep pivots made at
+  // set-up: two more N-element arrays (the context's default)
+  if (!periodic_z && n1 * n2 < 262144.0 && (precision != FZ_PREC_MIXED || nref < 0)) b += alloc_bytes(8.0 * ni, 2);
   if (precision == FZ_PREC_MIXED) {
     b += alloc_bytes(8.0 * nh, 5 + 4);  // u, v, w, p, po + dp, three spare velocities
     b += alloc_bytes(4.0 * nh, 9 + 1);  // psi, phi, kappa, normals, fluxes + psio (FP32)
@@ -69,6 +73,10 @@ extern "C" int fz_case_footprint_ex(const fz_params *prm, int precision, size_t 
   b += 4.0 * 1024 * 1024;    // grids, plans, scalars, allocation rounding
   *bytes = (size_t)b;
   return FZ_OK;
+}
+
+extern "C" int fz_case_footprint_ex(const fz_params *prm, int precision, size_t *bytes) {
+  return footprint(prm, precision, 1, bytes);  // a mixed case's default refinement count
 }
 
 extern "C" int fz_case_footprint(const fz_params *prm, size_t *bytes) {
@@ -233,7 +241,7 @@ extern "C" int fz_server_submit_ex(fz_server *s, const fz_params *prm, int nstep
   q.nsteps = nsteps;
   q.precision = precision;
   q.nref = nref;
-  FZ_TRY(fz_case_footprint_ex(prm, precision, &q.bytes));
+  FZ_TRY(footprint(prm, precision, nref, &q.bytes));
   if (q.bytes > s->budget)
     return fz_set_error(FZ_ERR_ARG, "fz_server_submit: the case needs %zu bytes, more than the whole budget %zu",
                         q.bytes, s->budget);

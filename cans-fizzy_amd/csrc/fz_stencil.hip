@@ -707,10 +707,9 @@ int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps,
                         const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
                         T *psi_out, T *flux_x, T *flux_y, T *flux_z) {
   FZ_TRY(fits_buffers(c, "acdi_step"));
-  // 0: by precision, measured at 256x256x512: LDS staging for FP32 (619
-  // against 663 us), the one-cell kernel for FP64 (1230 against 1471 us: the
-  // staged kernel needs 182 VGPRs, two waves per SIMD, and is VALU-bound)
-  const int variant = c->acdi_variant ? c->acdi_variant : (sizeof(T) == 4 ? 1 : 2);
+  // 0: the LDS-staged kernel, measured at 256x256x512: FP32 619 against
+  // 663 us, FP64 1047 against 1120 us (140 VGPRs, three waves per SIMD)
+  const int variant = c->acdi_variant ? c->acdi_variant : 1;
   if (variant == 1) {  // fields staged in LDS
     constexpr int KZ = 16;
     const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
@@ -875,12 +874,61 @@ __global__ void k_clip(long nh, double rmin, double rmax, T *__restrict__ p) {
     p[q] = fmin(fmax((T)rmin, p[q]), (T)rmax);
 }
 
+// sqrt(s) by the device's correctly rounded FP64 sequence without its range
+// scaling: the same result for s = 0 and s in [2^-767, 2^1023]
+__device__ inline double sqrt_inrange(double s) {
+  const double y0 = __builtin_amdgcn_rsq(s);
+  double g = s * y0, h = y0 * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double e = __builtin_fma(-g, g, s);
+  g = __builtin_fma(e, h, g);
+  e = __builtin_fma(-g, g, s);
+  g = __builtin_fma(e, h, g);
+  return s == 0.0 ? s : g;
+}
+
+// (x, y, z) / (sqrt(x*x + y*y + z*z) + eps), src/two_fluid.f90:140-162.
+// FAST (FP64 only): the square root as sqrt_inrange and the three divisions
+// with one reciprocal of the shared denominator (fzm::div_inrange's sequence
+// per numerator, then the device's division fix-up, which gives the signed
+// zero of a zero numerator). Bit for bit the plain sqrt and '/' when the sum
+// is at most 2^600 and every non-zero component is at least 2^-300 in
+// magnitude (the denominator then lies in [eps, 2^301]); ok is cleared for a
+// lane outside that range, whose caller must take the plain path.
+template <bool FAST, typename T>
+__device__ __forceinline__ void unit3(T &x, T &y, T &z, bool &ok) {
+  if constexpr (FAST && sizeof(T) == 8) {
+    const double s = x * x + y * y + z * z;
+    const int e = min(min(__builtin_amdgcn_frexp_exp(x), __builtin_amdgcn_frexp_exp(y)),
+                      __builtin_amdgcn_frexp_exp(z));
+    ok = ok && s <= 0x1p600 && e >= -299;
+    const double d = sqrt_inrange(s) + (double)FZ_EPS;
+    double w = __builtin_amdgcn_rcp(d);
+    double t = __builtin_fma(-d, w, 1.0);
+    w = __builtin_fma(w, t, w);
+    t = __builtin_fma(-d, w, 1.0);
+    w = __builtin_fma(w, t, w);
+    const double qx = x * w, qy = y * w, qz = z * w;
+    x = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qx, x), w, qx), d, x);
+    y = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qy, y), w, qy), d, y);
+    z = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qz, z), w, qz), d, z);
+  } else {
+    const T nrm = sqrt(x * x + y * y + z * z) + (T)FZ_EPS;
+    x = x / nrm;
+    y = y / nrm;
+    z = z / nrm;
+  }
+}
+
 // cmpt_norm_curv_youngs, src/two_fluid.f90:81-174, at one cell: the 27-point
 // phase field through the accessor PS (T = float: the mixed-precision step,
-// FP32 fields and arithmetic)
-template <typename T, class A>
+// FP32 fields and arithmetic). FAST: the normalisations by unit3's fast
+// sequences; ok is cleared where they do not apply.
+template <typename T, bool FAST = false, class A>
 __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dyi_d, double dzci_k, double dzci_km,
-                                            double dzfi_k, T &onx, T &ony, T &onz, T &okap) {
+                                            double dzfi_k, T &onx, T &ony, T &onz, T &okap, bool &ok) {
   const T dxi = (T)dxi_d, dyi = (T)dyi_d;
 #define P_(a, b, c) PS(a, b, c)
   const T mmm = P_(-1, -1, -1), mcm = P_(-1, 0, -1), mpm = P_(-1, 1, -1), cmm = P_(0, -1, -1), ccm = P_(0, 0, -1),
@@ -917,19 +965,14 @@ __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dy
   mz[6] = (T)0.25 * ((ccc + mcc + cpc + mpc) - (ccm + mcm + cpm + mpm)) * zcm;
   mz[7] = (T)0.25 * ((ccc + mcc + cmc + mmc) - (ccm + mcm + cmm + mmm)) * zcm;
 #pragma unroll
-  for (int m = 0; m < 8; m++) {
-    const T nrm = sqrt(mx[m] * mx[m] + my[m] * my[m] + mz[m] * mz[m]) + (T)FZ_EPS;
-    mx[m] = mx[m] / nrm;
-    my[m] = my[m] / nrm;
-    mz[m] = mz[m] / nrm;
-  }
-  const T nx = (T).125 * (mx[0] + mx[1] + mx[2] + mx[3] + mx[4] + mx[5] + mx[6] + mx[7]);
-  const T ny = (T).125 * (my[0] + my[1] + my[2] + my[3] + my[4] + my[5] + my[6] + my[7]);
-  const T nz = (T).125 * (mz[0] + mz[1] + mz[2] + mz[3] + mz[4] + mz[5] + mz[6] + mz[7]);
-  const T nrm = sqrt(nx * nx + ny * ny + nz * nz) + (T)FZ_EPS;
-  onx = nx / nrm;
-  ony = ny / nrm;
-  onz = nz / nrm;
+  for (int m = 0; m < 8; m++) unit3<FAST>(mx[m], my[m], mz[m], ok);
+  T nx = (T).125 * (mx[0] + mx[1] + mx[2] + mx[3] + mx[4] + mx[5] + mx[6] + mx[7]);
+  T ny = (T).125 * (my[0] + my[1] + my[2] + my[3] + my[4] + my[5] + my[6] + my[7]);
+  T nz = (T).125 * (mz[0] + mz[1] + mz[2] + mz[3] + mz[4] + mz[5] + mz[6] + mz[7]);
+  unit3<FAST>(nx, ny, nz, ok);
+  onx = nx;
+  ony = ny;
+  onz = nz;
   okap = -((T)0.25 * ((mx[0] + mx[1] + mx[2] + mx[3]) - (mx[4] + mx[5] + mx[6] + mx[7])) * dxi +
                (T)0.25 * ((my[0] + my[2] + my[4] + my[6]) - (my[1] + my[3] + my[5] + my[7])) * dyi +
                (T)0.25 * ((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7])) * (T)dzfi_k);
@@ -944,7 +987,14 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
   CELL_IJK(d);
   (void)r;
   T nx, ny, nz, ka;
-  youngs_cell<T>(buf_at(psi, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka);
+  bool ok = true;
+  youngs_cell<T, true>(buf_at(psi, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
+  if (!ok) {  // the plain sequences, from a fresh read of the phase field (an opaque copy of the
+              // pointer, so that the 27 values need not stay in registers for this rare path)
+    const T *ps = psi;
+    asm volatile("" : "+s"(ps));
+    youngs_cell<T>(buf_at(ps, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
+  }
   normx[q] = nx;
   normy[q] = ny;
   normz[q] = nz;
@@ -1827,5 +1877,39 @@ extern "C" int fz_init2fl(fz_ctx *c, const fz_params *prm, double seps, double *
     FZ_LAUNCH(c, "one_minus", k_one_minus, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, psi);
     FZ_LAUNCHED();
   }
+  return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The kernels' FP64 routines evaluated on the device, element by element, for
+// tests against the oracle's copies: fn 0 exp, 1 expm1, 2 log, 3 tanh, 4 sin,
+// 5 cos, 6 the in-range division x / y, 7 the compiler's division x / y
+__global__ void k_math_eval(int fn, long n, const double *__restrict__ x, const double *__restrict__ y,
+                            double *__restrict__ out) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double a = x[q];
+  double r;
+  switch (fn) {
+    case 0: r = fzm::exp(a); break;
+    case 1: r = fzm::expm1(a); break;
+    case 2: r = fzm::log(a); break;
+    case 3: r = fzm::tanh(a); break;
+    case 4: r = fzm::sin(a); break;
+    case 5: r = fzm::cos(a); break;
+    case 6: r = fzm::div_inrange(a, y[q]); break;
+    default: r = a / y[q]; break;
+  }
+  out[q] = r;
+}
+
+extern "C" int fz_math_eval(fz_ctx *c, int fn, long n, const double *x, const double *y, double *out) {
+  if (!c || !x || !out || n < 0 || fn < 0 || fn > 7 || (fn >= 6 && !y))
+    return fz_set_error(FZ_ERR_ARG, "fz_math_eval: bad argument");
+  if (n == 0) return FZ_OK;
+  const long nb = (n + 255) / 256;
+  if (nb > 0x7fffffffL) return fz_set_error(FZ_ERR_ARG, "fz_math_eval: n too large");
+  FZ_LAUNCH(c, "math_eval", k_math_eval, dim3((unsigned)nb), dim3(256), 0, fn, n, x, y, out);
+  FZ_LAUNCHED();
   return FZ_OK;
 }

@@ -216,6 +216,14 @@ int fz_chkdt(fz_ctx *ctx, int is_solve_ns, int is_track_interface, int vof, cons
              const double rho12[2], double sigma, const double gacc[3], const double *u, const double *v,
              const double *w, double gam, double seps, double *dtmax);
 
+/* ---- device math (tests) ---- */
+/* The library's FP64 routines evaluated on the device over n elements:
+ * fn 0 exp, 1 expm1, 2 log, 3 tanh, 4 sin, 5 cos (the routines of the ACDI,
+ * THINC/QQ and forcing kernels), 6 the in-range division x/y of the tanh,
+ * 7 the compiler's division x/y. No counterpart in the reference: for tests
+ * of the device routines against the oracle's copies (asynchronous). */
+int fz_math_eval(fz_ctx *ctx, int fn, long n, const double *x, const double *y, double *out);
+
 /* ---- field files ---- */
 /* load_one / io_field, src/load.f90:19-168: io 'w' writes the interior of the
  * halo'd device field p followed by [time, istep]; io 'r' reads such a file

@@ -141,19 +141,23 @@ static inline double om_log(double x) {
   return fma(kd, OM_LN2_HI, fma(kd, OM_LN2_LO, lm));
 }
 
+/* tanh(x) = expm1(2a) / (expm1(2a) + 2), a = |x|, evaluated for every
+ * argument the same way (no branches in the device build): expm1(2a) =
+ * 2^k expm1(r) + (2^k - 1) with 2a = k ln2 + r, |r| <= ln2/2 (k <= 64 after
+ * clamping a at 22, so both terms are exact); 1 above 22, a below 2^-28 */
 static inline double om_tanh(double x) {
-  if (x != x) return x;
   const double a = x < 0.0 ? -x : x;
-  double r;
-  if (a > 22.0) r = 1.0;
-  else if (a < 3.725290298461914e-09) r = a;
-  else if (a < 0.52) {
-    const double t = om_expm1_kernel20(2.0 * a);
-    r = t / (t + 2.0);
-  } else {
-    r = 1.0 - 2.0 / (om_exp(2.0 * a) + 1.0);
-  }
-  return x < 0.0 ? -r : r;
+  const double y = 2.0 * (a < 22.0 ? a : 22.0);
+  const double kf = floor(y * OM_INV_LN2 + 0.5);
+  const double r = fma(-kf, OM_LN2_LO, y - kf * OM_LN2_HI);
+  const double e = om_expm1_kernel(r);
+  const double p2 = om_bits_to_double((uint64_t)((int)kf + 1023) << 52); /* 2^k */
+  const double em1 = e * p2 + (p2 - 1.0);
+  const double t = em1 / (em1 + 2.0);
+  double res = a > 22.0 ? 1.0 : t;
+  res = a < 3.725290298461914e-09 ? a : res;
+  res = x < 0.0 ? -res : res;
+  return x != x ? x : res;
 }
 
 #define OM_PIO2_1 1.5707963109016418 /* pi/2 in three 26-bit pieces */
