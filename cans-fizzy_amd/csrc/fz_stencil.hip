@@ -300,9 +300,9 @@ static int fits_buffers(const fz_ctx *c, const char *who) {
 // one-cell x/y frame, so each value is read from HBM once instead of through
 // five (dp) or two (u, v) neighbour loads, whose address processing bounds
 // the one-cell-per-load kernel. Same expressions, same values.
-constexpr int COR_MX = 64, COR_MY = 4, COR_PX = COR_MX + 2, COR_PL = COR_PX * (COR_MY + 2);
-template <int KZ>
-__global__ void __launch_bounds__(COR_MX *COR_MY) k_correc_lds(fz_dims d, const double *__restrict__ dzci,
+constexpr int COR_MX = 64, COR_PX = COR_MX + 2;
+template <int KZ, int MY>
+__global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const double *__restrict__ dzci,
                                                               double factori_v, double factorj_v, double dt_v,
                                                               double dxi, double dyi, const double *__restrict__ stc,
                                                               double rhop, const double *__restrict__ dp,
@@ -313,13 +313,13 @@ __global__ void __launch_bounds__(COR_MX *COR_MY) k_correc_lds(fz_dims d, const 
                                                               double *__restrict__ vn, double *__restrict__ wn,
                                                               double *__restrict__ p_out,
                                                               unsigned long long *__restrict__ slots) {
-  constexpr int NT = COR_MX * COR_MY, PL = COR_PL, NPT = (PL + NT - 1) / NT;
+  constexpr int NT = COR_MX * MY, PL = COR_PX * (MY + 2), NPT = (PL + NT - 1) / NT;
   __shared__ double s_dp[2 * PL], s_u[2 * PL], s_v[2 * PL];
   const double dt = stc ? stc[FZ_STC_DT] : dt_v;
   const double factori = stc ? dt * dxi : factori_v, factorj = stc ? dt * dyi : factorj_v;
   FZ_BLOCK_XYZ(d, bx, by, bz);
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + COR_MX * ty;
-  const int i0 = (int)bx * COR_MX, j0 = (int)by * COR_MY;  // the range starts at the halo
+  const int i0 = (int)bx * COR_MX, j0 = (int)by * MY;  // the range starts at the halo
   const int i = i0 + tx, j = j0 + ty;
   const int k0 = (int)bz * KZ, kend = min(k0 + KZ - 1, d.n3 + 1), ktop = min(kend + 1, d.n3 + 1);
   const bool act = i <= d.n1 + 1 && j <= d.n2 + 1;
@@ -416,10 +416,15 @@ int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, 
   if (c->correc_kz == 1) {  // dp, u, v staged in LDS
     FZ_TRY(fits_buffers(c, "correc_step"));
     constexpr int KZ = 16;
-    const dim3 grid((c->d.n1 + 2 + COR_MX - 1) / COR_MX, (c->d.n2 + 2 + COR_MY - 1) / COR_MY,
-                    (c->d.n3 + 2 + KZ - 1) / KZ);
-    FZ_LAUNCH(c, "correc_step", k_correc_lds<KZ>, grid, dim3(COR_MX, COR_MY), 0, c->d, c->dzci, dt * c->dli[0],
-              dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots);
+    const int my = c->correc_my;
+    const dim3 grid((c->d.n1 + 2 + COR_MX - 1) / COR_MX, (c->d.n2 + 2 + my - 1) / my, (c->d.n3 + 2 + KZ - 1) / KZ);
+#define FZ_CORREC_LDS(MY)                                                                                          \
+  FZ_LAUNCH(c, "correc_step", (k_correc_lds<KZ, MY>), grid, dim3(COR_MX, MY), 0, c->d, c->dzci, dt * c->dli[0],     \
+            dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots)
+    if (my == 16) FZ_CORREC_LDS(16);
+    else if (my == 8) FZ_CORREC_LDS(8);
+    else FZ_CORREC_LDS(4);
+#undef FZ_CORREC_LDS
   } else if (c->correc_kz > 0) {
     dim3 grid = fz_grid_halo(c->d);
     const int kz = c->correc_kz == 8 ? 8 : 16;

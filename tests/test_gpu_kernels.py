@@ -123,6 +123,36 @@ def test_youngs_and_clip(pair, rng):
     assert np.array_equal(dx.download(), x)
 
 
+def test_youngs_outside_the_fast_range(pair, rng):
+    """The FP64 Youngs kernel normalises with unscaled square roots and one
+    shared reciprocal per corner; cells whose gradients leave that sequence's
+    range (components below 2^-300, sums above 2^600, zeros, subnormals,
+    infinities) take the plain sqrt and '/'. A phase field mixing such values
+    with ordinary ones must still give the oracle's normals and curvature bit
+    for bit (NaN where the oracle has NaN)."""
+    n = pair.n
+    phi = rand(rng, n)
+    pick = rng.random(phi.shape)
+    phi[pick < 0.08] = 0.0
+    phi[(pick >= 0.08) & (pick < 0.12)] *= 2.0 ** -320
+    phi[(pick >= 0.12) & (pick < 0.14)] = 5e-324 * rng.integers(1, 1000, phi.shape)[(pick >= 0.12) & (pick < 0.14)]
+    phi[(pick >= 0.14) & (pick < 0.17)] *= 2.0 ** 310
+    phi[(pick >= 0.17) & (pick < 0.175)] = np.inf
+    phi = np.asfortranarray(phi)
+    dphi = pair.h().upload(phi)
+    outs = [pair.h() for _ in range(4)]
+    F.call("fz_cmpt_norm_curv_youngs", pair.ctx.h, dphi.ptr, *[o.ptr for o in outs])
+    ref = [O.halo(n) for _ in range(4)]
+    with np.errstate(all="ignore"):
+        L.or_cmpt_norm_curv_youngs(I(n), D(pair.dli), P(pair.dzci), P(pair.dzfi), P(phi), *[P(r) for r in ref])
+    c = (slice(1, -1),) * 3
+    for o, r in zip(outs, ref):
+        got = o.download()[c]
+        assert np.array_equal(got, r[c], equal_nan=True)
+        keep = ~np.isnan(got)  # NaN sign bits differ between the host's and the device's arithmetic
+        assert np.array_equal(np.signbit(got[keep]), np.signbit(r[c][keep]))
+
+
 def test_vof_thinc(pair, rng):
     n = pair.n
     psi = rand(rng, n, 0.0, 1.0)
