@@ -213,3 +213,37 @@ def test_close_while_cases_are_being_set_up():
         assert srv.status(rid)[0] == 2
     finally:
         srv.close()
+
+
+def test_serving_mix_at_its_workload():
+    """The serving configuration itself: a seeded Poisson-arrival stream (mean
+    gap 50 ms) of 8 requests of the mix -- three 128^2x256 and one 64^2x128
+    3-D bubbles, two 256^2 and two 64^2 2-D ones, density ratios from 7.8 to
+    998 -- of 200 mixed-precision steps each, run through the server at their
+    arrival times. Every result equals the case run alone, bit for bit."""
+    reqs = serve.serving_requests(8, mean_interarrival=0.05, nsteps=200, precision="mixed")
+    grids = [tuple(s["ng"]) for _, s, _ in reqs]
+    assert (128, 128, 256) in grids and (256, 2, 256) in grids
+    assert max(s["rho12"][0] for _, s, _ in reqs) > 990.0
+    ref = [alone(s, n) for _, s, n in reqs]
+    free, _ = serve_memory()
+    srv = serve.Server(int(0.5 * free))
+    got = {}
+
+    def keep(rid):
+        assert srv.status(rid)[0] == serve.Server.FINISHED, srv.error(rid)
+        dc = srv.case(rid)
+        got[rid] = ({k: dc.get(k) for k in FIELDS}, dc.scalars(), len(srv.step_times(rid)))
+
+    try:
+        wall, ids = serve.run_stream(srv, reqs, release=True, on_finish=keep)
+        print("stream wall time", wall)
+        for rid, (fields, sc), (_, _, n) in zip(ids, ref, reqs):
+            gf, gs, nsteps = got[rid]
+            assert nsteps == n
+            for k in FIELDS:
+                assert np.array_equal(gf[k], fields[k]), (rid, k)
+            assert np.array_equal(gs, sc), rid
+        assert srv.memory()[0] == 0
+    finally:
+        srv.close()
