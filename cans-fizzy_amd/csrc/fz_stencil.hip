@@ -443,10 +443,24 @@ int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, 
 
 // transcendentals by phase-field type: the shared FP64 routines (identical
 // on the device and in the oracle), or the device library's FP32 ones
+// The FP32 instances (the phase field of the mixed-precision step, compared
+// with the FP64 step within stated per-field bounds) use the hardware's
+// approximate operations: v_rcp_f32, v_sqrt_f32, and exp/log through
+// v_exp_f32/v_log_f32 (base 2), each within about 1 ulp, instead of the
+// correctly rounded FP32 sequences; the FP64 instances are plain '/' and
+// sqrt (correctly rounded, as the oracle's).
+__device__ inline double fz_div(double a, double b) { return a / b; }
+__device__ inline float fz_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline double fz_sqrt(double x) { return sqrt(x); }
+__device__ inline float fz_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ inline double fz_tanh(double x) { return fzm::tanh(x); }
-__device__ inline float fz_tanh(float x) { return tanhf(x); }
+// tanh(x) = (1 - e) / (1 + e), e = exp(-2|x|) (no overflow for any x)
+__device__ inline float fz_tanh(float x) {
+  const float e = __expf(-2.0f * fabsf(x));
+  return copysignf((1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e), x);
+}
 __device__ inline double fz_log(double x) { return fzm::log(x); }
-__device__ inline float fz_log(float x) { return logf(x); }
+__device__ inline float fz_log(float x) { return __logf(x); }
 
 // ---------------------------------------------------------------------------
 // acdi_transport_pf at one cell, src/acdi.f90:102-224 (SDF-normal sharpening):
@@ -485,20 +499,20 @@ __device__ __forceinline__ void acdi_cell(int k, const double *__restrict__ dzci
   const T dpsidxm = gs * (psi_ccc - psi_mcc) * dxi, dpsidxp = gs * (psi_pcc - psi_ccc) * dxi;
   const T dpsidym = gs * (psi_ccc - psi_cmc) * dyi, dpsidyp = gs * (psi_cpc - psi_ccc) * dyi;
   const T dpsidzm = gs * (psi_ccc - psi_ccm) * dzci_m, dpsidzp = gs * (psi_ccp - psi_ccc) * dzci_c;
-  const T rn_01 = nx_xm / (sqrt(nx_xm * nx_xm + ny_xm * ny_xm + nz_xm * nz_xm) + (T)FZ_EPS);
-  const T rn_11 = nx_xp / (sqrt(nx_xp * nx_xp + ny_xp * ny_xp + nz_xp * nz_xp) + (T)FZ_EPS);
-  const T rn_02 = ny_ym / (sqrt(nx_ym * nx_ym + ny_ym * ny_ym + nz_ym * nz_ym) + (T)FZ_EPS);
-  const T rn_12 = ny_yp / (sqrt(nx_yp * nx_yp + ny_yp * ny_yp + nz_yp * nz_yp) + (T)FZ_EPS);
-  const T rn_03 = nz_zm / (sqrt(nx_zm * nx_zm + ny_zm * ny_zm + nz_zm * nz_zm) + (T)FZ_EPS);
-  const T rn_13 = nz_zp / (sqrt(nx_zp * nx_zp + ny_zp * ny_zp + nz_zp * nz_zp) + (T)FZ_EPS);
+  const T rn_01 = fz_div(nx_xm, fz_sqrt(nx_xm * nx_xm + ny_xm * ny_xm + nz_xm * nz_xm) + (T)FZ_EPS);
+  const T rn_11 = fz_div(nx_xp, fz_sqrt(nx_xp * nx_xp + ny_xp * ny_xp + nz_xp * nz_xp) + (T)FZ_EPS);
+  const T rn_02 = fz_div(ny_ym, fz_sqrt(nx_ym * nx_ym + ny_ym * ny_ym + nz_ym * nz_ym) + (T)FZ_EPS);
+  const T rn_12 = fz_div(ny_yp, fz_sqrt(nx_yp * nx_yp + ny_yp * ny_yp + nz_yp * nz_yp) + (T)FZ_EPS);
+  const T rn_03 = fz_div(nz_zm, fz_sqrt(nx_zm * nx_zm + ny_zm * ny_zm + nz_zm * nz_zm) + (T)FZ_EPS);
+  const T rn_13 = fz_div(nz_zp, fz_sqrt(nx_zp * nx_zp + ny_zp * ny_zp + nz_zp * nz_zp) + (T)FZ_EPS);
   const T g4 = (T)0.25 * gam;
   T t;
-  t = fz_tanh((T)0.25 * (phi_ccc + phi_mcc) / seps); const T sharpxm = g4 * (((T)1.0 - t * t) * rn_01);
-  t = fz_tanh((T)0.25 * (phi_pcc + phi_ccc) / seps); const T sharpxp = g4 * (((T)1.0 - t * t) * rn_11);
-  t = fz_tanh((T)0.25 * (phi_ccc + phi_cmc) / seps); const T sharpym = g4 * (((T)1.0 - t * t) * rn_02);
-  t = fz_tanh((T)0.25 * (phi_cpc + phi_ccc) / seps); const T sharpyp = g4 * (((T)1.0 - t * t) * rn_12);
-  t = fz_tanh((T)0.25 * (phi_ccc + phi_ccm) / seps); const T sharpzm = g4 * (((T)1.0 - t * t) * rn_03);
-  t = fz_tanh((T)0.25 * (phi_ccp + phi_ccc) / seps); const T sharpzp = g4 * (((T)1.0 - t * t) * rn_13);
+  t = fz_tanh(fz_div((T)0.25 * (phi_ccc + phi_mcc), seps)); const T sharpxm = g4 * (((T)1.0 - t * t) * rn_01);
+  t = fz_tanh(fz_div((T)0.25 * (phi_pcc + phi_ccc), seps)); const T sharpxp = g4 * (((T)1.0 - t * t) * rn_11);
+  t = fz_tanh(fz_div((T)0.25 * (phi_ccc + phi_cmc), seps)); const T sharpym = g4 * (((T)1.0 - t * t) * rn_02);
+  t = fz_tanh(fz_div((T)0.25 * (phi_cpc + phi_ccc), seps)); const T sharpyp = g4 * (((T)1.0 - t * t) * rn_12);
+  t = fz_tanh(fz_div((T)0.25 * (phi_ccc + phi_ccm), seps)); const T sharpzm = g4 * (((T)1.0 - t * t) * rn_03);
+  t = fz_tanh(fz_div((T)0.25 * (phi_ccp + phi_ccc), seps)); const T sharpzp = g4 * (((T)1.0 - t * t) * rn_13);
   const T rglrxm = dpsidxm - sharpxm, rglrxp = dpsidxp - sharpxp;
   const T rglrym = dpsidym - sharpym, rglryp = dpsidyp - sharpyp;
   const T rglrzm = dpsidzm - sharpzm, rglrzp = dpsidzp - sharpzp;
@@ -743,7 +757,7 @@ template <typename T>
 __global__ void k_acdi_phi(long nh, double seps_d, const T *__restrict__ psi, T *__restrict__ phi) {
   const T seps = (T)seps_d;
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
-    phi[q] = seps * fz_log((psi[q] + (T)FZ_EPS) / ((T)1.0 - psi[q] + (T)FZ_EPS));
+    phi[q] = seps * fz_log(fz_div(psi[q] + (T)FZ_EPS, (T)1.0 - psi[q] + (T)FZ_EPS));
 }
 
 // epsilon(1._rp) of the phase-field type (the reference's single-precision
@@ -753,7 +767,7 @@ __device__ __forceinline__ T fz_eps_of() {
   return sizeof(T) == 4 ? (T)1.1920928955078125e-07 : (T)FZ_EPS;
 }
 __device__ inline double fz_exp(double x) { return fzm::exp(x); }
-__device__ inline float fz_exp(float x) { return expf(x); }
+__device__ inline float fz_exp(float x) { return __expf(x); }
 
 // vof_thinc_cmpt_phi, src/vof_thinc_qq.f90:148-156 (T = float: the mixed step)
 template <typename T>
@@ -761,7 +775,7 @@ __global__ void k_vof_phi(long nh, double beta_d, const T *__restrict__ psi, T *
   const T beta = (T)beta_d, eps = fz_eps_of<T>();
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x) {
     const T pa = fmin(fmax(psi[q], eps), (T)1.0 - eps);
-    phi[q] = (T)0.5 / beta * fz_log(pa / ((T)1.0 - pa));
+    phi[q] = (T)0.5 / beta * fz_log(fz_div(pa, (T)1.0 - pa));
   }
 }
 
@@ -783,9 +797,9 @@ __global__ void k_vof_flux(fz_dims d, double beta_d, const double *__restrict__ 
   if (vel_x > (T)0.0) { qx = q; xx = (T)0.5; } else { qx = q + 1; xx = (T)-0.5; }
   if (vel_y > (T)0.0) { qy = q; yy = (T)0.5; } else { qy = q + d.sy; yy = (T)-0.5; }
   if (vel_z > (T)0.0) { qz = q; zz = (T)0.5; } else { qz = q + d.sz; zz = (T)-0.5; }
-  const T lfx = (T)1.0 / ((T)1.0 + fz_exp((T)-2.0 * beta * (normx[qx] * xx + phi[qx])));
-  const T lfy = (T)1.0 / ((T)1.0 + fz_exp((T)-2.0 * beta * (normy[qy] * yy + phi[qy])));
-  const T lfz = (T)1.0 / ((T)1.0 + fz_exp((T)-2.0 * beta * (normz[qz] * zz + phi[qz])));
+  const T lfx = fz_div((T)1.0, (T)1.0 + fz_exp((T)-2.0 * beta * (normx[qx] * xx + phi[qx])));
+  const T lfy = fz_div((T)1.0, (T)1.0 + fz_exp((T)-2.0 * beta * (normy[qy] * yy + phi[qy])));
+  const T lfz = fz_div((T)1.0, (T)1.0 + fz_exp((T)-2.0 * beta * (normz[qz] * zz + phi[qz])));
   flux_x[q] = vel_x * lfx;
   flux_y[q] = vel_y * lfy;
   flux_z[q] = vel_z * lfz;
@@ -801,7 +815,7 @@ __device__ __forceinline__ T vof_face_flux(const fz_dims &d, int dir, long f, T 
   long qq;
   T hh;
   if (vv > (T)0.0) { qq = f; hh = (T)0.5; } else { qq = f + up; hh = (T)-0.5; }
-  const T lf = (T)1.0 / ((T)1.0 + fz_exp((T)-2.0 * beta * (nrm[qq] * hh + phi[qq])));
+  const T lf = fz_div((T)1.0, (T)1.0 + fz_exp((T)-2.0 * beta * (nrm[qq] * hh + phi[qq])));
   return vv * lf;
 }
 
@@ -920,10 +934,10 @@ __device__ __forceinline__ void unit3(T &x, T &y, T &z, bool &ok) {
     y = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qy, y), w, qy), d, y);
     z = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qz, z), w, qz), d, z);
   } else {
-    const T nrm = sqrt(x * x + y * y + z * z) + (T)FZ_EPS;
-    x = x / nrm;
-    y = y / nrm;
-    z = z / nrm;
+    const T nrm = fz_sqrt(x * x + y * y + z * z) + (T)FZ_EPS;
+    x = fz_div(x, nrm);
+    y = fz_div(y, nrm);
+    z = fz_div(z, nrm);
   }
 }
 
