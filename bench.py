@@ -212,7 +212,9 @@ def main_serving(args):
     from fizzy_amd import serve
 
     reqs = serve.serving_requests(args.requests, mean_interarrival=args.arrival_ms / 1e3, nsteps=args.case_steps,
-                                  precision=args.precision)
+                                  precision=args.precision, refine=args.refine)
+    if args.no_group:
+        os.environ["FZ_SERVE_GROUP"] = "0"  # read when the server is made
     cells = [int(np.prod(spec["ng"])) * n for _, spec, n in reqs]
     f, t = C.c_size_t(), C.c_size_t()
     F.call("fz_device_memory", C.byref(f), C.byref(t))
@@ -251,6 +253,8 @@ def main_serving(args):
         "higher_is_better": False, "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "mixed",
         "data": "synthetic",
         "config": {"workload": "serving_mix", "requests": len(reqs), "steps_per_case": args.case_steps,
+                   "batched_steps": not args.no_group,
+                   "refinements": args.refine if args.precision == "mixed" else None,
                    "arrivals": f"Poisson, mean {args.arrival_ms:g} ms", "grids": "64^2, 128^2, 256^2 (2-D), 64^2x128, 128^2x256",
                    "density_ratio": "uniform [2, 1000]"},
         "cpu_baseline": cpu,
@@ -485,6 +489,8 @@ def main():
     ap.add_argument("--refine", type=int, default=1, help="mixed precision: FP64-residual refinements per solve")
     ap.add_argument("--no-graphs", action="store_true", help="time the steps launched directly, not from graphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
+    ap.add_argument("--no-group", action="store_true",
+                    help="c4: no batched steps of same-shaped cases (each case steps on its own)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="c4: precision of the cases")
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()

@@ -247,6 +247,23 @@ __global__ void k_bc_gather(GatherSet<T> g) {
   }
 }
 
+// the same for a batch of cases (fz_case_batch): blockIdx.z is the case,
+// blockIdx.y the (map, role) pair; the maps are those of the first case,
+// which every case of the batch shares (same grid and conditions)
+template <typename T>
+__global__ void k_bc_gather_b(GatherSet<T> g, fz_bref br, int r0, int r1, int r2, int r3) {
+  const unsigned y = blockIdx.y, cs = blockIdx.z;
+  const BcMapDev m = y == 0 ? g.m[0] : (y == 1 ? g.m[1] : (y == 2 ? g.m[2] : g.m[3]));
+  T *f = (T *)fz_brole(br, cs, y == 0 ? r0 : (y == 1 ? r1 : (y == 2 ? r2 : r3)));
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m.n; t += gridDim.x * blockDim.x) {
+    const BcChain &ch = m.chains[m.cid[t]];
+    const int sidx = m.src[t];
+    T v = sidx >= 0 ? f[sidx] : (T)0;
+    for (int o = 0; o < ch.nops; o++) v = (ch.mode[o] == 2) ? (T)ch.factor[o] + (T)ch.sgn[o] * v : (T)ch.factor[o];
+    f[m.dst[t]] = v;
+  }
+}
+
 namespace {
 // The cells a boundary operation can write: those with i, j or k in the
 // layers {0, n, n+1} of their direction (a staggered wall value sits on
@@ -575,6 +592,53 @@ int fz_bounds_group(fz_ctx *c, int n, const FzBcItem *items) {
 }
 template int fz_bounds_group<double>(fz_ctx *, int, const FzBcItem *);
 template int fz_bounds_group<float>(fz_ctx *, int, const FzBcItem *);
+
+// fz_bounds_group for a batch of ncase same-shaped cases: the items' fields
+// are given as roles (roles[3 q + fi] for field fi of item q), each case's
+// buffer of the role taken from the batch table. FZ_ERR_UNSUPPORTED when an
+// item has no composed map (the ordered phases are not batched).
+template <typename T>
+int fz_bounds_group_batch(fz_ctx *c, int n, const FzBcItem *items, const int *roles, fz_bref br, int ncase) {
+  std::vector<fz_bcmap *> maps;
+  std::vector<int> rl;
+  for (int q = 0; q < n; q++) {
+    const FzBcItem &it = items[q];
+    const int ncbc = it.kind == 0 ? 6 : 18;
+    if (!valid_codes(it.cbc, ncbc)) return fz_set_error(FZ_ERR_ARG, "boundary update: bad boundary code");
+    std::vector<Phase> ph;
+    if (it.kind == 0) boundp_phases(c, it.cbc, it.bc, ph);
+    else bounduvw_phases(c, it.cbc, it.bc, it.is_correc ? 1 : 0, ph);
+    const int nf = it.kind == 0 ? 1 : 3;
+    for (int fi = 0; fi < nf; fi++) {
+      fz_bcmap *m = g_gather_off ? nullptr
+                                 : get_map(c, signature(it.kind == 0 ? "p" : "uvw", it.cbc, ncbc, it.bc,
+                                                        it.is_correc ? 1 : 0, fi), ph, fi);
+      if (!m || !m->ok) return fz_set_error(FZ_ERR_UNSUPPORTED, "batched boundary update without a composed map");
+      maps.push_back(m);
+      rl.push_back(roles[3 * q + fi]);
+    }
+  }
+  for (size_t q0 = 0; q0 < maps.size(); q0 += FZ_GATHER_MAX) {
+    const int k = (int)std::min((size_t)FZ_GATHER_MAX, maps.size() - q0);
+    GatherSet<T> g;
+    int nmax = 0, r[FZ_GATHER_MAX] = {0, 0, 0, 0};
+    for (int q = 0; q < FZ_GATHER_MAX; q++) {
+      g.m[q] = devmap(q < k ? maps[q0 + q] : nullptr);
+      g.f[q] = nullptr;
+      if (q < k) {
+        nmax = std::max(nmax, maps[q0 + q]->n);
+        r[q] = rl[q0 + q];
+      }
+    }
+    if (nmax == 0) continue;
+    const int nb = std::min((nmax + 255) / 256, 4096);
+    FZ_LAUNCH(c, "bc_gather", k_bc_gather_b<T>, dim3(nb, k, ncase), dim3(256), 0, g, br, r[0], r[1], r[2], r[3]);
+    FZ_LAUNCHED();
+  }
+  return FZ_OK;
+}
+template int fz_bounds_group_batch<double>(fz_ctx *, int, const FzBcItem *, const int *, fz_bref, int);
+template int fz_bounds_group_batch<float>(fz_ctx *, int, const FzBcItem *, const int *, fz_bref, int);
 
 // FP32 fields of the mixed-precision step (same sequence and maps)
 int fz_boundp_f32(fz_ctx *c, const char cbc[6], const double bc[6], float *p) {

@@ -845,3 +845,289 @@ extern "C" int fz_rk_2fl(fz_ctx *c, fz_rk_state *s, const double rkpar[2], doubl
   swap_ptr(s->dpsi, s->dpsio);
   return FZ_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Batched steps of same-shaped cases (no reference counterpart): the step of
+// every case of the batch in one launch per stage. Each stage's kernel takes
+// its block's case from the block index (the cases' grids stacked in z) and
+// reads that case's buffers and scalars from a device table; the pointer
+// rotations of a step become a per-case state in device memory that the
+// step's last launch flips. The arithmetic is each case's own step's, so
+// every case's fields equal its unbatched run bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T>
+int fz_batch_acdi_step(fz_ctx *c, fz_bref br, int ncase);
+template <typename T>
+int fz_batch_phase_geometry(fz_ctx *c, fz_bref br, int ncase);
+template <typename PT, typename HT>
+int fz_batch_mom(fz_ctx *c, fz_bref br, int ncase);
+int fz_batch_correc_gamma(fz_ctx *c, fz_bref br, int ncase, bool gamma);
+int fz_batch_fillps(fz_ctx *c, fz_bref br, int ncase);
+int fz_batch_flip(fz_ctx *c, int *par, int ncase);
+template <typename T>
+int fz_bounds_group_batch(fz_ctx *c, int n, const FzBcItem *items, const int *roles, fz_bref br, int ncase);
+
+struct fz_case_batch {
+  std::vector<fz_case *> cs;
+  fz_bcase *dtab = nullptr;  // the role table, one entry per case
+  int *dpar = nullptr;       // each case's pointer state
+  void **dsolve = nullptr;   // the correction buffers, work arrays and pivots of the cases
+  std::vector<hipEvent_t> ev;
+  bool mixed = false;
+};
+
+// the buffer of every role in the case's current pointer state (what
+// enqueue_step hands to the kernels of the next step)
+static void case_roles(const fz_case *cs, void *at[BR_N]) {
+  const bool mixed = cs->precision == FZ_PREC_MIXED;
+  auto ph = [&](int id) -> void * { return mixed ? (void *)cs->g[id] : (void *)cs->f[id]; };
+  at[BR_U] = cs->f[F_U];
+  at[BR_V] = cs->f[F_V];
+  at[BR_W] = cs->f[F_W];
+  at[BR_UN] = cs->un;
+  at[BR_VN] = cs->vn;
+  at[BR_WN] = cs->wn;
+  at[BR_P] = cs->f[F_P];
+  at[BR_PO] = cs->f[F_PO];
+  at[BR_DP] = cs->dp;
+  at[BR_PSI_OLD] = ph(F_PSI);  // the step starts with psio <- psi by rotation
+  at[BR_PSI_NEW] = mixed ? (void *)cs->psio32 : (void *)cs->psio;
+  at[BR_PHI] = ph(F_PHI);
+  at[BR_NX] = ph(F_NX);
+  at[BR_NY] = ph(F_NY);
+  at[BR_NZ] = ph(F_NZ);
+  at[BR_KAPPA] = ph(F_KAPPA);
+  at[BR_FX] = ph(F_FX);
+  at[BR_FY] = ph(F_FY);
+  at[BR_FZ] = ph(F_FZ);
+  at[BR_DPSIO] = ph(F_DPSIO);
+  at[BR_DUO] = ph(F_DUO);
+  at[BR_DVO] = ph(F_DVO);
+  at[BR_DWO] = ph(F_DWO);
+}
+
+// why a case cannot be batched with the first one (null: it can)
+static const char *batch_refusal(const fz_case *a, const fz_case *b) {
+  const fz_params &P = b->prm, &Q = a->prm;
+  if (b->sc[S_FIRST] != 0.0) return "its first step has not been taken";
+  if (b->check_pending) return "a step has not ended";
+  if (b->ctx->prof_on) return "per-launch profiling is on";
+  if (P.vof || P.is_forced_hit) return "THINC/QQ and forced cases are not batched";
+  if (!P.is_track_interface || !P.is_solve_ns) return "it does not solve both the flow and the interface";
+  for (int d = 0; d < 3; d++)
+    if (fabs(P.gacc[d]) > 0.0 && b->ctx->per[d]) return "its bulk density mean is read back by the host";
+  if (b->precision == FZ_PREC_MIXED && b->nref >= 0) return "mixed precision with an FP32 projection is not batched";
+  if (!b->ctx->pois || b->ctx->pois->periodic_z) return "a periodic z is not batched";
+  if (!b->ctx->pois->rhsb_zero) return "non-homogeneous pressure boundary values";
+  if (!b->gam_on_device) return "gamma is not on the device";
+  if (b->precision != a->precision || b->nref != a->nref) return "another precision";
+  if (memcmp(P.ng, Q.ng, sizeof(P.ng)) || memcmp(P.l, Q.l, sizeof(P.l)) || P.gtype != Q.gtype || P.gr != Q.gr)
+    return "another grid";
+  if (memcmp(P.cbcvel, Q.cbcvel, sizeof(P.cbcvel)) || memcmp(P.bcvel, Q.bcvel, sizeof(P.bcvel)) ||
+      memcmp(P.cbcpre, Q.cbcpre, sizeof(P.cbcpre)) || memcmp(P.bcpre, Q.bcpre, sizeof(P.bcpre)) ||
+      memcmp(P.cbcpsi, Q.cbcpsi, sizeof(P.cbcpsi)) || memcmp(P.bcpsi, Q.bcpsi, sizeof(P.bcpsi)) ||
+      memcmp(P.cbcnor, Q.cbcnor, sizeof(P.cbcnor)) || memcmp(P.bcnor, Q.bcnor, sizeof(P.bcnor)) ||
+      memcmp(b->cbccur, a->cbccur, sizeof(a->cbccur)) || memcmp(b->bccur, a->bccur, sizeof(a->bccur)))
+    return "other boundary conditions";
+  return nullptr;
+}
+
+const char *fz_case_batch_refusal(const fz_case *first, const fz_case *cs) { return batch_refusal(first, cs); }
+
+extern "C" int fz_case_batch_destroy(fz_case_batch *b) {
+  if (!b) return FZ_OK;
+  if (!b->cs.empty()) (void)hipStreamSynchronize(b->cs[0]->ctx->st);
+  if (b->dtab) (void)hipFree(b->dtab);
+  if (b->dpar) (void)hipFree(b->dpar);
+  if (b->dsolve) (void)hipFree(b->dsolve);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete b;
+  return FZ_OK;
+}
+
+extern "C" int fz_case_batch_create(fz_case *const *cases, int n, fz_case_batch **out) {
+  if (!cases || n < 1 || !out) return fz_set_error(FZ_ERR_ARG, "fz_case_batch_create: bad argument");
+  if (n > 65535) return fz_set_error(FZ_ERR_ARG, "fz_case_batch_create: at most 65535 cases");
+  for (int q = 0; q < n; q++) {
+    if (!cases[q]) return fz_set_error(FZ_ERR_ARG, "fz_case_batch_create: null case %d", q);
+    for (int p = 0; p < q; p++)
+      if (cases[p] == cases[q]) return fz_set_error(FZ_ERR_ARG, "fz_case_batch_create: case %d given twice", q);
+    const char *why = batch_refusal(cases[0], cases[q]);
+    if (why) return fz_set_error(FZ_ERR_UNSUPPORTED, "fz_case_batch_create: case %d: %s", q, why);
+  }
+  fz_case_batch *b = new fz_case_batch();
+  b->cs.assign(cases, cases + n);
+  b->mixed = cases[0]->precision == FZ_PREC_MIXED;
+  std::vector<fz_bcase> tab(n);
+  std::vector<void *> sol(3 * (size_t)n);
+  for (int q = 0; q < n; q++) {
+    const fz_case *cs = cases[q];
+    const fz_params &P = cs->prm;
+    fz_bcase &e = tab[q];
+    memset(&e, 0, sizeof(e));
+    case_roles(cs, e.at[0]);
+    memcpy(e.at[1], e.at[0], sizeof(e.at[0]));  // one step later: psi and p rotated
+    std::swap(e.at[1][BR_PSI_OLD], e.at[1][BR_PSI_NEW]);
+    std::swap(e.at[1][BR_P], e.at[1][BR_PO]);
+    e.stc = cs->dstc;
+    e.dsc = cs->dsc;
+    e.gslots = cs->gslots;
+    e.rho = P.rho12[1];
+    e.drho = P.rho12[0] - P.rho12[1];
+    e.mu = P.mu12[1];
+    e.dmu = P.mu12[0] - P.mu12[1];
+    e.surf_factor = P.sigma * 2.0 / (P.rho12[0] + P.rho12[1]);
+    e.rho0 = cs->sc[S_RHO0];
+    e.seps = cs->sc[S_SEPS];
+    e.gam_factor = P.acdi_gam_factor;
+    e.gam_min = P.acdi_gam_min;
+    for (int d = 0; d < 3; d++) {
+      e.bforce[d] = P.bforce[d];
+      e.gacc[d] = P.gacc[d];
+    }
+    sol[q] = cs->dp;
+    sol[n + q] = cs->ctx->pois->work;
+    sol[2 * n + q] = cs->ctx->pois->piv;
+  }
+  int rc = FZ_OK;
+  auto hip = [&](hipError_t e, const char *what) {
+    if (rc == FZ_OK && e != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "fz_case_batch_create %s: %s", what, hipGetErrorString(e));
+  };
+  hip(hipMalloc((void **)&b->dtab, sizeof(fz_bcase) * n), "table");
+  if (rc == FZ_OK) hip(hipMalloc((void **)&b->dpar, sizeof(int) * n), "state");
+  if (rc == FZ_OK) hip(hipMalloc((void **)&b->dsolve, sizeof(void *) * 3 * n), "solve table");
+  std::vector<int> par(n, 0);
+  if (rc == FZ_OK) hip(hipMemcpy(b->dtab, tab.data(), sizeof(fz_bcase) * n, hipMemcpyHostToDevice), "upload");
+  if (rc == FZ_OK) hip(hipMemcpy(b->dpar, par.data(), sizeof(int) * n, hipMemcpyHostToDevice), "upload");
+  if (rc == FZ_OK) hip(hipMemcpy(b->dsolve, sol.data(), sizeof(void *) * 3 * n, hipMemcpyHostToDevice), "upload");
+  b->ev.assign(n, nullptr);
+  for (int q = 0; q < n && rc == FZ_OK; q++) hip(hipEventCreateWithFlags(&b->ev[q], hipEventDisableTiming), "event");
+  if (rc != FZ_OK) {
+    fz_case_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return FZ_OK;
+}
+
+// one step of every case of the batch, enqueued on the first case's stream
+static int batch_enqueue(fz_case_batch *b) {
+  const int n = (int)b->cs.size();
+  fz_case *c0s = b->cs[0];
+  fz_ctx *c0 = c0s->ctx;
+  const fz_params &P = c0s->prm;
+  const fz_bref br = {b->dtab, b->dpar, 0};
+  for (fz_case *cs : b->cs) {  // the host side of step_enqueue, per case
+    StepArgs a;
+    a.dt = cs->sc[S_DT];
+    a.first = false;
+    cs->sc[S_ISTEP] += 1.0;
+    cs->sc[S_TIME] += a.dt;
+    a.dt_r = a.dt / cs->sc[S_DTO];
+    const double tm1 = 1.0 + 0.5 * a.dt_r, tm2 = -0.5 * a.dt_r;
+    a.factor1 = tm1 * a.dt;
+    a.factor2 = tm2 * a.dt;
+    a.factor12 = a.factor1 + a.factor2;
+    a.stc = cs->dstc;
+    FZ_TRY(set_stc(cs, a));
+    cs->sc[S_DTO] = a.dt;
+  }
+  // rk_2fl with acdi_transport_pf, the boundary updates, phi and the normals
+  const FzBcItem fl[2] = {{1, P.cbcvel, P.bcvel, 0, {nullptr, nullptr, nullptr}},
+                          {0, P.cbcpsi, P.bcpsi, 0, {nullptr, nullptr, nullptr}}};
+  const int fl_roles[6] = {BR_FX, BR_FY, BR_FZ, BR_PSI_NEW, 0, 0};
+  const FzBcItem ge[4] = {{0, c0s->cbccur, c0s->bccur, 0, {nullptr, nullptr, nullptr}},
+                          {0, P.cbcnor + 0, P.bcnor + 0, 0, {nullptr, nullptr, nullptr}},
+                          {0, P.cbcnor + 6, P.bcnor + 6, 0, {nullptr, nullptr, nullptr}},
+                          {0, P.cbcnor + 12, P.bcnor + 12, 0, {nullptr, nullptr, nullptr}}};
+  const int ge_roles[12] = {BR_KAPPA, 0, 0, BR_NX, 0, 0, BR_NY, 0, 0, BR_NZ, 0, 0};
+  if (b->mixed) {
+    FZ_TRY(fz_batch_acdi_step<float>(c0, br, n));
+    FZ_TRY(fz_bounds_group_batch<float>(c0, 2, fl, fl_roles, br, n));
+    FZ_TRY(fz_batch_phase_geometry<float>(c0, br, n));
+    FZ_TRY(fz_bounds_group_batch<float>(c0, 4, ge, ge_roles, br, n));
+    FZ_TRY((fz_batch_mom<float, float>(c0, br, n)));
+  } else {
+    FZ_TRY(fz_batch_acdi_step<double>(c0, br, n));
+    FZ_TRY(fz_bounds_group_batch<double>(c0, 2, fl, fl_roles, br, n));
+    FZ_TRY(fz_batch_phase_geometry<double>(c0, br, n));
+    FZ_TRY(fz_bounds_group_batch<double>(c0, 4, ge, ge_roles, br, n));
+    FZ_TRY((fz_batch_mom<double, double>(c0, br, n)));
+  }
+  // rk's result is in the spare buffers: bounduvw, fillps, the solve into dp
+  // and boundp, then correc/updatep/gamma speeds into the first buffers
+  const FzBcItem uv[1] = {{1, P.cbcvel, P.bcvel, 0, {nullptr, nullptr, nullptr}}};
+  const int uv_roles[3] = {BR_UN, BR_VN, BR_WN};
+  FZ_TRY(fz_bounds_group_batch<double>(c0, 1, uv, uv_roles, br, n));
+  FZ_TRY(fz_batch_fillps(c0, br, n));
+  FZ_TRY(fz_batch_solve(c0, b->dsolve, n));
+  const FzBcItem pr[1] = {{0, P.cbcpre, P.bcpre, 0, {nullptr, nullptr, nullptr}}};
+  const int pr_roles[3] = {BR_DP, 0, 0};
+  FZ_TRY(fz_bounds_group_batch<double>(c0, 1, pr, pr_roles, br, n));
+  FZ_TRY(fz_batch_correc_gamma(c0, br, n, true));
+  const FzBcItem co[2] = {{1, P.cbcvel, P.bcvel, 1, {nullptr, nullptr, nullptr}},
+                          {0, P.cbcpre, P.bcpre, 0, {nullptr, nullptr, nullptr}}};
+  const int co_roles[6] = {BR_U, BR_V, BR_W, BR_PO, 0, 0};
+  FZ_TRY(fz_bounds_group_batch<double>(c0, 2, co, co_roles, br, n));
+  FZ_TRY(fz_batch_flip(c0, b->dpar, n));
+  for (fz_case *cs : b->cs) {  // the rotations the kernels applied through the flipped state
+    swap_ptr(cs->psio, cs->f[F_PSI]);
+    swap_ptr(cs->psio32, cs->g[F_PSI]);
+    swap_ptr(cs->f[F_P], cs->f[F_PO]);
+  }
+  return FZ_OK;
+}
+
+extern "C" int fz_case_batch_step(fz_case_batch *b, int nsteps, int *killed) {
+  if (!b || nsteps < 0) return fz_set_error(FZ_ERR_ARG, "fz_case_batch_step: bad argument");
+  const int n = (int)b->cs.size();
+  if (killed)
+    for (int q = 0; q < n; q++) killed[q] = 0;
+  for (fz_case *cs : b->cs)
+    if (cs->check_pending) return fz_set_error(FZ_ERR_STATE, "fz_case_batch_step: a case has a step not ended");
+  fz_ctx *c0 = b->cs[0]->ctx;
+  hipStream_t st = c0->st;
+  // the batch runs on the first case's stream, after each case's earlier work
+  std::vector<hipStream_t> own(n);
+  for (int q = 0; q < n; q++) {
+    own[q] = b->cs[q]->ctx->st;
+    if (q > 0) {
+      FZ_HIP(hipEventRecord(b->ev[q], own[q]));
+      FZ_HIP(hipStreamWaitEvent(st, b->ev[q], 0));
+    }
+  }
+  for (int q = 1; q < n; q++) b->cs[q]->ctx->st = st;  // the cases' own launches (dt scalars, checks) too
+  int rc = FZ_OK;
+  bool stop = false;
+  for (int s = 0; s < nsteps && !stop && rc == FZ_OK; s++) {
+    rc = batch_enqueue(b);
+    bool pending = false;
+    for (int q = 0; q < n && rc == FZ_OK; q++) {
+      fz_case *cs = b->cs[q];
+      const int istep = (int)cs->sc[S_ISTEP];
+      if (cs->prm.icheck > 0 && istep % cs->prm.icheck == 0) {
+        rc = check_enqueue(cs);
+        pending = true;
+      }
+    }
+    if (rc == FZ_OK && pending) {
+      if (hipStreamSynchronize(st) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "fz_case_batch_step: sync");
+      for (int q = 0; q < n && rc == FZ_OK; q++) {
+        fz_case *cs = b->cs[q];
+        if (!cs->check_pending) continue;
+        int kill = 0;
+        check_finish(cs, &kill);
+        if (kill) {
+          stop = true;
+          if (killed) killed[q] = 1;
+        }
+      }
+    }
+  }
+  for (int q = 1; q < n; q++) b->cs[q]->ctx->st = own[q];
+  // each case's later work after the batch's
+  if (hipEventRecord(b->ev[0], st) == hipSuccess)
+    for (int q = 1; q < n; q++) (void)hipStreamWaitEvent(own[q], b->ev[0], 0);
+  return rc;
+}

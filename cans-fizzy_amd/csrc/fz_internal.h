@@ -139,10 +139,15 @@ __device__ inline unsigned fz_xcd_block(unsigned P, unsigned nb) {
 // per block) in pencil order: after the XCD remap, consecutive blocks cover
 // one pencil of ty rows of blocks (full x extent) and march it through z, so
 // the k-1/k/k+1 planes an XCD's resident blocks read stay in its L2.
-__device__ inline void fz_stencil_block(unsigned ty, unsigned &bx, unsigned &by, unsigned &bz) {
-  const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
-  const unsigned nb = gx * gy * gz;
-  const unsigned L = fz_xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+// A batched launch (fz_case_batch) stacks the grids of its cases in z: gz
+// blocks per case, the case in cs (gz = gridDim.z and cs = 0 otherwise).
+__device__ inline void fz_stencil_block_b(unsigned ty, unsigned gz, unsigned &cs, unsigned &bx, unsigned &by,
+                                          unsigned &bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned nb = gx * gy * gridDim.z;
+  unsigned L = fz_xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+  cs = L / (gx * gy * gz);
+  L -= cs * (gx * gy * gz);
   if (ty == 0 || gy % ty != 0) {
     bz = L / (gx * gy);
     by = (L - bz * gx * gy) / gx;
@@ -157,6 +162,36 @@ __device__ inline void fz_stencil_block(unsigned ty, unsigned &bx, unsigned &by,
   by = pencil * ty + in / gx;
   bx = in - (in / gx) * gx;
 }
+__device__ inline void fz_stencil_block(unsigned ty, unsigned &bx, unsigned &by, unsigned &bz) {
+  unsigned cs;
+  fz_stencil_block_b(ty, gridDim.z, cs, bx, by, bz);
+}
+
+// One case of a batched step (fz_case_batch): the buffer of every role the
+// step's kernels read or write, in the two pointer states the case
+// alternates between from step to step (the rotations psio <-> psi and
+// p <-> po), and the case's own scalars. A batched kernel takes the case of
+// its block, reads the state s = par[case] and uses at[s][role].
+enum {
+  BR_U, BR_V, BR_W, BR_UN, BR_VN, BR_WN, BR_P, BR_PO, BR_DP, BR_PSI_OLD, BR_PSI_NEW, BR_PHI, BR_NX, BR_NY,
+  BR_NZ, BR_KAPPA, BR_FX, BR_FY, BR_FZ, BR_DPSIO, BR_DUO, BR_DVO, BR_DWO, BR_N
+};
+struct fz_bcase {
+  void *at[2][BR_N];
+  const double *stc;              // dt and the AB2 factors (FZ_STC_*)
+  double *dsc;                    // gamma in dsc[0]
+  unsigned long long *gslots;     // gamma slots of the correction step
+  double rho, drho, mu, dmu, surf_factor, rho0, seps, gam_factor, gam_min;
+  double bforce[3], gacc[3];
+};
+// the batch of a launch: null bt for a single case
+struct fz_bref {
+  const fz_bcase *bt;
+  const int *par;
+  unsigned gz;  // grid z blocks of one case
+};
+__device__ inline void *fz_brole(const fz_bref &b, unsigned cs, int role) { return b.bt[cs].at[b.par[cs]][role]; }
+static const fz_bref kNoBRef = {nullptr, nullptr, 0};
 
 // ---- launch helpers ----
 static inline dim3 fz_block3() { return dim3(64, 4, 1); }
@@ -170,6 +205,9 @@ static inline dim3 fz_grid_halo(const fz_dims &d) {
 // internal entry points shared between translation units
 int fz_reduce_ensure(fz_ctx *ctx, size_t nblocks);
 int fz_scratch_setup(fz_ctx *ctx);
+int fz_batch_solve(fz_ctx *c0, void **dptrs, int n);
+// why cs cannot join a batched step led by first (null: it can)
+const char *fz_case_batch_refusal(const fz_case *first, const fz_case *cs);
 void fz_bcmaps_free(fz_ctx *ctx);
 int fz_poisson_destroy(fz_poisson *p);
 int fz_solver_dev(fz_ctx *ctx, double *p);

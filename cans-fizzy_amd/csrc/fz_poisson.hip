@@ -1714,7 +1714,19 @@ extern "C" int fz_solver_batched(fz_ctx *const *ctxs, double *const *ps, int n) 
     P->batch_cap = 3 * (size_t)n;
   }
   FZ_HIP(hipMemcpy(P->batch_ptrs, h.data(), sizeof(void *) * 3 * (size_t)n, hipMemcpyHostToDevice));
-  void **dp = P->batch_ptrs;
+  FZ_TRY(fz_batch_solve(c0, P->batch_ptrs, n));
+  FZ_HIP(hipStreamSynchronize(c0->st));
+  return FZ_OK;
+}
+
+// The stages of n same-shaped solves (non-periodic z) in one launch each, on
+// c0's stream: dptrs is a device array of the n right-hand sides (halo'd,
+// overwritten by the solutions), then the n work arrays, then the n pivot
+// arrays. No synchronisation and no allocation (a batched step enqueues it).
+int fz_batch_solve(fz_ctx *c0, void **dptrs, int n) {
+  fz_poisson *P = c0->pois;
+  if (!P || P->periodic_z) return fz_set_error(FZ_ERR_UNSUPPORTED, "batched solve: periodic z is not batched");
+  void **dp = dptrs;
   const Batch p_w = {(const void *const *)dp, dp + n, n};              // p -> work
   const Batch w_w = {(const void *const *)(dp + n), dp + n, n};        // work in place
   const Batch w_p = {(const void *const *)(dp + n), dp, n};            // work -> p
@@ -1736,7 +1748,6 @@ extern "C" int fz_solver_batched(fz_ctx *const *ctxs, double *const *ps, int n) 
                                              w_y, (double *)nullptr, w_y, n1, n3, 1.0, 0, w_w)));
   FZ_TRY((launch_r2r<double, double, double>(c0, "fft_x_bwd_batch", P->px, P->px.kind_b, (const double *)nullptr,
                                              w_x, (double *)nullptr, p_x, n2, n3, P->normfft, 0, w_p)));
-  FZ_HIP(hipStreamSynchronize(c0->st));
   return FZ_OK;
 }
 

@@ -117,6 +117,15 @@ struct Finished {
   std::string err;   // that error's message
 };
 
+// same-shaped running cases stepped together (fz_case_batch), kept from
+// round to round while its members stay the same
+struct Group {
+  std::vector<int> ids;
+  fz_case_batch *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t st = nullptr;
+};
+
 struct Prepared {  // a set-up finished by the set-up thread
   Running r;
   int rc;
@@ -146,7 +155,16 @@ struct fz_server {
   // allocations or copies on another have invalidated the capture
   std::mutex hip_mu;
   std::thread worker;
+  bool group = true;  // batched steps of same-shaped cases (FZ_SERVE_GROUP=0: off)
+  std::vector<Group> groups;
 };
+
+static void drop_group(Group &g) {
+  if (g.b) fz_case_batch_destroy(g.b);
+  if (g.e0) (void)hipEventDestroy(g.e0);
+  if (g.e1) (void)hipEventDestroy(g.e1);
+  g = Group();
+}
 
 static void drop_running(Running &r) {
   for (hipEvent_t &e : r.ev)
@@ -206,6 +224,7 @@ extern "C" int fz_server_create(fz_server **out, size_t hbm_budget) {
   s->budget = hbm_budget;
   s->device = dev;
   if (const char *e = getenv("FZ_SERVE_BATCH")) s->max_batch = std::max(1, std::min(FZ_SERVE_MAX_BATCH, atoi(e)));
+  if (const char *e = getenv("FZ_SERVE_GROUP")) s->group = atoi(e) != 0;
   s->worker = std::thread(worker_loop, s);
   *out = s;
   return FZ_OK;
@@ -219,6 +238,7 @@ extern "C" int fz_server_destroy(fz_server *s) {
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  for (Group &g : s->groups) drop_group(g);
   for (Prepared &p : s->ready)
     if (p.rc == FZ_OK) drop_running(p.r);
   for (Running &r : s->active) drop_running(r);
@@ -300,6 +320,58 @@ extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
   // work on the hardware queues they share. A case whose graph is still to
   // be captured runs the step's launches directly while the set-up thread
   // is busy.
+  //
+  // Running cases of one shape (grid, boundary conditions, precision) that
+  // have taken their first step are stepped together instead, one batched
+  // step per round (fz_case_batch: one launch per stage for all of them),
+  // after the single cases' steps are enqueued.
+  const size_t na = s->active.size();
+  std::vector<int> gid(na, -1);
+  std::vector<Group> groups;
+  if (s->group) {
+    std::vector<std::vector<size_t>> members;
+    for (size_t q = 0; q < na; q++) {
+      const Running &r = s->active[q];
+      if (r.left <= 0 || r.last_ms <= 0.0f || fz_case_batch_refusal(r.cs, r.cs)) continue;
+      bool placed = false;
+      for (size_t g = 0; g < members.size() && !placed; g++)
+        if (!fz_case_batch_refusal(s->active[members[g][0]].cs, r.cs)) {
+          members[g].push_back(q);
+          placed = true;
+        }
+      if (!placed) members.push_back({q});
+    }
+    for (const std::vector<size_t> &m : members) {
+      if (m.size() < 2) continue;
+      Group g;
+      for (size_t q : m) g.ids.push_back(s->active[q].id);
+      bool found = false;
+      for (Group &o : s->groups)
+        if (o.b && o.ids == g.ids) {
+          g = o;
+          o = Group();
+          found = true;
+          break;
+        }
+      if (!found) {
+        std::vector<fz_case *> cs;
+        for (size_t q : m) cs.push_back(s->active[q].cs);
+        int rc = fz_case_batch_create(cs.data(), (int)cs.size(), &g.b);
+        if (rc == FZ_OK && hipEventCreate(&g.e0) != hipSuccess) rc = FZ_ERR_HIP;
+        if (rc == FZ_OK && hipEventCreate(&g.e1) != hipSuccess) rc = FZ_ERR_HIP;
+        g.st = s->active[m[0]].ctx->st;
+        if (rc != FZ_OK) {  // the cases step alone
+          drop_group(g);
+          continue;
+        }
+      }
+      for (size_t q : m) gid[q] = (int)groups.size();
+      groups.push_back(g);
+    }
+  }
+  for (Group &o : s->groups)
+    if (o.b) drop_group(o);  // groups whose members changed
+  s->groups.clear();
   float tmax = 0.0f;
   for (const Running &r : s->active) tmax = std::max(tmax, r.last_ms);
   std::vector<int> want(s->active.size()), open(s->active.size());
@@ -307,7 +379,7 @@ extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
     Running &r = s->active[q];
     int d = 1;
     if (r.last_ms > 0.0f && tmax > 0.0f) d = std::max(1, std::min(s->max_batch, (int)(tmax / r.last_ms)));
-    want[q] = std::min(d, std::max(r.left, 0));
+    want[q] = gid[q] >= 0 ? 0 : std::min(d, std::max(r.left, 0));
     open[q] = want[q] > 0;
     r.nq = 0;
     r.rc = FZ_OK;
@@ -336,6 +408,36 @@ extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
     }
   }
   if (may_capture) s->hip_mu.unlock();
+  // the groups' batched steps
+  std::vector<float> gms(groups.size(), 0.0f);
+  for (size_t g = 0; g < groups.size(); g++) {
+    Group &G = groups[g];
+    std::vector<int> killed(G.ids.size(), 0);
+    int rc = hipEventRecord(G.e0, G.st) == hipSuccess ? FZ_OK : fz_set_error(FZ_ERR_HIP, "hipEventRecord");
+    if (rc == FZ_OK) rc = fz_case_batch_step(G.b, 1, killed.data());
+    if (rc == FZ_OK && hipEventRecord(G.e1, G.st) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventRecord");
+    if (rc == FZ_OK && hipEventSynchronize(G.e1) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "hipEventSynchronize");
+    if (rc == FZ_OK && hipEventElapsedTime(&gms[g], G.e0, G.e1) != hipSuccess)
+      rc = fz_set_error(FZ_ERR_HIP, "hipEventElapsedTime");
+    const std::string err = rc == FZ_OK ? std::string() : std::string("batched step: ") + fz_last_error();
+    for (size_t q = 0; q < na; q++) {
+      if (gid[q] != (int)g) continue;
+      Running &r = s->active[q];
+      size_t k = 0;
+      while (G.ids[k] != r.id) k++;
+      if (rc != FZ_OK) {
+        r.rc = rc;
+        r.err = err;
+        continue;
+      }
+      s->step_ms[r.id].push_back(gms[g]);
+      s->all_ms.push_back(gms[g]);
+      r.last_ms = gms[g];
+      r.killed = r.killed || killed[k];
+      r.left = killed[k] ? 0 : r.left - 1;
+    }
+  }
+  s->groups.swap(groups);
   std::vector<Running> still;
   for (Running &r : s->active) {
     if (r.rc == FZ_OK && r.nq > 0) {  // stepped in this round (a request of 0 steps is not)
@@ -360,6 +462,10 @@ extern "C" int fz_server_round(fz_server *s, int *n_active, int *n_queued) {
       }
       r.killed = r.killed || killed;
       r.left = killed ? 0 : r.left - r.nq;
+    }
+    if (r.rc != FZ_OK || r.left <= 0) {  // its group, if any, is made again without it
+      for (Group &G : s->groups)
+        if (G.b && std::find(G.ids.begin(), G.ids.end(), r.id) != G.ids.end()) drop_group(G);
     }
     if (r.rc != FZ_OK) {
       // the failed case is dropped (its stream drained first), the request

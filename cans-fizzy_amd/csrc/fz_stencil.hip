@@ -19,8 +19,22 @@
   unsigned bx, by, bz;              \
   fz_stencil_block((d).tile_y, bx, by, bz);
 
+// the same in a kernel that may run batched (fz_bref br): the block's case in cs
+#define FZ_BLOCK_XYZB(d, br, cs, bx, by, bz) \
+  unsigned cs, bx, by, bz;                   \
+  fz_stencil_block_b((d).tile_y, BT ? (br).gz : gridDim.z, cs, bx, by, bz);
+
 #define CELL_IJK(d)                                   \
   FZ_BLOCK_XYZ(d, bx_, by_, bz_);                     \
+  const int i = bx_ * blockDim.x + threadIdx.x + 1;   \
+  const int j = by_ * blockDim.y + threadIdx.y + 1;   \
+  const int k = bz_ + 1;                              \
+  if (i > (d).n1 || j > (d).n2) return;               \
+  const long q = i + (d).sy * j + (d).sz * k;         \
+  const long r = (i - 1) + (long)(d).n1 * ((j - 1) + (long)(d).n2 * (k - 1));
+
+#define CELL_IJKB(d, br, cs)                          \
+  FZ_BLOCK_XYZB(d, br, cs, bx_, by_, bz_);            \
   const int i = bx_ * blockDim.x + threadIdx.x + 1;   \
   const int j = by_ * blockDim.y + threadIdx.y + 1;   \
   const int k = bz_ + 1;                              \
@@ -187,8 +201,16 @@ __global__ void __launch_bounds__(256) k_correc_step_mz(fz_dims d, const double 
 
 // gam = max(sqrt(velmax)*gam_factor, gam_min) (src/acdi.f90:57-60, src/main.f90:526-527)
 // from the slots of k_correc_step, which are cleared for the next step
+template <bool BT = false>
 __global__ void __launch_bounds__(FZ_GAMMA_SLOTS) k_gamma_fin(unsigned long long *__restrict__ slots, double factor,
-                                                              double gmin, double *__restrict__ out) {
+                                                              double gmin, double *__restrict__ out, fz_bref br) {
+  if constexpr (BT) {  // batched: one block per case
+    const fz_bcase &B = br.bt[blockIdx.x];
+    slots = B.gslots;
+    factor = B.gam_factor;
+    gmin = B.gam_min;
+    out = B.dsc;
+  }
   double m = __longlong_as_double((long long)slots[threadIdx.x]);
   slots[threadIdx.x] = 0ull;
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
@@ -203,7 +225,7 @@ __global__ void __launch_bounds__(FZ_GAMMA_SLOTS) k_gamma_fin(unsigned long long
 
 
 int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out) {
-  FZ_LAUNCH(c, "gamma_fin", k_gamma_fin, dim3(1), dim3(FZ_GAMMA_SLOTS), 0, slots, factor, gmin, out);
+  FZ_LAUNCH(c, "gamma_fin", k_gamma_fin<false>, dim3(1), dim3(FZ_GAMMA_SLOTS), 0, slots, factor, gmin, out, kNoBRef);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -301,7 +323,7 @@ static int fits_buffers(const fz_ctx *c, const char *who) {
 // five (dp) or two (u, v) neighbour loads, whose address processing bounds
 // the one-cell-per-load kernel. Same expressions, same values.
 constexpr int COR_MX = 64, COR_PX = COR_MX + 2;
-template <int KZ, int MY>
+template <int KZ, int MY, bool BT = false>
 __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const double *__restrict__ dzci,
                                                               double factori_v, double factorj_v, double dt_v,
                                                               double dxi, double dyi, const double *__restrict__ stc,
@@ -312,12 +334,27 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
                                                               const double *__restrict__ w, double *__restrict__ un,
                                                               double *__restrict__ vn, double *__restrict__ wn,
                                                               double *__restrict__ p_out,
-                                                              unsigned long long *__restrict__ slots) {
+                                                              unsigned long long *__restrict__ slots, fz_bref br) {
   constexpr int NT = COR_MX * MY, PL = COR_PX * (MY + 2), NPT = (PL + NT - 1) / NT;
   __shared__ double s_dp[2 * PL], s_u[2 * PL], s_v[2 * PL];
+  FZ_BLOCK_XYZB(d, br, cs, bx, by, bz);
+  if constexpr (BT) {  // this block's case of a batch: the momentum step's velocity in the spare buffers
+    const fz_bcase &B = br.bt[cs];
+    stc = B.stc;
+    rhop = B.rho0;
+    slots = B.gslots;
+    dp = (const double *)fz_brole(br, cs, BR_DP);
+    p = (const double *)fz_brole(br, cs, BR_P);
+    u = (const double *)fz_brole(br, cs, BR_UN);
+    v = (const double *)fz_brole(br, cs, BR_VN);
+    w = (const double *)fz_brole(br, cs, BR_WN);
+    un = (double *)fz_brole(br, cs, BR_U);
+    vn = (double *)fz_brole(br, cs, BR_V);
+    wn = (double *)fz_brole(br, cs, BR_W);
+    p_out = (double *)fz_brole(br, cs, BR_PO);
+  }
   const double dt = stc ? stc[FZ_STC_DT] : dt_v;
   const double factori = stc ? dt * dxi : factori_v, factorj = stc ? dt * dyi : factorj_v;
-  FZ_BLOCK_XYZ(d, bx, by, bz);
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + COR_MX * ty;
   const int i0 = (int)bx * COR_MX, j0 = (int)by * MY;  // the range starts at the halo
   const int i = i0 + tx, j = j0 + ty;
@@ -420,7 +457,7 @@ int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, 
     const dim3 grid((c->d.n1 + 2 + COR_MX - 1) / COR_MX, (c->d.n2 + 2 + my - 1) / my, (c->d.n3 + 2 + KZ - 1) / KZ);
 #define FZ_CORREC_LDS(MY)                                                                                          \
   FZ_LAUNCH(c, "correc_step", (k_correc_lds<KZ, MY>), grid, dim3(COR_MX, MY), 0, c->d, c->dzci, dt * c->dli[0],     \
-            dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots)
+            dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots, kNoBRef)
     if (my == 16) FZ_CORREC_LDS(16);
     else if (my == 8) FZ_CORREC_LDS(8);
     else FZ_CORREC_LDS(4);
@@ -603,7 +640,7 @@ struct LdsZ {
   }
 };
 
-template <typename T, int KZ>
+template <typename T, int KZ, bool BT = false>  // BT: a batched launch (fz_bref)
 __global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_lds(fz_dims d, const double *__restrict__ dzci,
                                                             const double *__restrict__ dzfi, double dxi, double dyi,
                                                             const double *__restrict__ d_gam, double gam_v,
@@ -615,13 +652,32 @@ __global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_lds(fz_dims d, const do
                                                             const T *__restrict__ phi, const T *__restrict__ psio,
                                                             T *__restrict__ dpsio, T *__restrict__ psi_out,
                                                             T *__restrict__ flux_x, T *__restrict__ flux_y,
-                                                            T *__restrict__ flux_z) {
+                                                            T *__restrict__ flux_z, fz_bref br) {
   constexpr int NT = ACD_MX * ACD_MY, PL = ACD_PL, NPT = (PL + NT - 1) / NT;
   __shared__ double s_u[2 * PL], s_v[2 * PL], s_w[2 * PL];
   __shared__ T s_nx[2 * PL], s_ny[2 * PL], s_nz[2 * PL], s_phi[2 * PL], s_psi[2 * PL];
+  FZ_BLOCK_XYZB(d, br, cs, bx, by, bz);
+  if constexpr (BT) {  // this block's case of a batch
+    d_gam = br.bt[cs].dsc;
+    seps = br.bt[cs].seps;
+    stc = br.bt[cs].stc;
+    first = 0;
+    u = (const double *)fz_brole(br, cs, BR_U);
+    v = (const double *)fz_brole(br, cs, BR_V);
+    w = (const double *)fz_brole(br, cs, BR_W);
+    normx = (const T *)fz_brole(br, cs, BR_NX);
+    normy = (const T *)fz_brole(br, cs, BR_NY);
+    normz = (const T *)fz_brole(br, cs, BR_NZ);
+    phi = (const T *)fz_brole(br, cs, BR_PHI);
+    psio = (const T *)fz_brole(br, cs, BR_PSI_OLD);
+    dpsio = (T *)fz_brole(br, cs, BR_DPSIO);
+    psi_out = (T *)fz_brole(br, cs, BR_PSI_NEW);
+    flux_x = (T *)fz_brole(br, cs, BR_FX);
+    flux_y = (T *)fz_brole(br, cs, BR_FY);
+    flux_z = (T *)fz_brole(br, cs, BR_FZ);
+  }
   const double gam = d_gam ? *d_gam : gam_v;
   const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
-  FZ_BLOCK_XYZ(d, bx, by, bz);
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + ACD_MX * ty;
   const int i0 = (int)bx * ACD_MX + 1, j0 = (int)by * ACD_MY + 1, k0 = (int)bz * KZ + 1;
   const int i = i0 + tx, j = j0 + ty;
@@ -734,7 +790,7 @@ int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps,
     const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
     FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", (k_acdi_lds<T, KZ>), grid, dim3(ACD_MX, ACD_MY), 0,
               c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc, first, u, v, w,
-              normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
+              normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z, kNoBRef);
   } else {
     FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", k_acdi_step<T>, fz_grid_interior(c->d),
               fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc,
@@ -753,8 +809,14 @@ template int fz_launch_acdi_step<float>(fz_ctx *, const double *, double, double
                                         float *, float *, float *);
 
 // acdi_cmpt_phi, src/acdi.f90:239-247 (halo-inclusive)
-template <typename T>
-__global__ void k_acdi_phi(long nh, double seps_d, const T *__restrict__ psi, T *__restrict__ phi) {
+template <typename T, bool BT = false>
+__global__ void k_acdi_phi(long nh, double seps_d, const T *__restrict__ psi, T *__restrict__ phi, fz_bref br) {
+  if constexpr (BT) {  // batched: the case is blockIdx.y
+    const unsigned cs = blockIdx.y;
+    seps_d = br.bt[cs].seps;
+    psi = (const T *)fz_brole(br, cs, BR_PSI_NEW);
+    phi = (T *)fz_brole(br, cs, BR_PHI);
+  }
   const T seps = (T)seps_d;
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
     phi[q] = seps * fz_log(fz_div(psi[q] + (T)FZ_EPS, (T)1.0 - psi[q] + (T)FZ_EPS));
@@ -997,13 +1059,20 @@ __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dy
                (T)0.25 * ((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7])) * (T)dzfi_k);
 }
 
-template <typename T>
+template <typename T, bool BT = false>
 __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restrict__ dzci,
                                                 const double *__restrict__ dzfi, double dxi_d, double dyi_d,
                                                 const T *__restrict__ psi, T *__restrict__ normx,
                                                 T *__restrict__ normy, T *__restrict__ normz,
-                                                T *__restrict__ kappa) {
-  CELL_IJK(d);
+                                                T *__restrict__ kappa, fz_bref br) {
+  CELL_IJKB(d, br, cs);
+  if constexpr (BT) {  // this block's case of a batch
+    psi = (const T *)fz_brole(br, cs, BR_PHI);
+    normx = (T *)fz_brole(br, cs, BR_NX);
+    normy = (T *)fz_brole(br, cs, BR_NY);
+    normz = (T *)fz_brole(br, cs, BR_NZ);
+    kappa = (T *)fz_brole(br, cs, BR_KAPPA);
+  }
   (void)r;
   T nx, ny, nz, ka;
   bool ok = true;
@@ -1020,11 +1089,17 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
   kappa[q] = ka;
 }
 
+// (a batch of ncase cases when br.bt is set: their grids stacked in z)
 template <typename T>
-static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T *normy, T *normz, T *kappa) {
+static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T *normy, T *normz, T *kappa,
+                         fz_bref br = kNoBRef, int ncase = 1) {
   FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
-  FZ_LAUNCH(c, name, k_youngs<T>, fz_grid_interior(c->d), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
-            c->dli[1], psi, normx, normy, normz, kappa);
+  dim3 grid = fz_grid_interior(c->d);
+  br.gz = grid.z;
+  grid.z *= (unsigned)ncase;
+  auto kern = br.bt ? k_youngs<T, true> : k_youngs<T, false>;
+  FZ_LAUNCH(c, name, kern, grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi, normx,
+            normy, normz, kappa, br);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -1251,7 +1326,7 @@ __global__ void __launch_bounds__(256) k_mom_oth(fz_dims d, const double *__rest
 // step. STAGED: the three advection-diffusion components one after another,
 // each with its own loads, behind scheduling barriers, so fewer values are
 // live at once; the arithmetic is the same.)
-template <typename PT, typename HT, bool STAGED>
+template <typename PT, typename HT, bool STAGED, bool BT>
 __device__ __forceinline__ void mom_fused_body(
     fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
     double drho, double mu, double dmu, double factor1, double factor2, int first, double factor12, double dt_r,
@@ -1261,8 +1336,37 @@ __device__ __forceinline__ void mom_fused_body(
     const double *__restrict__ w, const PT *__restrict__ psi, const PT *__restrict__ psio,
     const double *__restrict__ p, const double *__restrict__ po, const PT *__restrict__ kappa,
     HT *__restrict__ duo, HT *__restrict__ dvo, HT *__restrict__ dwo, double *__restrict__ un,
-    double *__restrict__ vn, double *__restrict__ wn) {
-  CELL_IJK(d);
+    double *__restrict__ vn, double *__restrict__ wn, const fz_bref &br) {
+  CELL_IJKB(d, br, cs);
+  if constexpr (BT) {  // this block's case of a batch
+    const fz_bcase &B = br.bt[cs];
+    const double *stc = B.stc;
+    factor1 = stc[FZ_STC_F1];
+    factor2 = stc[FZ_STC_F2];
+    factor12 = stc[FZ_STC_F12];
+    dt_r = stc[FZ_STC_DTR];
+    first = 0;
+    rho = B.rho, drho = B.drho, mu = B.mu, dmu = B.dmu, surf_factor = B.surf_factor, rho0 = B.rho0;
+    bfx = B.bforce[0], bfy = B.bforce[1], bfz = B.bforce[2], gx = B.gacc[0], gy = B.gacc[1], gz = B.gacc[2];
+    rax = ray = raz = 0.0;
+    psiflx_x = (const PT *)fz_brole(br, cs, BR_FX);
+    psiflx_y = (const PT *)fz_brole(br, cs, BR_FY);
+    psiflx_z = (const PT *)fz_brole(br, cs, BR_FZ);
+    u = (const double *)fz_brole(br, cs, BR_U);
+    v = (const double *)fz_brole(br, cs, BR_V);
+    w = (const double *)fz_brole(br, cs, BR_W);
+    psi = (const PT *)fz_brole(br, cs, BR_PSI_NEW);
+    psio = (const PT *)fz_brole(br, cs, BR_PSI_OLD);
+    p = (const double *)fz_brole(br, cs, BR_P);
+    po = (const double *)fz_brole(br, cs, BR_PO);
+    kappa = (const PT *)fz_brole(br, cs, BR_KAPPA);
+    duo = (HT *)fz_brole(br, cs, BR_DUO);
+    dvo = (HT *)fz_brole(br, cs, BR_DVO);
+    dwo = (HT *)fz_brole(br, cs, BR_DWO);
+    un = (double *)fz_brole(br, cs, BR_UN);
+    vn = (double *)fz_brole(br, cs, BR_VN);
+    wn = (double *)fz_brole(br, cs, BR_WN);
+  }
   const BufAt<double> U = buf_at(u, d, q), V = buf_at(v, d, q), W = buf_at(w, d, q);
   const BufAt<PT> PSI = buf_at(psi, d, q), FX = buf_at(psiflx_x, d, q), FY = buf_at(psiflx_y, d, q),
                   FZ = buf_at(psiflx_z, d, q);
@@ -1304,7 +1408,7 @@ __device__ __forceinline__ void mom_fused_body(
 }
 
 #define FZ_MOM_FUSED_KERNEL(NAME, ATTR, STG)                                                                     \
-  template <typename PT, typename HT>                                                                            \
+  template <typename PT, typename HT, bool BT = false>                                                           \
   __global__ void __launch_bounds__(256) ATTR NAME(                                                              \
       fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi,       \
       double rho, double drho, double mu, double dmu, double factor1_v, double factor2_v, int first,            \
@@ -1315,13 +1419,13 @@ __device__ __forceinline__ void mom_fused_body(
       const double *__restrict__ v, const double *__restrict__ w, const PT *__restrict__ psi,                    \
       const PT *__restrict__ psio, const double *__restrict__ p, const double *__restrict__ po,                  \
       const PT *__restrict__ kappa, HT *__restrict__ duo, HT *__restrict__ dvo, HT *__restrict__ dwo,            \
-      double *__restrict__ un, double *__restrict__ vn, double *__restrict__ wn) {                               \
+      double *__restrict__ un, double *__restrict__ vn, double *__restrict__ wn, fz_bref br) {                   \
     const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;      \
     const double factor12 = stc ? stc[FZ_STC_F12] : factor12_v, dt_r = stc ? stc[FZ_STC_DTR] : dt_r_v;         \
-    mom_fused_body<PT, HT, STG>(d, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, factor1, factor2, first, factor12,    \
+    mom_fused_body<PT, HT, STG, BT>(d, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, factor1, factor2, first, factor12,    \
                                 dt_r,                                                                            \
                            surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz, psiflx_x, psiflx_y,      \
-                           psiflx_z, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);               \
+                           psiflx_z, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn, br);           \
   }
 FZ_MOM_FUSED_KERNEL(k_mom_fused, , false)
 FZ_MOM_FUSED_KERNEL(k_mom_fused_st, __attribute__((amdgpu_waves_per_eu(4))), true)
@@ -1354,7 +1458,7 @@ struct LdsAt {
   }
 };
 
-template <typename PT, typename HT, int KZ, int WPE>  // WPE: waves per SIMD the registers must allow
+template <typename PT, typename HT, int KZ, int WPE, bool BT = false>  // WPE: waves per SIMD the registers must allow
 __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_per_eu(WPE))) k_mom_lds(
     fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
     double drho, double mu, double dmu, double factor1_v, double factor2_v, int first, double factor12_v,
@@ -1364,16 +1468,41 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
     const double *__restrict__ v, const double *__restrict__ w, const PT *__restrict__ psi,
     const PT *__restrict__ psio, const double *__restrict__ p, const double *__restrict__ po,
     const PT *__restrict__ kappa, HT *__restrict__ duo, HT *__restrict__ dvo, HT *__restrict__ dwo,
-    double *__restrict__ un, double *__restrict__ vn, double *__restrict__ wn) {
+    double *__restrict__ un, double *__restrict__ vn, double *__restrict__ wn, fz_bref br) {
   constexpr int NT = MOM_MX * MOM_MY, PL = MOM_PL;
   constexpr int NPT = (PL + NT - 1) / NT;  // plane positions per thread
   // u, v, w, psi and the z flux are read at k-1, k, k+1 (three plane slots);
   // the x and y fluxes at k and k+1 only (two slots)
   __shared__ double s_u[3 * PL], s_v[3 * PL], s_w[3 * PL];
   __shared__ PT s_psi[3 * PL], s_fx[2 * PL], s_fy[2 * PL], s_fz[3 * PL];
+  FZ_BLOCK_XYZB(d, br, cs, bx, by, bz);
+  if constexpr (BT) {  // this block's case of a batch
+    const fz_bcase &B = br.bt[cs];
+    stc = B.stc;
+    first = 0;
+    rho = B.rho, drho = B.drho, mu = B.mu, dmu = B.dmu, surf_factor = B.surf_factor, rho0 = B.rho0;
+    bfx = B.bforce[0], bfy = B.bforce[1], bfz = B.bforce[2], gx = B.gacc[0], gy = B.gacc[1], gz = B.gacc[2];
+    rax = ray = raz = 0.0;
+    psiflx_x = (const PT *)fz_brole(br, cs, BR_FX);
+    psiflx_y = (const PT *)fz_brole(br, cs, BR_FY);
+    psiflx_z = (const PT *)fz_brole(br, cs, BR_FZ);
+    u = (const double *)fz_brole(br, cs, BR_U);
+    v = (const double *)fz_brole(br, cs, BR_V);
+    w = (const double *)fz_brole(br, cs, BR_W);
+    psi = (const PT *)fz_brole(br, cs, BR_PSI_NEW);
+    psio = (const PT *)fz_brole(br, cs, BR_PSI_OLD);
+    p = (const double *)fz_brole(br, cs, BR_P);
+    po = (const double *)fz_brole(br, cs, BR_PO);
+    kappa = (const PT *)fz_brole(br, cs, BR_KAPPA);
+    duo = (HT *)fz_brole(br, cs, BR_DUO);
+    dvo = (HT *)fz_brole(br, cs, BR_DVO);
+    dwo = (HT *)fz_brole(br, cs, BR_DWO);
+    un = (double *)fz_brole(br, cs, BR_UN);
+    vn = (double *)fz_brole(br, cs, BR_VN);
+    wn = (double *)fz_brole(br, cs, BR_WN);
+  }
   const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
   const double factor12 = stc ? stc[FZ_STC_F12] : factor12_v, dt_r = stc ? stc[FZ_STC_DTR] : dt_r_v;
-  FZ_BLOCK_XYZ(d, bx, by, bz);
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + MOM_MX * ty;
   const int i0 = (int)bx * MOM_MX + 1, j0 = (int)by * MOM_MY + 1, k0 = (int)bz * KZ + 1;
   const int i = i0 + tx, j = j0 + ty;
@@ -1492,7 +1621,7 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
               grid, dim3(MOM_MX, MOM_MY), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1],
               rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12, dt_r, stc,
               surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1],
-              rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn);
+              rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un, vn, wn, kNoBRef);
     FZ_LAUNCHED();
     return FZ_OK;
   }
@@ -1504,7 +1633,7 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
             c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1], rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1,
             factor2, first, factor12, dt_r, stc, surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1],
             gacc[2], rho_av[0], rho_av[1], rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un,
-            vn, wn);
+            vn, wn, kNoBRef);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -1705,14 +1834,16 @@ extern "C" int fz_acdi_transport_pf(fz_ctx *c, double gam, double seps, const do
 
 extern "C" int fz_acdi_cmpt_phi(fz_ctx *c, double seps, const double *psi, double *phi) {
   if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_acdi_cmpt_phi: null argument");
-  FZ_LAUNCH(c, "acdi_phi", k_acdi_phi<double>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi);
+  FZ_LAUNCH(c, "acdi_phi", k_acdi_phi<double>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi,
+            kNoBRef);
   FZ_LAUNCHED();
   return FZ_OK;
 }
 
 // FP32 phase field of the mixed-precision step
 int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi) {
-  FZ_LAUNCH(c, "acdi_phi_f32", k_acdi_phi<float>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi);
+  FZ_LAUNCH(c, "acdi_phi_f32", k_acdi_phi<float>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi,
+            kNoBRef);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -1929,6 +2060,148 @@ extern "C" int fz_math_eval(fz_ctx *c, int fn, long n, const double *x, const do
   const long nb = (n + 255) / 256;
   if (nb > 0x7fffffffL) return fz_set_error(FZ_ERR_ARG, "fz_math_eval: n too large");
   FZ_LAUNCH(c, "math_eval", k_math_eval, dim3((unsigned)nb), dim3(256), 0, fn, n, x, y, out);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launches of the step's kernels for a batch of ncase same-shaped cases
+// (fz_case_batch): the grids of the cases stacked in z (or one block row per
+// case for the flat kernels), each block taking its case's buffers and
+// scalars from the table br. c is the first case's context: the grid, the
+// launch parameters and the stream. Each case's values are those of its own
+// launches bit for bit (the same kernels and arithmetic).
+template <typename T>
+int fz_batch_acdi_step(fz_ctx *c, fz_bref br, int ncase) {
+  FZ_TRY(fits_buffers(c, "acdi_step"));
+  constexpr int KZ = 16;
+  dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
+  br.gz = grid.z;
+  grid.z *= (unsigned)ncase;
+  FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", (k_acdi_lds<T, KZ, true>), grid, dim3(ACD_MX, ACD_MY), 0,
+            c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], (const double *)nullptr, 0.0, 0.0, 0.0, 0.0,
+            (const double *)nullptr, 0, (const double *)nullptr, (const double *)nullptr, (const double *)nullptr,
+            (const T *)nullptr, (const T *)nullptr, (const T *)nullptr, (const T *)nullptr, (const T *)nullptr,
+            (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr, (T *)nullptr, br);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+template int fz_batch_acdi_step<double>(fz_ctx *, fz_bref, int);
+template int fz_batch_acdi_step<float>(fz_ctx *, fz_bref, int);
+
+template <typename T>
+int fz_batch_phase_geometry(fz_ctx *c, fz_bref br, int ncase) {
+  if (ncase > 65535) return fz_set_error(FZ_ERR_ARG, "batched step: at most 65535 cases");
+  FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_phi_f32" : "acdi_phi", (k_acdi_phi<T, true>),
+            dim3((unsigned)flat_grid(c->nh), (unsigned)ncase), dim3(256), 0, c->nh, 0.0, (const T *)nullptr,
+            (T *)nullptr, br);
+  FZ_LAUNCHED();
+  return launch_youngs<T>(c, sizeof(T) == 4 ? "youngs_f32" : "youngs", (const T *)nullptr, (T *)nullptr,
+                          (T *)nullptr, (T *)nullptr, (T *)nullptr, br, ncase);
+}
+template int fz_batch_phase_geometry<double>(fz_ctx *, fz_bref, int);
+template int fz_batch_phase_geometry<float>(fz_ctx *, fz_bref, int);
+
+// the momentum kernel of the first case's variant (the LDS kernel, or the
+// plain one of the given occupancy)
+template <typename PT, typename HT>
+int fz_batch_mom(fz_ctx *c, fz_bref br, int ncase) {
+  FZ_TRY(fits_buffers(c, "mom_fused"));
+  int variant = c->mom_variant;
+  if (variant == 0) variant = sizeof(PT) == 8 ? 8 : 2;
+  const char *name = sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused";
+#define FZ_MOM_NULLS                                                                                              \
+  0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0.0, 0.0, (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, \
+      0.0, 0.0, (const PT *)nullptr, (const PT *)nullptr, (const PT *)nullptr, (const double *)nullptr,            \
+      (const double *)nullptr, (const double *)nullptr, (const PT *)nullptr, (const PT *)nullptr,                  \
+      (const double *)nullptr, (const double *)nullptr, (const PT *)nullptr, (HT *)nullptr, (HT *)nullptr,         \
+      (HT *)nullptr, (double *)nullptr, (double *)nullptr, (double *)nullptr, br
+  if (variant == 8) {
+    constexpr int KZ = 16;
+    dim3 grid((c->d.n1 + MOM_MX - 1) / MOM_MX, (c->d.n2 + MOM_MY - 1) / MOM_MY, (c->d.n3 + KZ - 1) / KZ);
+    br.gz = grid.z;
+    grid.z *= (unsigned)ncase;
+    FZ_LAUNCH(c, name, (k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1, true>), grid, dim3(MOM_MX, MOM_MY), 0,
+              c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], FZ_MOM_NULLS);
+  } else {
+    auto kern = variant == 4   ? k_mom_fused_w4<PT, HT, true>
+                : variant == 5 ? k_mom_fused_w5<PT, HT, true>
+                : variant == 1 ? k_mom_fused_st<PT, HT, true>
+                               : k_mom_fused<PT, HT, true>;
+    dim3 grid = fz_grid_interior(c->d);
+    br.gz = grid.z;
+    grid.z *= (unsigned)ncase;
+    FZ_LAUNCH(c, name, kern, grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], FZ_MOM_NULLS);
+  }
+#undef FZ_MOM_NULLS
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+template int fz_batch_mom<double, double>(fz_ctx *, fz_bref, int);
+template int fz_batch_mom<float, float>(fz_ctx *, fz_bref, int);
+
+// the correction step (the LDS kernel of the first case's tile height) and
+// gamma from its slots
+int fz_batch_correc_gamma(fz_ctx *c, fz_bref br, int ncase, bool gamma) {
+  FZ_TRY(fits_buffers(c, "correc_step"));
+  constexpr int KZ = 16;
+  const int my = c->correc_my;
+  dim3 grid((c->d.n1 + 2 + COR_MX - 1) / COR_MX, (c->d.n2 + 2 + my - 1) / my, (c->d.n3 + 2 + KZ - 1) / KZ);
+  fz_bref b = br;
+  b.gz = grid.z;
+  grid.z *= (unsigned)ncase;
+#define FZ_CORREC_LDS_B(MY)                                                                                        \
+  FZ_LAUNCH(c, "correc_step", (k_correc_lds<KZ, MY, true>), grid, dim3(COR_MX, MY), 0, c->d, c->dzci, 0.0, 0.0, 0.0, \
+            c->dli[0], c->dli[1], (const double *)nullptr, 0.0, (const double *)nullptr, (const double *)nullptr,   \
+            (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, (double *)nullptr,          \
+            (double *)nullptr, (double *)nullptr, (double *)nullptr, (unsigned long long *)nullptr, b)
+  if (my == 16) FZ_CORREC_LDS_B(16);
+  else if (my == 8) FZ_CORREC_LDS_B(8);
+  else FZ_CORREC_LDS_B(4);
+#undef FZ_CORREC_LDS_B
+  FZ_LAUNCHED();
+  if (gamma) {
+    FZ_LAUNCH(c, "gamma_fin", k_gamma_fin<true>, dim3((unsigned)ncase), dim3(FZ_GAMMA_SLOTS), 0,
+              (unsigned long long *)nullptr, 0.0, 0.0, (double *)nullptr, br);
+    FZ_LAUNCHED();
+  }
+  return FZ_OK;
+}
+
+// fillps (src/fillps.f90:33-50) for a batch: the momentum step's velocity
+// (the spare buffers) into each case's correction buffer, with its 1/dt and
+// rho0; the expression of the fillps evaluated in the forward x transform
+__global__ void __launch_bounds__(256) k_fillps_b(fz_dims d, const double *__restrict__ dzfi, double dxi, double dyi,
+                                                  fz_bref br) {
+  constexpr bool BT = true;
+  CELL_IJKB(d, br, cs);
+  (void)r;
+  const fz_bcase &B = br.bt[cs];
+  const double *__restrict__ u = (const double *)fz_brole(br, cs, BR_UN);
+  const double *__restrict__ v = (const double *)fz_brole(br, cs, BR_VN);
+  const double *__restrict__ w = (const double *)fz_brole(br, cs, BR_WN);
+  double *__restrict__ p = (double *)fz_brole(br, cs, BR_DP);
+  const double dti = B.stc[FZ_STC_DTI];
+  const double dtidxi = dti * dxi, dtidyi = dti * dyi;
+  p[q] = B.rho0 * ((w[q] - w[q - d.sz]) * dti * dzfi[k] + (v[q] - v[q - d.sy]) * dtidyi + (u[q] - u[q - 1]) * dtidxi);
+}
+
+int fz_batch_fillps(fz_ctx *c, fz_bref br, int ncase) {
+  dim3 grid = fz_grid_interior(c->d);
+  br.gz = grid.z;
+  grid.z *= (unsigned)ncase;
+  FZ_LAUNCH(c, "fillps", k_fillps_b, grid, fz_block3(), 0, c->d, c->dzfi, c->dli[0], c->dli[1], br);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
+// the next pointer state of every case of a batch
+__global__ void k_batch_flip(int n, int *__restrict__ par) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) par[t] ^= 1;
+}
+int fz_batch_flip(fz_ctx *c, int *par, int ncase) {
+  FZ_LAUNCH(c, "batch_flip", k_batch_flip, dim3((unsigned)((ncase + 255) / 256)), dim3(256), 0, ncase, par);
   FZ_LAUNCHED();
   return FZ_OK;
 }

@@ -90,6 +90,9 @@ SIGNATURES = {
     "fz_case_set_graphs": (C.c_int, [vp, C.c_int]),
     "fz_case_autotune": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_size_t]),
     "fz_case_destroy": (C.c_int, [vp]),
+    "fz_case_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+    "fz_case_batch_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+    "fz_case_batch_destroy": (C.c_int, [vp]),
     "fz_case_ctx": (C.c_int, [vp, C.POINTER(vp)]),
     "fz_case_field": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_size_t)]),
     "fz_case_get_scalars": (C.c_int, [vp, dp]),

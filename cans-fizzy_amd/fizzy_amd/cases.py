@@ -191,6 +191,36 @@ F32_FIELDS = {"psi", "phi", "kappa", "normx", "normy", "normz", "psiflx_x", "psi
               "dudt_old", "dvdt_old", "dwdt_old"}
 
 
+class CaseBatch:
+    """fz_case_batch_*: same-shaped cases stepped together, one launch per
+    stage of the step for all of them (each case's results are those of its
+    own steps bit for bit)."""
+
+    def __init__(self, cases):
+        self.cases = list(cases)
+        arr = (vp * len(self.cases))(*[c._h.value for c in self.cases])
+        h = vp()
+        call("fz_case_batch_create", arr, len(self.cases), C.byref(h))
+        self._h = h
+
+    def step(self, nsteps=1):
+        """nsteps steps of every case; returns the cases a check stopped."""
+        killed = (C.c_int * len(self.cases))()
+        call("fz_case_batch_step", self._h, int(nsteps), killed)
+        return [c for c, k in zip(self.cases, killed) if k]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fz_case_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceCase:
     """The device-resident case (fz_case_* of include/fizzy.h)."""
 

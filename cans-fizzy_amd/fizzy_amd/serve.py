@@ -146,10 +146,12 @@ def bubble_case(n, nz=None, two_d=False, density_ratio=10.0, mu12=(10.0, 1.0)):
 GRIDS = [(64, None, True), (128, None, True), (256, None, True), (64, 128, False), (128, 256, False)]
 
 
-def serving_requests(n, mean_interarrival=0.05, nsteps=200, seed=20261202, precision="fp64"):
+def serving_requests(n, mean_interarrival=0.05, nsteps=200, seed=20261202, precision="fp64", refine=1):
     """The serving mix: Poisson arrivals (exponential gaps, SplitMix64), grids
     uniform over {64^2, 128^2, 256^2, 64^2x128, 128^2x256}, density ratio
-    uniform in [2, 1000], nsteps steps each, in the given precision mode."""
+    uniform in [2, 1000], nsteps steps each, in the given precision mode
+    (mixed: refine FP64-residual refinements of an FP32 solve, or -1 for the
+    FP64 projection)."""
     g = splitmix64(seed)
     t, out = 0.0, []
     for _ in range(n):
@@ -159,6 +161,6 @@ def serving_requests(n, mean_interarrival=0.05, nsteps=200, seed=20261202, preci
         nxy, nz, two_d = GRIDS[gi]
         spec = bubble_case(nxy, nz, two_d, ratio)
         if precision != "fp64":
-            spec = dict(spec, precision=precision, refine=1)
+            spec = dict(spec, precision=precision, refine=refine)
         out.append((t, spec, nsteps))
     return out

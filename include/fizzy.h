@@ -289,6 +289,24 @@ int fz_case_set_graphs(fz_case *cs, int on);
  * bit for bit under every choice. report (if not null) receives the choice
  * and its time. No reference counterpart. */
 int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size_t cap);
+/* batched steps of same-shaped cases (no reference counterpart): one launch
+ * per stage of the step for all the cases, each block taking its case's
+ * buffers and scalars from a device table. Cases qualify when their first
+ * step is taken and they share the grid, every boundary condition and
+ * precision: ACDI two-fluid cases whose step needs no host read-back, FP64 or
+ * mixed precision with the FP64 projection (nref = -1), a non-periodic z and
+ * homogeneous pressure boundary values; create refuses others with
+ * FZ_ERR_UNSUPPORTED and the reason. A step of the batch gives every case
+ * the fields, scalars and checks of its own fz_case_step bit for bit; the
+ * work runs on the first case's stream after each case's earlier work, and
+ * each case's later work waits for it. nsteps steps, or fewer when a check
+ * stops a case (killed[q] = 1; killed may be null); cases must not be
+ * stepped otherwise while in a batch, and a batch must be made again once
+ * its cases have been stepped alone. */
+typedef struct fz_case_batch fz_case_batch;
+int fz_case_batch_create(fz_case *const *cases, int n, fz_case_batch **batch);
+int fz_case_batch_step(fz_case_batch *batch, int nsteps, int *killed);
+int fz_case_batch_destroy(fz_case_batch *batch);
 /* the checkpoint of src/main.f90:597-631: prefix_u.bin, _v, _w, _p, _psi
  * in the load_one format, with the case's time and step */
 int fz_case_save(fz_case *cs, const char *prefix);

@@ -247,3 +247,49 @@ def test_serving_mix_at_its_workload():
         assert srv.memory()[0] == 0
     finally:
         srv.close()
+
+
+def test_same_shaped_requests_are_stepped_in_batches():
+    """Requests of one shape (four FP64 2-D bubbles and three mixed-precision
+    3-D ones with the FP64 projection, each set with its own density ratio),
+    arriving while others run: the server steps them together once they have
+    taken their first step (fz_case_batch), and every result still equals
+    the case run alone, bit for bit. With batching off (FZ_SERVE_GROUP=0)
+    the results are the same."""
+    two_d = [dict(serve.bubble_case(32, two_d=True, density_ratio=r), icheck=5) for r in (900.0, 40.0, 3.0, 250.0)]
+    three_d = [dict(serve.bubble_case(16, 24, density_ratio=r), precision="mixed", refine=-1, icheck=5)
+               for r in (500.0, 8.0, 60.0)]
+    reqs = [(two_d[0], 17), (three_d[0], 13), (two_d[1], 21), (three_d[1], 9), (two_d[2], 12), (two_d[3], 15),
+            (three_d[2], 11)]
+    ref = [alone(sp, n) for sp, n in reqs]
+    free, _ = serve_memory()
+    for group in ("1", "0"):
+        import os
+        os.environ["FZ_SERVE_GROUP"] = group
+        try:
+            srv = serve.Server(int(0.5 * free))
+        finally:
+            del os.environ["FZ_SERVE_GROUP"]
+        try:
+            ids = [srv.submit(*reqs[0]), srv.submit(*reqs[1])]
+            for k, req in enumerate(reqs[2:]):
+                for _ in range(2 + k):
+                    srv.round()
+                ids.append(srv.submit(*req))
+            while any(srv.status(i)[0] != 2 for i in ids):
+                srv.round()
+            times = {}
+            for i, (fields, sc), (_, n) in zip(ids, ref, reqs):
+                dc = srv.case(i)
+                for k in FIELDS:
+                    assert np.array_equal(dc.get(k), fields[k]), (group, i, k)
+                assert np.array_equal(dc.scalars(), sc), (group, i)
+                times[i] = list(srv.step_times(i))
+                assert len(times[i]) == n
+                srv.release(i)
+            # a batched step is timed once for all its cases: the 2-D requests
+            # share step durations exactly when they were stepped together
+            shared = set(times[ids[0]]) & set(times[ids[2]])
+            assert (len(shared) > 0) == (group == "1"), (group, shared)
+        finally:
+            srv.close()
