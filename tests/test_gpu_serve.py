@@ -287,9 +287,10 @@ def test_same_shaped_requests_are_stepped_in_batches():
                 times[i] = list(srv.step_times(i))
                 assert len(times[i]) == n
                 srv.release(i)
-            # a batched step is timed once for all its cases: the 2-D requests
-            # share step durations exactly when they were stepped together
-            shared = set(times[ids[0]]) & set(times[ids[2]])
-            assert (len(shared) > 0) == (group == "1"), (group, shared)
+            if group == "1":
+                # a batched step is timed once for all its cases: the first two
+                # 2-D requests share the durations of the steps they took together
+                shared = set(times[ids[0]]) & set(times[ids[2]])
+                assert len(shared) >= 5, shared
         finally:
             srv.close()
