@@ -212,9 +212,9 @@ def main_serving(args):
     from fizzy_amd import serve
 
     reqs = serve.serving_requests(args.requests, mean_interarrival=args.arrival_ms / 1e3, nsteps=args.case_steps,
-                                  precision=args.precision, refine=args.refine)
-    if args.no_group:
-        os.environ["FZ_SERVE_GROUP"] = "0"  # read when the server is made
+                                  precision=args.precision, refine=args.refine,
+                                  grids=[g for g in serve.GRIDS if g[2]] if args.grids == "2d" else None)
+    os.environ["FZ_SERVE_GROUP"] = "1" if args.group else "0"  # read when the server is made
     cells = [int(np.prod(spec["ng"])) * n for _, spec, n in reqs]
     f, t = C.c_size_t(), C.c_size_t()
     F.call("fz_device_memory", C.byref(f), C.byref(t))
@@ -253,9 +253,9 @@ def main_serving(args):
         "higher_is_better": False, "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "mixed",
         "data": "synthetic",
         "config": {"workload": "serving_mix", "requests": len(reqs), "steps_per_case": args.case_steps,
-                   "batched_steps": not args.no_group,
+                   "batched_steps": bool(args.group), "grid_set": args.grids,
                    "refinements": args.refine if args.precision == "mixed" else None,
-                   "arrivals": f"Poisson, mean {args.arrival_ms:g} ms", "grids": "64^2, 128^2, 256^2 (2-D), 64^2x128, 128^2x256",
+                   "arrivals": f"Poisson, mean {args.arrival_ms:g} ms", "grids": "64^2, 128^2, 256^2 (2-D)" + ("" if args.grids == "2d" else ", 64^2x128, 128^2x256"),
                    "density_ratio": "uniform [2, 1000]"},
         "cpu_baseline": cpu,
     }
@@ -489,8 +489,10 @@ def main():
     ap.add_argument("--refine", type=int, default=1, help="mixed precision: FP64-residual refinements per solve")
     ap.add_argument("--no-graphs", action="store_true", help="time the steps launched directly, not from graphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
-    ap.add_argument("--no-group", action="store_true",
-                    help="c4: no batched steps of same-shaped cases (each case steps on its own)")
+    ap.add_argument("--grids", choices=["mix", "2d"], default="mix",
+                    help="c4: the five grids of the serving mix, or its three 2-D grids only")
+    ap.add_argument("--group", action="store_true",
+                    help="c4: batched steps of same-shaped small cases (fz_case_batch) in the server")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="c4: precision of the cases")
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()

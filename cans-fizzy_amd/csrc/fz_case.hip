@@ -873,7 +873,9 @@ struct fz_case_batch {
   int *dpar = nullptr;       // each case's pointer state
   void **dsolve = nullptr;   // the correction buffers, work arrays and pivots of the cases
   std::vector<hipEvent_t> ev;
+  std::vector<hipStream_t> own;  // the cases' own streams while the batch steps
   bool mixed = false;
+  bool pending = false;  // a step begun and not ended
 };
 
 // the buffer of every role in the case's current pointer state (what
@@ -1079,55 +1081,104 @@ static int batch_enqueue(fz_case_batch *b) {
   return FZ_OK;
 }
 
+// the batch's stream is the first case's; every other case's context uses it
+// while the batch steps (its own launches: dt scalars, checks), after the
+// case's earlier work on its own stream
+static int batch_enter(fz_case_batch *b) {
+  const int n = (int)b->cs.size();
+  hipStream_t st = b->cs[0]->ctx->st;
+  b->own.assign(n, nullptr);
+  for (int q = 0; q < n; q++) {
+    b->own[q] = b->cs[q]->ctx->st;
+    if (q > 0) {
+      FZ_HIP(hipEventRecord(b->ev[q], b->own[q]));
+      FZ_HIP(hipStreamWaitEvent(st, b->ev[q], 0));
+    }
+  }
+  for (int q = 1; q < n; q++) b->cs[q]->ctx->st = st;
+  return FZ_OK;
+}
+// each case's later work on its own stream after the batch's
+static void batch_leave(fz_case_batch *b) {
+  const int n = (int)b->cs.size();
+  for (int q = 1; q < n; q++) b->cs[q]->ctx->st = b->own[q];
+  if (hipEventRecord(b->ev[0], b->cs[0]->ctx->st) == hipSuccess)
+    for (int q = 1; q < n; q++) (void)hipStreamWaitEvent(b->own[q], b->ev[0], 0);
+}
+
+// the checks enqueued by a step read back: the next dt and the stop conditions
+static int batch_checks(fz_case_batch *b, int *killed, bool *stop) {
+  bool pending = false;
+  for (fz_case *cs : b->cs) pending = pending || cs->check_pending;
+  if (!pending) return FZ_OK;
+  FZ_HIP(hipStreamSynchronize(b->cs[0]->ctx->st));
+  for (size_t q = 0; q < b->cs.size(); q++) {
+    fz_case *cs = b->cs[q];
+    if (!cs->check_pending) continue;
+    int kill = 0;
+    check_finish(cs, &kill);
+    if (kill) {
+      *stop = true;
+      if (killed) killed[q] = 1;
+    }
+  }
+  return FZ_OK;
+}
+
+// one batched step enqueued, with each case's checks when due
+static int batch_step_enqueue(fz_case_batch *b, bool *pending) {
+  FZ_TRY(batch_enqueue(b));
+  *pending = false;
+  for (fz_case *cs : b->cs) {
+    const int istep = (int)cs->sc[S_ISTEP];
+    if (cs->prm.icheck > 0 && istep % cs->prm.icheck == 0) {
+      FZ_TRY(check_enqueue(cs));
+      *pending = true;
+    }
+  }
+  return FZ_OK;
+}
+
 extern "C" int fz_case_batch_step(fz_case_batch *b, int nsteps, int *killed) {
   if (!b || nsteps < 0) return fz_set_error(FZ_ERR_ARG, "fz_case_batch_step: bad argument");
+  if (b->pending) return fz_set_error(FZ_ERR_STATE, "fz_case_batch_step: fz_case_batch_step_end has not been called");
   const int n = (int)b->cs.size();
   if (killed)
     for (int q = 0; q < n; q++) killed[q] = 0;
   for (fz_case *cs : b->cs)
     if (cs->check_pending) return fz_set_error(FZ_ERR_STATE, "fz_case_batch_step: a case has a step not ended");
-  fz_ctx *c0 = b->cs[0]->ctx;
-  hipStream_t st = c0->st;
-  // the batch runs on the first case's stream, after each case's earlier work
-  std::vector<hipStream_t> own(n);
-  for (int q = 0; q < n; q++) {
-    own[q] = b->cs[q]->ctx->st;
-    if (q > 0) {
-      FZ_HIP(hipEventRecord(b->ev[q], own[q]));
-      FZ_HIP(hipStreamWaitEvent(st, b->ev[q], 0));
-    }
-  }
-  for (int q = 1; q < n; q++) b->cs[q]->ctx->st = st;  // the cases' own launches (dt scalars, checks) too
+  FZ_TRY(batch_enter(b));
   int rc = FZ_OK;
   bool stop = false;
   for (int s = 0; s < nsteps && !stop && rc == FZ_OK; s++) {
-    rc = batch_enqueue(b);
     bool pending = false;
-    for (int q = 0; q < n && rc == FZ_OK; q++) {
-      fz_case *cs = b->cs[q];
-      const int istep = (int)cs->sc[S_ISTEP];
-      if (cs->prm.icheck > 0 && istep % cs->prm.icheck == 0) {
-        rc = check_enqueue(cs);
-        pending = true;
-      }
-    }
-    if (rc == FZ_OK && pending) {
-      if (hipStreamSynchronize(st) != hipSuccess) rc = fz_set_error(FZ_ERR_HIP, "fz_case_batch_step: sync");
-      for (int q = 0; q < n && rc == FZ_OK; q++) {
-        fz_case *cs = b->cs[q];
-        if (!cs->check_pending) continue;
-        int kill = 0;
-        check_finish(cs, &kill);
-        if (kill) {
-          stop = true;
-          if (killed) killed[q] = 1;
-        }
-      }
-    }
+    rc = batch_step_enqueue(b, &pending);
+    if (rc == FZ_OK) rc = batch_checks(b, killed, &stop);
   }
-  for (int q = 1; q < n; q++) b->cs[q]->ctx->st = own[q];
-  // each case's later work after the batch's
-  if (hipEventRecord(b->ev[0], st) == hipSuccess)
-    for (int q = 1; q < n; q++) (void)hipStreamWaitEvent(own[q], b->ev[0], 0);
+  batch_leave(b);
   return rc;
+}
+
+extern "C" int fz_case_batch_step_begin(fz_case_batch *b, int *check_pending) {
+  if (!b) return fz_set_error(FZ_ERR_ARG, "fz_case_batch_step_begin: null batch");
+  if (b->pending) return fz_set_error(FZ_ERR_STATE, "fz_case_batch_step_begin: the previous step has not ended");
+  for (fz_case *cs : b->cs)
+    if (cs->check_pending) return fz_set_error(FZ_ERR_STATE, "fz_case_batch_step_begin: a case has a step not ended");
+  FZ_TRY(batch_enter(b));
+  bool pending = false;
+  const int rc = batch_step_enqueue(b, &pending);
+  batch_leave(b);
+  b->pending = rc == FZ_OK;
+  if (check_pending) *check_pending = pending ? 1 : 0;
+  return rc;
+}
+
+extern "C" int fz_case_batch_step_end(fz_case_batch *b, int *killed) {
+  if (!b) return fz_set_error(FZ_ERR_ARG, "fz_case_batch_step_end: null batch");
+  if (killed)
+    for (size_t q = 0; q < b->cs.size(); q++) killed[q] = 0;
+  if (!b->pending) return FZ_OK;
+  b->pending = false;
+  bool stop = false;
+  return batch_checks(b, killed, &stop);
 }

@@ -92,6 +92,8 @@ SIGNATURES = {
     "fz_case_destroy": (C.c_int, [vp]),
     "fz_case_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
     "fz_case_batch_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+    "fz_case_batch_step_begin": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "fz_case_batch_step_end": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "fz_case_batch_destroy": (C.c_int, [vp]),
     "fz_case_ctx": (C.c_int, [vp, C.POINTER(vp)]),
     "fz_case_field": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_size_t)]),

@@ -146,19 +146,21 @@ def bubble_case(n, nz=None, two_d=False, density_ratio=10.0, mu12=(10.0, 1.0)):
 GRIDS = [(64, None, True), (128, None, True), (256, None, True), (64, 128, False), (128, 256, False)]
 
 
-def serving_requests(n, mean_interarrival=0.05, nsteps=200, seed=20261202, precision="fp64", refine=1):
+def serving_requests(n, mean_interarrival=0.05, nsteps=200, seed=20261202, precision="fp64", refine=1,
+                     grids=None):
     """The serving mix: Poisson arrivals (exponential gaps, SplitMix64), grids
     uniform over {64^2, 128^2, 256^2, 64^2x128, 128^2x256}, density ratio
     uniform in [2, 1000], nsteps steps each, in the given precision mode
     (mixed: refine FP64-residual refinements of an FP32 solve, or -1 for the
-    FP64 projection)."""
+    FP64 projection); grids: a subset of GRIDS to draw from instead."""
     g = splitmix64(seed)
     t, out = 0.0, []
     for _ in range(n):
         t += -mean_interarrival * math.log(1.0 - next(g))
-        gi = min(int(next(g) * len(GRIDS)), len(GRIDS) - 1)
+        pool = grids or GRIDS
+        gi = min(int(next(g) * len(pool)), len(pool) - 1)
         ratio = 2.0 + 998.0 * next(g)
-        nxy, nz, two_d = GRIDS[gi]
+        nxy, nz, two_d = pool[gi]
         spec = bubble_case(nxy, nz, two_d, ratio)
         if precision != "fp64":
             spec = dict(spec, precision=precision, refine=refine)

@@ -306,6 +306,11 @@ int fz_case_autotune(fz_case *cs, int trial_steps, char *report, size_t cap);
 typedef struct fz_case_batch fz_case_batch;
 int fz_case_batch_create(fz_case *const *cases, int n, fz_case_batch **batch);
 int fz_case_batch_step(fz_case_batch *batch, int nsteps, int *killed);
+/* one batched step in two halves, as fz_case_step_begin/_end: begin enqueues
+ * it (and the checks due, *check_pending = 1) without waiting; end reads the
+ * checks back, if any, and sets killed[] (n entries, or null) */
+int fz_case_batch_step_begin(fz_case_batch *batch, int *check_pending);
+int fz_case_batch_step_end(fz_case_batch *batch, int *killed);
 int fz_case_batch_destroy(fz_case_batch *batch);
 /* the checkpoint of src/main.f90:597-631: prefix_u.bin, _v, _w, _p, _psi
  * in the load_one format, with the case's time and step */

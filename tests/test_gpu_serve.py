@@ -259,8 +259,8 @@ def test_same_shaped_requests_are_stepped_in_batches():
     two_d = [dict(serve.bubble_case(32, two_d=True, density_ratio=r), icheck=5) for r in (900.0, 40.0, 3.0, 250.0)]
     three_d = [dict(serve.bubble_case(16, 24, density_ratio=r), precision="mixed", refine=-1, icheck=5)
                for r in (500.0, 8.0, 60.0)]
-    reqs = [(two_d[0], 17), (three_d[0], 13), (two_d[1], 21), (three_d[1], 9), (two_d[2], 12), (two_d[3], 15),
-            (three_d[2], 11)]
+    reqs = [(two_d[0], 40), (three_d[0], 30), (two_d[1], 36), (three_d[1], 25), (two_d[2], 31), (two_d[3], 45),
+            (three_d[2], 28)]
     ref = [alone(sp, n) for sp, n in reqs]
     free, _ = serve_memory()
     for group in ("1", "0"):
@@ -271,9 +271,9 @@ def test_same_shaped_requests_are_stepped_in_batches():
         finally:
             del os.environ["FZ_SERVE_GROUP"]
         try:
-            ids = [srv.submit(*reqs[0]), srv.submit(*reqs[1])]
-            for k, req in enumerate(reqs[2:]):
-                for _ in range(2 + k):
+            ids = [srv.submit(*r) for r in reqs[:3]]
+            for k, req in enumerate(reqs[3:]):
+                for _ in range(1 + k // 2):
                     srv.round()
                 ids.append(srv.submit(*req))
             while any(srv.status(i)[0] != 2 for i in ids):
