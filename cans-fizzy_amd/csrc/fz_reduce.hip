@@ -51,9 +51,17 @@ __device__ inline double block_sum(double v) {
   return v;
 }
 
+// Each block reduces one contiguous range of the interior cells (in the
+// x-fastest order), and the XCD remap gives each XCD a contiguous run of
+// blocks: the y and z neighbours a cell reads lie in the ranges of the same
+// XCD, so its L2 holds them (strided ranges sent every neighbour row to
+// another XCD's L2, and the dt reduction fetched 3.7x its bytes).
 #define FOR_CELLS(d)                                                                                     \
   const long ncell = (long)(d).n1 * (d).n2 * (d).n3;                                                     \
-  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < ncell; idx += (long)gridDim.x * blockDim.x) { \
+  const long per_ = (ncell + gridDim.x - 1) / gridDim.x;                                                 \
+  const long b0_ = (long)fz_xcd_block(blockIdx.x, gridDim.x) * per_;                                     \
+  const long b1_ = b0_ + per_ < ncell ? b0_ + per_ : ncell;                                              \
+  for (long idx = b0_ + threadIdx.x; idx < b1_; idx += blockDim.x) {                                     \
     const int i = (int)(idx % (d).n1) + 1;                                                               \
     const long t_ = idx / (d).n1;                                                                        \
     const int j = (int)(t_ % (d).n2) + 1;                                                                \
