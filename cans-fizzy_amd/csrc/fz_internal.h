@@ -97,7 +97,7 @@ struct fz_ctx {
   double *scratch = nullptr;  // device address of this GPU's static reduction scratch (fz_scratch_setup)
   std::map<std::string, struct fz_bcmap *> bcmaps;  // composed boundary maps by signature
   // launch parameters (defaults, environment overrides, or fz_case_autotune)
-  int mom_variant = 0;  // momentum kernel: 0 by precision (8 for FP64, 2 for FP32 phase fields), 1 staged
+  int mom_variant = 0;  // momentum kernel: 0 the default (8), 1 staged
                         // loads, 2 plain, 4 or 5 waves per EU, 8 stencil fields in LDS (k_mom_lds)
   int tk64 = 16;        // rows per chunk of the FP64 Thomas kernel: 8, 16 or 32
   int tk32 = -8;        // the same for the FP32 one; negative: two columns per lane, -8 or -16 (the default,

@@ -363,22 +363,26 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
   const bool in_ij = i >= 1 && i <= d.n1 && j >= 1 && j <= d.n2;
   const int c = (ty + 1) * COR_PX + (tx + 1);
   const BufFld<double> GDP = buf_fld(dp, d), GU = buf_fld(u, d), GV = buf_fld(v, d);
-  // halo'd index of plane position pos (frame cells outside the array read
-  // another row, or zero past its end; no cell uses them)
+  // halo'd index of plane position pos; frame positions outside the array
+  // (no cell uses them: the tiles of the last block column and row hang
+  // past the halo-inclusive range) are not loaded
   unsigned gpos[NPT];
+  bool inr[NPT];
 #pragma unroll
   for (int r = 0; r < NPT; r++) {
     const int pos = min(t + r * NT, PL - 1), py = pos / COR_PX, px = pos - py * COR_PX;
-    gpos[r] = (unsigned)(i0 - 1 + px) + (unsigned)d.sy * (unsigned)(j0 - 1 + py);
+    const int gi = i0 - 1 + px, gj = j0 - 1 + py;
+    inr[r] = gi >= 0 && gi <= d.n1 + 1 && gj >= 0 && gj <= d.n2 + 1;
+    gpos[r] = (unsigned)gi + (unsigned)d.sy * (unsigned)gj;
   }
   double qd[NPT], qu[NPT], qv[NPT];
   auto load_plane = [&](int kp) {
     const unsigned zo = (unsigned)d.sz * (unsigned)kp;
 #pragma unroll
     for (int r = 0; r < NPT; r++) {
-      qd[r] = GDP(gpos[r] + zo, 0);
-      qu[r] = GU(gpos[r] + zo, 0);
-      qv[r] = GV(gpos[r] + zo, 0);
+      qd[r] = inr[r] ? GDP(gpos[r] + zo, 0) : 0.0;
+      qu[r] = inr[r] ? GU(gpos[r] + zo, 0) : 0.0;
+      qv[r] = inr[r] ? GV(gpos[r] + zo, 0) : 0.0;
     }
   };
   auto store_plane = [&](int kp) {
@@ -1446,6 +1450,7 @@ FZ_MOM_FUSED_KERNEL(k_mom_fused_w5, __attribute__((amdgpu_waves_per_eu(5))), fal
 // same bit for bit.
 constexpr int MOM_MX = 32, MOM_MY = 8, MOM_PX = MOM_MX + 2, MOM_PL = MOM_PX * (MOM_MY + 2);
 
+
 // a field's three plane slots in LDS, seen from one cell
 template <typename T>
 struct LdsAt {
@@ -1458,7 +1463,9 @@ struct LdsAt {
   }
 };
 
-template <typename PT, typename HT, int KZ, int WPE, bool BT = false>  // WPE: waves per SIMD the registers must allow
+// WPE: waves per SIMD the registers must allow; STEADY: a step after the
+// first (first = 0) with no bulk-density term (rho_av = 0), known at compile time
+template <typename PT, typename HT, int KZ, int WPE, bool BT = false, bool STEADY = false>
 __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_per_eu(WPE))) k_mom_lds(
     fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
     double drho, double mu, double dmu, double factor1_v, double factor2_v, int first, double factor12_v,
@@ -1476,6 +1483,10 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
   __shared__ double s_u[3 * PL], s_v[3 * PL], s_w[3 * PL];
   __shared__ PT s_psi[3 * PL], s_fx[2 * PL], s_fy[2 * PL], s_fz[3 * PL];
   FZ_BLOCK_XYZB(d, br, cs, bx, by, bz);
+  if constexpr (STEADY) {
+    first = 0;
+    rax = ray = raz = 0.0;
+  }
   if constexpr (BT) {  // this block's case of a batch
     const fz_bcase &B = br.bt[cs];
     stc = B.stc;
@@ -1569,7 +1580,17 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
       const LdsAt<PT> PSI{s_psi, c, sa, sb, sc}, FX{s_fx, c, tb, tb, tc}, FY{s_fy, c, tb, tb, tc},
           FZ{s_fz, c, sa, sb, sc};
       double ru, rv, rw;
-      mom_ad_cell(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, ru, rv, rw);
+      if constexpr (STEADY) {  // one component at a time: fewer stencil values live at once
+        double t0, t1;
+        mom_ad_cell<1>(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, ru, t0, t1);
+        __builtin_amdgcn_sched_barrier(0);
+        mom_ad_cell<2>(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, t0, rv, t1);
+        __builtin_amdgcn_sched_barrier(0);
+        mom_ad_cell<4>(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, t0, t1, rw);
+        __builtin_amdgcn_sched_barrier(0);
+      } else {
+        mom_ad_cell(d, k, dzci, dzfi, dxi, dyi, rho, drho, mu, dmu, FX, FY, FZ, U, V, W, PSI, ru, rv, rw);
+      }
       const double ruo = first ? ru : duo[r], rvo = first ? rv : dvo[r], rwo = first ? rw : dwo[r];
       duo[r] = ru;
       dvo[r] = rv;
@@ -1607,17 +1628,25 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
                         double *vn, double *wn) {
   FZ_TRY(fits_buffers(c, "mom_fused"));
   const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
-  // the context's variant; 0: the faster one by precision at 256x256x512,
-  // LDS staging for FP64 (1386 against 1741 us), the plain kernel for FP32
-  // phase fields (1292 against 1431 us)
+  // the context's variant; 0: LDS staging (at 256x256x512: FP64 1386
+  // against 1741 us, FP32 phase fields 1134 against 1317 us)
   int variant = c->mom_variant;
-  if (variant == 0) variant = sizeof(PT) == 8 ? 8 : 2;
+  if (variant == 0) variant = 8;
   if (variant == 8) {  // stencil fields staged in LDS
     constexpr int KZ = 16;
     const dim3 grid((c->d.n1 + MOM_MX - 1) / MOM_MX, (c->d.n2 + MOM_MY - 1) / MOM_MY, (c->d.n3 + KZ - 1) / KZ);
     // (3 waves per SIMD fit FP32 phase fields' registers, 169 -> 168 VGPRs;
     // FP64 needs 188, and a 3-wave FP64 build spills 36 and runs 2.2x slower)
-    FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", (k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1>),
+    // With FP32 phase fields, a step after the first without a bulk-density
+    // term (every step of the workloads but the first) knows both at compile
+    // time and computes the stencil's components one at a time: 136 VGPRs
+    // instead of 167 plus spills, 1134 against 1317 us for the plain kernel.
+    // In FP64 the same gives 133 VGPRs and three waves per SIMD but is
+    // slower (1532 against 1403-1452 us at two waves, 188 VGPRs).
+    const bool steady = first == 0 && rho_av[0] == 0.0 && rho_av[1] == 0.0 && rho_av[2] == 0.0;
+    auto kern = sizeof(PT) == 4 && steady ? k_mom_lds<PT, HT, KZ, 3, false, true>
+                                          : k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1, false, false>;
+    FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", kern,
               grid, dim3(MOM_MX, MOM_MY), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1],
               rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12, dt_r, stc,
               surf_factor, rho0, bforce[0], bforce[1], bforce[2], gacc[0], gacc[1], gacc[2], rho_av[0], rho_av[1],
@@ -2108,7 +2137,7 @@ template <typename PT, typename HT>
 int fz_batch_mom(fz_ctx *c, fz_bref br, int ncase) {
   FZ_TRY(fits_buffers(c, "mom_fused"));
   int variant = c->mom_variant;
-  if (variant == 0) variant = sizeof(PT) == 8 ? 8 : 2;
+  if (variant == 0) variant = 8;
   const char *name = sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused";
 #define FZ_MOM_NULLS                                                                                              \
   0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0.0, 0.0, (const double *)nullptr, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, \
@@ -2121,8 +2150,8 @@ int fz_batch_mom(fz_ctx *c, fz_bref br, int ncase) {
     dim3 grid((c->d.n1 + MOM_MX - 1) / MOM_MX, (c->d.n2 + MOM_MY - 1) / MOM_MY, (c->d.n3 + KZ - 1) / KZ);
     br.gz = grid.z;
     grid.z *= (unsigned)ncase;
-    FZ_LAUNCH(c, name, (k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1, true>), grid, dim3(MOM_MX, MOM_MY), 0,
-              c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], FZ_MOM_NULLS);
+    FZ_LAUNCH(c, name, (k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1, true, sizeof(PT) == 4>), grid,
+              dim3(MOM_MX, MOM_MY), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], FZ_MOM_NULLS);
   } else {
     auto kern = variant == 4   ? k_mom_fused_w4<PT, HT, true>
                 : variant == 5 ? k_mom_fused_w5<PT, HT, true>
