@@ -19,6 +19,8 @@ pytestmark = pytest.mark.gpu
 
 CASES = {
     "bubble3d_32x32x64": lambda: cases.rising_bubble_3d(ng=(32, 32, 64)),
+    # the plain FP64 rising-bubble configuration at its own size, 64x64x128
+    "c0_64x64x128": lambda: cases.config0(),
     "bubble3d_ratio1000": lambda: cases.rising_bubble_3d(ng=(32, 32, 64), rho12=(1000.0, 1.0), mu12=(10.0, 0.1)),
     "hysing2d_64x2x128": lambda: cases.hysing_2d(),
     "cavity_2x32x32": lambda: cases.lid_driven_cavity(ng=(2, 32, 32)),
