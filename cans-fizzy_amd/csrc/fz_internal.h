@@ -128,11 +128,13 @@ struct fz_prof_scope {
   } while (0)
 
 // Workgroups are dealt round-robin to the 8 XCDs in dispatch order, each XCD
-// with its own L2. Remapping the linear block id so that XCD x runs the
-// contiguous range [x*nb/8, (x+1)*nb/8) keeps the neighbouring rows and
-// planes of a stencil in one L2. Identity when nb is not a multiple of 8.
+// with its own L2. Remapping the linear block id so that XCD x runs one
+// contiguous range of the logical ids keeps the neighbouring rows and planes
+// of a stencil in one L2. With nb = 8q + r, XCD x receives q + (x < r)
+// blocks, and its range starts at x*q + min(x, r): a bijection of [0, nb).
 __device__ inline unsigned fz_xcd_block(unsigned P, unsigned nb) {
-  return (nb & 7u) ? P : (P & 7u) * (nb >> 3) + (P >> 3);
+  const unsigned x = P & 7u, q = nb >> 3, r = nb & 7u;
+  return x * q + (x < r ? x : r) + (P >> 3);
 }
 
 // Block coordinates of a 3-D stencil grid (x blocks, y blocks, one z plane
