@@ -107,6 +107,7 @@ struct fz_ctx {
   int correc_kz = 1;    // correction step: 0 one cell per thread, 1 dp, u, v staged in LDS (k_correc_lds),
                         // 8 or 16 cells per thread marching in z (k_correc_step_mz)
   int correc_my = 4;    // rows of cells per block of the LDS correction kernel: 4, 8 or 16
+  int p2_xdiv = 2;      // x-line transforms run 1 or 2 times fewer lines per block than y lines
   int thomas_pre = 0;   // 1: Thomas solves with the pivots made once at set-up (k_thomas_pre); set at
                         // creation (1 unless thomas_rc), cleared for cases whose solves are FP32
   int thomas_rc = 0;    // 1: Thomas kernels without a pivot array (the elimination re-run per chunk);

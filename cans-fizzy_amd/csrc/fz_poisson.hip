@@ -371,22 +371,30 @@ __global__ void __launch_bounds__(256) k_r2r(const TI *__restrict__ src, LineMap
 // (R: the arithmetic type, which sets the LDS bytes. FP32 blocks keep the
 // FP64 line counts: twice the lines per block, at 1024 threads, measured
 // slower for both the x and the y transforms.)
-template <int M, bool QUARTER, typename R = double>
+template <int M, bool QUARTER, typename R = double, int GDIV = 1>
 struct P2Shape {
   static constexpr int LG = __builtin_ctz(M);
   static constexpr int NP4 = LG / 2;           // radix-4 passes
   static constexpr bool HAS2 = (LG & 1) != 0;  // closing radix-2 pass
   static constexpr int N = QUARTER ? M / 2 : M;  // real line length
   static constexpr int G64 = QUARTER ? (2048 / M < 64 ? 2048 / M : 64) : (4096 / M < 64 ? (4096 / M > 2 ? 4096 / M : 2) : 64);
-  static constexpr int G = G64;
+  // lines whose elements are contiguous in the source and the destination
+  // (x lines) run GDIV times fewer lines per block: more, smaller blocks
+  static constexpr int G = G64 / GDIV >= (QUARTER ? 1 : 2) ? G64 / GDIV : (QUARTER ? 1 : 2);
   static constexpr int NFFT = QUARTER ? G : G / 2;
   static constexpr int NT = NFFT * (M / 4);
   // LDS rows, padded against bank conflicts: one complex every 128 bytes
-  // (SH: log2 of the complex values per 128 bytes), and one real per staged
-  // line, so that strided lines stage without bank conflicts
+  // (SH: log2 of the complex values per 128 bytes). The staged real lines of
+  // a strided source are written and read G lanes to a row (lane l: line
+  // l % G, element l / G); the LDS serves 256 bytes (64 banks of 4 bytes) a
+  // pass, SL values of type R, so a row stride RS = (SL/G) x (an odd number)
+  // puts the lanes of one pass on distinct banks
   static constexpr int SH = sizeof(typename CT<R>::C) == 16 ? 3 : 4;
   static constexpr int MP = M + (M >> SH);  // complex row stride
-  static constexpr int RS = N + 1;          // real row stride
+  static constexpr int SL = 256 / (int)sizeof(R);
+  static constexpr int RSM = SL / G > 1 ? SL / G : 1;
+  static constexpr int RSJ = (N + RSM - 1) / RSM;
+  static constexpr int RS = RSM * (RSJ | 1);  // real row stride
   static constexpr size_t LDSC = (size_t)NFFT * MP * sizeof(typename CT<R>::C);
   static constexpr size_t LDSR = (size_t)G * RS * sizeof(R);
   static constexpr size_t LDS = LDSC > LDSR ? LDSC : LDSR;
@@ -400,15 +408,15 @@ __device__ __forceinline__ int lds_pad(int c) {
 // (WPE: the waves per SIMD its registers must allow; 8 for array sources, a
 // level the LDS allows too, 1 (no cap) for computed sources, whose loads
 // need more registers)
-template <int M, bool QUARTER, typename R, typename TI, typename TO, int WPE>
-__global__ void __launch_bounds__((P2Shape<M, QUARTER, R>::NT)) __attribute__((amdgpu_waves_per_eu(WPE)))
+template <int M, bool QUARTER, typename R, typename TI, typename TO, int WPE, int GDIV>
+__global__ void __launch_bounds__((P2Shape<M, QUARTER, R, GDIV>::NT)) __attribute__((amdgpu_waves_per_eu(WPE)))
     k_r2r_p2(const TI *__restrict__ src, LineMap in, TO *__restrict__ dst,
                                                 LineMap out, int L1, int kind, double scale, int accum,
                                                 const typename CT<R>::C *__restrict__ tw,
                                                 const typename CT<R>::C *__restrict__ q,
                                                 const typename CT<R>::C *__restrict__ t2, FillSrc fs,
                                                 Batch bt) {
-  using S = P2Shape<M, QUARTER, R>;
+  using S = P2Shape<M, QUARTER, R, GDIV>;
   using C = typename CT<R>::C;
   constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4, SH = S::SH, MP = S::MP, RS = S::RS;
   if (bt.src) {  // case blockIdx.z of a batch of same-shaped solves
@@ -541,19 +549,32 @@ struct PlanTabs {
 static PlanTabs<double> tabs_of(const fz_fftplan &P, double) { return {P.tw, P.q, P.t2}; }
 static PlanTabs<float> tabs_of(const fz_fftplan &P, float) { return {P.tw32, P.q32, P.t232}; }
 
+template <int M, bool QUARTER, typename R, typename TI, typename TO, int GDIV>
+static void launch_p2_g(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
+                        TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs,
+                        const Batch &bt) {
+  using S = P2Shape<M, QUARTER, R, GDIV>;
+  dim3 grid((L1 + S::G - 1) / S::G, L2, bt.src ? bt.n : 1);
+  const PlanTabs<R> t = tabs_of(P, R());
+  if (fs.on)
+    FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, 1, GDIV>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1,
+              kind, scale, accum, t.tw, t.q, t.t2, fs, bt);
+  else
+    FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, 8, GDIV>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1,
+              kind, scale, accum, t.tw, t.q, t.t2, fs, bt);
+}
 template <int M, bool QUARTER, typename R, typename TI, typename TO>
 static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
                       TO *dst, LineMap out, int L1, int L2, double scale, int accum, const FillSrc &fs,
                       const Batch &bt) {
-  using S = P2Shape<M, QUARTER, R>;
-  dim3 grid((L1 + S::G - 1) / S::G, L2, bt.src ? bt.n : 1);
-  const PlanTabs<R> t = tabs_of(P, R());
-  if (fs.on)
-    FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, 1>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind,
-              scale, accum, t.tw, t.q, t.t2, fs, bt);
+  // x lines (contiguous on both sides): half the lines per block (x and y
+  // lines of 256 at 16 lines per block: 242 and 168 us; x lines at 8 per
+  // block 223 us, at 4 per block 228 us; y lines at 8 or 32 per block 177
+  // and 175 us)
+  if (in.se == 1 && out.se == 1 && ctx->p2_xdiv == 2)
+    launch_p2_g<M, QUARTER, R, TI, TO, 2>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, fs, bt);
   else
-    FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, 8>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1, kind,
-              scale, accum, t.tw, t.q, t.t2, fs, bt);
+    launch_p2_g<M, QUARTER, R, TI, TO, 1>(ctx, name, P, kind, src, in, dst, out, L1, L2, scale, accum, fs, bt);
 }
 
 static const Batch kNoBatch = {nullptr, nullptr, 0};
