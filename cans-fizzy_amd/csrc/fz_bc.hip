@@ -342,7 +342,11 @@ static void plane_geometry(const fz_dims &d, int dir, int &na, int &nb) {
 // Each written cell holds (source cell or -1, chain): the phases are applied
 // in order, an operation's destination taking its source's pair extended by
 // the operation (a source not written so far is its own original value).
-static fz_bcmap *compose(fz_ctx *c, const std::vector<Phase> &ph, int fi) {
+// xdone: the kernel that produced the field also wrote, for every interior
+// row (j in 1..n2, k in 1..n3), the x halo cells 0 and n1+1 with the values
+// of their periodic images n1 and 1; the cells whose composed value is
+// exactly that copy are left out of the map.
+static fz_bcmap *compose(fz_ctx *c, const std::vector<Phase> &ph, int fi, bool xdone) {
   fz_bcmap *m = new fz_bcmap();
   const fz_dims &d = c->d;
   if (c->nh >= 2147483647L) return m;
@@ -412,6 +416,18 @@ static fz_bcmap *compose(fz_ctx *c, const std::vector<Phase> &ph, int fi) {
     const long pe = wsp[rp.second];
     if (pe >= 0 && wch[pe] >= 0) return m;  // a written cell is also read: not a gather
   }
+  if (xdone) {
+    std::vector<std::pair<long, long>> keep;
+    keep.reserve(recs.size());
+    for (const auto &rp : recs) {
+      const long cell = rp.first, k = cell / d.sz, j = (cell - k * d.sz) / d.sy, i = cell - k * d.sz - j * d.sy;
+      const bool xrow = (i == 0 || i == d.n1 + 1) && j >= 1 && j <= d.n2 && k >= 1 && k <= d.n3;
+      const long image = cell + (i == 0 ? (long)d.n1 : -(long)d.n1);
+      if (xrow && tab.ch[wch[rp.second]].nops == 0 && wsrc[rp.second] == image) continue;
+      keep.push_back(rp);
+    }
+    recs.swap(keep);
+  }
   // chain numbers in order of first use over the sorted cells
   std::vector<int> renum(tab.ch.size(), -1);
   std::vector<BcChain> chains;
@@ -459,20 +475,23 @@ void fz_bcmaps_free(fz_ctx *c) {
   c->bcmaps.clear();
 }
 
-static fz_bcmap *get_map(fz_ctx *c, const std::string &key, const std::vector<Phase> &ph, int fi) {
+static fz_bcmap *get_map(fz_ctx *c, const std::string &key, const std::vector<Phase> &ph, int fi,
+                         bool xdone = false) {
   auto it = c->bcmaps.find(key);
   if (it != c->bcmaps.end()) return it->second;
-  fz_bcmap *m = compose(c, ph, fi);
+  fz_bcmap *m = compose(c, ph, fi, xdone);
   c->bcmaps[key] = m;
   return m;
 }
 
-static std::string signature(const char *kind, const char *cbc, int ncbc, const double *bc, int is_correc, int fi) {
+static std::string signature(const char *kind, const char *cbc, int ncbc, const double *bc, int is_correc, int fi,
+                             bool xdone = false) {
   std::string k(kind);
   k.append(cbc, ncbc);
   k.append((const char *)bc, sizeof(double) * ncbc);
   k.push_back((char)('0' + is_correc));
   k.push_back((char)('0' + fi));
+  k.push_back(xdone ? 'x' : '-');
   return k;
 }
 
@@ -574,8 +593,10 @@ int fz_bounds_group(fz_ctx *c, int n, const FzBcItem *items) {
     bool ok = !g_gather_off;
     fz_bcmap *m[3] = {nullptr, nullptr, nullptr};
     for (int fi = 0; fi < nf && ok; fi++) {
-      m[fi] = get_map(c, signature(it.kind == 0 ? "p" : "uvw", it.cbc, ncbc, it.bc, it.is_correc ? 1 : 0, fi), ph,
-                      fi);
+      m[fi] = get_map(c,
+                      signature(it.kind == 0 ? "p" : "uvw", it.cbc, ncbc, it.bc, it.is_correc ? 1 : 0, fi,
+                                it.xdone != 0),
+                      ph, fi, it.xdone != 0);
       ok = ok && m[fi]->ok;
     }
     if (!ok) {

@@ -33,13 +33,15 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
 int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
 int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
                           const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
-                          double *p_out, unsigned long long *slots);
+                          double *p_out, unsigned long long *slots, int xper);
 int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out);
 template <typename T>
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
                         const double *stc, int first, const double *u, const double *v, const double *w,
                         const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
                         T *psi_out, T *flux_x, T *flux_y, T *flux_z);
+template <typename T>
+int fz_youngs_step(fz_ctx *c, const T *psi, T *normx, T *normy, T *normz, T *kappa, int xper);
 int fz_chkdiv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
 int fz_chkdt_adv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
 double fz_chkdt_finish(const fz_ctx *c, double dti, int is_solve_ns, int is_track_interface, int vof,
@@ -111,13 +113,14 @@ static int bc_fluxes_psi(fz_ctx *c, const fz_params &P, T *fx, T *fy, T *fz, T *
   const FzBcItem it[2] = {{1, P.cbcvel, P.bcvel, 0, {fx, fy, fz}}, {0, P.cbcpsi, P.bcpsi, 0, {psi, nullptr, nullptr}}};
   return fz_bounds_group<T>(c, 2, it);
 }
+// (xdone: the producing kernel wrote the periodic x halos of the interior rows)
 template <typename T>
 static int bc_geometry(fz_ctx *c, const fz_params &P, const char *cbccur, const double *bccur, T *kappa, T *nx, T *ny,
-                       T *nz) {
-  const FzBcItem it[4] = {{0, cbccur, bccur, 0, {kappa, nullptr, nullptr}},
-                          {0, P.cbcnor + 0, P.bcnor + 0, 0, {nx, nullptr, nullptr}},
-                          {0, P.cbcnor + 6, P.bcnor + 6, 0, {ny, nullptr, nullptr}},
-                          {0, P.cbcnor + 12, P.bcnor + 12, 0, {nz, nullptr, nullptr}}};
+                       T *nz, int xdone) {
+  const FzBcItem it[4] = {{0, cbccur, bccur, 0, {kappa, nullptr, nullptr}, xdone},
+                          {0, P.cbcnor + 0, P.bcnor + 0, 0, {nx, nullptr, nullptr}, xdone},
+                          {0, P.cbcnor + 6, P.bcnor + 6, 0, {ny, nullptr, nullptr}, xdone},
+                          {0, P.cbcnor + 12, P.bcnor + 12, 0, {nz, nullptr, nullptr}, xdone}};
   return fz_bounds_group<T>(c, 4, it);
 }
 
@@ -310,21 +313,24 @@ extern "C" int fz_case_set_scalars(fz_case *cs, const double in[12]) {
   return FZ_OK;
 }
 
+// (the Youngs kernel writes the periodic x halos of its four fields when x
+// is periodic, and their boundary update leaves those cells alone)
 static int phase_geometry(fz_case *cs) {
   fz_ctx *c = cs->ctx;
   const fz_params &P = cs->prm;
   double **f = cs->f;
+  const int xp = c->per[0] ? 1 : 0;
   if (cs->precision == FZ_PREC_MIXED) {
     float **g = cs->g;
     if (P.vof) FZ_TRY(fz_vof_phi_f32(c, 1.0 / (2.0 * P.psi_thickness_factor), g[F_PSI], g[F_PHI]));
     else FZ_TRY(fz_acdi_cmpt_phi_f32(c, cs->sc[S_SEPS], g[F_PSI], g[F_PHI]));
-    FZ_TRY(fz_youngs_f32(c, g[F_PHI], g[F_NX], g[F_NY], g[F_NZ], g[F_KAPPA]));
-    return bc_geometry<float>(c, P, cs->cbccur, cs->bccur, g[F_KAPPA], g[F_NX], g[F_NY], g[F_NZ]);
+    FZ_TRY(fz_youngs_step<float>(c, g[F_PHI], g[F_NX], g[F_NY], g[F_NZ], g[F_KAPPA], xp));
+    return bc_geometry<float>(c, P, cs->cbccur, cs->bccur, g[F_KAPPA], g[F_NX], g[F_NY], g[F_NZ], xp);
   }
   if (!P.vof) FZ_TRY(fz_acdi_cmpt_phi(c, cs->sc[S_SEPS], f[F_PSI], f[F_PHI]));
   else FZ_TRY(fz_vof_thinc_cmpt_phi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_PSI], f[F_PHI]));
-  FZ_TRY(fz_cmpt_norm_curv_youngs(c, f[F_PHI], f[F_NX], f[F_NY], f[F_NZ], f[F_KAPPA]));
-  return bc_geometry<double>(c, P, cs->cbccur, cs->bccur, f[F_KAPPA], f[F_NX], f[F_NY], f[F_NZ]);
+  FZ_TRY(fz_youngs_step<double>(c, f[F_PHI], f[F_NX], f[F_NY], f[F_NZ], f[F_KAPPA], xp));
+  return bc_geometry<double>(c, P, cs->cbccur, cs->bccur, f[F_KAPPA], f[F_NX], f[F_NY], f[F_NZ], xp);
 }
 
 // chkdt + the dt selection of src/main.f90:396-404 and :532-550
@@ -464,6 +470,10 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
   const double rho0 = cs->sc[S_RHO0];
   const double dt = a.dt, dt_r = a.dt_r, factor1 = a.factor1, factor2 = a.factor2, factor12 = a.factor12;
   const bool is_first = a.first;
+  // x periodic: the correction kernel writes the x halos of the interior rows
+  // itself (the boundary update leaves those cells alone), as does the
+  // Youngs kernel of phase_geometry
+  const int xp = c->per[0] ? 1 : 0;
   // psio <- psi by rotation: the update writes into the older buffer
   swap_ptr(cs->psio, f[F_PSI]);
   swap_ptr(cs->psio32, g[F_PSI]);
@@ -539,14 +549,14 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
     // correc into the spare velocity buffers, updatep into the buffer of the
     // previous pressure and the speeds of acdi_set_gamma, in one pass
     FZ_TRY(fz_launch_correc_step(c, rho0, dt, a.stc, cs->dp, f[F_P], f[F_U], f[F_V], f[F_W], cs->un, cs->vn, cs->wn,
-                                 f[F_PO], P.vof ? nullptr : cs->gslots));
+                                 f[F_PO], P.vof ? nullptr : cs->gslots, xp));
     swap_ptr(f[F_U], cs->un);
     swap_ptr(f[F_V], cs->vn);
     swap_ptr(f[F_W], cs->wn);
     swap_ptr(f[F_P], f[F_PO]);  // p <- p + dp, po <- p of the step before (= pn)
     {
-      const FzBcItem it[2] = {{1, P.cbcvel, P.bcvel, 1, {f[F_U], f[F_V], f[F_W]}},
-                              {0, P.cbcpre, P.bcpre, 0, {f[F_P], nullptr, nullptr}}};
+      const FzBcItem it[2] = {{1, P.cbcvel, P.bcvel, 1, {f[F_U], f[F_V], f[F_W]}, xp},
+                              {0, P.cbcpre, P.bcpre, 0, {f[F_P], nullptr, nullptr}, xp}};
       FZ_TRY(fz_bounds_group<double>(c, 2, it));
     }
   } else {

@@ -223,13 +223,16 @@ int fz_projection_rhs_solve_mixed(fz_ctx *ctx, double dti, double rho0, const do
                                   const double *w, double *p, int nref, const double *stc = nullptr);
 int fz_projection_device_scalars_ok(fz_ctx *ctx, int mixed);
 const fz_params *fz_case_prm(const fz_case *cs);
-// one boundary update of a group: kind 0 boundp (f[0]), 1 bounduvw (f[0..2])
+// one boundary update of a group: kind 0 boundp (f[0]), 1 bounduvw (f[0..2]);
+// xdone: the producing kernel wrote the periodic x images of the interior
+// rows (the halo cells 0 and n1+1), which the update then leaves alone
 struct FzBcItem {
   int kind;
   const char *cbc;
   const double *bc;
   int is_correc;
   void *f[3];
+  int xdone;
 };
 template <typename T>
 int fz_bounds_group(fz_ctx *c, int n, const FzBcItem *items);

@@ -316,6 +316,69 @@ static int fits_buffers(const fz_ctx *c, const char *who) {
   return fz_set_error(FZ_ERR_UNSUPPORTED, "%s: halo'd fields above 4 GiB are not supported", who);
 }
 
+// A kernel that writes the interior cells of a field that is periodic in x
+// can write the x halo of the cell's row as well (xper): the halo cells 0
+// and n1+1 take the values of the cells n1 and 1, as the periodic boundary
+// update would, and that update then leaves them alone. Interior rows only
+// (j in 1..n2, k in 1..n3). The halo cell shares its cache line with the
+// row's first or last cells, which the same kernel writes, so the line is
+// written once instead of read and written again by the boundary update.
+// The stores go through buffer resources with 32-bit element offsets (the
+// launchers check fits_buffers), which costs a lane one offset instead of a
+// 64-bit address per field.
+struct XHalo {
+  bool lo, hi;  // the cell is n1 (its image is 0) / 1 (its image is n1+1)
+  unsigned n1;
+};
+__device__ __forceinline__ XHalo xhalo_of(const fz_dims &d, int i) {
+  return XHalo{i == d.n1, i == 1, (unsigned)d.n1};
+}
+template <typename T>
+struct BufOut {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ void put(unsigned e, T v) const {
+    if constexpr (sizeof(T) == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(
+          __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0u, 0)), v), r, e * 8u, 0u, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b32(
+          __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0u, 0u, 0)), v), r, e * 4u, 0u, 0);
+  }
+};
+template <typename T>
+__device__ __forceinline__ BufOut<T> buf_out(T *p, const fz_dims &d) {
+  const unsigned bytes = (unsigned)((unsigned long long)d.sz * (unsigned)(d.n3 + 2) * sizeof(T));
+  return BufOut<T>{__builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000)};
+}
+// the halo images of cell e (its halo'd index) for up to four fields
+template <typename T>
+__device__ __forceinline__ void xhalo_put(const XHalo &x, unsigned e, const BufOut<T> &f0, T v0) {
+  if (x.lo) f0.put(e - x.n1, v0);
+  if (x.hi) f0.put(e + x.n1, v0);
+}
+
+// the same after the fact, for kernel variants that do not write the halo:
+// one thread per interior row, up to four fields (blockIdx.y)
+template <typename T>
+__global__ void k_xhalo_copy(fz_dims d, T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2,
+                             T *__restrict__ f3) {
+  const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  T *f = blockIdx.y == 0 ? f0 : (blockIdx.y == 1 ? f1 : (blockIdx.y == 2 ? f2 : f3));
+  if (row >= (long)d.n2 * d.n3 || !f) return;
+  const long q0 = d.sy * (row % d.n2 + 1) + d.sz * (row / d.n2 + 1);
+  const T a = f[q0 + d.n1], b = f[q0 + 1];
+  f[q0] = a;
+  f[q0 + d.n1 + 1] = b;
+}
+template <typename T>
+static int xhalo_copy(fz_ctx *c, T *f0, T *f1, T *f2, T *f3) {
+  const long rows = (long)c->d.n2 * c->d.n3;
+  FZ_LAUNCH(c, "xhalo_copy", k_xhalo_copy<T>, dim3((unsigned)((rows + 255) / 256), 4), dim3(256), 0, c->d, f0, f1,
+            f2, f3);
+  FZ_LAUNCHED();
+  return FZ_OK;
+}
+
 // k_correc_step_mz with the correction dp and the velocities u, v staged in
 // LDS: a block of 64 x 4 cells (of the halo-inclusive range) marches up 16
 // planes and keeps, per field, the planes k and k+1 of its tile with a
@@ -334,7 +397,8 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
                                                               const double *__restrict__ w, double *__restrict__ un,
                                                               double *__restrict__ vn, double *__restrict__ wn,
                                                               double *__restrict__ p_out,
-                                                              unsigned long long *__restrict__ slots, fz_bref br) {
+                                                              unsigned long long *__restrict__ slots, fz_bref br,
+                                                              int xper) {
   constexpr int NT = COR_MX * MY, PL = COR_PX * (MY + 2), NPT = (PL + NT - 1) / NT;
   __shared__ double s_dp[2 * PL], s_u[2 * PL], s_v[2 * PL];
   FZ_BLOCK_XYZB(d, br, cs, bx, by, bz);
@@ -363,6 +427,8 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
   const bool in_ij = i >= 1 && i <= d.n1 && j >= 1 && j <= d.n2;
   const int c = (ty + 1) * COR_PX + (tx + 1);
   const BufFld<double> GDP = buf_fld(dp, d), GU = buf_fld(u, d), GV = buf_fld(v, d);
+  const XHalo xh = xhalo_of(d, i);
+  const BufOut<double> OUN = buf_out(un, d), OVN = buf_out(vn, d), OWN = buf_out(wn, d), OPO = buf_out(p_out, d);
   // halo'd index of plane position pos; frame positions outside the array
   // (no cell uses them: the tiles of the last block column and row hang
   // past the halo-inclusive range) are not loaded
@@ -421,10 +487,22 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
       const double vq =
           (j <= d.n2) ? s_v[sa + c] - factorj * (s_dp[sa + c + COR_PX] - dpc) / rhop : s_v[sa + c];
       const double wq = (k <= d.n3) ? w[q] - dt * dzci[k] * (dpz - dpc) / rhop : w[q];
-      un[q] = uq;
-      vn[q] = vq;
-      wn[q] = wq;
-      p_out[q] = dpc + p[q];
+      const double pq = dpc + p[q];
+      // (xper: in an interior row the halo cells take the values of their
+      // periodic images, which the cells n1 and 1 write)
+      const bool xrow = xper && j >= 1 && j <= d.n2 && k >= 1 && k <= d.n3;
+      if (!xrow || (i >= 1 && i <= d.n1)) {
+        un[q] = uq;
+        vn[q] = vq;
+        wn[q] = wq;
+        p_out[q] = pq;
+      }
+      if (xrow) {
+        xhalo_put(xh, (unsigned)q, OUN, uq);
+        xhalo_put(xh, (unsigned)q, OVN, vq);
+        xhalo_put(xh, (unsigned)q, OWN, wq);
+        xhalo_put(xh, (unsigned)q, OPO, pq);
+      }
       if (slots && in_ij && k >= 1 && k <= d.n3) {
         const double um = s_u[sa + c - 1] - factori * (dpc - s_dp[sa + c - 1]) / rhop;
         const double vm = s_v[sa + c - COR_PX] - factorj * (dpc - s_dp[sa + c - COR_PX]) / rhop;
@@ -453,7 +531,7 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
 
 int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
                           const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
-                          double *p_out, unsigned long long *slots) {
+                          double *p_out, unsigned long long *slots, int xper) {
   if (c->correc_kz == 1) {  // dp, u, v staged in LDS
     FZ_TRY(fits_buffers(c, "correc_step"));
     constexpr int KZ = 16;
@@ -461,7 +539,8 @@ int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, 
     const dim3 grid((c->d.n1 + 2 + COR_MX - 1) / COR_MX, (c->d.n2 + 2 + my - 1) / my, (c->d.n3 + 2 + KZ - 1) / KZ);
 #define FZ_CORREC_LDS(MY)                                                                                          \
   FZ_LAUNCH(c, "correc_step", (k_correc_lds<KZ, MY>), grid, dim3(COR_MX, MY), 0, c->d, c->dzci, dt * c->dli[0],     \
-            dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots, kNoBRef)
+            dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots, kNoBRef, \
+            xper)
     if (my == 16) FZ_CORREC_LDS(16);
     else if (my == 8) FZ_CORREC_LDS(8);
     else FZ_CORREC_LDS(4);
@@ -478,6 +557,7 @@ int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, 
               dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots);
   }
   FZ_LAUNCHED();
+  if (xper && c->correc_kz != 1) return xhalo_copy<double>(c, un, vn, wn, p_out);
   return FZ_OK;
 }
 
@@ -1068,7 +1148,7 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
                                                 const double *__restrict__ dzfi, double dxi_d, double dyi_d,
                                                 const T *__restrict__ psi, T *__restrict__ normx,
                                                 T *__restrict__ normy, T *__restrict__ normz,
-                                                T *__restrict__ kappa, fz_bref br) {
+                                                T *__restrict__ kappa, fz_bref br, int xper) {
   CELL_IJKB(d, br, cs);
   if constexpr (BT) {  // this block's case of a batch
     psi = (const T *)fz_brole(br, cs, BR_PHI);
@@ -1091,19 +1171,26 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
   normy[q] = ny;
   normz[q] = nz;
   kappa[q] = ka;
+  if (xper) {  // the periodic x halo of the row
+    const XHalo xh = xhalo_of(d, i);
+    xhalo_put(xh, (unsigned)q, buf_out(normx, d), nx);
+    xhalo_put(xh, (unsigned)q, buf_out(normy, d), ny);
+    xhalo_put(xh, (unsigned)q, buf_out(normz, d), nz);
+    xhalo_put(xh, (unsigned)q, buf_out(kappa, d), ka);
+  }
 }
 
 // (a batch of ncase cases when br.bt is set: their grids stacked in z)
 template <typename T>
 static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T *normy, T *normz, T *kappa,
-                         fz_bref br = kNoBRef, int ncase = 1) {
+                         fz_bref br = kNoBRef, int ncase = 1, int xper = 0) {
   FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
   dim3 grid = fz_grid_interior(c->d);
   br.gz = grid.z;
   grid.z *= (unsigned)ncase;
   auto kern = br.bt ? k_youngs<T, true> : k_youngs<T, false>;
   FZ_LAUNCH(c, name, kern, grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi, normx,
-            normy, normz, kappa, br);
+            normy, normz, kappa, br, xper);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -1880,6 +1967,15 @@ int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi) {
 int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float *normz, float *kappa) {
   return launch_youngs<float>(c, "youngs_f32", psi, normx, normy, normz, kappa);
 }
+// the normals and curvature of a step, with the periodic x halo when xper
+template <typename T>
+int fz_youngs_step(fz_ctx *c, const T *psi, T *normx, T *normy, T *normz, T *kappa, int xper) {
+  if (!psi || !normx || !normy || !normz || !kappa) return fz_set_error(FZ_ERR_ARG, "youngs: null argument");
+  return launch_youngs<T>(c, sizeof(T) == 4 ? "youngs_f32" : "youngs", psi, normx, normy, normz, kappa, kNoBRef, 1,
+                          xper);
+}
+template int fz_youngs_step<double>(fz_ctx *, const double *, double *, double *, double *, double *, int);
+template int fz_youngs_step<float>(fz_ctx *, const float *, float *, float *, float *, float *, int);
 
 extern "C" int fz_vof_thinc_cmpt_phi(fz_ctx *c, double beta, const double *psi, double *phi) {
   if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_vof_thinc_cmpt_phi: null argument");
@@ -2183,7 +2279,7 @@ int fz_batch_correc_gamma(fz_ctx *c, fz_bref br, int ncase, bool gamma) {
   FZ_LAUNCH(c, "correc_step", (k_correc_lds<KZ, MY, true>), grid, dim3(COR_MX, MY), 0, c->d, c->dzci, 0.0, 0.0, 0.0, \
             c->dli[0], c->dli[1], (const double *)nullptr, 0.0, (const double *)nullptr, (const double *)nullptr,   \
             (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, (double *)nullptr,          \
-            (double *)nullptr, (double *)nullptr, (double *)nullptr, (unsigned long long *)nullptr, b)
+            (double *)nullptr, (double *)nullptr, (double *)nullptr, (unsigned long long *)nullptr, b, 0)
   if (my == 16) FZ_CORREC_LDS_B(16);
   else if (my == 8) FZ_CORREC_LDS_B(8);
   else FZ_CORREC_LDS_B(4);
