@@ -158,6 +158,7 @@ extern "C" int fz_ctx_destroy(fz_ctx *c) {
   }
   for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
   if (c->red) (void)hipFree(c->red);
+  if (c->ftab) (void)hipFree(c->ftab);
   if (c->h_small) (void)hipHostFree(c->h_small);
   if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
   delete c;

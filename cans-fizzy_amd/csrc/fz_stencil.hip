@@ -1833,6 +1833,55 @@ __global__ void k_forcing(fz_dims d, int ftype, double f0, double factor_v, cons
   }
 }
 
+// The forcing's sines and cosines each depend on one coordinate: tables over
+// i, j and k (FT_ROWS rows of nt = max(n1, n2, n3) + 2 values, index = cell
+// index), made once per context with the per-cell kernel's expressions, so
+// k_forcing_tab adds the same values without evaluating them per cell.
+enum { FT_SXC, FT_CXC, FT_SXF, FT_CXF, FT_SYC, FT_CYC, FT_SYF, FT_CYF, FT_SZC, FT_CZC, FT_ROWS };
+__global__ void k_forcing_tables(fz_dims d, int nt, double l1, double l2, double l3, double dl1, double dl2,
+                                 const double *__restrict__ zc, double *__restrict__ tab) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < 1 || n >= nt) return;
+  const double k0 = 2.0;
+  if (n <= d.n1) {
+    const double xxc = (double)((float)n - 0.5f) * dl1 / l1;
+    const double xxf = xxc + 0.5 * dl1 / l1;
+    tab[FT_SXC * nt + n] = fzm::sin(k0 * xxc);
+    tab[FT_CXC * nt + n] = fzm::cos(k0 * xxc);
+    tab[FT_SXF * nt + n] = fzm::sin(k0 * xxf);
+    tab[FT_CXF * nt + n] = fzm::cos(k0 * xxf);
+  }
+  if (n <= d.n2) {
+    const double yyc = (double)((float)n - 0.5f) * dl2 / l2;
+    const double yyf = yyc + 0.5 * dl2 / l2;
+    tab[FT_SYC * nt + n] = fzm::sin(k0 * yyc);
+    tab[FT_CYC * nt + n] = fzm::cos(k0 * yyc);
+    tab[FT_SYF * nt + n] = fzm::sin(k0 * yyf);
+    tab[FT_CYF * nt + n] = fzm::cos(k0 * yyf);
+  }
+  if (n <= d.n3) {
+    const double zzc = zc[n] / l3;
+    tab[FT_SZC * nt + n] = fzm::sin(k0 * zzc);
+    tab[FT_CZC * nt + n] = fzm::cos(k0 * zzc);
+  }
+}
+__global__ void k_forcing_tab(fz_dims d, int ftype, double f0, double factor_v, const double *__restrict__ stc,
+                              int nt, const double *__restrict__ tab, double *__restrict__ u,
+                              double *__restrict__ v, double *__restrict__ w) {
+  CELL_IJK(d);
+  (void)r;
+  const double factor = stc ? stc[FZ_STC_DT] : factor_v;
+  auto T = [&](int row, int n) { return tab[row * nt + n]; };
+  if (ftype == 1) {  // ABC, a=b=c=1
+    u[q] = u[q] + factor * (f0 * (1.0 * T(FT_SZC, k) + 1.0 * T(FT_CYC, j)));
+    v[q] = v[q] + factor * (f0 * (1.0 * T(FT_SXC, i) + 1.0 * T(FT_CZC, k)));
+    w[q] = w[q] + factor * (f0 * (1.0 * T(FT_SYC, j) + 1.0 * T(FT_CXC, i)));
+  } else {
+    u[q] = u[q] + factor * (f0 * T(FT_SXF, i) * T(FT_CYC, j) * T(FT_CZC, k));
+    v[q] = v[q] - factor * (f0 * T(FT_CXC, i) * T(FT_SYF, j) * T(FT_CZC, k));
+  }
+}
+
 // init2fl sphere/cylinder/film branch, src/two_fluid.f90:392-436
 __global__ void k_init_spheres(fz_dims d, const double *__restrict__ zc, double dl1, double dl2, double l1, double l2,
                                double l3, int ip1, int ip2, int ip3, int id1, int id2, int id3, double sdist0,
@@ -2062,8 +2111,23 @@ extern "C" int fz_mom_xyz_oth(fz_ctx *c, double dt_r, const double rho12[2], con
 int fz_launch_forcing(fz_ctx *c, int ftype, double alpha, double dt, const double *stc, double *u, double *v,
                       double *w) {
   const double f0 = alpha * ((2.0 * FZ_PI / c->l[2]) * (2.0 * FZ_PI / c->l[2]));
-  FZ_LAUNCH(c, "forcing", k_forcing, fz_grid_interior(c->d), fz_block3(), 0, c->d, ftype, f0, dt, stc, c->l[0],
-            c->l[1], c->l[2], c->dl[0], c->dl[1], c->zc, u, v, w);
+  const int nt = std::max(c->d.n1, std::max(c->d.n2, c->d.n3)) + 2;
+  if (!c->ftab) {  // the tables, made on the first call outside a graph capture
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    FZ_HIP(hipStreamIsCapturing(c->st, &cap));
+    if (cap == hipStreamCaptureStatusNone) {
+      FZ_HIP(hipMalloc((void **)&c->ftab, sizeof(double) * FT_ROWS * nt));
+      FZ_LAUNCH(c, "forcing_tables", k_forcing_tables, dim3((nt + 255) / 256), dim3(256), 0, c->d, nt, c->l[0],
+                c->l[1], c->l[2], c->dl[0], c->dl[1], c->zc, c->ftab);
+      FZ_LAUNCHED();
+    }
+  }
+  if (c->ftab)
+    FZ_LAUNCH(c, "forcing", k_forcing_tab, fz_grid_interior(c->d), fz_block3(), 0, c->d, ftype, f0, dt, stc, nt,
+              c->ftab, u, v, w);
+  else
+    FZ_LAUNCH(c, "forcing", k_forcing, fz_grid_interior(c->d), fz_block3(), 0, c->d, ftype, f0, dt, stc, c->l[0],
+              c->l[1], c->l[2], c->dl[0], c->dl[1], c->zc, u, v, w);
   FZ_LAUNCHED();
   return FZ_OK;
 }
