@@ -34,14 +34,16 @@ int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
 int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
                           const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
                           double *p_out, unsigned long long *slots, int xper);
-int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out);
+int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out,
+                        unsigned *clear);
+int fz_acdi_cmpt_phi_rf(fz_ctx *c, double seps, const double *psi, double *phi, unsigned *rflag);
 template <typename T>
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
                         const double *stc, int first, const double *u, const double *v, const double *w,
                         const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
                         T *psi_out, T *flux_x, T *flux_y, T *flux_z);
 template <typename T>
-int fz_youngs_step(fz_ctx *c, const T *psi, T *normx, T *normy, T *normz, T *kappa, int xper);
+int fz_youngs_step(fz_ctx *c, const T *psi, T *normx, T *normy, T *normz, T *kappa, int xper, const unsigned *rflag);
 int fz_chkdiv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
 int fz_chkdt_adv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
 double fz_chkdt_finish(const fz_ctx *c, double dti, int is_solve_ns, int is_track_interface, int vof,
@@ -313,6 +315,10 @@ extern "C" int fz_case_set_scalars(fz_case *cs, const double in[12]) {
   return FZ_OK;
 }
 
+// the FP64 phase field's range flag (k_acdi_phi, k_youngs): a word of the
+// case's device scalars, zero at creation
+static unsigned *range_flag(fz_case *cs) { return (unsigned *)(cs->dsc + 15); }
+
 // (the Youngs kernel writes the periodic x halos of its four fields when x
 // is periodic, and their boundary update leaves those cells alone)
 static int phase_geometry(fz_case *cs) {
@@ -324,12 +330,14 @@ static int phase_geometry(fz_case *cs) {
     float **g = cs->g;
     if (P.vof) FZ_TRY(fz_vof_phi_f32(c, 1.0 / (2.0 * P.psi_thickness_factor), g[F_PSI], g[F_PHI]));
     else FZ_TRY(fz_acdi_cmpt_phi_f32(c, cs->sc[S_SEPS], g[F_PSI], g[F_PHI]));
-    FZ_TRY(fz_youngs_step<float>(c, g[F_PHI], g[F_NX], g[F_NY], g[F_NZ], g[F_KAPPA], xp));
+    FZ_TRY(fz_youngs_step<float>(c, g[F_PHI], g[F_NX], g[F_NY], g[F_NZ], g[F_KAPPA], xp, nullptr));
     return bc_geometry<float>(c, P, cs->cbccur, cs->bccur, g[F_KAPPA], g[F_NX], g[F_NY], g[F_NZ], xp);
   }
-  if (!P.vof) FZ_TRY(fz_acdi_cmpt_phi(c, cs->sc[S_SEPS], f[F_PSI], f[F_PHI]));
+  // (ACDI: the phase field's range flag, cleared by the step's gamma_fin)
+  unsigned *rf = P.vof ? nullptr : range_flag(cs);
+  if (!P.vof) FZ_TRY(fz_acdi_cmpt_phi_rf(c, cs->sc[S_SEPS], f[F_PSI], f[F_PHI], rf));
   else FZ_TRY(fz_vof_thinc_cmpt_phi(c, 1.0 / (2.0 * P.psi_thickness_factor), f[F_PSI], f[F_PHI]));
-  FZ_TRY(fz_youngs_step<double>(c, f[F_PHI], f[F_NX], f[F_NY], f[F_NZ], f[F_KAPPA], xp));
+  FZ_TRY(fz_youngs_step<double>(c, f[F_PHI], f[F_NX], f[F_NY], f[F_NZ], f[F_KAPPA], xp, rf));
   return bc_geometry<double>(c, P, cs->cbccur, cs->bccur, f[F_KAPPA], f[F_NX], f[F_NY], f[F_NZ], xp);
 }
 
@@ -564,7 +572,7 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
   }
   if (!P.vof) {
     if (P.is_solve_ns)
-      FZ_TRY(fz_launch_gamma_fin(c, cs->gslots, P.acdi_gam_factor, P.acdi_gam_min, cs->dsc));
+      FZ_TRY(fz_launch_gamma_fin(c, cs->gslots, P.acdi_gam_factor, P.acdi_gam_min, cs->dsc, range_flag(cs)));
     else FZ_TRY(fz_acdi_gamma_dev(c, P.acdi_gam_factor, P.acdi_gam_min, f[F_U], f[F_V], f[F_W], cs->dsc));
   }
   return FZ_OK;

@@ -96,6 +96,7 @@ struct fz_ctx {
   double *h_small = nullptr;  // pinned host landing zone for reduction results
   double *scratch = nullptr;  // device address of this GPU's static reduction scratch (fz_scratch_setup)
   double *ftab = nullptr;     // the forcing's sine/cosine tables by coordinate, made at its first call
+  bool youngs_fast_grid = false;  // dxi, dyi and every dzci in [2^-20, 2^20] (k_youngs' range argument)
   std::map<std::string, struct fz_bcmap *> bcmaps;  // composed boundary maps by signature
   // launch parameters (defaults, environment overrides, or fz_case_autotune)
   int mom_variant = 0;  // momentum kernel: 0 the default (8), 1 staged

@@ -201,9 +201,11 @@ __global__ void __launch_bounds__(256) k_correc_step_mz(fz_dims d, const double 
 
 // gam = max(sqrt(velmax)*gam_factor, gam_min) (src/acdi.f90:57-60, src/main.f90:526-527)
 // from the slots of k_correc_step, which are cleared for the next step
+// (clear: a word set to zero, the range flag of the next step's phase field)
 template <bool BT = false>
 __global__ void __launch_bounds__(FZ_GAMMA_SLOTS) k_gamma_fin(unsigned long long *__restrict__ slots, double factor,
-                                                              double gmin, double *__restrict__ out, fz_bref br) {
+                                                              double gmin, double *__restrict__ out, fz_bref br,
+                                                              unsigned *__restrict__ clear) {
   if constexpr (BT) {  // batched: one block per case
     const fz_bcase &B = br.bt[blockIdx.x];
     slots = B.gslots;
@@ -220,12 +222,15 @@ __global__ void __launch_bounds__(FZ_GAMMA_SLOTS) k_gamma_fin(unsigned long long
   if (threadIdx.x == 0) {
     for (int q = 1; q < FZ_GAMMA_SLOTS / 64; q++) m = fmax(m, sh[q]);
     out[0] = fmax(sqrt(m) * factor, gmin);
+    if (clear) clear[0] = 0u;
   }
 }
 
 
-int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out) {
-  FZ_LAUNCH(c, "gamma_fin", k_gamma_fin<false>, dim3(1), dim3(FZ_GAMMA_SLOTS), 0, slots, factor, gmin, out, kNoBRef);
+int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out,
+                        unsigned *clear) {
+  FZ_LAUNCH(c, "gamma_fin", k_gamma_fin<false>, dim3(1), dim3(FZ_GAMMA_SLOTS), 0, slots, factor, gmin, out, kNoBRef,
+            clear);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -893,8 +898,11 @@ template int fz_launch_acdi_step<float>(fz_ctx *, const double *, double, double
                                         float *, float *, float *);
 
 // acdi_cmpt_phi, src/acdi.f90:239-247 (halo-inclusive)
+// (rflag: set when some phi is neither zero nor of a magnitude in
+// [2^-100, 2^20], the range k_youngs needs to skip its per-lane checks)
 template <typename T, bool BT = false>
-__global__ void k_acdi_phi(long nh, double seps_d, const T *__restrict__ psi, T *__restrict__ phi, fz_bref br) {
+__global__ void k_acdi_phi(long nh, double seps_d, const T *__restrict__ psi, T *__restrict__ phi, fz_bref br,
+                           unsigned *__restrict__ rflag) {
   if constexpr (BT) {  // batched: the case is blockIdx.y
     const unsigned cs = blockIdx.y;
     seps_d = br.bt[cs].seps;
@@ -902,8 +910,13 @@ __global__ void k_acdi_phi(long nh, double seps_d, const T *__restrict__ psi, T 
     phi = (T *)fz_brole(br, cs, BR_PHI);
   }
   const T seps = (T)seps_d;
-  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x)
-    phi[q] = seps * fz_log(fz_div(psi[q] + (T)FZ_EPS, (T)1.0 - psi[q] + (T)FZ_EPS));
+  bool bad = false;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nh; q += (long)gridDim.x * blockDim.x) {
+    const T f = seps * fz_log(fz_div(psi[q] + (T)FZ_EPS, (T)1.0 - psi[q] + (T)FZ_EPS));
+    phi[q] = f;
+    if (rflag) bad = bad || !(f == (T)0 || (fabs(f) >= (T)0x1p-100 && fabs(f) <= (T)0x1p20));
+  }
+  if (rflag && __ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(rflag, 1u);
 }
 
 // epsilon(1._rp) of the phase-field type (the reference's single-precision
@@ -1062,13 +1075,16 @@ __device__ inline double sqrt_inrange(double s) {
 // is at most 2^600 and every non-zero component is at least 2^-300 in
 // magnitude (the denominator then lies in [eps, 2^301]); ok is cleared for a
 // lane outside that range, whose caller must take the plain path.
-template <bool FAST, typename T>
+// (NOCHECK: the caller knows the operands are in the fast sequences' range)
+template <bool FAST, typename T, bool NOCHECK = false>
 __device__ __forceinline__ void unit3(T &x, T &y, T &z, bool &ok) {
   if constexpr (FAST && sizeof(T) == 8) {
     const double s = x * x + y * y + z * z;
-    const int e = min(min(__builtin_amdgcn_frexp_exp(x), __builtin_amdgcn_frexp_exp(y)),
-                      __builtin_amdgcn_frexp_exp(z));
-    ok = ok && s <= 0x1p600 && e >= -299;
+    if constexpr (!NOCHECK) {
+      const int e = min(min(__builtin_amdgcn_frexp_exp(x), __builtin_amdgcn_frexp_exp(y)),
+                        __builtin_amdgcn_frexp_exp(z));
+      ok = ok && s <= 0x1p600 && e >= -299;
+    }
     const double d = sqrt_inrange(s) + (double)FZ_EPS;
     double w = __builtin_amdgcn_rcp(d);
     double t = __builtin_fma(-d, w, 1.0);
@@ -1091,7 +1107,7 @@ __device__ __forceinline__ void unit3(T &x, T &y, T &z, bool &ok) {
 // phase field through the accessor PS (T = float: the mixed-precision step,
 // FP32 fields and arithmetic). FAST: the normalisations by unit3's fast
 // sequences; ok is cleared where they do not apply.
-template <typename T, bool FAST = false, class A>
+template <typename T, bool FAST = false, bool NOCHECK = false, class A>
 __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dyi_d, double dzci_k, double dzci_km,
                                             double dzfi_k, T &onx, T &ony, T &onz, T &okap, bool &ok) {
   const T dxi = (T)dxi_d, dyi = (T)dyi_d;
@@ -1130,11 +1146,11 @@ __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dy
   mz[6] = (T)0.25 * ((ccc + mcc + cpc + mpc) - (ccm + mcm + cpm + mpm)) * zcm;
   mz[7] = (T)0.25 * ((ccc + mcc + cmc + mmc) - (ccm + mcm + cmm + mmm)) * zcm;
 #pragma unroll
-  for (int m = 0; m < 8; m++) unit3<FAST>(mx[m], my[m], mz[m], ok);
+  for (int m = 0; m < 8; m++) unit3<FAST, T, NOCHECK>(mx[m], my[m], mz[m], ok);
   T nx = (T).125 * (mx[0] + mx[1] + mx[2] + mx[3] + mx[4] + mx[5] + mx[6] + mx[7]);
   T ny = (T).125 * (my[0] + my[1] + my[2] + my[3] + my[4] + my[5] + my[6] + my[7]);
   T nz = (T).125 * (mz[0] + mz[1] + mz[2] + mz[3] + mz[4] + mz[5] + mz[6] + mz[7]);
-  unit3<FAST>(nx, ny, nz, ok);
+  unit3<FAST, T, NOCHECK>(nx, ny, nz, ok);
   onx = nx;
   ony = ny;
   onz = nz;
@@ -1148,7 +1164,8 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
                                                 const double *__restrict__ dzfi, double dxi_d, double dyi_d,
                                                 const T *__restrict__ psi, T *__restrict__ normx,
                                                 T *__restrict__ normy, T *__restrict__ normz,
-                                                T *__restrict__ kappa, fz_bref br, int xper) {
+                                                T *__restrict__ kappa, fz_bref br, int xper,
+                                                const unsigned *__restrict__ rflag) {
   CELL_IJKB(d, br, cs);
   if constexpr (BT) {  // this block's case of a batch
     psi = (const T *)fz_brole(br, cs, BR_PHI);
@@ -1160,12 +1177,25 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
   (void)r;
   T nx, ny, nz, ka;
   bool ok = true;
-  youngs_cell<T, true>(buf_at(psi, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
-  if (!ok) {  // the plain sequences, from a fresh read of the phase field (an opaque copy of the
-              // pointer, so that the 27 values need not stay in registers for this rare path)
-    const T *ps = psi;
-    asm volatile("" : "+s"(ps));
-    youngs_cell<T>(buf_at(ps, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
+  if (rflag && *rflag == 0u) {
+    // every phi of the array is zero or of a magnitude in [2^-100, 2^20]
+    // (k_acdi_phi), and the launcher has checked dxi, dyi and dzci against
+    // [2^-20, 2^20]: then every vertex-normal component is zero or above
+    // 2^-174 in magnitude (a non-zero difference of sums of such phi is a
+    // multiple of 2^-152) and below 2^41, each normalised component zero or
+    // above 2^-216, each cell-normal component zero or above 2^-271, all
+    // within the range of the fast sequences, whose per-lane checks are
+    // then left out
+    youngs_cell<T, true, true>(buf_at(psi, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka,
+                               ok);
+  } else {
+    youngs_cell<T, true>(buf_at(psi, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
+    if (!ok) {  // the plain sequences, from a fresh read of the phase field (an opaque copy of the
+                // pointer, so that the 27 values need not stay in registers for this rare path)
+      const T *ps = psi;
+      asm volatile("" : "+s"(ps));
+      youngs_cell<T>(buf_at(ps, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
+    }
   }
   normx[q] = nx;
   normy[q] = ny;
@@ -1183,14 +1213,16 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
 // (a batch of ncase cases when br.bt is set: their grids stacked in z)
 template <typename T>
 static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T *normy, T *normz, T *kappa,
-                         fz_bref br = kNoBRef, int ncase = 1, int xper = 0) {
+                         fz_bref br = kNoBRef, int ncase = 1, int xper = 0, const unsigned *rflag = nullptr) {
   FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
   dim3 grid = fz_grid_interior(c->d);
   br.gz = grid.z;
   grid.z *= (unsigned)ncase;
   auto kern = br.bt ? k_youngs<T, true> : k_youngs<T, false>;
+  // (the range flag only where the grid factors lie in [2^-20, 2^20])
+  if (!c->youngs_fast_grid) rflag = nullptr;
   FZ_LAUNCH(c, name, kern, grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi, normx,
-            normy, normz, kappa, br, xper);
+            normy, normz, kappa, br, xper, rflag);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -1997,18 +2029,22 @@ extern "C" int fz_acdi_transport_pf(fz_ctx *c, double gam, double seps, const do
                                flux_z);
 }
 
-extern "C" int fz_acdi_cmpt_phi(fz_ctx *c, double seps, const double *psi, double *phi) {
-  if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_acdi_cmpt_phi: null argument");
+// (rflag: or null; see k_acdi_phi)
+int fz_acdi_cmpt_phi_rf(fz_ctx *c, double seps, const double *psi, double *phi, unsigned *rflag) {
   FZ_LAUNCH(c, "acdi_phi", k_acdi_phi<double>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi,
-            kNoBRef);
+            kNoBRef, rflag);
   FZ_LAUNCHED();
   return FZ_OK;
+}
+extern "C" int fz_acdi_cmpt_phi(fz_ctx *c, double seps, const double *psi, double *phi) {
+  if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_acdi_cmpt_phi: null argument");
+  return fz_acdi_cmpt_phi_rf(c, seps, psi, phi, nullptr);
 }
 
 // FP32 phase field of the mixed-precision step
 int fz_acdi_cmpt_phi_f32(fz_ctx *c, double seps, const float *psi, float *phi) {
   FZ_LAUNCH(c, "acdi_phi_f32", k_acdi_phi<float>, dim3(flat_grid(c->nh)), dim3(256), 0, c->nh, seps, psi, phi,
-            kNoBRef);
+            kNoBRef, (unsigned *)nullptr);
   FZ_LAUNCHED();
   return FZ_OK;
 }
@@ -2017,14 +2053,17 @@ int fz_youngs_f32(fz_ctx *c, const float *psi, float *normx, float *normy, float
   return launch_youngs<float>(c, "youngs_f32", psi, normx, normy, normz, kappa);
 }
 // the normals and curvature of a step, with the periodic x halo when xper
+// (rflag: the range flag of k_acdi_phi for this phi, or null)
 template <typename T>
-int fz_youngs_step(fz_ctx *c, const T *psi, T *normx, T *normy, T *normz, T *kappa, int xper) {
+int fz_youngs_step(fz_ctx *c, const T *psi, T *normx, T *normy, T *normz, T *kappa, int xper, const unsigned *rflag) {
   if (!psi || !normx || !normy || !normz || !kappa) return fz_set_error(FZ_ERR_ARG, "youngs: null argument");
   return launch_youngs<T>(c, sizeof(T) == 4 ? "youngs_f32" : "youngs", psi, normx, normy, normz, kappa, kNoBRef, 1,
-                          xper);
+                          xper, rflag);
 }
-template int fz_youngs_step<double>(fz_ctx *, const double *, double *, double *, double *, double *, int);
-template int fz_youngs_step<float>(fz_ctx *, const float *, float *, float *, float *, float *, int);
+template int fz_youngs_step<double>(fz_ctx *, const double *, double *, double *, double *, double *, int,
+                                    const unsigned *);
+template int fz_youngs_step<float>(fz_ctx *, const float *, float *, float *, float *, float *, int,
+                                   const unsigned *);
 
 extern "C" int fz_vof_thinc_cmpt_phi(fz_ctx *c, double beta, const double *psi, double *phi) {
   if (!c || !psi || !phi) return fz_set_error(FZ_ERR_ARG, "fz_vof_thinc_cmpt_phi: null argument");
@@ -2283,7 +2322,7 @@ int fz_batch_phase_geometry(fz_ctx *c, fz_bref br, int ncase) {
   if (ncase > 65535) return fz_set_error(FZ_ERR_ARG, "batched step: at most 65535 cases");
   FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_phi_f32" : "acdi_phi", (k_acdi_phi<T, true>),
             dim3((unsigned)flat_grid(c->nh), (unsigned)ncase), dim3(256), 0, c->nh, 0.0, (const T *)nullptr,
-            (T *)nullptr, br);
+            (T *)nullptr, br, (unsigned *)nullptr);
   FZ_LAUNCHED();
   return launch_youngs<T>(c, sizeof(T) == 4 ? "youngs_f32" : "youngs", (const T *)nullptr, (T *)nullptr,
                           (T *)nullptr, (T *)nullptr, (T *)nullptr, br, ncase);
@@ -2351,7 +2390,7 @@ int fz_batch_correc_gamma(fz_ctx *c, fz_bref br, int ncase, bool gamma) {
   FZ_LAUNCHED();
   if (gamma) {
     FZ_LAUNCH(c, "gamma_fin", k_gamma_fin<true>, dim3((unsigned)ncase), dim3(FZ_GAMMA_SLOTS), 0,
-              (unsigned long long *)nullptr, 0.0, 0.0, (double *)nullptr, br);
+              (unsigned long long *)nullptr, 0.0, 0.0, (double *)nullptr, br, (unsigned *)nullptr);
     FZ_LAUNCHED();
   }
   return FZ_OK;
