@@ -865,16 +865,202 @@ __global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_lds(fz_dims d, const do
   }
 }
 
+// One face of acdi_transport_pf (src/acdi.f90:102-224) between the cells a
+// (below) and b (above) along direction DIR (0 x, 1 y, 2 z): the advective
+// flux 0.5 (psi_b + psi_a) vel and the regularisation flux
+// gs (psi_b - psi_a) dfac - g4 (1 - t^2) rn, with the face normal
+// (n_a + n_b) / 2 and t = tanh(0.25 (phi_b + phi_a) / seps). acdi_cell
+// evaluates these expressions at each face from both of its cells, and the
+// two evaluations differ only in the operand order of sums of two terms, so
+// a face evaluated once gives both cells' values bit for bit.
+template <int DIR, typename T>
+__device__ __forceinline__ void acdi_face(T psa, T psb, T pha, T phb, T nxa, T nxb, T nya, T nyb, T nza, T nzb, T vel,
+                                          T dfac, T gs, T g4, T seps, T &upsi, T &rglr) {
+  upsi = (T)0.5 * (psb + psa) * vel;
+  const T dpsi = gs * (psb - psa) * dfac;
+  const T nxf = (T)0.5 * (nxb + nxa), nyf = (T)0.5 * (nyb + nya), nzf = (T)0.5 * (nzb + nza);
+  const T nf = DIR == 0 ? nxf : (DIR == 1 ? nyf : nzf);
+  const T rn = fz_div(nf, fz_sqrt(nxf * nxf + nyf * nyf + nzf * nzf) + (T)FZ_EPS);
+  const T t = fz_tanh(fz_div((T)0.25 * (phb + pha), seps));
+  const T sharp = (T)g4 * (((T)1.0 - t * t) * rn);
+  rglr = dpsi - sharp;
+}
+
+// The ACDI step with each face evaluated once (acdi_cell evaluates all six
+// faces of a cell, so every face twice). The block of 64 x 4 cells marches
+// up 16 planes with the normals, phi and the old psi of the planes k, k+1 of
+// its tile and a one-cell frame in LDS; at each level every cell evaluates
+// its three upper faces (x, y from LDS, z with the plane above), keeps the
+// z face in registers for the level above, and puts the x and y faces in
+// LDS; the faces below the tile's first row and column are evaluated by the
+// first and second waves. After a barrier each cell assembles its
+// right-hand side from its six faces in acdi_cell's order, so the results
+// are those of k_acdi_lds bit for bit. The velocities are read at the cell
+// only (and below the tile's edges), straight from memory.
+template <typename T, int KZ>
+__global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_sf(fz_dims d, const double *__restrict__ dzci,
+                                                           const double *__restrict__ dzfi, double dxi_d,
+                                                           double dyi_d, const double *__restrict__ d_gam,
+                                                           double gam_v, double seps_d, double factor1_v,
+                                                           double factor2_v, const double *__restrict__ stc,
+                                                           int first, const double *__restrict__ u,
+                                                           const double *__restrict__ v, const double *__restrict__ w,
+                                                           const T *__restrict__ normx, const T *__restrict__ normy,
+                                                           const T *__restrict__ normz, const T *__restrict__ phi,
+                                                           const T *__restrict__ psio, T *__restrict__ dpsio,
+                                                           T *__restrict__ psi_out, T *__restrict__ flux_x,
+                                                           T *__restrict__ flux_y, T *__restrict__ flux_z) {
+  constexpr int NT = ACD_MX * ACD_MY, PL = ACD_PL, NPT = (PL + NT - 1) / NT;
+  __shared__ T s_nx[2 * PL], s_ny[2 * PL], s_nz[2 * PL], s_phi[2 * PL], s_psi[2 * PL];
+  // the faces of a level: x faces of the tile's rows (index 0 the face below
+  // the first column, tx+1 the face above cell tx), y faces of its columns
+  // (row 0 the faces below the first row, ty+1 those above row ty);
+  // advective and regularisation parts
+  __shared__ T f_xu[ACD_MY][ACD_MX + 1], f_xr[ACD_MY][ACD_MX + 1];
+  __shared__ T f_yu[ACD_MY + 1][ACD_MX], f_yr[ACD_MY + 1][ACD_MX];
+  unsigned cs_, bx, by, bz;
+  fz_stencil_block_b(d.tile_y, gridDim.z, cs_, bx, by, bz);
+  const double gam_dd = d_gam ? *d_gam : gam_v;
+  const double factor1 = stc ? stc[FZ_STC_F1] : factor1_v, factor2 = stc ? stc[FZ_STC_F2] : factor2_v;
+  const T dxi = (T)dxi_d, dyi = (T)dyi_d, gam = (T)gam_dd, seps = (T)seps_d;
+  const T gs = gam * seps, g4 = (T)0.25 * gam;
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + ACD_MX * ty;
+  const int i0 = (int)bx * ACD_MX + 1, j0 = (int)by * ACD_MY + 1, k0 = (int)bz * KZ + 1;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool act = i <= d.n1 && j <= d.n2;
+  const int kend = min(k0 + KZ - 1, d.n3);
+  const int c = (ty + 1) * ACD_PX + (tx + 1);
+  const BufFld<double> GU = buf_fld(u, d), GV = buf_fld(v, d), GW = buf_fld(w, d);
+  const BufFld<T> GNX = buf_fld(normx, d), GNY = buf_fld(normy, d), GNZ = buf_fld(normz, d), GPH = buf_fld(phi, d),
+                  GPS = buf_fld(psio, d);
+  unsigned gpos[NPT];
+#pragma unroll
+  for (int r = 0; r < NPT; r++) {
+    const int pos = min(t + r * NT, PL - 1), py = pos / ACD_PX, px = pos - py * ACD_PX;
+    gpos[r] = (unsigned)(i0 - 1 + px) + (unsigned)d.sy * (unsigned)(j0 - 1 + py);
+  }
+  T qx[NPT], qy[NPT], qz[NPT], qf[NPT], qs[NPT];
+  auto load_plane = [&](int kp) {
+    const unsigned zo = (unsigned)d.sz * (unsigned)kp;
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+      const unsigned e = gpos[r] + zo;
+      qx[r] = GNX(e, 0);
+      qy[r] = GNY(e, 0);
+      qz[r] = GNZ(e, 0);
+      qf[r] = GPH(e, 0);
+      qs[r] = GPS(e, 0);
+    }
+  };
+  auto store_plane = [&](int kp) {
+    const int so = (kp & 1) * PL;
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+      const int pos = t + r * NT;
+      if (pos < PL) {
+        s_nx[so + pos] = qx[r];
+        s_ny[so + pos] = qy[r];
+        s_nz[so + pos] = qz[r];
+        s_phi[so + pos] = qf[r];
+        s_psi[so + pos] = qs[r];
+      }
+    }
+  };
+  load_plane(k0);
+  store_plane(k0);
+  load_plane(k0 + 1);
+  store_plane(k0 + 1);
+  // the face below the first level, from the cell's values of the plane below
+  T zu = (T)0, zr = (T)0;
+  if (act) {
+    const unsigned e = (unsigned)i + (unsigned)d.sy * (unsigned)j + (unsigned)d.sz * (unsigned)(k0 - 1);
+    const T wb = (T)GW(e, 0), xb = GNX(e, 0), yb = GNY(e, 0), zb = GNZ(e, 0), fb = GPH(e, 0), sb = GPS(e, 0);
+    const unsigned ec = e + (unsigned)d.sz;
+    acdi_face<2, T>(sb, GPS(ec, 0), fb, GPH(ec, 0), xb, GNX(ec, 0), yb, GNY(ec, 0), zb, GNZ(ec, 0), wb,
+                    (T)dzci[k0 - 1], gs, g4, seps, zu, zr);
+  }
+  __syncthreads();
+  for (int k = k0; k <= kend; k++) {
+    if (k < kend) load_plane(k + 2);  // in flight during this level
+    const int s1 = (k & 1) * PL, s2 = ((k + 1) & 1) * PL;
+    const unsigned ek = (unsigned)d.sz * (unsigned)k;
+    T xu = (T)0, xr = (T)0, yu = (T)0, yr = (T)0, wu = (T)0, wr = (T)0, pso = (T)0;
+    if (act) {  // the cell's upper faces
+      const unsigned e = (unsigned)i + (unsigned)d.sy * (unsigned)j + ek;
+      const int a = s1 + c;
+      pso = s_psi[a];
+      acdi_face<0, T>(s_psi[a], s_psi[a + 1], s_phi[a], s_phi[a + 1], s_nx[a], s_nx[a + 1], s_ny[a], s_ny[a + 1],
+                      s_nz[a], s_nz[a + 1], (T)GU(e, 0), dxi, gs, g4, seps, xu, xr);
+      acdi_face<1, T>(s_psi[a], s_psi[a + ACD_PX], s_phi[a], s_phi[a + ACD_PX], s_nx[a], s_nx[a + ACD_PX], s_ny[a],
+                      s_ny[a + ACD_PX], s_nz[a], s_nz[a + ACD_PX], (T)GV(e, 0), dyi, gs, g4, seps, yu, yr);
+      const int b = s2 + c;
+      acdi_face<2, T>(s_psi[a], s_psi[b], s_phi[a], s_phi[b], s_nx[a], s_nx[b], s_ny[a], s_ny[b], s_nz[a], s_nz[b],
+                      (T)GW(e, 0), (T)dzci[k], gs, g4, seps, wu, wr);
+      f_xu[ty][tx + 1] = xu;
+      f_xr[ty][tx + 1] = xr;
+      f_yu[ty + 1][tx] = yu;
+      f_yr[ty + 1][tx] = yr;
+    }
+    if (ty == 0 && i <= d.n1) {  // the y face below the tile's first row, column tx
+      const int a = s1 + c - ACD_PX;
+      const unsigned e = (unsigned)i + (unsigned)d.sy * (unsigned)(j0 - 1) + ek;
+      T fu, fr;
+      acdi_face<1, T>(s_psi[a], s_psi[a + ACD_PX], s_phi[a], s_phi[a + ACD_PX], s_nx[a], s_nx[a + ACD_PX], s_ny[a],
+                      s_ny[a + ACD_PX], s_nz[a], s_nz[a + ACD_PX], (T)GV(e, 0), dyi, gs, g4, seps, fu, fr);
+      f_yu[0][tx] = fu;
+      f_yr[0][tx] = fr;
+    }
+    if (ty == 1 && tx < ACD_MY && j0 + tx <= d.n2) {  // the x face left of the tile's first column, row tx
+      const int a = s1 + (tx + 1) * ACD_PX;
+      const unsigned e = (unsigned)(i0 - 1) + (unsigned)d.sy * (unsigned)(j0 + tx) + ek;
+      T fu, fr;
+      acdi_face<0, T>(s_psi[a], s_psi[a + 1], s_phi[a], s_phi[a + 1], s_nx[a], s_nx[a + 1], s_ny[a], s_ny[a + 1],
+                      s_nz[a], s_nz[a + 1], (T)GU(e, 0), dxi, gs, g4, seps, fu, fr);
+      f_xu[tx][0] = fu;
+      f_xr[tx][0] = fr;
+    }
+    __syncthreads();  // the faces of the level are in LDS; the ring's slot of plane k is free
+    if (act) {
+      const long q = i + d.sy * j + d.sz * (long)k;
+      const long r = (i - 1) + (long)d.n1 * ((j - 1) + (long)d.n2 * (k - 1));
+      const T dzfi_c = (T)dzfi[k];
+      const T upsiim = f_xu[ty][tx], rglrxm = f_xr[ty][tx], vpsijm = f_yu[ty][tx], rglrym = f_yr[ty][tx];
+      const T adv = dxi * (xu - upsiim) + dyi * (yu - vpsijm) + dzfi_c * (wu - zu);
+      const T rglr = (xr - rglrxm) * dxi + (yr - rglrym) * dyi + (wr - zr) * dzfi_c;
+      const T R = -adv + rglr;
+      const T Ro = first ? R : dpsio[r];
+      dpsio[r] = R;
+      flux_x[q] = xu - xr;
+      flux_y[q] = yu - yr;
+      flux_z[q] = wu - wr;
+      const T pn = pso + (T)factor1 * R + (T)factor2 * Ro;
+      psi_out[q] = fmin(fmax((T)0.0, pn), (T)1.0);
+      zu = wu;  // this level's upper z face: the face below the next level
+      zr = wr;
+    }
+    if (k < kend) store_plane(k + 2);
+    __syncthreads();  // the plane k+2 is in place and the face arrays are free
+  }
+}
+
 template <typename T>
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
                         const double *stc, int first, const double *u, const double *v, const double *w,
                         const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
                         T *psi_out, T *flux_x, T *flux_y, T *flux_z) {
   FZ_TRY(fits_buffers(c, "acdi_step"));
-  // 0: the LDS-staged kernel, measured at 256x256x512: FP32 619 against
-  // 663 us, FP64 1047 against 1120 us (140 VGPRs, three waves per SIMD)
-  const int variant = c->acdi_variant ? c->acdi_variant : 1;
-  if (variant == 1) {  // fields staged in LDS
+  // 0: each face evaluated once (3), at 256x256x512 FP64 928 against
+  // 1000-1040 us for the LDS kernel of six faces per cell (1), FP32 547
+  // against 557 us; the LDS kernel measured FP32 619 against 663 us and
+  // FP64 1047 against 1120 us for the plain one (2)
+  const int variant = c->acdi_variant ? c->acdi_variant : 3;
+  if (variant == 3) {  // each face evaluated once
+    constexpr int KZ = 16;
+    const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
+    FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", (k_acdi_sf<T, KZ>), grid, dim3(ACD_MX, ACD_MY), 0,
+              c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc, first, u, v, w,
+              normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
+  } else if (variant == 1) {  // fields staged in LDS
     constexpr int KZ = 16;
     const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
     FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", (k_acdi_lds<T, KZ>), grid, dim3(ACD_MX, ACD_MY), 0,
