@@ -41,7 +41,7 @@ template <typename T>
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
                         const double *stc, int first, const double *u, const double *v, const double *w,
                         const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
-                        T *psi_out, T *flux_x, T *flux_y, T *flux_z);
+                        T *psi_out, T *flux_x, T *flux_y, T *flux_z, int xper);
 template <typename T>
 int fz_youngs_step(fz_ctx *c, const T *psi, T *normx, T *normy, T *normz, T *kappa, int xper, const unsigned *rflag);
 int fz_chkdiv_dev(fz_ctx *c, const double *u, const double *v, const double *w, double *d_out);
@@ -111,8 +111,10 @@ static bool is_f32_field(int id) {
 // grouped boundary updates of the step (one gather launch per group of up
 // to four fields; each field's values as its own boundp/bounduvw call gives)
 template <typename T>
-static int bc_fluxes_psi(fz_ctx *c, const fz_params &P, T *fx, T *fy, T *fz, T *psi) {
-  const FzBcItem it[2] = {{1, P.cbcvel, P.bcvel, 0, {fx, fy, fz}}, {0, P.cbcpsi, P.bcpsi, 0, {psi, nullptr, nullptr}}};
+// (xdone: the ACDI kernel wrote the periodic x halos of the interior rows)
+static int bc_fluxes_psi(fz_ctx *c, const fz_params &P, T *fx, T *fy, T *fz, T *psi, int xdone = 0) {
+  const FzBcItem it[2] = {{1, P.cbcvel, P.bcvel, 0, {fx, fy, fz}, xdone},
+                          {0, P.cbcpsi, P.bcpsi, 0, {psi, nullptr, nullptr}, xdone}};
   return fz_bounds_group<T>(c, 2, it);
 }
 // (xdone: the producing kernel wrote the periodic x halos of the interior rows)
@@ -478,9 +480,10 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
   const double rho0 = cs->sc[S_RHO0];
   const double dt = a.dt, dt_r = a.dt_r, factor1 = a.factor1, factor2 = a.factor2, factor12 = a.factor12;
   const bool is_first = a.first;
-  // x periodic: the correction kernel writes the x halos of the interior rows
-  // itself (the boundary update leaves those cells alone), as does the
-  // Youngs kernel of phase_geometry
+  // x periodic: the correction kernel and the FP32 ACDI kernel write the x
+  // halos of the interior rows themselves (the boundary update leaves those
+  // cells alone), as does the Youngs kernel of phase_geometry (the FP64 ACDI
+  // kernel measured 40 us slower with them, against 31 us of boundary work)
   const int xp = c->per[0] ? 1 : 0;
   // psio <- psi by rotation: the update writes into the older buffer
   swap_ptr(cs->psio, f[F_PSI]);
@@ -496,15 +499,15 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
     FZ_TRY(fz_launch_acdi_step<float>(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS],
                                       factor1, factor2, a.stc, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], g[F_NX], g[F_NY],
                                       g[F_NZ], g[F_PHI], cs->psio32, g[F_DPSIO], g[F_PSI], g[F_FX], g[F_FY],
-                                      g[F_FZ]));
-    FZ_TRY(bc_fluxes_psi<float>(c, P, g[F_FX], g[F_FY], g[F_FZ], g[F_PSI]));
+                                      g[F_FZ], xp));
+    FZ_TRY(bc_fluxes_psi<float>(c, P, g[F_FX], g[F_FY], g[F_FZ], g[F_PSI], xp));
     FZ_TRY(phase_geometry(cs));
   } else if (P.is_track_interface) {
     if (!P.vof) {
       // transport, AB2 update and clip in one pass; the history is updated in place
       FZ_TRY(fz_launch_acdi_step<double>(c, cs->gam_on_device ? cs->dsc : nullptr, cs->sc[S_GAM], cs->sc[S_SEPS],
                                            factor1, factor2, a.stc, is_first ? 1 : 0, f[F_U], f[F_V], f[F_W], f[F_NX], f[F_NY], f[F_NZ],
-                                 f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ]));
+                                 f[F_PHI], cs->psio, f[F_DPSIO], f[F_PSI], f[F_FX], f[F_FY], f[F_FZ], 0));
     } else {
       // vof_thinc_transport_psi, the AB2 update and clip in one pass (src/rk.f90:226-252)
       FZ_TRY(fz_launch_vof_step<double>(c, 1.0 / (2.0 * P.psi_thickness_factor), factor1, factor2, a.stc,

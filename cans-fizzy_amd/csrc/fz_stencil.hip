@@ -909,7 +909,7 @@ __global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_sf(fz_dims d, const dou
                                                            const T *__restrict__ normz, const T *__restrict__ phi,
                                                            const T *__restrict__ psio, T *__restrict__ dpsio,
                                                            T *__restrict__ psi_out, T *__restrict__ flux_x,
-                                                           T *__restrict__ flux_y, T *__restrict__ flux_z) {
+                                                           T *__restrict__ flux_y, T *__restrict__ flux_z, int xper) {
   constexpr int NT = ACD_MX * ACD_MY, PL = ACD_PL, NPT = (PL + NT - 1) / NT;
   __shared__ T s_nx[2 * PL], s_ny[2 * PL], s_nz[2 * PL], s_phi[2 * PL], s_psi[2 * PL];
   // the faces of a level: x faces of the tile's rows (index 0 the face below
@@ -1030,11 +1030,20 @@ __global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_sf(fz_dims d, const dou
       const T R = -adv + rglr;
       const T Ro = first ? R : dpsio[r];
       dpsio[r] = R;
-      flux_x[q] = xu - xr;
-      flux_y[q] = yu - yr;
-      flux_z[q] = wu - wr;
+      const T fx = xu - xr, fy = yu - yr, fz = wu - wr;
+      flux_x[q] = fx;
+      flux_y[q] = fy;
+      flux_z[q] = fz;
       const T pn = pso + (T)factor1 * R + (T)factor2 * Ro;
-      psi_out[q] = fmin(fmax((T)0.0, pn), (T)1.0);
+      const T ps = fmin(fmax((T)0.0, pn), (T)1.0);
+      psi_out[q] = ps;
+      if (xper && (i == 1 || i == d.n1)) {  // the periodic x halo of the row
+        const XHalo xh = xhalo_of(d, i);
+        xhalo_put(xh, (unsigned)q, buf_out(flux_x, d), fx);
+        xhalo_put(xh, (unsigned)q, buf_out(flux_y, d), fy);
+        xhalo_put(xh, (unsigned)q, buf_out(flux_z, d), fz);
+        xhalo_put(xh, (unsigned)q, buf_out(psi_out, d), ps);
+      }
       zu = wu;  // this level's upper z face: the face below the next level
       zr = wr;
     }
@@ -1047,7 +1056,7 @@ template <typename T>
 int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps, double factor1, double factor2,
                         const double *stc, int first, const double *u, const double *v, const double *w,
                         const T *normx, const T *normy, const T *normz, const T *phi, const T *psio, T *dpsio,
-                        T *psi_out, T *flux_x, T *flux_y, T *flux_z) {
+                        T *psi_out, T *flux_x, T *flux_y, T *flux_z, int xper) {
   FZ_TRY(fits_buffers(c, "acdi_step"));
   // 0: each face evaluated once (3), at 256x256x512 FP64 928 against
   // 1000-1040 us for the LDS kernel of six faces per cell (1), FP32 547
@@ -1059,7 +1068,7 @@ int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps,
     const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
     FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", (k_acdi_sf<T, KZ>), grid, dim3(ACD_MX, ACD_MY), 0,
               c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc, first, u, v, w,
-              normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
+              normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z, xper);
   } else if (variant == 1) {  // fields staged in LDS
     constexpr int KZ = 16;
     const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
@@ -1072,16 +1081,18 @@ int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps,
               first, u, v, w, normx, normy, normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z);
   }
   FZ_LAUNCHED();
+  // (the other variants leave the x halo to a copy)
+  if (xper && variant != 3) return xhalo_copy<T>(c, flux_x, flux_y, flux_z, psi_out);
   return FZ_OK;
 }
 template int fz_launch_acdi_step<double>(fz_ctx *, const double *, double, double, double, double, const double *,
                                          int, const double *, const double *, const double *, const double *,
                                          const double *, const double *, const double *, const double *, double *,
-                                         double *, double *, double *, double *);
+                                         double *, double *, double *, double *, int);
 template int fz_launch_acdi_step<float>(fz_ctx *, const double *, double, double, double, double, const double *,
                                         int, const double *, const double *, const double *, const float *,
                                         const float *, const float *, const float *, const float *, float *, float *,
-                                        float *, float *, float *);
+                                        float *, float *, float *, int);
 
 // acdi_cmpt_phi, src/acdi.f90:239-247 (halo-inclusive)
 // (rflag: set when some phi is neither zero nor of a magnitude in
