@@ -78,6 +78,7 @@ extern "C" int fz_ctx_create(fz_ctx **out, const int n[3], const double l[3], co
   if (const char *e = getenv("FZ_CORREC_MY")) c->correc_my = atoi(e);
   if (const char *e = getenv("FZ_P2_XDIV")) c->p2_xdiv = atoi(e);
   if (const char *e = getenv("FZ_ACDI_VARIANT")) c->acdi_variant = atoi(e);
+  if (const char *e = getenv("FZ_YOUNGS_VARIANT")) c->youngs_variant = atoi(e);
   // the Thomas kernel without a pivot array pays off once the columns fill
   // the device (measured: slower at 256x256 columns, faster at 512x512)
   c->thomas_rc = (long)n[0] * n[1] >= 262144 ? 1 : 0;

@@ -1407,17 +1407,107 @@ __global__ void __launch_bounds__(256) k_youngs(fz_dims d, const double *__restr
   }
 }
 
+// the 27 phase-field values of a cell held in registers: planes k-1, k, k+1
+// of its 3 x 3 column neighbourhood
+template <typename T>
+struct RegAt27 {
+  const T *p0, *p1, *p2;
+  __device__ __forceinline__ T operator()(int a, int b, int c) const {
+    const T *p = c < 0 ? p0 : (c == 0 ? p1 : p2);
+    return p[(b + 1) * 3 + (a + 1)];
+  }
+};
+
+// k_youngs with each thread marching up KZ cells in z: the 3 x 3 plane
+// neighbourhoods of k-1, k, k+1 stay in registers, so a level loads the nine
+// values of the plane k+2 (issued before the level's arithmetic) instead of
+// 27. The cell arithmetic is youngs_cell's through a register accessor, the
+// same expressions, the same values.
+template <typename T, int KZ>
+__global__ void __launch_bounds__(256) k_youngs_mz(fz_dims d, const double *__restrict__ dzci,
+                                                   const double *__restrict__ dzfi, double dxi_d, double dyi_d,
+                                                   const T *__restrict__ psi, T *__restrict__ normx,
+                                                   T *__restrict__ normy, T *__restrict__ normz,
+                                                   T *__restrict__ kappa, int xper,
+                                                   const unsigned *__restrict__ rflag) {
+  unsigned cs_, bx, by, bz;
+  fz_stencil_block_b(d.tile_y, gridDim.z, cs_, bx, by, bz);
+  const int i = (int)bx * blockDim.x + threadIdx.x + 1, j = (int)by * blockDim.y + threadIdx.y + 1;
+  const int k0 = (int)bz * KZ + 1, kend = min(k0 + KZ - 1, d.n3);
+  if (i > d.n1 || j > d.n2) return;  // (no barriers below)
+  const BufFld<T> G = buf_fld(psi, d);
+  const unsigned e00 = (unsigned)i + (unsigned)d.sy * (unsigned)j, sy = (unsigned)d.sy, sz = (unsigned)d.sz;
+  auto load = [&](T *pl, int kp) {
+#pragma unroll
+    for (int b = -1; b <= 1; b++)
+#pragma unroll
+      for (int a = -1; a <= 1; a++) pl[(b + 1) * 3 + (a + 1)] = G(e00 + (unsigned)a + (unsigned)b * sy + (unsigned)kp * sz, 0);
+  };
+  T p0[9], p1[9], p2[9], pn[9];
+  load(p0, k0 - 1);
+  load(p1, k0);
+  load(p2, k0 + 1);
+  const bool nocheck = rflag && *rflag == 0u;  // see k_youngs
+  const XHalo xh = xhalo_of(d, i);
+  for (int k = k0; k <= kend; k++) {
+    if (k < kend) load(pn, k + 2);  // in flight during this level
+    const long q = (long)e00 + (long)sz * k;
+    T nx, ny, nz, ka;
+    bool ok = true;
+    const RegAt27<T> R{p0, p1, p2};
+    if (nocheck) {
+      youngs_cell<T, true, true>(R, dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
+    } else {
+      youngs_cell<T, true>(R, dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
+      if (!ok) {  // the plain sequences, from a fresh read of the phase field (see k_youngs)
+        const T *ps = psi;
+        asm volatile("" : "+s"(ps));
+        youngs_cell<T>(buf_at(ps, d, q), dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
+      }
+    }
+    normx[q] = nx;
+    normy[q] = ny;
+    normz[q] = nz;
+    kappa[q] = ka;
+    if (xper) {  // the periodic x halo of the row
+      xhalo_put(xh, (unsigned)q, buf_out(normx, d), nx);
+      xhalo_put(xh, (unsigned)q, buf_out(normy, d), ny);
+      xhalo_put(xh, (unsigned)q, buf_out(normz, d), nz);
+      xhalo_put(xh, (unsigned)q, buf_out(kappa, d), ka);
+    }
+#pragma unroll
+    for (int m = 0; m < 9; m++) {
+      p0[m] = p1[m];
+      p1[m] = p2[m];
+      p2[m] = pn[m];
+    }
+  }
+}
+
 // (a batch of ncase cases when br.bt is set: their grids stacked in z)
 template <typename T>
 static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T *normy, T *normz, T *kappa,
                          fz_bref br = kNoBRef, int ncase = 1, int xper = 0, const unsigned *rflag = nullptr) {
   FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
+  // (the range flag only where the grid factors lie in [2^-20, 2^20])
+  if (!c->youngs_fast_grid) rflag = nullptr;
+  // 0: marching in z for FP32 (274 against 345-361 us at 256x256x512), one
+  // cell per thread for FP64 (the marching FP64 kernel needs 236 VGPRs, two
+  // waves per SIMD, and took 758 against 705-745 us; capped, it spills)
+  const int yv = c->youngs_variant ? c->youngs_variant : (sizeof(T) == 4 ? 1 : 2);
+  if (!br.bt && yv == 1) {  // marching in z
+    constexpr int KZ = 16;
+    dim3 grid = fz_grid_interior(c->d);
+    grid.z = (c->d.n3 + KZ - 1) / KZ;
+    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ>), grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi,
+              normx, normy, normz, kappa, xper, rflag);
+    FZ_LAUNCHED();
+    return FZ_OK;
+  }
   dim3 grid = fz_grid_interior(c->d);
   br.gz = grid.z;
   grid.z *= (unsigned)ncase;
   auto kern = br.bt ? k_youngs<T, true> : k_youngs<T, false>;
-  // (the range flag only where the grid factors lie in [2^-20, 2^20])
-  if (!c->youngs_fast_grid) rflag = nullptr;
   FZ_LAUNCH(c, name, kern, grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi, normx,
             normy, normz, kappa, br, xper, rflag);
   FZ_LAUNCHED();

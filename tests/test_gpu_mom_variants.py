@@ -42,12 +42,15 @@ def test_momentum_variants_agree_bitwise(name, variant, monkeypatch):
     assert np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("env", ["FZ_ACDI_VARIANT=1", "FZ_ACDI_VARIANT=2", "FZ_ACDI_VARIANT=3", "FZ_CORREC_KZ=0",
+@pytest.mark.parametrize("env", ["FZ_ACDI_VARIANT=1", "FZ_ACDI_VARIANT=2", "FZ_ACDI_VARIANT=3", "FZ_YOUNGS_VARIANT=1",
+                                 "FZ_YOUNGS_VARIANT=2", "FZ_CORREC_KZ=0",
                                  "FZ_CORREC_KZ=8"])
 @pytest.mark.parametrize("name", list(SPECS))
 def test_acdi_and_correction_variants_agree_bitwise(name, env, monkeypatch):
-    """The ACDI step staged in LDS or one cell per thread, and the correction
-    step one cell per thread or marching in z, give the defaults' fields."""
+    """The ACDI step with six faces per cell staged in LDS, one cell per
+    thread or each face once, the Youngs kernel marching in z or one cell per
+    thread, and the correction step one cell per thread or marching in z,
+    give the defaults' fields."""
     if name not in _reference:
         _reference[name] = run(SPECS[name], 8, 0, monkeypatch)
     a, sa = _reference[name]
