@@ -105,7 +105,8 @@ struct fz_ctx {
   int tk32 = -8;        // the same for the FP32 one; negative: two columns per lane, -8 or -16 (the default,
                         // -8, is the tuner's choice at 256x256x512 and 256^3)
   int acdi_variant = 0;  // ACDI step: 0 the default (3), 1 fields staged in LDS, 2 one cell per thread, 3 each face once
-  int youngs_variant = 0;  // Youngs kernel: 0 the default (FP32 1, FP64 2), 1 marching in z, 2 one cell per thread
+  int youngs_variant = 0;  // Youngs kernel: 0 the default (FP32 1, FP64 3), 1 marching in z, 2 one cell per thread,
+                           // 3 marching without range checks when the range flag allows, else one cell per thread
                          // (k_acdi_lds), 2 one cell per thread
   int correc_kz = 1;    // correction step: 0 one cell per thread, 1 dp, u, v staged in LDS (k_correc_lds),
                         // 8 or 16 cells per thread marching in z (k_correc_step_mz)

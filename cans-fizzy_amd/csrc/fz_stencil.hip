@@ -1422,8 +1422,12 @@ struct RegAt27 {
 // neighbourhoods of k-1, k, k+1 stay in registers, so a level loads the nine
 // values of the plane k+2 (issued before the level's arithmetic) instead of
 // 27. The cell arithmetic is youngs_cell's through a register accessor, the
-// same expressions, the same values.
-template <typename T, int KZ>
+// same expressions, the same values. MODE 0: the range checks per lane as in
+// k_youngs (left out when the range flag allows); 1: only the form without
+// them, and nothing when the range flag is raised or not given; 2: nothing
+// unless the flag is raised, then mode 0 (so 1 then 2 compute the field in
+// either case, and the register-hungry checked form runs only when needed).
+template <typename T, int KZ, int MODE = 0>
 __global__ void __launch_bounds__(256) k_youngs_mz(fz_dims d, const double *__restrict__ dzci,
                                                    const double *__restrict__ dzfi, double dxi_d, double dyi_d,
                                                    const T *__restrict__ psi, T *__restrict__ normx,
@@ -1435,6 +1439,11 @@ __global__ void __launch_bounds__(256) k_youngs_mz(fz_dims d, const double *__re
   const int i = (int)bx * blockDim.x + threadIdx.x + 1, j = (int)by * blockDim.y + threadIdx.y + 1;
   const int k0 = (int)bz * KZ + 1, kend = min(k0 + KZ - 1, d.n3);
   if (i > d.n1 || j > d.n2) return;  // (no barriers below)
+  if constexpr (MODE == 1) {
+    if (!rflag || *rflag != 0u) return;
+  } else if constexpr (MODE == 2) {
+    if (rflag && *rflag == 0u) return;
+  }
   const BufFld<T> G = buf_fld(psi, d);
   const unsigned e00 = (unsigned)i + (unsigned)d.sy * (unsigned)j, sy = (unsigned)d.sy, sz = (unsigned)d.sz;
   auto load = [&](T *pl, int kp) {
@@ -1455,7 +1464,7 @@ __global__ void __launch_bounds__(256) k_youngs_mz(fz_dims d, const double *__re
     T nx, ny, nz, ka;
     bool ok = true;
     const RegAt27<T> R{p0, p1, p2};
-    if (nocheck) {
+    if (MODE == 1 || nocheck) {
       youngs_cell<T, true, true>(R, dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
     } else {
       youngs_cell<T, true>(R, dxi_d, dyi_d, dzci[k], dzci[k - 1], dzfi[k], nx, ny, nz, ka, ok);
@@ -1494,12 +1503,24 @@ static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T 
   // 0: marching in z for FP32 (274 against 345-361 us at 256x256x512), one
   // cell per thread for FP64 (the marching FP64 kernel needs 236 VGPRs, two
   // waves per SIMD, and took 758 against 705-745 us; capped, it spills)
-  const int yv = c->youngs_variant ? c->youngs_variant : (sizeof(T) == 4 ? 1 : 2);
+  const int yv = c->youngs_variant ? c->youngs_variant : (sizeof(T) == 4 ? 1 : 3);
   if (!br.bt && yv == 1) {  // marching in z
     constexpr int KZ = 16;
     dim3 grid = fz_grid_interior(c->d);
     grid.z = (c->d.n3 + KZ - 1) / KZ;
     FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ>), grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi,
+              normx, normy, normz, kappa, xper, rflag);
+    FZ_LAUNCHED();
+    return FZ_OK;
+  }
+  if (!br.bt && yv == 3 && rflag) {  // the check-free marching kernel, then the checked one if the flag is raised
+    constexpr int KZ = 16;
+    dim3 grid = fz_grid_interior(c->d);
+    grid.z = (c->d.n3 + KZ - 1) / KZ;
+    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ, 1>), grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi,
+              normx, normy, normz, kappa, xper, rflag);
+    FZ_LAUNCHED();
+    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ, 2>), grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi,
               normx, normy, normz, kappa, xper, rflag);
     FZ_LAUNCHED();
     return FZ_OK;
