@@ -76,6 +76,7 @@ extern "C" int fz_ctx_create(fz_ctx **out, const int n[3], const double l[3], co
   if (const char *e = getenv("FZ_MOM_WPE")) c->mom_variant = atoi(e);
   if (const char *e = getenv("FZ_CORREC_KZ")) c->correc_kz = atoi(e);
   if (const char *e = getenv("FZ_CORREC_MY")) c->correc_my = atoi(e);
+  if (c->correc_my != 8 && c->correc_my != 16) c->correc_my = 4;  // the tile heights the kernel is built for
   if (const char *e = getenv("FZ_P2_XDIV")) c->p2_xdiv = atoi(e);
   if (const char *e = getenv("FZ_ACDI_VARIANT")) c->acdi_variant = atoi(e);
   if (const char *e = getenv("FZ_YOUNGS_VARIANT")) c->youngs_variant = atoi(e);

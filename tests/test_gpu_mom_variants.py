@@ -43,14 +43,18 @@ def test_momentum_variants_agree_bitwise(name, variant, monkeypatch):
 
 
 @pytest.mark.parametrize("env", ["FZ_ACDI_VARIANT=1", "FZ_ACDI_VARIANT=2", "FZ_ACDI_VARIANT=3", "FZ_YOUNGS_VARIANT=1",
-                                 "FZ_YOUNGS_VARIANT=2", "FZ_CORREC_KZ=0",
-                                 "FZ_CORREC_KZ=8"])
+                                 "FZ_YOUNGS_VARIANT=2", "FZ_YOUNGS_VARIANT=3", "FZ_CORREC_KZ=0",
+                                 "FZ_CORREC_KZ=8", "FZ_CORREC_MY=8", "FZ_CORREC_MY=16", "FZ_CORREC_MY=5",
+                                 "FZ_P2_XDIV=1", "FZ_TILE_Y=0", "FZ_TILE_Y=1", "FZ_TILE_Y=3"])
 @pytest.mark.parametrize("name", list(SPECS))
 def test_acdi_and_correction_variants_agree_bitwise(name, env, monkeypatch):
     """The ACDI step with six faces per cell staged in LDS, one cell per
-    thread or each face once, the Youngs kernel marching in z or one cell per
-    thread, and the correction step one cell per thread or marching in z,
-    give the defaults' fields."""
+    thread or each face once, the Youngs kernel marching in z, one cell per
+    thread or the check-free pair, the correction step one cell per thread,
+    marching in z or with other tile heights (5 is not one it is built for and
+    falls back to 4), x-line transforms with twice the default's lines per
+    block, and the stencils' block order plain or in pencils of 1 or 3 block
+    rows, give the defaults' fields."""
     if name not in _reference:
         _reference[name] = run(SPECS[name], 8, 0, monkeypatch)
     a, sa = _reference[name]
