@@ -113,7 +113,9 @@ extern "C" int fz_ctx_create(fz_ctx **out, const int n[3], const double l[3], co
   // phase field's range flag) only where its grid factors lie in [2^-20, 2^20]
   bool fast = true;
   for (int d = 0; d < 2; d++) fast = fast && c->dli[d] >= 0x1p-20 && c->dli[d] <= 0x1p20;
-  for (int k = 0; k < nz; k++) fast = fast && c->h_dzci[k] >= 0x1p-20 && c->h_dzci[k] <= 0x1p20;
+  for (int k = 0; k < nz; k++)
+    fast = fast && c->h_dzci[k] >= 0x1p-20 && c->h_dzci[k] <= 0x1p20 && c->h_dzfi[k] >= 0x1p-20 &&
+           c->h_dzfi[k] <= 0x1p20;
   c->youngs_fast_grid = fast;
   memcpy(c->cbcpre, cbcpre, 6);
   for (int d = 0; d < 3; d++) c->per[d] = (cbcpre[2 * d] == 'P' && cbcpre[2 * d + 1] == 'P');

@@ -1249,6 +1249,12 @@ __global__ void k_clip(long nh, double rmin, double rmax, T *__restrict__ p) {
     p[q] = fmin(fmax((T)rmin, p[q]), (T)rmax);
 }
 
+// 0.25 * x for a normal x of magnitude at least 2^-1020, as a decrement of the
+// exponent field (integer arithmetic: a uniform x stays on the scalar unit)
+__device__ inline double quarter_normal(double x) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) - (2ull << 52));
+}
+
 // sqrt(s) by the device's correctly rounded FP64 sequence without its range
 // scaling: the same result for s = 0 and s in [2^-767, 2^1023]
 __device__ inline double sqrt_inrange(double s) {
@@ -1272,7 +1278,10 @@ __device__ inline double sqrt_inrange(double s) {
 // is at most 2^600 and every non-zero component is at least 2^-300 in
 // magnitude (the denominator then lies in [eps, 2^301]); ok is cleared for a
 // lane outside that range, whose caller must take the plain path.
-// (NOCHECK: the caller knows the operands are in the fast sequences' range)
+// (NOCHECK: the caller knows the operands are in the fast sequences' range
+// and that no component is -0; the fix-up then changes nothing and is left out:
+// for finite non-zero operands it only restores the quotient's sign, and a +0
+// numerator gives +0 through the sequence by itself)
 template <bool FAST, typename T, bool NOCHECK = false>
 __device__ __forceinline__ void unit3(T &x, T &y, T &z, bool &ok) {
   if constexpr (FAST && sizeof(T) == 8) {
@@ -1289,9 +1298,15 @@ __device__ __forceinline__ void unit3(T &x, T &y, T &z, bool &ok) {
     t = __builtin_fma(-d, w, 1.0);
     w = __builtin_fma(w, t, w);
     const double qx = x * w, qy = y * w, qz = z * w;
-    x = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qx, x), w, qx), d, x);
-    y = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qy, y), w, qy), d, y);
-    z = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qz, z), w, qz), d, z);
+    if constexpr (NOCHECK) {
+      x = __builtin_fma(__builtin_fma(-d, qx, x), w, qx);
+      y = __builtin_fma(__builtin_fma(-d, qy, y), w, qy);
+      z = __builtin_fma(__builtin_fma(-d, qz, z), w, qz);
+    } else {
+      x = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qx, x), w, qx), d, x);
+      y = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qy, y), w, qy), d, y);
+      z = __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, qz, z), w, qz), d, z);
+    }
   } else {
     const T nrm = fz_sqrt(x * x + y * y + z * z) + (T)FZ_EPS;
     x = fz_div(x, nrm);
@@ -1316,32 +1331,46 @@ __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dy
   const T mmp = P_(-1, -1, 1), mcp = P_(-1, 0, 1), mpp = P_(-1, 1, 1), cmp = P_(0, -1, 1), ccp = P_(0, 0, 1),
                cpp = P_(0, 1, 1), pmp = P_(1, -1, 1), pcp = P_(1, 0, 1), ppp = P_(1, 1, 1);
 #undef P_
-  const T zc0 = (T)dzci_k, zcm = (T)dzci_km;
+  // NOCHECK (FP64): every difference of sums below is zero or at least
+  // 2^-152 in magnitude, so 0.25 * a is exact and 0.25 * a * f equals
+  // a * (0.25 * f) bit for bit (0.25 * f is exact for the grid factors in
+  // [2^-20, 2^20] the launcher requires): one product per component, with
+  // the quartered factors made on the scalar unit
+  constexpr bool FOLD = NOCHECK && sizeof(T) == 8;
+  T fx = dxi, fy = dyi, zc0 = (T)dzci_k, zcm = (T)dzci_km, fzf = (T)dzfi_k;
+  if constexpr (FOLD) {
+    fx = quarter_normal(dxi_d);
+    fy = quarter_normal(dyi_d);
+    zc0 = quarter_normal(dzci_k);
+    zcm = quarter_normal(dzci_km);
+    fzf = quarter_normal(dzfi_k);
+  }
+  auto S = [](T a, T f) { return FOLD ? a * f : (T)0.25 * a * f; };
   T mx[8], my[8], mz[8];
-  mx[0] = (T)0.25 * ((pcc + ppc + pcp + ppp) - (ccc + cpc + ccp + cpp)) * dxi;
-  mx[1] = (T)0.25 * ((pcc + pmc + pcp + pmp) - (ccc + cmc + ccp + cmp)) * dxi;
-  mx[2] = (T)0.25 * ((pcc + ppc + pcm + ppm) - (ccc + cpc + ccm + cpm)) * dxi;
-  mx[3] = (T)0.25 * ((pcc + pmc + pcm + pmm) - (ccc + cmc + ccm + cmm)) * dxi;
-  mx[4] = (T)0.25 * ((ccc + cpc + ccp + cpp) - (mcc + mpc + mcp + mpp)) * dxi;
-  mx[5] = (T)0.25 * ((ccc + cmc + ccp + cmp) - (mcc + mmc + mcp + mmp)) * dxi;
-  mx[6] = (T)0.25 * ((ccc + cpc + ccm + cpm) - (mcc + mpc + mcm + mpm)) * dxi;
-  mx[7] = (T)0.25 * ((ccc + cmc + ccm + cmm) - (mcc + mmc + mcm + mmm)) * dxi;
-  my[0] = (T)0.25 * ((cpc + ppc + cpp + ppp) - (ccc + pcc + ccp + pcp)) * dyi;
-  my[1] = (T)0.25 * ((ccc + pcc + ccp + pcp) - (cmc + pmc + cmp + pmp)) * dyi;
-  my[2] = (T)0.25 * ((cpc + ppc + cpm + ppm) - (ccc + pcc + ccm + pcm)) * dyi;
-  my[3] = (T)0.25 * ((ccc + pcc + ccm + pcm) - (cmc + pmc + cmm + pmm)) * dyi;
-  my[4] = (T)0.25 * ((cpc + mpc + cpp + mpp) - (ccc + mcc + ccp + mcp)) * dyi;
-  my[5] = (T)0.25 * ((ccc + mcc + ccp + mcp) - (cmc + mmc + cmp + mmp)) * dyi;
-  my[6] = (T)0.25 * ((cpc + mpc + cpm + mpm) - (ccc + mcc + ccm + mcm)) * dyi;
-  my[7] = (T)0.25 * ((ccc + mcc + ccm + mcm) - (cmc + mmc + cmm + mmm)) * dyi;
-  mz[0] = (T)0.25 * ((ccp + pcp + cpp + ppp) - (ccc + pcc + cpc + ppc)) * zc0;
-  mz[1] = (T)0.25 * ((ccp + pcp + cmp + pmp) - (ccc + pcc + cmc + pmc)) * zc0;
-  mz[2] = (T)0.25 * ((ccc + pcc + cpc + ppc) - (ccm + pcm + cpm + ppm)) * zcm;
-  mz[3] = (T)0.25 * ((ccc + pcc + cmc + pmc) - (ccm + pcm + cmm + pmm)) * zcm;
-  mz[4] = (T)0.25 * ((ccp + mcp + cpp + mpp) - (ccc + mcc + cpc + mpc)) * zc0;
-  mz[5] = (T)0.25 * ((ccp + mcp + cmp + mmp) - (ccc + mcc + cmc + mmc)) * zc0;
-  mz[6] = (T)0.25 * ((ccc + mcc + cpc + mpc) - (ccm + mcm + cpm + mpm)) * zcm;
-  mz[7] = (T)0.25 * ((ccc + mcc + cmc + mmc) - (ccm + mcm + cmm + mmm)) * zcm;
+  mx[0] = S((pcc + ppc + pcp + ppp) - (ccc + cpc + ccp + cpp), fx);
+  mx[1] = S((pcc + pmc + pcp + pmp) - (ccc + cmc + ccp + cmp), fx);
+  mx[2] = S((pcc + ppc + pcm + ppm) - (ccc + cpc + ccm + cpm), fx);
+  mx[3] = S((pcc + pmc + pcm + pmm) - (ccc + cmc + ccm + cmm), fx);
+  mx[4] = S((ccc + cpc + ccp + cpp) - (mcc + mpc + mcp + mpp), fx);
+  mx[5] = S((ccc + cmc + ccp + cmp) - (mcc + mmc + mcp + mmp), fx);
+  mx[6] = S((ccc + cpc + ccm + cpm) - (mcc + mpc + mcm + mpm), fx);
+  mx[7] = S((ccc + cmc + ccm + cmm) - (mcc + mmc + mcm + mmm), fx);
+  my[0] = S((cpc + ppc + cpp + ppp) - (ccc + pcc + ccp + pcp), fy);
+  my[1] = S((ccc + pcc + ccp + pcp) - (cmc + pmc + cmp + pmp), fy);
+  my[2] = S((cpc + ppc + cpm + ppm) - (ccc + pcc + ccm + pcm), fy);
+  my[3] = S((ccc + pcc + ccm + pcm) - (cmc + pmc + cmm + pmm), fy);
+  my[4] = S((cpc + mpc + cpp + mpp) - (ccc + mcc + ccp + mcp), fy);
+  my[5] = S((ccc + mcc + ccp + mcp) - (cmc + mmc + cmp + mmp), fy);
+  my[6] = S((cpc + mpc + cpm + mpm) - (ccc + mcc + ccm + mcm), fy);
+  my[7] = S((ccc + mcc + ccm + mcm) - (cmc + mmc + cmm + mmm), fy);
+  mz[0] = S((ccp + pcp + cpp + ppp) - (ccc + pcc + cpc + ppc), zc0);
+  mz[1] = S((ccp + pcp + cmp + pmp) - (ccc + pcc + cmc + pmc), zc0);
+  mz[2] = S((ccc + pcc + cpc + ppc) - (ccm + pcm + cpm + ppm), zcm);
+  mz[3] = S((ccc + pcc + cmc + pmc) - (ccm + pcm + cmm + pmm), zcm);
+  mz[4] = S((ccp + mcp + cpp + mpp) - (ccc + mcc + cpc + mpc), zc0);
+  mz[5] = S((ccp + mcp + cmp + mmp) - (ccc + mcc + cmc + mmc), zc0);
+  mz[6] = S((ccc + mcc + cpc + mpc) - (ccm + mcm + cpm + mpm), zcm);
+  mz[7] = S((ccc + mcc + cmc + mmc) - (ccm + mcm + cmm + mmm), zcm);
 #pragma unroll
   for (int m = 0; m < 8; m++) unit3<FAST, T, NOCHECK>(mx[m], my[m], mz[m], ok);
   T nx = (T).125 * (mx[0] + mx[1] + mx[2] + mx[3] + mx[4] + mx[5] + mx[6] + mx[7]);
@@ -1351,9 +1380,10 @@ __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dy
   onx = nx;
   ony = ny;
   onz = nz;
-  okap = -((T)0.25 * ((mx[0] + mx[1] + mx[2] + mx[3]) - (mx[4] + mx[5] + mx[6] + mx[7])) * dxi +
-               (T)0.25 * ((my[0] + my[2] + my[4] + my[6]) - (my[1] + my[3] + my[5] + my[7])) * dyi +
-               (T)0.25 * ((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7])) * (T)dzfi_k);
+  // (the differences here are zero or at least 2^-268 in magnitude on the NOCHECK path)
+  okap = -(S((mx[0] + mx[1] + mx[2] + mx[3]) - (mx[4] + mx[5] + mx[6] + mx[7]), fx) +
+           S((my[0] + my[2] + my[4] + my[6]) - (my[1] + my[3] + my[5] + my[7]), fy) +
+           S((mz[0] + mz[1] + mz[4] + mz[5]) - (mz[2] + mz[3] + mz[6] + mz[7]), fzf));
 }
 
 template <typename T, bool BT = false>
@@ -1684,6 +1714,48 @@ __global__ void __launch_bounds__(256) k_mom_ad(fz_dims d, const double *__restr
   dwdt[r] = rw;
 }
 
+// The velocity update of the momentum step at one cell, src/rk.f90:96-98 and
+// :125-127 fused: (a1 / d + a2 / d) + a3 / d per component, a1 = u rho_n,
+// a2 = factor1 dudtrk + factor2 dudtrko, a3 = factor12 dudtrk (the second
+// right-hand side), d = rho_p. The nine quotients share three refined
+// reciprocals (div_inrange's sequence, then the device's division fix-up,
+// which sets zero, infinite and NaN quotients as '/' does); where a finite
+// non-zero operand of the cell lies outside [2^-300, 2^300] in magnitude,
+// outside that sequence's range, the cell divides with '/'. Bit for bit '/'.
+__device__ __forceinline__ void mom_update(double au1, double au2, double au3, double du, double av1, double av2,
+                                           double av3, double dv, double aw1, double aw2, double aw3, double dw,
+                                           double &uo, double &vo, double &wo) {
+  int lo = 0, hi = 0;  // frexp exponents: 0 for zeros, infinities and NaN
+  auto span = [&](double x) {
+    const int e = __builtin_amdgcn_frexp_exp(x);
+    lo = min(lo, e);
+    hi = max(hi, e);
+  };
+  span(au1), span(au2), span(au3), span(du), span(av1), span(av2), span(av3), span(dv);
+  span(aw1), span(aw2), span(aw3), span(dw);
+  if (lo >= -299 && hi <= 301) {
+    auto q3 = [](double a1, double a2, double a3, double d) {
+      double y = __builtin_amdgcn_rcp(d);
+      double e = __builtin_fma(-d, y, 1.0);
+      y = __builtin_fma(y, e, y);
+      e = __builtin_fma(-d, y, 1.0);
+      y = __builtin_fma(y, e, y);
+      auto quo = [&](double a) {
+        const double q = a * y;
+        return __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-d, q, a), y, q), d, a);
+      };
+      return (quo(a1) + quo(a2)) + quo(a3);
+    };
+    uo = q3(au1, au2, au3, du);
+    vo = q3(av1, av2, av3, dv);
+    wo = q3(aw1, aw2, aw3, dw);
+  } else {
+    uo = (au1 / du + au2 / du) + au3 / du;
+    vo = (av1 / dv + av2 / dv) + av3 / dv;
+    wo = (aw1 / dw + aw2 / dw) + aw3 / dw;
+  }
+}
+
 // mom_xyz_oth at one cell, src/mom.f90:912-1038 (constant-coefficient split, Brackbill)
 template <class AD, class AP, class AK = AP>
 __device__ __forceinline__ void mom_oth_cell(int k, const double *__restrict__ dzci, double dxi, double dyi,
@@ -1828,14 +1900,17 @@ __device__ __forceinline__ void mom_fused_body(
   const double rhox_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(1, 0, 0));
   const double rhoy_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(0, 1, 0));
   const double rhoz_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(0, 0, 1));
-  double uu = U(0, 0, 0) * rhox_n / rhox_p + (factor1 * ru + factor2 * ruo) / rhox_p;
-  double vv = V(0, 0, 0) * rhoy_n / rhoy_p + (factor1 * rv + factor2 * rvo) / rhoy_p;
-  double ww = W(0, 0, 0) * rhoz_n / rhoz_p + (factor1 * rw + factor2 * rwo) / rhoz_p;
+  const double nu1 = U(0, 0, 0) * rhox_n, nu2 = factor1 * ru + factor2 * ruo;
+  const double nv1 = V(0, 0, 0) * rhoy_n, nv2 = factor1 * rv + factor2 * rvo;
+  const double nw1 = W(0, 0, 0) * rhoz_n, nw2 = factor1 * rw + factor2 * rwo;
   mom_oth_cell(k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz, P, PSI,
                KAP, P, PO, ru, rv, rw);
-  un[q] = uu + factor12 * ru / rhox_p;
-  vn[q] = vv + factor12 * rv / rhoy_p;
-  wn[q] = ww + factor12 * rw / rhoz_p;
+  double uo, vo, wo;
+  mom_update(nu1, nu2, factor12 * ru, rhox_p, nv1, nv2, factor12 * rv, rhoy_p, nw1, nw2, factor12 * rw, rhoz_p, uo, vo,
+             wo);
+  un[q] = uo;
+  vn[q] = vo;
+  wn[q] = wo;
 }
 
 #define FZ_MOM_FUSED_KERNEL(NAME, ATTR, STG)                                                                     \
@@ -2030,14 +2105,17 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
       const double rhox_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(1, 0, 0));
       const double rhoy_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(0, 1, 0));
       const double rhoz_p = rho + drho * 0.5 * ((double)PSI(0, 0, 0) + (double)PSI(0, 0, 1));
-      double uu = U(0, 0, 0) * rhox_n / rhox_p + (factor1 * ru + factor2 * ruo) / rhox_p;
-      double vv = V(0, 0, 0) * rhoy_n / rhoy_p + (factor1 * rv + factor2 * rvo) / rhoy_p;
-      double ww = W(0, 0, 0) * rhoz_n / rhoz_p + (factor1 * rw + factor2 * rwo) / rhoz_p;
+      const double nu1 = U(0, 0, 0) * rhox_n, nu2 = factor1 * ru + factor2 * ruo;
+      const double nv1 = V(0, 0, 0) * rhoy_n, nv2 = factor1 * rv + factor2 * rvo;
+      const double nw1 = W(0, 0, 0) * rhoz_n, nw2 = factor1 * rw + factor2 * rwo;
       mom_oth_cell(k, dzci, dxi, dyi, dt_r, rho, drho, surf_factor, rho0, bfx, bfy, bfz, gx, gy, gz, rax, ray, raz,
                    P, PSI, KAP, P, PO, ru, rv, rw);
-      un[q] = uu + factor12 * ru / rhox_p;
-      vn[q] = vv + factor12 * rv / rhoy_p;
-      wn[q] = ww + factor12 * rw / rhoz_p;
+      double uo, vo, wo;
+      mom_update(nu1, nu2, factor12 * ru, rhox_p, nv1, nv2, factor12 * rv, rhoy_p, nw1, nw2, factor12 * rw, rhoz_p, uo,
+                 vo, wo);
+      un[q] = uo;
+      vn[q] = vo;
+      wn[q] = wo;
     }
     __syncthreads();  // the slot of plane k-1 is free
     if (k < kend) store_plane(k + 2);
