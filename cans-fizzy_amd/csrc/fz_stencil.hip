@@ -1455,25 +1455,19 @@ struct RegAt27 {
 // same expressions, the same values. MODE 0: the range checks per lane as in
 // k_youngs (left out when the range flag allows); 1: only the form without
 // them, and nothing when the range flag is raised or not given; 2: nothing
-// unless the flag is raised, then mode 0 (so 1 then 2 compute the field in
-// either case, and the register-hungry checked form runs only when needed).
-template <typename T, int KZ, int MODE = 0>
-__global__ void __launch_bounds__(256) k_youngs_mz(fz_dims d, const double *__restrict__ dzci,
-                                                   const double *__restrict__ dzfi, double dxi_d, double dyi_d,
-                                                   const T *__restrict__ psi, T *__restrict__ normx,
-                                                   T *__restrict__ normy, T *__restrict__ normz,
-                                                   T *__restrict__ kappa, int xper,
-                                                   const unsigned *__restrict__ rflag) {
-  unsigned cs_, bx, by, bz;
-  fz_stencil_block_b(d.tile_y, gridDim.z, cs_, bx, by, bz);
+// unless the flag is raised, then mode 0 over every tile of the (gx, gy, gz)
+// tile grid, the blocks striding over the tiles (so 1 then 2 compute the
+// field in either case, the register-hungry checked form runs only when
+// needed, and a few hundred blocks of mode 2 find the flag clear and exit).
+template <typename T, int KZ, int MODE>
+__device__ __forceinline__ void youngs_mz_tile(const fz_dims &d, const double *__restrict__ dzci,
+                                               const double *__restrict__ dzfi, double dxi_d, double dyi_d,
+                                               const T *__restrict__ psi, T *__restrict__ normx,
+                                               T *__restrict__ normy, T *__restrict__ normz, T *__restrict__ kappa,
+                                               int xper, bool nocheck, unsigned bx, unsigned by, unsigned bz) {
   const int i = (int)bx * blockDim.x + threadIdx.x + 1, j = (int)by * blockDim.y + threadIdx.y + 1;
   const int k0 = (int)bz * KZ + 1, kend = min(k0 + KZ - 1, d.n3);
   if (i > d.n1 || j > d.n2) return;  // (no barriers below)
-  if constexpr (MODE == 1) {
-    if (!rflag || *rflag != 0u) return;
-  } else if constexpr (MODE == 2) {
-    if (rflag && *rflag == 0u) return;
-  }
   const BufFld<T> G = buf_fld(psi, d);
   const unsigned e00 = (unsigned)i + (unsigned)d.sy * (unsigned)j, sy = (unsigned)d.sy, sz = (unsigned)d.sz;
   auto load = [&](T *pl, int kp) {
@@ -1486,7 +1480,6 @@ __global__ void __launch_bounds__(256) k_youngs_mz(fz_dims d, const double *__re
   load(p0, k0 - 1);
   load(p1, k0);
   load(p2, k0 + 1);
-  const bool nocheck = rflag && *rflag == 0u;  // see k_youngs
   const XHalo xh = xhalo_of(d, i);
   for (int k = k0; k <= kend; k++) {
     if (k < kend) load(pn, k + 2);  // in flight during this level
@@ -1523,6 +1516,34 @@ __global__ void __launch_bounds__(256) k_youngs_mz(fz_dims d, const double *__re
   }
 }
 
+template <typename T, int KZ, int MODE = 0>
+__global__ void __launch_bounds__(256) k_youngs_mz(fz_dims d, const double *__restrict__ dzci,
+                                                   const double *__restrict__ dzfi, double dxi_d, double dyi_d,
+                                                   const T *__restrict__ psi, T *__restrict__ normx,
+                                                   T *__restrict__ normy, T *__restrict__ normz,
+                                                   T *__restrict__ kappa, int xper,
+                                                   const unsigned *__restrict__ rflag, uint3 tiles) {
+  const bool nocheck = rflag && *rflag == 0u;  // see k_youngs
+  if constexpr (MODE == 1) {
+    if (!nocheck) return;
+  } else if constexpr (MODE == 2) {
+    if (!rflag || nocheck) return;
+  }
+  if constexpr (MODE == 2) {
+    const unsigned per_plane = tiles.x * tiles.y, nt = per_plane * tiles.z;
+    for (unsigned t = blockIdx.x; t < nt; t += gridDim.x) {
+      const unsigned bz = t / per_plane, r = t - bz * per_plane, by = r / tiles.x, bx = r - by * tiles.x;
+      youngs_mz_tile<T, KZ, MODE>(d, dzci, dzfi, dxi_d, dyi_d, psi, normx, normy, normz, kappa, xper, false, bx, by,
+                                  bz);
+    }
+  } else {
+    unsigned cs_, bx, by, bz;
+    fz_stencil_block_b(d.tile_y, gridDim.z, cs_, bx, by, bz);
+    youngs_mz_tile<T, KZ, MODE>(d, dzci, dzfi, dxi_d, dyi_d, psi, normx, normy, normz, kappa, xper, nocheck, bx, by,
+                                bz);
+  }
+}
+
 // (a batch of ncase cases when br.bt is set: their grids stacked in z)
 template <typename T>
 static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T *normy, T *normz, T *kappa,
@@ -1530,28 +1551,30 @@ static int launch_youngs(fz_ctx *c, const char *name, const T *psi, T *normx, T 
   FZ_TRY(fits_buffers(c, "cmpt_norm_curv_youngs"));
   // (the range flag only where the grid factors lie in [2^-20, 2^20])
   if (!c->youngs_fast_grid) rflag = nullptr;
-  // 0: marching in z for FP32 (274 against 345-361 us at 256x256x512), one
-  // cell per thread for FP64 (the marching FP64 kernel needs 236 VGPRs, two
-  // waves per SIMD, and took 758 against 705-745 us; capped, it spills)
+  // 0: marching in z for FP32 (274 against 345-361 us at 256x256x512); for
+  // FP64 the check-free marching kernel and its checked twin where the range
+  // flag applies (the marching FP64 kernel with both forms needs 236 VGPRs,
+  // two waves per SIMD, and took 758 against 705-745 us for one cell per
+  // thread; capped, it spills), one cell per thread elsewhere
   const int yv = c->youngs_variant ? c->youngs_variant : (sizeof(T) == 4 ? 1 : 3);
+  constexpr int KZ = 16;
+  dim3 mgrid = fz_grid_interior(c->d);
+  mgrid.z = (c->d.n3 + KZ - 1) / KZ;
+  const uint3 tiles = {mgrid.x, mgrid.y, mgrid.z};
   if (!br.bt && yv == 1) {  // marching in z
-    constexpr int KZ = 16;
-    dim3 grid = fz_grid_interior(c->d);
-    grid.z = (c->d.n3 + KZ - 1) / KZ;
-    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ>), grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi,
-              normx, normy, normz, kappa, xper, rflag);
+    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ>), mgrid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1],
+              psi, normx, normy, normz, kappa, xper, rflag, tiles);
     FZ_LAUNCHED();
     return FZ_OK;
   }
   if (!br.bt && yv == 3 && rflag) {  // the check-free marching kernel, then the checked one if the flag is raised
-    constexpr int KZ = 16;
-    dim3 grid = fz_grid_interior(c->d);
-    grid.z = (c->d.n3 + KZ - 1) / KZ;
-    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ, 1>), grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi,
-              normx, normy, normz, kappa, xper, rflag);
+    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ, 1>), mgrid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1],
+              psi, normx, normy, normz, kappa, xper, rflag, tiles);
     FZ_LAUNCHED();
-    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ, 2>), grid, fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], psi,
-              normx, normy, normz, kappa, xper, rflag);
+    // (a grid-stride launch of at most 512 blocks: 2 per CU)
+    const unsigned nt = mgrid.x * mgrid.y * mgrid.z, nb2 = nt < 512u ? nt : 512u;
+    FZ_LAUNCH(c, name, (k_youngs_mz<T, KZ, 2>), dim3(nb2), fz_block3(), 0, c->d, c->dzci, c->dzfi, c->dli[0],
+              c->dli[1], psi, normx, normy, normz, kappa, xper, rflag, tiles);
     FZ_LAUNCHED();
     return FZ_OK;
   }
@@ -2676,6 +2699,24 @@ extern "C" int fz_math_eval(fz_ctx *c, int fn, long n, const double *x, const do
   FZ_LAUNCH(c, "math_eval", k_math_eval, dim3((unsigned)nb), dim3(256), 0, fn, n, x, y, out);
   FZ_LAUNCHED();
   return FZ_OK;
+}
+
+__global__ void k_clear_word(unsigned *__restrict__ w) {
+  if (threadIdx.x == 0) w[0] = 0u;
+}
+
+// (tests) acdi_cmpt_phi then cmpt_norm_curv_youngs as the FP64 step runs
+// them: phi from psi, raising the range flag (the device word *flag, cleared
+// first) where some phi is neither zero nor of a magnitude in [2^-100, 2^20],
+// then the Youngs kernels the flag selects
+extern "C" int fz_phase_geometry_eval(fz_ctx *c, double seps, const double *psi, double *phi, double *normx,
+                                      double *normy, double *normz, double *kappa, unsigned *flag) {
+  if (!c || !psi || !phi || !normx || !normy || !normz || !kappa || !flag)
+    return fz_set_error(FZ_ERR_ARG, "fz_phase_geometry_eval: null argument");
+  FZ_LAUNCH(c, "clear_word", k_clear_word, dim3(1), dim3(64), 0, flag);
+  FZ_LAUNCHED();
+  FZ_TRY(fz_acdi_cmpt_phi_rf(c, seps, psi, phi, flag));
+  return fz_youngs_step<double>(c, phi, normx, normy, normz, kappa, 0, flag);
 }
 
 // ---------------------------------------------------------------------------

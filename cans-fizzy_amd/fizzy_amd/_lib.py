@@ -81,6 +81,7 @@ SIGNATURES = {
     "fz_chkdiv": (C.c_int, [vp, vp, vp, vp, dp, dp]),
     "fz_chkdiv_noalloc": (C.c_int, [vp, vp, vp, vp, vp]),
     "fz_math_eval": (C.c_int, [vp, C.c_int, C.c_long, vp, vp, vp]),
+    "fz_phase_geometry_eval": (C.c_int, [vp, C.c_double] + [vp] * 7),
     "fz_chkdt": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, dp, vp, vp, vp, C.c_double,
                            C.c_double, dp]),
     "fz_case_create": (C.c_int, [C.POINTER(vp), C.POINTER(FzParams), vp]),

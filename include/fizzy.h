@@ -223,6 +223,14 @@ int fz_chkdt(fz_ctx *ctx, int is_solve_ns, int is_track_interface, int vof, cons
  * 7 the compiler's division x/y. No counterpart in the reference: for tests
  * of the device routines against the oracle's copies (asynchronous). */
 int fz_math_eval(fz_ctx *ctx, int fn, long n, const double *x, const double *y, double *out);
+/* acdi_cmpt_phi (src/acdi.f90:227-248) then cmpt_norm_curv_youngs
+ * (src/two_fluid.f90:81-174) as the FP64 ACDI step runs them: phi from psi,
+ * raising the device word *flag (cleared first) when some phi is neither
+ * zero nor of a magnitude in [2^-100, 2^20], then the Youngs kernels that
+ * flag selects (the check-free one when it is clear). For tests of both
+ * paths against the oracle (asynchronous). */
+int fz_phase_geometry_eval(fz_ctx *ctx, double seps, const double *psi, double *phi, double *normx, double *normy,
+                           double *normz, double *kappa, unsigned *flag);
 
 /* ---- field files ---- */
 /* load_one / io_field, src/load.f90:19-168: io 'w' writes the interior of the

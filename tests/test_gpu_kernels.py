@@ -15,7 +15,7 @@ import pytest
 import fizzy_amd as F
 import oracle as O
 from conftest import rel_err
-from fizzy_amd.device import dvec
+from fizzy_amd.device import DArray, dvec
 
 pytestmark = pytest.mark.gpu
 L = O.lib()
@@ -151,6 +151,43 @@ def test_youngs_outside_the_fast_range(pair, rng):
         assert np.array_equal(got, r[c], equal_nan=True)
         keep = ~np.isnan(got)  # NaN sign bits differ between the host's and the device's arithmetic
         assert np.array_equal(np.signbit(got[keep]), np.signbit(r[c][keep]))
+
+
+@pytest.mark.parametrize("seps_cells, raised", [(0.51, False), (2.0 ** 24, True)], ids=["flag-clear", "flag-raised"])
+def test_youngs_under_the_range_flag(pair, rng, seps_cells, raised):
+    """The FP64 step's phase geometry (fz_phase_geometry_eval): phi from psi
+    raising the range flag, then the check-free Youngs kernel when the flag
+    stays clear (quartered grid factors, no division fix-ups) or its checked
+    twin when it is raised (here by phi above 2^20 from a wide interface),
+    against the oracle's acdi_cmpt_phi and cmpt_norm_curv_youngs bit for bit.
+    psi holds cells of exactly 0, 1 and 0.5 and a slab of zeros, so that many
+    vertex gradients and some phi are exactly zero."""
+    n = pair.n
+    psi = rand(rng, n, 0.0, 1.0)
+    pick = rng.random(psi.shape)
+    psi[pick < 0.15] = 0.0
+    psi[(pick >= 0.15) & (pick < 0.3)] = 1.0
+    psi[(pick >= 0.3) & (pick < 0.35)] = 0.5
+    psi[:, :, : n[2] // 3] = 0.0
+    psi = np.asfortranarray(psi)
+    seps = seps_cells * pair.dl[0]
+    phi = O.halo(n)
+    L.or_acdi_cmpt_phi(I(n), C.c_double(seps), P(psi), P(phi))
+    ref = [O.halo(n) for _ in range(4)]
+    L.or_cmpt_norm_curv_youngs(I(n), D(pair.dli), P(pair.dzci), P(pair.dzfi), P(phi), *[P(r) for r in ref])
+    dpsi, dphi = pair.h().upload(psi), pair.h()
+    outs = [pair.h() for _ in range(4)]
+    word = DArray(pair.ctx, (1,))
+    try:
+        F.call("fz_phase_geometry_eval", pair.ctx.h, seps, dpsi.ptr, dphi.ptr, *[o.ptr for o in outs], word.ptr)
+        flag = int(word.download().view(np.uint32)[0])
+    finally:
+        word.free()
+    assert (flag != 0) == raised
+    assert np.array_equal(dphi.download(), phi)
+    c = (slice(1, -1),) * 3
+    for o, r in zip(outs, ref):
+        assert np.array_equal(o.download()[c], r[c]), rel_err(o.download()[c], r[c])
 
 
 def test_vof_thinc(pair, rng):
