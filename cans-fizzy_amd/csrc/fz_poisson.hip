@@ -407,11 +407,12 @@ __device__ __forceinline__ int lds_pad(int c) {
 
 // (WPE: the waves per SIMD its registers must allow; 8 for array sources, a
 // level the LDS allows too, 1 (no cap) for computed sources, whose loads
-// need more registers)
-template <int M, bool QUARTER, typename R, typename TI, typename TO, int WPE, int GDIV>
+// need more registers: only the WPE 1 instances read fs. KIND: the r2r kind
+// when fixed at compile time, else -1 and the kind argument)
+template <int M, bool QUARTER, typename R, typename TI, typename TO, int WPE, int GDIV, int KIND = -1>
 __global__ void __launch_bounds__((P2Shape<M, QUARTER, R, GDIV>::NT)) __attribute__((amdgpu_waves_per_eu(WPE)))
     k_r2r_p2(const TI *__restrict__ src, LineMap in, TO *__restrict__ dst,
-                                                LineMap out, int L1, int kind, double scale, int accum,
+                                                LineMap out, int L1, int kind_a, double scale, int accum,
                                                 const typename CT<R>::C *__restrict__ tw,
                                                 const typename CT<R>::C *__restrict__ q,
                                                 const typename CT<R>::C *__restrict__ t2, FillSrc fs,
@@ -419,6 +420,7 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R, GDIV>::NT)) __attribut
   using S = P2Shape<M, QUARTER, R, GDIV>;
   using C = typename CT<R>::C;
   constexpr int N = S::N, G = S::G, NT = S::NT, Q4 = M / 4, SH = S::SH, MP = S::MP, RS = S::RS;
+  const int kind = KIND >= 0 ? KIND : kind_a;  // (KIND: the kind fixed at compile time)
   if (bt.src) {  // case blockIdx.z of a batch of same-shaped solves
     src = (const TI *)bt.src[blockIdx.z];
     dst = (TO *)bt.dst[blockIdx.z];
@@ -441,7 +443,7 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R, GDIV>::NT)) __attribut
     R val = 0.0;
     if (g < gcount) {
       const long a = in.base + (long)(l1_0 + g) * in.s1 + l2 * in.s2 + (long)e * in.se;
-      if (fs.on) {
+      if (WPE == 1 && fs.on) {
         const double sv = src_value(fs, a, (int)l2 + 1);
         vmax = fmax(vmax, fabs(sv));
         val = (R)sv;
@@ -451,7 +453,7 @@ __global__ void __launch_bounds__((P2Shape<M, QUARTER, R, GDIV>::NT)) __attribut
     }
     Rw[g * RS + e] = val;
   }
-  if (fs.rmax) slots_max(fs.rmax, vmax, (blockIdx.x + gridDim.x * blockIdx.y) * (NT / 64) + tid / 64);
+  if (WPE == 1 && fs.rmax) slots_max(fs.rmax, vmax, (blockIdx.x + gridDim.x * blockIdx.y) * (NT / 64) + tid / 64);
   __syncthreads();
   const int f = tid / Q4, b = tid - f * Q4;
   C v[4];
@@ -556,12 +558,24 @@ static void launch_p2_g(fz_ctx *ctx, const char *name, const fz_fftplan &P, int 
   using S = P2Shape<M, QUARTER, R, GDIV>;
   dim3 grid((L1 + S::G - 1) / S::G, L2, bt.src ? bt.n : 1);
   const PlanTabs<R> t = tabs_of(P, R());
-  if (fs.on)
-    FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, 1, GDIV>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1,
-              kind, scale, accum, t.tw, t.q, t.t2, fs, bt);
-  else
-    FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, 8, GDIV>), grid, dim3(S::NT), S::LDS, src, in, dst, out, L1,
-              kind, scale, accum, t.tw, t.q, t.t2, fs, bt);
+#define FZ_P2_LAUNCH(WPE, KIND)                                                                                   \
+  FZ_LAUNCH(ctx, name, (k_r2r_p2<M, QUARTER, R, TI, TO, WPE, GDIV, KIND>), grid, dim3(S::NT), S::LDS, src, in, dst, \
+            out, L1, kind, scale, accum, t.tw, t.q, t.t2, fs, bt)
+  // the periodic kinds of lengths from 64 with the kind fixed at compile
+  // time: the pre- and post-processing without the other kinds' branches
+  // (y lines of 256: 174 and 163 -> 152 and 141 us forward and backward)
+  constexpr bool SPEC = !QUARTER && M >= 64;
+  if (fs.on) {
+    if (SPEC && kind == K_R2HC) FZ_P2_LAUNCH(1, SPEC ? K_R2HC : -1);
+    else FZ_P2_LAUNCH(1, -1);
+  } else if (SPEC && kind == K_R2HC) {
+    FZ_P2_LAUNCH(8, SPEC ? K_R2HC : -1);
+  } else if (SPEC && kind == K_HC2R) {
+    FZ_P2_LAUNCH(8, SPEC ? K_HC2R : -1);
+  } else {
+    FZ_P2_LAUNCH(8, -1);
+  }
+#undef FZ_P2_LAUNCH
 }
 template <int M, bool QUARTER, typename R, typename TI, typename TO>
 static void launch_p2(fz_ctx *ctx, const char *name, const fz_fftplan &P, int kind, const TI *src, LineMap in,
