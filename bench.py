@@ -58,7 +58,8 @@ WORKLOAD_NAMES = {"c2": "acdi_rising_bubble_256x256x512", "c0": "acdi_rising_bub
 
 KERNEL_SYMBOLS = {"mom_fused": ("k_mom_lds", "k_mom_fused"),
                   "correc_step": ("k_correc_lds", "k_correc_step_mz", "k_correc_step"),
-                  "acdi_step": ("k_acdi_step", "k_acdi_lds")}
+                  "acdi_step": ("k_acdi_sf", "k_acdi_step", "k_acdi_lds"),
+                  "youngs": ("k_youngs_mz", "k_youngs")}
 
 
 def profiled_traffic(kernel, workload):
