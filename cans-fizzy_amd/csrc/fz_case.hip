@@ -33,7 +33,7 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
 int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
 int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
                           const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
-                          double *p_out, unsigned long long *slots, int xper);
+                          double *p_out, unsigned long long *slots, int xper, unsigned rlo);
 int fz_launch_gamma_fin(fz_ctx *c, unsigned long long *slots, double factor, double gmin, double *out,
                         unsigned *clear);
 int fz_acdi_cmpt_phi_rf(fz_ctx *c, double seps, const double *psi, double *phi, unsigned *rflag);
@@ -321,6 +321,23 @@ extern "C" int fz_case_set_scalars(fz_case *cs, const double in[12]) {
 // case's device scalars, zero at creation
 static unsigned *range_flag(fz_case *cs) { return (unsigned *)(cs->dsc + 15); }
 
+// The directions (bit d) in which the pressure and all three velocity
+// components are periodic: there the boundary conditions after the
+// correction overwrite every halo cell of u, v, w and p (with is_correc only
+// a non-periodic direction keeps the correction's normal velocity in its
+// halo, src/bound.f90:46-68), so the correction kernel may leave those halo
+// cells out.
+static unsigned interior_dirs(const fz_params &P) {
+  unsigned m = 0;
+  for (int d = 0; d < 3; d++) {
+    bool per = P.cbcpre[2 * d] == 'P' && P.cbcpre[2 * d + 1] == 'P';
+    for (int comp = 0; comp < 3; comp++)
+      per = per && P.cbcvel[6 * comp + 2 * d] == 'P' && P.cbcvel[6 * comp + 2 * d + 1] == 'P';
+    if (per) m |= 1u << d;
+  }
+  return m;
+}
+
 // (the Youngs kernel writes the periodic x halos of its four fields when x
 // is periodic, and their boundary update leaves those cells alone)
 static int phase_geometry(fz_case *cs) {
@@ -560,7 +577,7 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
     // correc into the spare velocity buffers, updatep into the buffer of the
     // previous pressure and the speeds of acdi_set_gamma, in one pass
     FZ_TRY(fz_launch_correc_step(c, rho0, dt, a.stc, cs->dp, f[F_P], f[F_U], f[F_V], f[F_W], cs->un, cs->vn, cs->wn,
-                                 f[F_PO], P.vof ? nullptr : cs->gslots, xp));
+                                 f[F_PO], P.vof ? nullptr : cs->gslots, xp, interior_dirs(P)));
     swap_ptr(f[F_U], cs->un);
     swap_ptr(f[F_V], cs->vn);
     swap_ptr(f[F_W], cs->wn);

@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
                                                               double *__restrict__ vn, double *__restrict__ wn,
                                                               double *__restrict__ p_out,
                                                               unsigned long long *__restrict__ slots, fz_bref br,
-                                                              int xper) {
+                                                              int xper, unsigned rlo) {
   constexpr int NT = COR_MX * MY, PL = COR_PX * (MY + 2), NPT = (PL + NT - 1) / NT;
   __shared__ double s_dp[2 * PL], s_u[2 * PL], s_v[2 * PL];
   FZ_BLOCK_XYZB(d, br, cs, bx, by, bz);
@@ -425,10 +425,13 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
   const double dt = stc ? stc[FZ_STC_DT] : dt_v;
   const double factori = stc ? dt * dxi : factori_v, factorj = stc ? dt * dyi : factorj_v;
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + COR_MX * ty;
-  const int i0 = (int)bx * COR_MX, j0 = (int)by * MY;  // the range starts at the halo
+  // the range: halo to halo, or in the directions of bit d of rlo (whose
+  // halo cells the boundary conditions that follow overwrite) 1 to n
+  const int xlo = rlo & 1u, ylo = (rlo >> 1) & 1u, zlo = (rlo >> 2) & 1u;
+  const int i0 = xlo + (int)bx * COR_MX, j0 = ylo + (int)by * MY;
   const int i = i0 + tx, j = j0 + ty;
-  const int k0 = (int)bz * KZ, kend = min(k0 + KZ - 1, d.n3 + 1), ktop = min(kend + 1, d.n3 + 1);
-  const bool act = i <= d.n1 + 1 && j <= d.n2 + 1;
+  const int k0 = zlo + (int)bz * KZ, kend = min(k0 + KZ - 1, d.n3 + 1 - zlo), ktop = min(kend + 1, d.n3 + 1);
+  const bool act = i <= d.n1 + 1 - xlo && j <= d.n2 + 1 - ylo;
   const bool in_ij = i >= 1 && i <= d.n1 && j >= 1 && j <= d.n2;
   const int c = (ty + 1) * COR_PX + (tx + 1);
   const BufFld<double> GDP = buf_fld(dp, d), GU = buf_fld(u, d), GV = buf_fld(v, d);
@@ -536,16 +539,18 @@ __global__ void __launch_bounds__(COR_MX *MY) k_correc_lds(fz_dims d, const doub
 
 int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
                           const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
-                          double *p_out, unsigned long long *slots, int xper) {
+                          double *p_out, unsigned long long *slots, int xper, unsigned rlo) {
   if (c->correc_kz == 1) {  // dp, u, v staged in LDS
     FZ_TRY(fits_buffers(c, "correc_step"));
     constexpr int KZ = 16;
     const int my = c->correc_my;
-    const dim3 grid((c->d.n1 + 2 + COR_MX - 1) / COR_MX, (c->d.n2 + 2 + my - 1) / my, (c->d.n3 + 2 + KZ - 1) / KZ);
+    // (rlo: the directions whose range is 1 to n, see k_correc_lds)
+    const int ex = 2 - 2 * (int)(rlo & 1u), ey = 2 - 2 * (int)((rlo >> 1) & 1u), ez = 2 - 2 * (int)((rlo >> 2) & 1u);
+    const dim3 grid((c->d.n1 + ex + COR_MX - 1) / COR_MX, (c->d.n2 + ey + my - 1) / my, (c->d.n3 + ez + KZ - 1) / KZ);
 #define FZ_CORREC_LDS(MY)                                                                                          \
   FZ_LAUNCH(c, "correc_step", (k_correc_lds<KZ, MY>), grid, dim3(COR_MX, MY), 0, c->d, c->dzci, dt * c->dli[0],     \
             dt * c->dli[1], dt, c->dli[0], c->dli[1], stc, rho0, dp, p, u, v, w, un, vn, wn, p_out, slots, kNoBRef, \
-            xper)
+            xper, rlo)
     if (my == 16) FZ_CORREC_LDS(16);
     else if (my == 8) FZ_CORREC_LDS(8);
     else FZ_CORREC_LDS(4);
@@ -2812,7 +2817,7 @@ int fz_batch_correc_gamma(fz_ctx *c, fz_bref br, int ncase, bool gamma) {
   FZ_LAUNCH(c, "correc_step", (k_correc_lds<KZ, MY, true>), grid, dim3(COR_MX, MY), 0, c->d, c->dzci, 0.0, 0.0, 0.0, \
             c->dli[0], c->dli[1], (const double *)nullptr, 0.0, (const double *)nullptr, (const double *)nullptr,   \
             (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, (double *)nullptr,          \
-            (double *)nullptr, (double *)nullptr, (double *)nullptr, (unsigned long long *)nullptr, b, 0)
+            (double *)nullptr, (double *)nullptr, (double *)nullptr, (unsigned long long *)nullptr, b, 0, 0u)
   if (my == 16) FZ_CORREC_LDS_B(16);
   else if (my == 8) FZ_CORREC_LDS_B(8);
   else FZ_CORREC_LDS_B(4);
