@@ -238,8 +238,6 @@ extern "C" int fz_case_create_ex(fz_case **out, const fz_params *prm, void *hip_
   int rc = fz_initgrid(prm->gtype, n3, prm->gr, prm->l[2], dzc.data(), dzf.data(), zc.data(), zf.data());
   if (rc == FZ_OK) rc = fz_ctx_create(&cs->ctx, prm->ng, prm->l, dzc.data(), dzf.data(), zc.data(), zf.data(),
                                       prm->cbcpre, hip_stream);
-  // the precomputed pivots serve FP64 solves only
-  if (rc == FZ_OK && precision == FZ_PREC_MIXED && nref >= 0) cs->ctx->thomas_pre = 0;
   if (rc == FZ_OK) rc = fz_initsolver(cs->ctx, prm->bcpre, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                                       nullptr, nullptr);
   if (rc == FZ_OK && precision == FZ_PREC_MIXED && !cs->ctx->pois->rhsb_zero)

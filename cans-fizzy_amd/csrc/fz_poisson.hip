@@ -961,6 +961,65 @@ __global__ void __launch_bounds__(64) k_thomas_x2(float *__restrict__ W, double 
   }
 }
 
+// k_thomas_pre for FP32 columns, two adjacent columns per lane (an even
+// column count), with the pivots made once at set-up (k_piv_pre): each row
+// moves 8 bytes of W and 16 of pivots per lane, as k_thomas_x2 does, without
+// the division on the per-row dependency chain. The per-column arithmetic is
+// k_thomas_pre<TK, float>'s, and so k_thomas<TK, float>'s, exactly.
+template <int TK>
+__global__ void __launch_bounds__(64) k_thomas_pre_x2(float *__restrict__ W, const double *__restrict__ Z,
+                                                      const double *__restrict__ D, const double *__restrict__ a,
+                                                      int n1, int n2, int n3, long gauge) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;  // column pair
+  const long ps2 = (long)n1 * n2 / 2;
+  if (idx >= ps2) return;
+  float2 *col = (float2 *)W + idx;
+  const double2 *zc = (const double2 *)Z + idx, *dc = (const double2 *)D + idx;
+  const bool g0 = 2 * idx == gauge, g1 = 2 * idx + 1 == gauge;
+  double p0 = 0.0, p1 = 0.0;
+  for (int l0 = 0; l0 < n3; l0 += TK) {
+    float2 r[TK];
+    double2 z[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (l0 + t < n3) {
+        r[t] = col[(long)(l0 + t) * ps2];
+        z[t] = zc[(long)(l0 + t) * ps2];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++) {
+      const int l = l0 + t;
+      if (l < n3) {
+        p0 = l == 0 ? (double)r[t].x * z[t].x : ((double)r[t].x - a[l] * p0) * z[t].x;
+        p1 = l == 0 ? (double)r[t].y * z[t].y : ((double)r[t].y - a[l] * p1) * z[t].y;
+        if (l == n3 - 1) {
+          if (g0) p0 = 0.0;
+          if (g1) p1 = 0.0;
+        }
+        col[(long)l * ps2] = make_float2((float)p0, (float)p1);
+      }
+    }
+  }
+  double q0 = p0, q1 = p1;
+  for (int h0 = n3 - 2; h0 >= 0; h0 -= TK) {
+    float2 r[TK];
+    double2 dd[TK];
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        r[t] = col[(long)(h0 - t) * ps2];
+        dd[t] = dc[(long)(h0 - t) * ps2];
+      }
+#pragma unroll
+    for (int t = 0; t < TK; t++)
+      if (h0 - t >= 0) {
+        q0 = (double)r[t].x - dd[t].x * q0;
+        q1 = (double)r[t].y - dd[t].y * q1;
+        col[(long)(h0 - t) * ps2] = make_float2((float)q0, (float)q1);
+      }
+  }
+}
+
 // the second solve of gaussel_periodic (src/solver.f90:107-149): it
 // depends on the operator only, so it is made once per solver set-up, with
 // the operations of the per-solve form (pivots scratch in D)
@@ -1509,10 +1568,17 @@ static int solver_rest(fz_ctx *ctx, T *W, double *p, int accum) {
     } else if (mixed && tk < 0 && ps % 2 == 0) {  // two FP32 columns per lane (tk = -rows per chunk)
       const dim3 g2((int)((ps / 2 + 63) / 64));
       float *W32 = (float *)(void *)W;
-      if (tk == -8) FZ_LAUNCH(ctx, name, k_thomas_x2<8>, g2, blk, 0, W32, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
-                              n3, gauge);
-      else FZ_LAUNCH(ctx, name, k_thomas_x2<16>, g2, blk, 0, W32, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
-                     gauge);
+      if (P->zpre) {  // pivots made at set-up
+        if (tk == -8) FZ_LAUNCH(ctx, name, k_thomas_pre_x2<8>, g2, blk, 0, W32, P->zpre, P->dpre, P->a, n1, n2, n3,
+                                gauge);
+        else FZ_LAUNCH(ctx, name, k_thomas_pre_x2<16>, g2, blk, 0, W32, P->zpre, P->dpre, P->a, n1, n2, n3, gauge);
+      } else if (tk == -8) {
+        FZ_LAUNCH(ctx, name, k_thomas_x2<8>, g2, blk, 0, W32, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
+                  gauge);
+      } else {
+        FZ_LAUNCH(ctx, name, k_thomas_x2<16>, g2, blk, 0, W32, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2, n3,
+                  gauge);
+      }
     } else if (tk == 8) FZ_LAUNCH(ctx, name, (k_thomas<8, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c, n1, n2,
                            n3, gauge, kNoBatch);
     else if (tk == 32) FZ_LAUNCH(ctx, name, (k_thomas<32, T>), grid, blk, 0, W, P->piv, P->lambdaxy, P->a, P->b, P->c,

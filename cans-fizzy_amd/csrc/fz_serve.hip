@@ -52,9 +52,11 @@ static int footprint(const fz_params *prm, int precision, int nref, size_t *byte
   const double nh = (n1 + 2) * (n2 + 2) * (n3 + 2), ni = n1 * n2 * n3;
   const bool periodic_z = prm->cbcpre[4] == 'P' && prm->cbcpre[5] == 'P';
   double b = 0.0;
-  // FP64 solves of a non-periodic z below 512x512 columns keep pivots made at
-  // set-up: two more N-element arrays (the context's default)
-  if (!periodic_z && n1 * n2 < 262144.0 && (precision != FZ_PREC_MIXED || nref < 0)) b += alloc_bytes(8.0 * ni, 2);
+  // solves of a non-periodic z below 512x512 columns (FP64, or FP32 in pairs
+  // of columns) keep pivots made at set-up: two more N-element arrays (the
+  // context's default)
+  (void)nref;
+  if (!periodic_z && n1 * n2 < 262144.0) b += alloc_bytes(8.0 * ni, 2);
   if (precision == FZ_PREC_MIXED) {
     b += alloc_bytes(8.0 * nh, 5 + 4);  // u, v, w, p, po + dp, three spare velocities
     b += alloc_bytes(4.0 * nh, 9 + 1);  // psi, phi, kappa, normals, fluxes + psio (FP32)
