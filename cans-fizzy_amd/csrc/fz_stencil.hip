@@ -1355,6 +1355,52 @@ __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dy
   }
   auto S = [](T a, T f) { return FOLD ? a * f : (T)0.25 * a * f; };
   T mx[8], my[8], mz[8];
+  if constexpr (sizeof(T) == 4 && FAST) {
+    // FP32 (the mixed-precision step, whose phase-field kernels are held to
+    // error bounds, not to the reference's bits): the same vertex gradients
+    // from sums over the faces' quads, each shared by the corners it serves.
+    // A gradient component is the difference of two 4-value quads of one
+    // pair of planes: pairs of values along z make the quads in x and y
+    // planes, pairs along x the quads in z planes.
+    T zp[3][3];          // z pairs at column (i, j) of one k pair
+    T xl[3][3], xu[3][3];  // x pairs (i-1, i) and (i, i+1) at (j, k)
+    T qx[3][2][2], qy[3][2][2], qz[3][2][2];
+    for (int h = 0; h < 2; h++) {  // h: the k pair (k-1, k) or (k, k+1)
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) zp[a][b] = PS(a - 1, b - 1, h - 1) + PS(a - 1, b - 1, h);
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+          qx[a][q][h] = zp[a][q] + zp[a][q + 1];  // x plane a, j pair q
+          qy[a][q][h] = zp[q][a] + zp[q + 1][a];  // y plane a, i pair q
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        xl[b][c] = PS(-1, b - 1, c - 1) + PS(0, b - 1, c - 1);
+        xu[b][c] = PS(0, b - 1, c - 1) + PS(1, b - 1, c - 1);
+      }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        qz[c][0][q] = xl[q][c] + xl[q + 1][c];  // z plane c, i pair (i-1, i), j pair q
+        qz[c][1][q] = xu[q][c] + xu[q + 1][c];
+      }
+    const T fx4 = (T)0.25 * dxi, fy4 = (T)0.25 * dyi, fz0 = (T)0.25 * zc0, fzm = (T)0.25 * zcm;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {  // corner m: (+,+,+) (+,-,+) (+,+,-) (+,-,-), then the same with -x
+      const int sx = m < 4, sy = (m & 1) == 0, sz = (m & 2) == 0;
+      mx[m] = (sx ? qx[2][sy][sz] - qx[1][sy][sz] : qx[1][sy][sz] - qx[0][sy][sz]) * fx4;
+      my[m] = (sy ? qy[2][sx][sz] - qy[1][sx][sz] : qy[1][sx][sz] - qy[0][sx][sz]) * fy4;
+      mz[m] = sz ? (qz[2][sx][sy] - qz[1][sx][sy]) * fz0 : (qz[1][sx][sy] - qz[0][sx][sy]) * fzm;
+    }
+  } else {
   mx[0] = S((pcc + ppc + pcp + ppp) - (ccc + cpc + ccp + cpp), fx);
   mx[1] = S((pcc + pmc + pcp + pmp) - (ccc + cmc + ccp + cmp), fx);
   mx[2] = S((pcc + ppc + pcm + ppm) - (ccc + cpc + ccm + cpm), fx);
@@ -1379,6 +1425,7 @@ __device__ __forceinline__ void youngs_cell(const A &PS, double dxi_d, double dy
   mz[5] = S((ccp + mcp + cmp + mmp) - (ccc + mcc + cmc + mmc), zc0);
   mz[6] = S((ccc + mcc + cpc + mpc) - (ccm + mcm + cpm + mpm), zcm);
   mz[7] = S((ccc + mcc + cmc + mmc) - (ccm + mcm + cmm + mmm), zcm);
+  }
 #pragma unroll
   for (int m = 0; m < 8; m++) unit3<FAST, T, NOCHECK>(mx[m], my[m], mz[m], ok);
   T nx = (T).125 * (mx[0] + mx[1] + mx[2] + mx[3] + mx[4] + mx[5] + mx[6] + mx[7]);
