@@ -498,7 +498,8 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
   // x periodic: the correction kernel and the FP32 ACDI kernel write the x
   // halos of the interior rows themselves (the boundary update leaves those
   // cells alone), as does the Youngs kernel of phase_geometry (the FP64 ACDI
-  // kernel measured 40 us slower with them, against 31 us of boundary work)
+  // kernel measured 40, later 50 us slower with them, against 31 us of boundary
+  // work)
   const int xp = c->per[0] ? 1 : 0;
   // psio <- psi by rotation: the update writes into the older buffer
   swap_ptr(cs->psio, f[F_PSI]);
