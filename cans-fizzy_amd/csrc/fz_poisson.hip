@@ -377,7 +377,14 @@ struct P2Shape {
   static constexpr int NP4 = LG / 2;           // radix-4 passes
   static constexpr bool HAS2 = (LG & 1) != 0;  // closing radix-2 pass
   static constexpr int N = QUARTER ? M / 2 : M;  // real line length
-  static constexpr int G64 = QUARTER ? (2048 / M < 64 ? 2048 / M : 64) : (4096 / M < 64 ? (4096 / M > 2 ? 4096 / M : 2) : 64);
+  static constexpr int G0 = QUARTER ? (2048 / M < 64 ? 2048 / M : 64) : (4096 / M < 64 ? (4096 / M > 2 ? 4096 / M : 2) : 64);
+  // strided lines (GDIV 1, the y lines) whose row segments of G0 values would
+  // span less than 128 bytes take twice the lines (up to 1024 threads): 512
+  // FP64 y lines 702 and 688 -> 675 and 656 us, FP32 ones 608 and 620 -> 530
+  // and 529 us, FP32 256 lines 115 -> 102 us backward; 256 FP64 lines, whose
+  // 16-line segments are 128 bytes already, measured slower doubled
+  static constexpr int G64 =
+      GDIV == 1 && !QUARTER && G0 * (int)sizeof(R) < 128 && G0 * M / 4 <= 1024 ? 2 * G0 : G0;
   // lines whose elements are contiguous in the source and the destination
   // (x lines) run GDIV times fewer lines per block: more, smaller blocks
   static constexpr int G = G64 / GDIV >= (QUARTER ? 1 : 2) ? G64 / GDIV : (QUARTER ? 1 : 2);
