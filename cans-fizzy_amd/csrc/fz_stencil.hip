@@ -902,8 +902,8 @@ __device__ __forceinline__ void acdi_face(T psa, T psb, T pha, T phb, T nxa, T n
 // right-hand side from its six faces in acdi_cell's order, so the results
 // are those of k_acdi_lds bit for bit. The velocities are read at the cell
 // only (and below the tile's edges), straight from memory.
-template <typename T, int KZ, bool XPER = false>
-__global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_sf(fz_dims d, const double *__restrict__ dzci,
+template <typename T, int KZ, bool XPER = false, int MY = ACD_MY>
+__global__ void __launch_bounds__(ACD_MX *MY) k_acdi_sf(fz_dims d, const double *__restrict__ dzci,
                                                            const double *__restrict__ dzfi, double dxi_d,
                                                            double dyi_d, const double *__restrict__ d_gam,
                                                            double gam_v, double seps_d, double factor1_v,
@@ -915,14 +915,14 @@ __global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_sf(fz_dims d, const dou
                                                            const T *__restrict__ psio, T *__restrict__ dpsio,
                                                            T *__restrict__ psi_out, T *__restrict__ flux_x,
                                                            T *__restrict__ flux_y, T *__restrict__ flux_z, int xper) {
-  constexpr int NT = ACD_MX * ACD_MY, PL = ACD_PL, NPT = (PL + NT - 1) / NT;
+  constexpr int NT = ACD_MX * MY, PL = ACD_PX * (MY + 2), NPT = (PL + NT - 1) / NT;
   __shared__ T s_nx[2 * PL], s_ny[2 * PL], s_nz[2 * PL], s_phi[2 * PL], s_psi[2 * PL];
   // the faces of a level: x faces of the tile's rows (index 0 the face below
   // the first column, tx+1 the face above cell tx), y faces of its columns
   // (row 0 the faces below the first row, ty+1 those above row ty);
   // advective and regularisation parts
-  __shared__ T f_xu[ACD_MY][ACD_MX + 1], f_xr[ACD_MY][ACD_MX + 1];
-  __shared__ T f_yu[ACD_MY + 1][ACD_MX], f_yr[ACD_MY + 1][ACD_MX];
+  __shared__ T f_xu[MY][ACD_MX + 1], f_xr[MY][ACD_MX + 1];
+  __shared__ T f_yu[MY + 1][ACD_MX], f_yr[MY + 1][ACD_MX];
   unsigned cs_, bx, by, bz;
   fz_stencil_block_b(d.tile_y, gridDim.z, cs_, bx, by, bz);
   const double gam_dd = d_gam ? *d_gam : gam_v;
@@ -930,7 +930,7 @@ __global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_sf(fz_dims d, const dou
   const T dxi = (T)dxi_d, dyi = (T)dyi_d, gam = (T)gam_dd, seps = (T)seps_d;
   const T gs = gam * seps, g4 = (T)0.25 * gam;
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + ACD_MX * ty;
-  const int i0 = (int)bx * ACD_MX + 1, j0 = (int)by * ACD_MY + 1, k0 = (int)bz * KZ + 1;
+  const int i0 = (int)bx * ACD_MX + 1, j0 = (int)by * MY + 1, k0 = (int)bz * KZ + 1;
   const int i = i0 + tx, j = j0 + ty;
   const bool act = i <= d.n1 && j <= d.n2;
   const int kend = min(k0 + KZ - 1, d.n3);
@@ -1015,7 +1015,7 @@ __global__ void __launch_bounds__(ACD_MX *ACD_MY) k_acdi_sf(fz_dims d, const dou
       f_yu[0][tx] = fu;
       f_yr[0][tx] = fr;
     }
-    if (ty == 1 && tx < ACD_MY && j0 + tx <= d.n2) {  // the x face left of the tile's first column, row tx
+    if (ty == 1 && tx < MY && j0 + tx <= d.n2) {  // the x face left of the tile's first column, row tx
       const int a = s1 + (tx + 1) * ACD_PX;
       const unsigned e = (unsigned)(i0 - 1) + (unsigned)d.sy * (unsigned)(j0 + tx) + ek;
       T fu, fr;
@@ -1070,11 +1070,14 @@ int fz_launch_acdi_step(fz_ctx *c, const double *d_gam, double gam, double seps,
   const int variant = c->acdi_variant ? c->acdi_variant : 3;
   if (variant == 3) {  // each face evaluated once
     constexpr int KZ = 16;
-    const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + ACD_MY - 1) / ACD_MY, (c->d.n3 + KZ - 1) / KZ);
     // (the x-halo stores compiled only into the instance that makes them; a
-    // run-time test there too: measured 550 against 565 us in FP32 without it)
-    auto kern = xper ? k_acdi_sf<T, KZ, true> : k_acdi_sf<T, KZ, false>;
-    FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", kern, grid, dim3(ACD_MX, ACD_MY), 0, c->d, c->dzci,
+    // run-time test there too: measured 550 against 565 us in FP32 without
+    // it. FP32 tiles of 64 x 8: 550 -> 532 us; FP64 ones, 70 KB of LDS, 925
+    // against 923 us, keep 64 x 4)
+    constexpr int MY = sizeof(T) == 4 ? 8 : ACD_MY;
+    const dim3 grid((c->d.n1 + ACD_MX - 1) / ACD_MX, (c->d.n2 + MY - 1) / MY, (c->d.n3 + KZ - 1) / KZ);
+    auto kern = xper ? k_acdi_sf<T, KZ, true, MY> : k_acdi_sf<T, KZ, false, MY>;
+    FZ_LAUNCH(c, sizeof(T) == 4 ? "acdi_step_f32" : "acdi_step", kern, grid, dim3(ACD_MX, MY), 0, c->d, c->dzci,
               c->dzfi, c->dli[0], c->dli[1], d_gam, gam, seps, factor1, factor2, stc, first, u, v, w, normx, normy,
               normz, phi, psio, dpsio, psi_out, flux_x, flux_y, flux_z, xper);
   } else if (variant == 1) {  // fields staged in LDS
