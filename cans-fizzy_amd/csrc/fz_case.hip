@@ -29,7 +29,7 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
                         const double gacc[3], const double rho_av[3], const PT *fx, const PT *fy, const PT *fz,
                         const double *u, const double *v, const double *w, const PT *psi, const PT *psio,
                         const double *p, const double *po, const PT *kappa, HT *duo, HT *dvo, HT *dwo, double *un,
-                        double *vn, double *wn);
+                        double *vn, double *wn, int xper);
 int fz_launch_fill(fz_ctx *c, long n, double val, double *p);
 int fz_launch_correc_step(fz_ctx *c, double rho0, double dt, const double *stc, const double *dp, const double *p,
                           const double *u, const double *v, const double *w, double *un, double *vn, double *wn,
@@ -550,23 +550,31 @@ static int enqueue_step(fz_case *cs, const StepArgs &a) {
       }
     }
     // rk, src/rk.f90:72-130, in one pass into the spare velocity buffers
+    // (x periodic and no forcing after it: the FP64 kernel writes the x halos
+    // of the interior rows, and the boundary update leaves them alone: 1387
+    // -> 1397 us, the gather 36 -> 14 us at 256x256x512; the FP32-phase-field
+    // kernel measured 1150 -> 1280 us with them and leaves them to the gather)
+    const int xm = xp && !P.is_forced_hit && !mixed ? 1 : 0;
     if (mixed)
       FZ_TRY((fz_launch_mom_fused<float, float>(c, P.rho12, P.mu12, factor1, factor2, is_first ? 1 : 0, factor12,
                                                dt_r, a.stc, P.sigma, rho0, P.bforce, P.gacc, rho_av, g[F_FX], g[F_FY],
                                                g[F_FZ], f[F_U], f[F_V], f[F_W], g[F_PSI], cs->psio32, f[F_P],
                                                f[F_PO], g[F_KAPPA], g[F_DUO], g[F_DVO], g[F_DWO], cs->un, cs->vn,
-                                               cs->wn)));
+                                               cs->wn, xm)));
     else
       FZ_TRY((fz_launch_mom_fused<double, double>(c, P.rho12, P.mu12, factor1, factor2, is_first ? 1 : 0, factor12,
                                                  dt_r, a.stc, P.sigma, rho0, P.bforce, P.gacc, rho_av, f[F_FX], f[F_FY],
                                                  f[F_FZ], f[F_U], f[F_V], f[F_W], f[F_PSI], cs->psio, f[F_P],
                                                  f[F_PO], f[F_KAPPA], f[F_DUO], f[F_DVO], f[F_DWO], cs->un, cs->vn,
-                                                 cs->wn)));
+                                                 cs->wn, xm)));
     swap_ptr(f[F_U], cs->un);
     swap_ptr(f[F_V], cs->vn);
     swap_ptr(f[F_W], cs->wn);
     if (P.is_forced_hit) FZ_TRY(fz_launch_forcing(c, 2, 0.5, dt, a.stc, f[F_U], f[F_V], f[F_W]));
-    FZ_TRY(fz_bounduvw(c, P.cbcvel, P.bcvel, 0, f[F_U], f[F_V], f[F_W]));
+    {
+      const FzBcItem it[1] = {{1, P.cbcvel, P.bcvel, 0, {f[F_U], f[F_V], f[F_W]}, xm}};
+      FZ_TRY(fz_bounds_group<double>(c, 1, it));
+    }
     // projection on the correction buffer dp, src/main.f90:488-502
     // (nref = -1: the mixed step with the FP64 projection of the plain step)
     if (mixed && cs->nref >= 0)

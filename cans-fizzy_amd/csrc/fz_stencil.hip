@@ -2048,7 +2048,9 @@ struct LdsAt {
 
 // WPE: waves per SIMD the registers must allow; STEADY: a step after the
 // first (first = 0) with no bulk-density term (rho_av = 0), known at compile time
-template <typename PT, typename HT, int KZ, int WPE, bool BT = false, bool STEADY = false>
+// (XPER: the cells 1 and n1 also store their new velocity at the periodic
+// x-halo images n1+1 and 0 of their row)
+template <typename PT, typename HT, int KZ, int WPE, bool BT = false, bool STEADY = false, bool XPER = false>
 __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_per_eu(WPE))) k_mom_lds(
     fz_dims d, const double *__restrict__ dzci, const double *__restrict__ dzfi, double dxi, double dyi, double rho,
     double drho, double mu, double dmu, double factor1_v, double factor2_v, int first, double factor12_v,
@@ -2197,6 +2199,18 @@ __global__ void __launch_bounds__(MOM_MX *MOM_MY) __attribute__((amdgpu_waves_pe
       un[q] = uo;
       vn[q] = vo;
       wn[q] = wo;
+      if constexpr (XPER) {  // (plain stores: buffer resources for three more fields spill scalar registers)
+        if (i == d.n1) {
+          un[q - d.n1] = uo;
+          vn[q - d.n1] = vo;
+          wn[q - d.n1] = wo;
+        }
+        if (i == 1) {
+          un[q + d.n1] = uo;
+          vn[q + d.n1] = vo;
+          wn[q + d.n1] = wo;
+        }
+      }
     }
     __syncthreads();  // the slot of plane k-1 is free
     if (k < kend) store_plane(k + 2);
@@ -2211,7 +2225,7 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
                         const double gacc[3], const double rho_av[3], const PT *fx, const PT *fy, const PT *fz,
                         const double *u, const double *v, const double *w, const PT *psi, const PT *psio,
                         const double *p, const double *po, const PT *kappa, HT *duo, HT *dvo, HT *dwo, double *un,
-                        double *vn, double *wn) {
+                        double *vn, double *wn, int xper) {
   FZ_TRY(fits_buffers(c, "mom_fused"));
   const double surf_factor = sigma * 2.0 / (rho12[0] + rho12[1]);
   // the context's variant; 0: LDS staging (at 256x256x512: FP64 1386
@@ -2230,8 +2244,12 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
     // In FP64 the same gives 133 VGPRs and three waves per SIMD but is
     // slower (1532 against 1403-1452 us at two waves, 188 VGPRs).
     const bool steady = first == 0 && rho_av[0] == 0.0 && rho_av[1] == 0.0 && rho_av[2] == 0.0;
-    auto kern = sizeof(PT) == 4 && steady ? k_mom_lds<PT, HT, KZ, 3, false, true>
-                                          : k_mom_lds<PT, HT, KZ, sizeof(PT) == 4 ? 3 : 1, false, false>;
+    // (xper: the x-halo stores, compiled only into the instances that make them)
+    constexpr int W = sizeof(PT) == 4 ? 3 : 1;
+    auto kern = sizeof(PT) == 4 && steady ? (xper ? k_mom_lds<PT, HT, KZ, 3, false, true, true>
+                                                  : k_mom_lds<PT, HT, KZ, 3, false, true, false>)
+                                          : (xper ? k_mom_lds<PT, HT, KZ, W, false, false, true>
+                                                  : k_mom_lds<PT, HT, KZ, W, false, false, false>);
     FZ_LAUNCH(c, sizeof(PT) == 4 ? "mom_fused_f32" : "mom_fused", kern,
               grid, dim3(MOM_MX, MOM_MY), 0, c->d, c->dzci, c->dzfi, c->dli[0], c->dli[1], rho12[1],
               rho12[0] - rho12[1], mu12[1], mu12[0] - mu12[1], factor1, factor2, first, factor12, dt_r, stc,
@@ -2250,6 +2268,7 @@ int fz_launch_mom_fused(fz_ctx *c, const double rho12[2], const double mu12[2], 
             gacc[2], rho_av[0], rho_av[1], rho_av[2], fx, fy, fz, u, v, w, psi, psio, p, po, kappa, duo, dvo, dwo, un,
             vn, wn, kNoBRef);
   FZ_LAUNCHED();
+  if (xper) return xhalo_copy<double>(c, un, vn, wn, nullptr);
   return FZ_OK;
 }
 template int fz_launch_mom_fused<double, double>(fz_ctx *, const double *, const double *, double, double, int, double,
@@ -2258,14 +2277,14 @@ template int fz_launch_mom_fused<double, double>(fz_ctx *, const double *, const
                                                  const double *, const double *, const double *, const double *,
                                                  const double *, const double *, const double *, const double *,
                                                  const double *, const double *, const double *, const double *,
-                                                 double *, double *, double *, double *, double *, double *);
+                                                 double *, double *, double *, double *, double *, double *, int);
 template int fz_launch_mom_fused<float, float>(fz_ctx *, const double *, const double *, double, double, int, double,
                                                double, const double *, double, double, const double *,
                                                const double *, const double *,
                                                const float *, const float *, const float *, const double *,
                                                const double *, const double *, const float *, const float *,
                                                const double *, const double *, const float *, float *, float *,
-                                               float *, double *, double *, double *);
+                                               float *, double *, double *, double *, int);
 
 // AB2 momentum update with the density rescale, src/rk.f90:86-101
 __global__ void k_rk_mom_ab2(fz_dims d, double factor1, double factor2, double rho, double drho,
